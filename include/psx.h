@@ -1,0 +1,411 @@
+/*
+ * psx.h - C ABI of the MI355X render path for Psycles scenes.
+ *
+ * This is the drop-in boundary for Psycles' device render call.  Every entry
+ * point replaces one member of the reference's C++ contract:
+ *
+ *   psx_backend_create / psx_backend_destroy
+ *       luisa_backend::LuisaPathTracerBackend(Device, LuisaPathTracerOptions)
+ *       (include/psycles/luisa/path_tracer.h:45-78)
+ *   psx_scene_compile / psx_scene_destroy
+ *       contract::RendererBackend::compile_scene(const SceneSnapshot&)
+ *       -> SceneCompilation{unique_ptr<CompiledScene>, diagnostics}
+ *       (include/psycles/contract/render.h:172-181, 145-161)
+ *   psx_scene_update_parameters
+ *       the parameter-only rebind branch of MaterialLibrary::update
+ *       (src/compiler/material_library.cpp:53-90)
+ *   psx_session_create / psx_session_destroy
+ *       contract::RendererBackend::create_session(const CompiledScene&,
+ *       const RenderSettings&) (include/psycles/contract/render.h:178-180;
+ *       validation in src/luisa/path_tracer.cpp:22-75)
+ *   psx_session_reset   RenderSession::reset (render.h:166)
+ *   psx_session_render  RenderSession::render_samples(const SampleRange&,
+ *                       OutputSink&) (render.h:167;
+ *                       src/luisa/path_tracer_session.cpp:272-413)
+ *   psx_session_cancel  RenderSession::cancel (render.h:168;
+ *                       src/luisa/path_tracer_session.cpp:409-416)
+ *
+ * Conventions: opaque handles, int status codes (PSX_OK == 0, negative on
+ * error), no exception ever crosses this boundary, no torch or HIP types in
+ * any signature.  All scene arrays are plain host pointers that are only read
+ * during the call that receives them.  Matrices are 4x4 column-major float
+ * arrays exactly like psycles::Mat4f (elements[12..14] hold translation).
+ */
+#ifndef PSX_H
+#define PSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSX_ABI_VERSION 1u
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    PSX_OK = 0,
+    PSX_ERROR_INVALID_ARGUMENT = -1,
+    PSX_ERROR_DEVICE = -2,        /* HIP runtime or kernel launch failure   */
+    PSX_ERROR_OUT_OF_MEMORY = -3,
+    PSX_ERROR_CANCELLED = -4,     /* render_samples returned false: cancel   */
+    PSX_ERROR_RANGE = -5,         /* render_samples returned false: range    */
+    PSX_ERROR_INCOMPLETE = -6,    /* sample_count readback mismatch         */
+    PSX_ERROR_UNSUPPORTED = -7,
+    PSX_ERROR_NO_DEVICE = -8
+};
+
+/* ---- enums mirroring the reference contract ----------------------------- */
+
+/* contract::PassKind (include/psycles/contract/render.h:15-42).  The values
+ * are identical so a caller can forward its own enum unchanged. */
+enum psx_pass_kind {
+    PSX_PASS_COMBINED = 0,
+    PSX_PASS_DEPTH = 1,
+    PSX_PASS_NORMAL = 2,
+    PSX_PASS_POSITION = 3,
+    PSX_PASS_ALBEDO = 4,
+    PSX_PASS_EMISSION = 5,
+    PSX_PASS_DENOISING_NORMAL = 18,
+    PSX_PASS_DENOISING_ALBEDO = 19,
+    PSX_PASS_SAMPLE_COUNT = 20,
+    PSX_PASS_ENVIRONMENT = 25
+};
+
+/* contract::PixelFilter (render.h:65-69) */
+enum psx_pixel_filter {
+    PSX_FILTER_BOX = 0,
+    PSX_FILTER_GAUSSIAN = 1,
+    PSX_FILTER_BLACKMAN_HARRIS = 2
+};
+
+/* contract::DirectLightSampling (render.h:71-75) */
+enum psx_direct_light_sampling {
+    PSX_DIRECT_LIGHT_MIS = 0,
+    PSX_DIRECT_LIGHT_FORWARD = 1,
+    PSX_DIRECT_LIGHT_NEE = 2
+};
+
+/* contract::EmissionSampling (include/psycles/contract/scene.h:72-78) */
+enum psx_emission_sampling {
+    PSX_EMISSION_SAMPLING_NONE = 0,
+    PSX_EMISSION_SAMPLING_AUTOMATIC = 1,
+    PSX_EMISSION_SAMPLING_FRONT = 2,
+    PSX_EMISSION_SAMPLING_BACK = 3,
+    PSX_EMISSION_SAMPLING_FRONT_BACK = 4
+};
+
+/* contract::RayVisibility bits (scene.h:306-320) */
+enum psx_visibility {
+    PSX_VISIBILITY_CAMERA = 1u << 0,
+    PSX_VISIBILITY_DIFFUSE = 1u << 1,
+    PSX_VISIBILITY_GLOSSY = 1u << 2,
+    PSX_VISIBILITY_TRANSMISSION = 1u << 3,
+    PSX_VISIBILITY_SHADOW = 1u << 4,
+    PSX_VISIBILITY_VOLUME_SCATTER = 1u << 5,
+    PSX_VISIBILITY_ALL = 0x3fu
+};
+
+/* Closure-tree operations of a SurfaceProgram
+ * (compiler::ClosureOperation, include/psycles/compiler/surface_program.h:383-396),
+ * restricted to the closures this render path evaluates. */
+enum psx_closure_op {
+    PSX_CLOSURE_NULL = 0,
+    PSX_CLOSURE_ADD = 1,
+    PSX_CLOSURE_MIX = 2,
+    PSX_CLOSURE_DIFFUSE = 3,
+    PSX_CLOSURE_EMISSION = 4,
+    PSX_CLOSURE_TRANSPARENT = 5
+};
+
+/* contract::LightType (scene.h:369-375) */
+enum psx_light_type {
+    PSX_LIGHT_POINT = 0,
+    PSX_LIGHT_SPOT = 1,
+    PSX_LIGHT_AREA = 2,
+    PSX_LIGHT_DISTANT = 3,
+    PSX_LIGHT_BACKGROUND = 4
+};
+
+/* ---- scene snapshot (flat form of contract::SceneSnapshot) -------------- */
+
+/* One closure instruction.  Child and parameter fields are indices:
+ *   a, b       child instruction indices (ADD, MIX)
+ *   factor     scalar parameter index (MIX factor, clamped to [0,1])
+ *   color      first of three consecutive parameter floats (DIFFUSE,
+ *              EMISSION, TRANSPARENT)
+ *   strength   scalar parameter index (EMISSION)
+ *   roughness  scalar parameter index (DIFFUSE; 0 selects Lambert)
+ * Unused fields must be zero.  Mirrors compiler::ClosureInstruction
+ * (surface_program.h:398-440) with constant inputs lowered to parameters. */
+typedef struct psx_closure_instr {
+    uint32_t op;
+    uint32_t a;
+    uint32_t b;
+    uint32_t factor;
+    uint32_t color;
+    uint32_t strength;
+    uint32_t roughness;
+    uint32_t reserved;
+} psx_closure_instr;
+
+/* A material = SurfaceProgram (instructions + root) + SurfaceParameterBlock.
+ * Materials whose instruction arrays are equal share one compiled program
+ * (the structure signature of docs/architecture.md:63-72); they differ only in
+ * their parameter blocks. */
+typedef struct psx_material {
+    const psx_closure_instr *instructions;
+    uint32_t instruction_count;
+    uint32_t root;
+    const float *parameters;
+    uint32_t parameter_count;
+    uint32_t emission_sampling; /* enum psx_emission_sampling */
+} psx_material;
+
+/* contract::TriangleMeshDesc (scene.h:182-239), triangle subset. */
+typedef struct psx_mesh {
+    const float *positions;          /* 3 * vertex_count                    */
+    uint32_t vertex_count;
+    const uint32_t *triangles;       /* 3 * triangle_count                  */
+    uint32_t triangle_count;
+    const float *normals;            /* optional point-domain normals, 3*V  */
+    const uint8_t *triangle_smooth;  /* optional, one per triangle          */
+    const uint32_t *material_slots;  /* material index per slot             */
+    uint32_t material_slot_count;
+    const uint32_t *triangle_material_slots; /* optional, one per triangle  */
+} psx_mesh;
+
+/* contract::InstanceDesc (scene.h:304-333). */
+typedef struct psx_instance {
+    uint32_t mesh;
+    uint32_t visibility_mask;        /* enum psx_visibility bits            */
+    float transform[16];
+    const uint32_t *material_overrides;
+    uint32_t material_override_count;
+    float shadow_terminator_geometry_offset;
+} psx_instance;
+
+/* contract::LightDesc (scene.h:377-403).  Only BACKGROUND lights are
+ * rendered by this ABI version; they fold into the constant world radiance
+ * exactly like src/luisa/path_tracer_scene.cpp:1728-1733. */
+typedef struct psx_light {
+    uint32_t type;
+    uint32_t visibility_mask;
+    float transform[16];
+    float color[3];
+    float power;
+    float size;
+    float size_y;
+    float spread;
+    float spot_angle;
+    float spot_smooth;
+    float angle;
+    uint32_t flags;                  /* bit0 normalize, bit1 ellipse,
+                                        bit2 sphere, bit3 use_mis,
+                                        bit4 cast_shadow                    */
+    uint32_t max_bounces;
+} psx_light;
+
+/* contract::CameraDesc (scene.h:347-367), perspective subset. */
+typedef struct psx_camera {
+    uint32_t projection;             /* 0 = perspective                     */
+    uint32_t sensor_fit;             /* 0 horizontal, 1 vertical, 2 auto    */
+    float transform[16];
+    float field_of_view;
+    float horizontal_field_of_view;
+    float lens_shift_x;
+    float lens_shift_y;
+    float near_clip;
+    float far_clip;
+} psx_camera;
+
+typedef struct psx_scene_desc {
+    uint32_t abi_version;            /* PSX_ABI_VERSION                     */
+    const psx_material *materials;
+    uint32_t material_count;
+    const psx_mesh *meshes;
+    uint32_t mesh_count;
+    const psx_instance *instances;
+    uint32_t instance_count;
+    const psx_light *lights;
+    uint32_t light_count;
+    psx_camera camera;
+    /* Constant world shader radiance (Background color * strength); the
+     * reference never puts a spatially constant world into the light
+     * distribution (src/luisa/path_tracer_scene.cpp:1896-1911). */
+    float world_color[3];
+    uint32_t world_visibility_mask;
+} psx_scene_desc;
+
+/* ---- render settings (contract::RenderSettings, render.h:81-117) -------- */
+typedef struct psx_integrator_settings {
+    uint32_t max_bounces;
+    uint32_t min_bounces;
+    uint32_t diffuse_bounces;
+    uint32_t glossy_bounces;
+    uint32_t transmission_bounces;
+    uint32_t volume_bounces;
+    uint32_t transparent_min_bounces;
+    uint32_t transparent_max_bounces;
+    float sample_clamp_direct;
+    float sample_clamp_indirect;
+    float filter_glossy;
+    float film_exposure;
+    float light_sampling_threshold;
+    uint32_t reflective_caustics;
+    uint32_t refractive_caustics;
+    uint32_t use_light_tree;
+    uint32_t direct_light_sampling;  /* enum psx_direct_light_sampling      */
+} psx_integrator_settings;
+
+typedef struct psx_render_settings {
+    uint32_t width;                  /* full extent                         */
+    uint32_t height;
+    uint32_t window_x;               /* pixel window; width 0 = full frame  */
+    uint32_t window_y;
+    uint32_t window_width;
+    uint32_t window_height;
+    uint32_t seed;
+    uint32_t transparent_background;
+    uint32_t pixel_filter;           /* enum psx_pixel_filter               */
+    float filter_width;
+    float pass_alpha_threshold;
+    psx_integrator_settings integrator;
+    const uint32_t *passes;          /* enum psx_pass_kind per request      */
+    const uint32_t *pass_channels;   /* channels per request                */
+    uint32_t pass_count;
+} psx_render_settings;
+
+/* Default values of contract::RenderSettings / PathIntegratorSettings. */
+void psx_render_settings_default(psx_render_settings *settings);
+
+/* ---- output sink (contract::OutputSink, render.h:136-143) --------------- */
+typedef struct psx_sink {
+    void *user;
+    void (*begin)(void *user, const psx_render_settings *settings);
+    /* pixels are row-major over the window, `channels` floats per pixel;
+     * valid only for the duration of the call (PassTile::pixels). */
+    void (*write)(void *user, uint32_t pass_kind, uint32_t channels,
+                  uint32_t window_x, uint32_t window_y,
+                  uint32_t window_width, uint32_t window_height,
+                  uint32_t full_width, uint32_t full_height,
+                  const float *pixels);
+    void (*end)(void *user, int cancelled);
+} psx_sink;
+
+/* Diagnostic callback (contract::RenderDiagnostic). */
+typedef void (*psx_diag_fn)(void *user, const char *message);
+
+/* ---- handles ------------------------------------------------------------ */
+typedef struct psx_backend psx_backend;
+typedef struct psx_scene psx_scene;
+typedef struct psx_session psx_session;
+
+/* Backend options (luisa_backend::LuisaPathTracerOptions,
+ * include/psycles/luisa/path_tracer.h:45-59). */
+typedef struct psx_backend_options {
+    uint32_t next_event_estimation;  /* default 1                           */
+    uint32_t max_samples_per_dispatch; /* default 4; batches kept in order  */
+    uint32_t integrator;             /* 0 = megakernel, 1 = wavefront       */
+    uint32_t reserved;
+} psx_backend_options;
+
+const char *psx_version(void);
+int psx_device_count(void);
+
+int psx_backend_create(int device_ordinal, const psx_backend_options *options,
+                       psx_backend **out_backend);
+void psx_backend_destroy(psx_backend *backend);
+
+int psx_scene_compile(psx_backend *backend, const psx_scene_desc *desc,
+                      psx_diag_fn diagnostics, void *diagnostics_user,
+                      psx_scene **out_scene);
+void psx_scene_destroy(psx_scene *scene);
+/* Replace the parameter block of one material without recompiling its
+ * program.  Fails with PSX_ERROR_INVALID_ARGUMENT when parameter_count
+ * differs from the compiled block (a structure change needs a recompile). */
+int psx_scene_update_parameters(psx_scene *scene, uint32_t material,
+                                const float *parameters,
+                                uint32_t parameter_count);
+
+int psx_session_create(psx_scene *scene, const psx_render_settings *settings,
+                       psx_session **out_session);
+void psx_session_destroy(psx_session *session);
+int psx_session_reset(psx_session *session,
+                      const psx_render_settings *settings);
+/* SampleRange{first, count, offset, total}.  Returns PSX_OK when the sink
+ * received every requested pass, otherwise the reason render_samples would
+ * have returned false. */
+int psx_session_render(psx_session *session, uint32_t first, uint32_t count,
+                       uint32_t offset, uint32_t total, const psx_sink *sink);
+/* Thread-safe. */
+void psx_session_cancel(psx_session *session);
+
+/* ---- measurement and inspection ---------------------------------------- */
+
+typedef struct psx_render_stats {
+    uint64_t rays;            /* camera + continuation + shadow re-queries  */
+    uint64_t samples;         /* completed pixel-samples                    */
+    uint64_t dispatches;      /* kernel launches of the render loop         */
+    double kernel_ms;         /* device time of the render kernels          */
+    uint64_t node_visits;     /* BVH node records fetched (0 if disabled)   */
+    uint64_t triangle_tests;  /* triangle records tested (0 if disabled)    */
+} psx_render_stats;
+
+/* Statistics of the most recent psx_session_render. */
+int psx_session_stats(const psx_session *session, psx_render_stats *stats);
+
+/* Runs the render loop on samples [first, first+count) without reading the
+ * film back or calling a sink: the timed region of the benchmark.  The film
+ * keeps accumulating exactly like psx_session_render. */
+int psx_session_render_device(psx_session *session, uint32_t first,
+                              uint32_t count, uint32_t total);
+
+/* Raw accumulated film, row-major over the window: combined float4 (rgb sum,
+ * transparency sum), normal float4, albedo float4 and the u32 sample count.
+ * Any pointer may be NULL. */
+int psx_session_read_film(psx_session *session, float *combined,
+                          float *normal, float *albedo,
+                          uint32_t *sample_count);
+
+/* Primary-ray probe: for every window pixel at sample `sample_index`, the
+ * closest camera-ray hit as (object, primitive, t bits, hit flag). */
+int psx_session_primary_hits(psx_session *session, uint32_t sample_index,
+                             uint32_t total, uint32_t *out_records);
+
+/* Path event log for one sample index: per pixel, PSX_TRACE_EVENTS records of
+ * PSX_TRACE_WORDS u32 each (see psx_trace_word). */
+#define PSX_TRACE_EVENTS 8u
+#define PSX_TRACE_WORDS 8u
+enum psx_trace_word {
+    PSX_TRACE_FLAGS = 0,      /* bit0 valid, bit1 surface, bit2 background,
+                                 bit3 rr_terminated, bit4 nee_attempted,
+                                 bit5 scatter_valid, bit6 transparent      */
+    PSX_TRACE_RNG_OFFSET = 1,
+    PSX_TRACE_OBJECT = 2,
+    PSX_TRACE_PRIMITIVE = 3,
+    PSX_TRACE_LIGHT = 4,      /* selected distribution entry or ~0u         */
+    PSX_TRACE_CLOSURE = 5,    /* selected closure index or ~0u              */
+    PSX_TRACE_TERMINATE_BITS = 6, /* float bits of u_term                   */
+    PSX_TRACE_RR_BITS = 7     /* float bits of continuation probability    */
+};
+int psx_session_trace_events(psx_session *session, uint32_t sample_index,
+                             uint32_t total, uint32_t *out_records);
+
+/* Sobol LUT as built for `total` samples (256 patterns x sequence size
+ * float4); out may be NULL to query *out_count. */
+int psx_sobol_table(uint32_t total, float *out, uint32_t *out_count);
+
+/* Per-pixel sampler probe on the device: for each pixel of the window and the
+ * given sample index, the u32 bit patterns of sample_4d at `dimension_count`
+ * consecutive dimensions starting at `first_dimension`, plus rng_hash. */
+int psx_session_sampler_probe(psx_session *session, uint32_t sample_index,
+                              uint32_t total, uint32_t first_dimension,
+                              uint32_t dimension_count, uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSX_H */

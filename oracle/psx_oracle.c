@@ -1,0 +1,2735 @@
+/*
+ * psx_oracle.c - CPU restatement of Psycles' device render path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP render path, never part
+ * of the product.  Plain C99 compiled with -ffp-contract=off and without
+ * fast-math: every float expression below is evaluated in the order written,
+ * and fmaf() appears exactly where the reference fuses.  Transcendentals
+ * (sinf, cosf, acosf, atan2f, tanf, expf) come from the host libm, as they do
+ * in the reference's LLVM fallback backend.
+ *
+ * Conventions shared with the reference's Luisa lowering (the bit patterns of
+ * these helpers are not pinned by any reference vector; they are the stated
+ * contract of this restatement):
+ *   dot(a, b)      = (a.x*b.x + a.y*b.y) + a.z*b.z
+ *   cross(a, b)    = (a.y*b.z - a.z*b.y, a.z*b.x - a.x*b.z, a.x*b.y - a.y*b.x)
+ *   normalize(v)   = v * (1 / sqrt(dot(v, v)))
+ *   M * (v, w)     = ((c0*x + c1*y) + c2*z) + c3*w   (column-major float4x4)
+ *   lerp(a, b, t)  = a + t * (b - a)
+ *   min/max        = fminf/fmaxf
+ */
+#define _GNU_SOURCE
+#include "psx_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+/* ======================================================================= */
+/* small vector algebra                                                     */
+/* ======================================================================= */
+
+typedef struct { float x, y, z; } v3;
+typedef struct { float x, y; } v2;
+
+static inline v3 V3(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmul(v3 a, v3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 vdiv(v3 a, v3 b) { return V3(a.x / b.x, a.y / b.y, a.z / b.z); }
+static inline v3 vscale(v3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+static inline v3 vdivs(v3 a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
+static inline v3 vneg(v3 a) { return V3(-a.x, -a.y, -a.z); }
+static inline float vdot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 vcross(v3 a, v3 b) {
+    return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z,
+              a.x * b.y - a.y * b.x);
+}
+static inline float vlength(v3 a) { return sqrtf(vdot(a, a)); }
+static inline v3 vnormalize(v3 a) {
+    const float inv = 1.0f / sqrtf(vdot(a, a));
+    return vscale(a, inv);
+}
+static inline v3 vsel(v3 a, v3 b, int c) { return c ? b : a; }
+static inline float fsel(float a, float b, int c) { return c ? b : a; }
+static inline float fclamp(float x, float lo, float hi) {
+    return fminf(fmaxf(x, lo), hi);
+}
+static inline v3 vmax(v3 a, v3 b) {
+    return V3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z));
+}
+static inline v3 vclamp(v3 a, float lo, float hi) {
+    return V3(fclamp(a.x, lo, hi), fclamp(a.y, lo, hi), fclamp(a.z, lo, hi));
+}
+static inline int vany_nonzero(v3 a) {
+    return a.x != 0.0f || a.y != 0.0f || a.z != 0.0f;
+}
+static inline int vany_nan(v3 a) { return isnan(a.x) || isnan(a.y) || isnan(a.z); }
+static inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float bitsf(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+static inline v3 load3(const float *p) { return V3(p[0], p[1], p[2]); }
+
+/* safe_normalize (src/luisa/path_tracer_lighting.cpp:9-18 and
+ * src/luisa/graph_surface_state.cpp:176-189). */
+static v3 safe_normalize(v3 value, v3 fallback) {
+    const int valid = vdot(value, value) > 1.0e-20f;
+    v3 selected = vsel(fallback, value, valid);
+    const int fallback_valid = vdot(selected, selected) > 1.0e-20f;
+    selected = vsel(V3(0.0f, 0.0f, 1.0f), selected, fallback_valid);
+    return vnormalize(selected);
+}
+
+/* normalize_or (include/psycles/luisa/spherical_geometry.h:34-40;
+ * src/luisa/path_kernel_emissive_triangle.cpp:10-22). */
+static v3 normalize_or(v3 value, v3 fallback) {
+    const float length_squared = vdot(value, value);
+    return length_squared > 1.0e-20f
+               ? vdivs(value, sqrtf(fmaxf(length_squared, 1.0e-20f)))
+               : fallback;
+}
+
+/* Column-major 4x4 matrix, elements[12..14] = translation (psycles::Mat4f). */
+typedef struct { float m[16]; } mat4;
+
+/* cycles_transform::point / direction (src/luisa/cycles_transform.cpp:6-27):
+ * the explicit FMA tree of the Cycles affine transform. */
+static v3 affine(const mat4 *t, v3 v, int translate) {
+    const float *e = t->m;
+    const float tx = translate ? e[12] : 0.0f;
+    const float ty = translate ? e[13] : 0.0f;
+    const float tz = translate ? e[14] : 0.0f;
+    return V3(fmaf(v.x, e[0], fmaf(v.y, e[4], fmaf(v.z, e[8], tx))),
+              fmaf(v.x, e[1], fmaf(v.y, e[5], fmaf(v.z, e[9], ty))),
+              fmaf(v.x, e[2], fmaf(v.y, e[6], fmaf(v.z, e[10], tz))));
+}
+static inline v3 xform_point(const mat4 *t, v3 v) { return affine(t, v, 1); }
+static inline v3 xform_dir(const mat4 *t, v3 v) { return affine(t, v, 0); }
+
+/* Luisa float4x4 * float4 (columns accumulated in order, no fusion). */
+static v3 mat_mul(const mat4 *t, v3 v, float w) {
+    const float *e = t->m;
+    v3 r;
+    r.x = ((e[0] * v.x + e[4] * v.y) + e[8] * v.z) + e[12] * w;
+    r.y = ((e[1] * v.x + e[5] * v.y) + e[9] * v.z) + e[13] * w;
+    r.z = ((e[2] * v.x + e[6] * v.y) + e[10] * v.z) + e[14] * w;
+    return r;
+}
+
+static mat4 transpose4(const mat4 *a) {
+    mat4 r;
+    for (int c = 0; c < 4; ++c)
+        for (int row = 0; row < 4; ++row) r.m[c * 4 + row] = a->m[row * 4 + c];
+    return r;
+}
+
+/* cycles_inverse_transform (src/luisa/path_tracer_scene_geometry.cpp:231-268). */
+static mat4 cycles_inverse(const mat4 *t) {
+    const float *e = t->m;
+    v3 x = V3(e[0], e[1], e[2]);
+    v3 y = V3(e[4], e[5], e[6]);
+    v3 z = V3(e[8], e[9], e[10]);
+    const v3 translation = V3(e[12], e[13], e[14]);
+    float det = vdot(x, vcross(y, z));
+    if (det == 0.0f) {
+        x.x += 1.0e-8f;
+        y.y += 1.0e-8f;
+        z.z += 1.0e-8f;
+        det = vdot(x, vcross(y, z));
+        if (det == 0.0f) det = FLT_MAX;
+    }
+    const v3 ix = vdivs(vcross(y, z), det);
+    const v3 iy = vdivs(vcross(z, x), det);
+    const v3 iz = vdivs(vcross(x, y), det);
+    mat4 r;
+    const float vals[16] = {ix.x, iy.x, iz.x, 0.0f, ix.y, iy.y, iz.y, 0.0f,
+                            ix.z, iy.z, iz.z, 0.0f, -vdot(ix, translation),
+                            -vdot(iy, translation), -vdot(iz, translation),
+                            1.0f};
+    memcpy(r.m, vals, sizeof(vals));
+    return r;
+}
+
+/* ======================================================================= */
+/* tabulated Sobol (src/sampling/tabulated_sobol.cpp:24-199)                */
+/* ======================================================================= */
+
+static const uint32_t sobol_xors[4][32] = {
+    {0u},
+    {0x00000000u, 0x00000001u, 0x00000001u, 0x00000007u, 0x00000001u,
+     0x00000013u, 0x00000015u, 0x0000007fu, 0x00000001u, 0x00000103u,
+     0x00000105u, 0x0000070fu, 0x00000111u, 0x00001333u, 0x00001555u,
+     0x00007fffu, 0x00000001u, 0x00010003u, 0x00010005u, 0x0007000fu,
+     0x00010011u, 0x00130033u, 0x00150055u, 0x007f00ffu, 0x00010101u,
+     0x01030303u, 0x01050505u, 0x070f0f0fu, 0x01111111u, 0x13333333u,
+     0x15555555u, 0x7fffffffu},
+    {0x00000000u, 0x00000001u, 0x00000003u, 0x00000001u, 0x00000005u,
+     0x0000001fu, 0x0000002bu, 0x0000003du, 0x00000011u, 0x00000133u,
+     0x00000377u, 0x00000199u, 0x00000445u, 0x00001ccfu, 0x00002ddbu,
+     0x0000366du, 0x00000101u, 0x00010303u, 0x00030707u, 0x00010909u,
+     0x00051515u, 0x001f3f3fu, 0x002b6b6bu, 0x003dbdbdu, 0x00101011u,
+     0x01303033u, 0x03707077u, 0x01909099u, 0x04515145u, 0x1cf3f3cfu,
+     0x2db6b6dbu, 0x36dbdb6du},
+    {0x00000000u, 0x00000001u, 0x00000000u, 0x00000003u, 0x0000000du,
+     0x0000000cu, 0x00000005u, 0x0000004fu, 0x00000014u, 0x000000e7u,
+     0x00000329u, 0x0000039cu, 0x00000011u, 0x00001033u, 0x00000044u,
+     0x000030bbu, 0x0000d1cdu, 0x0000c2ecu, 0x00005415u, 0x0004fc3fu,
+     0x00015054u, 0x000e5c97u, 0x0032e5b9u, 0x0039725cu, 0x00000101u,
+     0x01000303u, 0x00000404u, 0x03000b0bu, 0x0d001d1du, 0x0c002c2cu,
+     0x05004545u, 0x4f00cfcfu}};
+
+static uint32_t hash_hp_uint(uint32_t v) {
+    v ^= v >> 16u;
+    v *= 0x21f0aaadu;
+    v ^= v >> 15u;
+    v *= 0xd35a2d97u;
+    v ^= v >> 15u;
+    return v ^ 0xe6fe3bebu;
+}
+
+static uint32_t hash_hp_seeded_uint(uint32_t v, uint32_t seed) {
+    seed ^= seed << 19u;
+    return hash_hp_uint(v ^ seed);
+}
+
+static float hash_hp_float(uint32_t v) {
+    return (float)hash_hp_uint(v) * (1.0f / 4294967808.0f);
+}
+
+static void sobol_generate_sequence(float *points, uint32_t size,
+                                     uint32_t rng_seed) {
+    uint32_t rng_index = hash_hp_seeded_uint(rng_seed, 0x44605a73u);
+    for (int c = 0; c < 4; ++c) points[c] = hash_hp_float(rng_index++);
+    uint32_t log_n = 0u;
+    for (uint32_t n = 1u; n < size; ++log_n, n *= 2u) {
+        const float strata = (float)(n * 2u);
+        for (uint32_t i = 0u; i < n && n + i < size; ++i) {
+            uint32_t occupied[4];
+            for (int c = 0; c < 4; ++c)
+                occupied[c] = (uint32_t)(points[4u * (i ^ sobol_xors[c][log_n]) + c] *
+                                         strata);
+            float *point = points + 4u * (n + i);
+            for (int c = 0; c < 4; ++c)
+                point[c] = ((float)(occupied[c] ^ 1u) + hash_hp_float(rng_index++)) /
+                           strata;
+        }
+    }
+}
+
+uint32_t oracle_sequence_size_for_samples(uint32_t sample_count) {
+    uint32_t clamped = sample_count < 1u ? 1u : sample_count;
+    if (clamped > 8192u) clamped = 8192u;
+    uint32_t unbounded = 1u;
+    if (clamped != 1u) {
+        unbounded = 1u << (32u - (uint32_t)__builtin_clz(clamped - 1u));
+    }
+    if (unbounded < 256u) unbounded = 256u;
+    if (unbounded > 8192u) unbounded = 8192u;
+    return unbounded;
+}
+
+int oracle_sobol_table(uint32_t sequence_size, float *out) {
+    if (sequence_size < 256u || sequence_size > 8192u ||
+        (sequence_size & (sequence_size - 1u)) != 0u)
+        return -1;
+    for (uint32_t pattern = 0u; pattern < 256u; ++pattern)
+        sobol_generate_sequence(out + 4u * (size_t)pattern * sequence_size,
+                                sequence_size, pattern);
+    return 0;
+}
+
+/* ======================================================================= */
+/* device sampler (include/psycles/luisa/cycles_sampler.h:36-271)           */
+/* ======================================================================= */
+
+static uint32_t reverse_bits(uint32_t v) {
+    v = ((v >> 1u) & 0x55555555u) | ((v & 0x55555555u) << 1u);
+    v = ((v >> 2u) & 0x33333333u) | ((v & 0x33333333u) << 2u);
+    v = ((v >> 4u) & 0x0f0f0f0fu) | ((v & 0x0f0f0f0fu) << 4u);
+    v = ((v >> 8u) & 0x00ff00ffu) | ((v & 0x00ff00ffu) << 8u);
+    return (v >> 16u) | (v << 16u);
+}
+
+static uint32_t reversed_bit_owen(uint32_t v, uint32_t seed) {
+    v ^= v * 0x3d20adeau;
+    v += seed;
+    v *= (seed >> 16u) | 1u;
+    v ^= v * 0x05526c56u;
+    v ^= v * 0x53a22864u;
+    return v;
+}
+
+static uint32_t nested_uniform_scramble(uint32_t v, uint32_t seed) {
+    return reverse_bits(reversed_bit_owen(reverse_bits(v), seed));
+}
+
+static uint32_t hash_wang_seeded_uint(uint32_t v, uint32_t seed) {
+    v = (v ^ 61u) ^ seed;
+    v += v << 3u;
+    v ^= v >> 4u;
+    v *= 0x27d4eb2du;
+    return v;
+}
+
+static uint32_t shuffled_pattern_index(uint32_t dimension, uint32_t seed) {
+    uint32_t v = dimension & 0xffu;
+    const uint32_t mask = 0xffu;
+    v ^= seed;
+    v *= 0xe170893du;
+    v ^= seed >> 16u;
+    v ^= (v & mask) >> 4u;
+    v ^= seed >> 8u;
+    v *= 0x0929eb3fu;
+    v ^= seed >> 23u;
+    v ^= (v & mask) >> 1u;
+    v *= 1u | (seed >> 27u);
+    v *= 0x6935fa69u;
+    v ^= (v & mask) >> 11u;
+    v *= 0x74dcb303u;
+    v ^= (v & mask) >> 2u;
+    v *= 0x9e501cc3u;
+    v ^= (v & mask) >> 2u;
+    v *= 0xc860a3dfu;
+    v &= mask;
+    v ^= v >> 5u;
+    return v;
+}
+
+uint32_t oracle_pixel_hash(uint32_t x, uint32_t y, uint32_t seed) {
+    const uint32_t qx = 1103515245u * ((x >> 1u) ^ y);
+    const uint32_t qy = 1103515245u * ((y >> 1u) ^ x);
+    const uint32_t hash = 1103515245u * (qx ^ (qy >> 3u));
+    return hash ^ seed;
+}
+
+uint32_t oracle_shuffled_sample_index(uint32_t sample, uint32_t dimension,
+                                      uint32_t seed, uint32_t sequence_size) {
+    const uint32_t pattern = shuffled_pattern_index(dimension, seed);
+    const uint32_t mask = sequence_size - 1u;
+    const uint32_t shuffled =
+        nested_uniform_scramble(sample, hash_wang_seeded_uint(dimension, seed));
+    sample = (sample & ~mask) | (shuffled & mask);
+    return ((pattern * sequence_size) + sample) % (sequence_size * 256u);
+}
+
+typedef struct sampler {
+    const float *table;
+    uint32_t sequence_size;
+} sampler;
+
+static const float *sample_4d(const sampler *s, uint32_t sample,
+                              uint32_t rng_hash, uint32_t dimension) {
+    return s->table +
+           4u * (size_t)oracle_shuffled_sample_index(sample, dimension, rng_hash,
+                                                     s->sequence_size);
+}
+
+/* ======================================================================= */
+/* pixel filter (src/sampling/pixel_filter.cpp:30-86;                       */
+/*               include/psycles/luisa/pixel_filter.h:16-28)                */
+/* ======================================================================= */
+
+static float filter_eval(uint32_t filter, float value, float width) {
+    if (filter == PSX_FILTER_GAUSSIAN) {
+        value *= 6.0f / width;
+        return expf(-2.0f * value * value);
+    }
+    if (filter == PSX_FILTER_BLACKMAN_HARRIS) {
+        const float two_pi = 6.28318530717958647692f;
+        value = two_pi * (value / width + 0.5f);
+        return 0.35875f - 0.48829f * cosf(value) + 0.14128f * cosf(2.0f * value) -
+               0.01168f * cosf(3.0f * value);
+    }
+    return 1.0f;
+}
+
+void oracle_pixel_filter_table(uint32_t filter, float width, float *out) {
+    enum { size = 1024 };
+    width = fmaxf(width, 1.0e-5f);
+    if (filter == PSX_FILTER_GAUSSIAN) width *= 3.0f;
+    else if (filter == PSX_FILTER_BLACKMAN_HARRIS) width *= 2.0f;
+    const unsigned resolution = size - 1u;
+    float cdf[size];
+    memset(cdf, 0, sizeof(cdf));
+    const float range = width * 0.5f;
+    for (unsigned i = 0; i < resolution; ++i) {
+        const float value = range * (float)i / (float)(resolution - 1u);
+        cdf[i + 1u] = cdf[i] + fabsf(filter_eval(filter, value, width));
+    }
+    const float normalization =
+        cdf[resolution] == 0.0f ? 0.0f : 1.0f / cdf[resolution];
+    for (unsigned i = 0; i < size; ++i) cdf[i] *= normalization;
+    cdf[resolution] = 1.0f;
+    memset(out, 0, sizeof(float) * size);
+    const unsigned half = (size - 1u) / 2u;
+    for (unsigned i = 0; i <= half; ++i) {
+        const float x = (float)i / (float)half;
+        /* std::upper_bound: first element > x */
+        unsigned index = 0;
+        while (index < size && !(x < cdf[index])) ++index;
+        float interpolation = 0.0f;
+        if (index < resolution) {
+            interpolation = (x - cdf[index]) / (cdf[index + 1u] - cdf[index]);
+        } else {
+            index = resolution;
+        }
+        const float y =
+            ((float)index + interpolation) / (float)(size - 1u) * (2.0f * range);
+        out[half + i] = 0.5f * (1.0f + y);
+        out[half - i] = 0.5f * (1.0f - y);
+    }
+}
+
+float oracle_pixel_filter_sample(const float *table, float random) {
+    const unsigned last = 1023u;
+    const float coordinate = fclamp(random, 0.0f, 1.0f) * (float)last;
+    unsigned index = (unsigned)coordinate;
+    if (index > last) index = last;
+    const unsigned next = index + 1u < last ? index + 1u : last;
+    const float t = coordinate - (float)index;
+    return table[index] + t * (table[next] - table[index]);
+}
+
+/* ======================================================================= */
+/* sample mappings (include/psycles/luisa/cycles_sample_mapping.h:49-121)  */
+/* ======================================================================= */
+
+static const float PI_F = 3.14159265358979323846f;
+static const float INV_PI_F = 0.31830988618379067154f;
+
+static v2 sample_uniform_disk(v2 random) {
+    const float a = 2.0f * random.x - 1.0f;
+    const float b = 2.0f * random.y - 1.0f;
+    const int x_major = a * a > b * b;
+    const float safe_a = a != 0.0f ? a : 1.0f;
+    const float safe_b = b != 0.0f ? b : 1.0f;
+    const float radius = x_major ? a : b;
+    const float angle = x_major ? 0.25f * PI_F * (b / safe_a)
+                                : 0.5f * PI_F - 0.25f * PI_F * (a / safe_b);
+    v2 mapped = {cosf(angle) * radius, sinf(angle) * radius};
+    if (a == 0.0f && b == 0.0f) mapped.x = mapped.y = 0.0f;
+    return mapped;
+}
+
+static void make_orthonormals(v3 n, v3 *tangent, v3 *bitangent) {
+    const v3 general = V3(n.z - n.y, n.x - n.z, n.y - n.x);
+    const v3 equal = V3(n.z - n.y, n.x + n.z, -n.y - n.x);
+    const int use_general = (n.x != n.y) || (n.x != n.z);
+    *tangent = vnormalize(use_general ? general : equal);
+    *bitangent = vcross(n, *tangent);
+}
+
+static v3 sample_cosine_hemisphere(v3 normal, v2 random, float *pdf) {
+    const v2 disk = sample_uniform_disk(random);
+    const float cosine = sqrtf(fmaxf(1.0f - (disk.x * disk.x + disk.y * disk.y), 0.0f));
+    v3 t, b;
+    make_orthonormals(normal, &t, &b);
+    if (pdf) *pdf = cosine * INV_PI_F;
+    return vadd(vadd(vscale(t, disk.x), vscale(b, disk.y)), vscale(normal, cosine));
+}
+
+void oracle_sample_uniform_disk(const float random[2], float out[2]) {
+    v2 r = {random[0], random[1]};
+    v2 d = sample_uniform_disk(r);
+    out[0] = d.x;
+    out[1] = d.y;
+}
+
+void oracle_sample_cosine_hemisphere(const float normal[3], const float random[2],
+                                     float out[4]) {
+    v2 r = {random[0], random[1]};
+    float pdf;
+    v3 d = sample_cosine_hemisphere(load3(normal), r, &pdf);
+    out[0] = d.x;
+    out[1] = d.y;
+    out[2] = d.z;
+    out[3] = pdf;
+}
+
+/* ======================================================================= */
+/* camera helpers (include/psycles/luisa/camera_sampling.h:81-107)         */
+/* ======================================================================= */
+
+void oracle_camera_clip_range(float near_clip, float far_clip, float cosine,
+                              float out[2]) {
+    const float safe = fmaxf(fabsf(cosine), 1.0e-20f);
+    out[0] = near_clip / safe;
+    out[1] = (far_clip - near_clip) / safe;
+}
+
+void oracle_affine_point(const float m[16], const float p[3], float out[3]) {
+    mat4 t;
+    memcpy(t.m, m, sizeof(t.m));
+    v3 r = xform_point(&t, load3(p));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+/* ======================================================================= */
+/* Cycles path state (include/psycles/luisa/cycles_path_state.h)           */
+/* ======================================================================= */
+
+enum {
+    FLAG_REFLECT = 1u << 0, FLAG_SINGULAR = 1u << 1, FLAG_TRANSPARENT = 1u << 2,
+    FLAG_IMPORTANCE_BAKE = 1u << 3, FLAG_DIFFUSE_ANCESTOR = 1u << 4,
+    FLAG_EMISSION = 1u << 5, FLAG_MIS_HAD_TRANSMISSION = 1u << 6,
+    FLAG_MIS_SKIP = 1u << 7, FLAG_SINGLE_PASS_DONE = 1u << 8,
+    FLAG_TRANSPARENT_BACKGROUND = 1u << 9,
+    FLAG_TERMINATE_ON_NEXT_SURFACE = 1u << 10,
+    FLAG_TERMINATE_IN_NEXT_VOLUME = 1u << 11,
+    FLAG_TERMINATE_AFTER_TRANSPARENT = 1u << 12,
+    FLAG_SURFACE_PASS = 1u << 18, FLAG_VOLUME_PASS = 1u << 19,
+    FLAG_ANY_PASS = (1u << 18) | (1u << 19)
+};
+enum {
+    CVIS_CAMERA = 1u << 0, CVIS_TRANSMIT = 1u << 1, CVIS_DIFFUSE = 1u << 2,
+    CVIS_GLOSSY = 1u << 3, CVIS_VOLUME_SCATTER = 1u << 4
+};
+enum {
+    LABEL_TRANSMIT = 1u, LABEL_REFLECT = 2u, LABEL_DIFFUSE = 4u,
+    LABEL_GLOSSY = 8u, LABEL_SINGULAR = 16u, LABEL_TRANSPARENT = 32u,
+    LABEL_TRANSMIT_TRANSPARENT = 128u, LABEL_SUBSURFACE = 256u,
+    LABEL_RAY_PORTAL = 512u
+};
+enum {
+    RT_BACKFACING = 1u << 0, RT_EMISSION = 1u << 2, RT_BSDF = 1u << 3,
+    RT_BSDF_HAS_EVAL = 1u << 4, RT_TRANSPARENT = 1u << 10,
+    RT_BSDF_HAS_TRANSMISSION = 1u << 11, RT_RAY_PORTAL = 1u << 12
+};
+enum {
+    EV_DIFFUSE = 1u << 0, EV_GLOSSY = 1u << 1, EV_SINGULAR = 1u << 2,
+    EV_REFLECTION = 1u << 3, EV_TRANSMISSION = 1u << 4,
+    EV_TRANSPARENT = 1u << 5, EV_SUBSURFACE = 1u << 6
+};
+
+void oracle_initial_state(oracle_path_state *s) {
+    memset(s, 0, sizeof(*s));
+    s->flag = FLAG_MIS_SKIP | FLAG_TRANSPARENT_BACKGROUND;
+    s->visibility = CVIS_CAMERA;
+    s->rng_offset = 16u;
+}
+
+/* next_surface (cycles_path_state.h:205-362) */
+void oracle_next_surface(const oracle_path_state *before, uint32_t label,
+                         uint32_t runtime_flag, const oracle_path_limits *lim,
+                         oracle_path_state *after) {
+    const int transparent = (label & (LABEL_TRANSPARENT | LABEL_RAY_PORTAL)) != 0u;
+    if (transparent) {
+        oracle_path_state t = *before;
+        t.transparent_bounce += 1u;
+        t.rng_offset += 16u;
+        t.flag |= FLAG_TRANSPARENT;
+        if (t.transparent_bounce >= lim->maximum_transparent)
+            t.flag |= FLAG_TERMINATE_ON_NEXT_SURFACE;
+        if ((runtime_flag & RT_RAY_PORTAL) != 0u) t.flag |= FLAG_MIS_SKIP;
+        *after = t;
+        return;
+    }
+    oracle_path_state s = *before;
+    s.bounce += 1u;
+    s.rng_offset += 16u;
+    if (s.bounce >= lim->maximum) s.flag |= FLAG_TERMINATE_AFTER_TRANSPARENT;
+    s.visibility = 0u;
+    s.flag &= ~(FLAG_REFLECT | FLAG_SINGULAR | FLAG_TRANSPARENT |
+                FLAG_IMPORTANCE_BAKE | FLAG_MIS_SKIP | FLAG_MIS_HAD_TRANSMISSION);
+    const int reflection = (label & LABEL_REFLECT) != 0u;
+    const int diffuse = (label & LABEL_DIFFUSE) != 0u;
+    const int glossy = (label & LABEL_GLOSSY) != 0u;
+    const int transmit_transparent = (label & LABEL_TRANSMIT_TRANSPARENT) != 0u;
+    if (reflection) s.flag |= FLAG_REFLECT;
+    if (reflection || !transmit_transparent) s.flag &= ~FLAG_TRANSPARENT_BACKGROUND;
+    const int diffuse_reflection = reflection && diffuse;
+    const int glossy_reflection = reflection && !diffuse;
+    const int transmission = !reflection;
+    if (diffuse_reflection) s.diffuse_bounce += 1u;
+    if (glossy_reflection) s.glossy_bounce += 1u;
+    if (transmission) s.transmission_bounce += 1u;
+    if (diffuse_reflection && s.diffuse_bounce >= lim->maximum_diffuse)
+        s.flag |= FLAG_TERMINATE_AFTER_TRANSPARENT;
+    if (glossy_reflection && s.glossy_bounce >= lim->maximum_glossy)
+        s.flag |= FLAG_TERMINATE_AFTER_TRANSPARENT;
+    if (transmission && s.transmission_bounce >= lim->maximum_transmission)
+        s.flag |= FLAG_TERMINATE_AFTER_TRANSPARENT;
+    if (transmission) s.visibility |= CVIS_TRANSMIT;
+    s.visibility |= diffuse ? CVIS_DIFFUSE : CVIS_GLOSSY;
+    if (diffuse) s.flag |= FLAG_DIFFUSE_ANCESTOR;
+    if (!diffuse && !glossy) s.flag |= FLAG_SINGULAR | FLAG_MIS_SKIP;
+    if ((runtime_flag & RT_BSDF_HAS_TRANSMISSION) != 0u)
+        s.flag |= FLAG_MIS_HAD_TRANSMISSION;
+    if ((s.flag & FLAG_ANY_PASS) == 0u && (s.flag & FLAG_TRANSPARENT_BACKGROUND) == 0u)
+        s.flag |= FLAG_SURFACE_PASS;
+    *after = s;
+}
+
+/* contract_visibility (cycles_path_state.h:448-489) */
+uint32_t oracle_contract_visibility(uint32_t v) {
+    const uint32_t t = (v & CVIS_TRANSMIT) ? (v & ~(CVIS_DIFFUSE | CVIS_GLOSSY)) : v;
+    uint32_t r = 0u;
+    if (t & CVIS_CAMERA) r |= PSX_VISIBILITY_CAMERA;
+    if (t & CVIS_TRANSMIT) r |= PSX_VISIBILITY_TRANSMISSION;
+    if (t & CVIS_DIFFUSE) r |= PSX_VISIBILITY_DIFFUSE;
+    if (t & CVIS_GLOSSY) r |= PSX_VISIBILITY_GLOSSY;
+    if (t & CVIS_VOLUME_SCATTER) r |= PSX_VISIBILITY_VOLUME_SCATTER;
+    return r;
+}
+
+/* continuation_probability (cycles_path_state.h:416-446) */
+float oracle_continuation_probability(uint32_t flag, uint32_t bounce,
+                                      uint32_t transparent_bounce,
+                                      uint32_t minimum, uint32_t transparent_minimum,
+                                      const float th[3]) {
+    const int transparent = (flag & FLAG_TRANSPARENT) != 0u;
+    const int before_minimum = transparent ? transparent_bounce <= transparent_minimum
+                                           : bounce <= minimum;
+    const float roulette = fminf(
+        sqrtf(fmaxf(fabsf(th[0]), fmaxf(fabsf(th[1]), fabsf(th[2])))), 1.0f);
+    return before_minimum ? 1.0f : roulette;
+}
+
+static uint32_t label_from_events(uint32_t e) {
+    uint32_t l = 0u;
+    if (e & EV_TRANSMISSION) l |= LABEL_TRANSMIT;
+    if (e & EV_REFLECTION) l |= LABEL_REFLECT;
+    if (e & EV_DIFFUSE) l |= LABEL_DIFFUSE;
+    if (e & EV_GLOSSY) l |= LABEL_GLOSSY;
+    if (e & EV_SINGULAR) l |= LABEL_SINGULAR;
+    if (e & EV_TRANSPARENT) l |= LABEL_TRANSPARENT;
+    if (e & EV_SUBSURFACE) l |= LABEL_SUBSURFACE;
+    return l;
+}
+
+/* ======================================================================= */
+/* light distribution (src/sampling/light_distribution.cpp:25-138)         */
+/* ======================================================================= */
+
+typedef struct dist_entry {
+    float cumulative;
+    float selection_pdf;
+    uint32_t kind;  /* 0 triangle, 1 analytic, 2 environment, 3 sentinel */
+    uint32_t index;
+} dist_entry;
+
+static float valid_area(float a) { return (isfinite(a) && a > 0.0f) ? a : 0.0f; }
+
+static uint32_t build_distribution(const float *areas, uint32_t tri_count,
+                                   uint32_t analytic_count, int include_env,
+                                   dist_entry *e, float *tri_pdf,
+                                   float *light_pdf) {
+    const uint32_t env_count = include_env ? 1u : 0u;
+    const uint32_t light_count = analytic_count + env_count;
+    const uint32_t emitter_count = tri_count + light_count;
+    uint32_t n = 0;
+    float total = 0.0f;
+    for (uint32_t i = 0; i < tri_count; ++i) {
+        e[n].cumulative = total; e[n].selection_pdf = 0.0f; e[n].kind = 0u; e[n].index = i;
+        ++n;
+        total += valid_area(areas[i]);
+    }
+    const float triangle_area = total;
+    const float light_area =
+        light_count > 0u ? (total > 0.0f ? total / (float)light_count : 1.0f) : 0.0f;
+    for (uint32_t i = 0; i < analytic_count; ++i) {
+        e[n].cumulative = total; e[n].selection_pdf = 0.0f; e[n].kind = 1u; e[n].index = i;
+        ++n;
+        total += light_area;
+    }
+    if (include_env) {
+        e[n].cumulative = total; e[n].selection_pdf = 0.0f; e[n].kind = 2u; e[n].index = 0u;
+        ++n;
+        total += light_area;
+    }
+    e[n].cumulative = total; e[n].selection_pdf = 0.0f; e[n].kind = 3u; e[n].index = 0u;
+    float tp = 0.0f, lp = 0.0f;
+    if (triangle_area > 0.0f) {
+        tp = 1.0f / triangle_area;
+        if (light_count > 0u) tp *= 0.5f;
+    }
+    if (light_count > 0u) {
+        lp = 1.0f / (float)light_count;
+        if (triangle_area > 0.0f) lp *= 0.5f;
+    }
+    for (uint32_t i = 0; i < tri_count; ++i) e[i].selection_pdf = valid_area(areas[i]) * tp;
+    for (uint32_t i = tri_count; i < emitter_count; ++i) e[i].selection_pdf = lp;
+    if (total > 0.0f) {
+        const float inv = 1.0f / total;
+        for (uint32_t i = 0; i < emitter_count; ++i) e[i].cumulative *= inv;
+        e[emitter_count].cumulative = 1.0f;
+    }
+    *tri_pdf = tp;
+    *light_pdf = lp;
+    return emitter_count;
+}
+
+uint32_t oracle_light_distribution(const float *areas, uint32_t tri_count,
+                                   uint32_t analytic_count, int include_env,
+                                   uint32_t *entries, float *tri_pdf,
+                                   float *light_pdf) {
+    dist_entry *e = (dist_entry *)malloc(sizeof(dist_entry) *
+                                         (tri_count + analytic_count + 2u));
+    const uint32_t n = build_distribution(areas, tri_count, analytic_count,
+                                          include_env, e, tri_pdf, light_pdf);
+    for (uint32_t i = 0; i <= n; ++i) {
+        entries[4 * i + 0] = fbits(e[i].cumulative);
+        entries[4 * i + 1] = fbits(e[i].selection_pdf);
+        entries[4 * i + 2] = e[i].kind;
+        entries[4 * i + 3] = e[i].index;
+    }
+    free(e);
+    return n;
+}
+
+/* Upper-bound search (src/luisa/path_tracer_light_distribution.cpp:12-52). */
+static const dist_entry *distribution_sample(const dist_entry *e, uint32_t count,
+                                             float u) {
+    uint32_t first = 0u, length = count + 1u;
+    while (length > 0u) {
+        const uint32_t half = length >> 1u;
+        const uint32_t middle = first + half;
+        if (u < e[middle].cumulative) {
+            length = half;
+        } else {
+            first = middle + 1u;
+            length = length - half - 1u;
+        }
+    }
+    uint32_t selected = first > 0u ? first - 1u : 0u;
+    if (selected > count - 1u) selected = count - 1u;
+    return &e[selected];
+}
+
+/* ======================================================================= */
+/* Pluecker triangle test                                                   */
+/* (src/luisa/cycles_triangle_intersection_component.cpp:14-108)           */
+/* ======================================================================= */
+
+static v3 pluecker_cross(v3 a, v3 b) {
+    return V3(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)),
+              fmaf(a.x, b.y, -(a.y * b.x)));
+}
+static float pluecker_dot(v3 a, v3 b) {
+    return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z));
+}
+static float clamp_bvh_direction(float v) {
+    const float m = 8.271806e-25f;
+    return fabsf(v) > m ? v : copysignf(m, v);
+}
+
+typedef struct tri_hit {
+    int valid;
+    float t, u, v;
+} tri_hit;
+
+/* origin/direction are already in the intersection space of the triangle. */
+static tri_hit pluecker(v3 origin, v3 raw_direction, float tmin, float tmax,
+                        v3 p0, v3 p1, v3 p2) {
+    const v3 d = V3(clamp_bvh_direction(raw_direction.x),
+                    clamp_bvh_direction(raw_direction.y),
+                    clamp_bvh_direction(raw_direction.z));
+    const v3 v0 = vsub(p0, origin), v1 = vsub(p1, origin), v2 = vsub(p2, origin);
+    const v3 e0 = vsub(v2, v0), e1 = vsub(v0, v1), e2 = vsub(v1, v2);
+    const float u = pluecker_dot(pluecker_cross(e0, vadd(v2, v0)), d);
+    const float v = pluecker_dot(pluecker_cross(e1, vadd(v0, v1)), d);
+    const float w = pluecker_dot(pluecker_cross(e2, vadd(v1, v2)), d);
+    const float sum = u + v + w;
+    const float tol = FLT_EPSILON * fabsf(sum);
+    const float mn = fminf(u, fminf(v, w));
+    const float mx = fmaxf(u, fmaxf(v, w));
+    const int inside = (mn >= -tol) || (mx <= tol);
+    const v3 n2 = vscale(pluecker_cross(e1, e0), 2.0f);
+    const float den = pluecker_dot(n2, d);
+    const float safe_den = den != 0.0f ? den : 1.0f;
+    const float t = pluecker_dot(v0, n2) / safe_den;
+    const int in_range = (t >= tmin) && (t <= tmax);
+    const float rs = fabsf(sum) >= 1.0e-18f ? 1.0f / sum : 0.0f;
+    tri_hit h;
+    h.valid = inside && den != 0.0f && in_range;
+    h.t = t;
+    h.u = fminf(u * rs, 1.0f);
+    h.v = fminf(v * rs, 1.0f);
+    return h;
+}
+
+int oracle_pluecker_intersect(const float origin[3], const float direction[3],
+                              float tmin, float tmax, const float p0[3],
+                              const float p1[3], const float p2[3], float out[3]) {
+    tri_hit h = pluecker(load3(origin), load3(direction), tmin, tmax, load3(p0),
+                         load3(p1), load3(p2));
+    out[0] = h.t;
+    out[1] = h.u;
+    out[2] = h.v;
+    return h.valid;
+}
+
+/* ======================================================================= */
+/* scene                                                                    */
+/* ======================================================================= */
+
+enum { KIND_DIFFUSE = 1, KIND_TRANSPARENT = 2 };
+
+typedef struct omaterial {
+    psx_closure_instr *instr;
+    uint32_t count, root;
+    float *params;
+    uint32_t param_count;
+    uint32_t emission_sampling;
+    int may_emit;
+} omaterial;
+
+typedef struct bvh_node {
+    float lo[3], hi[3];
+    uint32_t first;  /* left child index for inner nodes, first prim for leaves */
+    uint32_t count;  /* 0 for inner nodes */
+} bvh_node;
+
+typedef struct blas {
+    uint32_t mesh;
+    int world;              /* positions are final world support positions */
+    float *ipos;            /* intersection positions, 3 * V                */
+    bvh_node *nodes;
+    uint32_t node_count;
+    uint32_t *prims;        /* leaf order -> local primitive                */
+    float root_lo[3], root_hi[3];
+} blas;
+
+typedef struct oinstance {
+    uint32_t mesh, blas, visibility;
+    int applied;
+    mat4 o2w, w2o, n2w;
+    uint32_t *override;
+    uint32_t override_count;
+    float lo[3], hi[3];     /* world bounds */
+} oinstance;
+
+typedef struct omesh {
+    const float *pos;       /* borrowed copies below */
+    float *positions;
+    float *normals;
+    uint32_t *tris;
+    uint8_t *smooth;
+    uint32_t *slots;
+    uint32_t slot_count;
+    uint32_t *tri_slots;
+    uint32_t vcount, tcount;
+    uint32_t prim_offset;
+    uint32_t users;
+} omesh;
+
+typedef struct emitter {
+    uint32_t instance, prim;
+    uint32_t material;
+    uint32_t emission_sampling;
+} emitter;
+
+struct oracle_scene {
+    omaterial *materials;
+    uint32_t material_count;
+    omesh *meshes;
+    uint32_t mesh_count;
+    oinstance *instances;
+    uint32_t instance_count;
+    blas *blases;
+    uint32_t blas_count;
+    bvh_node *tlas;
+    uint32_t tlas_count;
+    uint32_t *tlas_prims;
+    emitter *emitters;
+    uint32_t emitter_count;
+    dist_entry *distribution;
+    uint32_t distribution_count;
+    float triangle_area_pdf;
+    float light_selection_pdf;
+    psx_camera camera;
+    v3 background;
+    uint32_t world_visibility;
+    int brute_force;
+};
+
+/* ---- materials: closure-tree flattening ------------------------------- */
+/* (src/luisa/graph_surface_closures.cpp:6-353,
+ *  src/luisa/graph_surface_physical_closures.cpp:200-470,
+ *  src/luisa/graph_surface_state.cpp:20-58) */
+
+typedef struct closure_rec {
+    uint32_t kind;
+    v3 weight;
+    float sample_weight;
+    v3 normal;
+    float roughness;
+    v3 color;
+} closure_rec;
+
+#define MAX_CLOSURES 16
+
+typedef struct closure_set {
+    closure_rec c[MAX_CLOSURES];
+    uint32_t count;
+    v3 emission;
+} closure_set;
+
+static float avg3(v3 v) { return (v.x + v.y + v.z) / 3.0f; }
+static float sample_weight_of(v3 v) { return fabsf(avg3(v)); }
+
+static float param(const omaterial *m, const float *block, uint32_t i) {
+    (void)m;
+    return block[i];
+}
+static v3 param3(const omaterial *m, const float *block, uint32_t i) {
+    (void)m;
+    return V3(block[i], block[i + 1], block[i + 2]);
+}
+
+static void flatten(const omaterial *m, uint32_t id, float mix_weight,
+                    v3 shading_normal, closure_set *out) {
+    const psx_closure_instr *ins = &m->instr[id];
+    const float *block = m->params;
+    switch (ins->op) {
+    case PSX_CLOSURE_ADD:
+        flatten(m, ins->a, mix_weight, shading_normal, out);
+        flatten(m, ins->b, mix_weight, shading_normal, out);
+        return;
+    case PSX_CLOSURE_MIX: {
+        const float f = fclamp(param(m, block, ins->factor), 0.0f, 1.0f);
+        flatten(m, ins->a, mix_weight * (1.0f - f), shading_normal, out);
+        flatten(m, ins->b, mix_weight * f, shading_normal, out);
+        return;
+    }
+    case PSX_CLOSURE_DIFFUSE: {
+        const v3 color = param3(m, block, ins->color);
+        /* bsdf_allocated_weight */
+        v3 w = vmax(vscale(color, mix_weight), V3(0, 0, 0));
+        if (!(avg3(w) >= 1.0e-5f)) w = V3(0, 0, 0);
+        /* physical diffuse record */
+        const float allocation = sample_weight_of(vmax(w, V3(0, 0, 0)));
+        const int allocated = allocation >= 1.0e-5f;
+        if (allocated && out->count < MAX_CLOSURES) {
+            closure_rec *c = &out->c[out->count++];
+            c->kind = KIND_DIFFUSE;
+            c->weight = vmax(w, V3(0, 0, 0));
+            c->sample_weight = allocation;
+            c->normal = safe_normalize(shading_normal, shading_normal);
+            c->roughness = param(m, block, ins->roughness);
+            c->color = color;
+        }
+        return;
+    }
+    case PSX_CLOSURE_EMISSION: {
+        const v3 color = param3(m, block, ins->color);
+        const float strength = param(m, block, ins->strength);
+        out->emission = vadd(out->emission, vscale(vscale(color, strength), mix_weight));
+        return;
+    }
+    case PSX_CLOSURE_TRANSPARENT: {
+        const v3 color = param3(m, block, ins->color);
+        const v3 w = vscale(color, mix_weight);
+        const float sw = sample_weight_of(w);
+        if (sw >= 1.0e-5f && out->count < MAX_CLOSURES) {
+            closure_rec *c = &out->c[out->count++];
+            c->kind = KIND_TRANSPARENT;
+            c->weight = w;
+            c->sample_weight = sw;
+            c->normal = shading_normal;
+            c->roughness = 0.0f;
+            c->color = V3(1, 1, 1);
+        }
+        return;
+    }
+    default:
+        return;
+    }
+}
+
+static void material_closures(const omaterial *m, v3 shading_normal, closure_set *out) {
+    out->count = 0;
+    out->emission = V3(0, 0, 0);
+    if (m->count == 0) return;
+    flatten(m, m->root, 1.0f, shading_normal, out);
+}
+
+/* constant_emission (src/luisa/graph_surface_emission.cpp:33-80): one running
+ * sum, added to in tree-visit order. */
+static void constant_emission_rec(const omaterial *m, uint32_t id, float w, v3 *sum) {
+    const psx_closure_instr *ins = &m->instr[id];
+    switch (ins->op) {
+    case PSX_CLOSURE_EMISSION:
+        *sum = vadd(*sum, vscale(vscale(param3(m, m->params, ins->color),
+                                        param(m, m->params, ins->strength)), w));
+        return;
+    case PSX_CLOSURE_ADD:
+        constant_emission_rec(m, ins->a, w, sum);
+        constant_emission_rec(m, ins->b, w, sum);
+        return;
+    case PSX_CLOSURE_MIX: {
+        const float f = fclamp(param(m, m->params, ins->factor), 0.0f, 1.0f);
+        constant_emission_rec(m, ins->a, w * (1.0f - f), sum);
+        constant_emission_rec(m, ins->b, w * f, sum);
+        return;
+    }
+    default:
+        return;
+    }
+}
+
+static v3 constant_emission(const omaterial *m) {
+    v3 sum = V3(0, 0, 0);
+    if (m->count) constant_emission_rec(m, m->root, 1.0f, &sum);
+    return sum;
+}
+
+/* estimate_surface_emission (src/compiler/surface_program.cpp:244-330) */
+static v3 emission_estimate(const omaterial *m, uint32_t id) {
+    const psx_closure_instr *ins = &m->instr[id];
+    switch (ins->op) {
+    case PSX_CLOSURE_EMISSION:
+        return vscale(param3(m, m->params, ins->color), param(m, m->params, ins->strength));
+    case PSX_CLOSURE_ADD:
+        return vadd(emission_estimate(m, ins->a), emission_estimate(m, ins->b));
+    case PSX_CLOSURE_MIX: {
+        const float f = param(m, m->params, ins->factor);
+        return vadd(vscale(emission_estimate(m, ins->a), 1.0f - f),
+                    vscale(emission_estimate(m, ins->b), f));
+    }
+    default:
+        return V3(0, 0, 0);
+    }
+}
+
+static void material_refresh(omaterial *m) {
+    v3 e = m->count ? emission_estimate(m, m->root) : V3(0, 0, 0);
+    m->may_emit = (e.x != 0.0f || e.y != 0.0f || e.z != 0.0f) &&
+                  m->emission_sampling != PSX_EMISSION_SAMPLING_NONE;
+}
+
+/* ---- BVH (binned SAH, 16 bins, leaf <= 4) ------------------------------ */
+
+typedef struct build_item {
+    float lo[3], hi[3], c[3];
+    uint32_t id;
+} build_item;
+
+typedef struct node_vec {
+    bvh_node *data;
+    uint32_t size, cap;
+} node_vec;
+
+static uint32_t node_push(node_vec *v) {
+    if (v->size == v->cap) {
+        v->cap = v->cap ? v->cap * 2 : 64;
+        v->data = (bvh_node *)realloc(v->data, sizeof(bvh_node) * v->cap);
+    }
+    memset(&v->data[v->size], 0, sizeof(bvh_node));
+    return v->size++;
+}
+
+static float box_area(const float lo[3], const float hi[3]) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    if (dx < 0 || dy < 0 || dz < 0) return 0.0f;
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+static void pad_box(float lo[3], float hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const float ext = hi[a] - lo[a];
+        const float s = fmaxf(fmaxf(fabsf(lo[a]), fabsf(hi[a])), ext);
+        const float pad = s * 1.0e-5f + 1.0e-30f;
+        lo[a] -= pad;
+        hi[a] += pad;
+    }
+}
+
+static void build_rec(node_vec *nodes, uint32_t node, build_item *items,
+                      uint32_t begin, uint32_t end, uint32_t *order) {
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t i = begin; i < end; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(lo[a], items[i].lo[a]);
+            hi[a] = fmaxf(hi[a], items[i].hi[a]);
+            clo[a] = fminf(clo[a], items[i].c[a]);
+            chi[a] = fmaxf(chi[a], items[i].c[a]);
+        }
+    pad_box(lo, hi);
+    memcpy(nodes->data[node].lo, lo, sizeof(lo));
+    memcpy(nodes->data[node].hi, hi, sizeof(hi));
+    const uint32_t n = end - begin;
+    if (n <= 4u) {
+        nodes->data[node].first = begin;
+        nodes->data[node].count = n;
+        for (uint32_t i = begin; i < end; ++i) order[i] = items[i].id;
+        return;
+    }
+    enum { BINS = 16 };
+    float best_cost = FLT_MAX;
+    int best_axis = -1;
+    int best_split = 0;
+    for (int a = 0; a < 3; ++a) {
+        const float extent = chi[a] - clo[a];
+        if (!(extent > 0.0f)) continue;
+        uint32_t cnt[BINS] = {0};
+        float blo[BINS][3], bhi[BINS][3];
+        for (int b = 0; b < BINS; ++b)
+            for (int k = 0; k < 3; ++k) { blo[b][k] = FLT_MAX; bhi[b][k] = -FLT_MAX; }
+        const float scale = (float)BINS / extent;
+        for (uint32_t i = begin; i < end; ++i) {
+            int b = (int)((items[i].c[a] - clo[a]) * scale);
+            if (b >= BINS) b = BINS - 1;
+            if (b < 0) b = 0;
+            cnt[b]++;
+            for (int k = 0; k < 3; ++k) {
+                blo[b][k] = fminf(blo[b][k], items[i].lo[k]);
+                bhi[b][k] = fmaxf(bhi[b][k], items[i].hi[k]);
+            }
+        }
+        float left_area[BINS], right_area[BINS];
+        uint32_t left_count[BINS], right_count[BINS];
+        float l[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, h[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        uint32_t c = 0;
+        for (int b = 0; b < BINS; ++b) {
+            c += cnt[b];
+            for (int k = 0; k < 3; ++k) { l[k] = fminf(l[k], blo[b][k]); h[k] = fmaxf(h[k], bhi[b][k]); }
+            left_area[b] = box_area(l, h);
+            left_count[b] = c;
+        }
+        for (int k = 0; k < 3; ++k) { l[k] = FLT_MAX; h[k] = -FLT_MAX; }
+        c = 0;
+        for (int b = BINS - 1; b >= 0; --b) {
+            c += cnt[b];
+            for (int k = 0; k < 3; ++k) { l[k] = fminf(l[k], blo[b][k]); h[k] = fmaxf(h[k], bhi[b][k]); }
+            right_area[b] = box_area(l, h);
+            right_count[b] = c;
+        }
+        for (int b = 0; b < BINS - 1; ++b) {
+            if (left_count[b] == 0 || right_count[b + 1] == 0) continue;
+            const float cost = left_area[b] * (float)left_count[b] +
+                               right_area[b + 1] * (float)right_count[b + 1];
+            if (cost < best_cost) { best_cost = cost; best_axis = a; best_split = b; }
+        }
+    }
+    uint32_t mid;
+    if (best_axis < 0) {
+        mid = begin + n / 2u;
+    } else {
+        const float extent = chi[best_axis] - clo[best_axis];
+        const float scale = (float)BINS / extent;
+        uint32_t i = begin, j = end;
+        while (i < j) {
+            int b = (int)((items[i].c[best_axis] - clo[best_axis]) * scale);
+            if (b >= BINS) b = BINS - 1;
+            if (b < 0) b = 0;
+            if (b <= best_split) {
+                ++i;
+            } else {
+                --j;
+                build_item t = items[i]; items[i] = items[j]; items[j] = t;
+            }
+        }
+        mid = i;
+        if (mid == begin || mid == end) mid = begin + n / 2u;
+    }
+    const uint32_t left = node_push(nodes);
+    const uint32_t right = node_push(nodes);
+    (void)right;
+    nodes->data[node].first = left;
+    nodes->data[node].count = 0;
+    build_rec(nodes, left, items, begin, mid, order);
+    build_rec(nodes, left + 1u, items, mid, end, order);
+}
+
+static void build_bvh(build_item *items, uint32_t n, bvh_node **out_nodes,
+                      uint32_t *out_count, uint32_t *order) {
+    node_vec nodes = {0};
+    const uint32_t root = node_push(&nodes);
+    if (n == 0) {
+        nodes.data[root].lo[0] = nodes.data[root].lo[1] = nodes.data[root].lo[2] = 1.0f;
+        nodes.data[root].hi[0] = nodes.data[root].hi[1] = nodes.data[root].hi[2] = -1.0f;
+        nodes.data[root].count = 0;
+        nodes.data[root].first = 0;
+    } else {
+        build_rec(&nodes, root, items, 0, n, order);
+    }
+    *out_nodes = nodes.data;
+    *out_count = nodes.size;
+}
+
+/* ---- scene creation -------------------------------------------------- */
+
+static void tri_positions(const float *pos, const uint32_t *tris, uint32_t prim,
+                          v3 *p0, v3 *p1, v3 *p2) {
+    *p0 = load3(pos + 3u * tris[3u * prim + 0]);
+    *p1 = load3(pos + 3u * tris[3u * prim + 1]);
+    *p2 = load3(pos + 3u * tris[3u * prim + 2]);
+}
+
+static void build_blas(blas *b, const omesh *m) {
+    build_item *items = (build_item *)malloc(sizeof(build_item) * (m->tcount ? m->tcount : 1));
+    for (uint32_t t = 0; t < m->tcount; ++t) {
+        v3 p0, p1, p2;
+        tri_positions(b->ipos, m->tris, t, &p0, &p1, &p2);
+        items[t].lo[0] = fminf(p0.x, fminf(p1.x, p2.x));
+        items[t].lo[1] = fminf(p0.y, fminf(p1.y, p2.y));
+        items[t].lo[2] = fminf(p0.z, fminf(p1.z, p2.z));
+        items[t].hi[0] = fmaxf(p0.x, fmaxf(p1.x, p2.x));
+        items[t].hi[1] = fmaxf(p0.y, fmaxf(p1.y, p2.y));
+        items[t].hi[2] = fmaxf(p0.z, fmaxf(p1.z, p2.z));
+        for (int a = 0; a < 3; ++a) items[t].c[a] = 0.5f * (items[t].lo[a] + items[t].hi[a]);
+        items[t].id = t;
+    }
+    b->prims = (uint32_t *)malloc(sizeof(uint32_t) * (m->tcount ? m->tcount : 1));
+    build_bvh(items, m->tcount, &b->nodes, &b->node_count, b->prims);
+    memcpy(b->root_lo, b->nodes[0].lo, sizeof(b->root_lo));
+    memcpy(b->root_hi, b->nodes[0].hi, sizeof(b->root_hi));
+    free(items);
+}
+
+static uint32_t material_of(const oracle_scene *s, const oinstance *inst, uint32_t prim) {
+    const omesh *m = &s->meshes[inst->mesh];
+    const uint32_t slot = m->tri_slots ? m->tri_slots[prim] : 0u;
+    if (slot < inst->override_count) return inst->override[slot];
+    if (m->slot_count == 0) return UINT32_MAX;
+    return m->slots[slot < m->slot_count ? slot : m->slot_count - 1u];
+}
+
+static void *dupmem(const void *src, size_t bytes) {
+    if (!src || bytes == 0) return NULL;
+    void *p = malloc(bytes);
+    memcpy(p, src, bytes);
+    return p;
+}
+
+oracle_scene *oracle_scene_create(const psx_scene_desc *d, int brute_force) {
+    if (!d) return NULL;
+    oracle_scene *s = (oracle_scene *)calloc(1, sizeof(oracle_scene));
+    s->brute_force = brute_force;
+    s->camera = d->camera;
+    s->world_visibility = d->world_visibility_mask;
+    s->background = load3(d->world_color);
+    /* background lights fold into the constant world radiance
+     * (src/luisa/path_tracer_scene.cpp:1723-1734) */
+    for (uint32_t i = 0; i < d->light_count; ++i) {
+        const psx_light *l = &d->lights[i];
+        if (l->type == PSX_LIGHT_BACKGROUND) {
+            s->background.x += l->color[0] * l->power;
+            s->background.y += l->color[1] * l->power;
+            s->background.z += l->color[2] * l->power;
+        }
+    }
+    s->material_count = d->material_count;
+    s->materials = (omaterial *)calloc(d->material_count ? d->material_count : 1, sizeof(omaterial));
+    for (uint32_t i = 0; i < d->material_count; ++i) {
+        const psx_material *src = &d->materials[i];
+        omaterial *m = &s->materials[i];
+        m->count = src->instruction_count;
+        m->root = src->root;
+        m->instr = (psx_closure_instr *)dupmem(src->instructions,
+                                               sizeof(psx_closure_instr) * src->instruction_count);
+        m->param_count = src->parameter_count;
+        m->params = (float *)dupmem(src->parameters, sizeof(float) * src->parameter_count);
+        m->emission_sampling = src->emission_sampling;
+        material_refresh(m);
+    }
+    s->mesh_count = d->mesh_count;
+    s->meshes = (omesh *)calloc(d->mesh_count ? d->mesh_count : 1, sizeof(omesh));
+    uint32_t prim_offset = 0;
+    for (uint32_t i = 0; i < d->mesh_count; ++i) {
+        const psx_mesh *src = &d->meshes[i];
+        omesh *m = &s->meshes[i];
+        m->vcount = src->vertex_count;
+        m->tcount = src->triangle_count;
+        m->positions = (float *)dupmem(src->positions, sizeof(float) * 3u * src->vertex_count);
+        m->normals = (float *)dupmem(src->normals, sizeof(float) * 3u * src->vertex_count);
+        m->tris = (uint32_t *)dupmem(src->triangles, sizeof(uint32_t) * 3u * src->triangle_count);
+        m->smooth = (uint8_t *)dupmem(src->triangle_smooth, src->triangle_count);
+        m->slots = (uint32_t *)dupmem(src->material_slots, sizeof(uint32_t) * src->material_slot_count);
+        m->slot_count = src->material_slot_count;
+        m->tri_slots = (uint32_t *)dupmem(src->triangle_material_slots,
+                                          sizeof(uint32_t) * src->triangle_count);
+        m->prim_offset = prim_offset;
+        prim_offset += src->triangle_count;
+    }
+    for (uint32_t i = 0; i < d->instance_count; ++i)
+        if (d->instances[i].mesh < d->mesh_count) s->meshes[d->instances[i].mesh].users++;
+
+    /* BLASes: one object-space BLAS per shared mesh, one world-space BLAS per
+     * single-user mesh (build_cycles_instance_intersection_plan,
+     * src/luisa/path_tracer_scene_geometry.cpp:272-318). */
+    s->blases = (blas *)calloc(d->instance_count + d->mesh_count + 1, sizeof(blas));
+    uint32_t *shared_blas = (uint32_t *)malloc(sizeof(uint32_t) * (d->mesh_count + 1));
+    for (uint32_t i = 0; i < d->mesh_count; ++i) shared_blas[i] = UINT32_MAX;
+    s->instance_count = d->instance_count;
+    s->instances = (oinstance *)calloc(d->instance_count ? d->instance_count : 1, sizeof(oinstance));
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        const psx_instance *src = &d->instances[i];
+        oinstance *inst = &s->instances[i];
+        inst->mesh = src->mesh;
+        inst->visibility = src->visibility_mask;
+        memcpy(inst->o2w.m, src->transform, sizeof(inst->o2w.m));
+        inst->w2o = cycles_inverse(&inst->o2w);
+        inst->n2w = transpose4(&inst->w2o);
+        inst->override = (uint32_t *)dupmem(src->material_overrides,
+                                            sizeof(uint32_t) * src->material_override_count);
+        inst->override_count = src->material_override_count;
+        const omesh *m = &s->meshes[src->mesh];
+        inst->applied = m->users == 1u;
+        if (inst->applied) {
+            blas *b = &s->blases[s->blas_count];
+            b->mesh = src->mesh;
+            b->world = 1;
+            b->ipos = (float *)malloc(sizeof(float) * 3u * (m->vcount ? m->vcount : 1));
+            for (uint32_t v = 0; v < m->vcount; ++v) {
+                v3 p = xform_point(&inst->o2w, load3(m->positions + 3u * v));
+                b->ipos[3u * v] = p.x;
+                b->ipos[3u * v + 1] = p.y;
+                b->ipos[3u * v + 2] = p.z;
+            }
+            build_blas(b, m);
+            inst->blas = s->blas_count++;
+        } else {
+            if (shared_blas[src->mesh] == UINT32_MAX) {
+                blas *b = &s->blases[s->blas_count];
+                b->mesh = src->mesh;
+                b->world = 0;
+                b->ipos = (float *)dupmem(m->positions, sizeof(float) * 3u * m->vcount);
+                build_blas(b, m);
+                shared_blas[src->mesh] = s->blas_count++;
+            }
+            inst->blas = shared_blas[src->mesh];
+        }
+        /* world bounds of the instance */
+        const blas *b = &s->blases[inst->blas];
+        float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        if (b->root_lo[0] <= b->root_hi[0]) {
+            for (int c = 0; c < 8; ++c) {
+                v3 corner = V3((c & 1) ? b->root_hi[0] : b->root_lo[0],
+                               (c & 2) ? b->root_hi[1] : b->root_lo[1],
+                               (c & 4) ? b->root_hi[2] : b->root_lo[2]);
+                v3 w = b->world ? corner : xform_point(&inst->o2w, corner);
+                lo[0] = fminf(lo[0], w.x); lo[1] = fminf(lo[1], w.y); lo[2] = fminf(lo[2], w.z);
+                hi[0] = fmaxf(hi[0], w.x); hi[1] = fmaxf(hi[1], w.y); hi[2] = fmaxf(hi[2], w.z);
+            }
+            pad_box(lo, hi);
+        }
+        memcpy(inst->lo, lo, sizeof(lo));
+        memcpy(inst->hi, hi, sizeof(hi));
+    }
+    free(shared_blas);
+    /* TLAS */
+    {
+        build_item *items = (build_item *)malloc(sizeof(build_item) * (d->instance_count ? d->instance_count : 1));
+        uint32_t n = 0;
+        for (uint32_t i = 0; i < d->instance_count; ++i) {
+            if (s->instances[i].lo[0] > s->instances[i].hi[0]) continue;
+            memcpy(items[n].lo, s->instances[i].lo, sizeof(items[n].lo));
+            memcpy(items[n].hi, s->instances[i].hi, sizeof(items[n].hi));
+            for (int a = 0; a < 3; ++a) items[n].c[a] = 0.5f * (items[n].lo[a] + items[n].hi[a]);
+            items[n].id = i;
+            ++n;
+        }
+        s->tlas_prims = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+        build_bvh(items, n, &s->tlas, &s->tlas_count, s->tlas_prims);
+        free(items);
+    }
+    /* emissive triangles (src/luisa/path_tracer_scene.cpp:1548-1640) */
+    {
+        uint32_t cap = 16, n = 0;
+        emitter *em = (emitter *)malloc(sizeof(emitter) * cap);
+        float *areas = (float *)malloc(sizeof(float) * cap);
+        for (uint32_t i = 0; i < d->instance_count; ++i) {
+            const oinstance *inst = &s->instances[i];
+            const int light_visible = inst->visibility == UINT32_MAX ||
+                                      (inst->visibility & (PSX_VISIBILITY_DIFFUSE | PSX_VISIBILITY_GLOSSY |
+                                                           PSX_VISIBILITY_TRANSMISSION |
+                                                           PSX_VISIBILITY_VOLUME_SCATTER)) != 0u;
+            if (!light_visible) continue;
+            const omesh *m = &s->meshes[inst->mesh];
+            for (uint32_t p = 0; p < m->tcount; ++p) {
+                const uint32_t mat = material_of(s, inst, p);
+                if (mat >= s->material_count || !s->materials[mat].may_emit) continue;
+                if (n == cap) {
+                    cap *= 2;
+                    em = (emitter *)realloc(em, sizeof(emitter) * cap);
+                    areas = (float *)realloc(areas, sizeof(float) * cap);
+                }
+                em[n].instance = i;
+                em[n].prim = p;
+                em[n].material = mat;
+                em[n].emission_sampling = s->materials[mat].emission_sampling;
+                v3 p0, p1, p2;
+                tri_positions(m->positions, m->tris, p, &p0, &p1, &p2);
+                p0 = xform_point(&inst->o2w, p0);
+                p1 = xform_point(&inst->o2w, p1);
+                p2 = xform_point(&inst->o2w, p2);
+                const v3 n01 = vsub(p1, p0), n02 = vsub(p2, p0);
+                const v3 nn = vcross(n01, n02);
+                areas[n] = 0.5f * sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
+                ++n;
+            }
+        }
+        s->emitters = em;
+        s->emitter_count = n;
+        s->distribution = (dist_entry *)malloc(sizeof(dist_entry) * (n + 2u));
+        s->distribution_count = build_distribution(areas, n, 0u, 0, s->distribution,
+                                                   &s->triangle_area_pdf,
+                                                   &s->light_selection_pdf);
+        if (!(s->distribution[s->distribution_count].cumulative > 0.0f))
+            s->distribution_count = 0u;
+        free(areas);
+    }
+    return s;
+}
+
+void oracle_scene_destroy(oracle_scene *s) {
+    if (!s) return;
+    for (uint32_t i = 0; i < s->material_count; ++i) {
+        free(s->materials[i].instr);
+        free(s->materials[i].params);
+    }
+    free(s->materials);
+    for (uint32_t i = 0; i < s->mesh_count; ++i) {
+        omesh *m = &s->meshes[i];
+        free(m->positions); free(m->normals); free(m->tris); free(m->smooth);
+        free(m->slots); free(m->tri_slots);
+    }
+    free(s->meshes);
+    for (uint32_t i = 0; i < s->instance_count; ++i) free(s->instances[i].override);
+    free(s->instances);
+    for (uint32_t i = 0; i < s->blas_count; ++i) {
+        free(s->blases[i].ipos); free(s->blases[i].nodes); free(s->blases[i].prims);
+    }
+    free(s->blases);
+    free(s->tlas);
+    free(s->tlas_prims);
+    free(s->emitters);
+    free(s->distribution);
+    free(s);
+}
+
+int oracle_scene_update_parameters(oracle_scene *s, uint32_t material,
+                                   const float *parameters, uint32_t count) {
+    if (!s || material >= s->material_count) return -1;
+    omaterial *m = &s->materials[material];
+    if (count != m->param_count) return -1;
+    memcpy(m->params, parameters, sizeof(float) * count);
+    material_refresh(m);
+    return 0;
+}
+
+uint32_t oracle_scene_emissive_triangles(const oracle_scene *s) {
+    return s ? s->emitter_count : 0u;
+}
+
+/* ======================================================================= */
+/* closest hit (src/luisa/path_kernel_scene_traversal.cpp:34-68, 212-493)  */
+/* ======================================================================= */
+
+typedef struct ray_t {
+    v3 o, d;
+    float tmin, tmax;
+} ray_t;
+
+typedef struct hit_t {
+    int valid;
+    uint32_t inst, prim;  /* prim is local to the mesh */
+    float t, u, v;
+    uint32_t object, cprim;
+} hit_t;
+
+typedef struct trace_counters {
+    uint64_t nodes, tris;
+} trace_counters;
+
+typedef struct closest_state {
+    hit_t best;
+    uint32_t visibility;
+    uint32_t src_obj, src_prim, light_obj, light_prim;
+    float tmin, tmax;  /* original closed interval */
+    trace_counters *counters;
+} closest_state;
+
+/* CyclesClosestHitOrder::accepts */
+static int order_accepts(const hit_t *best, float t, uint32_t object, uint32_t prim) {
+    if (!best->valid) return 1;
+    if (t < best->t) return 1;
+    if (t == best->t) {
+        if (object > best->object) return 1;
+        if (object == best->object && prim > best->cprim) return 1;
+    }
+    return 0;
+}
+
+static void test_triangle(const oracle_scene *s, closest_state *st, uint32_t inst_index,
+                          const oinstance *inst, const blas *b, v3 o, v3 d, uint32_t prim) {
+    const omesh *m = &s->meshes[inst->mesh];
+    if (st->counters) st->counters->tris++;
+    v3 p0, p1, p2;
+    tri_positions(b->ipos, m->tris, prim, &p0, &p1, &p2);
+    const tri_hit h = pluecker(o, d, st->tmin, st->tmax, p0, p1, p2);
+    if (!h.valid) return;
+    const uint32_t object = inst_index;
+    const uint32_t cprim = m->prim_offset + prim;
+    const int excluded = (object == st->src_obj && cprim == st->src_prim) ||
+                         (object == st->light_obj && cprim == st->light_prim);
+    if (excluded) return;
+    if ((inst->visibility & st->visibility) == 0u) return;
+    if (!order_accepts(&st->best, h.t, object, cprim)) return;
+    st->best.valid = 1;
+    st->best.inst = inst_index;
+    st->best.prim = prim;
+    st->best.t = h.t;
+    st->best.u = h.u;
+    st->best.v = h.v;
+    st->best.object = object;
+    st->best.cprim = cprim;
+}
+
+/* conservative slab test: the box may only be culled when no triangle in it
+ * can satisfy the exact test within [tmin, best]. */
+static int box_hit(const float lo[3], const float hi[3], v3 o, v3 inv, float tmin, float tmax) {
+    float t0x = (lo[0] - o.x) * inv.x, t1x = (hi[0] - o.x) * inv.x;
+    float t0y = (lo[1] - o.y) * inv.y, t1y = (hi[1] - o.y) * inv.y;
+    float t0z = (lo[2] - o.z) * inv.z, t1z = (hi[2] - o.z) * inv.z;
+    const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    const float tf_safe = tf * 1.00001f + 1.0e-30f;
+    return tn <= tf_safe && tf_safe >= tmin && tn <= tmax;
+}
+
+static float query_upper(float t) {
+    /* widen the pruning bound so that ties and rounding stay visible */
+    return t * 1.00001f + 1.0e-30f;
+}
+
+static void traverse_blas(const oracle_scene *s, closest_state *st, uint32_t inst_index,
+                          const oinstance *inst, v3 o, v3 d) {
+    const blas *b = &s->blases[inst->blas];
+    const omesh *m = &s->meshes[inst->mesh];
+    if (s->brute_force) {
+        for (uint32_t p = 0; p < m->tcount; ++p) test_triangle(s, st, inst_index, inst, b, o, d, p);
+        return;
+    }
+    const v3 dc = V3(clamp_bvh_direction(d.x), clamp_bvh_direction(d.y), clamp_bvh_direction(d.z));
+    const v3 inv = V3(1.0f / dc.x, 1.0f / dc.y, 1.0f / dc.z);
+    const float lower = st->tmin > 0.0f ? st->tmin * 0.99999f : st->tmin - 1.0e-30f;
+    uint32_t stack[128];
+    uint32_t sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const uint32_t ni = stack[--sp];
+        const bvh_node *n = &b->nodes[ni];
+        if (st->counters) st->counters->nodes++;
+        const float upper = st->best.valid ? query_upper(st->best.t) : query_upper(st->tmax);
+        if (!box_hit(n->lo, n->hi, o, inv, lower, upper)) continue;
+        if (n->count) {
+            for (uint32_t i = 0; i < n->count; ++i)
+                test_triangle(s, st, inst_index, inst, b, o, d, b->prims[n->first + i]);
+        } else {
+            /* visit the nearer child first */
+            const bvh_node *l = &b->nodes[n->first], *r = &b->nodes[n->first + 1];
+            const float cl = (l->lo[0] + l->hi[0] - 2.0f * o.x) * dc.x +
+                             (l->lo[1] + l->hi[1] - 2.0f * o.y) * dc.y +
+                             (l->lo[2] + l->hi[2] - 2.0f * o.z) * dc.z;
+            const float cr = (r->lo[0] + r->hi[0] - 2.0f * o.x) * dc.x +
+                             (r->lo[1] + r->hi[1] - 2.0f * o.y) * dc.y +
+                             (r->lo[2] + r->hi[2] - 2.0f * o.z) * dc.z;
+            if (sp + 2 > 128) continue; /* cannot happen for sane depths */
+            if (cl <= cr) { stack[sp++] = n->first + 1; stack[sp++] = n->first; }
+            else { stack[sp++] = n->first; stack[sp++] = n->first + 1; }
+        }
+    }
+}
+
+static void visit_instance(const oracle_scene *s, closest_state *st, const ray_t *r,
+                           uint32_t i) {
+    const oinstance *inst = &s->instances[i];
+    if ((inst->visibility & st->visibility) == 0u) return;
+    v3 o = r->o, d = r->d;
+    if (!inst->applied) {
+        o = xform_point(&inst->w2o, r->o);
+        d = xform_dir(&inst->w2o, r->d);
+    }
+    traverse_blas(s, st, i, inst, o, d);
+}
+
+static hit_t closest_hit(const oracle_scene *s, const ray_t *r, uint32_t visibility,
+                         uint32_t src_obj, uint32_t src_prim, uint32_t light_obj,
+                         uint32_t light_prim, trace_counters *counters) {
+    closest_state st;
+    memset(&st, 0, sizeof(st));
+    st.visibility = visibility;
+    st.src_obj = src_obj;
+    st.src_prim = src_prim;
+    st.light_obj = light_obj;
+    st.light_prim = light_prim;
+    st.tmin = r->tmin;
+    st.tmax = r->tmax;
+    st.counters = counters;
+    if (s->brute_force || s->tlas_count == 0) {
+        for (uint32_t i = 0; i < s->instance_count; ++i) visit_instance(s, &st, r, i);
+        return st.best;
+    }
+    const v3 dc = V3(clamp_bvh_direction(r->d.x), clamp_bvh_direction(r->d.y),
+                     clamp_bvh_direction(r->d.z));
+    const v3 inv = V3(1.0f / dc.x, 1.0f / dc.y, 1.0f / dc.z);
+    const float lower = r->tmin > 0.0f ? r->tmin * 0.99999f : r->tmin - 1.0e-30f;
+    uint32_t stack[128];
+    uint32_t sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const bvh_node *n = &s->tlas[stack[--sp]];
+        if (counters) counters->nodes++;
+        const float upper = st.best.valid ? query_upper(st.best.t) : query_upper(r->tmax);
+        if (!box_hit(n->lo, n->hi, r->o, inv, lower, upper)) continue;
+        if (n->count) {
+            for (uint32_t k = 0; k < n->count; ++k) visit_instance(s, &st, r, s->tlas_prims[n->first + k]);
+        } else {
+            if (sp + 2 > 128) continue;
+            stack[sp++] = n->first + 1;
+            stack[sp++] = n->first;
+        }
+    }
+    return st.best;
+}
+
+/* ======================================================================= */
+/* surface reconstruction                                                   */
+/* (src/luisa/cycles_triangle_surface_component.cpp:13-103,                */
+/*  src/luisa/path_kernel_surface_primitive.cpp:139-460)                   */
+/* ======================================================================= */
+
+typedef struct surface_t {
+    uint32_t inst, prim, object, cprim, material;
+    int applied, smooth, back_facing;
+    v3 p0, p1, p2;        /* intersection-space positions (cycles p) */
+    v3 wp0, wp1, wp2;     /* world positions */
+    v3 position, object_hit_position;
+    v3 ng, ns;
+    v3 incoming;
+    float u, v;
+    uint32_t emission_sampling; /* effective triangle emission sampling */
+} surface_t;
+
+static v3 interp3(float u, float v, v3 a, v3 b, v3 c) {
+    /* triangle_interpolate: (1 - u - v) a + u b + v c */
+    return vadd(vadd(vscale(a, 1.0f - u - v), vscale(b, u)), vscale(c, v));
+}
+
+static void resolve_surface(const oracle_scene *s, const hit_t *h, const ray_t *r,
+                            surface_t *sf) {
+    const oinstance *inst = &s->instances[h->inst];
+    const omesh *m = &s->meshes[inst->mesh];
+    const blas *b = &s->blases[inst->blas];
+    sf->inst = h->inst;
+    sf->prim = h->prim;
+    sf->object = h->object;
+    sf->cprim = h->cprim;
+    sf->material = material_of(s, inst, h->prim);
+    sf->applied = inst->applied;
+    sf->smooth = (m->smooth && m->normals) ? (m->smooth[h->prim] != 0) : 0;
+    sf->u = h->u;
+    sf->v = h->v;
+    v3 op0, op1, op2, fp0, fp1, fp2;
+    tri_positions(m->positions, m->tris, h->prim, &op0, &op1, &op2);
+    tri_positions(b->ipos, m->tris, h->prim, &fp0, &fp1, &fp2);
+    const v3 p0 = sf->applied ? fp0 : op0;
+    const v3 p1 = sf->applied ? fp1 : op1;
+    const v3 p2 = sf->applied ? fp2 : op2;
+    sf->p0 = p0; sf->p1 = p1; sf->p2 = p2;
+    const v3 tp0 = xform_point(&inst->o2w, op0);
+    const v3 tp1 = xform_point(&inst->o2w, op1);
+    const v3 tp2 = xform_point(&inst->o2w, op2);
+    sf->wp0 = sf->applied ? p0 : tp0;
+    sf->wp1 = sf->applied ? p1 : tp1;
+    sf->wp2 = sf->applied ? p2 : tp2;
+    const v3 ogn = safe_normalize(vcross(vsub(op1, op0), vsub(op2, op0)), V3(0, 0, 1));
+    const v3 tgn = safe_normalize(xform_dir(&inst->n2w, ogn), vneg(r->d));
+    v3 sgn = safe_normalize(vcross(vsub(p1, p0), vsub(p2, p0)), tgn);
+    if (vdot(sgn, tgn) < 0.0f) sgn = vneg(sgn);
+    v3 gn = sf->applied ? sgn : tgn;
+    v3 n0 = V3(0, 0, 0), n1 = n0, n2 = n0;
+    if (m->normals) {
+        const uint32_t *t = m->tris + 3u * h->prim;
+        n0 = load3(m->normals + 3u * t[0]);
+        n1 = load3(m->normals + 3u * t[1]);
+        n2 = load3(m->normals + 3u * t[2]);
+    }
+    const v3 wn0 = safe_normalize(xform_dir(&inst->n2w, n0), gn);
+    const v3 wn1 = safe_normalize(xform_dir(&inst->n2w, n1), gn);
+    const v3 wn2 = safe_normalize(xform_dir(&inst->n2w, n2), gn);
+    const v3 cn0 = sf->applied ? wn0 : n0;
+    const v3 cn1 = sf->applied ? wn1 : n1;
+    const v3 cn2 = sf->applied ? wn2 : n2;
+    const v3 osn = sf->smooth ? interp3(h->u, h->v, n0, n1, n2) : ogn;
+    const v3 tsn = safe_normalize(xform_dir(&inst->n2w, osn), gn);
+    const v3 ssn = safe_normalize(sf->smooth ? interp3(h->u, h->v, cn0, cn1, cn2) : gn, gn);
+    v3 ns = sf->applied ? ssn : tsn;
+    const int back_facing = vdot(gn, vneg(r->d)) < 0.0f;
+    if (back_facing) { gn = vneg(gn); ns = vneg(ns); }
+    if (vdot(ns, gn) < 0.0f) ns = vneg(ns);
+    sf->back_facing = back_facing;
+    sf->ng = gn;
+    sf->ns = ns;
+    const v3 object_position =
+        vadd(vadd(op0, vscale(vsub(op1, op0), h->u)), vscale(vsub(op2, op0), h->v));
+    const v3 support_position =
+        vadd(vadd(p0, vscale(vsub(p1, p0), h->u)), vscale(vsub(p2, p0), h->v));
+    sf->position = sf->applied ? support_position : xform_point(&inst->o2w, object_position);
+    sf->object_hit_position = sf->applied ? sf->position : mat_mul(&inst->w2o, sf->position, 1.0f);
+    sf->incoming = vneg(r->d);
+    /* triangle emission sampling (src/luisa/path_kernel_primitive_material.cpp:44-66) */
+    sf->emission_sampling = PSX_EMISSION_SAMPLING_NONE;
+    if (sf->material < s->material_count) {
+        const omaterial *mat = &s->materials[sf->material];
+        const int light_visible = (inst->visibility & (PSX_VISIBILITY_DIFFUSE | PSX_VISIBILITY_GLOSSY |
+                                                       PSX_VISIBILITY_TRANSMISSION |
+                                                       PSX_VISIBILITY_VOLUME_SCATTER)) != 0u;
+        if (mat->may_emit && light_visible) sf->emission_sampling = mat->emission_sampling;
+    }
+}
+
+/* offset_ray_origin: Waechter & Binder, "A Fast and Robust Method for
+ * Avoiding Self-Intersection", Ray Tracing Gems (2019), chapter 6 - the
+ * method behind LuisaCompute's offset_ray_origin used at
+ * include/psycles/luisa/surface_ray.h:101. */
+static float offset_component(float p, float n) {
+    const float origin = 1.0f / 32.0f;
+    const float float_scale = 1.0f / 65536.0f;
+    const float int_scale = 256.0f;
+    const int32_t of_i = (int32_t)(int_scale * n);
+    int32_t bits;
+    memcpy(&bits, &p, 4);
+    bits += p < 0.0f ? -of_i : of_i;
+    float p_i;
+    memcpy(&p_i, &bits, 4);
+    return fabsf(p) < origin ? p + float_scale * n : p_i;
+}
+
+static v3 offset_ray_origin(v3 p, v3 n) {
+    return V3(offset_component(p.x, n.x), offset_component(p.y, n.y),
+              offset_component(p.z, n.z));
+}
+
+/* include/psycles/luisa/surface_ray.h:50-106 */
+static int source_covers_origin(v3 o, v3 d, v3 p0, v3 p1, v3 p2) {
+    const v3 v0 = vsub(p0, o), v1 = vsub(p1, o), v2 = vsub(p2, o);
+    const v3 e0 = vsub(v2, v0), e1 = vsub(v0, v1), e2 = vsub(v1, v2);
+    const float u = vdot(vcross(vadd(v2, v0), e0), d);
+    const float v = vdot(vcross(vadd(v0, v1), e1), d);
+    const float w = vdot(vcross(vadd(v1, v2), e2), d);
+    const float eps = FLT_EPSILON * fabsf(u + v + w);
+    const float mn = fminf(u, fminf(v, w));
+    const float mx = fmaxf(u, fmaxf(v, w));
+    return (mn >= eps) || (mx <= -eps);
+}
+
+static v3 origin_with_self_exclusion(v3 world_origin, v3 world_ng, v3 obj_origin,
+                                     v3 obj_dir, v3 p0, v3 p1, v3 p2) {
+    return source_covers_origin(obj_origin, obj_dir, p0, p1, p2)
+               ? world_origin
+               : offset_ray_origin(world_origin, world_ng);
+}
+
+/* make_ray_origin (src/luisa/path_kernel_surface_geometry.cpp:151-165) */
+static v3 surface_ray_origin(const oracle_scene *s, const surface_t *sf, v3 dir) {
+    const oinstance *inst = &s->instances[sf->inst];
+    const v3 object_dir = xform_dir(&inst->w2o, dir);
+    const v3 co = sf->applied ? sf->position : sf->object_hit_position;
+    const v3 cd = sf->applied ? dir : object_dir;
+    return origin_with_self_exclusion(sf->position, sf->ng, co, cd, sf->p0, sf->p1, sf->p2);
+}
+
+/* make_shadow_origin + surface_shadow_origin with a zero shadow-terminator
+ * geometry offset (surface_ray.h:268-330): the smooth-surface lift is
+ * inactive and the source primitive stays excluded. */
+static v3 surface_shadow_origin(const oracle_scene *s, const surface_t *sf, v3 dir) {
+    const oinstance *inst = &s->instances[sf->inst];
+    const v3 oo = sf->applied ? sf->position : xform_point(&inst->w2o, sf->position);
+    const v3 od = sf->applied ? dir : xform_dir(&inst->w2o, dir);
+    return origin_with_self_exclusion(sf->position, sf->ng, oo, od, sf->p0, sf->p1, sf->p2);
+}
+
+static uint32_t intersection_t_offset_bits(float t) {
+    return t == 0.0f ? fbits(FLT_MIN) : fbits(t) + 1u;
+}
+
+/* ======================================================================= */
+/* closure evaluation and sampling                                          */
+/* (src/luisa/surface_closure_evaluation.cpp:67-464,                       */
+/*  src/luisa/surface_closure_sampling.cpp:11-95,                          */
+/*  src/luisa/surface_closure_sampling_core.cpp:96-676,                    */
+/*  src/luisa/graph_surface_scattering.cpp:463-506)                        */
+/* ======================================================================= */
+
+static float oren_nayar_g(float cosine) {
+    const float c = fclamp(cosine, 0.0f, 1.0f);
+    const float sine = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+    const float theta = acosf(c);
+    const float safe_c = fmaxf(c, 1.0e-6f);
+    const float regular = sine * (theta - 2.0f / 3.0f - sine * c) +
+                          2.0f / 3.0f * (sine / safe_c) * (1.0f - sine * sine * sine);
+    const float taylor = (PI_F * 0.5f - 2.0f / 3.0f) - c;
+    return c < 1.0e-6f ? taylor : regular;
+}
+
+static v3 diffuse_intensity(const closure_rec *c, v3 in, v3 out) {
+    const float nl = fmaxf(vdot(c->normal, out), 0.0f);
+    const v3 lambert = V3(nl * INV_PI_F, nl * INV_PI_F, nl * INV_PI_F);
+    const float sigma = fclamp(c->roughness, 0.0f, 1.0f);
+    if (sigma < 1.0e-5f) return lambert;
+    const float a = 1.0f / (PI_F + sigma * (PI_F * 0.5f - 2.0f / 3.0f));
+    const float b = sigma * a;
+    const float nv = fmaxf(vdot(c->normal, in), 0.0f);
+    float t = vdot(out, in) - nl * nv;
+    if (t > 0.0f) t = t / (fmaxf(nl, nv) + 1.17549435e-38f);
+    const float single = a + b * t;
+    const v3 albedo = vclamp(c->color, 0.0f, 1.0f);
+    const float two_pi = 6.2831853071795864f;
+    const float e_avg = a * PI_F + ((two_pi - 5.6f) / 3.0f) * b;
+    const v3 alb2 = vmul(albedo, albedo);
+    const v3 e_ms = vdiv(vscale(vscale(alb2, INV_PI_F), e_avg / (1.0f - e_avg)),
+                         vsub(V3(1, 1, 1), vscale(albedo, 1.0f - e_avg)));
+    const float e_in = a * PI_F + b * oren_nayar_g(nv);
+    const v3 ms = vscale(e_ms, 1.0f - e_in);
+    const float e_out = a * PI_F + b * oren_nayar_g(nl);
+    return vscale(vadd(V3(single, single, single), vscale(ms, 1.0f - e_out)), nl);
+}
+
+typedef struct eval_t {
+    v3 f;
+    float pdf;
+    uint32_t events;
+} eval_t;
+
+enum { LOBE_DIFFUSE = EV_DIFFUSE, LOBE_TRANSPARENT = EV_TRANSPARENT, LOBE_ALL = 0x3f };
+
+/* Mixture evaluation of every retained closure in direction `out`.
+ * `selected` is the index of the sampled closure or UINT32_MAX. */
+static eval_t evaluate_mixture(const closure_set *cs, v3 point_incoming, v3 ns, v3 out_dir,
+                               uint32_t lobe_mask, uint32_t selected) {
+    const v3 outgoing = safe_normalize(out_dir, ns);
+    const v3 incoming = safe_normalize(point_incoming, vneg(outgoing));
+    v3 f = V3(0, 0, 0);
+    float total = 0.0f, weighted_pdf = 0.0f;
+    uint32_t events = 0u;
+    const int diffuse_enabled = (lobe_mask & EV_DIFFUSE) != 0u;
+    const int transparent_enabled = (lobe_mask & EV_TRANSPARENT) != 0u;
+    for (uint32_t i = 0; i < cs->count; ++i) {
+        const closure_rec *c = &cs->c[i];
+        (void)selected;
+        if (c->kind == KIND_DIFFUSE) {
+            /* bump shadowing is exactly one: the closure normal equals the
+             * shading normal for every closure this path evaluates */
+            float pdf = fmaxf(vdot(c->normal, outgoing), 0.0f) * INV_PI_F;
+            const v3 value = vscale(vmul(c->weight, diffuse_intensity(c, incoming, outgoing)), 1.0f);
+            const int allowed = diffuse_enabled;
+            const v3 contrib = allowed ? value : V3(0, 0, 0);
+            const float w = allowed ? c->sample_weight : 0.0f;
+            if (allowed && sample_weight_of(contrib) > 0.0f) events |= EV_DIFFUSE | EV_REFLECTION;
+            f = vadd(f, contrib);
+            total += w;
+            weighted_pdf += w * (allowed ? pdf : 0.0f);
+        } else if (c->kind == KIND_TRANSPARENT) {
+            if (transparent_enabled) total += c->sample_weight;
+        }
+    }
+    eval_t e;
+    e.f = f;
+    e.pdf = total > 0.0f ? weighted_pdf / fmaxf(total, 1.0e-20f) : 0.0f;
+    e.events = weighted_pdf > 0.0f ? events : 0u;
+    return e;
+}
+
+typedef struct bsdf_sample_t {
+    int valid;
+    v3 wi;
+    eval_t eval;
+    uint32_t runtime_flags;
+    uint32_t selected;
+} bsdf_sample_t;
+
+static uint32_t closure_runtime_flags(const closure_rec *c) {
+    if (c->kind == KIND_DIFFUSE) return RT_BSDF | RT_BSDF_HAS_EVAL;
+    if (c->kind == KIND_TRANSPARENT) return RT_BSDF | RT_TRANSPARENT;
+    return 0u;
+}
+
+static uint32_t surface_runtime_flags(const closure_set *cs, int back_facing) {
+    uint32_t f = back_facing ? RT_BACKFACING : 0u;
+    for (uint32_t i = 0; i < cs->count; ++i) f |= closure_runtime_flags(&cs->c[i]);
+    return f;
+}
+
+static bsdf_sample_t sample_closures(const closure_set *cs, const surface_t *sf,
+                                     float u_lobe, v2 u_dir, uint32_t lobe_mask) {
+    bsdf_sample_t r;
+    memset(&r, 0, sizeof(r));
+    r.selected = UINT32_MAX;
+    const v3 incoming = safe_normalize(sf->incoming, sf->ns);
+    (void)incoming;
+    float weights[MAX_CLOSURES];
+    float total = 0.0f;
+    r.runtime_flags = sf->back_facing ? RT_BACKFACING : 0u;
+    for (uint32_t i = 0; i < cs->count; ++i) {
+        const closure_rec *c = &cs->c[i];
+        int eligible = 0;
+        if (c->kind == KIND_TRANSPARENT) eligible = (lobe_mask & EV_TRANSPARENT) != 0u;
+        if (c->kind == KIND_DIFFUSE) eligible = (lobe_mask & EV_DIFFUSE) != 0u;
+        weights[i] = eligible ? c->sample_weight : 0.0f;
+        total += weights[i];
+        r.runtime_flags |= closure_runtime_flags(c);
+    }
+    /* categorical inversion (surface_closure_sampling_core.cpp:410-437) */
+    const float random = fclamp(u_lobe, 0.0f, 0.99999994f);
+    const float target = random * total;
+    float accumulated = 0.0f;
+    int selected = 0;
+    uint32_t sel = UINT32_MAX;
+    float rescaled = 0.0f;
+    for (uint32_t i = 0; i < cs->count; ++i) {
+        const float next = accumulated + weights[i];
+        const int choose = !selected && weights[i] > 0.0f && target < next;
+        const float rs = cs->count > 1u
+                             ? (target - accumulated) / fmaxf(weights[i], 1.0e-20f)
+                             : random;
+        if (choose) { sel = i; rescaled = rs; }
+        selected |= choose;
+        accumulated = next;
+    }
+    (void)rescaled;
+    if (!selected) return r;
+    r.selected = sel;
+    const closure_rec *c = &cs->c[sel];
+    v3 direction;
+    int transparent = c->kind == KIND_TRANSPARENT;
+    if (transparent) {
+        direction = vneg(sf->incoming);
+    } else {
+        direction = sample_cosine_hemisphere(c->normal, u_dir, NULL);
+    }
+    const eval_t mix = evaluate_mixture(cs, sf->incoming, sf->ns, direction, lobe_mask, sel);
+    const int geometric_valid = vdot(sf->ng, direction) > 0.0f;
+    const int valid = transparent || geometric_valid;
+    r.valid = valid;
+    r.wi = direction;
+    eval_t regular;
+    regular.f = valid ? mix.f : V3(0, 0, 0);
+    regular.pdf = valid ? mix.pdf : 0.0f;
+    const v3 delta = (valid && transparent) ? vscale(c->weight, 1.0e6f) : V3(0, 0, 0);
+    const float delta_num = (valid && transparent) ? 1.0e6f * weights[sel] : 0.0f;
+    r.eval.f = vadd(regular.f, delta);
+    r.eval.pdf = regular.pdf + delta_num / fmaxf(total, 1.0e-20f);
+    uint32_t events = transparent ? (EV_TRANSMISSION | EV_TRANSPARENT)
+                                  : (EV_DIFFUSE | EV_REFLECTION);
+    r.eval.events = valid ? events : 0u;
+    return r;
+}
+
+/* AOV (src/luisa/surface_closure_operations.cpp:387-462) */
+static void surface_aov(const closure_set *cs, v3 ns, v3 *albedo, v3 *normal, v3 *transparency) {
+    v3 a = V3(0, 0, 0), n = V3(0, 0, 0), tr = V3(0, 0, 0);
+    float total = 0.0f;
+    for (uint32_t i = 0; i < cs->count; ++i) {
+        const closure_rec *c = &cs->c[i];
+        if (c->kind == KIND_DIFFUSE) {
+            const float pw = sample_weight_of(c->weight);
+            a = vadd(a, c->weight);
+            n = vadd(n, vscale(c->normal, pw));
+            total += pw;
+        } else if (c->kind == KIND_TRANSPARENT) {
+            tr = vadd(tr, c->weight);
+        }
+    }
+    *albedo = a;
+    *transparency = tr;
+    *normal = safe_normalize(total > 0.0f ? n : ns, ns);
+}
+
+static v3 transparent_extinction(const closure_set *cs) {
+    v3 r = V3(0, 0, 0);
+    for (uint32_t i = 0; i < cs->count; ++i)
+        if (cs->c[i].kind == KIND_TRANSPARENT) r = vadd(r, cs->c[i].weight);
+    return r;
+}
+
+/* ======================================================================= */
+/* triangle light sampling (include/psycles/luisa/spherical_geometry.h)    */
+/* ======================================================================= */
+
+static float triangle_solid_angle(v3 ref, v3 p0, v3 p1, v3 p2) {
+    const v3 a = normalize_or(vsub(p0, ref), V3(1, 0, 0));
+    const v3 b = normalize_or(vsub(p1, ref), V3(0, 1, 0));
+    const v3 c = normalize_or(vsub(p2, ref), V3(0, 0, 1));
+    const float num = fabsf(vdot(a, vcross(b, c)));
+    const float den = 1.0f + vdot(a, b) + vdot(b, c) + vdot(c, a);
+    return 2.0f * atan2f(num, den);
+}
+
+static int use_solid_angle(v3 ref, v3 p0, v3 p1, v3 p2) {
+    const v3 e0 = vsub(p1, p0), e1 = vsub(p2, p0), e2 = vsub(p2, p1);
+    const float longest = fmaxf(vdot(e0, e0), fmaxf(vdot(e1, e1), vdot(e2, e2)));
+    const v3 nn = vcross(e0, e1);
+    const float nl2 = vdot(nn, nn);
+    const float safe = fmaxf(nl2, 1.0e-20f);
+    const float dist = vdot(nn, vsub(p0, ref)) / safe;
+    return (nl2 > 1.0e-20f) && (longest > dist * dist);
+}
+
+static v2 low_distortion_barycentric(v2 r) {
+    const float x = fclamp(r.x, 0.0f, 1.0f);
+    const float y = fclamp(r.y, 0.0f, 1.0f);
+    const int upper = y > x;
+    const float u_upper = 0.5f * x;
+    const float v_upper = y - u_upper;
+    const float v_lower = 0.5f * y;
+    const float u_lower = x - v_lower;
+    v2 b = {upper ? u_upper : u_lower, upper ? v_upper : v_lower};
+    return b;
+}
+
+typedef struct tri_pdf_t {
+    float value;
+    int valid;
+} tri_pdf_t;
+
+static tri_pdf_t triangle_directional_pdf(v3 ref, v3 lp, v3 p0, v3 p1, v3 p2) {
+    const v3 e0 = vsub(p1, p0), e1 = vsub(p2, p0);
+    const v3 un = vcross(e0, e1);
+    const float da = sqrtf(fmaxf(vdot(un, un), 0.0f));
+    const float area = 0.5f * da;
+    const v3 normal = vdivs(un, fmaxf(da, 1.0e-20f));
+    const v3 off = vsub(lp, ref);
+    const float d2 = vdot(off, off);
+    const v3 dir = vdivs(off, sqrtf(fmaxf(d2, 1.0e-20f)));
+    const float cosine = fabsf(vdot(normal, vneg(dir)));
+    const int solid = use_solid_angle(ref, p0, p1, p2);
+    const float sa = triangle_solid_angle(ref, p0, p1, p2);
+    const float area_pdf = d2 / fmaxf(cosine * area, 1.0e-20f);
+    const float sa_pdf = 1.0f / fmaxf(sa, 1.0e-20f);
+    const int valid_area = (area > 0.0f) && (d2 > 1.0e-12f) && (cosine > 0.0f);
+    tri_pdf_t r;
+    r.value = solid ? sa_pdf : area_pdf;
+    r.valid = solid ? (valid_area && sa > 0.0f) : valid_area;
+    return r;
+}
+
+typedef struct light_sample_t {
+    v3 position, direction;
+    float distance, pdf;
+    int valid;
+} light_sample_t;
+
+static light_sample_t sample_triangle(v3 ref, v3 p0, v3 p1, v3 p2, v2 random) {
+    const v2 ab = low_distortion_barycentric(random);
+    const v3 apos = vadd(vadd(vscale(p0, 1.0f - ab.x - ab.y), vscale(p1, ab.x)), vscale(p2, ab.y));
+    const v3 aoff = vsub(apos, ref);
+    const float ad2 = vdot(aoff, aoff);
+    const float ad = sqrtf(fmaxf(ad2, 1.0e-20f));
+    const v3 adir = vdivs(aoff, ad);
+    const v3 a = normalize_or(vsub(p0, ref), V3(1, 0, 0));
+    const v3 b = normalize_or(vsub(p1, ref), V3(0, 1, 0));
+    const v3 c = normalize_or(vsub(p2, ref), V3(0, 0, 1));
+    const float cos_a = fclamp(vdot(b, c), -1.0f, 1.0f);
+    const float cos_b = fclamp(vdot(a, c), -1.0f, 1.0f);
+    const float cos_c = fclamp(vdot(a, b), -1.0f, 1.0f);
+    (void)cos_a;
+    const float sa = triangle_solid_angle(ref, p0, p1, p2);
+    const v3 ab_n = normalize_or(vcross(a, b), V3(1, 0, 0));
+    const v3 ac_n = normalize_or(vcross(a, c), V3(0, 1, 0));
+    const float cos_alpha = fclamp(vdot(ab_n, ac_n), -1.0f, 1.0f);
+    const float sin_alpha = sqrtf(fmaxf(1.0f - cos_alpha * cos_alpha, 0.0f));
+    const float alpha = acosf(cos_alpha);
+    const float phi = fclamp(random.x, 0.0f, 1.0f) * sa - alpha;
+    const float sin_phi = sinf(phi);
+    const float cos_phi = cosf(phi);
+    const float u = cos_phi - cos_alpha;
+    const float v = sin_phi + sin_alpha * cos_c;
+    const float num = (v * cos_phi - u * sin_phi) * cos_alpha - v;
+    const float den = (v * sin_phi + u * cos_phi) * sin_alpha;
+    const float safe_den = fabsf(den) > 1.0e-20f ? den : (den >= 0.0f ? 1.0e-20f : -1.0e-20f);
+    const float q = fabsf(den) <= 1.0e-20f ? 1.0f : num / safe_den;
+    const v3 tangent = normalize_or(vsub(c, vscale(a, cos_b)), V3(0, 1, 0));
+    const float qc = fclamp(q, -1.0f, 1.0f);
+    const v3 c_prime = normalize_or(vadd(vscale(a, qc), vscale(tangent, sqrtf(fmaxf(1.0f - q * q, 0.0f)))), a);
+    const float z = 1.0f - fclamp(random.y, 0.0f, 1.0f) * (1.0f - vdot(c_prime, b));
+    const v3 edge_dir = normalize_or(vsub(c_prime, vscale(b, vdot(c_prime, b))), a);
+    const v3 sdir = normalize_or(vadd(vscale(b, z), vscale(edge_dir, sqrtf(fmaxf(1.0f - z * z, 0.0f)))), b);
+    const v3 e01 = vsub(p1, p0), e02 = vsub(p2, p0);
+    const v3 pv = vcross(sdir, e02);
+    const float det = vdot(e01, pv);
+    const float safe_det = fabsf(det) > 1.0e-20f ? det : (det >= 0.0f ? 1.0e-20f : -1.0e-20f);
+    const float inv_det = 1.0f / safe_det;
+    const v3 tv = vsub(ref, p0);
+    const float su = vdot(tv, pv) * inv_det;
+    const v3 qv = vcross(tv, e01);
+    const float sv = vdot(sdir, qv) * inv_det;
+    const float sdist = vdot(e02, qv) * inv_det;
+    const v3 spos = vadd(ref, vscale(sdir, sdist));
+    const int svalid = (fabsf(det) > 1.0e-20f) && (sdist > 0.0f) && (su >= -1.0e-5f) &&
+                       (sv >= -1.0e-5f) && (su + sv <= 1.0f + 1.0e-5f) && (sa > 0.0f);
+    const int solid = use_solid_angle(ref, p0, p1, p2);
+    light_sample_t r;
+    r.position = solid ? spos : apos;
+    const tri_pdf_t pdf = triangle_directional_pdf(ref, r.position, p0, p1, p2);
+    r.direction = solid ? sdir : adir;
+    r.distance = solid ? sdist : ad;
+    r.pdf = pdf.value;
+    r.valid = pdf.valid && (solid ? svalid : (ad2 > 1.0e-12f));
+    return r;
+}
+
+static int samples_front(uint32_t s) {
+    return s == PSX_EMISSION_SAMPLING_AUTOMATIC || s == PSX_EMISSION_SAMPLING_FRONT ||
+           s == PSX_EMISSION_SAMPLING_FRONT_BACK;
+}
+static int samples_back(uint32_t s) {
+    return s == PSX_EMISSION_SAMPLING_AUTOMATIC || s == PSX_EMISSION_SAMPLING_BACK ||
+           s == PSX_EMISSION_SAMPLING_FRONT_BACK;
+}
+static int sampled_side(uint32_t s, v3 ng, v3 dir) {
+    const int back = vdot(ng, vneg(dir)) < 0.0f;
+    return back ? samples_back(s) : samples_front(s);
+}
+
+/* ======================================================================= */
+/* kernel parameters (src/luisa/path_tracer_kernel.cpp:145-374)             */
+/* ======================================================================= */
+
+typedef struct kparams {
+    uint32_t window_x, window_y, window_width, window_height;
+    uint32_t full_width, full_height, seed;
+    uint32_t max_bounces, min_bounces, max_diffuse, max_glossy, max_transmission;
+    uint32_t transparent_min, transparent_max, max_path_steps;
+    uint32_t transparent_background;
+    float clamp_direct, clamp_indirect;
+    float light_inv_rr_threshold;
+    float tan_h, tan_v, shift_x, shift_y, near_clip, far_clip;
+    float pass_alpha_threshold;
+    int nee;
+    int mis_mode; /* psx_direct_light_sampling actually used */
+    mat4 camera;
+    sampler smp;
+    float filter[1024];
+} kparams;
+
+static uint32_t synced(uint32_t v) { return v == UINT32_MAX ? v : v + 1u; }
+
+static void setup_params(const oracle_scene *s, const psx_render_settings *rs,
+                         const float *table, uint32_t sequence_size, kparams *k) {
+    memset(k, 0, sizeof(*k));
+    k->window_x = rs->window_x;
+    k->window_y = rs->window_y;
+    k->window_width = rs->window_width ? rs->window_width : rs->width;
+    k->window_height = rs->window_height ? rs->window_height : rs->height;
+    k->full_width = rs->width ? rs->width : 1u;
+    k->full_height = rs->height ? rs->height : 1u;
+    k->seed = rs->seed;
+    const psx_integrator_settings *in = &rs->integrator;
+    k->max_bounces = synced(in->max_bounces);
+    k->min_bounces = synced(in->min_bounces);
+    k->max_diffuse = synced(in->diffuse_bounces);
+    k->max_glossy = synced(in->glossy_bounces);
+    k->max_transmission = synced(in->transmission_bounces);
+    k->transparent_min = synced(in->transparent_min_bounces);
+    k->transparent_max = in->transparent_max_bounces;
+    {
+        uint64_t req = 2ull * k->max_bounces +
+                       (in->transparent_max_bounces > 1u ? in->transparent_max_bounces : 1u) + 1ull;
+        k->max_path_steps = (uint32_t)(req < 1024ull ? req : 1024ull);
+    }
+    k->transparent_background = rs->transparent_background;
+    k->clamp_direct = in->sample_clamp_direct > 0.0f ? in->sample_clamp_direct * 3.0f : 0.0f;
+    k->clamp_indirect = in->sample_clamp_indirect > 0.0f ? in->sample_clamp_indirect * 3.0f : 0.0f;
+    k->light_inv_rr_threshold = (!in->use_light_tree && in->light_sampling_threshold > 0.0f)
+                                    ? in->film_exposure / in->light_sampling_threshold
+                                    : 0.0f;
+    const int available = s->distribution_count > 0u;
+    k->nee = in->direct_light_sampling != PSX_DIRECT_LIGHT_FORWARD && available;
+    k->mis_mode = k->nee ? (int)in->direct_light_sampling : PSX_DIRECT_LIGHT_FORWARD;
+    const float aspect = (float)k->full_width / (float)k->full_height;
+    const int hfit = s->camera.sensor_fit == 0u || (s->camera.sensor_fit == 2u && aspect >= 1.0f);
+    k->tan_v = hfit ? tanf(s->camera.horizontal_field_of_view * 0.5f) / aspect
+                    : tanf(s->camera.field_of_view * 0.5f);
+    k->tan_h = k->tan_v * aspect;
+    k->shift_x = s->camera.lens_shift_x * (hfit ? 1.0f : 1.0f / aspect);
+    k->shift_y = s->camera.lens_shift_y * (hfit ? aspect : 1.0f);
+    k->near_clip = s->camera.near_clip;
+    k->far_clip = s->camera.far_clip;
+    k->pass_alpha_threshold = fclamp(rs->pass_alpha_threshold, 0.0f, 1.0f);
+    memcpy(k->camera.m, s->camera.transform, sizeof(k->camera.m));
+    k->smp.table = table;
+    k->smp.sequence_size = sequence_size;
+    oracle_pixel_filter_table(rs->pixel_filter, rs->filter_width, k->filter);
+}
+
+/* ======================================================================= */
+/* light transport callables (src/luisa/path_tracer_lighting.cpp)           */
+/* ======================================================================= */
+
+static float power_heuristic(float a, float b) {
+    const float a2 = a * a, b2 = b * b;
+    return a2 / fmaxf(a2 + b2, 1.0e-20f);
+}
+
+static float forward_light_weight(const kparams *k, float forward_pdf, float nee_pdf,
+                                  int competing, int nee_available) {
+    if (k->mis_mode == PSX_DIRECT_LIGHT_FORWARD) return 1.0f;
+    if (k->mis_mode == PSX_DIRECT_LIGHT_NEE) return (competing && nee_available) ? 0.0f : 1.0f;
+    return (competing && nee_available) ? power_heuristic(forward_pdf, nee_pdf) : 1.0f;
+}
+
+static float nee_light_weight(const kparams *k, float nee_pdf, float forward_pdf) {
+    if (k->mis_mode == PSX_DIRECT_LIGHT_NEE) return 1.0f;
+    if (k->mis_mode == PSX_DIRECT_LIGHT_FORWARD) return forward_pdf <= 0.0f ? 1.0f : 0.0f;
+    return power_heuristic(nee_pdf, forward_pdf);
+}
+
+/* cycles_film_light::clamp (include/psycles/luisa/cycles_film_light.h:27-50) */
+static v3 film_clamp(v3 c, uint32_t depth, float direct, float indirect) {
+    c.x = isfinite(c.x) ? c.x : 0.0f;
+    c.y = isfinite(c.y) ? c.y : 0.0f;
+    c.z = isfinite(c.z) ? c.z : 0.0f;
+    const float limit = depth > 0u ? indirect : direct;
+    const float mag = fabsf(c.x) + fabsf(c.y) + fabsf(c.z);
+    if (limit > 0.0f && mag > limit) return vscale(c, limit / fmaxf(mag, 1.0e-20f));
+    return c;
+}
+
+static float light_roulette(v3 u, float random, float inv_threshold) {
+    const float maximum = fmaxf(fabsf(u.x), fmaxf(fabsf(u.y), fabsf(u.z)));
+    const float probability = maximum * inv_threshold;
+    const int roulette = (inv_threshold > 0.0f) && (probability < 1.0f);
+    const int survives = !roulette || (random < probability);
+    const float inverse = roulette ? 1.0f / fmaxf(probability, 1.0e-20f) : 1.0f;
+    return survives ? inverse : 0.0f;
+}
+
+/* ======================================================================= */
+/* the path integrator (src/luisa/path_kernel_pipeline.cpp:73-185 and the  */
+/* stages it calls)                                                        */
+/* ======================================================================= */
+
+typedef struct film_pixel {
+    float combined[4], normal[4], albedo[4];
+    uint32_t count;
+} film_pixel;
+
+typedef struct trace_log {
+    uint32_t *records; /* PSX_TRACE_EVENTS * PSX_TRACE_WORDS, or NULL */
+    uint32_t events;
+} trace_log;
+
+static void log_event(trace_log *log, uint32_t step, uint32_t word, uint32_t value) {
+    if (!log || !log->records || step >= PSX_TRACE_EVENTS) return;
+    log->records[step * PSX_TRACE_WORDS + word] = value;
+}
+static void log_flag(trace_log *log, uint32_t step, uint32_t bit) {
+    if (!log || !log->records || step >= PSX_TRACE_EVENTS) return;
+    log->records[step * PSX_TRACE_WORDS + PSX_TRACE_FLAGS] |= bit;
+}
+
+typedef struct camera_ray_t {
+    ray_t ray;
+    uint32_t rng_hash;
+} camera_ray_t;
+
+/* sample_camera_dimensions + construct_camera_ray (perspective)
+ * (src/luisa/path_tracer_camera.cpp:10-244) */
+static camera_ray_t camera_ray(const kparams *k, uint32_t full_x, uint32_t full_y,
+                               uint32_t sample_index) {
+    camera_ray_t r;
+    const uint32_t cycles_y = k->full_height - 1u - full_y;
+    r.rng_hash = oracle_pixel_hash(full_x, cycles_y, k->seed);
+    const float *f = sample_4d(&k->smp, sample_index, r.rng_hash, 0u);
+    v2 filter = {f[0], f[1]};
+    if (sample_index == 0u) filter.x = filter.y = 0.5f;
+    const float jx = oracle_pixel_filter_sample(k->filter, filter.x);
+    const float jy = 1.0f - oracle_pixel_filter_sample(k->filter, filter.y);
+    const float width = (float)k->full_width;
+    const float height = (float)k->full_height;
+    const float sx = 2.0f * ((float)full_x + jx) / width - 1.0f + 2.0f * k->shift_x;
+    const float sy = 1.0f - 2.0f * ((float)full_y + jy) / height + 2.0f * k->shift_y;
+    const v3 local_dir = vnormalize(V3(sx * k->tan_h, sy * k->tan_v, -1.0f));
+    const float clip_cos = -local_dir.z;
+    const v3 world_o = xform_point(&k->camera, V3(0, 0, 0));
+    const v3 world_d = xform_dir(&k->camera, local_dir);
+    const v3 d = safe_normalize(world_d, V3(0, 0, -1));
+    float clip[2];
+    oracle_camera_clip_range(k->near_clip, k->far_clip, clip_cos, clip);
+    r.ray.o = vadd(world_o, vscale(d, clip[0]));
+    r.ray.d = d;
+    r.ray.tmin = 0.0f;
+    r.ray.tmax = clip[1];
+    return r;
+}
+
+typedef struct path_out {
+    v3 radiance, normal, albedo;
+    float transparency;
+    uint64_t rays;
+} path_out;
+
+static v3 triangle_world_position_mm(const oinstance *inst, v3 p) {
+    return mat_mul(&inst->o2w, p, 1.0f);
+}
+
+static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
+                      uint32_t full_y, uint32_t sample_index, path_out *out,
+                      trace_log *log, trace_counters *counters) {
+    const camera_ray_t cam = camera_ray(k, full_x, full_y, sample_index);
+    const uint32_t rng_hash = cam.rng_hash;
+    ray_t ray = cam.ray;
+    uint32_t src_obj = UINT32_MAX, src_prim = UINT32_MAX;
+    uint32_t ray_visibility = PSX_VISIBILITY_CAMERA;
+    v3 radiance = V3(0, 0, 0), throughput = V3(1, 1, 1);
+    v3 sample_normal = V3(0, 0, 0), sample_albedo = V3(0, 0, 0);
+    float transparency = 0.0f;
+    int primary_recorded = 0;
+    float previous_bsdf_pdf = 0.0f;
+    float minimum_bsdf_pdf = FLT_MAX;
+    int previous_delta = 1;
+    uint32_t ray_events = 0u;
+    oracle_path_state ps;
+    oracle_initial_state(&ps);
+    int terminate_after_transparent = 0, terminate_on_next_surface = 0;
+    uint64_t rays = 0;
+    const oracle_path_limits lim = {k->max_bounces, k->max_diffuse, k->max_glossy,
+                                    k->max_transmission, k->transparent_max};
+
+    for (uint32_t step = 0; step < k->max_path_steps; ++step) {
+        log_flag(log, step, 1u);
+        log_event(log, step, PSX_TRACE_RNG_OFFSET, ps.rng_offset);
+        log_event(log, step, PSX_TRACE_LIGHT, UINT32_MAX);
+        log_event(log, step, PSX_TRACE_CLOSURE, UINT32_MAX);
+        log_event(log, step, PSX_TRACE_OBJECT, UINT32_MAX);
+        log_event(log, step, PSX_TRACE_PRIMITIVE, UINT32_MAX);
+        /* bounce setup (src/luisa/path_kernel_bounce_setup.cpp:19-152) */
+        const float u_term = sample_4d(&k->smp, sample_index, rng_hash, ps.rng_offset + 0u)[0];
+        const float *ul = sample_4d(&k->smp, sample_index, rng_hash, ps.rng_offset + 1u);
+        const v3 u_light = V3(ul[0], ul[1], ul[2]);
+        const dist_entry *light = NULL;
+        if (s->distribution_count > 0u)
+            light = distribution_sample(s->distribution, s->distribution_count, u_light.z);
+        const float u_lightterm = sample_4d(&k->smp, sample_index, rng_hash, ps.rng_offset + 2u)[0];
+        const float *ub = sample_4d(&k->smp, sample_index, rng_hash, ps.rng_offset + 3u);
+        const v3 u_bsdf = V3(ub[0], ub[1], ub[2]);
+        log_event(log, step, PSX_TRACE_TERMINATE_BITS, fbits(u_term));
+        const hit_t hit = closest_hit(s, &ray, ray_visibility, src_obj, src_prim, UINT32_MAX,
+                                      UINT32_MAX, counters);
+        rays++;
+        if (!hit.valid) {
+            /* background event (src/luisa/path_kernel_background_event.cpp:20-159) */
+            log_flag(log, step, 4u);
+            const v3 contribution = film_clamp(vscale(vmul(throughput, s->background), 1.0f),
+                                               ps.bounce > 1u ? 1u : 0u, k->clamp_direct,
+                                               k->clamp_indirect);
+            const int world_visible = (s->world_visibility & ray_visibility) != 0u;
+            const int transparent_bg = k->transparent_background &&
+                                       (ps.flag & FLAG_TRANSPARENT_BACKGROUND) != 0u;
+            if (transparent_bg) {
+                transparency += (throughput.x + throughput.y + throughput.z) * (1.0f / 3.0f);
+            } else if (world_visible) {
+                radiance = vadd(radiance, contribution);
+            }
+            break;
+        }
+        log_flag(log, step, 2u);
+        log_event(log, step, PSX_TRACE_OBJECT, hit.object);
+        log_event(log, step, PSX_TRACE_PRIMITIVE, hit.cprim);
+        /* surface geometry */
+        surface_t sf;
+        resolve_surface(s, &hit, &ray, &sf);
+        const omaterial *mat = sf.material < s->material_count ? &s->materials[sf.material] : NULL;
+        closure_set cs;
+        if (mat) material_closures(mat, sf.ns, &cs);
+        else { cs.count = 0; cs.emission = V3(0, 0, 0); }
+        uint32_t lobe_mask = LOBE_ALL;
+        if (terminate_after_transparent) lobe_mask = EV_TRANSPARENT;
+        /* surface shading (src/luisa/path_kernel_surface_shading.cpp:115-250) */
+        const v3 emitted = cs.emission;
+        float emission_weight = 1.0f;
+        if (k->nee && s->emitter_count > 0u &&
+            sf.emission_sampling != PSX_EMISSION_SAMPLING_NONE) {
+            const int competing = ps.bounce > 0u && !previous_delta;
+            const v3 oriented = sf.back_facing ? vneg(sf.ng) : sf.ng;
+            const float selection_pdf =
+                0.5f * vlength(vcross(vsub(sf.wp1, sf.wp0), vsub(sf.wp2, sf.wp0))) *
+                s->triangle_area_pdf;
+            const tri_pdf_t tp = triangle_directional_pdf(ray.o, sf.position, sf.wp0, sf.wp1, sf.wp2);
+            const v3 off = vsub(sf.position, ray.o);
+            const v3 dir = normalize_or(off, V3(0, 0, 1));
+            const int side = sampled_side(sf.emission_sampling, oriented, dir);
+            const float area = 0.5f * vlength(vcross(vsub(sf.wp1, sf.wp0), vsub(sf.wp2, sf.wp0)));
+            const float value = tp.value * selection_pdf;
+            const int valid = side && tp.valid && area > 0.0f && value > 0.0f;
+            const float light_pdf = valid ? value : 0.0f;
+            emission_weight = forward_light_weight(k, previous_bsdf_pdf, light_pdf, competing,
+                                                   light_pdf > 0.0f);
+        }
+        const v3 emission_contribution =
+            film_clamp(vscale(vmul(throughput, emitted), emission_weight),
+                       ps.bounce > 1u ? 1u : 0u, k->clamp_direct, k->clamp_indirect);
+        radiance = vadd(radiance, emission_contribution);
+        if (terminate_on_next_surface) break;
+        /* Russian roulette */
+        {
+            const float th[3] = {throughput.x, throughput.y, throughput.z};
+            const float p = oracle_continuation_probability(ps.flag, ps.bounce, ps.transparent_bounce,
+                                                            k->min_bounces, k->transparent_min, th);
+            log_event(log, step, PSX_TRACE_RR_BITS, fbits(p));
+            if (p <= 0.0f) { log_flag(log, step, 8u); break; }
+            if (p != 1.0f) {
+                if (u_term >= p) { log_flag(log, step, 8u); break; }
+                throughput = vdivs(throughput, p);
+            }
+        }
+        if (ps.bounce == 0u) {
+            v3 aov_albedo, aov_normal, aov_tr;
+            surface_aov(&cs, sf.ns, &aov_albedo, &aov_normal, &aov_tr);
+            sample_albedo = vadd(sample_albedo, vmul(throughput, aov_albedo));
+            const v3 alpha = vclamp(vsub(V3(1, 1, 1), aov_tr), 0.0f, 1.0f);
+            const float avg_alpha = (alpha.x + alpha.y + alpha.z) * (1.0f / 3.0f);
+            const int writes = !primary_recorded &&
+                               (k->pass_alpha_threshold == 0.0f || avg_alpha >= k->pass_alpha_threshold);
+            if (writes) sample_normal = aov_normal;
+            primary_recorded |= writes;
+            if (writes) ps.flag |= FLAG_SINGLE_PASS_DONE;
+        }
+        const uint32_t runtime_flags = surface_runtime_flags(&cs, sf.back_facing);
+        /* next-event estimation: emissive mesh component
+         * (src/luisa/path_kernel_nee_emissive_mesh.cpp:30-298) */
+        if (k->nee && (runtime_flags & RT_BSDF_HAS_EVAL) != 0u && light && light->kind == 0u) {
+            log_flag(log, step, 16u);
+            log_event(log, step, PSX_TRACE_LIGHT, (uint32_t)(light - s->distribution));
+            const emitter *em = &s->emitters[light->index];
+            const oinstance *einst = &s->instances[em->instance];
+            const omesh *emesh = &s->meshes[einst->mesh];
+            v3 lp0, lp1, lp2;
+            tri_positions(emesh->positions, emesh->tris, em->prim, &lp0, &lp1, &lp2);
+            const v3 p0 = triangle_world_position_mm(einst, lp0);
+            const v3 p1 = triangle_world_position_mm(einst, lp1);
+            const v3 p2 = triangle_world_position_mm(einst, lp2);
+            const v3 local_n = normalize_or(vcross(vsub(lp1, lp0), vsub(lp2, lp0)), V3(0, 0, 1));
+            const v3 egn = normalize_or(mat_mul(&einst->n2w, local_n, 0.0f), V3(0, 0, 1));
+            const float earea = 0.5f * vlength(vcross(vsub(p1, p0), vsub(p2, p0)));
+            const v2 rnd = {u_light.x, u_light.y};
+            const light_sample_t ls = sample_triangle(sf.position, p0, p1, p2, rnd);
+            const int side = sampled_side(em->emission_sampling, egn, ls.direction);
+            const float lpdf = ls.pdf * light->selection_pdf;
+            const int lvalid = ls.valid && side && earea > 0.0f && lpdf > 0.0f;
+            if (lvalid) {
+                const uint32_t eobj = em->instance;
+                const uint32_t eprim = emesh->prim_offset + em->prim;
+                const int is_transmission = vdot(ls.direction, sf.ns) < 0.0f;
+                const int same = eprim == sf.cprim && eobj == sf.object;
+                const v3 oriented = is_transmission ? vneg(sf.ng) : sf.ng;
+                const int reject_self = same && vdot(ls.direction, oriented) > 0.0f;
+                if (!reject_self) {
+                    const v3 le = constant_emission(&s->materials[em->material]);
+                    const eval_t ev = evaluate_mixture(&cs, sf.incoming, sf.ns, ls.direction,
+                                                       lobe_mask, UINT32_MAX);
+                    const float mis = nee_light_weight(k, lpdf, ev.pdf);
+                    const v3 weighted = vscale(vmul(ev.f, le), mis / lpdf);
+                    if (vany_nonzero(weighted)) {
+                        const v3 unshadowed = weighted;
+                        const float rr = light_roulette(unshadowed, u_lightterm, k->light_inv_rr_threshold);
+                        const v3 surviving = vscale(unshadowed, rr);
+                        if (vany_nonzero(surviving)) {
+                            const v3 so = surface_shadow_origin(s, &sf, ls.direction);
+                            const v3 soff = vsub(ls.position, so);
+                            const float sdist = sqrtf(fmaxf(vdot(soff, soff), 1.0e-20f));
+                            ray_t sray;
+                            sray.o = so;
+                            sray.d = vdivs(soff, sdist);
+                            sray.tmin = 0.0f;
+                            sray.tmax = sdist;
+                            /* trace_shadow (src/luisa/path_tracer_geometry.cpp:75-174) */
+                            v3 transmittance = V3(1, 1, 1);
+                            uint32_t tdepth = ps.transparent_bounce;
+                            for (;;) {
+                                const hit_t sh = closest_hit(s, &sray, PSX_VISIBILITY_SHADOW, sf.object,
+                                                             sf.cprim, eobj, eprim, counters);
+                                rays++;
+                                if (!sh.valid) break;
+                                if (tdepth >= k->transparent_max) { transmittance = V3(0, 0, 0); break; }
+                                const oinstance *si = &s->instances[sh.inst];
+                                const uint32_t smat = material_of(s, si, sh.prim);
+                                v3 ext = V3(0, 0, 0);
+                                if (smat < s->material_count) {
+                                    surface_t ssf;
+                                    resolve_surface(s, &sh, &sray, &ssf);
+                                    closure_set scs;
+                                    material_closures(&s->materials[smat], ssf.ns, &scs);
+                                    ext = vclamp(transparent_extinction(&scs), 0.0f, 1.0f);
+                                }
+                                if (fmaxf(ext.x, fmaxf(ext.y, ext.z)) > 0.0f) {
+                                    transmittance = vmul(transmittance, ext);
+                                    tdepth += 1u;
+                                    sray.tmin = bitsf(intersection_t_offset_bits(sh.t));
+                                } else {
+                                    transmittance = V3(0, 0, 0);
+                                    break;
+                                }
+                            }
+                            if (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f) {
+                                const v3 c = film_clamp(vmul(vmul(throughput, surviving), transmittance),
+                                                        ps.bounce, k->clamp_direct, k->clamp_indirect);
+                                radiance = vadd(radiance, c);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        /* surface scatter (src/luisa/path_kernel_surface_scatter.cpp:16-324) */
+        const v2 udir = {u_bsdf.x, u_bsdf.y};
+        const bsdf_sample_t bs = sample_closures(&cs, &sf, u_bsdf.z, udir, lobe_mask);
+        log_event(log, step, PSX_TRACE_CLOSURE, bs.selected);
+        if (!bs.valid || bs.eval.pdf <= 0.0f) break;
+        log_flag(log, step, 32u);
+        const int transparent = (bs.eval.events & EV_TRANSPARENT) != 0u;
+        const int singular = (bs.eval.events & EV_SINGULAR) != 0u;
+        if (transparent) log_flag(log, step, 64u);
+        const uint32_t label = label_from_events(bs.eval.events);
+        throughput = vmul(throughput, vdivs(bs.eval.f, bs.eval.pdf));
+        if (vany_nan(throughput) || throughput.x < 0.0f || throughput.y < 0.0f || throughput.z < 0.0f)
+            break;
+        ray_events = transparent ? (ray_events | bs.eval.events) : bs.eval.events;
+        oracle_path_state next;
+        oracle_next_surface(&ps, label, bs.runtime_flags, &lim, &next);
+        ps = next;
+        ray_visibility = oracle_contract_visibility(ps.visibility);
+        terminate_on_next_surface = (ps.flag & FLAG_TERMINATE_ON_NEXT_SURFACE) != 0u;
+        terminate_after_transparent = (ps.flag & FLAG_TERMINATE_AFTER_TRANSPARENT) != 0u;
+        if (!transparent) {
+            previous_bsdf_pdf = bs.eval.pdf;
+            minimum_bsdf_pdf = fminf(minimum_bsdf_pdf, bs.eval.pdf);
+            previous_delta = singular;
+        }
+        const v3 nd = vnormalize(bs.wi);
+        if (transparent) {
+            ray.tmin = bitsf(intersection_t_offset_bits(hit.t));
+        } else {
+            ray.o = surface_ray_origin(s, &sf, nd);
+            ray.d = nd;
+            ray.tmin = 0.0f;
+            ray.tmax = 1.0e30f;
+        }
+        src_obj = sf.object;
+        src_prim = sf.cprim;
+    }
+    (void)minimum_bsdf_pdf;
+    (void)ray_events;
+    out->radiance = radiance;
+    out->normal = sample_normal;
+    out->albedo = sample_albedo;
+    out->transparency = transparency;
+    out->rays = rays;
+}
+
+/* ======================================================================= */
+/* rendering, threading and passes                                          */
+/* ======================================================================= */
+
+typedef struct job {
+    const oracle_scene *scene;
+    const kparams *k;
+    uint32_t first, count;
+    uint32_t row_begin, row_end;
+    float *combined, *normal, *albedo;
+    uint32_t *sample_count;
+    uint32_t next_row;
+    pthread_mutex_t lock;
+    uint64_t rays, nodes, tris;
+} job;
+
+static void render_pixel(job *j, uint32_t px, uint32_t py, trace_counters *tc, uint64_t *rays) {
+    const kparams *k = j->k;
+    const size_t index = (size_t)py * k->window_width + px;
+    float *c = j->combined + 4 * index;
+    float *n = j->normal + 4 * index;
+    float *a = j->albedo + 4 * index;
+    uint32_t completed = j->sample_count[index];
+    for (uint32_t si = 0; si < j->count; ++si) {
+        path_out o;
+        integrate(j->scene, k, px + k->window_x, py + k->window_y, j->first + si, &o, NULL, tc);
+        *rays += o.rays;
+        v3 r = o.radiance;
+        if (vany_nan(r)) r = V3(0, 0, 0);
+        c[0] += r.x; c[1] += r.y; c[2] += r.z; c[3] += o.transparency;
+        n[0] += o.normal.x; n[1] += o.normal.y; n[2] += o.normal.z; n[3] += 1.0f;
+        a[0] += o.albedo.x; a[1] += o.albedo.y; a[2] += o.albedo.z; a[3] += 1.0f;
+        completed += 1u;
+    }
+    j->sample_count[index] = completed;
+}
+
+static void *worker(void *arg) {
+    job *j = (job *)arg;
+    trace_counters tc = {0, 0};
+    uint64_t rays = 0;
+    for (;;) {
+        pthread_mutex_lock(&j->lock);
+        const uint32_t row = j->next_row++;
+        pthread_mutex_unlock(&j->lock);
+        if (row >= j->row_end) break;
+        for (uint32_t x = 0; x < j->k->window_width; ++x) render_pixel(j, x, row, &tc, &rays);
+    }
+    pthread_mutex_lock(&j->lock);
+    j->rays += rays;
+    j->nodes += tc.nodes;
+    j->tris += tc.tris;
+    pthread_mutex_unlock(&j->lock);
+    return NULL;
+}
+
+int oracle_render_rows(const oracle_scene *s, const psx_render_settings *rs, uint32_t first,
+                       uint32_t count, uint32_t total, int threads, uint32_t row_begin,
+                       uint32_t row_end, float *combined, float *normal, float *albedo,
+                       uint32_t *sample_count, oracle_stats *stats) {
+    if (!s || !rs || total == 0u || (uint64_t)first + count > total) return -1;
+    const uint32_t seq = oracle_sequence_size_for_samples(total);
+    float *table = (float *)malloc(sizeof(float) * 4u * 256u * seq);
+    oracle_sobol_table(seq, table);
+    kparams *k = (kparams *)malloc(sizeof(kparams));
+    setup_params(s, rs, table, seq, k);
+    if (row_end > k->window_height) row_end = k->window_height;
+    job j;
+    memset(&j, 0, sizeof(j));
+    j.scene = s;
+    j.k = k;
+    j.first = first;
+    j.count = count;
+    j.row_begin = row_begin;
+    j.row_end = row_end;
+    j.next_row = row_begin;
+    j.combined = combined;
+    j.normal = normal;
+    j.albedo = albedo;
+    j.sample_count = sample_count;
+    pthread_mutex_init(&j.lock, NULL);
+    if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t tids[256];
+    for (int t = 0; t < threads; ++t) pthread_create(&tids[t], NULL, worker, &j);
+    for (int t = 0; t < threads; ++t) pthread_join(tids[t], NULL);
+    pthread_mutex_destroy(&j.lock);
+    if (stats) {
+        stats->rays = j.rays;
+        stats->node_visits = j.nodes;
+        stats->triangle_tests = j.tris;
+        stats->samples = (uint64_t)(row_end - row_begin) * k->window_width * count;
+    }
+    free(k);
+    free(table);
+    return 0;
+}
+
+int oracle_render(const oracle_scene *s, const psx_render_settings *rs, uint32_t first,
+                  uint32_t count, uint32_t total, int threads, float *combined, float *normal,
+                  float *albedo, uint32_t *sample_count, oracle_stats *stats) {
+    const uint32_t h = rs->window_height ? rs->window_height : rs->height;
+    return oracle_render_rows(s, rs, first, count, total, threads, 0u, h, combined, normal,
+                              albedo, sample_count, stats);
+}
+
+void oracle_write_pass(uint32_t kind, uint32_t channels, uint32_t pixel_count, float exposure,
+                       const float *combined, const float *normal, const float *albedo,
+                       const uint32_t *sample_count, float *out) {
+    for (uint32_t i = 0; i < pixel_count; ++i) {
+        const float den = (float)(sample_count[i] > 1u ? sample_count[i] : 1u);
+        float v[4] = {0, 0, 0, 0};
+        switch (kind) {
+        case PSX_PASS_COMBINED:
+            v[0] = combined[4 * i] * (exposure / den);
+            v[1] = combined[4 * i + 1] * (exposure / den);
+            v[2] = combined[4 * i + 2] * (exposure / den);
+            v[3] = fclamp(1.0f - combined[4 * i + 3] / den, 0.0f, 1.0f);
+            break;
+        case PSX_PASS_NORMAL:
+        case PSX_PASS_DENOISING_NORMAL:
+            for (int c = 0; c < 4; ++c) v[c] = normal[4 * i + c] / den;
+            break;
+        case PSX_PASS_ALBEDO:
+        case PSX_PASS_DENOISING_ALBEDO:
+            for (int c = 0; c < 4; ++c) v[c] = albedo[4 * i + c] / den;
+            break;
+        case PSX_PASS_SAMPLE_COUNT:
+            v[0] = v[1] = v[2] = v[3] = (float)sample_count[i];
+            break;
+        default:
+            break;
+        }
+        for (uint32_t c = 0; c < channels; ++c) out[i * channels + c] = v[c < 3 ? c : 3];
+    }
+}
+
+int oracle_primary_hits(const oracle_scene *s, const psx_render_settings *rs,
+                        uint32_t sample_index, uint32_t total, uint32_t *out) {
+    const uint32_t seq = oracle_sequence_size_for_samples(total);
+    float *table = (float *)malloc(sizeof(float) * 4u * 256u * seq);
+    oracle_sobol_table(seq, table);
+    kparams *k = (kparams *)malloc(sizeof(kparams));
+    setup_params(s, rs, table, seq, k);
+    for (uint32_t y = 0; y < k->window_height; ++y)
+        for (uint32_t x = 0; x < k->window_width; ++x) {
+            const camera_ray_t cam = camera_ray(k, x + k->window_x, y + k->window_y, sample_index);
+            const hit_t h = closest_hit(s, &cam.ray, PSX_VISIBILITY_CAMERA, UINT32_MAX, UINT32_MAX,
+                                        UINT32_MAX, UINT32_MAX, NULL);
+            uint32_t *r = out + 4u * ((size_t)y * k->window_width + x);
+            r[0] = h.valid ? h.object : UINT32_MAX;
+            r[1] = h.valid ? h.cprim : UINT32_MAX;
+            r[2] = h.valid ? fbits(h.t) : 0u;
+            r[3] = h.valid ? 1u : 0u;
+        }
+    free(k);
+    free(table);
+    return 0;
+}
+
+int oracle_trace_events(const oracle_scene *s, const psx_render_settings *rs,
+                        uint32_t sample_index, uint32_t total, uint32_t *out) {
+    const uint32_t seq = oracle_sequence_size_for_samples(total);
+    float *table = (float *)malloc(sizeof(float) * 4u * 256u * seq);
+    oracle_sobol_table(seq, table);
+    kparams *k = (kparams *)malloc(sizeof(kparams));
+    setup_params(s, rs, table, seq, k);
+    const size_t per = PSX_TRACE_EVENTS * PSX_TRACE_WORDS;
+    for (uint32_t y = 0; y < k->window_height; ++y)
+        for (uint32_t x = 0; x < k->window_width; ++x) {
+            trace_log log;
+            log.records = out + per * ((size_t)y * k->window_width + x);
+            memset(log.records, 0, sizeof(uint32_t) * per);
+            path_out o;
+            integrate(s, k, x + k->window_x, y + k->window_y, sample_index, &o, &log, NULL);
+        }
+    free(k);
+    free(table);
+    return 0;
+}
+
+int oracle_sampler_probe(const psx_render_settings *rs, uint32_t sample_index, uint32_t total,
+                         uint32_t first_dimension, uint32_t dimension_count, uint32_t *out) {
+    const uint32_t seq = oracle_sequence_size_for_samples(total);
+    float *table = (float *)malloc(sizeof(float) * 4u * 256u * seq);
+    oracle_sobol_table(seq, table);
+    const uint32_t ww = rs->window_width ? rs->window_width : rs->width;
+    const uint32_t wh = rs->window_height ? rs->window_height : rs->height;
+    const uint32_t per = 1u + 4u * dimension_count;
+    sampler smp = {table, seq};
+    for (uint32_t y = 0; y < wh; ++y)
+        for (uint32_t x = 0; x < ww; ++x) {
+            const uint32_t fx = x + rs->window_x, fy = y + rs->window_y;
+            const uint32_t cy = rs->height - 1u - fy;
+            const uint32_t hash = oracle_pixel_hash(fx, cy, rs->seed);
+            uint32_t *r = out + (size_t)per * ((size_t)y * ww + x);
+            r[0] = hash;
+            for (uint32_t d = 0; d < dimension_count; ++d) {
+                const float *v = sample_4d(&smp, sample_index, hash, first_dimension + d);
+                for (int c = 0; c < 4; ++c) r[1 + 4 * d + c] = fbits(v[c]);
+            }
+        }
+    free(table);
+    return 0;
+}

@@ -1,0 +1,156 @@
+/*
+ * psx_oracle.h - CPU restatement of Psycles' device render path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker the HIP render path is
+ * compared against.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may load it; the product library never links or calls it.
+ *
+ * Plain C99, built with -O2 -ffp-contract=off and without fast-math so that
+ * every floating-point operation happens in the order written.  Each function
+ * cites the reference file it restates.  Scene and settings use the public
+ * ABI structs from include/psx.h.
+ */
+#ifndef PSX_ORACLE_H
+#define PSX_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/psx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* --- tabulated Sobol (src/sampling/tabulated_sobol.cpp) ---------------- */
+uint32_t oracle_sequence_size_for_samples(uint32_t sample_count);
+/* out: 4 * 256 * sequence_size floats; returns 0 or -1 for a bad size. */
+int oracle_sobol_table(uint32_t sequence_size, float *out);
+
+/* --- device sampler (include/psycles/luisa/cycles_sampler.h) ----------- */
+uint32_t oracle_pixel_hash(uint32_t x, uint32_t y, uint32_t seed);
+uint32_t oracle_shuffled_sample_index(uint32_t sample, uint32_t dimension,
+                                      uint32_t seed, uint32_t sequence_size);
+
+/* --- pixel filter (src/sampling/pixel_filter.cpp) ----------------------- */
+void oracle_pixel_filter_table(uint32_t filter, float width, float *out1024);
+float oracle_pixel_filter_sample(const float *table, float random);
+
+/* --- sample mappings (include/psycles/luisa/cycles_sample_mapping.h) ---- */
+void oracle_sample_uniform_disk(const float random[2], float out[2]);
+/* out: direction xyz, pdf */
+void oracle_sample_cosine_hemisphere(const float normal[3],
+                                     const float random[2], float out[4]);
+
+/* --- camera (src/luisa/path_tracer_camera.cpp) -------------------------- */
+void oracle_camera_clip_range(float near_clip, float far_clip, float cosine,
+                              float out[2]);
+void oracle_affine_point(const float m[16], const float p[3], float out[3]);
+
+/* --- path state (include/psycles/luisa/cycles_path_state.h) ------------- */
+typedef struct oracle_path_state {
+    uint32_t flag, visibility, bounce, diffuse_bounce, glossy_bounce,
+        transmission_bounce, transparent_bounce, rng_offset;
+} oracle_path_state;
+typedef struct oracle_path_limits {
+    uint32_t maximum, maximum_diffuse, maximum_glossy, maximum_transmission,
+        maximum_transparent;
+} oracle_path_limits;
+void oracle_initial_state(oracle_path_state *state);
+void oracle_next_surface(const oracle_path_state *before, uint32_t label,
+                         uint32_t runtime_flag,
+                         const oracle_path_limits *limits,
+                         oracle_path_state *after);
+uint32_t oracle_contract_visibility(uint32_t cycles_visibility);
+float oracle_continuation_probability(uint32_t flag, uint32_t bounce,
+                                      uint32_t transparent_bounce,
+                                      uint32_t minimum,
+                                      uint32_t transparent_minimum,
+                                      const float throughput[3]);
+
+/* --- light distribution (src/sampling/light_distribution.cpp) ----------- */
+/* entries: 4 words per entry (cumulative f32, selection_pdf f32, kind u32,
+ * index u32), emitter_count + 1 entries.  Returns emitter_count. */
+uint32_t oracle_light_distribution(const float *areas, uint32_t triangle_count,
+                                   uint32_t analytic_count,
+                                   int include_environment,
+                                   uint32_t *entries,
+                                   float *triangle_area_pdf,
+                                   float *light_selection_pdf);
+
+/* --- triangle intersection (cycles_triangle_intersection_component.cpp) -- */
+/* out: valid, distance, u, v.  world_to_object may be NULL (identity). */
+int oracle_pluecker_intersect(const float origin[3], const float direction[3],
+                              float tmin, float tmax, const float p0[3],
+                              const float p1[3], const float p2[3],
+                              float out[3]);
+
+/* --- render ------------------------------------------------------------- */
+typedef struct oracle_stats {
+    uint64_t rays;
+    uint64_t node_visits;      /* binary-SAH nodes visited, all rays       */
+    uint64_t triangle_tests;   /* triangles tested, all rays               */
+    uint64_t samples;
+} oracle_stats;
+
+typedef struct oracle_scene oracle_scene;
+
+/* Build the oracle's own scene tables (binary SAH BVH with 16 bins and
+ * leaves of at most 4 triangles, light distribution).  `brute_force`
+ * replaces the BVH with an O(T) loop over every triangle. */
+oracle_scene *oracle_scene_create(const psx_scene_desc *desc, int brute_force);
+void oracle_scene_destroy(oracle_scene *scene);
+int oracle_scene_update_parameters(oracle_scene *scene, uint32_t material,
+                                   const float *parameters, uint32_t count);
+uint32_t oracle_scene_emissive_triangles(const oracle_scene *scene);
+
+/* Accumulate samples [first, first + count) of a `total`-sample render into
+ * the window film (row-major, same layout as psx_session_read_film).  The
+ * film arrays must hold the previous sums (zero for a fresh render).
+ * `threads` <= 0 uses every online CPU.  Returns 0 on success. */
+int oracle_render(const oracle_scene *scene,
+                  const psx_render_settings *settings, uint32_t first,
+                  uint32_t count, uint32_t total, int threads,
+                  float *combined, float *normal, float *albedo,
+                  uint32_t *sample_count, oracle_stats *stats);
+
+/* Same as oracle_render restricted to window rows [row_begin, row_end);
+ * film pointers address the full window. */
+int oracle_render_rows(const oracle_scene *scene,
+                       const psx_render_settings *settings, uint32_t first,
+                       uint32_t count, uint32_t total, int threads,
+                       uint32_t row_begin, uint32_t row_end, float *combined,
+                       float *normal, float *albedo, uint32_t *sample_count,
+                       oracle_stats *stats);
+
+/* Normalise accumulated film into a pass, as write_passes does
+ * (src/luisa/path_tracer_session.cpp:71-254).  out holds `channels` floats
+ * per pixel. */
+void oracle_write_pass(uint32_t pass_kind, uint32_t channels,
+                       uint32_t pixel_count, float exposure,
+                       const float *combined, const float *normal,
+                       const float *albedo, const uint32_t *sample_count,
+                       float *out);
+
+/* Primary probe: per window pixel (object, primitive, t bits, hit). */
+int oracle_primary_hits(const oracle_scene *scene,
+                        const psx_render_settings *settings,
+                        uint32_t sample_index, uint32_t total,
+                        uint32_t *out_records);
+
+/* Path event log, PSX_TRACE_EVENTS x PSX_TRACE_WORDS u32 per pixel. */
+int oracle_trace_events(const oracle_scene *scene,
+                        const psx_render_settings *settings,
+                        uint32_t sample_index, uint32_t total,
+                        uint32_t *out_records);
+
+/* Sampler probe with the same layout as psx_session_sampler_probe. */
+int oracle_sampler_probe(const psx_render_settings *settings,
+                         uint32_t sample_index, uint32_t total,
+                         uint32_t first_dimension, uint32_t dimension_count,
+                         uint32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,24 @@
+import os
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+TESTS = Path(__file__).resolve().parent
+if str(TESTS) not in sys.path:
+    sys.path.insert(0, str(TESTS))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP render path)")
+    config.addinivalue_line("markers", "slow: long CPU oracle run")
+
+
+@pytest.fixture(scope="session")
+def golden():
+    import json
+    with open(TESTS / "golden" / "reference_vectors.json") as f:
+        return json.load(f)

@@ -1,0 +1,216 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is the checker of the HIP render path.  Only tests/, the smoke
+check and bench.py's CPU-baseline leg load this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from psycles_amd import abi
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+REF_LIB = ORACLE_DIR / "_ref" / "libpsycles_ref.so"
+
+P = C.POINTER
+u32 = C.c_uint32
+f32 = C.c_float
+
+
+class OracleStats(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("node_visits", C.c_uint64),
+                ("triangle_tests", C.c_uint64), ("samples", C.c_uint64)]
+
+
+class PathState(C.Structure):
+    _fields_ = [(n, u32) for n in ("flag", "visibility", "bounce", "diffuse_bounce",
+                                   "glossy_bounce", "transmission_bounce",
+                                   "transparent_bounce", "rng_offset")]
+
+
+class PathLimits(C.Structure):
+    _fields_ = [(n, u32) for n in ("maximum", "maximum_diffuse", "maximum_glossy",
+                                   "maximum_transmission", "maximum_transparent")]
+
+
+_lib = None
+
+
+def build_oracle() -> None:
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR), "oracle"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not ORACLE_LIB.exists():
+        build_oracle()
+    lib = C.CDLL(str(ORACLE_LIB))
+    vp = C.c_void_p
+    lib.oracle_sequence_size_for_samples.argtypes = [u32]
+    lib.oracle_sequence_size_for_samples.restype = u32
+    lib.oracle_sobol_table.argtypes = [u32, P(f32)]
+    lib.oracle_pixel_hash.argtypes = [u32, u32, u32]
+    lib.oracle_pixel_hash.restype = u32
+    lib.oracle_shuffled_sample_index.argtypes = [u32, u32, u32, u32]
+    lib.oracle_shuffled_sample_index.restype = u32
+    lib.oracle_pixel_filter_table.argtypes = [u32, f32, P(f32)]
+    lib.oracle_pixel_filter_sample.argtypes = [P(f32), f32]
+    lib.oracle_pixel_filter_sample.restype = f32
+    lib.oracle_sample_uniform_disk.argtypes = [P(f32), P(f32)]
+    lib.oracle_sample_cosine_hemisphere.argtypes = [P(f32), P(f32), P(f32)]
+    lib.oracle_camera_clip_range.argtypes = [f32, f32, f32, P(f32)]
+    lib.oracle_affine_point.argtypes = [P(f32), P(f32), P(f32)]
+    lib.oracle_initial_state.argtypes = [P(PathState)]
+    lib.oracle_next_surface.argtypes = [P(PathState), u32, u32, P(PathLimits), P(PathState)]
+    lib.oracle_contract_visibility.argtypes = [u32]
+    lib.oracle_contract_visibility.restype = u32
+    lib.oracle_continuation_probability.argtypes = [u32, u32, u32, u32, u32, P(f32)]
+    lib.oracle_continuation_probability.restype = f32
+    lib.oracle_light_distribution.argtypes = [P(f32), u32, u32, C.c_int, P(u32),
+                                              P(f32), P(f32)]
+    lib.oracle_light_distribution.restype = u32
+    lib.oracle_pluecker_intersect.argtypes = [P(f32), P(f32), f32, f32, P(f32), P(f32),
+                                              P(f32), P(f32)]
+    lib.oracle_pluecker_intersect.restype = C.c_int
+    lib.oracle_scene_create.argtypes = [P(abi.SceneDesc), C.c_int]
+    lib.oracle_scene_create.restype = vp
+    lib.oracle_scene_destroy.argtypes = [vp]
+    lib.oracle_scene_update_parameters.argtypes = [vp, u32, P(f32), u32]
+    lib.oracle_scene_emissive_triangles.argtypes = [vp]
+    lib.oracle_scene_emissive_triangles.restype = u32
+    lib.oracle_render.argtypes = [vp, P(abi.RenderSettings), u32, u32, u32, C.c_int,
+                                  P(f32), P(f32), P(f32), P(u32), P(OracleStats)]
+    lib.oracle_render_rows.argtypes = [vp, P(abi.RenderSettings), u32, u32, u32, C.c_int,
+                                       u32, u32, P(f32), P(f32), P(f32), P(u32),
+                                       P(OracleStats)]
+    lib.oracle_write_pass.argtypes = [u32, u32, u32, f32, P(f32), P(f32), P(f32), P(u32),
+                                      P(f32)]
+    lib.oracle_primary_hits.argtypes = [vp, P(abi.RenderSettings), u32, u32, P(u32)]
+    lib.oracle_trace_events.argtypes = [vp, P(abi.RenderSettings), u32, u32, P(u32)]
+    lib.oracle_sampler_probe.argtypes = [P(abi.RenderSettings), u32, u32, u32, u32, P(u32)]
+    _lib = lib
+    return lib
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(P(f32))
+
+
+def uptr(a: np.ndarray):
+    return a.ctypes.data_as(P(u32))
+
+
+def sobol_table(sequence_size: int) -> np.ndarray:
+    out = np.zeros((256 * sequence_size, 4), dtype=np.float32)
+    assert load().oracle_sobol_table(sequence_size, fptr(out)) == 0
+    return out
+
+
+def filter_table(kind: int, width: float) -> np.ndarray:
+    out = np.zeros(1024, dtype=np.float32)
+    load().oracle_pixel_filter_table(kind, width, fptr(out))
+    return out
+
+
+class OracleScene:
+    def __init__(self, snapshot, brute_force: bool = False):
+        self.snapshot = snapshot
+        self.desc, self._keep = snapshot.to_desc()
+        self.handle = load().oracle_scene_create(C.byref(self.desc), 1 if brute_force else 0)
+        assert self.handle
+
+    def close(self):
+        if self.handle:
+            load().oracle_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def update_parameters(self, material: int, params: np.ndarray) -> int:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        return load().oracle_scene_update_parameters(self.handle, material, fptr(p), p.size)
+
+    def render(self, settings, first, count, total, threads=0, film=None, rows=None):
+        w = settings.window_width or settings.width
+        h = settings.window_height or settings.height
+        if film is None:
+            film = Film(w, h)
+        stats = OracleStats()
+        if rows is None:
+            rc = load().oracle_render(self.handle, C.byref(settings), first, count, total,
+                                      threads, fptr(film.combined), fptr(film.normal),
+                                      fptr(film.albedo), uptr(film.sample_count),
+                                      C.byref(stats))
+        else:
+            rc = load().oracle_render_rows(self.handle, C.byref(settings), first, count,
+                                           total, threads, rows[0], rows[1],
+                                           fptr(film.combined), fptr(film.normal),
+                                           fptr(film.albedo), uptr(film.sample_count),
+                                           C.byref(stats))
+        assert rc == 0
+        return film, stats
+
+    def primary_hits(self, settings, sample_index, total):
+        w = settings.window_width or settings.width
+        h = settings.window_height or settings.height
+        out = np.zeros((h, w, 4), dtype=np.uint32)
+        load().oracle_primary_hits(self.handle, C.byref(settings), sample_index, total,
+                                   uptr(out))
+        return out
+
+    def trace_events(self, settings, sample_index, total):
+        w = settings.window_width or settings.width
+        h = settings.window_height or settings.height
+        out = np.zeros((h, w, abi.TRACE_EVENTS, abi.TRACE_WORDS), dtype=np.uint32)
+        load().oracle_trace_events(self.handle, C.byref(settings), sample_index, total,
+                                   uptr(out))
+        return out
+
+
+class Film:
+    def __init__(self, w, h):
+        self.width = w
+        self.height = h
+        self.combined = np.zeros((h, w, 4), dtype=np.float32)
+        self.normal = np.zeros((h, w, 4), dtype=np.float32)
+        self.albedo = np.zeros((h, w, 4), dtype=np.float32)
+        self.sample_count = np.zeros((h, w), dtype=np.uint32)
+
+    def write_pass(self, kind: int, channels: int, exposure: float = 1.0) -> np.ndarray:
+        out = np.zeros((self.height, self.width, channels), dtype=np.float32)
+        load().oracle_write_pass(kind, channels, self.width * self.height, exposure,
+                                 fptr(self.combined), fptr(self.normal), fptr(self.albedo),
+                                 uptr(self.sample_count), fptr(out))
+        return out
+
+
+def sampler_probe(settings, sample_index, total, first_dim, dim_count) -> np.ndarray:
+    w = settings.window_width or settings.width
+    h = settings.window_height or settings.height
+    out = np.zeros((h, w, 1 + 4 * dim_count), dtype=np.uint32)
+    load().oracle_sampler_probe(C.byref(settings), sample_index, total, first_dim, dim_count,
+                                uptr(out))
+    return out
+
+
+def ref_library():
+    """The reference's own sampling sources compiled in place (may be absent)."""
+    if not REF_LIB.exists():
+        return None
+    lib = C.CDLL(str(REF_LIB))
+    lib.ref_sequence_size_for_samples.argtypes = [u32]
+    lib.ref_sequence_size_for_samples.restype = u32
+    lib.ref_sobol_table.argtypes = [u32, P(f32)]
+    lib.ref_pixel_filter_table.argtypes = [u32, f32, P(f32)]
+    lib.ref_light_distribution.argtypes = [P(f32), u32, u32, C.c_int, P(u32), P(f32), P(f32)]
+    lib.ref_light_distribution.restype = u32
+    return lib
