@@ -1,0 +1,70 @@
+"""Build the in-tree render library psycles_amd/libpsx.so.
+
+HIP kernels are compiled for gfx950 with hipcc; host C++ with g++ against the
+HIP runtime headers.  Floating-point flags are part of the render contract:
+no contraction, correctly rounded division and square root, no fast math,
+and packed FP32 instructions disabled.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+BUILD = PKG.parent / "_build"
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+ARCH = os.environ.get("PSX_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-ffp-contract=off", "-fno-fast-math", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+HOST_FLAGS = [
+    "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+    f"-I{ROCM / 'include'}", "-D__HIP_PLATFORM_AMD__", "-Wall",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(map(str, cmd)) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"compile failed: {cmd[-1]}")
+    return r
+
+
+def build_library(verbose: bool = False) -> Path:
+    """Compile every source and link libpsx.so; returns its path."""
+    hipcc = shutil.which("hipcc") or str(ROCM / "bin" / "hipcc")
+    BUILD.mkdir(exist_ok=True)
+    jobs = []
+    objects = []
+    for src in sorted(CSRC.glob("*.hip")):
+        obj = BUILD / (src.stem + ".hip.o")
+        jobs.append([hipcc, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
+        objects.append(obj)
+    for src in sorted(CSRC.glob("*.cpp")):
+        obj = BUILD / (src.stem + ".o")
+        jobs.append(["g++", *HOST_FLAGS, "-c", str(src), "-o", str(obj)])
+        objects.append(obj)
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for r in ex.map(_run, jobs):
+            if verbose and (r.stdout or r.stderr):
+                sys.stderr.write(r.stdout + r.stderr)
+    out = PKG / "libpsx.so"
+    tmp = PKG / "libpsx.so.tmp"
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
+          *map(str, objects), f"-L{ROCM / 'lib'}", "-lamdhip64"])
+    os.replace(tmp, out)
+    return out
+
+
+if __name__ == "__main__":
+    print(build_library(verbose=True))

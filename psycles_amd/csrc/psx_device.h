@@ -1,0 +1,185 @@
+// psx_device.h - device record layouts of the MI355X render path.
+//
+// Shared by the host scene compiler (psx_host.cpp) and the HIP kernels
+// (psx_kernels.hip).  Every record is sized for coalesced HBM fetches:
+// BVH4 nodes are one 128-byte line, leaf triangles are three float4 (48 B).
+#pragma once
+
+#include <cstdint>
+
+namespace psx {
+
+// ---------------------------------------------------------------------------
+// acceleration structure
+// ---------------------------------------------------------------------------
+
+// 4-wide BVH node, one 128-byte line.  Child boxes are stored structure-of-
+// arrays so one node fetch feeds four slab tests.  count[i]:
+//   0            inner child, child[i] = node index
+//   1..         leaf with count[i] items starting at child[i]
+//   kEmptySlot   unused slot
+struct alignas(16) Node4 {
+    float lo_x[4], hi_x[4];
+    float lo_y[4], hi_y[4];
+    float lo_z[4], hi_z[4];
+    uint32_t child[4];
+    uint32_t count[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
+constexpr uint32_t kEmptySlot = 0xffffffffu;
+
+// Leaf triangle in intersection space (object space for shared meshes, final
+// world space for single-user meshes).  p0.w carries the local primitive
+// index as raw bits.
+struct alignas(16) LeafTri {
+    float p0x, p0y, p0z, prim_bits;
+    float p1x, p1y, p1z, pad1;
+    float p2x, p2y, p2z, pad2;
+};
+static_assert(sizeof(LeafTri) == 48, "LeafTri must be 48 bytes");
+
+// Affine transform stored column-major without the constant last row:
+// m[0..2] column 0, m[3..5] column 1, m[6..8] column 2, m[9..11] translation.
+struct Affine {
+    float m[12];
+};
+
+struct alignas(16) Instance {
+    Affine w2o;            // cycles_inverse_transform(transform)
+    Affine o2w;            // transform
+    Affine n2w;            // transpose(w2o) (translation column unused)
+    uint32_t blas_root;    // node index of the BLAS root in the node array
+    uint32_t mesh;
+    uint32_t visibility;
+    uint32_t applied;      // 1: positions are final world support positions
+    uint32_t prim_offset;  // Cycles primitive offset of the mesh
+    uint32_t ipos_offset;  // first intersection-space vertex (float3 index)
+    uint32_t override_offset;
+    uint32_t override_count;
+};
+
+struct Mesh {
+    uint32_t vertex_offset;    // object-space positions (float3 index)
+    uint32_t index_offset;     // first triangle (uint3 index)
+    uint32_t normal_offset;    // float3 index or kNone
+    uint32_t smooth_offset;    // byte index or kNone
+    uint32_t slot_offset;      // material slot table
+    uint32_t slot_count;
+    uint32_t tri_slot_offset;  // per-triangle slot or kNone
+    uint32_t triangle_count;
+};
+constexpr uint32_t kNone = 0xffffffffu;
+
+// entries of the per-lane LDS traversal stack; scene compilation rejects trees
+// whose worst-case stack use exceeds it.
+constexpr uint32_t kTraversalStackDepth = 64u;
+
+// ---------------------------------------------------------------------------
+// materials: a SurfaceProgram lowered to a flat leaf list.  A leaf's mix
+// weight is the product, in root-to-leaf order, of one factor per enclosing
+// mix: (1 - f) for the first operand, f for the second.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMaxLeafDepth = 6;
+constexpr uint32_t kMaxLeaves = 8;
+
+enum LeafOp : uint32_t { kLeafDiffuse = 3u, kLeafEmission = 4u, kLeafTransparent = 5u };
+
+struct Leaf {
+    uint32_t op;
+    uint32_t color;           // parameter index of the rgb triple
+    uint32_t aux;             // strength (emission) or roughness (diffuse)
+    uint32_t depth;           // number of enclosing mixes
+    uint32_t factor[kMaxLeafDepth];
+    uint32_t second_mask;     // bit d set: second operand of mix d
+    uint32_t pad;
+};
+
+struct Material {
+    uint32_t leaf_offset;
+    uint32_t leaf_count;
+    uint32_t param_offset;
+    uint32_t emission_sampling;
+    uint32_t may_emit;
+    uint32_t pad[3];
+};
+
+// ---------------------------------------------------------------------------
+// lights
+// ---------------------------------------------------------------------------
+struct Emitter {
+    uint32_t instance;
+    uint32_t prim;
+    uint32_t material;
+    uint32_t emission_sampling;
+};
+
+struct DistEntry {
+    float cumulative;
+    float selection_pdf;
+    uint32_t kind;   // 0 triangle, 1 analytic, 2 environment, 3 sentinel
+    uint32_t index;
+};
+
+// ---------------------------------------------------------------------------
+// scene view passed to kernels (device pointers)
+// ---------------------------------------------------------------------------
+struct SceneView {
+    const Node4 *nodes;
+    const LeafTri *tris;
+    const uint32_t *tlas_items;     // TLAS leaf item -> instance
+    const Instance *instances;
+    const Mesh *meshes;
+    const float *positions;         // float3 array (object + applied world)
+    const uint32_t *indices;        // uint3 array
+    const float *normals;           // float3 array
+    const uint8_t *smooth;
+    const uint32_t *slots;          // mesh slot -> material
+    const uint32_t *tri_slots;
+    const uint32_t *overrides;
+    const Material *materials;
+    const Leaf *leaves;
+    const float *params;
+    const Emitter *emitters;
+    const DistEntry *distribution;
+    uint32_t tlas_root;
+    uint32_t instance_count;
+    uint32_t material_count;
+    uint32_t emitter_count;
+    uint32_t distribution_count;
+    float triangle_area_pdf;
+    float background[3];
+    uint32_t world_visibility;
+    uint32_t has_tlas;
+};
+
+// ---------------------------------------------------------------------------
+// per-render kernel parameters (RenderKernelParameters,
+// src/luisa/path_tracer_types.h:409-448, restricted)
+// ---------------------------------------------------------------------------
+struct KernelParams {
+    uint32_t window_x, window_y, window_width, window_height;
+    uint32_t full_width, full_height, seed, sequence_size;
+    uint32_t max_bounces, min_bounces, max_diffuse, max_glossy;
+    uint32_t max_transmission, transparent_min, transparent_max, max_path_steps;
+    uint32_t transparent_background, nee, mis_mode, pad0;
+    float clamp_direct, clamp_indirect, light_inv_rr_threshold, pass_alpha_threshold;
+    float tan_h, tan_v, shift_x, shift_y;
+    float near_clip, far_clip, pad1, pad2;
+    float camera[16];
+};
+
+struct FilmView {
+    float *combined;   // float4 per pixel
+    float *normal;     // float4 per pixel
+    float *albedo;     // float4 per pixel
+    uint32_t *sample_count;
+};
+
+struct Counters {
+    unsigned long long rays;
+    unsigned long long nodes;
+    unsigned long long tris;
+    unsigned long long pad;
+};
+
+}  // namespace psx

@@ -1,0 +1,1264 @@
+// psx_host.cpp - host half of the MI355X render path and its C ABI.
+//
+// Implements include/psx.h: scene compilation (the flat SceneSnapshot is
+// lowered to device tables and a two-level BVH4), render sessions with film
+// buffers resident in HBM, the ordered sample-batch dispatch loop and the
+// write_passes normalisation.  Reference counterparts are cited per function.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/psx.h"
+#include "psx_bvh.h"
+#include "psx_device.h"
+
+namespace psx {
+
+hipError_t launch_render(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
+                         const FilmView &film, uint32_t first, uint32_t count, Counters *counters, bool count_nodes,
+                         hipStream_t stream);
+hipError_t launch_trace(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
+                        uint32_t sample, uint32_t *records, hipStream_t stream);
+hipError_t launch_primary(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
+                          uint32_t sample, uint32_t *records, hipStream_t stream);
+hipError_t launch_sampler(const KernelParams &k, const float4 *sobol, uint32_t sample, uint32_t first_dim,
+                          uint32_t dim_count, uint32_t *out, hipStream_t stream);
+
+namespace {
+
+struct DeviceError {
+    int status;
+};
+
+void hip_check(hipError_t e) {
+    if (e == hipErrorOutOfMemory) throw DeviceError{PSX_ERROR_OUT_OF_MEMORY};
+    if (e != hipSuccess) throw DeviceError{PSX_ERROR_DEVICE};
+}
+
+// RAII device allocation
+class DeviceBuffer {
+  public:
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { reset(); }
+    void reset() {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        bytes_ = 0;
+    }
+    void allocate(size_t bytes) {
+        reset();
+        if (bytes == 0) bytes = 16;
+        hip_check(hipMalloc(&ptr_, bytes));
+        bytes_ = bytes;
+    }
+    template <typename T>
+    void upload(const std::vector<T> &v) {
+        allocate(v.size() * sizeof(T));
+        if (!v.empty()) hip_check(hipMemcpy(ptr_, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(ptr_);
+    }
+    size_t bytes() const { return bytes_; }
+
+  private:
+    void *ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// host math shared by scene compilation (same contracts as the kernels)
+// ---------------------------------------------------------------------------
+struct Vec3 {
+    float x, y, z;
+};
+inline Vec3 v3(float x, float y, float z) { return Vec3{x, y, z}; }
+inline Vec3 sub(Vec3 a, Vec3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline Vec3 cross(Vec3 a, Vec3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline float dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec3 divs(Vec3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+
+// transform_point (src/luisa/path_tracer_scene_geometry.cpp:93-113)
+inline Vec3 transform_point(const float *e, Vec3 p) {
+    return v3(std::fma(p.x, e[0], std::fma(p.y, e[4], std::fma(p.z, e[8], e[12]))),
+              std::fma(p.x, e[1], std::fma(p.y, e[5], std::fma(p.z, e[9], e[13]))),
+              std::fma(p.x, e[2], std::fma(p.y, e[6], std::fma(p.z, e[10], e[14]))));
+}
+
+// cycles_inverse_transform (src/luisa/path_tracer_scene_geometry.cpp:231-268)
+void cycles_inverse(const float *e, float *r) {
+    Vec3 x = v3(e[0], e[1], e[2]), y = v3(e[4], e[5], e[6]), z = v3(e[8], e[9], e[10]);
+    const Vec3 t = v3(e[12], e[13], e[14]);
+    float det = dot(x, cross(y, z));
+    if (det == 0.0f) {
+        x.x += 1.0e-8f;
+        y.y += 1.0e-8f;
+        z.z += 1.0e-8f;
+        det = dot(x, cross(y, z));
+        if (det == 0.0f) det = std::numeric_limits<float>::max();
+    }
+    const Vec3 ix = divs(cross(y, z), det), iy = divs(cross(z, x), det), iz = divs(cross(x, y), det);
+    const float vals[16] = {ix.x, iy.x, iz.x, 0.0f, ix.y, iy.y, iz.y, 0.0f, ix.z, iy.z, iz.z, 0.0f,
+                            -dot(ix, t), -dot(iy, t), -dot(iz, t), 1.0f};
+    std::memcpy(r, vals, sizeof(vals));
+}
+
+Affine to_affine(const float *e) {
+    Affine a;
+    const int src[12] = {0, 1, 2, 4, 5, 6, 8, 9, 10, 12, 13, 14};
+    for (int i = 0; i < 12; ++i) a.m[i] = e[src[i]];
+    return a;
+}
+
+void transpose4(const float *a, float *r) {
+    for (int c = 0; c < 4; ++c)
+        for (int row = 0; row < 4; ++row) r[c * 4 + row] = a[row * 4 + c];
+}
+
+// ---------------------------------------------------------------------------
+// tabulated Sobol LUT (src/sampling/tabulated_sobol.cpp:24-199)
+// ---------------------------------------------------------------------------
+const uint32_t kSobolXors[4][32] = {
+    {0u},
+    {0x00000000u, 0x00000001u, 0x00000001u, 0x00000007u, 0x00000001u, 0x00000013u, 0x00000015u, 0x0000007fu,
+     0x00000001u, 0x00000103u, 0x00000105u, 0x0000070fu, 0x00000111u, 0x00001333u, 0x00001555u, 0x00007fffu,
+     0x00000001u, 0x00010003u, 0x00010005u, 0x0007000fu, 0x00010011u, 0x00130033u, 0x00150055u, 0x007f00ffu,
+     0x00010101u, 0x01030303u, 0x01050505u, 0x070f0f0fu, 0x01111111u, 0x13333333u, 0x15555555u, 0x7fffffffu},
+    {0x00000000u, 0x00000001u, 0x00000003u, 0x00000001u, 0x00000005u, 0x0000001fu, 0x0000002bu, 0x0000003du,
+     0x00000011u, 0x00000133u, 0x00000377u, 0x00000199u, 0x00000445u, 0x00001ccfu, 0x00002ddbu, 0x0000366du,
+     0x00000101u, 0x00010303u, 0x00030707u, 0x00010909u, 0x00051515u, 0x001f3f3fu, 0x002b6b6bu, 0x003dbdbdu,
+     0x00101011u, 0x01303033u, 0x03707077u, 0x01909099u, 0x04515145u, 0x1cf3f3cfu, 0x2db6b6dbu, 0x36dbdb6du},
+    {0x00000000u, 0x00000001u, 0x00000000u, 0x00000003u, 0x0000000du, 0x0000000cu, 0x00000005u, 0x0000004fu,
+     0x00000014u, 0x000000e7u, 0x00000329u, 0x0000039cu, 0x00000011u, 0x00001033u, 0x00000044u, 0x000030bbu,
+     0x0000d1cdu, 0x0000c2ecu, 0x00005415u, 0x0004fc3fu, 0x00015054u, 0x000e5c97u, 0x0032e5b9u, 0x0039725cu,
+     0x00000101u, 0x01000303u, 0x00000404u, 0x03000b0bu, 0x0d001d1du, 0x0c002c2cu, 0x05004545u, 0x4f00cfcfu}};
+
+uint32_t hash_hp(uint32_t v) {
+    v ^= v >> 16u;
+    v *= 0x21f0aaadu;
+    v ^= v >> 15u;
+    v *= 0xd35a2d97u;
+    v ^= v >> 15u;
+    return v ^ 0xe6fe3bebu;
+}
+
+float hash_hp_float(uint32_t v) { return static_cast<float>(hash_hp(v)) * (1.0f / 4294967808.0f); }
+
+uint32_t sequence_size_for(uint32_t samples) {
+    const uint32_t c = std::min(std::max(samples, 1u), 8192u);
+    uint32_t u = 1u;
+    if (c != 1u) u = 1u << (32u - static_cast<uint32_t>(__builtin_clz(c - 1u)));
+    return std::min(std::max(u, 256u), 8192u);
+}
+
+std::vector<float> sobol_table(uint32_t size) {
+    std::vector<float> t(static_cast<size_t>(size) * 256u * 4u);
+    for (uint32_t pattern = 0; pattern < 256u; ++pattern) {
+        float *p = t.data() + static_cast<size_t>(pattern) * size * 4u;
+        // hash_hp_seeded_uint(pattern, 0x44605a73): the seed is pre-mixed
+        uint32_t seed = 0x44605a73u;
+        seed ^= seed << 19u;
+        uint32_t rng = hash_hp(pattern ^ seed);
+        for (int c = 0; c < 4; ++c) p[c] = hash_hp_float(rng++);
+        uint32_t log_n = 0;
+        for (uint32_t n = 1; n < size; ++log_n, n *= 2u) {
+            const float strata = static_cast<float>(n * 2u);
+            for (uint32_t i = 0; i < n && n + i < size; ++i) {
+                uint32_t occ[4];
+                for (int c = 0; c < 4; ++c)
+                    occ[c] = static_cast<uint32_t>(p[4u * (i ^ kSobolXors[c][log_n]) + c] * strata);
+                for (int c = 0; c < 4; ++c)
+                    p[4u * (n + i) + c] = (static_cast<float>(occ[c] ^ 1u) + hash_hp_float(rng++)) / strata;
+            }
+        }
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// pixel filter LUT (src/sampling/pixel_filter.cpp:30-86)
+// ---------------------------------------------------------------------------
+std::vector<float> pixel_filter_table(uint32_t filter, float width) {
+    constexpr size_t size = 1024;
+    width = std::max(width, 1.0e-5f);
+    if (filter == PSX_FILTER_GAUSSIAN) width *= 3.0f;
+    if (filter == PSX_FILTER_BLACKMAN_HARRIS) width *= 2.0f;
+    auto eval = [&](float value) -> float {
+        if (filter == PSX_FILTER_GAUSSIAN) {
+            value *= 6.0f / width;
+            return std::exp(-2.0f * value * value);
+        }
+        if (filter == PSX_FILTER_BLACKMAN_HARRIS) {
+            constexpr float two_pi = 6.28318530717958647692f;
+            value = two_pi * (value / width + 0.5f);
+            return 0.35875f - 0.48829f * std::cos(value) + 0.14128f * std::cos(2.0f * value) -
+                   0.01168f * std::cos(3.0f * value);
+        }
+        return 1.0f;
+    };
+    const size_t resolution = size - 1u;
+    std::vector<float> cdf(size, 0.0f);
+    const float range = width * 0.5f;
+    for (size_t i = 0; i < resolution; ++i) {
+        const float value = range * static_cast<float>(i) / static_cast<float>(resolution - 1u);
+        cdf[i + 1u] = cdf[i] + std::fabs(eval(value));
+    }
+    const float norm = cdf[resolution] == 0.0f ? 0.0f : 1.0f / cdf[resolution];
+    for (auto &v : cdf) v *= norm;
+    cdf[resolution] = 1.0f;
+    std::vector<float> inv(size, 0.0f);
+    const size_t half = (size - 1u) / 2u;
+    for (size_t i = 0; i <= half; ++i) {
+        const float x = static_cast<float>(i) / static_cast<float>(half);
+        size_t index = static_cast<size_t>(std::upper_bound(cdf.begin(), cdf.end(), x) - cdf.begin());
+        float t = 0.0f;
+        if (index < resolution) {
+            t = (x - cdf[index]) / (cdf[index + 1u] - cdf[index]);
+        } else {
+            index = resolution;
+        }
+        const float y = (static_cast<float>(index) + t) / static_cast<float>(size - 1u) * (2.0f * range);
+        inv[half + i] = 0.5f * (1.0f + y);
+        inv[half - i] = 0.5f * (1.0f - y);
+    }
+    return inv;
+}
+
+// ---------------------------------------------------------------------------
+// light distribution (src/sampling/light_distribution.cpp:25-138)
+// ---------------------------------------------------------------------------
+uint32_t build_distribution(const std::vector<float> &areas, std::vector<DistEntry> &e, float &tri_pdf) {
+    auto valid = [](float a) { return std::isfinite(a) && a > 0.0f ? a : 0.0f; };
+    const uint32_t n = static_cast<uint32_t>(areas.size());
+    e.assign(n + 1u, DistEntry{});
+    float total = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        e[i] = DistEntry{total, 0.0f, 0u, i};
+        total += valid(areas[i]);
+    }
+    e[n] = DistEntry{total, 0.0f, 3u, 0u};
+    tri_pdf = total > 0.0f ? 1.0f / total : 0.0f;
+    for (uint32_t i = 0; i < n; ++i) e[i].selection_pdf = valid(areas[i]) * tri_pdf;
+    if (total > 0.0f) {
+        const float inv = 1.0f / total;
+        for (uint32_t i = 0; i < n; ++i) e[i].cumulative *= inv;
+        e[n].cumulative = 1.0f;
+    }
+    return total > 0.0f ? n : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// materials: SurfaceProgram -> flat leaf list
+// (compile_surface_program / bind_surface_parameters,
+//  include/psycles/compiler/surface_program.h:635-647)
+// ---------------------------------------------------------------------------
+struct HostMaterial {
+    std::vector<psx_closure_instr> instr;
+    uint32_t root = 0;
+    std::vector<float> params;
+    uint32_t emission_sampling = 0;
+    bool may_emit = false;
+    uint64_t signature = 0;
+};
+
+bool flatten_program(const HostMaterial &m, uint32_t id, Leaf path, std::vector<Leaf> &out, int guard,
+                     std::string &error) {
+    if (id >= m.instr.size() || guard > 64) {
+        error = "closure instruction index out of range";
+        return false;
+    }
+    const psx_closure_instr &ins = m.instr[id];
+    auto param_ok = [&](uint32_t p, uint32_t n) { return p + n <= m.params.size(); };
+    switch (ins.op) {
+    case PSX_CLOSURE_NULL:
+        return true;
+    case PSX_CLOSURE_ADD:
+        return flatten_program(m, ins.a, path, out, guard + 1, error) &&
+               flatten_program(m, ins.b, path, out, guard + 1, error);
+    case PSX_CLOSURE_MIX: {
+        if (path.depth >= kMaxLeafDepth) {
+            error = "mix nesting deeper than supported";
+            return false;
+        }
+        if (!param_ok(ins.factor, 1)) {
+            error = "mix factor parameter out of range";
+            return false;
+        }
+        Leaf a = path, b = path;
+        a.factor[a.depth] = ins.factor;
+        a.depth += 1;
+        b.factor[b.depth] = ins.factor;
+        b.second_mask |= 1u << b.depth;
+        b.depth += 1;
+        return flatten_program(m, ins.a, a, out, guard + 1, error) &&
+               flatten_program(m, ins.b, b, out, guard + 1, error);
+    }
+    case PSX_CLOSURE_DIFFUSE:
+    case PSX_CLOSURE_EMISSION:
+    case PSX_CLOSURE_TRANSPARENT: {
+        Leaf l = path;
+        l.op = ins.op;
+        l.color = ins.color;
+        l.aux = ins.op == PSX_CLOSURE_EMISSION ? ins.strength : (ins.op == PSX_CLOSURE_DIFFUSE ? ins.roughness : 0u);
+        if (!param_ok(l.color, 3) || !param_ok(l.aux, 1)) {
+            error = "closure parameter out of range";
+            return false;
+        }
+        out.push_back(l);
+        return true;
+    }
+    default:
+        error = "unsupported closure operation";
+        return false;
+    }
+}
+
+// estimate_surface_emission (src/compiler/surface_program.cpp:244-330)
+void emission_estimate(const HostMaterial &m, uint32_t id, float out[3], int guard) {
+    out[0] = out[1] = out[2] = 0.0f;
+    if (id >= m.instr.size() || guard > 64) return;
+    const psx_closure_instr &ins = m.instr[id];
+    if (ins.op == PSX_CLOSURE_EMISSION) {
+        for (int c = 0; c < 3; ++c) out[c] = m.params[ins.color + c] * m.params[ins.strength];
+    } else if (ins.op == PSX_CLOSURE_ADD || ins.op == PSX_CLOSURE_MIX) {
+        float a[3], b[3];
+        emission_estimate(m, ins.a, a, guard + 1);
+        emission_estimate(m, ins.b, b, guard + 1);
+        const float f = ins.op == PSX_CLOSURE_MIX ? m.params[ins.factor] : 0.0f;
+        for (int c = 0; c < 3; ++c)
+            out[c] = ins.op == PSX_CLOSURE_MIX ? a[c] * (1.0f - f) + b[c] * f : a[c] + b[c];
+    }
+}
+
+void refresh_may_emit(HostMaterial &m) {
+    float e[3] = {0, 0, 0};
+    if (!m.instr.empty()) emission_estimate(m, m.root, e, 0);
+    m.may_emit = (e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f) && m.emission_sampling != PSX_EMISSION_SAMPLING_NONE;
+}
+
+uint64_t structure_signature(const HostMaterial &m) {
+    uint64_t h = 14695981039346656037ull;
+    auto mix = [&](uint32_t v) {
+        for (int b = 0; b < 4; ++b) {
+            h ^= (v >> (8 * b)) & 0xffu;
+            h *= 1099511628211ull;
+        }
+    };
+    mix(m.root);
+    mix(static_cast<uint32_t>(m.params.size()));
+    for (const auto &i : m.instr) {
+        mix(i.op);
+        mix(i.a);
+        mix(i.b);
+        mix(i.factor);
+        mix(i.color);
+        mix(i.strength);
+        mix(i.roughness);
+    }
+    return h;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct Backend {
+    int device = 0;
+    psx_backend_options options{};
+};
+
+struct Scene {
+    Backend *backend = nullptr;
+    // host copies needed after compile
+    std::vector<HostMaterial> materials;
+    std::vector<Material> gpu_materials;
+    std::vector<psx_instance> instances;
+    std::vector<std::vector<uint32_t>> overrides;
+    std::vector<Mesh> meshes;
+    std::vector<std::vector<float>> mesh_positions;
+    std::vector<std::vector<uint32_t>> mesh_tris;
+    std::vector<std::vector<uint32_t>> mesh_tri_slots;
+    std::vector<std::vector<uint32_t>> mesh_slots;
+    psx_camera camera{};
+    uint32_t max_stack = 0;
+    uint32_t tlas_depth = 0, blas_depth = 0;
+    // device buffers
+    DeviceBuffer nodes, tris, tlas_items, instances_buf, meshes_buf, positions, indices, normals, smooth, slots,
+        tri_slots, overrides_buf, materials_buf, leaves, params, emitters, distribution;
+    SceneView view{};
+    uint64_t triangles = 0;
+
+    void rebuild_emitters();
+};
+
+// emissive triangles (src/luisa/path_tracer_scene.cpp:1548-1640)
+void Scene::rebuild_emitters() {
+    std::vector<Emitter> em;
+    std::vector<float> areas;
+    for (uint32_t i = 0; i < instances.size(); ++i) {
+        const psx_instance &inst = instances[i];
+        const bool light_visible =
+            inst.visibility_mask == 0xffffffffu ||
+            (inst.visibility_mask & (PSX_VISIBILITY_DIFFUSE | PSX_VISIBILITY_GLOSSY | PSX_VISIBILITY_TRANSMISSION |
+                                     PSX_VISIBILITY_VOLUME_SCATTER)) != 0u;
+        if (!light_visible) continue;
+        const uint32_t mi = inst.mesh;
+        const Mesh &mesh = meshes[mi];
+        const auto &pos = mesh_positions[mi];
+        const auto &tri = mesh_tris[mi];
+        for (uint32_t p = 0; p < mesh.triangle_count; ++p) {
+            const uint32_t slot = mesh_tri_slots[mi].empty() ? 0u : mesh_tri_slots[mi][p];
+            uint32_t mat = kNone;
+            if (slot < overrides[i].size())
+                mat = overrides[i][slot];
+            else if (!mesh_slots[mi].empty())
+                mat = mesh_slots[mi][std::min<size_t>(slot, mesh_slots[mi].size() - 1u)];
+            if (mat >= materials.size() || !materials[mat].may_emit) continue;
+            Vec3 q[3];
+            for (int c = 0; c < 3; ++c) {
+                const uint32_t vi = tri[3u * p + c];
+                q[c] = transform_point(inst.transform, v3(pos[3u * vi], pos[3u * vi + 1], pos[3u * vi + 2]));
+            }
+            const Vec3 n = cross(sub(q[1], q[0]), sub(q[2], q[0]));
+            areas.push_back(0.5f * std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z));
+            em.push_back(Emitter{i, p, mat, materials[mat].emission_sampling});
+        }
+    }
+    std::vector<DistEntry> dist;
+    float tri_pdf = 0.0f;
+    const uint32_t count = build_distribution(areas, dist, tri_pdf);
+    emitters.upload(em);
+    distribution.upload(dist);
+    view.emitters = emitters.as<Emitter>();
+    view.distribution = distribution.as<DistEntry>();
+    view.emitter_count = static_cast<uint32_t>(em.size());
+    view.distribution_count = count;
+    view.triangle_area_pdf = tri_pdf;
+}
+
+struct Session {
+    Scene *scene = nullptr;
+    psx_render_settings settings{};
+    std::vector<uint32_t> passes, pass_channels;
+    KernelParams params{};
+    uint32_t window_w = 0, window_h = 0;
+    DeviceBuffer combined, normal, albedo, sample_count, sobol, filter, counters;
+    uint32_t sobol_size = 0;
+    uint32_t total_aa = 0;
+    uint32_t rendered = 0;
+    std::atomic<bool> cancelled{false};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    psx_render_stats stats{};
+    bool count_nodes = false;
+
+    ~Session() {
+        if (ev_begin) (void)hipEventDestroy(ev_begin);
+        if (ev_end) (void)hipEventDestroy(ev_end);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// LuisaRenderSession::initialize (src/luisa/path_tracer_kernel.cpp:44-439)
+void initialize_session(Session &s, const psx_render_settings &rs) {
+    s.settings = rs;
+    s.passes.assign(rs.passes, rs.passes + rs.pass_count);
+    if (rs.pass_channels)
+        s.pass_channels.assign(rs.pass_channels, rs.pass_channels + rs.pass_count);
+    else
+        s.pass_channels.assign(rs.pass_count, 4u);
+    s.settings.passes = s.passes.data();
+    s.settings.pass_channels = s.pass_channels.data();
+    KernelParams &k = s.params;
+    k = KernelParams{};
+    k.window_x = rs.window_x;
+    k.window_y = rs.window_y;
+    k.window_width = rs.window_width ? rs.window_width : rs.width;
+    k.window_height = rs.window_height ? rs.window_height : rs.height;
+    k.full_width = std::max(rs.width, 1u);
+    k.full_height = std::max(rs.height, 1u);
+    k.seed = rs.seed;
+    const psx_integrator_settings &in = rs.integrator;
+    auto synced = [](uint32_t v) { return v == 0xffffffffu ? v : v + 1u; };
+    k.max_bounces = synced(in.max_bounces);
+    k.min_bounces = synced(in.min_bounces);
+    k.max_diffuse = synced(in.diffuse_bounces);
+    k.max_glossy = synced(in.glossy_bounces);
+    k.max_transmission = synced(in.transmission_bounces);
+    k.transparent_min = synced(in.transparent_min_bounces);
+    k.transparent_max = in.transparent_max_bounces;
+    const uint64_t req = 2ull * k.max_bounces + std::max(in.transparent_max_bounces, 1u) + 1ull;
+    k.max_path_steps = static_cast<uint32_t>(std::min<uint64_t>(req, 1024ull));
+    k.transparent_background = rs.transparent_background;
+    k.clamp_direct = in.sample_clamp_direct > 0.0f ? in.sample_clamp_direct * 3.0f : 0.0f;
+    k.clamp_indirect = in.sample_clamp_indirect > 0.0f ? in.sample_clamp_indirect * 3.0f : 0.0f;
+    k.light_inv_rr_threshold =
+        (!in.use_light_tree && in.light_sampling_threshold > 0.0f) ? in.film_exposure / in.light_sampling_threshold
+                                                                   : 0.0f;
+    const bool available = s.scene->view.distribution_count > 0u;
+    const bool nee = s.scene->backend->options.next_event_estimation != 0u &&
+                     in.direct_light_sampling != PSX_DIRECT_LIGHT_FORWARD && available;
+    k.nee = nee ? 1u : 0u;
+    k.mis_mode = nee ? in.direct_light_sampling : PSX_DIRECT_LIGHT_FORWARD;
+    const psx_camera &cam = s.scene->camera;
+    const float aspect = static_cast<float>(k.full_width) / static_cast<float>(k.full_height);
+    const bool hfit = cam.sensor_fit == 0u || (cam.sensor_fit == 2u && aspect >= 1.0f);
+    k.tan_v = hfit ? std::tan(cam.horizontal_field_of_view * 0.5f) / aspect : std::tan(cam.field_of_view * 0.5f);
+    k.tan_h = k.tan_v * aspect;
+    k.shift_x = cam.lens_shift_x * (hfit ? 1.0f : 1.0f / aspect);
+    k.shift_y = cam.lens_shift_y * (hfit ? aspect : 1.0f);
+    k.near_clip = cam.near_clip;
+    k.far_clip = cam.far_clip;
+    k.pass_alpha_threshold = std::min(std::max(rs.pass_alpha_threshold, 0.0f), 1.0f);
+    std::memcpy(k.camera, cam.transform, sizeof(k.camera));
+    s.window_w = k.window_width;
+    s.window_h = k.window_height;
+    const size_t pixels = static_cast<size_t>(s.window_w) * s.window_h;
+    if (s.sample_count.bytes() != std::max<size_t>(pixels * 4u, 16u)) {
+        s.combined.allocate(pixels * 16u);
+        s.normal.allocate(pixels * 16u);
+        s.albedo.allocate(pixels * 16u);
+        s.sample_count.allocate(pixels * 4u);
+    }
+    hip_check(hipMemset(s.combined.as<void>(), 0, pixels * 16u));
+    hip_check(hipMemset(s.normal.as<void>(), 0, pixels * 16u));
+    hip_check(hipMemset(s.albedo.as<void>(), 0, pixels * 16u));
+    hip_check(hipMemset(s.sample_count.as<void>(), 0, pixels * 4u));
+    s.filter.upload(pixel_filter_table(rs.pixel_filter, rs.filter_width));
+    if (!s.counters.bytes()) s.counters.allocate(sizeof(Counters));
+    s.total_aa = 0;
+    s.rendered = 0;
+    s.sobol_size = 0;
+}
+
+// prepare_sobol_table (src/luisa/path_tracer_session.cpp:41-69)
+void prepare_sobol(Session &s, uint32_t total) {
+    const uint32_t size = sequence_size_for(total);
+    if (size != s.sobol_size) {
+        s.sobol.upload(sobol_table(size));
+        s.sobol_size = size;
+    }
+    s.params.sequence_size = size;
+}
+
+bool valid_settings(const psx_render_settings *rs) {
+    if (!rs || rs->width == 0u || rs->height == 0u) return false;
+    const uint32_t ww = rs->window_width ? rs->window_width : rs->width;
+    const uint32_t wh = rs->window_height ? rs->window_height : rs->height;
+    if (static_cast<uint64_t>(rs->window_x) + ww > rs->width) return false;
+    if (static_cast<uint64_t>(rs->window_y) + wh > rs->height) return false;
+    if (rs->pass_count && !rs->passes) return false;
+    for (uint32_t i = 0; i < rs->pass_count; ++i) {
+        const uint32_t p = rs->passes[i];
+        if (p != PSX_PASS_COMBINED && p != PSX_PASS_NORMAL && p != PSX_PASS_ALBEDO && p != PSX_PASS_SAMPLE_COUNT &&
+            p != PSX_PASS_DENOISING_NORMAL && p != PSX_PASS_DENOISING_ALBEDO)
+            return false;
+    }
+    return true;
+}
+
+// Runs samples [first, first + count) as ordered batches of at most
+// max_samples_per_dispatch (SampleDispatchPartition::make,
+// src/luisa/sample_dispatch_partition.h:134-235).
+int dispatch_samples(Session &s, uint32_t first, uint32_t count) {
+    const uint32_t per = std::max(s.scene->backend->options.max_samples_per_dispatch, 1u);
+    hip_check(hipMemsetAsync(s.counters.as<void>(), 0, sizeof(Counters), s.stream));
+    FilmView film{s.combined.as<float>(), s.normal.as<float>(), s.albedo.as<float>(), s.sample_count.as<uint32_t>()};
+    hip_check(hipEventRecord(s.ev_begin, s.stream));
+    uint64_t dispatches = 0;
+    for (uint32_t done = 0; done < count;) {
+        if (s.cancelled.load()) {
+            hip_check(hipStreamSynchronize(s.stream));
+            return PSX_ERROR_CANCELLED;
+        }
+        const uint32_t n = std::min(per, count - done);
+        hip_check(launch_render(s.scene->view, s.params, s.sobol.as<float4>(), s.filter.as<float>(), film,
+                                first + done, n, s.counters.as<Counters>(), s.count_nodes, s.stream));
+        ++dispatches;
+        done += n;
+    }
+    hip_check(hipEventRecord(s.ev_end, s.stream));
+    hip_check(hipStreamSynchronize(s.stream));
+    Counters c{};
+    hip_check(hipMemcpy(&c, s.counters.as<void>(), sizeof(c), hipMemcpyDeviceToHost));
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, s.ev_begin, s.ev_end));
+    s.stats.rays = c.rays;
+    s.stats.node_visits = c.nodes;
+    s.stats.triangle_tests = c.tris;
+    s.stats.samples = static_cast<uint64_t>(count) * s.window_w * s.window_h;
+    s.stats.dispatches = dispatches;
+    s.stats.kernel_ms = ms;
+    s.rendered += count;
+    return PSX_OK;
+}
+
+// write_passes (src/luisa/path_tracer_session.cpp:71-254)
+int write_passes(Session &s, const psx_sink *sink) {
+    const size_t n = static_cast<size_t>(s.window_w) * s.window_h;
+    std::vector<float> combined(n * 4u), normal(n * 4u), albedo(n * 4u);
+    std::vector<uint32_t> counts(n);
+    hip_check(hipMemcpy(combined.data(), s.combined.as<void>(), n * 16u, hipMemcpyDeviceToHost));
+    hip_check(hipMemcpy(normal.data(), s.normal.as<void>(), n * 16u, hipMemcpyDeviceToHost));
+    hip_check(hipMemcpy(albedo.data(), s.albedo.as<void>(), n * 16u, hipMemcpyDeviceToHost));
+    hip_check(hipMemcpy(counts.data(), s.sample_count.as<void>(), n * 4u, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i)
+        if (counts[i] != s.rendered) return PSX_ERROR_INCOMPLETE;
+    if (sink && sink->begin) sink->begin(sink->user, &s.settings);
+    const float exposure = s.settings.integrator.film_exposure;
+    std::vector<float> pixels;
+    for (uint32_t p = 0; p < s.passes.size(); ++p) {
+        const uint32_t kind = s.passes[p];
+        const uint32_t channels = std::max(1u, std::min(s.pass_channels[p], 4u));
+        pixels.assign(n * channels, 0.0f);
+        for (size_t i = 0; i < n; ++i) {
+            const float den = static_cast<float>(std::max(counts[i], 1u));
+            float v[4] = {0, 0, 0, 0};
+            switch (kind) {
+            case PSX_PASS_COMBINED:
+                for (int c = 0; c < 3; ++c) v[c] = combined[4 * i + c] * (exposure / den);
+                v[3] = std::min(std::max(1.0f - combined[4 * i + 3] / den, 0.0f), 1.0f);
+                break;
+            case PSX_PASS_NORMAL:
+            case PSX_PASS_DENOISING_NORMAL:
+                for (int c = 0; c < 4; ++c) v[c] = normal[4 * i + c] / den;
+                break;
+            case PSX_PASS_ALBEDO:
+            case PSX_PASS_DENOISING_ALBEDO:
+                for (int c = 0; c < 4; ++c) v[c] = albedo[4 * i + c] / den;
+                break;
+            case PSX_PASS_SAMPLE_COUNT:
+                v[0] = v[1] = v[2] = v[3] = static_cast<float>(counts[i]);
+                break;
+            default:
+                break;
+            }
+            for (uint32_t c = 0; c < channels; ++c) pixels[i * channels + c] = v[std::min(c, 3u)];
+        }
+        if (sink && sink->write)
+            sink->write(sink->user, kind, channels, s.settings.window_x, s.settings.window_y, s.window_w, s.window_h,
+                        s.settings.width, s.settings.height, pixels.data());
+    }
+    if (sink && sink->end) sink->end(sink->user, s.cancelled.load() ? 1 : 0);
+    return PSX_OK;
+}
+
+// Scene compilation (LuisaPathTracerBackend::compile_scene,
+// src/luisa/path_tracer_scene.cpp:50-1995, triangle subset)
+int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *diag_user, Scene **out) {
+    auto fail = [&](const std::string &msg) {
+        if (diag) diag(diag_user, msg.c_str());
+        return PSX_ERROR_INVALID_ARGUMENT;
+    };
+    if (d->abi_version != PSX_ABI_VERSION) return fail("scene ABI version mismatch");
+    if (d->camera.projection != 0u) return fail("only perspective cameras are supported");
+    auto scene = std::make_unique<Scene>();
+    scene->backend = backend;
+    scene->camera = d->camera;
+    // materials
+    std::vector<Leaf> leaves;
+    std::vector<float> params;
+    for (uint32_t i = 0; i < d->material_count; ++i) {
+        const psx_material &src = d->materials[i];
+        HostMaterial m;
+        if (src.instruction_count && !src.instructions) return fail("material without instructions");
+        m.instr.assign(src.instructions, src.instructions + src.instruction_count);
+        m.root = src.root;
+        if (src.parameter_count && !src.parameters) return fail("material without parameters");
+        m.params.assign(src.parameters, src.parameters + src.parameter_count);
+        m.emission_sampling = src.emission_sampling;
+        m.signature = structure_signature(m);
+        Material g{};
+        g.leaf_offset = static_cast<uint32_t>(leaves.size());
+        g.param_offset = static_cast<uint32_t>(params.size());
+        if (!m.instr.empty()) {
+            std::vector<Leaf> ml;
+            Leaf root{};
+            std::string error;
+            if (!flatten_program(m, m.root, root, ml, 0, error)) return fail("material " + std::to_string(i) + ": " + error);
+            uint32_t scattering = 0;
+            for (const auto &l : ml) scattering += l.op != kLeafEmission ? 1u : 0u;
+            if (ml.size() > kMaxLeaves || scattering > 4u)
+                return fail("material " + std::to_string(i) + ": closure tree larger than supported");
+            leaves.insert(leaves.end(), ml.begin(), ml.end());
+            g.leaf_count = static_cast<uint32_t>(ml.size());
+        }
+        params.insert(params.end(), m.params.begin(), m.params.end());
+        refresh_may_emit(m);
+        g.emission_sampling = m.emission_sampling;
+        g.may_emit = m.may_emit ? 1u : 0u;
+        scene->gpu_materials.push_back(g);
+        scene->materials.push_back(std::move(m));
+    }
+    // meshes
+    std::vector<float> positions;
+    std::vector<uint32_t> indices, tri_slots, slots;
+    std::vector<float> normals;
+    std::vector<uint8_t> smooth;
+    uint32_t prim_offset = 0;
+    std::vector<uint32_t> mesh_prim_offset;
+    std::vector<uint32_t> users(d->mesh_count, 0u);
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        if (d->instances[i].mesh >= d->mesh_count) return fail("instance references a missing mesh");
+        users[d->instances[i].mesh]++;
+    }
+    for (uint32_t i = 0; i < d->mesh_count; ++i) {
+        const psx_mesh &src = d->meshes[i];
+        if ((src.vertex_count && !src.positions) || (src.triangle_count && !src.triangles))
+            return fail("mesh " + std::to_string(i) + " without geometry arrays");
+        for (uint64_t t = 0; t < 3ull * src.triangle_count; ++t)
+            if (src.triangles[t] >= src.vertex_count)
+                return fail("mesh " + std::to_string(i) + " has out-of-range vertex indices");
+        for (uint32_t s = 0; s < src.material_slot_count; ++s)
+            if (src.material_slots[s] >= d->material_count)
+                return fail("mesh " + std::to_string(i) + " references a missing material");
+        Mesh m{};
+        m.vertex_offset = static_cast<uint32_t>(positions.size() / 3u);
+        m.index_offset = static_cast<uint32_t>(indices.size() / 3u);
+        m.triangle_count = src.triangle_count;
+        positions.insert(positions.end(), src.positions, src.positions + 3ull * src.vertex_count);
+        indices.insert(indices.end(), src.triangles, src.triangles + 3ull * src.triangle_count);
+        m.normal_offset = kNone;
+        if (src.normals) {
+            m.normal_offset = static_cast<uint32_t>(normals.size() / 3u);
+            normals.insert(normals.end(), src.normals, src.normals + 3ull * src.vertex_count);
+        }
+        m.smooth_offset = kNone;
+        if (src.triangle_smooth) {
+            m.smooth_offset = static_cast<uint32_t>(smooth.size());
+            smooth.insert(smooth.end(), src.triangle_smooth, src.triangle_smooth + src.triangle_count);
+        }
+        m.slot_offset = static_cast<uint32_t>(slots.size());
+        m.slot_count = src.material_slot_count;
+        slots.insert(slots.end(), src.material_slots, src.material_slots + src.material_slot_count);
+        m.tri_slot_offset = kNone;
+        if (src.triangle_material_slots) {
+            m.tri_slot_offset = static_cast<uint32_t>(tri_slots.size());
+            tri_slots.insert(tri_slots.end(), src.triangle_material_slots,
+                             src.triangle_material_slots + src.triangle_count);
+        }
+        scene->meshes.push_back(m);
+        scene->mesh_positions.emplace_back(src.positions, src.positions + 3ull * src.vertex_count);
+        scene->mesh_tris.emplace_back(src.triangles, src.triangles + 3ull * src.triangle_count);
+        scene->mesh_tri_slots.emplace_back();
+        if (src.triangle_material_slots)
+            scene->mesh_tri_slots.back().assign(src.triangle_material_slots,
+                                                src.triangle_material_slots + src.triangle_count);
+        scene->mesh_slots.emplace_back(src.material_slots, src.material_slots + src.material_slot_count);
+        mesh_prim_offset.push_back(prim_offset);
+        prim_offset += src.triangle_count;
+    }
+    // BLASes and instances (build_cycles_instance_intersection_plan,
+    // src/luisa/path_tracer_scene_geometry.cpp:272-318)
+    std::vector<Node4> nodes(1);  // node 0 reserved for the TLAS root
+    std::vector<LeafTri> tris;
+    std::vector<Instance> instances;
+    std::vector<uint32_t> overrides;
+    std::vector<uint32_t> shared_root(d->mesh_count, kNone);
+    std::vector<std::array<float, 6>> shared_bounds(d->mesh_count);
+    std::vector<uint32_t> shared_depth(d->mesh_count, 0u);
+    std::vector<BuildItem> tlas_items;
+    std::vector<uint32_t> tlas_instance;
+    uint32_t max_blas_depth = 0;
+    auto build_blas = [&](const float *pos, const uint32_t *tri, uint32_t tcount, uint32_t &root, float lo[3],
+                          float hi[3], uint32_t &depth) {
+        std::vector<BuildItem> items(tcount);
+        for (uint32_t t = 0; t < tcount; ++t) {
+            for (int a = 0; a < 3; ++a) {
+                const float x = pos[3u * tri[3u * t] + a], y = pos[3u * tri[3u * t + 1] + a],
+                            z = pos[3u * tri[3u * t + 2] + a];
+                items[t].lo[a] = std::min(x, std::min(y, z));
+                items[t].hi[a] = std::max(x, std::max(y, z));
+            }
+        }
+        const uint32_t base = static_cast<uint32_t>(nodes.size());
+        const uint32_t tri_base = static_cast<uint32_t>(tris.size());
+        Bvh4 b = build_bvh4(items, base, 4u);
+        for (auto &n : b.nodes)
+            for (int i = 0; i < 4; ++i)
+                if (n.count[i] != kEmptySlot && n.count[i] > 0u) n.child[i] += tri_base;
+        for (uint32_t idx : b.order) {
+            LeafTri lt{};
+            const uint32_t *t3 = tri + 3u * idx;
+            lt.p0x = pos[3u * t3[0]];
+            lt.p0y = pos[3u * t3[0] + 1];
+            lt.p0z = pos[3u * t3[0] + 2];
+            std::memcpy(&lt.prim_bits, &idx, 4);
+            lt.p1x = pos[3u * t3[1]];
+            lt.p1y = pos[3u * t3[1] + 1];
+            lt.p1z = pos[3u * t3[1] + 2];
+            lt.p2x = pos[3u * t3[2]];
+            lt.p2y = pos[3u * t3[2] + 1];
+            lt.p2z = pos[3u * t3[2] + 2];
+            tris.push_back(lt);
+        }
+        nodes.insert(nodes.end(), b.nodes.begin(), b.nodes.end());
+        root = base;
+        std::memcpy(lo, b.lo, sizeof(float) * 3);
+        std::memcpy(hi, b.hi, sizeof(float) * 3);
+        depth = b.depth;
+    };
+    scene->overrides.resize(d->instance_count);
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        const psx_instance &src = d->instances[i];
+        scene->instances.push_back(src);
+        const uint32_t mi = src.mesh;
+        for (uint32_t o = 0; o < src.material_override_count; ++o)
+            if (src.material_overrides[o] >= d->material_count)
+                return fail("instance " + std::to_string(i) + " overrides with a missing material");
+        scene->overrides[i].assign(src.material_overrides, src.material_overrides + src.material_override_count);
+        Instance g{};
+        float w2o[16], n2w[16];
+        cycles_inverse(src.transform, w2o);
+        transpose4(w2o, n2w);
+        g.w2o = to_affine(w2o);
+        g.o2w = to_affine(src.transform);
+        g.n2w = to_affine(n2w);
+        g.mesh = mi;
+        g.visibility = src.visibility_mask;
+        g.prim_offset = mesh_prim_offset[mi];
+        g.override_offset = static_cast<uint32_t>(overrides.size());
+        g.override_count = src.material_override_count;
+        overrides.insert(overrides.end(), src.material_overrides, src.material_overrides + src.material_override_count);
+        const Mesh &mesh = scene->meshes[mi];
+        float lo[3], hi[3];
+        uint32_t depth = 0;
+        g.applied = users[mi] == 1u ? 1u : 0u;
+        if (g.applied) {
+            // single-user mesh: host-transformed final world support positions
+            const auto &op = scene->mesh_positions[mi];
+            std::vector<float> wp(op.size());
+            for (size_t v = 0; v < op.size() / 3u; ++v) {
+                const Vec3 p = transform_point(src.transform, v3(op[3 * v], op[3 * v + 1], op[3 * v + 2]));
+                wp[3 * v] = p.x;
+                wp[3 * v + 1] = p.y;
+                wp[3 * v + 2] = p.z;
+            }
+            g.ipos_offset = static_cast<uint32_t>(positions.size() / 3u);
+            positions.insert(positions.end(), wp.begin(), wp.end());
+            build_blas(wp.data(), scene->mesh_tris[mi].data(), mesh.triangle_count, g.blas_root, lo, hi, depth);
+        } else {
+            g.ipos_offset = mesh.vertex_offset;
+            if (shared_root[mi] == kNone) {
+                float slo[3], shi[3];
+                build_blas(scene->mesh_positions[mi].data(), scene->mesh_tris[mi].data(), mesh.triangle_count,
+                           shared_root[mi], slo, shi, shared_depth[mi]);
+                std::copy(slo, slo + 3, shared_bounds[mi].begin());
+                std::copy(shi, shi + 3, shared_bounds[mi].begin() + 3);
+            }
+            g.blas_root = shared_root[mi];
+            depth = shared_depth[mi];
+            // world box of the object-space root box
+            float wl[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, wh[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            for (int c = 0; c < 8; ++c) {
+                const Vec3 corner = v3((c & 1) ? shared_bounds[mi][3] : shared_bounds[mi][0],
+                                       (c & 2) ? shared_bounds[mi][4] : shared_bounds[mi][1],
+                                       (c & 4) ? shared_bounds[mi][5] : shared_bounds[mi][2]);
+                const Vec3 w = transform_point(src.transform, corner);
+                wl[0] = std::min(wl[0], w.x);
+                wl[1] = std::min(wl[1], w.y);
+                wl[2] = std::min(wl[2], w.z);
+                wh[0] = std::max(wh[0], w.x);
+                wh[1] = std::max(wh[1], w.y);
+                wh[2] = std::max(wh[2], w.z);
+            }
+            for (int a = 0; a < 3; ++a) {
+                const float s = std::max(std::max(std::fabs(wl[a]), std::fabs(wh[a])), wh[a] - wl[a]);
+                lo[a] = wl[a] - (s * 1.0e-5f + 1.0e-30f);
+                hi[a] = wh[a] + (s * 1.0e-5f + 1.0e-30f);
+            }
+        }
+        max_blas_depth = std::max(max_blas_depth, depth);
+        if (mesh.triangle_count > 0u && lo[0] <= hi[0]) {
+            BuildItem it;
+            std::copy(lo, lo + 3, it.lo);
+            std::copy(hi, hi + 3, it.hi);
+            tlas_items.push_back(it);
+            tlas_instance.push_back(i);
+        }
+        instances.push_back(g);
+        scene->triangles += mesh.triangle_count;
+    }
+    // TLAS, one instance per leaf item
+    uint32_t tlas_depth = 0;
+    std::vector<uint32_t> tlas_leaf_items;
+    bool has_tlas = !tlas_items.empty();
+    if (has_tlas) {
+        const uint32_t base = static_cast<uint32_t>(nodes.size());
+        Bvh4 t = build_bvh4(tlas_items, base, 1u);
+        // the TLAS root must live at node 0: place it there and append the rest
+        for (uint32_t idx : t.order) tlas_leaf_items.push_back(tlas_instance[idx]);
+        // inner child indices point at base + local index; local 0 moves to slot 0
+        std::vector<Node4> tn = t.nodes;
+        for (auto &n : tn)
+            for (int i = 0; i < 4; ++i)
+                if (n.count[i] == 0u) n.child[i] -= 1u;  // local k >= 1 lives at base + k - 1
+        nodes[0] = tn[0];
+        nodes.insert(nodes.end(), tn.begin() + 1, tn.end());
+        tlas_depth = t.depth;
+    }
+    scene->tlas_depth = tlas_depth;
+    scene->blas_depth = max_blas_depth;
+    // stack bound: a TLAS node pushes at most four entries (instance markers
+    // and inner children), a BLAS node at most three
+    scene->max_stack = 4u * tlas_depth + 3u * max_blas_depth + 2u;
+    if (scene->max_stack > kTraversalStackDepth)
+        return fail("BVH too deep for the traversal stack (" + std::to_string(scene->max_stack) + " > " +
+                    std::to_string(kTraversalStackDepth) + ")");
+    // upload
+    Scene &s = *scene;
+    s.nodes.upload(nodes);
+    s.tris.upload(tris);
+    s.tlas_items.upload(tlas_leaf_items);
+    s.instances_buf.upload(instances);
+    s.meshes_buf.upload(scene->meshes);
+    s.positions.upload(positions);
+    s.indices.upload(indices);
+    s.normals.upload(normals);
+    s.smooth.upload(smooth);
+    s.slots.upload(slots);
+    s.tri_slots.upload(tri_slots);
+    s.overrides_buf.upload(overrides);
+    s.materials_buf.upload(scene->gpu_materials);
+    s.leaves.upload(leaves);
+    s.params.upload(params);
+    SceneView &v = s.view;
+    v.nodes = s.nodes.as<Node4>();
+    v.tris = s.tris.as<LeafTri>();
+    v.tlas_items = s.tlas_items.as<uint32_t>();
+    v.instances = s.instances_buf.as<Instance>();
+    v.meshes = s.meshes_buf.as<Mesh>();
+    v.positions = s.positions.as<float>();
+    v.indices = s.indices.as<uint32_t>();
+    v.normals = s.normals.as<float>();
+    v.smooth = s.smooth.as<uint8_t>();
+    v.slots = s.slots.as<uint32_t>();
+    v.tri_slots = s.tri_slots.as<uint32_t>();
+    v.overrides = s.overrides_buf.as<uint32_t>();
+    v.materials = s.materials_buf.as<Material>();
+    v.leaves = s.leaves.as<Leaf>();
+    v.params = s.params.as<float>();
+    v.tlas_root = 0u;
+    v.has_tlas = has_tlas ? 1u : 0u;
+    v.instance_count = d->instance_count;
+    v.material_count = d->material_count;
+    // background lights fold into the constant world radiance
+    // (src/luisa/path_tracer_scene.cpp:1723-1734)
+    float bg[3] = {d->world_color[0], d->world_color[1], d->world_color[2]};
+    for (uint32_t i = 0; i < d->light_count; ++i) {
+        const psx_light &l = d->lights[i];
+        if (l.type != PSX_LIGHT_BACKGROUND) return fail("only background lights are supported by this ABI version");
+        bg[0] += l.color[0] * l.power;
+        bg[1] += l.color[1] * l.power;
+        bg[2] += l.color[2] * l.power;
+    }
+    v.background[0] = bg[0];
+    v.background[1] = bg[1];
+    v.background[2] = bg[2];
+    v.world_visibility = d->world_visibility_mask;
+    s.rebuild_emitters();
+    *out = scene.release();
+    return PSX_OK;
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const DeviceError &e) {
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        return PSX_ERROR_OUT_OF_MEMORY;
+    } catch (...) {
+        return PSX_ERROR_DEVICE;
+    }
+}
+
+}  // namespace
+}  // namespace psx
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace psx;
+
+struct psx_backend : Backend {};
+struct psx_scene : Scene {};
+struct psx_session : Session {};
+
+extern "C" {
+
+const char *psx_version(void) { return "psx 0.1 (gfx950)"; }
+
+int psx_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void psx_render_settings_default(psx_render_settings *s) {
+    if (!s) return;
+    std::memset(s, 0, sizeof(*s));
+    s->pixel_filter = PSX_FILTER_BOX;
+    s->filter_width = 1.0f;
+    s->pass_alpha_threshold = 0.5f;
+    psx_integrator_settings &i = s->integrator;
+    i.max_bounces = 12u;
+    i.min_bounces = 0u;
+    i.diffuse_bounces = 4u;
+    i.glossy_bounces = 4u;
+    i.transmission_bounces = 12u;
+    i.volume_bounces = 0u;
+    i.transparent_min_bounces = 0u;
+    i.transparent_max_bounces = 8u;
+    i.film_exposure = 1.0f;
+    i.light_sampling_threshold = 0.01f;
+    i.reflective_caustics = 1u;
+    i.refractive_caustics = 1u;
+    i.direct_light_sampling = PSX_DIRECT_LIGHT_MIS;
+}
+
+int psx_backend_create(int device, const psx_backend_options *options, psx_backend **out) {
+    if (!out) return PSX_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    return guarded([&] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return static_cast<int>(PSX_ERROR_NO_DEVICE);
+        if (device < 0 || device >= n) return static_cast<int>(PSX_ERROR_INVALID_ARGUMENT);
+        hip_check(hipSetDevice(device));
+        auto *b = new psx_backend();
+        b->device = device;
+        b->options.next_event_estimation = 1u;
+        b->options.max_samples_per_dispatch = 4u;
+        if (options) {
+            b->options = *options;
+            if (b->options.max_samples_per_dispatch == 0u) {
+                delete b;
+                return static_cast<int>(PSX_ERROR_INVALID_ARGUMENT);
+            }
+        }
+        *out = b;
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+void psx_backend_destroy(psx_backend *b) { delete b; }
+
+int psx_scene_compile(psx_backend *b, const psx_scene_desc *d, psx_diag_fn diag, void *user, psx_scene **out) {
+    if (!b || !d || !out) return PSX_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    return guarded([&] {
+        hip_check(hipSetDevice(b->device));
+        Scene *s = nullptr;
+        const int rc = compile(b, d, diag, user, &s);
+        if (rc != PSX_OK) return rc;
+        // re-home into the ABI type
+        *out = reinterpret_cast<psx_scene *>(s);
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+void psx_scene_destroy(psx_scene *s) { delete reinterpret_cast<Scene *>(s); }
+
+int psx_scene_update_parameters(psx_scene *handle, uint32_t material, const float *p, uint32_t count) {
+    Scene *s = reinterpret_cast<Scene *>(handle);
+    if (!s || !p || material >= s->materials.size()) return PSX_ERROR_INVALID_ARGUMENT;
+    HostMaterial &m = s->materials[material];
+    if (count != m.params.size()) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->backend->device));
+        m.params.assign(p, p + count);
+        const bool emitted = m.may_emit;
+        refresh_may_emit(m);
+        Material &g = s->gpu_materials[material];
+        g.may_emit = m.may_emit ? 1u : 0u;
+        hip_check(hipMemcpy(s->params.as<float>() + g.param_offset, p, sizeof(float) * count, hipMemcpyHostToDevice));
+        hip_check(hipMemcpy(s->materials_buf.as<Material>() + material, &g, sizeof(Material), hipMemcpyHostToDevice));
+        if (emitted != m.may_emit) s->rebuild_emitters();
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+int psx_session_create(psx_scene *handle, const psx_render_settings *rs, psx_session **out) {
+    Scene *scene = reinterpret_cast<Scene *>(handle);
+    if (!scene || !out) return PSX_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!valid_settings(rs)) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        hip_check(hipSetDevice(scene->backend->device));
+        auto s = std::make_unique<Session>();
+        s->scene = scene;
+        hip_check(hipStreamCreate(&s->stream));
+        hip_check(hipEventCreate(&s->ev_begin));
+        hip_check(hipEventCreate(&s->ev_end));
+        initialize_session(*s, *rs);
+        *out = reinterpret_cast<psx_session *>(s.release());
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+void psx_session_destroy(psx_session *s) { delete reinterpret_cast<Session *>(s); }
+
+int psx_session_reset(psx_session *handle, const psx_render_settings *rs) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s || !valid_settings(rs)) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->scene->backend->device));
+        s->cancelled.store(false);
+        initialize_session(*s, *rs);
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+namespace {
+int check_range(Session *s, uint32_t first, uint32_t count, uint32_t offset, uint32_t total) {
+    if (s->cancelled.load()) return PSX_ERROR_CANCELLED;
+    if (count == 0u) return PSX_ERROR_RANGE;
+    const uint64_t end = static_cast<uint64_t>(first) + offset + count;
+    if (total == 0u || end > total || (s->total_aa != 0u && s->total_aa != total)) return PSX_ERROR_RANGE;
+    return PSX_OK;
+}
+}  // namespace
+
+int psx_session_render(psx_session *handle, uint32_t first, uint32_t count, uint32_t offset, uint32_t total,
+                       const psx_sink *sink) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        const int rc = check_range(s, first, count, offset, total);
+        if (rc != PSX_OK) return rc;
+        hip_check(hipSetDevice(s->scene->backend->device));
+        s->total_aa = total;
+        prepare_sobol(*s, total);
+        const int d = dispatch_samples(*s, first + offset, count);
+        if (d != PSX_OK) return d;
+        if (s->cancelled.load()) return static_cast<int>(PSX_ERROR_CANCELLED);
+        return write_passes(*s, sink);
+    });
+}
+
+void psx_session_cancel(psx_session *handle) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (s) s->cancelled.store(true);
+}
+
+int psx_session_stats(const psx_session *handle, psx_render_stats *out) {
+    const Session *s = reinterpret_cast<const Session *>(handle);
+    if (!s || !out) return PSX_ERROR_INVALID_ARGUMENT;
+    *out = s->stats;
+    return PSX_OK;
+}
+
+int psx_session_render_device(psx_session *handle, uint32_t first, uint32_t count, uint32_t total) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        const int rc = check_range(s, first, count, 0u, total);
+        if (rc != PSX_OK) return rc;
+        hip_check(hipSetDevice(s->scene->backend->device));
+        s->total_aa = total;
+        prepare_sobol(*s, total);
+        return dispatch_samples(*s, first, count);
+    });
+}
+
+int psx_session_read_film(psx_session *handle, float *combined, float *normal, float *albedo, uint32_t *counts) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->scene->backend->device));
+        const size_t n = static_cast<size_t>(s->window_w) * s->window_h;
+        if (combined) hip_check(hipMemcpy(combined, s->combined.as<void>(), n * 16u, hipMemcpyDeviceToHost));
+        if (normal) hip_check(hipMemcpy(normal, s->normal.as<void>(), n * 16u, hipMemcpyDeviceToHost));
+        if (albedo) hip_check(hipMemcpy(albedo, s->albedo.as<void>(), n * 16u, hipMemcpyDeviceToHost));
+        if (counts) hip_check(hipMemcpy(counts, s->sample_count.as<void>(), n * 4u, hipMemcpyDeviceToHost));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+int psx_session_primary_hits(psx_session *handle, uint32_t sample, uint32_t total, uint32_t *out) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s || !out || total == 0u || sample >= total) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->scene->backend->device));
+        prepare_sobol(*s, total);
+        const size_t n = static_cast<size_t>(s->window_w) * s->window_h;
+        DeviceBuffer rec;
+        rec.allocate(n * 16u);
+        hip_check(launch_primary(s->scene->view, s->params, s->sobol.as<float4>(), s->filter.as<float>(), sample,
+                                 rec.as<uint32_t>(), s->stream));
+        hip_check(hipStreamSynchronize(s->stream));
+        hip_check(hipMemcpy(out, rec.as<void>(), n * 16u, hipMemcpyDeviceToHost));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+int psx_session_trace_events(psx_session *handle, uint32_t sample, uint32_t total, uint32_t *out) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s || !out || total == 0u || sample >= total) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->scene->backend->device));
+        prepare_sobol(*s, total);
+        const size_t n = static_cast<size_t>(s->window_w) * s->window_h;
+        const size_t words = n * PSX_TRACE_EVENTS * PSX_TRACE_WORDS;
+        DeviceBuffer rec;
+        rec.allocate(words * 4u);
+        hip_check(launch_trace(s->scene->view, s->params, s->sobol.as<float4>(), s->filter.as<float>(), sample,
+                               rec.as<uint32_t>(), s->stream));
+        hip_check(hipStreamSynchronize(s->stream));
+        hip_check(hipMemcpy(out, rec.as<void>(), words * 4u, hipMemcpyDeviceToHost));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+int psx_sobol_table(uint32_t total, float *out, uint32_t *out_count) {
+    if (total == 0u) return PSX_ERROR_INVALID_ARGUMENT;
+    const uint32_t size = sequence_size_for(total);
+    const uint32_t count = size * 256u;
+    if (out_count) *out_count = count;
+    if (out) {
+        const std::vector<float> t = sobol_table(size);
+        std::memcpy(out, t.data(), t.size() * sizeof(float));
+    }
+    return PSX_OK;
+}
+
+int psx_session_sampler_probe(psx_session *handle, uint32_t sample, uint32_t total, uint32_t first_dim,
+                              uint32_t dim_count, uint32_t *out) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s || !out || total == 0u || dim_count == 0u || dim_count > 64u) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->scene->backend->device));
+        prepare_sobol(*s, total);
+        const size_t n = static_cast<size_t>(s->window_w) * s->window_h;
+        const size_t words = n * (1u + 4u * dim_count);
+        DeviceBuffer rec;
+        rec.allocate(words * 4u);
+        hip_check(launch_sampler(s->params, s->sobol.as<float4>(), sample, first_dim, dim_count, rec.as<uint32_t>(),
+                                 s->stream));
+        hip_check(hipStreamSynchronize(s->stream));
+        hip_check(hipMemcpy(out, rec.as<void>(), words * 4u, hipMemcpyDeviceToHost));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,269 @@
+"""Python mirror of Psycles' render contract over the psx C ABI.
+
+Same names and meanings as include/psycles/contract/render.h:119-181:
+
+* ``RendererBackend.compile_scene(snapshot) -> SceneCompilation`` with
+  ``scene`` (None on failure) and ``diagnostics``;
+* ``RendererBackend.create_session(scene, settings) -> RenderSession`` or
+  None for invalid settings (src/luisa/path_tracer.cpp:22-75);
+* ``RenderSession.render_samples(SampleRange, OutputSink) -> bool`` - False
+  on cancel, a bad range or a changed total, exactly like
+  src/luisa/path_tracer_session.cpp:272-413;
+* ``OutputSink.begin / write(PassTile) / end(cancelled)``.
+
+Every call goes through libpsx.so; there is no other implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+
+@dataclass
+class SampleRange:
+    first: int = 0
+    count: int = 0
+    offset: int = 0
+    total: int = 0
+
+
+@dataclass
+class PassTile:
+    pass_kind: int
+    channels: int
+    window: tuple
+    full_extent: tuple
+    pixels: np.ndarray
+
+
+class OutputSink:
+    def begin(self, settings) -> None:
+        pass
+
+    def write(self, tile: PassTile) -> None:
+        pass
+
+    def end(self, cancelled: bool) -> None:
+        pass
+
+
+class MemoryOutputSink(OutputSink):
+    """Keeps every pass as an (H, W, C) array (include/psycles/io/image.h)."""
+
+    def __init__(self):
+        self.passes: dict[int, np.ndarray] = {}
+        self.began = False
+        self.ended = None
+
+    def begin(self, settings) -> None:
+        self.began = True
+
+    def write(self, tile: PassTile) -> None:
+        self.passes[tile.pass_kind] = tile.pixels
+
+    def end(self, cancelled: bool) -> None:
+        self.ended = bool(cancelled)
+
+
+@dataclass
+class RenderDiagnostic:
+    message: str
+
+
+@dataclass
+class SceneCompilation:
+    scene: "CompiledScene | None"
+    diagnostics: list = field(default_factory=list)
+
+    def ok(self) -> bool:
+        return self.scene is not None
+
+
+class CompiledScene:
+    def __init__(self, backend: "RendererBackend", handle, snapshot):
+        self._backend = backend
+        self.handle = handle
+        self.snapshot = snapshot
+
+    def update_parameters(self, material: int, parameters) -> bool:
+        """Parameter-only rebind; False when the block size changed."""
+        p = np.ascontiguousarray(parameters, dtype=np.float32)
+        rc = self._backend.lib.psx_scene_update_parameters(
+            self.handle, material, p.ctypes.data_as(C.POINTER(C.c_float)), p.size)
+        return rc == abi.PSX_OK
+
+    def close(self) -> None:
+        if self.handle:
+            self._backend.lib.psx_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RenderSession:
+    def __init__(self, backend: "RendererBackend", scene: CompiledScene, handle, settings):
+        self._backend = backend
+        self.scene = scene
+        self.handle = handle
+        self.settings = settings
+        self.last_status = abi.PSX_OK
+
+    @property
+    def window(self) -> tuple:
+        s = self.settings
+        return (s.window_width or s.width, s.window_height or s.height)
+
+    def reset(self, settings) -> None:
+        abi.check("psx_session_reset",
+                  self._backend.lib.psx_session_reset(self.handle, C.byref(settings)))
+        self.settings = settings
+
+    def render_samples(self, samples: SampleRange, output: OutputSink) -> bool:
+        lib = self._backend.lib
+        sink_ref = {}
+
+        def on_begin(user, settings_ptr):
+            output.begin(self.settings)
+
+        def on_write(user, kind, channels, wx, wy, ww, wh, fw, fh, pixels):
+            arr = np.ctypeslib.as_array(pixels, shape=(wh * ww * channels,)).copy()
+            output.write(PassTile(kind, channels, (wx, wy, ww, wh), (fw, fh),
+                                  arr.reshape(wh, ww, channels)))
+
+        def on_end(user, cancelled):
+            output.end(bool(cancelled))
+
+        sink = abi.Sink()
+        sink_ref["b"] = abi.SINK_BEGIN(on_begin)
+        sink_ref["w"] = abi.SINK_WRITE(on_write)
+        sink_ref["e"] = abi.SINK_END(on_end)
+        sink.begin = sink_ref["b"]
+        sink.write = sink_ref["w"]
+        sink.end = sink_ref["e"]
+        rc = lib.psx_session_render(self.handle, samples.first, samples.count, samples.offset,
+                                    samples.total, C.byref(sink))
+        self.last_status = rc
+        if rc in (abi.PSX_ERROR_DEVICE, abi.PSX_ERROR_OUT_OF_MEMORY, abi.PSX_ERROR_NO_DEVICE):
+            raise abi.PsxError("psx_session_render", rc)
+        return rc == abi.PSX_OK
+
+    def render_device(self, first: int, count: int, total: int) -> None:
+        """Accumulate samples on the device without reading the film back."""
+        abi.check("psx_session_render_device",
+                  self._backend.lib.psx_session_render_device(self.handle, first, count, total))
+
+    def cancel(self) -> None:
+        self._backend.lib.psx_session_cancel(self.handle)
+
+    def stats(self) -> abi.RenderStats:
+        st = abi.RenderStats()
+        abi.check("psx_session_stats", self._backend.lib.psx_session_stats(self.handle, C.byref(st)))
+        return st
+
+    def read_film(self):
+        w, h = self.window
+        combined = np.zeros((h, w, 4), dtype=np.float32)
+        normal = np.zeros((h, w, 4), dtype=np.float32)
+        albedo = np.zeros((h, w, 4), dtype=np.float32)
+        counts = np.zeros((h, w), dtype=np.uint32)
+        f = C.POINTER(C.c_float)
+        abi.check("psx_session_read_film", self._backend.lib.psx_session_read_film(
+            self.handle, combined.ctypes.data_as(f), normal.ctypes.data_as(f),
+            albedo.ctypes.data_as(f), counts.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return combined, normal, albedo, counts
+
+    def primary_hits(self, sample_index: int, total: int) -> np.ndarray:
+        w, h = self.window
+        out = np.zeros((h, w, 4), dtype=np.uint32)
+        abi.check("psx_session_primary_hits", self._backend.lib.psx_session_primary_hits(
+            self.handle, sample_index, total, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
+
+    def trace_events(self, sample_index: int, total: int) -> np.ndarray:
+        w, h = self.window
+        out = np.zeros((h, w, abi.TRACE_EVENTS, abi.TRACE_WORDS), dtype=np.uint32)
+        abi.check("psx_session_trace_events", self._backend.lib.psx_session_trace_events(
+            self.handle, sample_index, total, out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
+
+    def sampler_probe(self, sample_index: int, total: int, first_dim: int, count: int) -> np.ndarray:
+        w, h = self.window
+        out = np.zeros((h, w, 1 + 4 * count), dtype=np.uint32)
+        abi.check("psx_session_sampler_probe", self._backend.lib.psx_session_sampler_probe(
+            self.handle, sample_index, total, first_dim, count,
+            out.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            self._backend.lib.psx_session_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RendererBackend:
+    """luisa_backend::LuisaPathTracerBackend counterpart on one MI355X."""
+
+    def __init__(self, device: int = 0, next_event_estimation: bool = True,
+                 max_samples_per_dispatch: int = 4, integrator: int = 0):
+        self.lib = abi.load_library()
+        opts = abi.BackendOptions(1 if next_event_estimation else 0, max_samples_per_dispatch,
+                                  integrator, 0)
+        handle = C.c_void_p()
+        abi.check("psx_backend_create", self.lib.psx_backend_create(device, C.byref(opts),
+                                                                    C.byref(handle)))
+        self.handle = handle
+
+    def compile_scene(self, snapshot) -> SceneCompilation:
+        desc, keep = snapshot.to_desc()
+        diagnostics: list[RenderDiagnostic] = []
+
+        def on_diag(user, msg):
+            diagnostics.append(RenderDiagnostic(msg.decode()))
+
+        cb = abi.DIAG_FN(on_diag)
+        handle = C.c_void_p()
+        rc = self.lib.psx_scene_compile(self.handle, C.byref(desc), cb, None, C.byref(handle))
+        del keep
+        if rc != abi.PSX_OK:
+            if rc in (abi.PSX_ERROR_DEVICE, abi.PSX_ERROR_OUT_OF_MEMORY):
+                raise abi.PsxError("psx_scene_compile", rc)
+            return SceneCompilation(None, diagnostics)
+        return SceneCompilation(CompiledScene(self, handle, snapshot), diagnostics)
+
+    def create_session(self, scene: CompiledScene, settings) -> RenderSession | None:
+        handle = C.c_void_p()
+        rc = self.lib.psx_session_create(scene.handle, C.byref(settings), C.byref(handle))
+        if rc == abi.PSX_ERROR_INVALID_ARGUMENT:
+            return None
+        abi.check("psx_session_create", rc)
+        return RenderSession(self, scene, handle, settings)
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.psx_backend_destroy(self.handle)
+            self.handle = None
+
+
+def with_passes(settings, passes=((abi.PASS_COMBINED, 4), (abi.PASS_NORMAL, 3),
+                                  (abi.PASS_ALBEDO, 3), (abi.PASS_SAMPLE_COUNT, 1))):
+    """Attach pass requests to RenderSettings; returns (settings, keepalive)."""
+    kinds = np.asarray([p[0] for p in passes], dtype=np.uint32)
+    chans = np.asarray([p[1] for p in passes], dtype=np.uint32)
+    settings.passes = kinds.ctypes.data_as(C.POINTER(C.c_uint32))
+    settings.pass_channels = chans.ctypes.data_as(C.POINTER(C.c_uint32))
+    settings.pass_count = len(passes)
+    return settings, (kinds, chans)
