@@ -97,16 +97,23 @@ def test_path_events(backend, name):
     scene = compile_scene(backend, snap)
     session = backend.create_session(scene, s)
     osc = ob.OracleScene(snap)
+    discrete = [abi.TRACE_FLAGS, abi.TRACE_RNG_OFFSET, abi.TRACE_OBJECT, abi.TRACE_PRIMITIVE,
+                abi.TRACE_LIGHT, abi.TRACE_CLOSURE, abi.TRACE_TERMINATE_BITS]
     for sample in (0, 7):
         gpu = session.trace_events(sample, 16)
         cpu = osc.trace_events(s, sample, 16)
-        # first event of every pixel: exact
+        # first event of every pixel: exact, including the RR probability
         assert np.array_equal(gpu[:, :, 0, :], cpu[:, :, 0, :])
         valid = (cpu[..., 0] & 1) != 0
-        same = np.all(gpu == cpu, axis=-1)
+        same = np.all(gpu[..., discrete] == cpu[..., discrete], axis=-1)
         frac = same[valid].mean()
-        log(f"  sample {sample}: {valid.sum()} events, identical fraction {frac:.6f}")
+        log(f"  sample {sample}: {valid.sum()} events, discrete identical {frac:.6f}")
         assert frac >= PIXEL_FRACTION
+        # the continuation probability is a float of the throughput chain:
+        # it may differ in the last ulp after a libm/ocml sin or cos
+        pg = gpu[..., abi.TRACE_RR_BITS][valid].view(np.float32)
+        pc = cpu[..., abi.TRACE_RR_BITS][valid].view(np.float32)
+        assert np.all(np.abs(pg - pc) <= 1e-6 * np.maximum(np.abs(pc), 1e-30))
 
 
 # ------------------------------------------------------------------ film
