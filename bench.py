@@ -45,7 +45,7 @@ def cpu_baseline(snap, settings, seconds_target=15.0):
     t0 = time.perf_counter()
     _, st = scene.render(settings, 0, 1, 64, threads=threads)
     dt = max(time.perf_counter() - t0, 1e-3)
-    spp = int(max(1, min(63, round(seconds_target * st.samples / dt / st.samples))))
+    spp = int(max(1, min(63, round(seconds_target / dt))))
     t0 = time.perf_counter()
     _, st = scene.render(settings, 1, spp, 64, threads=threads)
     dt = time.perf_counter() - t0
