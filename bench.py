@@ -5,9 +5,14 @@ triangle displaced icosphere instanced 64 times, 1920x1080 at 64 spp, max 8
 bounces, mix(diffuse, transparent) materials, constant world background.
 Inputs (scene tables, BVH, film) are resident in HBM before the timed region.
 
+The timed path is the batched wavefront integrator (trace and shade kernels
+over path state in HBM, 8 samples of every pixel per dispatch).  The plain
+megakernel (one lane runs whole paths) is timed on one frame beside it.
+
 Prints one JSON line with samples/s (value), Mrays/s, the HBM roofline of the
-render kernel and the CPU oracle timed on a bounded sample of the same
-workload on the host cores.
+dominant kernel (the wavefront trace kernel, timed with HIP events around
+each of its launches on the session stream) and the CPU oracle timed on a
+bounded sample of the same workload on the host cores.
 """
 from __future__ import annotations
 
@@ -31,6 +36,7 @@ ORACLE_TRIS_PER_RAY = 8.59
 BYTES_PER_RAY = 48.0 + 32.0 * ORACLE_NODES_PER_RAY + 36.0 * ORACLE_TRIS_PER_RAY
 FILM_BYTES_PER_PIXEL_DISPATCH = 104.0
 HBM_PEAK_GBS = 8000.0
+SAMPLES_PER_DISPATCH = 8
 
 
 def cpu_baseline(snap, settings, seconds_target=15.0):
@@ -43,7 +49,7 @@ def cpu_baseline(snap, settings, seconds_target=15.0):
     threads = min(os.cpu_count() or 1, 16)
     scene = ob.OracleScene(snap)
     t0 = time.perf_counter()
-    _, st = scene.render(settings, 0, 1, 64, threads=threads)
+    scene.render(settings, 0, 1, 64, threads=threads)
     dt = max(time.perf_counter() - t0, 1e-3)
     spp = int(max(1, min(63, round(seconds_target / dt))))
     t0 = time.perf_counter()
@@ -66,19 +72,21 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the megakernel frame")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(every box here has one MI355X)", file=sys.stderr)
         return 2
 
-    from psycles_amd import scenes
+    from psycles_amd import abi, scenes
     from psycles_amd.render import RendererBackend
 
     print("bench: building instanced scene (1,003,520 tris x 64 instances)", file=sys.stderr, flush=True)
     snap = scenes.instanced_icospheres()
     settings = scenes.instanced_settings()
-    backend = RendererBackend(0)
+    backend = RendererBackend(0, integrator=abi.INTEGRATOR_WAVEFRONT,
+                              max_samples_per_dispatch=SAMPLES_PER_DISPATCH)
     t0 = time.perf_counter()
     comp = backend.compile_scene(snap)
     if not comp.ok():
@@ -91,36 +99,34 @@ def main() -> int:
 
     session = backend.create_session(comp.scene, settings)
 
-    def step():
+    def step(sess):
         # a fresh frame: reset zeroes the resident film, then the whole
         # sample range is dispatched and waited for
-        session.reset(settings)
-        session.render_device(0, spp, spp)
-        return session.stats()
+        sess.reset(settings)
+        sess.render_device(0, spp, spp)
+        return sess.stats()
 
     for i in range(args.warmup):
-        st = step()
+        st = step(session)
         print(f"bench: warmup {i} {st.kernel_ms:.1f} ms", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     rays = 0
-    kernel_ms = 0.0
-    dispatches = 0
+    trace_ms = 0.0
+    trace_launches = 0
     for i in range(args.steps):
-        st = step()
+        st = step(session)
         rays += st.rays
-        kernel_ms += st.kernel_ms
-        dispatches += st.dispatches
-        print(f"bench: step {i} {st.kernel_ms:.1f} ms, {st.rays / 1e6:.1f} Mrays", file=sys.stderr,
-              flush=True)
+        trace_ms += st.trace_ms
+        trace_launches += st.trace_launches
+        print(f"bench: step {i} {st.kernel_ms:.1f} ms (trace {st.trace_ms:.1f} ms), {st.rays / 1e6:.1f} Mrays",
+              file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     samples = pixels * spp * args.steps
     value = samples / wall
     mrays = rays / wall / 1e6
-    launch_ms = kernel_ms / max(dispatches, 1)
-    rays_per_launch = rays / max(dispatches, 1)
-    samples_per_launch = spp * args.steps / max(dispatches, 1)
-    bytes_per_launch = rays_per_launch * BYTES_PER_RAY + pixels * FILM_BYTES_PER_PIXEL_DISPATCH
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    trace_launch_ms = trace_ms / max(trace_launches, 1)
+    rays_per_launch = rays / max(trace_launches, 1)
+    achieved = rays_per_launch * BYTES_PER_RAY / (trace_launch_ms * 1e-3) / 1e9
     result = {
         "metric": METRIC,
         "value": value,
@@ -138,8 +144,8 @@ def main() -> int:
             "workload": "instanced icospheres: 1,003,520-tri displaced icosphere x 64 instances, "
                         "mix(diffuse, transparent) materials, ground quad, constant background; "
                         f"1920x1080, {spp} spp, max 8 bounces, seed 0",
-            "integrator": "megakernel",
-            "samples_per_dispatch": samples_per_launch,
+            "integrator": "wavefront (batched)",
+            "samples_per_dispatch": SAMPLES_PER_DISPATCH,
         },
         "mrays_per_s": mrays,
         "rays_per_sample": rays / samples,
@@ -150,11 +156,35 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "render_kernel",
-            "launch_ms": launch_ms,
+            "kernel": "wf_trace_kernel",
+            "launch_ms": trace_launch_ms,
+            "launches": trace_launches,
             "bytes_per_ray": BYTES_PER_RAY,
         },
     }
+    if not args.no_plain:
+        # the plain per-request path on one frame of the same workload
+        plain = RendererBackend(0, integrator=abi.INTEGRATOR_MEGAKERNEL, max_samples_per_dispatch=2)
+        pcomp = plain.compile_scene(snap)
+        psess = plain.create_session(pcomp.scene, settings)
+        step(psess)
+        t1 = time.perf_counter()
+        pst = step(psess)
+        pwall = time.perf_counter() - t1
+        film_bytes = pixels * FILM_BYTES_PER_PIXEL_DISPATCH
+        pl_ms = pst.kernel_ms / max(pst.dispatches, 1)
+        pl_achieved = (pst.rays / max(pst.dispatches, 1) * BYTES_PER_RAY + film_bytes) / (pl_ms * 1e-3) / 1e9
+        result["plain"] = {
+            "integrator": "megakernel",
+            "samples_per_s": pixels * spp / pwall,
+            "mrays_per_s": pst.rays / pwall / 1e6,
+            "ms_per_step": pwall * 1e3,
+            "launch_ms": pl_ms,
+            "roofline_frac": pl_achieved / HBM_PEAK_GBS,
+        }
+        result["batched_vs_plain"] = value / result["plain"]["samples_per_s"]
+        print(f"bench: plain megakernel frame {pwall * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        del psess, pcomp, plain
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(snap, settings)

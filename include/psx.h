@@ -304,11 +304,20 @@ typedef struct psx_session psx_session;
 
 /* Backend options (luisa_backend::LuisaPathTracerOptions,
  * include/psycles/luisa/path_tracer.h:45-59). */
+enum psx_integrator {
+    PSX_INTEGRATOR_MEGAKERNEL = 0, /* one lane runs whole paths (plain path)  */
+    PSX_INTEGRATOR_WAVEFRONT = 1   /* trace/shade kernels over HBM path state */
+};
+
+enum psx_backend_flags {
+    PSX_BACKEND_COUNT_TRAVERSAL = 1u /* count node visits and triangle tests */
+};
+
 typedef struct psx_backend_options {
     uint32_t next_event_estimation;  /* default 1                           */
     uint32_t max_samples_per_dispatch; /* default 4; batches kept in order  */
-    uint32_t integrator;             /* 0 = megakernel, 1 = wavefront       */
-    uint32_t reserved;
+    uint32_t integrator;             /* enum psx_integrator                 */
+    uint32_t flags;                  /* enum psx_backend_flags              */
 } psx_backend_options;
 
 const char *psx_version(void);
@@ -351,6 +360,8 @@ typedef struct psx_render_stats {
     double kernel_ms;         /* device time of the render kernels          */
     uint64_t node_visits;     /* BVH node records fetched (0 if disabled)   */
     uint64_t triangle_tests;  /* triangle records tested (0 if disabled)    */
+    double trace_ms;          /* wavefront: device time of the trace kernel */
+    uint64_t trace_launches;  /* wavefront: trace kernel launches           */
 } psx_render_stats;
 
 /* Statistics of the most recent psx_session_render. */

@@ -75,6 +75,10 @@ LIGHT_AREA = 2
 LIGHT_DISTANT = 3
 LIGHT_BACKGROUND = 4
 
+INTEGRATOR_MEGAKERNEL = 0
+INTEGRATOR_WAVEFRONT = 1
+BACKEND_COUNT_TRAVERSAL = 1
+
 TRACE_EVENTS = 8
 TRACE_WORDS = 8
 TRACE_FLAGS = 0
@@ -180,13 +184,14 @@ class Sink(C.Structure):
 class BackendOptions(C.Structure):
     _fields_ = [("next_event_estimation", u32),
                 ("max_samples_per_dispatch", u32), ("integrator", u32),
-                ("reserved", u32)]
+                ("flags", u32)]
 
 
 class RenderStats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("samples", C.c_uint64),
                 ("dispatches", C.c_uint64), ("kernel_ms", C.c_double),
-                ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64)]
+                ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64),
+                ("trace_ms", C.c_double), ("trace_launches", C.c_uint64)]
 
 
 def default_settings(width: int, height: int) -> RenderSettings:

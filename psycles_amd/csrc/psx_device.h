@@ -1,9 +1,11 @@
 // psx_device.h - device record layouts of the MI355X render path.
 //
 // Shared by the host scene compiler (psx_host.cpp) and the HIP kernels
-// (psx_kernels.hip).  Every record is sized for coalesced HBM fetches:
+// (psx_path.h, psx_kernels.hip, psx_wavefront.hip).  Every record is sized for coalesced HBM fetches:
 // BVH4 nodes are one 128-byte line, leaf triangles are three float4 (48 B).
 #pragma once
+
+#include <hip/hip_vector_types.h>
 
 #include <cstdint>
 
@@ -16,7 +18,8 @@ namespace psx {
 // 4-wide BVH node, one 128-byte line.  Child boxes are stored structure-of-
 // arrays so one node fetch feeds four slab tests.  count[i]:
 //   0            inner child, child[i] = node index
-//   1..         leaf with count[i] items starting at child[i]
+//   1..          leaf: BLAS nodes hold count[i] triangles starting at
+//                child[i]; TLAS nodes hold one instance, child[i] = instance
 //   kEmptySlot   unused slot
 struct alignas(16) Node4 {
     float lo_x[4], hi_x[4];
@@ -70,9 +73,11 @@ struct Mesh {
 };
 constexpr uint32_t kNone = 0xffffffffu;
 
-// entries of the per-lane LDS traversal stack; scene compilation rejects trees
-// whose worst-case stack use exceeds it.
+// entries of the per-lane traversal stack (LDS short stack plus HBM spill);
+// scene compilation rejects trees whose worst-case stack use exceeds it.
 constexpr uint32_t kTraversalStackDepth = 64u;
+// leaf stack entries pack (first triangle << 3 | count - 1) under two kind bits
+constexpr uint32_t kMaxLeafTriangles = 1u << 27;
 
 // ---------------------------------------------------------------------------
 // materials: a SurfaceProgram lowered to a flat leaf list.  A leaf's mix
@@ -126,7 +131,6 @@ struct DistEntry {
 struct SceneView {
     const Node4 *nodes;
     const LeafTri *tris;
-    const uint32_t *tlas_items;     // TLAS leaf item -> instance
     const Instance *instances;
     const Mesh *meshes;
     const float *positions;         // float3 array (object + applied world)
@@ -141,6 +145,8 @@ struct SceneView {
     const float *params;
     const Emitter *emitters;
     const DistEntry *distribution;
+    const float4 *material_shadow;    // per material: transparent extinction for shadow rays
+    const float4 *material_emission;  // per material: constant emission
     uint32_t tlas_root;
     uint32_t instance_count;
     uint32_t material_count;
@@ -180,6 +186,42 @@ struct Counters {
     unsigned long long nodes;
     unsigned long long tris;
     unsigned long long pad;
+};
+
+// ---------------------------------------------------------------------------
+// wavefront integrator: per-path state in HBM, structure of arrays indexed by
+// path = sample_in_batch * padded_pixels + tile * 64 + lane (8x8 tiles)
+// ---------------------------------------------------------------------------
+struct WavefrontCounters {
+    uint32_t trace_count;   // entries in this iteration's trace queue
+    uint32_t trace_fetch;   // work-fetch cursor of the persistent trace kernel
+    uint32_t shade_count;   // paths in this iteration's shade list
+    uint32_t pad;
+};
+
+struct WavefrontView {
+    float4 *ray_o;       // origin.xyz, tmin
+    float4 *ray_d;       // direction.xyz, tmax
+    uint4 *aux;          // source object, source primitive, step, bits
+    float4 *throughput;  // xyz, previous BSDF pdf
+    float4 *radiance;    // xyz, transparency
+    float4 *normal;      // primary normal AOV
+    float4 *albedo;      // primary albedo AOV
+    uint4 *state0;       // flag, visibility, bounce, diffuse bounce
+    uint4 *state1;       // glossy, transmission, transparent bounce, rng offset
+    uint2 *ident;        // sample index, pixel rng hash
+    uint4 *hit;          // instance (kNone = miss), primitive, Cycles primitive, t bits
+    float2 *hit_uv;
+    float4 *shadow_o;    // origin.xyz, tmax
+    float4 *shadow_d;    // direction.xyz, clamp depth bits
+    uint4 *shadow_q;     // source object, source primitive, light object, light primitive
+    float4 *shadow_l;    // light estimate xyz, transparent depth bits
+    float4 *shadow_t;    // transmittance xyz
+    uint32_t *queue[2];  // trace entries: path << 1 | is_shadow
+    uint32_t *list[2];   // shade entries: path
+    WavefrontCounters *counters;  // one per iteration
+    uint32_t capacity;            // paths
+    uint32_t padded_pixels;       // 64 * tiles of the window
 };
 
 }  // namespace psx

@@ -27,15 +27,26 @@
 
 namespace psx {
 
+// psx_kernels.hip
+uint32_t megakernel_lanes();
 hipError_t launch_render(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
                          const FilmView &film, uint32_t first, uint32_t count, Counters *counters, bool count_nodes,
-                         hipStream_t stream);
+                         uint32_t *spill, uint32_t *cursor, hipStream_t stream);
 hipError_t launch_trace(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
-                        uint32_t sample, uint32_t *records, hipStream_t stream);
+                        uint32_t sample, uint32_t *records, uint32_t *spill, hipStream_t stream);
 hipError_t launch_primary(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
-                          uint32_t sample, uint32_t *records, hipStream_t stream);
+                          uint32_t sample, uint32_t *records, uint32_t *spill, hipStream_t stream);
 hipError_t launch_sampler(const KernelParams &k, const float4 *sobol, uint32_t sample, uint32_t first_dim,
                           uint32_t dim_count, uint32_t *out, hipStream_t stream);
+hipError_t launch_material_tables(const SceneView &s, float4 *shadow, float4 *emission, uint32_t count,
+                                  hipStream_t stream);
+// psx_wavefront.hip
+uint32_t wavefront_lanes();
+uint32_t wavefront_padded_pixels(const KernelParams &k);
+hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
+                            const FilmView &film, const WavefrontView &w, uint32_t first, uint32_t count,
+                            Counters *counters, bool count_nodes, uint32_t *spill, hipEvent_t *events,
+                            hipStream_t stream);
 
 namespace {
 
@@ -400,13 +411,28 @@ struct Scene {
     uint32_t max_stack = 0;
     uint32_t tlas_depth = 0, blas_depth = 0;
     // device buffers
-    DeviceBuffer nodes, tris, tlas_items, instances_buf, meshes_buf, positions, indices, normals, smooth, slots,
-        tri_slots, overrides_buf, materials_buf, leaves, params, emitters, distribution;
+    DeviceBuffer nodes, tris, instances_buf, meshes_buf, positions, indices, normals, smooth, slots, tri_slots,
+        overrides_buf, materials_buf, leaves, params, emitters, distribution, material_shadow, material_emission;
     SceneView view{};
     uint64_t triangles = 0;
 
     void rebuild_emitters();
+    void refresh_material_tables();
 };
+
+// per-material shadow extinction and constant emission, evaluated on the
+// device by the same closure code the kernels run
+void Scene::refresh_material_tables() {
+    const uint32_t n = static_cast<uint32_t>(gpu_materials.size());
+    if (material_shadow.bytes() < std::max<size_t>(n * sizeof(float4), 16u)) {
+        material_shadow.allocate(n * sizeof(float4));
+        material_emission.allocate(n * sizeof(float4));
+    }
+    view.material_shadow = material_shadow.as<float4>();
+    view.material_emission = material_emission.as<float4>();
+    hip_check(launch_material_tables(view, material_shadow.as<float4>(), material_emission.as<float4>(), n, nullptr));
+    hip_check(hipDeviceSynchronize());
+}
 
 // emissive triangles (src/luisa/path_tracer_scene.cpp:1548-1640)
 void Scene::rebuild_emitters() {
@@ -453,23 +479,87 @@ void Scene::rebuild_emitters() {
     view.triangle_area_pdf = tri_pdf;
 }
 
+// Wavefront path state of one session (psx_device.h, WavefrontView), sized
+// for max_samples_per_dispatch samples of every window pixel.
+struct WavefrontState {
+    DeviceBuffer ray_o, ray_d, aux, throughput, radiance, normal, albedo, state0, state1, ident, hit, hit_uv,
+        shadow_o, shadow_d, shadow_q, shadow_l, shadow_t, queue0, queue1, list0, list1, counters;
+    WavefrontView view{};
+
+    void allocate(uint32_t capacity, uint32_t padded_pixels, uint32_t iterations) {
+        const size_t n = capacity;
+        ray_o.allocate(n * 16u);
+        ray_d.allocate(n * 16u);
+        aux.allocate(n * 16u);
+        throughput.allocate(n * 16u);
+        radiance.allocate(n * 16u);
+        normal.allocate(n * 16u);
+        albedo.allocate(n * 16u);
+        state0.allocate(n * 16u);
+        state1.allocate(n * 16u);
+        ident.allocate(n * 8u);
+        hit.allocate(n * 16u);
+        hit_uv.allocate(n * 8u);
+        shadow_o.allocate(n * 16u);
+        shadow_d.allocate(n * 16u);
+        shadow_q.allocate(n * 16u);
+        shadow_l.allocate(n * 16u);
+        shadow_t.allocate(n * 16u);
+        // a live path queues at most a continuation ray and a shadow ray
+        queue0.allocate(n * 8u);
+        queue1.allocate(n * 8u);
+        list0.allocate(n * 4u);
+        list1.allocate(n * 4u);
+        counters.allocate((iterations + 1u) * sizeof(WavefrontCounters));
+        WavefrontView &v = view;
+        v.ray_o = ray_o.as<float4>();
+        v.ray_d = ray_d.as<float4>();
+        v.aux = aux.as<uint4>();
+        v.throughput = throughput.as<float4>();
+        v.radiance = radiance.as<float4>();
+        v.normal = normal.as<float4>();
+        v.albedo = albedo.as<float4>();
+        v.state0 = state0.as<uint4>();
+        v.state1 = state1.as<uint4>();
+        v.ident = ident.as<uint2>();
+        v.hit = hit.as<uint4>();
+        v.hit_uv = hit_uv.as<float2>();
+        v.shadow_o = shadow_o.as<float4>();
+        v.shadow_d = shadow_d.as<float4>();
+        v.shadow_q = shadow_q.as<uint4>();
+        v.shadow_l = shadow_l.as<float4>();
+        v.shadow_t = shadow_t.as<float4>();
+        v.queue[0] = queue0.as<uint32_t>();
+        v.queue[1] = queue1.as<uint32_t>();
+        v.list[0] = list0.as<uint32_t>();
+        v.list[1] = list1.as<uint32_t>();
+        v.counters = counters.as<WavefrontCounters>();
+        v.capacity = capacity;
+        v.padded_pixels = padded_pixels;
+    }
+};
+
 struct Session {
     Scene *scene = nullptr;
     psx_render_settings settings{};
     std::vector<uint32_t> passes, pass_channels;
     KernelParams params{};
     uint32_t window_w = 0, window_h = 0;
-    DeviceBuffer combined, normal, albedo, sample_count, sobol, filter, counters;
+    DeviceBuffer combined, normal, albedo, sample_count, sobol, filter, counters, spill, tile_cursor;
+    WavefrontState wavefront;
+    uint32_t wavefront_iterations = 0;
     uint32_t sobol_size = 0;
     uint32_t total_aa = 0;
     uint32_t rendered = 0;
     std::atomic<bool> cancelled{false};
     hipStream_t stream = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    std::vector<hipEvent_t> trace_events;  // pairs around wavefront trace launches
     psx_render_stats stats{};
     bool count_nodes = false;
 
     ~Session() {
+        for (hipEvent_t e : trace_events) (void)hipEventDestroy(e);
         if (ev_begin) (void)hipEventDestroy(ev_begin);
         if (ev_end) (void)hipEventDestroy(ev_end);
         if (stream) (void)hipStreamDestroy(stream);
@@ -579,20 +669,60 @@ bool valid_settings(const psx_render_settings *rs) {
 // Runs samples [first, first + count) as ordered batches of at most
 // max_samples_per_dispatch (SampleDispatchPartition::make,
 // src/luisa/sample_dispatch_partition.h:134-235).
+// Wavefront buffers for `per` samples per dispatch; kept across dispatches
+// and sessions resets while the window and limits fit.
+void prepare_wavefront(Session &s, uint32_t per) {
+    const uint32_t padded = wavefront_padded_pixels(s.params);
+    const uint64_t want = static_cast<uint64_t>(padded) * per;
+    if (want > 0x7fffffffull) throw DeviceError{PSX_ERROR_INVALID_ARGUMENT};
+    const uint32_t iterations = s.params.max_path_steps + 1u;
+    const WavefrontView &v = s.wavefront.view;
+    if (v.capacity < want || v.padded_pixels != padded || s.wavefront_iterations < iterations) {
+        s.wavefront.allocate(static_cast<uint32_t>(want), padded, iterations);
+        s.wavefront_iterations = iterations;
+    }
+    const size_t events = 2u * static_cast<size_t>(iterations);
+    while (s.trace_events.size() < events) {
+        hipEvent_t e = nullptr;
+        hip_check(hipEventCreate(&e));
+        s.trace_events.push_back(e);
+    }
+}
+
 int dispatch_samples(Session &s, uint32_t first, uint32_t count) {
-    const uint32_t per = std::max(s.scene->backend->options.max_samples_per_dispatch, 1u);
+    const psx_backend_options &opt = s.scene->backend->options;
+    const uint32_t per = std::max(opt.max_samples_per_dispatch, 1u);
+    const bool wavefront = opt.integrator == PSX_INTEGRATOR_WAVEFRONT;
+    if (wavefront) prepare_wavefront(s, std::min(per, count));
     hip_check(hipMemsetAsync(s.counters.as<void>(), 0, sizeof(Counters), s.stream));
     FilmView film{s.combined.as<float>(), s.normal.as<float>(), s.albedo.as<float>(), s.sample_count.as<uint32_t>()};
     hip_check(hipEventRecord(s.ev_begin, s.stream));
     uint64_t dispatches = 0;
+    double trace_ms = 0.0;
+    uint64_t trace_launches = 0;
     for (uint32_t done = 0; done < count;) {
         if (s.cancelled.load()) {
             hip_check(hipStreamSynchronize(s.stream));
             return PSX_ERROR_CANCELLED;
         }
         const uint32_t n = std::min(per, count - done);
-        hip_check(launch_render(s.scene->view, s.params, s.sobol.as<float4>(), s.filter.as<float>(), film,
-                                first + done, n, s.counters.as<Counters>(), s.count_nodes, s.stream));
+        if (wavefront) {
+            hip_check(launch_wavefront(s.scene->view, s.params, s.sobol.as<float4>(), s.filter.as<float>(), film,
+                                       s.wavefront.view, first + done, n, s.counters.as<Counters>(), s.count_nodes,
+                                       s.spill.as<uint32_t>(), s.trace_events.data(), s.stream));
+            // the event pool is reused by the next dispatch: read it now
+            hip_check(hipStreamSynchronize(s.stream));
+            for (uint32_t it = 0; it < s.params.max_path_steps + 1u; ++it) {
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, s.trace_events[2u * it], s.trace_events[2u * it + 1u]));
+                trace_ms += ms;
+                ++trace_launches;
+            }
+        } else {
+            hip_check(launch_render(s.scene->view, s.params, s.sobol.as<float4>(), s.filter.as<float>(), film,
+                                    first + done, n, s.counters.as<Counters>(), s.count_nodes, s.spill.as<uint32_t>(),
+                                    s.tile_cursor.as<uint32_t>(), s.stream));
+        }
         ++dispatches;
         done += n;
     }
@@ -608,6 +738,8 @@ int dispatch_samples(Session &s, uint32_t first, uint32_t count) {
     s.stats.samples = static_cast<uint64_t>(count) * s.window_w * s.window_h;
     s.stats.dispatches = dispatches;
     s.stats.kernel_ms = ms;
+    s.stats.trace_ms = trace_ms;
+    s.stats.trace_launches = trace_launches;
     s.rendered += count;
     return PSX_OK;
 }
@@ -910,8 +1042,10 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         // inner child indices point at base + local index; local 0 moves to slot 0
         std::vector<Node4> tn = t.nodes;
         for (auto &n : tn)
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i) {
                 if (n.count[i] == 0u) n.child[i] -= 1u;  // local k >= 1 lives at base + k - 1
+                else if (n.count[i] != kEmptySlot) n.child[i] = tlas_leaf_items[n.child[i]];  // one instance
+            }
         nodes[0] = tn[0];
         nodes.insert(nodes.end(), tn.begin() + 1, tn.end());
         tlas_depth = t.depth;
@@ -921,6 +1055,9 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     // stack bound: a TLAS node pushes at most four entries (instance markers
     // and inner children), a BLAS node at most three
     scene->max_stack = 4u * tlas_depth + 3u * max_blas_depth + 2u;
+    if (tris.size() >= kMaxLeafTriangles || nodes.size() >= (1ull << 30) || d->instance_count >= (1u << 30))
+        return fail("scene exceeds the traversal entry encoding (" + std::to_string(tris.size()) +
+                    " leaf triangles, " + std::to_string(nodes.size()) + " nodes)");
     if (scene->max_stack > kTraversalStackDepth)
         return fail("BVH too deep for the traversal stack (" + std::to_string(scene->max_stack) + " > " +
                     std::to_string(kTraversalStackDepth) + ")");
@@ -928,7 +1065,6 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     Scene &s = *scene;
     s.nodes.upload(nodes);
     s.tris.upload(tris);
-    s.tlas_items.upload(tlas_leaf_items);
     s.instances_buf.upload(instances);
     s.meshes_buf.upload(scene->meshes);
     s.positions.upload(positions);
@@ -944,7 +1080,6 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     SceneView &v = s.view;
     v.nodes = s.nodes.as<Node4>();
     v.tris = s.tris.as<LeafTri>();
-    v.tlas_items = s.tlas_items.as<uint32_t>();
     v.instances = s.instances_buf.as<Instance>();
     v.meshes = s.meshes_buf.as<Mesh>();
     v.positions = s.positions.as<float>();
@@ -976,6 +1111,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     v.background[2] = bg[2];
     v.world_visibility = d->world_visibility_mask;
     s.rebuild_emitters();
+    s.refresh_material_tables();
     *out = scene.release();
     return PSX_OK;
 }
@@ -1094,6 +1230,7 @@ int psx_scene_update_parameters(psx_scene *handle, uint32_t material, const floa
         hip_check(hipMemcpy(s->params.as<float>() + g.param_offset, p, sizeof(float) * count, hipMemcpyHostToDevice));
         hip_check(hipMemcpy(s->materials_buf.as<Material>() + material, &g, sizeof(Material), hipMemcpyHostToDevice));
         if (emitted != m.may_emit) s->rebuild_emitters();
+        s->refresh_material_tables();
         return static_cast<int>(PSX_OK);
     });
 }
@@ -1110,6 +1247,11 @@ int psx_session_create(psx_scene *handle, const psx_render_settings *rs, psx_ses
         hip_check(hipStreamCreate(&s->stream));
         hip_check(hipEventCreate(&s->ev_begin));
         hip_check(hipEventCreate(&s->ev_end));
+        // traversal spill columns for every lane the traversal kernels keep resident
+        const size_t lanes = std::max(megakernel_lanes(), wavefront_lanes());
+        s->spill.allocate(lanes * kTraversalStackDepth * sizeof(uint32_t));
+        s->tile_cursor.allocate(sizeof(uint32_t));
+        s->count_nodes = (scene->backend->options.flags & PSX_BACKEND_COUNT_TRAVERSAL) != 0u;
         initialize_session(*s, *rs);
         *out = reinterpret_cast<psx_session *>(s.release());
         return static_cast<int>(PSX_OK);
@@ -1205,7 +1347,7 @@ int psx_session_primary_hits(psx_session *handle, uint32_t sample, uint32_t tota
         DeviceBuffer rec;
         rec.allocate(n * 16u);
         hip_check(launch_primary(s->scene->view, s->params, s->sobol.as<float4>(), s->filter.as<float>(), sample,
-                                 rec.as<uint32_t>(), s->stream));
+                                 rec.as<uint32_t>(), s->spill.as<uint32_t>(), s->stream));
         hip_check(hipStreamSynchronize(s->stream));
         hip_check(hipMemcpy(out, rec.as<void>(), n * 16u, hipMemcpyDeviceToHost));
         return static_cast<int>(PSX_OK);
@@ -1223,7 +1365,7 @@ int psx_session_trace_events(psx_session *handle, uint32_t sample, uint32_t tota
         DeviceBuffer rec;
         rec.allocate(words * 4u);
         hip_check(launch_trace(s->scene->view, s->params, s->sobol.as<float4>(), s->filter.as<float>(), sample,
-                               rec.as<uint32_t>(), s->stream));
+                               rec.as<uint32_t>(), s->spill.as<uint32_t>(), s->stream));
         hip_check(hipStreamSynchronize(s->stream));
         hip_check(hipMemcpy(out, rec.as<void>(), words * 4u, hipMemcpyDeviceToHost));
         return static_cast<int>(PSX_OK);

@@ -1,0 +1,1514 @@
+// psx_path.h - device code of the MI355X render path shared by the
+// megakernel and the wavefront integrator (gfx950).
+//
+// Camera rays, two-level BVH4 traversal with an LDS short stack and the exact
+// Cycles closest-hit contract, closure-tree shading, emissive-triangle NEE
+// with transparent shadows, Russian roulette and the path step.  Each stage
+// cites the reference file whose behaviour it reproduces.
+//
+// Floating point: compiled with -ffp-contract=off and correctly rounded
+// division/sqrt; fmaf appears exactly where the reference fuses.  Helper
+// conventions (bit patterns fixed by this implementation):
+//   dot(a, b)    = (a.x*b.x + a.y*b.y) + a.z*b.z
+//   normalize(v) = v * (1 / sqrt(dot(v, v)))
+//   M * (v, w)   = ((c0*x + c1*y) + c2*z) + c3*w
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "psx_device.h"
+
+#pragma clang fp contract(off)
+
+namespace psx {
+namespace {
+
+// ===========================================================================
+// vector algebra
+// ===========================================================================
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 V(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return V(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ f3 operator/(f3 a, f3 b) { return V(a.x / b.x, a.y / b.y, a.z / b.z); }
+__device__ __forceinline__ f3 neg(f3 a) { return V(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ f3 normalize(f3 a) {
+    const float inv = 1.0f / sqrtf(dot(a, a));
+    return a * inv;
+}
+__device__ __forceinline__ f3 sel(f3 a, f3 b, bool c) { return c ? b : a; }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ f3 vmax(f3 a, f3 b) {
+    return V(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z));
+}
+__device__ __forceinline__ f3 vclamp(f3 a, float lo, float hi) {
+    return V(clampf(a.x, lo, hi), clampf(a.y, lo, hi), clampf(a.z, lo, hi));
+}
+__device__ __forceinline__ bool any_nonzero(f3 a) { return a.x != 0.0f || a.y != 0.0f || a.z != 0.0f; }
+__device__ __forceinline__ bool any_nan(f3 a) { return isnan(a.x) || isnan(a.y) || isnan(a.z); }
+__device__ __forceinline__ f3 load3(const float *p) { return V(p[0], p[1], p[2]); }
+
+// safe_normalize (src/luisa/path_tracer_lighting.cpp:9-18)
+__device__ __forceinline__ f3 safe_normalize(f3 value, f3 fallback) {
+    const bool valid = dot(value, value) > 1.0e-20f;
+    f3 s = sel(fallback, value, valid);
+    const bool fallback_valid = dot(s, s) > 1.0e-20f;
+    s = sel(V(0.0f, 0.0f, 1.0f), s, fallback_valid);
+    return normalize(s);
+}
+
+// normalize_or (include/psycles/luisa/spherical_geometry.h:34-40)
+__device__ __forceinline__ f3 normalize_or(f3 value, f3 fallback) {
+    const float l2 = dot(value, value);
+    return l2 > 1.0e-20f ? value / sqrtf(fmaxf(l2, 1.0e-20f)) : fallback;
+}
+
+// cycles_transform::point / direction (src/luisa/cycles_transform.cpp:6-27)
+__device__ __forceinline__ f3 affine_point(const Affine &t, f3 v) {
+    const float *e = t.m;
+    return V(fmaf(v.x, e[0], fmaf(v.y, e[3], fmaf(v.z, e[6], e[9]))),
+             fmaf(v.x, e[1], fmaf(v.y, e[4], fmaf(v.z, e[7], e[10]))),
+             fmaf(v.x, e[2], fmaf(v.y, e[5], fmaf(v.z, e[8], e[11]))));
+}
+__device__ __forceinline__ f3 affine_dir(const Affine &t, f3 v) {
+    const float *e = t.m;
+    return V(fmaf(v.x, e[0], fmaf(v.y, e[3], fmaf(v.z, e[6], 0.0f))),
+             fmaf(v.x, e[1], fmaf(v.y, e[4], fmaf(v.z, e[7], 0.0f))),
+             fmaf(v.x, e[2], fmaf(v.y, e[5], fmaf(v.z, e[8], 0.0f))));
+}
+// Luisa float4x4 * float4 with w in {0, 1}
+__device__ __forceinline__ f3 mat_mul(const Affine &t, f3 v, float w) {
+    const float *e = t.m;
+    return V(((e[0] * v.x + e[3] * v.y) + e[6] * v.z) + e[9] * w,
+             ((e[1] * v.x + e[4] * v.y) + e[7] * v.z) + e[10] * w,
+             ((e[2] * v.x + e[5] * v.y) + e[8] * v.z) + e[11] * w);
+}
+__device__ __forceinline__ f3 cam_point(const float *m, f3 v) {
+    return V(fmaf(v.x, m[0], fmaf(v.y, m[4], fmaf(v.z, m[8], m[12]))),
+             fmaf(v.x, m[1], fmaf(v.y, m[5], fmaf(v.z, m[9], m[13]))),
+             fmaf(v.x, m[2], fmaf(v.y, m[6], fmaf(v.z, m[10], m[14]))));
+}
+__device__ __forceinline__ f3 cam_dir(const float *m, f3 v) {
+    return V(fmaf(v.x, m[0], fmaf(v.y, m[4], fmaf(v.z, m[8], 0.0f))),
+             fmaf(v.x, m[1], fmaf(v.y, m[5], fmaf(v.z, m[9], 0.0f))),
+             fmaf(v.x, m[2], fmaf(v.y, m[6], fmaf(v.z, m[10], 0.0f))));
+}
+
+// ===========================================================================
+// sampler (include/psycles/luisa/cycles_sampler.h:36-271)
+// ===========================================================================
+__device__ __forceinline__ uint32_t reverse_bits(uint32_t v) { return __brev(v); }
+
+__device__ __forceinline__ uint32_t reversed_bit_owen(uint32_t v, uint32_t seed) {
+    v ^= v * 0x3d20adeau;
+    v += seed;
+    v *= (seed >> 16u) | 1u;
+    v ^= v * 0x05526c56u;
+    v ^= v * 0x53a22864u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t hash_wang_seeded(uint32_t v, uint32_t seed) {
+    v = (v ^ 61u) ^ seed;
+    v += v << 3u;
+    v ^= v >> 4u;
+    v *= 0x27d4eb2du;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t pattern_index(uint32_t dimension, uint32_t seed) {
+    uint32_t v = dimension & 0xffu;
+    v ^= seed;
+    v *= 0xe170893du;
+    v ^= seed >> 16u;
+    v ^= (v & 0xffu) >> 4u;
+    v ^= seed >> 8u;
+    v *= 0x0929eb3fu;
+    v ^= seed >> 23u;
+    v ^= (v & 0xffu) >> 1u;
+    v *= 1u | (seed >> 27u);
+    v *= 0x6935fa69u;
+    v ^= (v & 0xffu) >> 11u;
+    v *= 0x74dcb303u;
+    v ^= (v & 0xffu) >> 2u;
+    v *= 0x9e501cc3u;
+    v ^= (v & 0xffu) >> 2u;
+    v *= 0xc860a3dfu;
+    v &= 0xffu;
+    v ^= v >> 5u;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t pixel_hash(uint32_t x, uint32_t y, uint32_t seed) {
+    const uint32_t qx = 1103515245u * ((x >> 1u) ^ y);
+    const uint32_t qy = 1103515245u * ((y >> 1u) ^ x);
+    const uint32_t h = 1103515245u * (qx ^ (qy >> 3u));
+    return h ^ seed;
+}
+
+__device__ __forceinline__ uint32_t shuffled_index(uint32_t sample, uint32_t dimension, uint32_t seed,
+                                                   uint32_t size) {
+    const uint32_t pattern = pattern_index(dimension, seed);
+    const uint32_t mask = size - 1u;
+    const uint32_t shuffled =
+        reverse_bits(reversed_bit_owen(reverse_bits(sample), hash_wang_seeded(dimension, seed)));
+    sample = (sample & ~mask) | (shuffled & mask);
+    return ((pattern * size) + sample) % (size * 256u);
+}
+
+__device__ __forceinline__ float4 sample4(const float4 *table, uint32_t size, uint32_t sample,
+                                          uint32_t hash, uint32_t dimension) {
+    return table[shuffled_index(sample, dimension, hash, size)];
+}
+
+// ===========================================================================
+// path state constants (include/psycles/luisa/cycles_path_state.h)
+// ===========================================================================
+constexpr uint32_t FLAG_REFLECT = 1u << 0, FLAG_SINGULAR = 1u << 1, FLAG_TRANSPARENT = 1u << 2,
+                   FLAG_IMPORTANCE_BAKE = 1u << 3, FLAG_DIFFUSE_ANCESTOR = 1u << 4,
+                   FLAG_MIS_HAD_TRANSMISSION = 1u << 6, FLAG_MIS_SKIP = 1u << 7,
+                   FLAG_SINGLE_PASS_DONE = 1u << 8, FLAG_TRANSPARENT_BACKGROUND = 1u << 9,
+                   FLAG_TERMINATE_ON_NEXT_SURFACE = 1u << 10,
+                   FLAG_TERMINATE_AFTER_TRANSPARENT = 1u << 12, FLAG_SURFACE_PASS = 1u << 18,
+                   FLAG_ANY_PASS = (1u << 18) | (1u << 19);
+constexpr uint32_t CVIS_CAMERA = 1u, CVIS_TRANSMIT = 2u, CVIS_DIFFUSE = 4u, CVIS_GLOSSY = 8u,
+                   CVIS_VOLUME_SCATTER = 16u;
+constexpr uint32_t LABEL_TRANSMIT = 1u, LABEL_REFLECT = 2u, LABEL_DIFFUSE = 4u, LABEL_GLOSSY = 8u,
+                   LABEL_SINGULAR = 16u, LABEL_TRANSPARENT = 32u, LABEL_TRANSMIT_TRANSPARENT = 128u,
+                   LABEL_RAY_PORTAL = 512u;
+constexpr uint32_t RT_BACKFACING = 1u, RT_BSDF = 8u, RT_BSDF_HAS_EVAL = 16u, RT_TRANSPARENT = 1024u,
+                   RT_BSDF_HAS_TRANSMISSION = 2048u, RT_RAY_PORTAL = 4096u;
+constexpr uint32_t EV_DIFFUSE = 1u, EV_GLOSSY = 2u, EV_SINGULAR = 4u, EV_REFLECTION = 8u,
+                   EV_TRANSMISSION = 16u, EV_TRANSPARENT = 32u;
+constexpr uint32_t VIS_CAMERA = 1u, VIS_DIFFUSE = 2u, VIS_GLOSSY = 4u, VIS_TRANSMISSION = 8u,
+                   VIS_SHADOW = 16u, VIS_VOLUME_SCATTER = 32u;
+constexpr uint32_t MESH_LIGHT_VISIBILITY = VIS_DIFFUSE | VIS_GLOSSY | VIS_TRANSMISSION | VIS_VOLUME_SCATTER;
+
+struct PathState {
+    uint32_t flag, visibility, bounce, diffuse_bounce, glossy_bounce, transmission_bounce,
+        transparent_bounce, rng_offset;
+};
+
+// next_surface (cycles_path_state.h:205-362)
+__device__ __forceinline__ void next_surface(PathState &s, uint32_t label, uint32_t runtime_flag,
+                                             const KernelParams &k) {
+    const bool transparent = (label & (LABEL_TRANSPARENT | LABEL_RAY_PORTAL)) != 0u;
+    if (transparent) {
+        s.transparent_bounce += 1u;
+        s.rng_offset += 16u;
+        s.flag |= FLAG_TRANSPARENT;
+        if (s.transparent_bounce >= k.transparent_max) s.flag |= FLAG_TERMINATE_ON_NEXT_SURFACE;
+        if ((runtime_flag & RT_RAY_PORTAL) != 0u) s.flag |= FLAG_MIS_SKIP;
+        return;
+    }
+    s.bounce += 1u;
+    s.rng_offset += 16u;
+    if (s.bounce >= k.max_bounces) s.flag |= FLAG_TERMINATE_AFTER_TRANSPARENT;
+    s.visibility = 0u;
+    s.flag &= ~(FLAG_REFLECT | FLAG_SINGULAR | FLAG_TRANSPARENT | FLAG_IMPORTANCE_BAKE | FLAG_MIS_SKIP |
+                FLAG_MIS_HAD_TRANSMISSION);
+    const bool reflection = (label & LABEL_REFLECT) != 0u;
+    const bool diffuse = (label & LABEL_DIFFUSE) != 0u;
+    const bool glossy = (label & LABEL_GLOSSY) != 0u;
+    const bool transmit_transparent = (label & LABEL_TRANSMIT_TRANSPARENT) != 0u;
+    if (reflection) s.flag |= FLAG_REFLECT;
+    if (reflection || !transmit_transparent) s.flag &= ~FLAG_TRANSPARENT_BACKGROUND;
+    const bool diffuse_reflection = reflection && diffuse;
+    const bool glossy_reflection = reflection && !diffuse;
+    const bool transmission = !reflection;
+    if (diffuse_reflection) s.diffuse_bounce += 1u;
+    if (glossy_reflection) s.glossy_bounce += 1u;
+    if (transmission) s.transmission_bounce += 1u;
+    if (diffuse_reflection && s.diffuse_bounce >= k.max_diffuse) s.flag |= FLAG_TERMINATE_AFTER_TRANSPARENT;
+    if (glossy_reflection && s.glossy_bounce >= k.max_glossy) s.flag |= FLAG_TERMINATE_AFTER_TRANSPARENT;
+    if (transmission && s.transmission_bounce >= k.max_transmission)
+        s.flag |= FLAG_TERMINATE_AFTER_TRANSPARENT;
+    if (transmission) s.visibility |= CVIS_TRANSMIT;
+    s.visibility |= diffuse ? CVIS_DIFFUSE : CVIS_GLOSSY;
+    if (diffuse) s.flag |= FLAG_DIFFUSE_ANCESTOR;
+    if (!diffuse && !glossy) s.flag |= FLAG_SINGULAR | FLAG_MIS_SKIP;
+    if ((runtime_flag & RT_BSDF_HAS_TRANSMISSION) != 0u) s.flag |= FLAG_MIS_HAD_TRANSMISSION;
+    if ((s.flag & FLAG_ANY_PASS) == 0u && (s.flag & FLAG_TRANSPARENT_BACKGROUND) == 0u)
+        s.flag |= FLAG_SURFACE_PASS;
+}
+
+// contract_visibility (cycles_path_state.h:448-489)
+__device__ __forceinline__ uint32_t contract_visibility(uint32_t v) {
+    const uint32_t t = (v & CVIS_TRANSMIT) ? (v & ~(CVIS_DIFFUSE | CVIS_GLOSSY)) : v;
+    uint32_t r = 0u;
+    if (t & CVIS_CAMERA) r |= VIS_CAMERA;
+    if (t & CVIS_TRANSMIT) r |= VIS_TRANSMISSION;
+    if (t & CVIS_DIFFUSE) r |= VIS_DIFFUSE;
+    if (t & CVIS_GLOSSY) r |= VIS_GLOSSY;
+    if (t & CVIS_VOLUME_SCATTER) r |= VIS_VOLUME_SCATTER;
+    return r;
+}
+
+__device__ __forceinline__ uint32_t label_from_events(uint32_t e) {
+    uint32_t l = 0u;
+    if (e & EV_TRANSMISSION) l |= LABEL_TRANSMIT;
+    if (e & EV_REFLECTION) l |= LABEL_REFLECT;
+    if (e & EV_DIFFUSE) l |= LABEL_DIFFUSE;
+    if (e & EV_GLOSSY) l |= LABEL_GLOSSY;
+    if (e & EV_SINGULAR) l |= LABEL_SINGULAR;
+    if (e & EV_TRANSPARENT) l |= LABEL_TRANSPARENT;
+    return l;
+}
+
+// ===========================================================================
+// traversal: two-level BVH4 with an LDS short stack, exact Cycles closest-hit
+// contract (src/luisa/path_kernel_scene_traversal.cpp:34-68, 212-493;
+//  src/luisa/cycles_triangle_intersection_component.cpp:14-108)
+// ===========================================================================
+constexpr int kStackDepth = static_cast<int>(kTraversalStackDepth);
+constexpr uint32_t kShortStack = 16u;  // LDS entries per lane; deeper entries spill to HBM
+
+// Stack entries carry their kind in the two top bits.
+constexpr uint32_t kKindMask = 0xc0000000u;
+constexpr uint32_t kKindBlas = 0x00000000u;      // BLAS inner node index
+constexpr uint32_t kKindTlas = 0x40000000u;      // TLAS inner node index
+constexpr uint32_t kKindLeaf = 0x80000000u;      // first triangle << 3 | (count - 1)
+constexpr uint32_t kKindInstance = 0xc0000000u;  // instance index
+
+struct Ray {
+    f3 o, d;
+    float tmin, tmax;
+};
+
+struct Hit {
+    uint32_t inst, prim, object, cprim;
+    float t, u, v;
+    bool valid;
+};
+
+// Per-lane traversal stack: the top kShortStack entries live in LDS laid out
+// [entry][lane] (bank-conflict free), older entries move to a per-lane column
+// of an HBM spill buffer laid out [entry][lane of the grid].  Scene
+// compilation bounds the depth by kTraversalStackDepth.
+struct Stack {
+    uint32_t *lds;    // block base of the LDS ring
+    uint32_t *spill;  // this lane's spill column
+    uint32_t lanes;   // LDS row stride (lanes per block)
+    uint32_t lane;    // lane within the block
+    uint32_t stride;  // spill row stride (lanes of the grid)
+    int sp, bottom;   // entries [bottom, sp) are in LDS, [0, bottom) spilled
+
+    __device__ __forceinline__ void push(uint32_t e) {
+        if (sp >= kStackDepth) return;
+        if (sp - bottom == static_cast<int>(kShortStack)) {
+            spill[static_cast<size_t>(bottom) * stride] = lds[(bottom & (kShortStack - 1u)) * lanes + lane];
+            ++bottom;
+        }
+        lds[(sp & (kShortStack - 1u)) * lanes + lane] = e;
+        ++sp;
+    }
+    __device__ __forceinline__ uint32_t pop() {
+        --sp;
+        if (sp < bottom) {
+            bottom = sp;
+            return spill[static_cast<size_t>(sp) * stride];
+        }
+        return lds[(sp & (kShortStack - 1u)) * lanes + lane];
+    }
+};
+
+__device__ __forceinline__ Stack make_stack(uint32_t *lds, uint32_t *spill, uint32_t lanes, uint32_t lane,
+                                            uint32_t global_lane, uint32_t stride) {
+    Stack st;
+    st.lds = lds;
+    st.spill = spill + global_lane;
+    st.lanes = lanes;
+    st.lane = lane;
+    st.stride = stride;
+    st.sp = 0;
+    st.bottom = 0;
+    return st;
+}
+
+__device__ __forceinline__ float clamp_bvh_dir(float v) {
+    const float m = 8.271806e-25f;
+    return fabsf(v) > m ? v : copysignf(m, v);
+}
+
+__device__ __forceinline__ f3 pluecker_cross(f3 a, f3 b) {
+    return V(fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x)));
+}
+__device__ __forceinline__ float pluecker_dot(f3 a, f3 b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
+
+// d is already clamped with clamp_bvh_dir
+__device__ __forceinline__ bool pluecker(f3 o, f3 d, float tmin, float tmax, f3 p0, f3 p1, f3 p2,
+                                         float &t, float &bu, float &bv) {
+    const f3 v0 = p0 - o, v1 = p1 - o, v2 = p2 - o;
+    const f3 e0 = v2 - v0, e1 = v0 - v1, e2 = v1 - v2;
+    const float u = pluecker_dot(pluecker_cross(e0, v2 + v0), d);
+    const float v = pluecker_dot(pluecker_cross(e1, v0 + v1), d);
+    const float w = pluecker_dot(pluecker_cross(e2, v1 + v2), d);
+    const float sum = u + v + w;
+    const float tol = FLT_EPSILON * fabsf(sum);
+    const float mn = fminf(u, fminf(v, w));
+    const float mx = fmaxf(u, fmaxf(v, w));
+    const bool inside = (mn >= -tol) || (mx <= tol);
+    const f3 n2 = pluecker_cross(e1, e0) * 2.0f;
+    const float den = pluecker_dot(n2, d);
+    const float safe_den = den != 0.0f ? den : 1.0f;
+    t = pluecker_dot(v0, n2) / safe_den;
+    const bool in_range = (t >= tmin) && (t <= tmax);
+    const float rs = fabsf(sum) >= 1.0e-18f ? 1.0f / sum : 0.0f;
+    bu = fminf(u * rs, 1.0f);
+    bv = fminf(v * rs, 1.0f);
+    return inside && den != 0.0f && in_range;
+}
+
+struct TraceQuery {
+    uint32_t visibility;
+    uint32_t src_obj, src_prim, light_obj, light_prim;
+};
+
+__device__ __forceinline__ bool order_accepts(const Hit &best, float t, uint32_t object, uint32_t prim) {
+    if (!best.valid) return true;
+    if (t < best.t) return true;
+    if (t == best.t) return object > best.object || (object == best.object && prim > best.cprim);
+    return false;
+}
+
+// compare-exchange of a sorting network on (distance, entry) pairs
+__device__ __forceinline__ void sort_pair(float &ka, uint32_t &ea, float &kb, uint32_t &eb) {
+    const bool swap = kb < ka;
+    const float k0 = swap ? kb : ka, k1 = swap ? ka : kb;
+    const uint32_t e0 = swap ? eb : ea, e1 = swap ? ea : eb;
+    ka = k0;
+    kb = k1;
+    ea = e0;
+    eb = e1;
+}
+
+// Closest hit along [tmin, tmax] under the Cycles order: smallest t, ties to
+// the greatest (object, primitive).  Box culling is conservative (boxes are
+// padded at build time and the far distance is widened), so traversal order
+// never changes the result, only its cost.
+//
+// The traversal is a resumable state machine (begin, then step until it
+// reports completion) so that persistent kernels can refill lanes whose ray
+// has finished while the rest of the wave keeps traversing.
+struct Traversal {
+    Ray ray;        // world ray of the query
+    TraceQuery q;
+    f3 o, d, inv, oi;  // ray in the current intersection space; oi = o * inv
+    float lower;
+    Hit best;
+    uint32_t current;
+    uint32_t inst_index, inst_prim_offset;
+    bool in_blas;
+};
+
+__device__ __forceinline__ f3 clamped_dir(f3 d) {
+    return V(clamp_bvh_dir(d.x), clamp_bvh_dir(d.y), clamp_bvh_dir(d.z));
+}
+
+__device__ __forceinline__ void traversal_world_space(Traversal &t) {
+    t.in_blas = false;
+    t.o = t.ray.o;
+    t.d = clamped_dir(t.ray.d);
+    t.inv = V(1.0f / t.d.x, 1.0f / t.d.y, 1.0f / t.d.z);
+    t.oi = t.o * t.inv;  // slab distances are fma(bound, inv, -o * inv)
+}
+
+__device__ __forceinline__ void traversal_begin(const SceneView &s, Traversal &t, Stack &st) {
+    t.best.valid = false;
+    t.best.t = t.ray.tmax;
+    t.best.inst = t.best.prim = t.best.object = t.best.cprim = 0xffffffffu;
+    t.best.u = t.best.v = 0.0f;
+    t.lower = t.ray.tmin > 0.0f ? t.ray.tmin * 0.99999f : t.ray.tmin - 1.0e-30f;
+    t.inst_index = t.inst_prim_offset = 0u;
+    traversal_world_space(t);
+    st.sp = 0;
+    st.bottom = 0;
+    t.current = s.has_tlas ? (s.tlas_root | kKindTlas) : kEmptySlot;
+}
+
+// One traversal iteration: an instance entry, a leaf or an inner node.
+// Returns false once the query is complete (t.best holds the result).
+template <bool COUNT>
+__device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t, Stack &st, uint32_t &node_count,
+                                               uint32_t &tri_count) {
+    if (t.current == kEmptySlot) {
+        if (st.sp == 0) return false;
+        t.current = st.pop();
+    }
+    const uint32_t entry = t.current;
+    t.current = kEmptySlot;
+    const uint32_t kind = entry & kKindMask;
+    const uint32_t index = entry & ~kKindMask;
+    // back at the top level: world ray again
+    if (t.in_blas && (kind == kKindTlas || kind == kKindInstance)) traversal_world_space(t);
+    if (kind == kKindInstance) {
+        // enter an instance: object-space ray with the Cycles FMA transform
+        const Instance &inst = s.instances[index];
+        if ((inst.visibility & t.q.visibility) == 0u) return true;
+        t.inst_index = index;
+        t.inst_prim_offset = inst.prim_offset;
+        t.in_blas = true;
+        if (!inst.applied) {
+            t.o = affine_point(inst.w2o, t.ray.o);
+            t.d = clamped_dir(affine_dir(inst.w2o, t.ray.d));
+            t.inv = V(1.0f / t.d.x, 1.0f / t.d.y, 1.0f / t.d.z);
+            t.oi = t.o * t.inv;
+        }
+        t.current = inst.blas_root | kKindBlas;
+        return true;
+    }
+    if (kind == kKindLeaf) {
+        const uint32_t first = index >> 3;
+        const uint32_t n = (index & 7u) + 1u;
+        for (uint32_t k = 0; k < n; ++k) {
+            const LeafTri &tri = s.tris[first + k];
+            if (COUNT) tri_count++;
+            float th, bu, bv;
+            const bool ok = pluecker(t.o, t.d, t.ray.tmin, t.ray.tmax, V(tri.p0x, tri.p0y, tri.p0z),
+                                     V(tri.p1x, tri.p1y, tri.p1z), V(tri.p2x, tri.p2y, tri.p2z), th, bu, bv);
+            if (!ok) continue;
+            const uint32_t prim = __float_as_uint(tri.prim_bits);
+            const uint32_t cprim = t.inst_prim_offset + prim;
+            const bool excluded = (t.inst_index == t.q.src_obj && cprim == t.q.src_prim) ||
+                                  (t.inst_index == t.q.light_obj && cprim == t.q.light_prim);
+            if (excluded || !order_accepts(t.best, th, t.inst_index, cprim)) continue;
+            t.best.valid = true;
+            t.best.t = th;
+            t.best.u = bu;
+            t.best.v = bv;
+            t.best.inst = t.inst_index;
+            t.best.prim = prim;
+            t.best.object = t.inst_index;
+            t.best.cprim = cprim;
+        }
+        return true;
+    }
+    // inner node: test the four child boxes, continue with the nearest hit
+    // child and push the others far-to-near
+    const bool tlas = kind == kKindTlas;
+    const Node4 &n = s.nodes[index];
+    if (COUNT) node_count++;
+    const float upper = (t.best.valid ? t.best.t : t.ray.tmax) * 1.00001f + 1.0e-30f;
+    float key[4];
+    uint32_t ent[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        // conservative: boxes are padded by 1e-5 of their coordinate
+        // magnitude and `far` is widened by 1e-5, both far above the
+        // rounding of o * inv
+        const float t0x = fmaf(n.lo_x[i], t.inv.x, -t.oi.x), t1x = fmaf(n.hi_x[i], t.inv.x, -t.oi.x);
+        const float t0y = fmaf(n.lo_y[i], t.inv.y, -t.oi.y), t1y = fmaf(n.hi_y[i], t.inv.y, -t.oi.y);
+        const float t0z = fmaf(n.lo_z[i], t.inv.z, -t.oi.z), t1z = fmaf(n.hi_z[i], t.inv.z, -t.oi.z);
+        const float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+        const float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+        const float far_safe = far * 1.00001f + 1.0e-30f;
+        const uint32_t count = n.count[i];
+        const bool hit = count != kEmptySlot && near <= far_safe && far_safe >= t.lower && near <= upper;
+        const uint32_t child = n.child[i];
+        const uint32_t inner = child | (tlas ? kKindTlas : kKindBlas);
+        const uint32_t leaf = tlas ? (child | kKindInstance) : (kKindLeaf | (child << 3) | (count - 1u));
+        key[i] = hit ? near : INFINITY;
+        ent[i] = count == 0u ? inner : leaf;
+    }
+    sort_pair(key[0], ent[0], key[1], ent[1]);
+    sort_pair(key[2], ent[2], key[3], ent[3]);
+    sort_pair(key[0], ent[0], key[2], ent[2]);
+    sort_pair(key[1], ent[1], key[3], ent[3]);
+    sort_pair(key[1], ent[1], key[2], ent[2]);
+    if (key[3] != INFINITY) st.push(ent[3]);
+    if (key[2] != INFINITY) st.push(ent[2]);
+    if (key[1] != INFINITY) st.push(ent[1]);
+    if (key[0] != INFINITY) t.current = ent[0];
+    return true;
+}
+
+template <bool COUNT>
+__device__ Hit closest_hit(const SceneView &s, const Ray &ray, const TraceQuery &q, Stack &st,
+                           uint32_t &node_count, uint32_t &tri_count) {
+    Traversal t;
+    t.ray = ray;
+    t.q = q;
+    traversal_begin(s, t, st);
+    while (traversal_step<COUNT>(s, t, st, node_count, tri_count)) {
+    }
+    return t.best;
+}
+
+// ===========================================================================
+// surface reconstruction (src/luisa/cycles_triangle_surface_component.cpp,
+// src/luisa/path_kernel_surface_primitive.cpp:139-460)
+// ===========================================================================
+struct Surface {
+    uint32_t inst, object, cprim, material, emission_sampling;
+    bool applied, back_facing;
+    f3 p0, p1, p2;      // intersection-space positions
+    f3 wp0, wp1, wp2;   // world positions
+    f3 position, object_hit_position, ng, ns, incoming;
+};
+
+__device__ __forceinline__ uint32_t material_of(const SceneView &s, const Instance &inst, const Mesh &mesh,
+                                                uint32_t prim) {
+    const uint32_t slot = mesh.tri_slot_offset != kNone ? s.tri_slots[mesh.tri_slot_offset + prim] : 0u;
+    if (slot < inst.override_count) return s.overrides[inst.override_offset + slot];
+    if (mesh.slot_count == 0u) return kNone;
+    return s.slots[mesh.slot_offset + (slot < mesh.slot_count ? slot : mesh.slot_count - 1u)];
+}
+
+__device__ __forceinline__ f3 interp3(float u, float v, f3 a, f3 b, f3 c) {
+    return (a * (1.0f - u - v) + b * u) + c * v;
+}
+
+__device__ void resolve_surface(const SceneView &s, const Hit &h, const Ray &ray, Surface &sf) {
+    const Instance &inst = s.instances[h.inst];
+    const Mesh &mesh = s.meshes[inst.mesh];
+    sf.inst = h.inst;
+    sf.object = h.object;
+    sf.cprim = h.cprim;
+    sf.material = material_of(s, inst, mesh, h.prim);
+    sf.applied = inst.applied != 0u;
+    const uint32_t *tri = s.indices + 3u * (mesh.index_offset + h.prim);
+    const uint32_t i0 = tri[0], i1 = tri[1], i2 = tri[2];
+    const f3 op0 = load3(s.positions + 3u * (mesh.vertex_offset + i0));
+    const f3 op1 = load3(s.positions + 3u * (mesh.vertex_offset + i1));
+    const f3 op2 = load3(s.positions + 3u * (mesh.vertex_offset + i2));
+    f3 p0 = op0, p1 = op1, p2 = op2;
+    if (sf.applied) {
+        p0 = load3(s.positions + 3u * (inst.ipos_offset + i0));
+        p1 = load3(s.positions + 3u * (inst.ipos_offset + i1));
+        p2 = load3(s.positions + 3u * (inst.ipos_offset + i2));
+    }
+    sf.p0 = p0;
+    sf.p1 = p1;
+    sf.p2 = p2;
+    sf.wp0 = sf.applied ? p0 : affine_point(inst.o2w, op0);
+    sf.wp1 = sf.applied ? p1 : affine_point(inst.o2w, op1);
+    sf.wp2 = sf.applied ? p2 : affine_point(inst.o2w, op2);
+    const f3 ogn = safe_normalize(cross(op1 - op0, op2 - op0), V(0.0f, 0.0f, 1.0f));
+    const f3 tgn = safe_normalize(affine_dir(inst.n2w, ogn), neg(ray.d));
+    f3 gn = tgn;
+    if (sf.applied) {
+        f3 sgn = safe_normalize(cross(p1 - p0, p2 - p0), tgn);
+        if (dot(sgn, tgn) < 0.0f) sgn = neg(sgn);
+        gn = sgn;
+    }
+    const bool smooth = mesh.smooth_offset != kNone && mesh.normal_offset != kNone &&
+                        s.smooth[mesh.smooth_offset + h.prim] != 0u;
+    f3 ns;
+    if (smooth) {
+        const f3 n0 = load3(s.normals + 3u * (mesh.normal_offset + i0));
+        const f3 n1 = load3(s.normals + 3u * (mesh.normal_offset + i1));
+        const f3 n2 = load3(s.normals + 3u * (mesh.normal_offset + i2));
+        if (sf.applied) {
+            const f3 w0 = safe_normalize(affine_dir(inst.n2w, n0), gn);
+            const f3 w1 = safe_normalize(affine_dir(inst.n2w, n1), gn);
+            const f3 w2 = safe_normalize(affine_dir(inst.n2w, n2), gn);
+            ns = safe_normalize(interp3(h.u, h.v, w0, w1, w2), gn);
+        } else {
+            ns = safe_normalize(affine_dir(inst.n2w, interp3(h.u, h.v, n0, n1, n2)), gn);
+        }
+    } else {
+        ns = sf.applied ? safe_normalize(gn, gn) : safe_normalize(affine_dir(inst.n2w, ogn), gn);
+    }
+    const bool back_facing = dot(gn, neg(ray.d)) < 0.0f;
+    if (back_facing) {
+        gn = neg(gn);
+        ns = neg(ns);
+    }
+    if (dot(ns, gn) < 0.0f) ns = neg(ns);
+    sf.back_facing = back_facing;
+    sf.ng = gn;
+    sf.ns = ns;
+    if (sf.applied) {
+        sf.position = (p0 + (p1 - p0) * h.u) + (p2 - p0) * h.v;
+        sf.object_hit_position = sf.position;
+    } else {
+        const f3 object_position = (op0 + (op1 - op0) * h.u) + (op2 - op0) * h.v;
+        sf.position = affine_point(inst.o2w, object_position);
+        sf.object_hit_position = mat_mul(inst.w2o, sf.position, 1.0f);
+    }
+    sf.incoming = neg(ray.d);
+    sf.emission_sampling = 0u;
+    if (sf.material != kNone) {
+        const Material &m = s.materials[sf.material];
+        if (m.may_emit && (inst.visibility & MESH_LIGHT_VISIBILITY) != 0u) sf.emission_sampling = m.emission_sampling;
+    }
+}
+
+// offset_ray_origin: Waechter & Binder, Ray Tracing Gems (2019), chapter 6.
+__device__ __forceinline__ float offset_component(float p, float n) {
+    const int32_t of_i = (int32_t)(256.0f * n);
+    const float p_i = __int_as_float(__float_as_int(p) + (p < 0.0f ? -of_i : of_i));
+    return fabsf(p) < (1.0f / 32.0f) ? p + (1.0f / 65536.0f) * n : p_i;
+}
+
+// include/psycles/luisa/surface_ray.h:50-106
+__device__ __forceinline__ f3 origin_with_self_exclusion(f3 wo, f3 wn, f3 oo, f3 od, f3 p0, f3 p1, f3 p2) {
+    const f3 v0 = p0 - oo, v1 = p1 - oo, v2 = p2 - oo;
+    const f3 e0 = v2 - v0, e1 = v0 - v1, e2 = v1 - v2;
+    const float u = dot(cross(v2 + v0, e0), od);
+    const float v = dot(cross(v0 + v1, e1), od);
+    const float w = dot(cross(v1 + v2, e2), od);
+    const float eps = FLT_EPSILON * fabsf(u + v + w);
+    const float mn = fminf(u, fminf(v, w));
+    const float mx = fmaxf(u, fmaxf(v, w));
+    const bool covered = (mn >= eps) || (mx <= -eps);
+    if (covered) return wo;
+    return V(offset_component(wo.x, wn.x), offset_component(wo.y, wn.y), offset_component(wo.z, wn.z));
+}
+
+__device__ __forceinline__ f3 surface_ray_origin(const SceneView &s, const Surface &sf, f3 dir) {
+    const Instance &inst = s.instances[sf.inst];
+    const f3 co = sf.applied ? sf.position : sf.object_hit_position;
+    const f3 cd = sf.applied ? dir : affine_dir(inst.w2o, dir);
+    return origin_with_self_exclusion(sf.position, sf.ng, co, cd, sf.p0, sf.p1, sf.p2);
+}
+
+__device__ __forceinline__ f3 surface_shadow_origin(const SceneView &s, const Surface &sf, f3 dir) {
+    const Instance &inst = s.instances[sf.inst];
+    const f3 oo = sf.applied ? sf.position : affine_point(inst.w2o, sf.position);
+    const f3 od = sf.applied ? dir : affine_dir(inst.w2o, dir);
+    return origin_with_self_exclusion(sf.position, sf.ng, oo, od, sf.p0, sf.p1, sf.p2);
+}
+
+__device__ __forceinline__ float next_float_up(float t) {
+    return t == 0.0f ? FLT_MIN : __uint_as_float(__float_as_uint(t) + 1u);
+}
+// ===========================================================================
+// closures (src/luisa/graph_surface_closures.cpp:6-353,
+// src/luisa/graph_surface_physical_closures.cpp:200-470,
+// src/luisa/surface_closure_evaluation.cpp:67-464,
+// src/luisa/surface_closure_sampling_core.cpp:96-676)
+// ===========================================================================
+constexpr uint32_t kMaxClosures = 4;
+constexpr uint32_t KIND_DIFFUSE = 1u, KIND_TRANSPARENT = 2u;
+constexpr float INV_PI = 0.31830988618379067154f;
+constexpr float PI_F = 3.14159265358979323846f;
+
+struct ClosureSet {
+    uint32_t count;
+    uint32_t kind[kMaxClosures];
+    f3 weight[kMaxClosures];
+    float sw[kMaxClosures];
+    float roughness[kMaxClosures];
+    f3 color[kMaxClosures];
+    f3 normal;          // safe_normalize(Ns, Ns): every closure uses the shading normal
+    f3 emission;
+};
+
+__device__ __forceinline__ float avg3(f3 v) { return (v.x + v.y + v.z) / 3.0f; }
+
+__device__ void material_closures(const SceneView &s, uint32_t material, f3 ns, ClosureSet &cs) {
+    cs.count = 0u;
+    cs.emission = V(0.0f, 0.0f, 0.0f);
+    cs.normal = safe_normalize(ns, ns);
+    if (material == kNone) return;
+    const Material &m = s.materials[material];
+    const float *p = s.params + m.param_offset;
+    for (uint32_t l = 0; l < m.leaf_count; ++l) {
+        const Leaf &leaf = s.leaves[m.leaf_offset + l];
+        float w = 1.0f;
+        for (uint32_t d = 0; d < leaf.depth; ++d) {
+            const float f = clampf(p[leaf.factor[d]], 0.0f, 1.0f);
+            w = w * (((leaf.second_mask >> d) & 1u) ? f : (1.0f - f));
+        }
+        const f3 color = V(p[leaf.color], p[leaf.color + 1u], p[leaf.color + 2u]);
+        if (leaf.op == kLeafEmission) {
+            cs.emission = cs.emission + (color * p[leaf.aux]) * w;
+            continue;
+        }
+        // diffuse or transparent closure candidate
+        const bool diffuse = leaf.op == kLeafDiffuse;
+        if (!diffuse && leaf.op != kLeafTransparent) continue;
+        f3 wt;
+        float sw;
+        if (diffuse) {
+            wt = vmax(color * w, V(0.0f, 0.0f, 0.0f));
+            if (!(avg3(wt) >= 1.0e-5f)) wt = V(0.0f, 0.0f, 0.0f);
+            sw = fabsf(avg3(vmax(wt, V(0.0f, 0.0f, 0.0f))));
+            wt = vmax(wt, V(0.0f, 0.0f, 0.0f));
+        } else {
+            wt = color * w;
+            sw = fabsf(avg3(wt));
+        }
+        if (!(sw >= 1.0e-5f) || cs.count >= kMaxClosures) continue;
+        const uint32_t kind = diffuse ? KIND_DIFFUSE : KIND_TRANSPARENT;
+        const float roughness = diffuse ? p[leaf.aux] : 0.0f;
+        const f3 ccolor = diffuse ? color : V(1.0f, 1.0f, 1.0f);
+        // slot writes through constant indices keep the set in registers
+#pragma unroll
+        for (uint32_t c = 0; c < kMaxClosures; ++c) {
+            if (c == cs.count) {
+                cs.kind[c] = kind;
+                cs.weight[c] = wt;
+                cs.sw[c] = sw;
+                cs.roughness[c] = roughness;
+                cs.color[c] = ccolor;
+            }
+        }
+        cs.count += 1u;
+    }
+}
+
+__device__ __forceinline__ f3 material_constant_emission(const SceneView &s, uint32_t material) {
+    f3 e = V(0.0f, 0.0f, 0.0f);
+    const Material &m = s.materials[material];
+    const float *p = s.params + m.param_offset;
+    for (uint32_t l = 0; l < m.leaf_count; ++l) {
+        const Leaf &leaf = s.leaves[m.leaf_offset + l];
+        if (leaf.op != kLeafEmission) continue;
+        float w = 1.0f;
+        for (uint32_t d = 0; d < leaf.depth; ++d) {
+            const float f = clampf(p[leaf.factor[d]], 0.0f, 1.0f);
+            w = w * (((leaf.second_mask >> d) & 1u) ? f : (1.0f - f));
+        }
+        e = e + (V(p[leaf.color], p[leaf.color + 1u], p[leaf.color + 2u]) * p[leaf.aux]) * w;
+    }
+    return e;
+}
+
+__device__ __forceinline__ float oren_nayar_g(float cosine) {
+    const float c = clampf(cosine, 0.0f, 1.0f);
+    const float sine = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+    const float theta = acosf(c);
+    const float safe_c = fmaxf(c, 1.0e-6f);
+    const float regular =
+        sine * (theta - 2.0f / 3.0f - sine * c) + 2.0f / 3.0f * (sine / safe_c) * (1.0f - sine * sine * sine);
+    const float taylor = (PI_F * 0.5f - 2.0f / 3.0f) - c;
+    return c < 1.0e-6f ? taylor : regular;
+}
+
+// diffuse_intensity (src/luisa/graph_surface_scattering.cpp:475-506)
+__device__ __forceinline__ f3 diffuse_intensity(f3 n, float roughness, f3 color, f3 in, f3 out) {
+    const float nl = fmaxf(dot(n, out), 0.0f);
+    const float sigma = clampf(roughness, 0.0f, 1.0f);
+    if (sigma < 1.0e-5f) {
+        const float l = nl * INV_PI;
+        return V(l, l, l);
+    }
+    const float a = 1.0f / (PI_F + sigma * (PI_F * 0.5f - 2.0f / 3.0f));
+    const float b = sigma * a;
+    const float nv = fmaxf(dot(n, in), 0.0f);
+    float t = dot(out, in) - nl * nv;
+    if (t > 0.0f) t = t / (fmaxf(nl, nv) + 1.17549435e-38f);
+    const float single = a + b * t;
+    const f3 albedo = vclamp(color, 0.0f, 1.0f);
+    const float two_pi = 6.2831853071795864f;
+    const float e_avg = a * PI_F + ((two_pi - 5.6f) / 3.0f) * b;
+    const f3 e_ms = ((albedo * albedo) * INV_PI * (e_avg / (1.0f - e_avg))) /
+                    (V(1.0f, 1.0f, 1.0f) - albedo * (1.0f - e_avg));
+    const float e_in = a * PI_F + b * oren_nayar_g(nv);
+    const f3 ms = e_ms * (1.0f - e_in);
+    const float e_out = a * PI_F + b * oren_nayar_g(nl);
+    return (V(single, single, single) + ms * (1.0f - e_out)) * nl;
+}
+
+struct Eval {
+    f3 f;
+    float pdf;
+    uint32_t events;
+};
+
+__device__ Eval evaluate_mixture(const ClosureSet &cs, f3 point_incoming, f3 ns, f3 out_dir, uint32_t lobe_mask) {
+    const f3 outgoing = safe_normalize(out_dir, ns);
+    const f3 incoming = safe_normalize(point_incoming, neg(outgoing));
+    f3 f = V(0.0f, 0.0f, 0.0f);
+    float total = 0.0f, weighted = 0.0f;
+    uint32_t events = 0u;
+    const bool diffuse_enabled = (lobe_mask & EV_DIFFUSE) != 0u;
+    const bool transparent_enabled = (lobe_mask & EV_TRANSPARENT) != 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxClosures; ++i) {
+        if (i >= cs.count) break;
+        if (cs.kind[i] == KIND_DIFFUSE) {
+            const float pdf = fmaxf(dot(cs.normal, outgoing), 0.0f) * INV_PI;
+            const f3 value = (cs.weight[i] * diffuse_intensity(cs.normal, cs.roughness[i], cs.color[i], incoming,
+                                                               outgoing)) * 1.0f;
+            const f3 contrib = diffuse_enabled ? value : V(0.0f, 0.0f, 0.0f);
+            const float w = diffuse_enabled ? cs.sw[i] : 0.0f;
+            if (diffuse_enabled && fabsf(avg3(contrib)) > 0.0f) events |= EV_DIFFUSE | EV_REFLECTION;
+            f = f + contrib;
+            total += w;
+            weighted += w * (diffuse_enabled ? pdf : 0.0f);
+        } else {
+            if (transparent_enabled) total += cs.sw[i];
+        }
+    }
+    Eval e;
+    e.f = f;
+    e.pdf = total > 0.0f ? weighted / fmaxf(total, 1.0e-20f) : 0.0f;
+    e.events = weighted > 0.0f ? events : 0u;
+    return e;
+}
+
+__device__ __forceinline__ uint32_t runtime_flags(const ClosureSet &cs, bool back_facing) {
+    uint32_t f = back_facing ? RT_BACKFACING : 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxClosures; ++i) {
+        if (i >= cs.count) break;
+        f |= cs.kind[i] == KIND_DIFFUSE ? (RT_BSDF | RT_BSDF_HAS_EVAL) : (RT_BSDF | RT_TRANSPARENT);
+    }
+    return f;
+}
+
+// Shirley-Chiu concentric map (include/psycles/luisa/cycles_sample_mapping.h:49-62)
+__device__ __forceinline__ void uniform_disk(float rx, float ry, float &dx, float &dy) {
+    const float a = 2.0f * rx - 1.0f;
+    const float b = 2.0f * ry - 1.0f;
+    const bool x_major = a * a > b * b;
+    const float safe_a = a != 0.0f ? a : 1.0f;
+    const float safe_b = b != 0.0f ? b : 1.0f;
+    const float radius = x_major ? a : b;
+    const float angle = x_major ? 0.25f * PI_F * (b / safe_a) : 0.5f * PI_F - 0.25f * PI_F * (a / safe_b);
+    dx = cosf(angle) * radius;
+    dy = sinf(angle) * radius;
+    if (a == 0.0f && b == 0.0f) dx = dy = 0.0f;
+}
+
+__device__ __forceinline__ f3 cosine_hemisphere(f3 n, float rx, float ry) {
+    float dx, dy;
+    uniform_disk(rx, ry, dx, dy);
+    const float cosine = sqrtf(fmaxf(1.0f - (dx * dx + dy * dy), 0.0f));
+    const f3 general = V(n.z - n.y, n.x - n.z, n.y - n.x);
+    const f3 equal = V(n.z - n.y, n.x + n.z, -n.y - n.x);
+    const bool use_general = (n.x != n.y) || (n.x != n.z);
+    const f3 t = normalize(use_general ? general : equal);
+    const f3 b = cross(n, t);
+    return (t * dx + b * dy) + n * cosine;
+}
+
+struct BsdfSample {
+    bool valid;
+    f3 wi;
+    Eval eval;
+    uint32_t runtime_flags;
+    uint32_t selected;
+};
+
+__device__ BsdfSample sample_closures(const ClosureSet &cs, const Surface &sf, float u_lobe, float ux, float uy,
+                                      uint32_t lobe_mask) {
+    BsdfSample r;
+    r.valid = false;
+    r.selected = 0xffffffffu;
+    r.eval.f = V(0.0f, 0.0f, 0.0f);
+    r.eval.pdf = 0.0f;
+    r.eval.events = 0u;
+    r.wi = V(0.0f, 0.0f, 0.0f);
+    r.runtime_flags = runtime_flags(cs, sf.back_facing);
+    float total = 0.0f;
+    float weights[kMaxClosures];
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxClosures; ++i) {
+        weights[i] = 0.0f;
+        if (i >= cs.count) continue;
+        const bool eligible = cs.kind[i] == KIND_TRANSPARENT ? (lobe_mask & EV_TRANSPARENT) != 0u
+                                                              : (lobe_mask & EV_DIFFUSE) != 0u;
+        weights[i] = eligible ? cs.sw[i] : 0.0f;
+        total += weights[i];
+    }
+    // categorical inversion (surface_closure_sampling_core.cpp:410-437)
+    const float random = clampf(u_lobe, 0.0f, 0.99999994f);
+    const float target = random * total;
+    float accumulated = 0.0f;
+    bool selected = false;
+    uint32_t sel = 0u;
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxClosures; ++i) {
+        if (i >= cs.count) break;
+        const float next = accumulated + weights[i];
+        const bool choose = !selected && weights[i] > 0.0f && target < next;
+        if (choose) sel = i;
+        selected |= choose;
+        accumulated = next;
+    }
+    if (!selected) return r;
+    r.selected = sel;
+    uint32_t kind = KIND_DIFFUSE;
+    f3 weight = V(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxClosures; ++i) {
+        if (i == sel) {
+            kind = cs.kind[i];
+            weight = cs.weight[i];
+        }
+    }
+    const bool transparent = kind == KIND_TRANSPARENT;
+    const f3 direction = transparent ? neg(sf.incoming) : cosine_hemisphere(cs.normal, ux, uy);
+    const Eval mix = evaluate_mixture(cs, sf.incoming, sf.ns, direction, lobe_mask);
+    const bool valid = transparent || dot(sf.ng, direction) > 0.0f;
+    r.valid = valid;
+    r.wi = direction;
+    const f3 regular_f = valid ? mix.f : V(0.0f, 0.0f, 0.0f);
+    const float regular_pdf = valid ? mix.pdf : 0.0f;
+    float sel_weight = 0.0f;
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxClosures; ++i)
+        if (i == sel) sel_weight = weights[i];
+    const f3 delta = (valid && transparent) ? weight * 1.0e6f : V(0.0f, 0.0f, 0.0f);
+    const float delta_num = (valid && transparent) ? 1.0e6f * sel_weight : 0.0f;
+    r.eval.f = regular_f + delta;
+    r.eval.pdf = regular_pdf + delta_num / fmaxf(total, 1.0e-20f);
+    const uint32_t events = transparent ? (EV_TRANSMISSION | EV_TRANSPARENT) : (EV_DIFFUSE | EV_REFLECTION);
+    r.eval.events = valid ? events : 0u;
+    return r;
+}
+
+__device__ void surface_aov(const ClosureSet &cs, f3 ns, f3 &albedo, f3 &normal, f3 &transparency) {
+    f3 a = V(0.0f, 0.0f, 0.0f), n = V(0.0f, 0.0f, 0.0f), tr = V(0.0f, 0.0f, 0.0f);
+    float total = 0.0f;
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxClosures; ++i) {
+        if (i >= cs.count) break;
+        if (cs.kind[i] == KIND_DIFFUSE) {
+            const float pw = fabsf(avg3(cs.weight[i]));
+            a = a + cs.weight[i];
+            n = n + cs.normal * pw;
+            total += pw;
+        } else {
+            tr = tr + cs.weight[i];
+        }
+    }
+    albedo = a;
+    transparency = tr;
+    normal = safe_normalize(total > 0.0f ? n : ns, ns);
+}
+
+// ===========================================================================
+// triangle light sampling (include/psycles/luisa/spherical_geometry.h)
+// ===========================================================================
+__device__ __forceinline__ float triangle_solid_angle(f3 ref, f3 p0, f3 p1, f3 p2) {
+    const f3 a = normalize_or(p0 - ref, V(1.0f, 0.0f, 0.0f));
+    const f3 b = normalize_or(p1 - ref, V(0.0f, 1.0f, 0.0f));
+    const f3 c = normalize_or(p2 - ref, V(0.0f, 0.0f, 1.0f));
+    const float num = fabsf(dot(a, cross(b, c)));
+    const float den = 1.0f + dot(a, b) + dot(b, c) + dot(c, a);
+    return 2.0f * atan2f(num, den);
+}
+
+__device__ __forceinline__ bool use_solid_angle(f3 ref, f3 p0, f3 p1, f3 p2) {
+    const f3 e0 = p1 - p0, e1 = p2 - p0, e2 = p2 - p1;
+    const float longest = fmaxf(dot(e0, e0), fmaxf(dot(e1, e1), dot(e2, e2)));
+    const f3 nn = cross(e0, e1);
+    const float nl2 = dot(nn, nn);
+    const float dist = dot(nn, p0 - ref) / fmaxf(nl2, 1.0e-20f);
+    return (nl2 > 1.0e-20f) && (longest > dist * dist);
+}
+
+__device__ __forceinline__ bool triangle_pdf(f3 ref, f3 lp, f3 p0, f3 p1, f3 p2, float &value) {
+    const f3 un = cross(p1 - p0, p2 - p0);
+    const float da = sqrtf(fmaxf(dot(un, un), 0.0f));
+    const float area = 0.5f * da;
+    const f3 normal = un / fmaxf(da, 1.0e-20f);
+    const f3 off = lp - ref;
+    const float d2 = dot(off, off);
+    const f3 dir = off / sqrtf(fmaxf(d2, 1.0e-20f));
+    const float cosine = fabsf(dot(normal, neg(dir)));
+    const bool solid = use_solid_angle(ref, p0, p1, p2);
+    const float sa = triangle_solid_angle(ref, p0, p1, p2);
+    const bool valid_area = (area > 0.0f) && (d2 > 1.0e-12f) && (cosine > 0.0f);
+    value = solid ? 1.0f / fmaxf(sa, 1.0e-20f) : d2 / fmaxf(cosine * area, 1.0e-20f);
+    return solid ? (valid_area && sa > 0.0f) : valid_area;
+}
+
+struct LightSample {
+    f3 position, direction;
+    float distance, pdf;
+    bool valid;
+};
+
+__device__ LightSample sample_triangle(f3 ref, f3 p0, f3 p1, f3 p2, float rx, float ry) {
+    const float x = clampf(rx, 0.0f, 1.0f), y = clampf(ry, 0.0f, 1.0f);
+    const bool upper = y > x;
+    const float bu = upper ? 0.5f * x : x - 0.5f * y;
+    const float bv = upper ? y - 0.5f * x : 0.5f * y;
+    const f3 apos = (p0 * (1.0f - bu - bv) + p1 * bu) + p2 * bv;
+    const f3 aoff = apos - ref;
+    const float ad2 = dot(aoff, aoff);
+    const float ad = sqrtf(fmaxf(ad2, 1.0e-20f));
+    const f3 adir = aoff / ad;
+    LightSample r;
+    const bool solid = use_solid_angle(ref, p0, p1, p2);
+    if (solid) {
+        const f3 a = normalize_or(p0 - ref, V(1.0f, 0.0f, 0.0f));
+        const f3 b = normalize_or(p1 - ref, V(0.0f, 1.0f, 0.0f));
+        const f3 c = normalize_or(p2 - ref, V(0.0f, 0.0f, 1.0f));
+        const float cos_b = clampf(dot(a, c), -1.0f, 1.0f);
+        const float cos_c = clampf(dot(a, b), -1.0f, 1.0f);
+        const float sa = triangle_solid_angle(ref, p0, p1, p2);
+        const f3 ab_n = normalize_or(cross(a, b), V(1.0f, 0.0f, 0.0f));
+        const f3 ac_n = normalize_or(cross(a, c), V(0.0f, 1.0f, 0.0f));
+        const float cos_alpha = clampf(dot(ab_n, ac_n), -1.0f, 1.0f);
+        const float sin_alpha = sqrtf(fmaxf(1.0f - cos_alpha * cos_alpha, 0.0f));
+        const float alpha = acosf(cos_alpha);
+        const float phi = x * sa - alpha;
+        const float sin_phi = sinf(phi);
+        const float cos_phi = cosf(phi);
+        const float u = cos_phi - cos_alpha;
+        const float v = sin_phi + sin_alpha * cos_c;
+        const float num = (v * cos_phi - u * sin_phi) * cos_alpha - v;
+        const float den = (v * sin_phi + u * cos_phi) * sin_alpha;
+        const float safe_den = fabsf(den) > 1.0e-20f ? den : (den >= 0.0f ? 1.0e-20f : -1.0e-20f);
+        const float q = fabsf(den) <= 1.0e-20f ? 1.0f : num / safe_den;
+        const f3 tangent = normalize_or(c - a * cos_b, V(0.0f, 1.0f, 0.0f));
+        const float qc = clampf(q, -1.0f, 1.0f);
+        const f3 c_prime = normalize_or(a * qc + tangent * sqrtf(fmaxf(1.0f - q * q, 0.0f)), a);
+        const float z = 1.0f - y * (1.0f - dot(c_prime, b));
+        const f3 edge = normalize_or(c_prime - b * dot(c_prime, b), a);
+        const f3 sdir = normalize_or(b * z + edge * sqrtf(fmaxf(1.0f - z * z, 0.0f)), b);
+        const f3 e01 = p1 - p0, e02 = p2 - p0;
+        const f3 pv = cross(sdir, e02);
+        const float det = dot(e01, pv);
+        const float safe_det = fabsf(det) > 1.0e-20f ? det : (det >= 0.0f ? 1.0e-20f : -1.0e-20f);
+        const float inv_det = 1.0f / safe_det;
+        const f3 tv = ref - p0;
+        const float su = dot(tv, pv) * inv_det;
+        const f3 qv = cross(tv, e01);
+        const float sv = dot(sdir, qv) * inv_det;
+        const float sdist = dot(e02, qv) * inv_det;
+        r.position = ref + sdir * sdist;
+        r.direction = sdir;
+        r.distance = sdist;
+        const bool svalid = (fabsf(det) > 1.0e-20f) && (sdist > 0.0f) && (su >= -1.0e-5f) && (sv >= -1.0e-5f) &&
+                            (su + sv <= 1.0f + 1.0e-5f) && (sa > 0.0f);
+        float pdf;
+        const bool pv_ok = triangle_pdf(ref, r.position, p0, p1, p2, pdf);
+        r.pdf = pdf;
+        r.valid = pv_ok && svalid;
+    } else {
+        r.position = apos;
+        r.direction = adir;
+        r.distance = ad;
+        float pdf;
+        const bool pv_ok = triangle_pdf(ref, r.position, p0, p1, p2, pdf);
+        r.pdf = pdf;
+        r.valid = pv_ok && ad2 > 1.0e-12f;
+    }
+    return r;
+}
+
+__device__ __forceinline__ bool sampled_side(uint32_t s, f3 ng, f3 dir) {
+    const bool back = dot(ng, neg(dir)) < 0.0f;
+    const bool front_ok = s == 1u || s == 2u || s == 4u;
+    const bool back_ok = s == 1u || s == 3u || s == 4u;
+    return back ? back_ok : front_ok;
+}
+
+// ===========================================================================
+// light transport callables (src/luisa/path_tracer_lighting.cpp)
+// ===========================================================================
+__device__ __forceinline__ float power_heuristic(float a, float b) {
+    const float a2 = a * a, b2 = b * b;
+    return a2 / fmaxf(a2 + b2, 1.0e-20f);
+}
+__device__ __forceinline__ float forward_light_weight(const KernelParams &k, float fpdf, float npdf, bool competing,
+                                                      bool available) {
+    if (k.mis_mode == 1u) return 1.0f;
+    if (k.mis_mode == 2u) return (competing && available) ? 0.0f : 1.0f;
+    return (competing && available) ? power_heuristic(fpdf, npdf) : 1.0f;
+}
+__device__ __forceinline__ float nee_light_weight(const KernelParams &k, float npdf, float fpdf) {
+    if (k.mis_mode == 2u) return 1.0f;
+    if (k.mis_mode == 1u) return fpdf <= 0.0f ? 1.0f : 0.0f;
+    return power_heuristic(npdf, fpdf);
+}
+__device__ __forceinline__ f3 film_clamp(f3 c, uint32_t depth, float direct, float indirect) {
+    c.x = isfinite(c.x) ? c.x : 0.0f;
+    c.y = isfinite(c.y) ? c.y : 0.0f;
+    c.z = isfinite(c.z) ? c.z : 0.0f;
+    const float limit = depth > 0u ? indirect : direct;
+    const float mag = fabsf(c.x) + fabsf(c.y) + fabsf(c.z);
+    if (limit > 0.0f && mag > limit) return c * (limit / fmaxf(mag, 1.0e-20f));
+    return c;
+}
+__device__ __forceinline__ float light_roulette(f3 u, float random, float inv) {
+    const float maximum = fmaxf(fabsf(u.x), fmaxf(fabsf(u.y), fabsf(u.z)));
+    const float probability = maximum * inv;
+    const bool roulette = (inv > 0.0f) && (probability < 1.0f);
+    const bool survives = !roulette || (random < probability);
+    const float inverse = roulette ? 1.0f / fmaxf(probability, 1.0e-20f) : 1.0f;
+    return survives ? inverse : 0.0f;
+}
+
+// ===========================================================================
+// camera (src/luisa/path_tracer_camera.cpp:10-244, perspective)
+// ===========================================================================
+__device__ __forceinline__ float filter_lookup(const float *table, float random) {
+    const float coordinate = clampf(random, 0.0f, 1.0f) * 1023.0f;
+    uint32_t index = (uint32_t)coordinate;
+    if (index > 1023u) index = 1023u;
+    const uint32_t next = index + 1u < 1023u ? index + 1u : 1023u;
+    const float t = coordinate - (float)index;
+    const float a = table[index], b = table[next];
+    return a + t * (b - a);
+}
+
+__device__ Ray camera_ray(const KernelParams &k, const float4 *sobol, const float *filter, uint32_t fx, uint32_t fy,
+                          uint32_t sample, uint32_t &rng_hash) {
+    const uint32_t cycles_y = k.full_height - 1u - fy;
+    rng_hash = pixel_hash(fx, cycles_y, k.seed);
+    float4 fs = sample4(sobol, k.sequence_size, sample, rng_hash, 0u);
+    if (sample == 0u) {
+        fs.x = 0.5f;
+        fs.y = 0.5f;
+    }
+    const float jx = filter_lookup(filter, fs.x);
+    const float jy = 1.0f - filter_lookup(filter, fs.y);
+    const float width = (float)k.full_width, height = (float)k.full_height;
+    const float sx = 2.0f * ((float)fx + jx) / width - 1.0f + 2.0f * k.shift_x;
+    const float sy = 1.0f - 2.0f * ((float)fy + jy) / height + 2.0f * k.shift_y;
+    const f3 ld = normalize(V(sx * k.tan_h, sy * k.tan_v, -1.0f));
+    const float clip_cos = -ld.z;
+    const f3 wo = cam_point(k.camera, V(0.0f, 0.0f, 0.0f));
+    const f3 d = safe_normalize(cam_dir(k.camera, ld), V(0.0f, 0.0f, -1.0f));
+    const float safe = fmaxf(fabsf(clip_cos), 1.0e-20f);
+    const float c0 = k.near_clip / safe;
+    const float c1 = (k.far_clip - k.near_clip) / safe;
+    Ray r;
+    r.o = wo + d * c0;
+    r.d = d;
+    r.tmin = 0.0f;
+    r.tmax = c1;
+    return r;
+}
+// ===========================================================================
+// per-material tables evaluated once per parameter set
+// ===========================================================================
+
+// Transparent extinction of a material for shadow rays (the transparent
+// closure weights of its tree, summed in closure order and clamped to [0, 1];
+// src/luisa/path_tracer_geometry.cpp:75-174) and its constant emission.
+__device__ __forceinline__ void material_tables(const SceneView &s, uint32_t material, float4 &shadow,
+                                                float4 &emission) {
+    ClosureSet cs;
+    material_closures(s, material, V(0.0f, 0.0f, 1.0f), cs);
+    f3 ext = V(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (uint32_t i = 0; i < kMaxClosures; ++i)
+        if (i < cs.count && cs.kind[i] == KIND_TRANSPARENT) ext = ext + cs.weight[i];
+    ext = vclamp(ext, 0.0f, 1.0f);
+    shadow = make_float4(ext.x, ext.y, ext.z, 0.0f);
+    const f3 e = material_constant_emission(s, material);
+    emission = make_float4(e.x, e.y, e.z, 0.0f);
+}
+
+__device__ __forceinline__ uint32_t hit_material(const SceneView &s, uint32_t inst_index, uint32_t prim) {
+    const Instance &inst = s.instances[inst_index];
+    const Mesh &mesh = s.meshes[inst.mesh];
+    return material_of(s, inst, mesh, prim);
+}
+
+// ===========================================================================
+// the path integrator as a step function (src/luisa/path_kernel_pipeline.cpp:
+// 73-185).  One step consumes the closest hit of the current ray, adds
+// emission, applies Russian roulette, writes the primary AOVs, builds the NEE
+// shadow ray and samples the continuation.  The megakernel runs the steps of
+// one path back to back; the wavefront integrator keeps the same state in HBM
+// between its trace and shade kernels.  Both add radiance in the same order.
+// ===========================================================================
+constexpr uint32_t PATH_PREV_DELTA = 1u;       // previous scatter was singular
+constexpr uint32_t PATH_SHADOW_PENDING = 2u;   // a shadow ray awaits its result
+constexpr uint32_t PATH_FINISHING = 4u;        // path ended, pending shadow only
+constexpr uint32_t PATH_DONE = 8u;             // path complete
+
+struct PathVars {
+    Ray ray;
+    uint32_t src_obj, src_prim;
+    f3 throughput;
+    float prev_pdf;
+    f3 radiance;
+    float transparency;
+    f3 normal, albedo;
+    PathState ps;
+    uint32_t sample, rng_hash, step, bits;
+};
+
+struct ShadowRay {
+    Ray ray;
+    TraceQuery q;
+    f3 L;            // throughput * surviving light estimate
+    uint32_t depth;  // bounce at NEE time (clamp selection)
+    uint32_t tdepth; // transparent bounces already taken
+    bool active;
+};
+
+__device__ __forceinline__ TraceQuery path_query(const PathVars &p) {
+    TraceQuery q;
+    q.visibility = contract_visibility(p.ps.visibility);
+    q.src_obj = p.src_obj;
+    q.src_prim = p.src_prim;
+    q.light_obj = q.light_prim = 0xffffffffu;
+    return q;
+}
+
+// camera ray and initial state (src/luisa/path_kernel_setup.cpp)
+__device__ __forceinline__ void init_path(const KernelParams &k, const float4 *sobol, const float *filter,
+                                          uint32_t fx, uint32_t fy, uint32_t sample, PathVars &p) {
+    p.ray = camera_ray(k, sobol, filter, fx, fy, sample, p.rng_hash);
+    p.src_obj = p.src_prim = 0xffffffffu;
+    p.throughput = V(1.0f, 1.0f, 1.0f);
+    p.prev_pdf = 0.0f;
+    p.radiance = V(0.0f, 0.0f, 0.0f);
+    p.transparency = 0.0f;
+    p.normal = V(0.0f, 0.0f, 0.0f);
+    p.albedo = V(0.0f, 0.0f, 0.0f);
+    p.ps.flag = FLAG_MIS_SKIP | FLAG_TRANSPARENT_BACKGROUND;
+    p.ps.visibility = CVIS_CAMERA;
+    p.ps.bounce = p.ps.diffuse_bounce = p.ps.glossy_bounce = p.ps.transmission_bounce = p.ps.transparent_bounce = 0u;
+    p.ps.rng_offset = 16u;
+    p.sample = sample;
+    p.step = 0u;
+    p.bits = PATH_PREV_DELTA | (k.max_path_steps == 0u ? PATH_DONE : 0u);
+}
+
+// transparent shadow continuation (src/luisa/path_tracer_geometry.cpp:75-174):
+// given the closest hit of the shadow ray, either finish (returns true) or
+// advance past a transparent surface
+__device__ __forceinline__ bool shadow_hit(const SceneView &s, const KernelParams &k, const Hit &sh, Ray &sray,
+                                           uint32_t &tdepth, f3 &transmittance) {
+    if (!sh.valid) return true;
+    if (tdepth >= k.transparent_max) {
+        transmittance = V(0.0f, 0.0f, 0.0f);
+        return true;
+    }
+    const uint32_t material = hit_material(s, sh.inst, sh.prim);
+    f3 ext = V(0.0f, 0.0f, 0.0f);
+    if (material != kNone) {
+        const float4 e = s.material_shadow[material];
+        ext = V(e.x, e.y, e.z);
+    }
+    if (fmaxf(ext.x, fmaxf(ext.y, ext.z)) > 0.0f) {
+        transmittance = transmittance * ext;
+        tdepth += 1u;
+        sray.tmin = next_float_up(sh.t);
+        return false;
+    }
+    transmittance = V(0.0f, 0.0f, 0.0f);
+    return true;
+}
+
+__device__ __forceinline__ void apply_shadow(const KernelParams &k, PathVars &p, f3 L, uint32_t depth,
+                                             f3 transmittance) {
+    if (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f)
+        p.radiance = p.radiance + film_clamp(L * transmittance, depth, k.clamp_direct, k.clamp_indirect);
+}
+
+// One path step.  Returns true when the path continues with p.ray; sr.active
+// is set when a shadow ray must be traced and applied before the next step.
+template <bool TRACE>
+__device__ bool path_step(const SceneView &s, const KernelParams &k, const float4 *sobol, PathVars &p,
+                          const Hit &hit, ShadowRay &sr, uint32_t *ev) {
+    sr.active = false;
+    if (TRACE && ev) {
+        ev[0] = 1u;
+        ev[1] = p.ps.rng_offset;
+        ev[2] = ev[3] = ev[4] = ev[5] = 0xffffffffu;
+        ev[7] = 0u;
+    }
+    const float u_term = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 0u).x;
+    if (TRACE && ev) ev[6] = __float_as_uint(u_term);
+    if (!hit.valid) {
+        if (TRACE && ev) ev[0] |= 4u;
+        // background (src/luisa/path_kernel_background_event.cpp:20-159)
+        const f3 bg = V(s.background[0], s.background[1], s.background[2]);
+        const f3 c = film_clamp((p.throughput * bg) * 1.0f, p.ps.bounce > 1u ? 1u : 0u, k.clamp_direct,
+                                k.clamp_indirect);
+        const bool world_visible = (s.world_visibility & contract_visibility(p.ps.visibility)) != 0u;
+        const bool transparent_bg =
+            k.transparent_background != 0u && (p.ps.flag & FLAG_TRANSPARENT_BACKGROUND) != 0u;
+        if (transparent_bg) {
+            p.transparency += (p.throughput.x + p.throughput.y + p.throughput.z) * (1.0f / 3.0f);
+        } else if (world_visible) {
+            p.radiance = p.radiance + c;
+        }
+        return false;
+    }
+    if (TRACE && ev) {
+        ev[0] |= 2u;
+        ev[2] = hit.object;
+        ev[3] = hit.cprim;
+    }
+    Surface sf;
+    resolve_surface(s, hit, p.ray, sf);
+    ClosureSet cs;
+    material_closures(s, sf.material, sf.ns, cs);
+    const bool terminate_after_transparent = (p.ps.flag & FLAG_TERMINATE_AFTER_TRANSPARENT) != 0u;
+    const bool terminate_on_next_surface = (p.ps.flag & FLAG_TERMINATE_ON_NEXT_SURFACE) != 0u;
+    const uint32_t lobe_mask = terminate_after_transparent ? EV_TRANSPARENT : 0x3fu;
+    // surface emission and forward MIS (src/luisa/path_kernel_surface_shading.cpp:115-181)
+    float emission_weight = 1.0f;
+    if (k.nee && s.emitter_count > 0u && sf.emission_sampling != 0u) {
+        const bool competing = p.ps.bounce > 0u && (p.bits & PATH_PREV_DELTA) == 0u;
+        const f3 oriented = sf.back_facing ? neg(sf.ng) : sf.ng;
+        const float area = 0.5f * length(cross(sf.wp1 - sf.wp0, sf.wp2 - sf.wp0));
+        const float selection_pdf = area * s.triangle_area_pdf;
+        float tpdf;
+        const bool tvalid = triangle_pdf(p.ray.o, sf.position, sf.wp0, sf.wp1, sf.wp2, tpdf);
+        const f3 dir = normalize_or(sf.position - p.ray.o, V(0.0f, 0.0f, 1.0f));
+        const bool side = sampled_side(sf.emission_sampling, oriented, dir);
+        const float value = tpdf * selection_pdf;
+        const bool valid = side && tvalid && area > 0.0f && value > 0.0f;
+        const float light_pdf = valid ? value : 0.0f;
+        emission_weight = forward_light_weight(k, p.prev_pdf, light_pdf, competing, light_pdf > 0.0f);
+    }
+    p.radiance = p.radiance + film_clamp((p.throughput * cs.emission) * emission_weight,
+                                         p.ps.bounce > 1u ? 1u : 0u, k.clamp_direct, k.clamp_indirect);
+    if (terminate_on_next_surface) return false;
+    // Russian roulette (cycles_path_state.h:416-446)
+    {
+        const bool transparent = (p.ps.flag & FLAG_TRANSPARENT) != 0u;
+        const bool before_minimum =
+            transparent ? p.ps.transparent_bounce <= k.transparent_min : p.ps.bounce <= k.min_bounces;
+        const f3 t = p.throughput;
+        const float roulette = fminf(sqrtf(fmaxf(fabsf(t.x), fmaxf(fabsf(t.y), fabsf(t.z)))), 1.0f);
+        const float prob = before_minimum ? 1.0f : roulette;
+        if (TRACE && ev) ev[7] = __float_as_uint(prob);
+        if (prob <= 0.0f) {
+            if (TRACE && ev) ev[0] |= 8u;
+            return false;
+        }
+        if (prob != 1.0f) {
+            if (u_term >= prob) {
+                if (TRACE && ev) ev[0] |= 8u;
+                return false;
+            }
+            p.throughput = p.throughput / prob;
+        }
+    }
+    if (p.ps.bounce == 0u) {
+        f3 aov_albedo, aov_normal, aov_tr;
+        surface_aov(cs, sf.ns, aov_albedo, aov_normal, aov_tr);
+        p.albedo = p.albedo + p.throughput * aov_albedo;
+        const f3 alpha = vclamp(V(1.0f, 1.0f, 1.0f) - aov_tr, 0.0f, 1.0f);
+        const float avg_alpha = (alpha.x + alpha.y + alpha.z) * (1.0f / 3.0f);
+        const bool recorded = (p.ps.flag & FLAG_SINGLE_PASS_DONE) != 0u;
+        const bool writes = !recorded && (k.pass_alpha_threshold == 0.0f || avg_alpha >= k.pass_alpha_threshold);
+        if (writes) {
+            p.normal = aov_normal;
+            p.ps.flag |= FLAG_SINGLE_PASS_DONE;
+        }
+    }
+    const uint32_t rflags = runtime_flags(cs, sf.back_facing);
+    // NEE: emissive mesh component (src/luisa/path_kernel_nee_emissive_mesh.cpp:30-298)
+    if (k.nee && (rflags & RT_BSDF_HAS_EVAL) != 0u) {
+        const float4 ul = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 1u);
+        // flat-CDF light choice (src/luisa/path_tracer_light_distribution.cpp:12-52)
+        uint32_t first = 0u, len = s.distribution_count + 1u;
+        while (len > 0u) {
+            const uint32_t half = len >> 1u;
+            const uint32_t mid = first + half;
+            if (ul.z < s.distribution[mid].cumulative) {
+                len = half;
+            } else {
+                first = mid + 1u;
+                len = len - half - 1u;
+            }
+        }
+        uint32_t light = first > 0u ? first - 1u : 0u;
+        if (light > s.distribution_count - 1u) light = s.distribution_count - 1u;
+        const DistEntry entry = s.distribution[light];
+        if (entry.kind == 0u) {
+            if (TRACE && ev) {
+                ev[0] |= 16u;
+                ev[4] = light;
+            }
+            const Emitter em = s.emitters[entry.index];
+            const Instance &einst = s.instances[em.instance];
+            const Mesh &emesh = s.meshes[einst.mesh];
+            const uint32_t *et = s.indices + 3u * (emesh.index_offset + em.prim);
+            const f3 lp0 = load3(s.positions + 3u * (emesh.vertex_offset + et[0]));
+            const f3 lp1 = load3(s.positions + 3u * (emesh.vertex_offset + et[1]));
+            const f3 lp2 = load3(s.positions + 3u * (emesh.vertex_offset + et[2]));
+            const f3 p0 = mat_mul(einst.o2w, lp0, 1.0f);
+            const f3 p1 = mat_mul(einst.o2w, lp1, 1.0f);
+            const f3 p2 = mat_mul(einst.o2w, lp2, 1.0f);
+            const f3 local_n = normalize_or(cross(lp1 - lp0, lp2 - lp0), V(0.0f, 0.0f, 1.0f));
+            const f3 egn = normalize_or(mat_mul(einst.n2w, local_n, 0.0f), V(0.0f, 0.0f, 1.0f));
+            const float earea = 0.5f * length(cross(p1 - p0, p2 - p0));
+            const LightSample ls = sample_triangle(sf.position, p0, p1, p2, ul.x, ul.y);
+            const bool side = sampled_side(em.emission_sampling, egn, ls.direction);
+            const float lpdf = ls.pdf * entry.selection_pdf;
+            if (ls.valid && side && earea > 0.0f && lpdf > 0.0f) {
+                const uint32_t eobj = em.instance;
+                const uint32_t eprim = einst.prim_offset + em.prim;
+                const bool is_transmission = dot(ls.direction, sf.ns) < 0.0f;
+                const bool same = eprim == sf.cprim && eobj == sf.object;
+                const f3 oriented = is_transmission ? neg(sf.ng) : sf.ng;
+                const bool reject_self = same && dot(ls.direction, oriented) > 0.0f;
+                if (!reject_self) {
+                    const float4 le4 = s.material_emission[em.material];
+                    const f3 le = V(le4.x, le4.y, le4.z);
+                    const Eval evl = evaluate_mixture(cs, sf.incoming, sf.ns, ls.direction, lobe_mask);
+                    const float mis = nee_light_weight(k, lpdf, evl.pdf);
+                    const f3 weighted = (evl.f * le) * (mis / lpdf);
+                    if (any_nonzero(weighted)) {
+                        const float u_lightterm =
+                            sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 2u).x;
+                        const float rr = light_roulette(weighted, u_lightterm, k.light_inv_rr_threshold);
+                        const f3 surviving = weighted * rr;
+                        if (any_nonzero(surviving)) {
+                            const f3 so = surface_shadow_origin(s, sf, ls.direction);
+                            const f3 soff = ls.position - so;
+                            const float sdist = sqrtf(fmaxf(dot(soff, soff), 1.0e-20f));
+                            sr.ray.o = so;
+                            sr.ray.d = soff / sdist;
+                            sr.ray.tmin = 0.0f;
+                            sr.ray.tmax = sdist;
+                            sr.q.visibility = VIS_SHADOW;
+                            sr.q.src_obj = sf.object;
+                            sr.q.src_prim = sf.cprim;
+                            sr.q.light_obj = eobj;
+                            sr.q.light_prim = eprim;
+                            sr.L = p.throughput * surviving;
+                            sr.depth = p.ps.bounce;
+                            sr.tdepth = p.ps.transparent_bounce;
+                            sr.active = true;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // surface scatter (src/luisa/path_kernel_surface_scatter.cpp:16-324)
+    const float4 ub = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 3u);
+    const BsdfSample bs = sample_closures(cs, sf, ub.z, ub.x, ub.y, lobe_mask);
+    if (TRACE && ev) ev[5] = bs.selected;
+    if (!bs.valid || bs.eval.pdf <= 0.0f) return false;
+    if (TRACE && ev) ev[0] |= 32u;
+    const bool transparent = (bs.eval.events & EV_TRANSPARENT) != 0u;
+    const bool singular = (bs.eval.events & EV_SINGULAR) != 0u;
+    if (TRACE && ev && transparent) ev[0] |= 64u;
+    p.throughput = p.throughput * (bs.eval.f / bs.eval.pdf);
+    const f3 t = p.throughput;
+    if (any_nan(t) || t.x < 0.0f || t.y < 0.0f || t.z < 0.0f) return false;
+    next_surface(p.ps, label_from_events(bs.eval.events), bs.runtime_flags, k);
+    if (!transparent) {
+        p.prev_pdf = bs.eval.pdf;
+        p.bits = singular ? (p.bits | PATH_PREV_DELTA) : (p.bits & ~PATH_PREV_DELTA);
+        const f3 nd = normalize(bs.wi);
+        p.ray.o = surface_ray_origin(s, sf, nd);
+        p.ray.d = nd;
+        p.ray.tmin = 0.0f;
+        p.ray.tmax = 1.0e30f;
+    } else {
+        p.ray.tmin = next_float_up(hit.t);
+    }
+    p.src_obj = sf.object;
+    p.src_prim = sf.cprim;
+    p.step += 1u;
+    return p.step < k.max_path_steps;
+}
+
+}  // namespace
+}  // namespace psx

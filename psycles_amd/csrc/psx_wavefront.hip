@@ -1,0 +1,398 @@
+// psx_wavefront.hip - wavefront integrator (the batched render path, gfx950).
+//
+// A dispatch of `count` samples holds count x padded_pixels paths in HBM and
+// advances them in lock-step iterations of two kernels:
+//
+//   trace  persistent waves run closest-hit traversal for continuation rays,
+//          or the whole transparent shadow loop for NEE shadow rays; lanes
+//          whose query finished are refilled from the queue (one atomic per
+//          refill) while the rest of the wave keeps traversing
+//   shade  consumes the hit of each live path: pending shadow result, then one
+//          path step (emission, RR, AOVs, NEE sampling, BSDF sampling); new
+//          rays are appended to the next queue with one atomic per wave
+//          (ballot + prefix count)
+//
+// Shadow results are applied at the start of the next shade, before that
+// step's emission, which is the order in which the megakernel adds them, and a
+// final pass adds each pixel's samples in sample order; the film is therefore
+// bit-identical to the megakernel's.  Queue counts live in device memory, so
+// the host enqueues all iterations without synchronising; an empty iteration
+// costs two near-empty launches.
+#include "psx_launch.h"
+#include "psx_path.h"
+
+namespace psx {
+namespace {
+
+constexpr uint32_t kTraceBlock = 256;
+constexpr uint32_t kShadeBlock = 256;
+constexpr uint32_t kSkip = 0xffffffffu;  // queue/list entry of a padding lane
+constexpr uint32_t kRefill = 16u;        // idle lanes that trigger a refill
+
+// one atomic per wave; returns this lane's slot (valid where pred is true)
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return 0u;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1);
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, static_cast<int>(leader), 64);
+    return base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__device__ __forceinline__ void add_counter(unsigned long long *dst, uint32_t value) {
+    unsigned long long v = value;
+    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_down(v, offset, 64);
+    if ((threadIdx.x & 63) == 0 && v != 0ull) atomicAdd(dst, v);
+}
+
+__device__ __forceinline__ void store_path(const WavefrontView &w, uint32_t i, const PathVars &p) {
+    w.ray_o[i] = make_float4(p.ray.o.x, p.ray.o.y, p.ray.o.z, p.ray.tmin);
+    w.ray_d[i] = make_float4(p.ray.d.x, p.ray.d.y, p.ray.d.z, p.ray.tmax);
+    w.aux[i] = make_uint4(p.src_obj, p.src_prim, p.step, p.bits);
+    w.throughput[i] = make_float4(p.throughput.x, p.throughput.y, p.throughput.z, p.prev_pdf);
+    w.radiance[i] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, p.transparency);
+    w.normal[i] = make_float4(p.normal.x, p.normal.y, p.normal.z, 0.0f);
+    w.albedo[i] = make_float4(p.albedo.x, p.albedo.y, p.albedo.z, 0.0f);
+    w.state0[i] = make_uint4(p.ps.flag, p.ps.visibility, p.ps.bounce, p.ps.diffuse_bounce);
+    w.state1[i] = make_uint4(p.ps.glossy_bounce, p.ps.transmission_bounce, p.ps.transparent_bounce, p.ps.rng_offset);
+    w.ident[i] = make_uint2(p.sample, p.rng_hash);
+}
+
+__device__ __forceinline__ PathVars load_path(const WavefrontView &w, uint32_t i) {
+    PathVars p;
+    const float4 o = w.ray_o[i], d = w.ray_d[i];
+    p.ray.o = V(o.x, o.y, o.z);
+    p.ray.tmin = o.w;
+    p.ray.d = V(d.x, d.y, d.z);
+    p.ray.tmax = d.w;
+    const uint4 a = w.aux[i];
+    p.src_obj = a.x;
+    p.src_prim = a.y;
+    p.step = a.z;
+    p.bits = a.w;
+    const float4 t = w.throughput[i];
+    p.throughput = V(t.x, t.y, t.z);
+    p.prev_pdf = t.w;
+    const float4 r = w.radiance[i];
+    p.radiance = V(r.x, r.y, r.z);
+    p.transparency = r.w;
+    const float4 n = w.normal[i];
+    p.normal = V(n.x, n.y, n.z);
+    const float4 al = w.albedo[i];
+    p.albedo = V(al.x, al.y, al.z);
+    const uint4 s0 = w.state0[i], s1 = w.state1[i];
+    p.ps.flag = s0.x;
+    p.ps.visibility = s0.y;
+    p.ps.bounce = s0.z;
+    p.ps.diffuse_bounce = s0.w;
+    p.ps.glossy_bounce = s1.x;
+    p.ps.transmission_bounce = s1.y;
+    p.ps.transparent_bounce = s1.z;
+    p.ps.rng_offset = s1.w;
+    const uint2 id = w.ident[i];
+    p.sample = id.x;
+    p.rng_hash = id.y;
+    return p;
+}
+
+__global__ void reset_kernel(WavefrontCounters *counters, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counters[i] = WavefrontCounters{0u, 0u, 0u, 0u};
+}
+
+// camera rays for samples [first, first + count) of every window pixel
+__global__ __launch_bounds__(kShadeBlock) void generate_kernel(KernelParams k, const float4 *sobol,
+                                                                const float *filter, WavefrontView w, uint32_t first,
+                                                                uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t total = count * w.padded_pixels;
+    bool live = false;
+    if (i < total) {
+        const uint32_t sample = first + i / w.padded_pixels;
+        const uint32_t rem = i % w.padded_pixels;
+        const uint32_t tile = rem >> 6, lane = rem & 63u;
+        const uint32_t tiles_x = (k.window_width + 7u) >> 3;
+        const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
+        const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
+        PathVars p;
+        if (px < k.window_width && py < k.window_height) {
+            init_path(k, sobol, filter, px + k.window_x, py + k.window_y, sample, p);
+            live = (p.bits & PATH_DONE) == 0u;
+        } else {
+            p.bits = PATH_DONE;
+            p.ray.o = p.ray.d = V(0.0f, 0.0f, 0.0f);
+            p.ray.tmin = p.ray.tmax = 0.0f;
+            p.src_obj = p.src_prim = 0xffffffffu;
+            p.throughput = p.radiance = p.normal = p.albedo = V(0.0f, 0.0f, 0.0f);
+            p.prev_pdf = p.transparency = 0.0f;
+            p.ps = PathState{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+            p.sample = sample;
+            p.rng_hash = 0u;
+            p.step = 0u;
+        }
+        store_path(w, i, p);
+    }
+    // iteration 0 lists every path in order; padding lanes are skip markers
+    if (i < total) {
+        w.queue[0][i] = live ? (i << 1) : kSkip;
+        w.list[0][i] = live ? i : kSkip;
+    }
+    if (i == 0u) {
+        w.counters[0].trace_count = total;
+        w.counters[0].shade_count = total;
+    }
+}
+
+// Persistent trace kernel.  Each lane owns one query at a time; when at least
+// kRefill lanes of a wave have finished, the wave claims that many queue
+// entries with one atomic and the idle lanes start them while the others keep
+// their traversal state (Aila & Laine, "Understanding the efficiency of ray
+// traversal on GPUs", 2009: persistent threads with ray replacement).
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void wf_trace_kernel(SceneView s, KernelParams k, WavefrontView w,
+                                                               uint32_t it, Counters *counters, uint32_t *spill) {
+    __shared__ uint32_t lds[kShortStack * kTraceBlock];
+    Stack st = make_stack(lds, spill, kTraceBlock, threadIdx.x, blockIdx.x * kTraceBlock + threadIdx.x,
+                          gridDim.x * kTraceBlock);
+    WavefrontCounters *c = w.counters + it;
+    const uint32_t count = c->trace_count;
+    const uint32_t *queue = w.queue[it & 1u];
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t rays = 0u, nodes = 0u, tris = 0u;
+    Traversal t;
+    bool active = false, exhausted = false, shadow = false;
+    uint32_t path = 0u, tdepth = 0u;
+    f3 transmittance = V(1.0f, 1.0f, 1.0f);
+    for (;;) {
+        if (!exhausted) {
+            // claim one queue entry per idle lane
+            const unsigned long long idle = __ballot(!active);
+            const uint32_t want = static_cast<uint32_t>(__popcll(idle));
+            const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(idle)) - 1);
+            uint32_t base = 0u;
+            if (lane == leader) base = atomicAdd(&c->trace_fetch, want);
+            base = __shfl(base, static_cast<int>(leader), 64);
+            exhausted = base + want >= count;
+            const uint32_t qi = base + static_cast<uint32_t>(__popcll(idle & below));
+            if (!active && qi < count) {
+                const uint32_t entry = queue[qi];
+                if (entry != kSkip) {
+                    path = entry >> 1;
+                    shadow = (entry & 1u) != 0u;
+                    if (shadow) {
+                        const float4 o = w.shadow_o[path], d = w.shadow_d[path];
+                        const uint4 sq = w.shadow_q[path];
+                        t.ray.o = V(o.x, o.y, o.z);
+                        t.ray.tmax = o.w;
+                        t.ray.d = V(d.x, d.y, d.z);
+                        t.ray.tmin = 0.0f;
+                        t.q.visibility = VIS_SHADOW;
+                        t.q.src_obj = sq.x;
+                        t.q.src_prim = sq.y;
+                        t.q.light_obj = sq.z;
+                        t.q.light_prim = sq.w;
+                        tdepth = __float_as_uint(w.shadow_l[path].w);
+                        transmittance = V(1.0f, 1.0f, 1.0f);
+                    } else {
+                        const float4 o = w.ray_o[path], d = w.ray_d[path];
+                        const uint4 a = w.aux[path];
+                        t.ray.o = V(o.x, o.y, o.z);
+                        t.ray.tmin = o.w;
+                        t.ray.d = V(d.x, d.y, d.z);
+                        t.ray.tmax = d.w;
+                        t.q.visibility = contract_visibility(w.state0[path].y);
+                        t.q.src_obj = a.x;
+                        t.q.src_prim = a.y;
+                        t.q.light_obj = t.q.light_prim = 0xffffffffu;
+                    }
+                    traversal_begin(s, t, st);
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        // traverse until enough lanes are idle to refill (or all are, at the end)
+        for (;;) {
+            if (active && !traversal_step<COUNT>(s, t, st, nodes, tris)) {
+                ++rays;
+                if (!shadow) {
+                    const Hit &h = t.best;
+                    w.hit[path] = make_uint4(h.valid ? h.inst : kNone, h.prim, h.cprim, __float_as_uint(h.t));
+                    w.hit_uv[path] = make_float2(h.u, h.v);
+                    active = false;
+                } else if (shadow_hit(s, k, t.best, t.ray, tdepth, transmittance)) {
+                    w.shadow_t[path] = make_float4(transmittance.x, transmittance.y, transmittance.z, 0.0f);
+                    active = false;
+                } else {
+                    // past a transparent surface: same shadow ray from the next float
+                    traversal_begin(s, t, st);
+                }
+            }
+            const uint32_t idle = static_cast<uint32_t>(__popcll(__ballot(!active)));
+            if (idle == 64u || (!exhausted && idle >= kRefill)) break;
+        }
+    }
+    add_counter(&counters->rays, rays);
+    if (COUNT) {
+        add_counter(&counters->nodes, nodes);
+        add_counter(&counters->tris, tris);
+    }
+}
+
+__global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, KernelParams k, const float4 *sobol,
+                                                               WavefrontView w, uint32_t it) {
+    const WavefrontCounters *c = w.counters + it;
+    WavefrontCounters *next = w.counters + it + 1u;
+    const uint32_t count = c->shade_count;
+    const uint32_t *list = w.list[it & 1u];
+    uint32_t *next_queue = w.queue[(it + 1u) & 1u];
+    uint32_t *next_list = w.list[(it + 1u) & 1u];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x % 64u < count; i += stride) {
+        bool cont = false, shadow = false;
+        uint32_t path = 0u;
+        if (i < count && list[i] != kSkip) {
+            path = list[i];
+            PathVars p = load_path(w, path);
+            if ((p.bits & PATH_SHADOW_PENDING) != 0u) {
+                const float4 L = w.shadow_l[path];
+                const float4 tr = w.shadow_t[path];
+                apply_shadow(k, p, V(L.x, L.y, L.z), __float_as_uint(w.shadow_d[path].w), V(tr.x, tr.y, tr.z));
+                p.bits &= ~PATH_SHADOW_PENDING;
+            }
+            if ((p.bits & PATH_FINISHING) != 0u) {
+                p.bits = (p.bits & ~PATH_FINISHING) | PATH_DONE;
+            } else {
+                const uint4 hv = w.hit[path];
+                const float2 uv = w.hit_uv[path];
+                Hit h;
+                h.valid = hv.x != kNone;
+                h.inst = hv.x;
+                h.object = hv.x;
+                h.prim = hv.y;
+                h.cprim = hv.z;
+                h.t = __uint_as_float(hv.w);
+                h.u = uv.x;
+                h.v = uv.y;
+                ShadowRay sr;
+                cont = path_step<false>(s, k, sobol, p, h, sr, nullptr);
+                shadow = sr.active;
+                if (shadow) {
+                    w.shadow_o[path] = make_float4(sr.ray.o.x, sr.ray.o.y, sr.ray.o.z, sr.ray.tmax);
+                    w.shadow_d[path] = make_float4(sr.ray.d.x, sr.ray.d.y, sr.ray.d.z, __uint_as_float(sr.depth));
+                    w.shadow_q[path] = make_uint4(sr.q.src_obj, sr.q.src_prim, sr.q.light_obj, sr.q.light_prim);
+                    w.shadow_l[path] = make_float4(sr.L.x, sr.L.y, sr.L.z, __uint_as_float(sr.tdepth));
+                    p.bits |= PATH_SHADOW_PENDING;
+                }
+                if (!cont) p.bits |= shadow ? PATH_FINISHING : PATH_DONE;
+            }
+            store_path(w, path, p);
+        }
+        const uint32_t qe = wave_append(&next->trace_count, cont);
+        if (cont) next_queue[qe] = path << 1;
+        const uint32_t qs = wave_append(&next->trace_count, shadow);
+        if (shadow) next_queue[qs] = (path << 1) | 1u;
+        const uint32_t ls = wave_append(&next->shade_count, cont || shadow);
+        if (cont || shadow) next_list[ls] = path;
+    }
+}
+
+// in-order film accumulation of the batch, one lane per window pixel
+__global__ __launch_bounds__(256) void accumulate_kernel(KernelParams k, WavefrontView w, FilmView film,
+                                                         uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w.padded_pixels) return;
+    const uint32_t tile = i >> 6, lane = i & 63u;
+    const uint32_t tiles_x = (k.window_width + 7u) >> 3;
+    const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
+    const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
+    if (px >= k.window_width || py >= k.window_height) return;
+    const size_t index = (size_t)py * k.window_width + px;
+    float4 c = reinterpret_cast<float4 *>(film.combined)[index];
+    float4 n = reinterpret_cast<float4 *>(film.normal)[index];
+    float4 a = reinterpret_cast<float4 *>(film.albedo)[index];
+    uint32_t completed = film.sample_count[index];
+    for (uint32_t s = 0; s < count; ++s) {
+        const uint32_t path = s * w.padded_pixels + i;
+        const float4 r = w.radiance[path];
+        const float4 pn = w.normal[path];
+        const float4 pa = w.albedo[path];
+        f3 rad = V(r.x, r.y, r.z);
+        if (any_nan(rad)) rad = V(0.0f, 0.0f, 0.0f);
+        c.x += rad.x;
+        c.y += rad.y;
+        c.z += rad.z;
+        c.w += r.w;
+        n.x += pn.x;
+        n.y += pn.y;
+        n.z += pn.z;
+        n.w += 1.0f;
+        a.x += pa.x;
+        a.y += pa.y;
+        a.z += pa.z;
+        a.w += 1.0f;
+        completed += 1u;
+    }
+    reinterpret_cast<float4 *>(film.combined)[index] = c;
+    reinterpret_cast<float4 *>(film.normal)[index] = n;
+    reinterpret_cast<float4 *>(film.albedo)[index] = a;
+    film.sample_count[index] = completed;
+}
+
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers called by the host library
+// ---------------------------------------------------------------------------
+uint32_t wavefront_lanes() {
+    const uint32_t a = resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<false>), kTraceBlock);
+    const uint32_t b = resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<true>), kTraceBlock);
+    return (a > b ? a : b) * kTraceBlock;
+}
+
+uint32_t wavefront_padded_pixels(const KernelParams &k) {
+    return ((k.window_width + 7u) >> 3) * ((k.window_height + 7u) >> 3) * 64u;
+}
+
+// One dispatch of samples [first, first + count).  When `events` is given, it
+// holds 2 * (k.max_path_steps + 1) events recorded around every trace launch.
+hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
+                            const FilmView &film, const WavefrontView &w, uint32_t first, uint32_t count,
+                            Counters *counters, bool count_nodes, uint32_t *spill, hipEvent_t *events,
+                            hipStream_t stream) {
+    const uint32_t total = count * w.padded_pixels;
+    if (total == 0u) return hipSuccess;
+    if (total > w.capacity) return hipErrorInvalidValue;
+    const uint32_t iterations = k.max_path_steps + 1u;
+    hipLaunchKernelGGL(reset_kernel, dim3((iterations + 1u + 63u) / 64u), dim3(64), 0, stream, w.counters,
+                       iterations + 1u);
+    hipLaunchKernelGGL(generate_kernel, dim3((total + kShadeBlock - 1u) / kShadeBlock), dim3(kShadeBlock), 0, stream, k,
+                       sobol, filter, w, first, count);
+    const void *trace_fn = count_nodes ? reinterpret_cast<const void *>(wf_trace_kernel<true>)
+                                       : reinterpret_cast<const void *>(wf_trace_kernel<false>);
+    const uint32_t trace_blocks = resident_blocks(trace_fn, kTraceBlock);
+    const uint32_t shade_blocks = resident_blocks(reinterpret_cast<const void *>(wf_shade_kernel), kShadeBlock);
+    for (uint32_t it = 0; it < iterations; ++it) {
+        if (events) (void)hipEventRecord(events[2u * it], stream);
+        if (count_nodes)
+            hipLaunchKernelGGL(wf_trace_kernel<true>, dim3(trace_blocks), dim3(kTraceBlock), 0, stream, s, k, w, it,
+                               counters, spill);
+        else
+            hipLaunchKernelGGL(wf_trace_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, stream, s, k, w, it,
+                               counters, spill);
+        if (events) (void)hipEventRecord(events[2u * it + 1u], stream);
+        hipLaunchKernelGGL(wf_shade_kernel, dim3(shade_blocks), dim3(kShadeBlock), 0, stream, s, k, sobol, w, it);
+    }
+    hipLaunchKernelGGL(accumulate_kernel, dim3((w.padded_pixels + 255u) / 256u), dim3(256), 0, stream, k, w, film,
+                       count);
+    return hipGetLastError();
+}
+
+}  // namespace psx

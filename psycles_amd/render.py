@@ -218,10 +218,12 @@ class RendererBackend:
     """luisa_backend::LuisaPathTracerBackend counterpart on one MI355X."""
 
     def __init__(self, device: int = 0, next_event_estimation: bool = True,
-                 max_samples_per_dispatch: int = 4, integrator: int = 0):
+                 max_samples_per_dispatch: int = 4, integrator: int = abi.INTEGRATOR_MEGAKERNEL,
+                 count_traversal: bool = False):
         self.lib = abi.load_library()
+        flags = abi.BACKEND_COUNT_TRAVERSAL if count_traversal else 0
         opts = abi.BackendOptions(1 if next_event_estimation else 0, max_samples_per_dispatch,
-                                  integrator, 0)
+                                  integrator, flags)
         handle = C.c_void_p()
         abi.check("psx_backend_create", self.lib.psx_backend_create(device, C.byref(opts),
                                                                     C.byref(handle)))
