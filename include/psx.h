@@ -201,11 +201,18 @@ typedef struct psx_light {
     float spot_angle;
     float spot_smooth;
     float angle;
-    uint32_t flags;                  /* bit0 normalize, bit1 ellipse,
-                                        bit2 sphere, bit3 use_mis,
-                                        bit4 cast_shadow                    */
+    uint32_t flags;                  /* enum psx_light_flags               */
     uint32_t max_bounces;
 } psx_light;
+
+/* LightDesc booleans (scene.h:377-403). */
+enum psx_light_flags {
+    PSX_LIGHT_FLAG_NORMALIZE = 1u,
+    PSX_LIGHT_FLAG_ELLIPSE = 2u,
+    PSX_LIGHT_FLAG_SPHERE = 4u,
+    PSX_LIGHT_FLAG_USE_MIS = 8u,
+    PSX_LIGHT_FLAG_CAST_SHADOW = 16u
+};
 
 /* contract::CameraDesc (scene.h:347-367), perspective subset. */
 typedef struct psx_camera {

@@ -824,7 +824,23 @@ typedef struct emitter {
     uint32_t emission_sampling;
 } emitter;
 
+/* Analytic lamp (LightGpu, src/luisa/path_tracer_types.h:142-168, built by
+ * src/luisa/path_tracer_scene.cpp:1723-1870): point, spot or distant. */
+#define LAMP_NORMALIZE 1u
+#define LAMP_SPHERE 4u
+#define LAMP_USE_MIS 8u
+#define LAMP_FORWARD 32u
+
+typedef struct olamp {
+    uint32_t type;
+    v3 position, axis_x, axis_y, axis_z, axis_scale, color;
+    float power, radius, spot_angle, spot_smooth, angle;
+    uint32_t flags, visibility, max_bounces;
+} olamp;
+
 struct oracle_scene {
+    olamp *lamps;
+    uint32_t lamp_count;
     omaterial *materials;
     uint32_t material_count;
     omesh *meshes;
@@ -1199,6 +1215,44 @@ static void *dupmem(const void *src, size_t bytes) {
     return p;
 }
 
+/* LightDesc -> LightGpu (src/luisa/path_tracer_scene.cpp:1735-1870) */
+static void lamp_from_desc(const psx_light *l, olamp *lp) {
+    lp->type = l->type;
+    float len[3];
+    v3 axis[3];
+    for (int c = 0; c < 3; ++c) {
+        const v3 a = V3(l->transform[4 * c], l->transform[4 * c + 1], l->transform[4 * c + 2]);
+        const float n = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+        if (n <= 1.0e-20f) {
+            axis[c] = V3(0, 0, 0);
+            len[c] = 0.0f;
+        } else {
+            axis[c] = V3(a.x / n, a.y / n, a.z / n);
+            len[c] = n;
+        }
+    }
+    if (axis[2].x == 0.0f && axis[2].y == 0.0f && axis[2].z == 0.0f) axis[2] = V3(0, 0, 1);
+    lp->position = V3(l->transform[12], l->transform[13], l->transform[14]);
+    lp->axis_x = axis[0];
+    lp->axis_y = axis[1];
+    lp->axis_z = axis[2];
+    lp->axis_scale = V3(len[0], len[1], len[2]);
+    lp->color = load3(l->color);
+    lp->power = l->power;
+    lp->radius = l->size;
+    lp->spot_angle = l->spot_angle;
+    lp->spot_smooth = l->spot_smooth;
+    lp->angle = l->angle;
+    lp->visibility = l->visibility_mask;
+    lp->max_bounces = l->max_bounces;
+    const int use_mis = (l->flags & PSX_LIGHT_FLAG_USE_MIS) != 0u;
+    const int effective_mis = use_mis && (l->type == PSX_LIGHT_DISTANT ? l->angle > 0.0f : l->size > 0.0f);
+    lp->flags = ((l->flags & PSX_LIGHT_FLAG_NORMALIZE) ? LAMP_NORMALIZE : 0u) |
+                ((l->flags & PSX_LIGHT_FLAG_SPHERE) ? LAMP_SPHERE : 0u) |
+                (effective_mis ? LAMP_USE_MIS : 0u) |
+                (effective_mis && l->type != PSX_LIGHT_DISTANT ? LAMP_FORWARD : 0u);
+}
+
 oracle_scene *oracle_scene_create(const psx_scene_desc *d, int brute_force) {
     if (!d) return NULL;
     oracle_scene *s = (oracle_scene *)calloc(1, sizeof(oracle_scene));
@@ -1206,15 +1260,21 @@ oracle_scene *oracle_scene_create(const psx_scene_desc *d, int brute_force) {
     s->camera = d->camera;
     s->world_visibility = d->world_visibility_mask;
     s->background = load3(d->world_color);
-    /* background lights fold into the constant world radiance
-     * (src/luisa/path_tracer_scene.cpp:1723-1734) */
+    /* background lights fold into the constant world radiance; point, spot
+     * and distant lamps become analytic emitters
+     * (src/luisa/path_tracer_scene.cpp:1723-1870) */
+    s->lamps = (olamp *)calloc(d->light_count ? d->light_count : 1, sizeof(olamp));
     for (uint32_t i = 0; i < d->light_count; ++i) {
         const psx_light *l = &d->lights[i];
         if (l->type == PSX_LIGHT_BACKGROUND) {
             s->background.x += l->color[0] * l->power;
             s->background.y += l->color[1] * l->power;
             s->background.z += l->color[2] * l->power;
+            continue;
         }
+        if (l->type != PSX_LIGHT_POINT && l->type != PSX_LIGHT_SPOT && l->type != PSX_LIGHT_DISTANT)
+            continue;
+        lamp_from_desc(l, &s->lamps[s->lamp_count++]);
     }
     s->material_count = d->material_count;
     s->materials = (omaterial *)calloc(d->material_count ? d->material_count : 1, sizeof(omaterial));
@@ -1369,8 +1429,8 @@ oracle_scene *oracle_scene_create(const psx_scene_desc *d, int brute_force) {
         }
         s->emitters = em;
         s->emitter_count = n;
-        s->distribution = (dist_entry *)malloc(sizeof(dist_entry) * (n + 2u));
-        s->distribution_count = build_distribution(areas, n, 0u, 0, s->distribution,
+        s->distribution = (dist_entry *)malloc(sizeof(dist_entry) * (n + s->lamp_count + 2u));
+        s->distribution_count = build_distribution(areas, n, s->lamp_count, 0, s->distribution,
                                                    &s->triangle_area_pdf,
                                                    &s->light_selection_pdf);
         if (!(s->distribution[s->distribution_count].cumulative > 0.0f))
@@ -1403,6 +1463,7 @@ void oracle_scene_destroy(oracle_scene *s) {
     free(s->tlas_prims);
     free(s->emitters);
     free(s->distribution);
+    free(s->lamps);
     free(s);
 }
 
@@ -2205,6 +2266,414 @@ static float light_roulette(v3 u, float random, float inv_threshold) {
 }
 
 /* ======================================================================= */
+/* analytic lamps: sampling and forward intersection                       */
+/* (include/psycles/luisa/analytic_light_sampling.h:84-1118,               */
+/*  include/psycles/luisa/analytic_light_intersection.h:23-260,            */
+/*  include/psycles/luisa/cycles_sample_mapping.h:331-387)                 */
+/* ======================================================================= */
+#define SM_PI 3.14159265358979323846f
+#define SM_INV_PI 0.31830988618379067154f
+#define SM_INV_TWO_PI 0.15915494309189533577f
+#define AL_PI 3.1415926535897932f
+#define AL_INV_PI 0.3183098861837907f
+#define RAY_MAXIMUM 1.0e30f
+
+/* analytic_light_sampling::safe_normalize (:120-131) */
+static v3 lamp_normalize(v3 value, v3 fallback) {
+    const float m2 = vdot(value, value);
+    return m2 > 0.0f ? vdivs(value, sqrtf(fmaxf(m2, 1.0e-30f))) : fallback;
+}
+
+/* sin_squared_to_one_minus_cosine (cycles_sample_mapping.h:343-350) */
+static float sin2_to_one_minus_cos(float s2) {
+    return s2 > 0.0004f ? 1.0f - sqrtf(fmaxf(1.0f - s2, 0.0f)) : 0.5f * s2;
+}
+
+typedef struct cone_t {
+    v3 direction;
+    float cosine, pdf;
+} cone_t;
+
+/* sample_uniform_cone (cycles_sample_mapping.h:356-378) */
+static cone_t sample_uniform_cone(v3 axis, float omc, v2 random) {
+    const v2 disk = sample_uniform_disk(random);
+    const float r2 = disk.x * disk.x + disk.y * disk.y;
+    const float cosine = 1.0f - r2 * omc;
+    const float radial = sqrtf(fmaxf(omc * (2.0f - omc * r2), 0.0f));
+    const float mx = disk.x * radial, my = disk.y * radial;
+    v3 t, b;
+    make_orthonormals(axis, &t, &b);
+    const v3 fd = vadd(vadd(vscale(t, mx), vscale(b, my)), vscale(axis, cosine));
+    cone_t c;
+    const int finite = omc > 0.0f;
+    c.direction = finite ? fd : axis;
+    c.cosine = finite ? cosine : 1.0f;
+    c.pdf = finite ? SM_INV_TWO_PI / omc : 1.0f;
+    return c;
+}
+
+/* sample_uniform_sphere (cycles_sample_mapping.h:380-387) */
+static v3 sample_uniform_sphere(v2 r) {
+    const float z = 1.0f - 2.0f * r.x;
+    const float radius = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+    const float phi = 2.0f * SM_PI * r.y;
+    return V3(radius * cosf(phi), radius * sinf(phi), z);
+}
+
+typedef struct lamp_xform {
+    v3 cx, cy, cz, ix, iy, iz;
+    int valid;
+} lamp_xform;
+
+/* light_linear_transform (:137-166) */
+static lamp_xform light_linear_transform(const olamp *l) {
+    lamp_xform t;
+    t.cx = vscale(l->axis_x, l->axis_scale.x);
+    t.cy = vscale(l->axis_y, l->axis_scale.y);
+    t.cz = vscale(l->axis_z, l->axis_scale.z);
+    const v3 adj = vcross(t.cy, t.cz);
+    const float det = vdot(t.cx, adj);
+    t.valid = fabsf(det) > 1.0e-20f;
+    const float inv = 1.0f / (t.valid ? det : 1.0f);
+    t.ix = vscale(adj, inv);
+    t.iy = vscale(vcross(t.cz, t.cx), inv);
+    t.iz = vscale(vcross(t.cx, t.cy), inv);
+    return t;
+}
+
+static v3 world_to_light_direction(v3 d, const lamp_xform *t) {
+    return V3(vdot(t->ix, d), vdot(t->iy, d), vdot(t->iz, d));
+}
+
+typedef struct prim_hit {
+    int valid;
+    float distance;
+    v3 position;
+} prim_hit;
+
+/* intersect_sphere (:195-238): open interval */
+static prim_hit intersect_sphere(v3 ro, v3 rd, float tmin, float tmax, v3 c, float r) {
+    const v3 co = vsub(c, ro);
+    const float r2 = r * r;
+    const float cd2 = vdot(co, co);
+    const float pd = vdot(co, rd);
+    const v3 perp = vsub(co, vscale(rd, pd));
+    const float perp2 = vdot(perp, perp);
+    const float dist = pd - copysignf(sqrtf(fmaxf(r2 - perp2, 0.0f)), cd2 - r2);
+    prim_hit h;
+    h.valid = !((cd2 > r2) && (pd < 0.0f)) && (perp2 <= r2) && (dist > tmin) && (dist < tmax);
+    h.distance = dist;
+    h.position = vadd(ro, vscale(rd, dist));
+    return h;
+}
+
+/* intersect_disk (:240-273) */
+static prim_hit intersect_disk(v3 ro, v3 rd, float tmin, float tmax, v3 c, v3 n, float r) {
+    const v3 co = vsub(ro, c);
+    const float pdist = vdot(co, n);
+    const float cosine = vdot(n, vneg(rd));
+    const float dist = cosine != 0.0f ? pdist / cosine : 0.0f;
+    const v3 pos = vadd(ro, vscale(rd, dist));
+    const v3 radial = vsub(pos, c);
+    prim_hit h;
+    h.valid = (pdist * cosine > 0.0f) && (dist >= 0.0f) && (vdot(radial, radial) < r * r) &&
+              (dist > tmin) && (dist < tmax);
+    h.distance = dist;
+    h.position = pos;
+    return h;
+}
+
+/* sphere_light_pdf (:281-307) */
+static float sphere_light_pdf(float cd2, float r2, v3 sn, v3 dir, int had_transmission, float outside_omc) {
+    const float outside = SM_INV_TWO_PI / fmaxf(outside_omc, 1.0e-30f);
+    float inside;
+    if (had_transmission) {
+        inside = 0.5f * SM_INV_TWO_PI;
+    } else {
+        const float c = vdot(sn, dir);
+        inside = c > 0.0f ? c * SM_INV_PI : 0.0f;
+    }
+    return cd2 > r2 ? outside : inside;
+}
+
+/* spot_light_to_local (:309-320) */
+static v3 spot_light_to_local(v3 ray, const lamp_xform *t) {
+    v3 l = lamp_normalize(world_to_light_direction(ray, t), V3(0, 0, -1));
+    l.z = -l.z;
+    return l;
+}
+
+/* spot_light_attenuation (:322-352) */
+static float spot_light_attenuation(v3 lr, float angle, float smooth) {
+    const float ch = cosf(0.5f * angle);
+    const float bw = (1.0f - ch) * smooth;
+    const float lin = fclamp(bw != 0.0f ? (lr.z - ch) / bw : 0.0f, 0.0f, 1.0f);
+    const float sm = lin * lin * (3.0f - 2.0f * lin);
+    const float hard = lr.z >= ch ? 1.0f : 0.0f;
+    return bw > 0.0f ? sm : hard;
+}
+
+/* spot_one_minus_cosine_larger_spread (:354-376) */
+static float spot_omc_larger_spread(float angle, v3 sc) {
+    const float tn = tanf(0.5f * angle);
+    const float ts = fmaxf(sc.x * sc.x, sc.y * sc.y);
+    const float as = sc.z * sc.z;
+    const float c = 1.0f / sqrtf(1.0f + tn * tn * ts / fmaxf(as, 1.0e-30f));
+    return 1.0f - c;
+}
+
+/* point_disk_pdf (:856-873) */
+static float point_disk_pdf(float d2, float light_cosine, float r) {
+    const float r2 = r * r;
+    const float inverse_area = r2 > 0.0f ? 1.0f / fmaxf(r2 * AL_PI, 1.0e-20f) : 1.0f;
+    return light_cosine > 0.0f ? inverse_area * d2 / light_cosine : 0.0f;
+}
+
+/* point_eval_factor (:880-896) */
+static float point_eval_factor(float r, int normalize) {
+    const float r2 = r * r;
+    const float finite_area = 4.0f * AL_PI * r2;
+    const float area = r2 > 0.0f ? finite_area : 4.0f;
+    const float inverse_area = normalize ? 1.0f / fmaxf(area, 1.0e-20f) : 1.0f;
+    return inverse_area * AL_INV_PI;
+}
+
+typedef struct lamp_geo {
+    int valid;
+    v3 direction, position, normal;
+    float distance, pdf;
+} lamp_geo;
+
+/* sample_sphere_geometry (:429-547) */
+static lamp_geo sample_sphere_geometry(v3 ref, v3 sn, int has_transmission, v3 c, float r, v3 outside_axis,
+                                       float outside_omc, int intersect_sampled, v2 rnd) {
+    const float r2 = r * r;
+    v3 ln = vsub(ref, c);
+    const float cd2 = vdot(ln, ln);
+    const float cd = sqrtf(fmaxf(cd2, 1.0e-30f));
+    ln = vdivs(ln, cd);
+    const int outside = cd2 > r2;
+    v3 dir = V3(0, 0, 0);
+    float cosine = 1.0f, pdf = 0.0f, dist = 0.0f;
+    v3 pos = ref;
+    int known = 0;
+    int valid = (r > 0.0f) && (cd2 > 0.0f);
+    if (outside) {
+        const cone_t cone = sample_uniform_cone(outside_axis, outside_omc, rnd);
+        dir = cone.direction;
+        cosine = cone.cosine;
+        pdf = cone.pdf;
+        if (intersect_sampled) {
+            const prim_hit h = intersect_sphere(ref, dir, 0.0f, RAY_MAXIMUM, c, r);
+            valid = valid && h.valid;
+            dist = h.distance;
+            pos = h.position;
+            known = h.valid;
+        }
+    } else {
+        if (has_transmission) {
+            dir = sample_uniform_sphere(rnd);
+            pdf = 0.5f * SM_INV_TWO_PI;
+        } else {
+            float hp;
+            dir = sample_cosine_hemisphere(sn, rnd, &hp);
+            pdf = hp;
+        }
+        cosine = -vdot(dir, ln);
+    }
+    if (!known) {
+        dist = cd * cosine - copysignf(sqrtf(fmaxf(r2 - cd2 + cd2 * cosine * cosine, 0.0f)), cd2 - r2);
+        pos = vadd(ref, vscale(dir, dist));
+    }
+    lamp_geo g;
+    g.normal = lamp_normalize(vsub(pos, c), vneg(dir));
+    g.position = vadd(vscale(g.normal, r), c);
+    g.valid = valid;
+    g.direction = dir;
+    g.distance = dist;
+    g.pdf = pdf;
+    return g;
+}
+
+/* sample_disk_geometry (:549-603) */
+static lamp_geo sample_disk_geometry(v3 ref, v3 c, float r, v2 rnd) {
+    v3 ln = vsub(ref, c);
+    const float cd2 = vdot(ln, ln);
+    const float cd = sqrtf(fmaxf(cd2, 1.0e-30f));
+    ln = vdivs(ln, cd);
+    v3 t, b;
+    make_orthonormals(ln, &t, &b);
+    const v2 disk = sample_uniform_disk(rnd);
+    const v3 pos = vadd(c, vscale(vadd(vscale(t, disk.x), vscale(b, disk.y)), r));
+    const v3 off = vsub(pos, ref);
+    const float d2 = vdot(off, off);
+    const float d = sqrtf(fmaxf(d2, 1.0e-30f));
+    const v3 dir = vdivs(off, d);
+    const float cosine = vdot(ln, vneg(dir));
+    lamp_geo g;
+    g.valid = (cd2 > 0.0f) && (cosine > 0.0f);
+    g.direction = dir;
+    g.position = pos;
+    g.normal = vneg(dir);
+    g.distance = d;
+    g.pdf = point_disk_pdf(d2, cosine, r);
+    return g;
+}
+
+typedef struct lamp_sample_t {
+    int valid;
+    v3 direction, position, normal;
+    float distance, pdf, eval;
+} lamp_sample_t;
+
+/* sample_point_light (:898-979) and sample_spot_light (:981-1105) */
+static lamp_sample_t sample_point_or_spot(const olamp *l, v3 ref, v3 sn, int has_transmission, v2 rnd) {
+    const int spot = l->type == PSX_LIGHT_SPOT;
+    const int sphere = (l->flags & LAMP_SPHERE) != 0u;
+    const int normalize = (l->flags & LAMP_NORMALIZE) != 0u;
+    const v3 co = vsub(ref, l->position);
+    const float cd2 = vdot(co, co);
+    const float r2 = l->radius * l->radius;
+    const v3 light_normal = lamp_normalize(co, V3(0, 0, -1));
+    const float sphere_cap = sin2_to_one_minus_cos(r2 / fmaxf(cd2, 1.0e-30f));
+    v3 outside_axis = vneg(light_normal);
+    float outside_cap = sphere_cap;
+    int spread_cone = 0;
+    if (spot) {
+        const float spot_cap = spot_omc_larger_spread(l->spot_angle, l->axis_scale);
+        spread_cone = (cd2 > r2) && !(sphere_cap < spot_cap);
+        if (spread_cone) {
+            outside_axis = l->axis_z;
+            outside_cap = spot_cap;
+        }
+    }
+    const int finite_sphere = sphere && (l->radius > 0.0f);
+    lamp_geo g;
+    if (finite_sphere)
+        g = sample_sphere_geometry(ref, sn, has_transmission, l->position, l->radius, outside_axis, outside_cap,
+                                   spread_cone, rnd);
+    else
+        g = sample_disk_geometry(ref, l->position, l->radius, rnd);
+    lamp_sample_t s;
+    s.direction = g.direction;
+    s.position = g.position;
+    s.normal = g.normal;
+    s.distance = g.distance;
+    s.pdf = g.pdf;
+    s.eval = point_eval_factor(l->radius, normalize);
+    s.valid = g.valid;
+    if (spot) {
+        const lamp_xform xf = light_linear_transform(l);
+        const v3 local = spot_light_to_local(vneg(g.direction), &xf);
+        const float attenuation = spot_light_attenuation(local, l->spot_angle, l->spot_smooth);
+        const int use_attenuation = !finite_sphere || (cd2 > r2);
+        s.eval = s.eval * (use_attenuation ? attenuation : 1.0f);
+        s.valid = g.valid && (s.eval != 0.0f);
+    }
+    return s;
+}
+
+/* sample_distant_light (:84-108) */
+static lamp_sample_t sample_distant_light(const olamp *l, v2 rnd) {
+    const float half_angle = 0.5f * fmaxf(l->angle, 0.0f);
+    const float sq = sinf(0.5f * half_angle);
+    const float omc = 2.0f * sq * sq;
+    const cone_t ns = sample_uniform_cone(vneg(l->axis_z), omc, rnd);
+    const int finite = half_angle > 0.0f;
+    const float sh = sinf(half_angle);
+    const float disk_area = AL_PI * sh * sh;
+    lamp_sample_t s;
+    s.direction = vneg(ns.direction);
+    s.pdf = ns.pdf;
+    s.eval = ((l->flags & LAMP_NORMALIZE) != 0u && finite) ? 1.0f / fmaxf(disk_area, 1.0e-20f) : 1.0f;
+    s.position = vneg(s.direction);
+    s.normal = vneg(s.direction);
+    s.distance = RAY_MAXIMUM;
+    s.valid = 1;
+    return s;
+}
+
+typedef struct lamp_hit_t {
+    int valid;
+    float distance, pdf, eval;
+} lamp_hit_t;
+
+/* intersect_point / intersect_spot (analytic_light_intersection.h:33-260) */
+static lamp_hit_t intersect_lamp(const olamp *l, v3 ro, v3 rd, float tmin, float tmax, v3 previous_normal,
+                                 int had_transmission) {
+    const int spot = l->type == PSX_LIGHT_SPOT;
+    const int sphere = (l->flags & LAMP_SPHERE) != 0u;
+    const v3 co = vsub(ro, l->position);
+    const float cd2 = vdot(co, co);
+    const float r2 = l->radius * l->radius;
+    const float sphere_cap = sin2_to_one_minus_cos(r2 / fmaxf(cd2, 1.0e-30f));
+    float outside_cap = sphere_cap;
+    if (spot) outside_cap = fminf(sphere_cap, spot_omc_larger_spread(l->spot_angle, l->axis_scale));
+    /* intersect_point_geometry (:33-117) */
+    const v3 light_normal = lamp_normalize(co, vneg(rd));
+    prim_hit g;
+    float pdf;
+    v3 normal = vneg(rd);
+    if (sphere) {
+        g = intersect_sphere(ro, rd, tmin, tmax, l->position, l->radius);
+        normal = lamp_normalize(vsub(g.position, l->position), vneg(rd));
+        pdf = sphere_light_pdf(cd2, r2, previous_normal, rd, had_transmission, outside_cap);
+    } else {
+        g = intersect_disk(ro, rd, tmin, tmax, l->position, light_normal, l->radius);
+        const float light_cosine = vdot(light_normal, vneg(rd));
+        pdf = point_disk_pdf(g.distance * g.distance, light_cosine, l->radius);
+    }
+    (void)normal;
+    lamp_hit_t h;
+    h.valid = g.valid && (l->radius > 0.0f);
+    h.distance = g.distance;
+    h.pdf = pdf;
+    h.eval = point_eval_factor(l->radius, (l->flags & LAMP_NORMALIZE) != 0u);
+    if (spot) {
+        const lamp_xform xf = light_linear_transform(l);
+        const v3 local = spot_light_to_local(vneg(rd), &xf);
+        const int use_attenuation = !sphere || (cd2 > r2);
+        const float attenuation = spot_light_attenuation(local, l->spot_angle, l->spot_smooth);
+        h.eval = h.eval * (use_attenuation ? attenuation : 1.0f);
+        h.valid = h.valid && (vdot(rd, co) < 0.0f) && (h.eval > 0.0f);
+    } else {
+        h.valid = h.valid && (h.eval > 0.0f);
+    }
+    return h;
+}
+
+int oracle_sample_lamp(const psx_light *light, const float reference[3], const float shading_normal[3],
+                       int has_transmission, const float random[2], float out[13]) {
+    if (!light || light->type > PSX_LIGHT_DISTANT || light->type == PSX_LIGHT_AREA) return -1;
+    olamp l;
+    lamp_from_desc(light, &l);
+    const v2 rnd = {random[0], random[1]};
+    const lamp_sample_t ls = l.type == PSX_LIGHT_DISTANT
+                                 ? sample_distant_light(&l, rnd)
+                                 : sample_point_or_spot(&l, load3(reference), load3(shading_normal),
+                                                        has_transmission, rnd);
+    const float v[13] = {ls.direction.x, ls.direction.y, ls.direction.z, ls.pdf, ls.position.x, ls.position.y,
+                         ls.position.z, ls.distance, ls.normal.x, ls.normal.y, ls.normal.z, ls.eval,
+                         ls.valid ? 1.0f : 0.0f};
+    memcpy(out, v, sizeof(v));
+    return 0;
+}
+
+int oracle_intersect_lamp(const psx_light *light, const float origin[3], const float direction[3], float tmin,
+                          float tmax, const float previous_normal[3], int had_transmission, float out[4]) {
+    if (!light || (light->type != PSX_LIGHT_POINT && light->type != PSX_LIGHT_SPOT)) return -1;
+    olamp l;
+    lamp_from_desc(light, &l);
+    const lamp_hit_t h = intersect_lamp(&l, load3(origin), load3(direction), tmin, tmax, load3(previous_normal),
+                                        had_transmission);
+    out[0] = h.valid ? 1.0f : 0.0f;
+    out[1] = h.distance;
+    out[2] = h.pdf;
+    out[3] = h.eval;
+    return 0;
+}
+
+/* ======================================================================= */
 /* the path integrator (src/luisa/path_kernel_pipeline.cpp:73-185 and the  */
 /* stages it calls)                                                        */
 /* ======================================================================= */
@@ -2288,6 +2757,7 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
     float previous_bsdf_pdf = 0.0f;
     float minimum_bsdf_pdf = FLT_MAX;
     int previous_delta = 1;
+    v3 previous_mis_origin_normal = V3(0, 0, 0);
     uint32_t ray_events = 0u;
     oracle_path_state ps;
     oracle_initial_state(&ps);
@@ -2317,6 +2787,54 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
         const hit_t hit = closest_hit(s, &ray, ray_visibility, src_obj, src_prim, UINT32_MAX,
                                       UINT32_MAX, counters);
         rays++;
+        /* Lamps are transparent closest events resolved before the mesh or
+         * background event, without Sobol dimensions
+         * (src/luisa/path_kernel_pipeline.cpp:79-137,
+         *  src/luisa/path_kernel_closest_event.cpp:24-266,
+         *  src/luisa/path_kernel_forward_light.cpp:59-117) */
+        int lamp_terminated = 0;
+        {
+            uint32_t previous_lamp = UINT32_MAX;
+            const float surface_distance = hit.valid ? hit.t : ray.tmax;
+            const int camera_path = (ray_visibility & PSX_VISIBILITY_CAMERA) != 0u;
+            const int had_transmission = (ps.flag & FLAG_MIS_HAD_TRANSMISSION) != 0u;
+            for (;;) {
+                float event_distance = surface_distance;
+                uint32_t found = UINT32_MAX;
+                lamp_hit_t best;
+                for (uint32_t li = 0; li < s->lamp_count; ++li) {
+                    const olamp *l = &s->lamps[li];
+                    if (l->type == PSX_LIGHT_DISTANT) continue;
+                    const int eligible = (l->visibility & ray_visibility) != 0u &&
+                                         (camera_path || (l->flags & LAMP_USE_MIS) != 0u) && li != previous_lamp;
+                    if (!eligible) continue;
+                    const lamp_hit_t c = intersect_lamp(l, ray.o, ray.d, ray.tmin, event_distance,
+                                                        previous_mis_origin_normal, had_transmission);
+                    if (c.valid) {
+                        found = li;
+                        best = c;
+                        event_distance = c.distance;
+                    }
+                }
+                if (found == UINT32_MAX) break;
+                const olamp *l = &s->lamps[found];
+                const float light_pdf = best.pdf * s->light_selection_pdf;
+                const int competing = ps.bounce > 0u && !previous_delta;
+                const float mis = forward_light_weight(k, previous_bsdf_pdf, light_pdf, competing, light_pdf > 0.0f);
+                const v3 light_radiance = vscale(l->color, l->power * best.eval);
+                const v3 contribution = film_clamp(vscale(vmul(throughput, light_radiance), mis),
+                                                   ps.bounce > 1u ? 1u : 0u, k->clamp_direct, k->clamp_indirect);
+                radiance = vadd(radiance, contribution);
+                ps.transparent_bounce += 1u;
+                ray.tmin = bitsf(intersection_t_offset_bits(best.distance));
+                previous_lamp = found;
+                if (ps.transparent_bounce >= k->transparent_max) {
+                    lamp_terminated = 1;
+                    break;
+                }
+            }
+        }
+        if (lamp_terminated) break;
         if (!hit.valid) {
             /* background event (src/luisa/path_kernel_background_event.cpp:20-159) */
             log_flag(log, step, 4u);
@@ -2481,6 +2999,73 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                 }
             }
         }
+        /* next-event estimation: analytic lamps
+         * (src/luisa/path_kernel_nee_analytic.cpp:98-410) */
+        if (k->nee && (runtime_flags & RT_BSDF_HAS_EVAL) != 0u && light && light->kind == 1u) {
+            log_flag(log, step, 16u);
+            log_event(log, step, PSX_TRACE_LIGHT, (uint32_t)(light - s->distribution));
+            const olamp *l = &s->lamps[light->index];
+            const int reached_max_bounces = ps.bounce > l->max_bounces;
+            const v2 rnd = {u_light.x, u_light.y};
+            const int distant = l->type == PSX_LIGHT_DISTANT;
+            const int has_transmission = (runtime_flags & RT_BSDF_HAS_TRANSMISSION) != 0u;
+            const lamp_sample_t ls = distant ? sample_distant_light(l, rnd)
+                                             : sample_point_or_spot(l, sf.position, sf.ns, has_transmission, rnd);
+            const v3 light_radiance = vscale(l->color, l->power * ls.eval);
+            const float light_pdf = ls.pdf * light->selection_pdf;
+            if (!reached_max_bounces && ls.valid && light_pdf > 0.0f) {
+                const eval_t ev = evaluate_mixture(&cs, sf.incoming, sf.ns, ls.direction, lobe_mask, UINT32_MAX);
+                const float cycles_mis = nee_light_weight(k, light_pdf, ev.pdf);
+                const float mis = (l->flags & LAMP_FORWARD) != 0u ? cycles_mis : 1.0f;
+                const v3 weighted = vscale(vmul(ev.f, light_radiance), mis / fmaxf(light_pdf, 1.0e-20f));
+                if (vany_nonzero(weighted)) {
+                    const float rr = light_roulette(weighted, u_lightterm, k->light_inv_rr_threshold);
+                    const v3 surviving = vscale(weighted, rr);
+                    if (vany_nonzero(surviving)) {
+                        const v3 so = surface_shadow_origin(s, &sf, ls.direction);
+                        const v3 foff = vsub(ls.position, so);
+                        const float fdist = sqrtf(fmaxf(vdot(foff, foff), 1.0e-20f));
+                        ray_t sray;
+                        sray.o = so;
+                        sray.d = distant ? ls.direction : vdivs(foff, fdist);
+                        sray.tmin = 0.0f;
+                        sray.tmax = distant ? RAY_MAXIMUM : fdist;
+                        v3 transmittance = V3(1, 1, 1);
+                        uint32_t tdepth = ps.transparent_bounce;
+                        for (;;) {
+                            const hit_t sh = closest_hit(s, &sray, PSX_VISIBILITY_SHADOW, sf.object, sf.cprim,
+                                                         UINT32_MAX, UINT32_MAX, counters);
+                            rays++;
+                            if (!sh.valid) break;
+                            if (tdepth >= k->transparent_max) { transmittance = V3(0, 0, 0); break; }
+                            const oinstance *si = &s->instances[sh.inst];
+                            const uint32_t smat = material_of(s, si, sh.prim);
+                            v3 ext = V3(0, 0, 0);
+                            if (smat < s->material_count) {
+                                surface_t ssf;
+                                resolve_surface(s, &sh, &sray, &ssf);
+                                closure_set scs;
+                                material_closures(&s->materials[smat], ssf.ns, &scs);
+                                ext = vclamp(transparent_extinction(&scs), 0.0f, 1.0f);
+                            }
+                            if (fmaxf(ext.x, fmaxf(ext.y, ext.z)) > 0.0f) {
+                                transmittance = vmul(transmittance, ext);
+                                tdepth += 1u;
+                                sray.tmin = bitsf(intersection_t_offset_bits(sh.t));
+                            } else {
+                                transmittance = V3(0, 0, 0);
+                                break;
+                            }
+                        }
+                        if (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f) {
+                            const v3 c = film_clamp(vmul(vmul(throughput, surviving), transmittance), ps.bounce,
+                                                    k->clamp_direct, k->clamp_indirect);
+                            radiance = vadd(radiance, c);
+                        }
+                    }
+                }
+            }
+        }
         /* surface scatter (src/luisa/path_kernel_surface_scatter.cpp:16-324) */
         const v2 udir = {u_bsdf.x, u_bsdf.y};
         const bsdf_sample_t bs = sample_closures(&cs, &sf, u_bsdf.z, udir, lobe_mask);
@@ -2505,6 +3090,7 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
             previous_bsdf_pdf = bs.eval.pdf;
             minimum_bsdf_pdf = fminf(minimum_bsdf_pdf, bs.eval.pdf);
             previous_delta = singular;
+            previous_mis_origin_normal = sf.ns;
         }
         const v3 nd = vnormalize(bs.wi);
         if (transparent) {

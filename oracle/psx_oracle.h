@@ -84,6 +84,18 @@ int oracle_pluecker_intersect(const float origin[3], const float direction[3],
                               const float p1[3], const float p2[3],
                               float out[3]);
 
+/* --- analytic lamps (include/psycles/luisa/analytic_light_sampling.h) ---- */
+/* out: direction[3], pdf, position[3], distance, normal[3], eval factor,
+ * valid (1/0). */
+int oracle_sample_lamp(const psx_light *light, const float reference[3],
+                       const float shading_normal[3], int has_transmission,
+                       const float random[2], float out[13]);
+/* Forward hit of a point or spot lamp: out = valid, distance, pdf, eval. */
+int oracle_intersect_lamp(const psx_light *light, const float origin[3],
+                          const float direction[3], float tmin, float tmax,
+                          const float previous_normal[3], int had_transmission,
+                          float out[4]);
+
 /* --- render ------------------------------------------------------------- */
 typedef struct oracle_stats {
     uint64_t rays;

@@ -75,6 +75,12 @@ LIGHT_AREA = 2
 LIGHT_DISTANT = 3
 LIGHT_BACKGROUND = 4
 
+LIGHT_FLAG_NORMALIZE = 1
+LIGHT_FLAG_ELLIPSE = 2
+LIGHT_FLAG_SPHERE = 4
+LIGHT_FLAG_USE_MIS = 8
+LIGHT_FLAG_CAST_SHADOW = 16
+
 INTEGRATOR_MEGAKERNEL = 0
 INTEGRATOR_WAVEFRONT = 1
 BACKEND_COUNT_TRAVERSAL = 1

@@ -118,6 +118,30 @@ struct Emitter {
     uint32_t emission_sampling;
 };
 
+// Analytic lamp (LightGpu, src/luisa/path_tracer_types.h:142-168): point,
+// spot or distant.  Axes are the normalised columns of the lamp transform,
+// axis_scale their lengths.
+struct Lamp {
+    float position[3];
+    uint32_t type;  // psx_light_type
+    float axis_x[3];
+    float power;
+    float axis_y[3];
+    float radius;
+    float axis_z[3];
+    float spot_angle;
+    float axis_scale[3];
+    float spot_smooth;
+    float color[3];
+    float angle;
+    uint32_t flags;  // kLamp* bits
+    uint32_t visibility;
+    uint32_t max_bounces;
+    uint32_t pad;
+};
+constexpr uint32_t kLampNormalize = 1u, kLampSphere = 4u, kLampUseMis = 8u, kLampForward = 32u;
+constexpr uint32_t kLampPoint = 0u, kLampSpot = 1u, kLampDistant = 3u;
+
 struct DistEntry {
     float cumulative;
     float selection_pdf;
@@ -147,6 +171,9 @@ struct SceneView {
     const DistEntry *distribution;
     const float4 *material_shadow;    // per material: transparent extinction for shadow rays
     const float4 *material_emission;  // per material: constant emission
+    const Lamp *lamps;
+    uint32_t lamp_count;
+    float light_selection_pdf;        // flat-CDF probability of one lamp
     uint32_t tlas_root;
     uint32_t instance_count;
     uint32_t material_count;
@@ -204,6 +231,7 @@ struct WavefrontView {
     float4 *ray_d;       // direction.xyz, tmax
     uint4 *aux;          // source object, source primitive, step, bits
     float4 *throughput;  // xyz, previous BSDF pdf
+    float4 *mis_normal;  // shading normal of the last non-transparent scatter
     float4 *radiance;    // xyz, transparency
     float4 *normal;      // primary normal AOV
     float4 *albedo;      // primary albedo AOV

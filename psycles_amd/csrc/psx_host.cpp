@@ -254,24 +254,45 @@ std::vector<float> pixel_filter_table(uint32_t filter, float width) {
 // ---------------------------------------------------------------------------
 // light distribution (src/sampling/light_distribution.cpp:25-138)
 // ---------------------------------------------------------------------------
-uint32_t build_distribution(const std::vector<float> &areas, std::vector<DistEntry> &e, float &tri_pdf) {
+// Emissive triangles weighted by world area, then `lamps` analytic lamps;
+// when both families exist each takes half of the probability.
+uint32_t build_distribution(const std::vector<float> &areas, uint32_t lamps, std::vector<DistEntry> &e,
+                            float &tri_pdf, float &light_pdf) {
     auto valid = [](float a) { return std::isfinite(a) && a > 0.0f ? a : 0.0f; };
     const uint32_t n = static_cast<uint32_t>(areas.size());
-    e.assign(n + 1u, DistEntry{});
+    const uint32_t count = n + lamps;
+    e.assign(count + 1u, DistEntry{});
     float total = 0.0f;
     for (uint32_t i = 0; i < n; ++i) {
         e[i] = DistEntry{total, 0.0f, 0u, i};
         total += valid(areas[i]);
     }
-    e[n] = DistEntry{total, 0.0f, 3u, 0u};
-    tri_pdf = total > 0.0f ? 1.0f / total : 0.0f;
-    for (uint32_t i = 0; i < n; ++i) e[i].selection_pdf = valid(areas[i]) * tri_pdf;
+    const float triangle_area = total;
+    const float lamp_area = lamps > 0u ? (total > 0.0f ? total / static_cast<float>(lamps) : 1.0f) : 0.0f;
+    for (uint32_t j = 0; j < lamps; ++j) {
+        e[n + j] = DistEntry{total, 0.0f, 1u, j};
+        total += lamp_area;
+    }
+    e[count] = DistEntry{total, 0.0f, 3u, 0u};
+    float tp = 0.0f, lp = 0.0f;
+    if (triangle_area > 0.0f) {
+        tp = 1.0f / triangle_area;
+        if (lamps > 0u) tp *= 0.5f;
+    }
+    if (lamps > 0u) {
+        lp = 1.0f / static_cast<float>(lamps);
+        if (triangle_area > 0.0f) lp *= 0.5f;
+    }
+    for (uint32_t i = 0; i < n; ++i) e[i].selection_pdf = valid(areas[i]) * tp;
+    for (uint32_t i = n; i < count; ++i) e[i].selection_pdf = lp;
     if (total > 0.0f) {
         const float inv = 1.0f / total;
-        for (uint32_t i = 0; i < n; ++i) e[i].cumulative *= inv;
-        e[n].cumulative = 1.0f;
+        for (uint32_t i = 0; i < count; ++i) e[i].cumulative *= inv;
+        e[count].cumulative = 1.0f;
     }
-    return total > 0.0f ? n : 0u;
+    tri_pdf = tp;
+    light_pdf = lp;
+    return total > 0.0f ? count : 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -411,14 +432,59 @@ struct Scene {
     uint32_t max_stack = 0;
     uint32_t tlas_depth = 0, blas_depth = 0;
     // device buffers
+    std::vector<Lamp> lamp_list;
     DeviceBuffer nodes, tris, instances_buf, meshes_buf, positions, indices, normals, smooth, slots, tri_slots,
-        overrides_buf, materials_buf, leaves, params, emitters, distribution, material_shadow, material_emission;
+        overrides_buf, materials_buf, leaves, params, emitters, distribution, material_shadow, material_emission, lamps;
     SceneView view{};
     uint64_t triangles = 0;
 
     void rebuild_emitters();
     void refresh_material_tables();
 };
+
+// LightDesc -> LightGpu for point, spot and distant lamps
+// (src/luisa/path_tracer_scene.cpp:1735-1870, no lamp shaders)
+Lamp make_lamp(const psx_light &l) {
+    Lamp g{};
+    g.type = l.type;
+    float axes[3][3], lengths[3];
+    for (int c = 0; c < 3; ++c) {
+        const float x = l.transform[4 * c], y = l.transform[4 * c + 1], z = l.transform[4 * c + 2];
+        const float n = std::sqrt(x * x + y * y + z * z);
+        if (n <= 1.0e-20f) {
+            axes[c][0] = axes[c][1] = axes[c][2] = 0.0f;
+            lengths[c] = 0.0f;
+        } else {
+            axes[c][0] = x / n;
+            axes[c][1] = y / n;
+            axes[c][2] = z / n;
+            lengths[c] = n;
+        }
+    }
+    if (axes[2][0] == 0.0f && axes[2][1] == 0.0f && axes[2][2] == 0.0f) axes[2][2] = 1.0f;
+    for (int i = 0; i < 3; ++i) {
+        g.position[i] = l.transform[12 + i];
+        g.axis_x[i] = axes[0][i];
+        g.axis_y[i] = axes[1][i];
+        g.axis_z[i] = axes[2][i];
+        g.axis_scale[i] = lengths[i];
+        g.color[i] = l.color[i];
+    }
+    g.power = l.power;
+    g.radius = l.size;
+    g.spot_angle = l.spot_angle;
+    g.spot_smooth = l.spot_smooth;
+    g.angle = l.angle;
+    g.visibility = l.visibility_mask;
+    g.max_bounces = l.max_bounces;
+    // the MIS property only holds where a forward technique exists
+    const bool use_mis = (l.flags & PSX_LIGHT_FLAG_USE_MIS) != 0u;
+    const bool effective_mis = use_mis && (l.type == PSX_LIGHT_DISTANT ? l.angle > 0.0f : l.size > 0.0f);
+    g.flags = ((l.flags & PSX_LIGHT_FLAG_NORMALIZE) ? kLampNormalize : 0u) |
+              ((l.flags & PSX_LIGHT_FLAG_SPHERE) ? kLampSphere : 0u) | (effective_mis ? kLampUseMis : 0u) |
+              (effective_mis && l.type != PSX_LIGHT_DISTANT ? kLampForward : 0u);
+    return g;
+}
 
 // per-material shadow extinction and constant emission, evaluated on the
 // device by the same closure code the kernels run
@@ -469,9 +535,11 @@ void Scene::rebuild_emitters() {
     }
     std::vector<DistEntry> dist;
     float tri_pdf = 0.0f;
-    const uint32_t count = build_distribution(areas, dist, tri_pdf);
+    float lamp_pdf = 0.0f;
+    const uint32_t count = build_distribution(areas, static_cast<uint32_t>(lamp_list.size()), dist, tri_pdf, lamp_pdf);
     emitters.upload(em);
     distribution.upload(dist);
+    view.light_selection_pdf = lamp_pdf;
     view.emitters = emitters.as<Emitter>();
     view.distribution = distribution.as<DistEntry>();
     view.emitter_count = static_cast<uint32_t>(em.size());
@@ -482,7 +550,7 @@ void Scene::rebuild_emitters() {
 // Wavefront path state of one session (psx_device.h, WavefrontView), sized
 // for max_samples_per_dispatch samples of every window pixel.
 struct WavefrontState {
-    DeviceBuffer ray_o, ray_d, aux, throughput, radiance, normal, albedo, state0, state1, ident, hit, hit_uv,
+    DeviceBuffer ray_o, ray_d, aux, throughput, mis_normal, radiance, normal, albedo, state0, state1, ident, hit, hit_uv,
         shadow_o, shadow_d, shadow_q, shadow_l, shadow_t, queue0, queue1, list0, list1, counters;
     WavefrontView view{};
 
@@ -492,6 +560,7 @@ struct WavefrontState {
         ray_d.allocate(n * 16u);
         aux.allocate(n * 16u);
         throughput.allocate(n * 16u);
+        mis_normal.allocate(n * 16u);
         radiance.allocate(n * 16u);
         normal.allocate(n * 16u);
         albedo.allocate(n * 16u);
@@ -516,6 +585,7 @@ struct WavefrontState {
         v.ray_d = ray_d.as<float4>();
         v.aux = aux.as<uint4>();
         v.throughput = throughput.as<float4>();
+        v.mis_normal = mis_normal.as<float4>();
         v.radiance = radiance.as<float4>();
         v.normal = normal.as<float4>();
         v.albedo = albedo.as<float4>();
@@ -1101,7 +1171,12 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     float bg[3] = {d->world_color[0], d->world_color[1], d->world_color[2]};
     for (uint32_t i = 0; i < d->light_count; ++i) {
         const psx_light &l = d->lights[i];
-        if (l.type != PSX_LIGHT_BACKGROUND) return fail("only background lights are supported by this ABI version");
+        if (l.type == PSX_LIGHT_POINT || l.type == PSX_LIGHT_SPOT || l.type == PSX_LIGHT_DISTANT) {
+            s.lamp_list.push_back(make_lamp(l));
+            continue;
+        }
+        if (l.type != PSX_LIGHT_BACKGROUND)
+            return fail("light " + std::to_string(i) + ": area lights are not supported by this ABI version");
         bg[0] += l.color[0] * l.power;
         bg[1] += l.color[1] * l.power;
         bg[2] += l.color[2] * l.power;
@@ -1110,6 +1185,9 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     v.background[1] = bg[1];
     v.background[2] = bg[2];
     v.world_visibility = d->world_visibility_mask;
+    s.lamps.upload(s.lamp_list);
+    v.lamps = s.lamps.as<Lamp>();
+    v.lamp_count = static_cast<uint32_t>(s.lamp_list.size());
     s.rebuild_emitters();
     s.refresh_material_tables();
     *out = scene.release();

@@ -1183,6 +1183,342 @@ __device__ Ray camera_ray(const KernelParams &k, const float4 *sobol, const floa
     return r;
 }
 // ===========================================================================
+// analytic lamps: sampling and forward intersection
+// (include/psycles/luisa/analytic_light_sampling.h:84-1118,
+//  include/psycles/luisa/analytic_light_intersection.h:23-260,
+//  include/psycles/luisa/cycles_sample_mapping.h:331-387)
+// ===========================================================================
+constexpr float SM_PI = 3.14159265358979323846f;
+constexpr float SM_INV_PI = 0.31830988618379067154f;
+constexpr float SM_INV_TWO_PI = 0.15915494309189533577f;
+constexpr float AL_PI = 3.1415926535897932f;
+constexpr float AL_INV_PI = 0.3183098861837907f;
+constexpr float RAY_MAXIMUM = 1.0e30f;
+
+__device__ __forceinline__ f3 lamp3(const float *v) { return V(v[0], v[1], v[2]); }
+
+// analytic_light_sampling::safe_normalize (:120-131)
+__device__ __forceinline__ f3 lamp_normalize(f3 value, f3 fallback) {
+    const float m2 = dot(value, value);
+    return m2 > 0.0f ? value / sqrtf(fmaxf(m2, 1.0e-30f)) : fallback;
+}
+
+__device__ __forceinline__ float sin2_to_one_minus_cos(float s2) {
+    return s2 > 0.0004f ? 1.0f - sqrtf(fmaxf(1.0f - s2, 0.0f)) : 0.5f * s2;
+}
+
+// make_orthonormals (include/psycles/luisa/cycles_sample_mapping.h)
+__device__ __forceinline__ void make_orthonormals(f3 n, f3 &t, f3 &b) {
+    const f3 general = V(n.z - n.y, n.x - n.z, n.y - n.x);
+    const f3 equal = V(n.z - n.y, n.x + n.z, -n.y - n.x);
+    const bool use_general = (n.x != n.y) || (n.x != n.z);
+    t = normalize(use_general ? general : equal);
+    b = cross(n, t);
+}
+
+struct Cone {
+    f3 direction;
+    float cosine, pdf;
+};
+
+// sample_uniform_cone (cycles_sample_mapping.h:356-378)
+__device__ __forceinline__ Cone sample_uniform_cone(f3 axis, float omc, float rx, float ry) {
+    float dx, dy;
+    uniform_disk(rx, ry, dx, dy);
+    const float r2 = dx * dx + dy * dy;
+    const float cosine = 1.0f - r2 * omc;
+    const float radial = sqrtf(fmaxf(omc * (2.0f - omc * r2), 0.0f));
+    const float mx = dx * radial, my = dy * radial;
+    f3 t, b;
+    make_orthonormals(axis, t, b);
+    const f3 fd = (t * mx + b * my) + axis * cosine;
+    const bool finite = omc > 0.0f;
+    Cone c;
+    c.direction = finite ? fd : axis;
+    c.cosine = finite ? cosine : 1.0f;
+    c.pdf = finite ? SM_INV_TWO_PI / omc : 1.0f;
+    return c;
+}
+
+__device__ __forceinline__ f3 sample_uniform_sphere(float rx, float ry) {
+    const float z = 1.0f - 2.0f * rx;
+    const float radius = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+    const float phi = 2.0f * SM_PI * ry;
+    return V(radius * cosf(phi), radius * sinf(phi), z);
+}
+
+struct PrimHit {
+    bool valid;
+    float distance;
+    f3 position;
+};
+
+// intersect_sphere (:195-238), open interval
+__device__ __forceinline__ PrimHit intersect_sphere(f3 ro, f3 rd, float tmin, float tmax, f3 c, float r) {
+    const f3 co = c - ro;
+    const float r2 = r * r;
+    const float cd2 = dot(co, co);
+    const float pd = dot(co, rd);
+    const f3 perp = co - rd * pd;
+    const float perp2 = dot(perp, perp);
+    const float dist = pd - copysignf(sqrtf(fmaxf(r2 - perp2, 0.0f)), cd2 - r2);
+    PrimHit h;
+    h.valid = !((cd2 > r2) && (pd < 0.0f)) && (perp2 <= r2) && (dist > tmin) && (dist < tmax);
+    h.distance = dist;
+    h.position = ro + rd * dist;
+    return h;
+}
+
+// intersect_disk (:240-273)
+__device__ __forceinline__ PrimHit intersect_disk(f3 ro, f3 rd, float tmin, float tmax, f3 c, f3 n, float r) {
+    const f3 co = ro - c;
+    const float pdist = dot(co, n);
+    const float cosine = dot(n, neg(rd));
+    const float dist = cosine != 0.0f ? pdist / cosine : 0.0f;
+    const f3 pos = ro + rd * dist;
+    const f3 radial = pos - c;
+    PrimHit h;
+    h.valid = (pdist * cosine > 0.0f) && (dist >= 0.0f) && (dot(radial, radial) < r * r) && (dist > tmin) &&
+              (dist < tmax);
+    h.distance = dist;
+    h.position = pos;
+    return h;
+}
+
+// sphere_light_pdf (:281-307)
+__device__ __forceinline__ float sphere_light_pdf(float cd2, float r2, f3 sn, f3 dir, bool had_transmission,
+                                                  float outside_omc) {
+    const float outside = SM_INV_TWO_PI / fmaxf(outside_omc, 1.0e-30f);
+    float inside;
+    if (had_transmission) {
+        inside = 0.5f * SM_INV_TWO_PI;
+    } else {
+        const float c = dot(sn, dir);
+        inside = c > 0.0f ? c * SM_INV_PI : 0.0f;
+    }
+    return cd2 > r2 ? outside : inside;
+}
+
+// spot_light_to_local (:309-320) through light_linear_transform (:137-166)
+__device__ __forceinline__ f3 spot_light_to_local(const Lamp &l, f3 ray) {
+    const f3 cx = lamp3(l.axis_x) * l.axis_scale[0];
+    const f3 cy = lamp3(l.axis_y) * l.axis_scale[1];
+    const f3 cz = lamp3(l.axis_z) * l.axis_scale[2];
+    const f3 adj = cross(cy, cz);
+    const float det = dot(cx, adj);
+    const bool valid = fabsf(det) > 1.0e-20f;
+    const float inv = 1.0f / (valid ? det : 1.0f);
+    const f3 ix = adj * inv, iy = cross(cz, cx) * inv, iz = cross(cx, cy) * inv;
+    f3 local = lamp_normalize(V(dot(ix, ray), dot(iy, ray), dot(iz, ray)), V(0.0f, 0.0f, -1.0f));
+    local.z = -local.z;
+    return local;
+}
+
+// spot_light_attenuation (:322-352)
+__device__ __forceinline__ float spot_light_attenuation(f3 lr, float angle, float smooth) {
+    const float ch = cosf(0.5f * angle);
+    const float bw = (1.0f - ch) * smooth;
+    const float lin = clampf(bw != 0.0f ? (lr.z - ch) / bw : 0.0f, 0.0f, 1.0f);
+    const float sm = lin * lin * (3.0f - 2.0f * lin);
+    const float hard = lr.z >= ch ? 1.0f : 0.0f;
+    return bw > 0.0f ? sm : hard;
+}
+
+// spot_one_minus_cosine_larger_spread (:354-376)
+__device__ __forceinline__ float spot_omc_larger_spread(const Lamp &l) {
+    const float tn = tanf(0.5f * l.spot_angle);
+    const float ts = fmaxf(l.axis_scale[0] * l.axis_scale[0], l.axis_scale[1] * l.axis_scale[1]);
+    const float as = l.axis_scale[2] * l.axis_scale[2];
+    const float c = 1.0f / sqrtf(1.0f + tn * tn * ts / fmaxf(as, 1.0e-30f));
+    return 1.0f - c;
+}
+
+// point_disk_pdf (:856-873)
+__device__ __forceinline__ float point_disk_pdf(float d2, float light_cosine, float r) {
+    const float r2 = r * r;
+    const float inverse_area = r2 > 0.0f ? 1.0f / fmaxf(r2 * AL_PI, 1.0e-20f) : 1.0f;
+    return light_cosine > 0.0f ? inverse_area * d2 / light_cosine : 0.0f;
+}
+
+// point_eval_factor (:880-896)
+__device__ __forceinline__ float point_eval_factor(float r, bool normalize) {
+    const float r2 = r * r;
+    const float finite_area = 4.0f * AL_PI * r2;
+    const float area = r2 > 0.0f ? finite_area : 4.0f;
+    const float inverse_area = normalize ? 1.0f / fmaxf(area, 1.0e-20f) : 1.0f;
+    return inverse_area * AL_INV_PI;
+}
+
+struct LampSample {
+    bool valid;
+    f3 direction, position;
+    float distance, pdf, eval;
+};
+
+// sample_point_light (:898-979), sample_spot_light (:981-1105) and their
+// sphere/disk geometry (:429-603)
+__device__ LampSample sample_point_or_spot(const Lamp &l, f3 ref, f3 sn, bool has_transmission, float rx, float ry) {
+    const bool spot = l.type == kLampSpot;
+    const bool sphere = (l.flags & kLampSphere) != 0u;
+    const f3 center = lamp3(l.position);
+    const float r = l.radius;
+    const f3 co = ref - center;
+    const float cd2 = dot(co, co);
+    const float r2 = r * r;
+    const f3 light_normal = lamp_normalize(co, V(0.0f, 0.0f, -1.0f));
+    const float sphere_cap = sin2_to_one_minus_cos(r2 / fmaxf(cd2, 1.0e-30f));
+    f3 outside_axis = neg(light_normal);
+    float outside_cap = sphere_cap;
+    bool spread_cone = false;
+    if (spot) {
+        const float spot_cap = spot_omc_larger_spread(l);
+        spread_cone = (cd2 > r2) && !(sphere_cap < spot_cap);
+        if (spread_cone) {
+            outside_axis = lamp3(l.axis_z);
+            outside_cap = spot_cap;
+        }
+    }
+    const bool finite_sphere = sphere && (r > 0.0f);
+    LampSample s;
+    if (finite_sphere) {
+        // sample_sphere_geometry
+        const float cd = sqrtf(fmaxf(cd2, 1.0e-30f));
+        const f3 ln = co / cd;
+        const bool outside = cd2 > r2;
+        f3 dir = V(0.0f, 0.0f, 0.0f);
+        float cosine = 1.0f, pdf = 0.0f, dist = 0.0f;
+        f3 pos = ref;
+        bool known = false;
+        bool valid = (r > 0.0f) && (cd2 > 0.0f);
+        if (outside) {
+            const Cone cone = sample_uniform_cone(outside_axis, outside_cap, rx, ry);
+            dir = cone.direction;
+            cosine = cone.cosine;
+            pdf = cone.pdf;
+            if (spread_cone) {
+                const PrimHit h = intersect_sphere(ref, dir, 0.0f, RAY_MAXIMUM, center, r);
+                valid = valid && h.valid;
+                dist = h.distance;
+                pos = h.position;
+                known = h.valid;
+            }
+        } else {
+            if (has_transmission) {
+                dir = sample_uniform_sphere(rx, ry);
+                pdf = 0.5f * SM_INV_TWO_PI;
+            } else {
+                dir = cosine_hemisphere(sn, rx, ry);
+                float dx, dy;
+                uniform_disk(rx, ry, dx, dy);
+                pdf = sqrtf(fmaxf(1.0f - (dx * dx + dy * dy), 0.0f)) * SM_INV_PI;
+            }
+            cosine = -dot(dir, ln);
+        }
+        if (!known) {
+            dist = cd * cosine - copysignf(sqrtf(fmaxf(r2 - cd2 + cd2 * cosine * cosine, 0.0f)), cd2 - r2);
+            pos = ref + dir * dist;
+        }
+        const f3 nrm = lamp_normalize(pos - center, neg(dir));
+        s.position = nrm * r + center;
+        s.direction = dir;
+        s.distance = dist;
+        s.pdf = pdf;
+        s.valid = valid;
+    } else {
+        // sample_disk_geometry
+        const float cd = sqrtf(fmaxf(cd2, 1.0e-30f));
+        const f3 ln = co / cd;
+        f3 t, b;
+        make_orthonormals(ln, t, b);
+        float dx, dy;
+        uniform_disk(rx, ry, dx, dy);
+        const f3 pos = center + (t * dx + b * dy) * r;
+        const f3 off = pos - ref;
+        const float d2 = dot(off, off);
+        const float d = sqrtf(fmaxf(d2, 1.0e-30f));
+        const f3 dir = off / d;
+        const float cosine = dot(ln, neg(dir));
+        s.valid = (cd2 > 0.0f) && (cosine > 0.0f);
+        s.direction = dir;
+        s.position = pos;
+        s.distance = d;
+        s.pdf = point_disk_pdf(d2, cosine, r);
+    }
+    s.eval = point_eval_factor(r, (l.flags & kLampNormalize) != 0u);
+    if (spot) {
+        const f3 local = spot_light_to_local(l, neg(s.direction));
+        const float attenuation = spot_light_attenuation(local, l.spot_angle, l.spot_smooth);
+        const bool use_attenuation = !finite_sphere || (cd2 > r2);
+        s.eval = s.eval * (use_attenuation ? attenuation : 1.0f);
+        s.valid = s.valid && (s.eval != 0.0f);
+    }
+    return s;
+}
+
+// sample_distant_light (:84-108)
+__device__ __forceinline__ LampSample sample_distant_light(const Lamp &l, float rx, float ry) {
+    const float half_angle = 0.5f * fmaxf(l.angle, 0.0f);
+    const float sq = sinf(0.5f * half_angle);
+    const float omc = 2.0f * sq * sq;
+    const Cone ns = sample_uniform_cone(neg(lamp3(l.axis_z)), omc, rx, ry);
+    const bool finite = half_angle > 0.0f;
+    const float sh = sinf(half_angle);
+    const float disk_area = AL_PI * sh * sh;
+    LampSample s;
+    s.direction = neg(ns.direction);
+    s.pdf = ns.pdf;
+    s.eval = ((l.flags & kLampNormalize) != 0u && finite) ? 1.0f / fmaxf(disk_area, 1.0e-20f) : 1.0f;
+    s.position = neg(s.direction);
+    s.distance = RAY_MAXIMUM;
+    s.valid = true;
+    return s;
+}
+
+struct LampHit {
+    bool valid;
+    float distance, pdf, eval;
+};
+
+// intersect_point / intersect_spot (analytic_light_intersection.h:33-260)
+__device__ LampHit intersect_lamp(const Lamp &l, f3 ro, f3 rd, float tmin, float tmax, f3 previous_normal,
+                                  bool had_transmission) {
+    const bool spot = l.type == kLampSpot;
+    const bool sphere = (l.flags & kLampSphere) != 0u;
+    const f3 center = lamp3(l.position);
+    const f3 co = ro - center;
+    const float cd2 = dot(co, co);
+    const float r2 = l.radius * l.radius;
+    const float sphere_cap = sin2_to_one_minus_cos(r2 / fmaxf(cd2, 1.0e-30f));
+    const float outside_cap = spot ? fminf(sphere_cap, spot_omc_larger_spread(l)) : sphere_cap;
+    PrimHit g;
+    float pdf;
+    if (sphere) {
+        g = intersect_sphere(ro, rd, tmin, tmax, center, l.radius);
+        pdf = sphere_light_pdf(cd2, r2, previous_normal, rd, had_transmission, outside_cap);
+    } else {
+        const f3 light_normal = lamp_normalize(co, neg(rd));
+        g = intersect_disk(ro, rd, tmin, tmax, center, light_normal, l.radius);
+        const float light_cosine = dot(light_normal, neg(rd));
+        pdf = point_disk_pdf(g.distance * g.distance, light_cosine, l.radius);
+    }
+    LampHit h;
+    h.valid = g.valid && (l.radius > 0.0f);
+    h.distance = g.distance;
+    h.pdf = pdf;
+    h.eval = point_eval_factor(l.radius, (l.flags & kLampNormalize) != 0u);
+    if (spot) {
+        const f3 local = spot_light_to_local(l, neg(rd));
+        const bool use_attenuation = !sphere || (cd2 > r2);
+        const float attenuation = spot_light_attenuation(local, l.spot_angle, l.spot_smooth);
+        h.eval = h.eval * (use_attenuation ? attenuation : 1.0f);
+        h.valid = h.valid && (dot(rd, co) < 0.0f) && (h.eval > 0.0f);
+    } else {
+        h.valid = h.valid && (h.eval > 0.0f);
+    }
+    return h;
+}
+
+// ===========================================================================
 // per-material tables evaluated once per parameter set
 // ===========================================================================
 
@@ -1227,6 +1563,7 @@ struct PathVars {
     uint32_t src_obj, src_prim;
     f3 throughput;
     float prev_pdf;
+    f3 mis_normal;  // shading normal of the last non-transparent scatter
     f3 radiance;
     float transparency;
     f3 normal, albedo;
@@ -1259,6 +1596,7 @@ __device__ __forceinline__ void init_path(const KernelParams &k, const float4 *s
     p.src_obj = p.src_prim = 0xffffffffu;
     p.throughput = V(1.0f, 1.0f, 1.0f);
     p.prev_pdf = 0.0f;
+    p.mis_normal = V(0.0f, 0.0f, 0.0f);
     p.radiance = V(0.0f, 0.0f, 0.0f);
     p.transparency = 0.0f;
     p.normal = V(0.0f, 0.0f, 0.0f);
@@ -1318,6 +1656,50 @@ __device__ bool path_step(const SceneView &s, const KernelParams &k, const float
     }
     const float u_term = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 0u).x;
     if (TRACE && ev) ev[6] = __float_as_uint(u_term);
+    // Lamps are transparent closest events resolved before the mesh or
+    // background event, without Sobol dimensions
+    // (src/luisa/path_kernel_pipeline.cpp:79-137,
+    //  src/luisa/path_kernel_closest_event.cpp:24-266,
+    //  src/luisa/path_kernel_forward_light.cpp:59-117)
+    if (s.lamp_count > 0u) {
+        const float surface_distance = hit.valid ? hit.t : p.ray.tmax;
+        const uint32_t ray_visibility = contract_visibility(p.ps.visibility);
+        const bool camera_path = (ray_visibility & VIS_CAMERA) != 0u;
+        const bool had_transmission = (p.ps.flag & FLAG_MIS_HAD_TRANSMISSION) != 0u;
+        uint32_t previous = 0xffffffffu;
+        for (;;) {
+            float event_distance = surface_distance;
+            uint32_t found = 0xffffffffu;
+            LampHit best;
+            best.distance = best.pdf = best.eval = 0.0f;
+            for (uint32_t li = 0; li < s.lamp_count; ++li) {
+                const Lamp &l = s.lamps[li];
+                if (l.type == kLampDistant) continue;
+                const bool eligible = (l.visibility & ray_visibility) != 0u &&
+                                      (camera_path || (l.flags & kLampUseMis) != 0u) && li != previous;
+                if (!eligible) continue;
+                const LampHit c =
+                    intersect_lamp(l, p.ray.o, p.ray.d, p.ray.tmin, event_distance, p.mis_normal, had_transmission);
+                if (c.valid) {
+                    found = li;
+                    best = c;
+                    event_distance = c.distance;
+                }
+            }
+            if (found == 0xffffffffu) break;
+            const Lamp &l = s.lamps[found];
+            const float light_pdf = best.pdf * s.light_selection_pdf;
+            const bool competing = p.ps.bounce > 0u && (p.bits & PATH_PREV_DELTA) == 0u;
+            const float mis = forward_light_weight(k, p.prev_pdf, light_pdf, competing, light_pdf > 0.0f);
+            const f3 light_radiance = lamp3(l.color) * (l.power * best.eval);
+            p.radiance = p.radiance + film_clamp((p.throughput * light_radiance) * mis, p.ps.bounce > 1u ? 1u : 0u,
+                                                 k.clamp_direct, k.clamp_indirect);
+            p.ps.transparent_bounce += 1u;
+            p.ray.tmin = next_float_up(best.distance);
+            previous = found;
+            if (p.ps.transparent_bounce >= k.transparent_max) return false;
+        }
+    }
     if (!hit.valid) {
         if (TRACE && ev) ev[0] |= 4u;
         // background (src/luisa/path_kernel_background_event.cpp:20-159)
@@ -1478,6 +1860,49 @@ __device__ bool path_step(const SceneView &s, const KernelParams &k, const float
                     }
                 }
             }
+        } else if (entry.kind == 1u) {
+            // NEE: analytic lamp component (src/luisa/path_kernel_nee_analytic.cpp:98-410)
+            if (TRACE && ev) {
+                ev[0] |= 16u;
+                ev[4] = light;
+            }
+            const Lamp &l = s.lamps[entry.index];
+            const bool reached_max_bounces = p.ps.bounce > l.max_bounces;
+            const bool distant = l.type == kLampDistant;
+            const bool has_transmission = (rflags & RT_BSDF_HAS_TRANSMISSION) != 0u;
+            const LampSample ls = distant ? sample_distant_light(l, ul.x, ul.y)
+                                          : sample_point_or_spot(l, sf.position, sf.ns, has_transmission, ul.x, ul.y);
+            const f3 light_radiance = lamp3(l.color) * (l.power * ls.eval);
+            const float light_pdf = ls.pdf * entry.selection_pdf;
+            if (!reached_max_bounces && ls.valid && light_pdf > 0.0f) {
+                const Eval evl = evaluate_mixture(cs, sf.incoming, sf.ns, ls.direction, lobe_mask);
+                const float cycles_mis = nee_light_weight(k, light_pdf, evl.pdf);
+                const float mis = (l.flags & kLampForward) != 0u ? cycles_mis : 1.0f;
+                const f3 weighted = (evl.f * light_radiance) * (mis / fmaxf(light_pdf, 1.0e-20f));
+                if (any_nonzero(weighted)) {
+                    const float u_lightterm =
+                        sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 2u).x;
+                    const float rr = light_roulette(weighted, u_lightterm, k.light_inv_rr_threshold);
+                    const f3 surviving = weighted * rr;
+                    if (any_nonzero(surviving)) {
+                        const f3 so = surface_shadow_origin(s, sf, ls.direction);
+                        const f3 foff = ls.position - so;
+                        const float fdist = sqrtf(fmaxf(dot(foff, foff), 1.0e-20f));
+                        sr.ray.o = so;
+                        sr.ray.d = distant ? ls.direction : foff / fdist;
+                        sr.ray.tmin = 0.0f;
+                        sr.ray.tmax = distant ? RAY_MAXIMUM : fdist;
+                        sr.q.visibility = VIS_SHADOW;
+                        sr.q.src_obj = sf.object;
+                        sr.q.src_prim = sf.cprim;
+                        sr.q.light_obj = sr.q.light_prim = 0xffffffffu;
+                        sr.L = p.throughput * surviving;
+                        sr.depth = p.ps.bounce;
+                        sr.tdepth = p.ps.transparent_bounce;
+                        sr.active = true;
+                    }
+                }
+            }
         }
     }
     // surface scatter (src/luisa/path_kernel_surface_scatter.cpp:16-324)
@@ -1496,6 +1921,7 @@ __device__ bool path_step(const SceneView &s, const KernelParams &k, const float
     if (!transparent) {
         p.prev_pdf = bs.eval.pdf;
         p.bits = singular ? (p.bits | PATH_PREV_DELTA) : (p.bits & ~PATH_PREV_DELTA);
+        p.mis_normal = sf.ns;
         const f3 nd = normalize(bs.wi);
         p.ray.o = surface_ray_origin(s, sf, nd);
         p.ray.d = nd;

@@ -52,6 +52,7 @@ __device__ __forceinline__ void store_path(const WavefrontView &w, uint32_t i, c
     w.ray_d[i] = make_float4(p.ray.d.x, p.ray.d.y, p.ray.d.z, p.ray.tmax);
     w.aux[i] = make_uint4(p.src_obj, p.src_prim, p.step, p.bits);
     w.throughput[i] = make_float4(p.throughput.x, p.throughput.y, p.throughput.z, p.prev_pdf);
+    w.mis_normal[i] = make_float4(p.mis_normal.x, p.mis_normal.y, p.mis_normal.z, 0.0f);
     w.radiance[i] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, p.transparency);
     w.normal[i] = make_float4(p.normal.x, p.normal.y, p.normal.z, 0.0f);
     w.albedo[i] = make_float4(p.albedo.x, p.albedo.y, p.albedo.z, 0.0f);
@@ -75,6 +76,8 @@ __device__ __forceinline__ PathVars load_path(const WavefrontView &w, uint32_t i
     const float4 t = w.throughput[i];
     p.throughput = V(t.x, t.y, t.z);
     p.prev_pdf = t.w;
+    const float4 mn = w.mis_normal[i];
+    p.mis_normal = V(mn.x, mn.y, mn.z);
     const float4 r = w.radiance[i];
     p.radiance = V(r.x, r.y, r.z);
     p.transparency = r.w;
@@ -125,7 +128,7 @@ __global__ __launch_bounds__(kShadeBlock) void generate_kernel(KernelParams k, c
             p.ray.o = p.ray.d = V(0.0f, 0.0f, 0.0f);
             p.ray.tmin = p.ray.tmax = 0.0f;
             p.src_obj = p.src_prim = 0xffffffffu;
-            p.throughput = p.radiance = p.normal = p.albedo = V(0.0f, 0.0f, 0.0f);
+            p.throughput = p.radiance = p.normal = p.albedo = p.mis_normal = V(0.0f, 0.0f, 0.0f);
             p.prev_pdf = p.transparency = 0.0f;
             p.ps = PathState{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
             p.sample = sample;

@@ -388,6 +388,64 @@ def cornell_box() -> SceneSnapshot:
                          name="cornell")
 
 
+def lamp(kind: int, transform, color=(1.0, 1.0, 1.0), power=1.0, size=0.0, spot_angle=math.pi / 4,
+         spot_smooth=0.15, angle=0.0, normalize=True, sphere=True, use_mis=True,
+         visibility=abi.VIS_ALL, max_bounces=1024) -> abi.Light:
+    """contract::LightDesc for a point, spot, distant or background lamp
+    (include/psycles/contract/scene.h:377-403); `transform` is 4x4 row-major
+    numpy, its z axis the lamp's axis (spots shine along -z, the distant
+    lamp's +z points at the sun)."""
+    l = abi.Light()
+    l.type = kind
+    l.visibility_mask = visibility
+    for i, v in enumerate(column_major(np.asarray(transform, dtype=np.float32))):
+        l.transform[i] = v
+    for i in range(3):
+        l.color[i] = color[i]
+    l.power = power
+    l.size = size
+    l.size_y = 0.0
+    l.spread = math.pi
+    l.spot_angle = spot_angle
+    l.spot_smooth = spot_smooth
+    l.angle = angle
+    l.flags = ((abi.LIGHT_FLAG_NORMALIZE if normalize else 0) | (abi.LIGHT_FLAG_SPHERE if sphere else 0) |
+               (abi.LIGHT_FLAG_USE_MIS if use_mis else 0) | abi.LIGHT_FLAG_CAST_SHADOW)
+    l.max_bounces = max_bounces
+    return l
+
+
+def frame_z(direction) -> np.ndarray:
+    """Rotation whose z axis is the unit `direction`; x and y complete a
+    right-handed orthonormal frame."""
+    z = np.asarray(direction, dtype=np.float64)
+    z = z / np.linalg.norm(z)
+    helper = np.array([0.0, 1.0, 0.0]) if abs(z[1]) < 0.9 else np.array([1.0, 0.0, 0.0])
+    x = np.cross(helper, z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    m = identity()
+    m[:3, 0], m[:3, 1], m[:3, 2] = x, y, z
+    return m
+
+
+def cornell_lamps(with_quad: bool = True) -> SceneSnapshot:
+    """The Cornell box lit by a sphere point lamp (r 0.08), a sphere spot
+    lamp (r 0.05) aimed at the floor and a distant sun (angle 0.1 rad)
+    entering through the open +z side; the ceiling quad is kept optionally."""
+    snap = cornell_box()
+    if not with_quad:
+        snap.instances = snap.instances[:3]
+    snap.lights = [
+        lamp(abi.LIGHT_POINT, translation(0.45, 0.55, 0.2), (1.0, 0.9, 0.8), 12.0, size=0.08),
+        lamp(abi.LIGHT_SPOT, translation(-0.5, 0.8, 0.4) @ frame_z((0.3, 1.0, -0.2)), (0.6, 0.8, 1.0), 25.0,
+             size=0.05, spot_angle=0.9, spot_smooth=0.3),
+        lamp(abi.LIGHT_DISTANT, frame_z((0.2, 0.5, 1.0)), (1.0, 1.0, 0.9), 2.0, angle=0.1),
+    ]
+    snap.name = "cornell_lamps"
+    return snap
+
+
 def cornell_settings(width=640, height=400) -> abi.RenderSettings:
     """16 spp, max 4 bounces, RR from bounce 3 (min_bounces 1), seed 0."""
     s = abi.default_settings(width, height)

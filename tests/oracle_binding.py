@@ -96,6 +96,9 @@ def load():
     lib.oracle_primary_hits.argtypes = [vp, P(abi.RenderSettings), u32, u32, P(u32)]
     lib.oracle_trace_events.argtypes = [vp, P(abi.RenderSettings), u32, u32, P(u32)]
     lib.oracle_sampler_probe.argtypes = [P(abi.RenderSettings), u32, u32, u32, u32, P(u32)]
+    lib.oracle_sample_lamp.argtypes = [P(abi.Light), P(f32), P(f32), C.c_int, P(f32), P(f32)]
+    lib.oracle_intersect_lamp.argtypes = [P(abi.Light), P(f32), P(f32), f32, f32, P(f32), C.c_int,
+                                          P(f32)]
     _lib = lib
     return lib
 
@@ -214,3 +217,27 @@ def ref_library():
     lib.ref_light_distribution.argtypes = [P(f32), u32, u32, C.c_int, P(u32), P(f32), P(f32)]
     lib.ref_light_distribution.restype = u32
     return lib
+
+
+def _vec(v, n=3):
+    return (C.c_float * n)(*[float(x) for x in v])
+
+
+def sample_lamp(light, reference, shading_normal, has_transmission, random):
+    """Oracle lamp sample: dict of direction, pdf, position, distance,
+    normal, eval, valid."""
+    out = (C.c_float * 13)()
+    rc = load().oracle_sample_lamp(C.byref(light), _vec(reference), _vec(shading_normal),
+                                   int(has_transmission), _vec(random, 2), out)
+    assert rc == 0
+    o = list(out)
+    return {"direction": o[0:3], "pdf": o[3], "position": o[4:7], "distance": o[7],
+            "normal": o[8:11], "eval": o[11], "valid": o[12] != 0.0}
+
+
+def intersect_lamp(light, origin, direction, tmin, tmax, previous_normal, had_transmission):
+    out = (C.c_float * 4)()
+    rc = load().oracle_intersect_lamp(C.byref(light), _vec(origin), _vec(direction), tmin, tmax,
+                                      _vec(previous_normal), int(had_transmission), out)
+    assert rc == 0
+    return {"valid": out[0] != 0.0, "distance": out[1], "pdf": out[2], "eval": out[3]}
