@@ -36,6 +36,11 @@ ORACLE_TRIS_PER_RAY = 8.59
 BYTES_PER_RAY = 48.0 + 32.0 * ORACLE_NODES_PER_RAY + 36.0 * ORACLE_TRIS_PER_RAY
 FILM_BYTES_PER_PIXEL_DISPATCH = 104.0
 HBM_PEAK_GBS = 8000.0
+# HBM bytes the trace kernel moves per ray on this workload: rocprofv3
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale factors from a 1 GiB copy),
+# divided by the rays of the profiled dispatch
+# (profiles/wf_trace_kernel_instanced_pmc.md)
+TRACE_TRAFFIC_BYTES_PER_RAY = 1091.0
 SAMPLES_PER_DISPATCH = 8
 
 
@@ -65,6 +70,27 @@ def cpu_baseline(snap, settings, seconds_target=15.0):
     }
 
 
+def calibration_copy(nbytes: int = 1 << 30) -> None:
+    """One device-to-device copy of `nbytes` through the HIP runtime (its copy
+    kernel reads and writes exactly nbytes): the known-size reference that
+    scales FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        hip.hipMemset(src, 1, C.c_size_t(nbytes))
+        hip.hipDeviceSynchronize()
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    print(f"bench: calibration copy of {nbytes} bytes done", file=sys.stderr, flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +99,8 @@ def main() -> int:
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the megakernel frame")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="finish with a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -155,7 +183,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": rays_per_launch * TRACE_TRAFFIC_BYTES_PER_RAY,
             "kernel": "wf_trace_kernel",
             "launch_ms": trace_launch_ms,
             "launches": trace_launches,
@@ -189,6 +217,8 @@ def main() -> int:
         print("bench: timing the CPU oracle baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(snap, settings)
     print(json.dumps(result))
+    if args.calibrate_copy:
+        calibration_copy()
     return 0
 
 
