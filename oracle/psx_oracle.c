@@ -841,6 +841,8 @@ typedef struct olamp {
 struct oracle_scene {
     olamp *lamps;
     uint32_t lamp_count;
+    oracle_light_tree tree;     /* light tree over emissive triangles, then lamps */
+    uint32_t *emitter_cprim;    /* Cycles primitive of each emissive triangle     */
     omaterial *materials;
     uint32_t material_count;
     omesh *meshes;
@@ -1215,6 +1217,397 @@ static void *dupmem(const void *src, size_t bytes) {
     return p;
 }
 
+/* ======================================================================= */
+/* light tree: emitter measures and device-side selection                  */
+/* (src/luisa/path_tracer_light_tree_scene.cpp:130-262,                    */
+/*  src/luisa/path_tracer_light_tree.cpp:14-826)                           */
+/* ======================================================================= */
+#define LT_PI 3.14159265358979323846f
+#define LT_HALF_PI (0.5f * LT_PI)
+#define LT_SAFE 1.0e-20f
+#define LT_INVALID 0xffffffffu
+
+static v3 lt_normalize_or(v3 v, v3 fallback) {
+    const float m = sqrtf(fmaxf(vdot(v, v), 0.0f));
+    return m > 1.0e-20f ? vscale(v, 1.0f / m) : fallback;
+}
+static float lt_average_abs(v3 v) { return (fabsf(v.x) + fabsf(v.y) + fabsf(v.z)) / 3.0f; }
+
+/* make_triangle_light_tree_emitter (:130-178) */
+static oracle_tree_emitter triangle_tree_emitter(uint32_t id, const oinstance *inst, v3 p0, v3 p1, v3 p2,
+                                                 v3 estimate, uint32_t emission_sampling) {
+    const v3 v0 = xform_point(&inst->o2w, p0), v1 = xform_point(&inst->o2w, p1), v2 = xform_point(&inst->o2w, p2);
+    const v3 cr = vcross(vsub(v1, v0), vsub(v2, v0));
+    const float area = 0.5f * sqrtf(fmaxf(vdot(cr, cr), 0.0f));
+    const int front = emission_sampling == PSX_EMISSION_SAMPLING_FRONT;
+    const int back = emission_sampling == PSX_EMISSION_SAMPLING_BACK;
+    v3 axis = (front || back) ? lt_normalize_or(cr, V3(0, 0, 1)) : lt_normalize_or(vsub(v0, v1), V3(1, 0, 0));
+    if (back) axis = vscale(axis, -1.0f);
+    if (front || back) {
+        const v3 c0 = V3(inst->o2w.m[0], inst->o2w.m[1], inst->o2w.m[2]);
+        const v3 c1 = V3(inst->o2w.m[4], inst->o2w.m[5], inst->o2w.m[6]);
+        const v3 c2 = V3(inst->o2w.m[8], inst->o2w.m[9], inst->o2w.m[10]);
+        if (vdot(c0, vcross(c1, c2)) < 0.0f) axis = vscale(axis, -1.0f);
+    }
+    oracle_tree_emitter e;
+    memset(&e, 0, sizeof(e));
+    const v3 lo = V3(fminf(fminf(v0.x, v1.x), v2.x), fminf(fminf(v0.y, v1.y), v2.y), fminf(fminf(v0.z, v1.z), v2.z));
+    const v3 hi = V3(fmaxf(fmaxf(v0.x, v1.x), v2.x), fmaxf(fmaxf(v0.y, v1.y), v2.y), fmaxf(fmaxf(v0.z, v1.z), v2.z));
+    e.bounds_min[0] = lo.x; e.bounds_min[1] = lo.y; e.bounds_min[2] = lo.z;
+    e.bounds_max[0] = hi.x; e.bounds_max[1] = hi.y; e.bounds_max[2] = hi.z;
+    e.bounds_empty = 0;
+    e.axis[0] = axis.x; e.axis[1] = axis.y; e.axis[2] = axis.z;
+    e.theta_o = (front || back) ? 0.0f : LT_HALF_PI;
+    e.theta_e = LT_HALF_PI;
+    e.orientation_empty = 0;
+    e.energy = area * lt_average_abs(estimate);
+    const v3 c = vscale(vadd(vadd(v0, v1), v2), 1.0f / 3.0f);
+    e.centroid[0] = c.x; e.centroid[1] = c.y; e.centroid[2] = c.z;
+    e.emitter_id = id;
+    e.distant = 0;
+    return e;
+}
+
+/* make_analytic_light_tree_emitter (:180-250), lamps without shaders */
+static oracle_tree_emitter lamp_tree_emitter(uint32_t id, const olamp *l) {
+    v3 strength = vscale(l->color, l->power);
+    if ((l->flags & LAMP_NORMALIZE) == 0u) {
+        float area = 1.0f;
+        if (l->type == PSX_LIGHT_POINT || l->type == PSX_LIGHT_SPOT)
+            area = l->radius > 0.0f ? 4.0f * LT_PI * l->radius * l->radius : 4.0f;
+        else if (l->type == PSX_LIGHT_DISTANT && l->angle > 0.0f) {
+            const float s = sinf(0.5f * l->angle);
+            area = LT_PI * s * s;
+        }
+        strength = vscale(strength, area);
+    }
+    oracle_tree_emitter e;
+    memset(&e, 0, sizeof(e));
+    e.bounds_empty = 1;
+    const v3 axis = vscale(l->axis_z, -1.0f);
+    e.axis[0] = axis.x; e.axis[1] = axis.y; e.axis[2] = axis.z;
+    e.orientation_empty = 0;
+    if (l->type == PSX_LIGHT_POINT) {
+        e.theta_o = LT_PI;
+        e.theta_e = LT_HALF_PI;
+    } else if (l->type == PSX_LIGHT_SPOT) {
+        float theta_e = fminf(0.5f * l->spot_angle, LT_HALF_PI);
+        if (fabsf(LT_HALF_PI - theta_e) >= 1.0e-6f)
+            theta_e = atanf(tanf(theta_e) * fmaxf(l->axis_scale.x, l->axis_scale.y) / fmaxf(l->axis_scale.z, 1.0e-20f));
+        e.theta_e = theta_e;
+    } else {
+        e.theta_e = 0.5f * l->angle;
+        e.distant = 1;
+    }
+    if (l->type == PSX_LIGHT_POINT || l->type == PSX_LIGHT_SPOT) {
+        const v3 r = V3(l->radius, l->radius, l->radius);
+        const v3 a = vsub(l->position, r), b = vadd(l->position, r);
+        e.bounds_min[0] = fminf(a.x, b.x); e.bounds_min[1] = fminf(a.y, b.y); e.bounds_min[2] = fminf(a.z, b.z);
+        e.bounds_max[0] = fmaxf(a.x, b.x); e.bounds_max[1] = fmaxf(a.y, b.y); e.bounds_max[2] = fmaxf(a.z, b.z);
+        e.bounds_empty = 0;
+        strength = vscale(strength, 0.25f / LT_PI);
+    }
+    e.energy = lt_average_abs(strength);
+    e.centroid[0] = l->position.x; e.centroid[1] = l->position.y; e.centroid[2] = l->position.z;
+    e.emitter_id = id;
+    return e;
+}
+
+typedef struct lt_measure {
+    v3 bmin, bmax, axis;
+    float theta_o, theta_e, energy;
+    int has_bounds, has_orientation, distant;
+} lt_measure;
+
+static lt_measure lt_unpack(const float *m, const uint32_t *flags, int distant) {
+    lt_measure r;
+    r.bmin = V3(m[0], m[1], m[2]);
+    r.bmax = V3(m[3], m[4], m[5]);
+    r.axis = V3(m[6], m[7], m[8]);
+    r.theta_o = m[9];
+    r.theta_e = m[10];
+    r.energy = m[11];
+    r.has_bounds = flags[0] == 0u;
+    r.has_orientation = flags[1] == 0u;
+    r.distant = distant;
+    return r;
+}
+
+static float lt_sin_from_cos(float c) { return sqrtf(fmaxf(1.0f - c * c, 0.0f)); }
+
+/* safe_normalize of path_tracer_light_tree.cpp:19-27 */
+static v3 lt_safe_normalize(v3 v, v3 fallback) {
+    const float l2 = vdot(v, v);
+    return l2 > LT_SAFE ? vscale(v, 1.0f / sqrtf(fmaxf(l2, LT_SAFE))) : fallback;
+}
+
+/* cos_bound_subtended_angle (:40-53) */
+static float lt_cos_bound(v3 bmax, v3 centroid, v3 point) {
+    const v3 pc = vsub(point, centroid), bc = vsub(bmax, centroid);
+    const float d2 = vdot(pc, pc), r2 = vdot(bc, bc);
+    return d2 <= r2 ? -1.0f : sqrtf(fmaxf(1.0f - r2 / fmaxf(d2, LT_SAFE), 0.0f));
+}
+
+typedef struct lt_imp { float max, min; } lt_imp;
+
+/* bounded_importance (:83-180), surface variant */
+static lt_imp lt_bounded(v3 n, int has_transmission, v3 p2c, float cos_theta_u, v3 cone_axis, float theta_o,
+                         float theta_e, float maximum_distance, float minimum_distance, float energy) {
+    cos_theta_u = fclamp(cos_theta_u, -1.0f, 1.0f);
+    const float sin_theta_u = lt_sin_from_cos(cos_theta_u);
+    int visible = energy > 0.0f;
+    const float ic = has_transmission ? fabsf(vdot(p2c, n)) : vdot(p2c, n);
+    const float is = lt_sin_from_cos(fclamp(ic, -1.0f, 1.0f));
+    const float minimum_incidence = ic >= cos_theta_u ? 1.0f : ic * cos_theta_u + is * sin_theta_u;
+    const float maximum_incidence = fmaxf(ic * cos_theta_u - is * sin_theta_u, 0.0f);
+    visible = visible && (has_transmission || minimum_incidence >= 0.0f);
+    const float tc = fclamp(vdot(cone_axis, vneg(p2c)), -1.0f, 1.0f);
+    const float ts = lt_sin_from_cos(tc);
+    const float tmu_c = tc * cos_theta_u + ts * sin_theta_u;
+    const float to_c = cosf(theta_o), to_s = sinf(theta_o);
+    const int fully = (tc >= cos_theta_u) || (tmu_c >= to_c);
+    const int partially = !fully && ((theta_o + theta_e > LT_PI) || (tmu_c > cosf(theta_o + theta_e)));
+    const float tmu_s = lt_sin_from_cos(fclamp(tmu_c, -1.0f, 1.0f));
+    const float minimum_outgoing = fully ? 1.0f : (partially ? tmu_c * to_c + tmu_s * to_s : 0.0f);
+    visible = visible && (fully || partially);
+    const float mind = fmaxf(minimum_distance, LT_SAFE);
+    const float maxd = fmaxf(maximum_distance, LT_SAFE);
+    const float geometric = 1.0f / (mind * mind);
+    lt_imp r;
+    r.max = visible ? fabsf(minimum_incidence * energy * minimum_outgoing * geometric) : 0.0f;
+    const float tpu_c = tc * cos_theta_u - ts * sin_theta_u;
+    const int minimum_is_zero = (theta_e - theta_o < 0.0f) || (tc < 0.0f) || (cos_theta_u < 0.0f) ||
+                                (tpu_c < cosf(theta_e - theta_o));
+    const float tpu_s = lt_sin_from_cos(fclamp(tpu_c, -1.0f, 1.0f));
+    const float maximum_outgoing = tpu_c * to_c - tpu_s * to_s;
+    r.min = (minimum_is_zero || !visible) ? 0.0f
+                                          : fabsf(maximum_incidence * energy * maximum_outgoing / (maxd * maxd));
+    return r;
+}
+
+/* measure_importance (:185-283), surface variant */
+static lt_imp lt_measure_importance(const lt_measure *m, v3 point, v3 n, int has_transmission) {
+    lt_imp r = {0.0f, 0.0f};
+    if (!(m->has_orientation && m->energy > 0.0f)) return r;
+    const v3 cone_axis = lt_safe_normalize(m->axis, V3(0, 0, 1));
+    v3 p2c = vneg(cone_axis);
+    float cos_theta_u = cosf(m->theta_o + m->theta_e);
+    float emitter_distance = 1.0f;
+    int visible = 1;
+    if (!m->distant) {
+        const v3 centroid = vscale(vadd(m->bmin, m->bmax), 0.5f);
+        const v3 extent = vsub(m->bmax, centroid);
+        const v3 an = V3(fabsf(n.x), fabsf(n.y), fabsf(n.z));
+        const v3 ae = V3(fabsf(extent.x), fabsf(extent.y), fabsf(extent.z));
+        visible = has_transmission || (vdot(n, vsub(centroid, point)) + vdot(an, ae) > 0.0f);
+        const v3 delta = vsub(centroid, point);
+        emitter_distance = sqrtf(vdot(delta, delta));
+        p2c = lt_safe_normalize(delta, vneg(cone_axis));
+        cos_theta_u = lt_cos_bound(m->bmax, centroid, point);
+        const v3 cb = vsub(centroid, m->bmax);
+        emitter_distance = fmaxf(0.5f * sqrtf(vdot(cb, cb)), emitter_distance);
+    }
+    if (visible)
+        r = lt_bounded(n, has_transmission, p2c, cos_theta_u, cone_axis, m->theta_o, m->theta_e, emitter_distance,
+                       emitter_distance, m->energy);
+    return r;
+}
+
+static lt_imp lt_emitter_importance(const oracle_light_tree *t, uint32_t i, v3 point, v3 n, int has_transmission) {
+    const oracle_tree_record *e = &t->emitters[i];
+    const lt_measure m = lt_unpack(e->measure, e->flags, e->distant != 0u);
+    return lt_measure_importance(&m, point, n, has_transmission);
+}
+
+/* child_importance (:317-350) */
+static lt_imp lt_child_importance(const oracle_light_tree *t, uint32_t index, v3 point, v3 n, int has_transmission) {
+    const oracle_tree_node *node = &t->nodes[index];
+    lt_imp r = {0.0f, 0.0f};
+    if (node->kind == ORACLE_TREE_INNER || node->count > 1u) {
+        const lt_measure m = lt_unpack(node->measure, node->flags, node->kind == ORACLE_TREE_DISTANT);
+        r = lt_measure_importance(&m, point, n, has_transmission);
+    } else if (node->count == 1u) {
+        r = lt_emitter_importance(t, node->first, point, n, has_transmission);
+    }
+    return r;
+}
+
+/* left_probability (:352-386): (probability of the left child, usable) */
+static void lt_left_probability(const oracle_light_tree *t, uint32_t left, uint32_t right, v3 point, v3 n,
+                                int has_transmission, float *p_left, float *usable) {
+    const lt_imp li = lt_child_importance(t, left, point, n, has_transmission);
+    const lt_imp ri = lt_child_importance(t, right, point, n, has_transmission);
+    const float max_total = li.max + ri.max;
+    const float min_total = li.min + ri.min;
+    const float max_p = li.max / fmaxf(max_total, LT_SAFE);
+    const float min_p = min_total > 0.0f ? li.min / fmaxf(min_total, LT_SAFE)
+                                         : 0.5f * ((li.max > 0.0f ? 1.0f : 0.0f) + (ri.max == 0.0f ? 1.0f : 0.0f));
+    *p_left = 0.5f * (max_p + min_p);
+    *usable = max_total > 0.0f ? 1.0f : 0.0f;
+}
+
+static void lt_reservoir(uint32_t current, float weight, uint32_t *selected, float *selected_weight, float *total,
+                         float *random) {
+    if (!(weight > 0.0f)) return;
+    *total += weight;
+    if (*selected == LT_INVALID) {
+        *selected = current;
+        *selected_weight = weight;
+        return;
+    }
+    const float threshold = weight / *total;
+    if (*random <= threshold) {
+        *selected = current;
+        *selected_weight = weight;
+        *random = *random / threshold;
+    } else {
+        *random = (*random - threshold) / fmaxf(1.0f - threshold, LT_SAFE);
+    }
+    *random = fclamp(*random, 0.0f, 1.0f);
+}
+
+/* select_leaf_emitter (:388-535) */
+static void lt_select_leaf(const oracle_light_tree *t, const oracle_tree_node *node, v3 point, v3 n,
+                           int has_transmission, float *random, uint32_t *selected, float *pdf) {
+    float sel_max = 0.0f, sel_min = 0.0f, tot_max = 0.0f, tot_min = 0.0f;
+    uint32_t positive = 0u;
+    const int sample_maximum = *random > 0.5f;
+    if (node->count > 1u) *random = *random * 2.0f - (sample_maximum ? 1.0f : 0.0f);
+    for (uint32_t i = 0; i < node->count; ++i) {
+        const uint32_t idx = node->first + i;
+        const lt_imp imp = lt_emitter_importance(t, idx, point, n, has_transmission);
+        tot_max += imp.max;
+        tot_min += imp.min;
+        positive += imp.max > 0.0f ? 1u : 0u;
+        const float primary = sample_maximum ? imp.max : imp.min;
+        const float primary_total = sample_maximum ? tot_max : tot_min;
+        if (primary > 0.0f) {
+            if (*selected == LT_INVALID) {
+                *selected = idx;
+                sel_max = imp.max;
+                sel_min = imp.min;
+            } else {
+                const float threshold = primary / primary_total;
+                if (*random <= threshold) {
+                    *selected = idx;
+                    sel_max = imp.max;
+                    sel_min = imp.min;
+                    *random = *random / threshold;
+                } else {
+                    *random = (*random - threshold) / fmaxf(1.0f - threshold, LT_SAFE);
+                }
+                *random = fclamp(*random, 0.0f, 1.0f);
+            }
+        }
+    }
+    if (positive > 0u && tot_min == 0.0f) {
+        if (sample_maximum) {
+            sel_min = 1.0f;
+            tot_min = (float)positive;
+        } else {
+            *selected = LT_INVALID;
+            float sel_uniform = 0.0f, tot_uniform = 0.0f;
+            for (uint32_t i = 0; i < node->count; ++i) {
+                const uint32_t idx = node->first + i;
+                const lt_imp imp = lt_emitter_importance(t, idx, point, n, has_transmission);
+                lt_reservoir(idx, imp.max > 0.0f ? 1.0f : 0.0f, selected, &sel_uniform, &tot_uniform, random);
+                if (*selected == idx) {
+                    sel_max = imp.max;
+                    sel_min = 1.0f;
+                }
+            }
+            tot_min = tot_uniform;
+        }
+    }
+    if (*selected != LT_INVALID)
+        *pdf *= 0.5f * (sel_max / fmaxf(tot_max, LT_SAFE) + sel_min / fmaxf(tot_min, LT_SAFE));
+}
+
+/* make_sample_callable (:551-637): returns the emitter id (LT_INVALID when
+ * nothing is selected) and its selection pdf. */
+static uint32_t lt_sample(const oracle_light_tree *t, float u, v3 point, v3 n, int has_transmission, float *pdf) {
+    *pdf = 0.0f;
+    if (t->node_count == 0u || t->emitter_count == 0u || t->root >= t->node_count) return LT_INVALID;
+    float random = fclamp(u, 0.0f, 1.0f);
+    float selection_pdf = 1.0f;
+    uint32_t index = t->root, selected = LT_INVALID, steps = 0u;
+    int active = 1;
+    while (active && steps <= t->node_count) {
+        steps += 1u;
+        const oracle_tree_node *node = &t->nodes[index];
+        if (node->kind != ORACLE_TREE_INNER) {
+            lt_select_leaf(t, node, point, n, has_transmission, &random, &selected, &selection_pdf);
+            active = 0;
+        } else {
+            float p_left, usable;
+            lt_left_probability(t, node->left, node->right, point, n, has_transmission, &p_left, &usable);
+            if (usable <= 0.0f) {
+                active = 0;
+            } else {
+                const float p_right = 1.0f - p_left;
+                const int choose_right = random <= p_right;
+                index = choose_right ? node->right : node->left;
+                selection_pdf *= choose_right ? p_right : p_left;
+                random = choose_right ? random / fmaxf(p_right, LT_SAFE)
+                                      : (random - p_right) / fmaxf(p_left, LT_SAFE);
+                random = fclamp(random, 0.0f, 1.0f);
+            }
+        }
+    }
+    if (selected == LT_INVALID) return LT_INVALID;
+    *pdf = selection_pdf;
+    return t->emitters[selected].emitter_id;
+}
+
+/* make_pdf_callable (:639-744) */
+static float lt_pdf(const oracle_light_tree *t, uint32_t emitter_id, v3 point, v3 n, int has_transmission) {
+    if (t->node_count == 0u || t->emitter_count == 0u || emitter_id >= t->emitter_count) return 0.0f;
+    const uint32_t target = t->emitter_to_tree[emitter_id];
+    uint32_t current = t->emitter_to_leaf[emitter_id];
+    if (target >= t->emitter_count || current >= t->node_count) return 0.0f;
+    const oracle_tree_node *leaf = &t->nodes[current];
+    float tmax = 0.0f, tmin = 0.0f, tot_max = 0.0f, tot_min = 0.0f;
+    uint32_t positive = 0u;
+    for (uint32_t i = 0; i < leaf->count; ++i) {
+        const uint32_t idx = leaf->first + i;
+        const lt_imp imp = lt_emitter_importance(t, idx, point, n, has_transmission);
+        tot_max += imp.max;
+        tot_min += imp.min;
+        positive += imp.max > 0.0f ? 1u : 0u;
+        if (idx == target) {
+            tmax = imp.max;
+            tmin = imp.min;
+        }
+    }
+    if (!(tmax > 0.0f)) return 0.0f;
+    float result = 0.5f * (tmax / fmaxf(tot_max, LT_SAFE) +
+                           (tot_min > 0.0f ? tmin / fmaxf(tot_min, LT_SAFE)
+                                           : 1.0f / fmaxf((float)positive, 1.0f)));
+    uint32_t steps = 0u;
+    int valid = 1;
+    while (valid && current != t->root && steps <= t->node_count) {
+        steps += 1u;
+        const uint32_t parent_index = t->nodes[current].parent;
+        if (parent_index >= t->node_count) {
+            valid = 0;
+            result = 0.0f;
+        } else {
+            const oracle_tree_node *parent = &t->nodes[parent_index];
+            float p_left, usable;
+            lt_left_probability(t, parent->left, parent->right, point, n, has_transmission, &p_left, &usable);
+            if (usable <= 0.0f) {
+                valid = 0;
+                result = 0.0f;
+            } else {
+                result *= current == parent->left ? p_left : 1.0f - p_left;
+                current = parent_index;
+            }
+        }
+    }
+    if (current != t->root) result = 0.0f;
+    return result;
+}
+
 /* LightDesc -> LightGpu (src/luisa/path_tracer_scene.cpp:1735-1870) */
 static void lamp_from_desc(const psx_light *l, olamp *lp) {
     lp->type = l->type;
@@ -1437,6 +1830,31 @@ oracle_scene *oracle_scene_create(const psx_scene_desc *d, int brute_force) {
             s->distribution_count = 0u;
         free(areas);
     }
+    /* light tree over the same population: triangles, then lamps
+     * (src/luisa/path_tracer_light_sampling_scene.cpp:150-245) */
+    {
+        const uint32_t total = s->emitter_count + s->lamp_count;
+        oracle_tree_emitter *te = (oracle_tree_emitter *)calloc(total ? total : 1, sizeof(oracle_tree_emitter));
+        s->emitter_cprim = (uint32_t *)calloc(s->emitter_count ? s->emitter_count : 1, sizeof(uint32_t));
+        for (uint32_t i = 0; i < s->emitter_count; ++i) {
+            const emitter *em = &s->emitters[i];
+            const oinstance *inst = &s->instances[em->instance];
+            const omesh *m = &s->meshes[inst->mesh];
+            v3 p0, p1, p2;
+            tri_positions(m->positions, m->tris, em->prim, &p0, &p1, &p2);
+            const omaterial *mat = &s->materials[em->material];
+            const v3 estimate = mat->count ? emission_estimate(mat, mat->root) : V3(0, 0, 0);
+            te[i] = triangle_tree_emitter(i, inst, p0, p1, p2, estimate, em->emission_sampling);
+            s->emitter_cprim[i] = m->prim_offset + em->prim;
+        }
+        for (uint32_t j = 0; j < s->lamp_count; ++j)
+            te[s->emitter_count + j] = lamp_tree_emitter(s->emitter_count + j, &s->lamps[j]);
+        if (oracle_light_tree_build(te, total, 8u, &s->tree) != 0) memset(&s->tree, 0, sizeof(s->tree));
+        if (!(s->tree.node_count > 0u && s->tree.nodes[s->tree.root].measure[11] > 0.0f)) {
+            oracle_light_tree_free(&s->tree);
+        }
+        free(te);
+    }
     return s;
 }
 
@@ -1464,6 +1882,8 @@ void oracle_scene_destroy(oracle_scene *s) {
     free(s->emitters);
     free(s->distribution);
     free(s->lamps);
+    free(s->emitter_cprim);
+    oracle_light_tree_free(&s->tree);
     free(s);
 }
 
@@ -2168,6 +2588,7 @@ typedef struct kparams {
     float pass_alpha_threshold;
     int nee;
     int mis_mode; /* psx_direct_light_sampling actually used */
+    int use_tree; /* light tree instead of the flat CDF */
     mat4 camera;
     sampler smp;
     float filter[1024];
@@ -2204,6 +2625,7 @@ static void setup_params(const oracle_scene *s, const psx_render_settings *rs,
     k->light_inv_rr_threshold = (!in->use_light_tree && in->light_sampling_threshold > 0.0f)
                                     ? in->film_exposure / in->light_sampling_threshold
                                     : 0.0f;
+    k->use_tree = in->use_light_tree != 0u;
     const int available = s->distribution_count > 0u;
     k->nee = in->direct_light_sampling != PSX_DIRECT_LIGHT_FORWARD && available;
     k->mis_mode = k->nee ? (int)in->direct_light_sampling : PSX_DIRECT_LIGHT_FORWARD;
@@ -2673,6 +3095,46 @@ int oracle_intersect_lamp(const psx_light *light, const float origin[3], const f
     return 0;
 }
 
+/* make_triangle_emitter_callable (src/luisa/path_tracer_light_tree.cpp:
+ * 746-790): emitter id of an emissive (object, Cycles primitive) */
+static uint32_t lt_triangle_emitter(const oracle_scene *s, uint32_t object, uint32_t cprim) {
+    uint32_t first = 0u, length = s->emitter_count;
+    while (length > 0u) {
+        const uint32_t half = length >> 1u;
+        const uint32_t middle = first + half;
+        const uint32_t eo = s->emitters[middle].instance, ep = s->emitter_cprim[middle];
+        if (eo < object || (eo == object && ep < cprim)) {
+            first = middle + 1u;
+            length = length - half - 1u;
+        } else {
+            length = half;
+        }
+    }
+    if (first < s->emitter_count && s->emitters[first].instance == object && s->emitter_cprim[first] == cprim)
+        return first;
+    return LT_INVALID;
+}
+
+int oracle_light_tree_query(const oracle_scene *s, const float point[3], const float normal[3],
+                            int has_transmission, float u, uint32_t *emitter_id, float *pdf_sample,
+                            float *pdf_reverse) {
+    if (!s) return -1;
+    const v3 p = load3(point), n = load3(normal);
+    float pdf = 0.0f;
+    const uint32_t id = lt_sample(&s->tree, u, p, n, has_transmission, &pdf);
+    *emitter_id = id;
+    *pdf_sample = pdf;
+    *pdf_reverse = id == LT_INVALID ? 0.0f : lt_pdf(&s->tree, id, p, n, has_transmission);
+    return 0;
+}
+
+float oracle_light_tree_pdf(const oracle_scene *s, uint32_t emitter_id, const float point[3],
+                            const float normal[3], int has_transmission) {
+    return s ? lt_pdf(&s->tree, emitter_id, load3(point), load3(normal), has_transmission) : 0.0f;
+}
+
+uint32_t oracle_light_tree_emitters(const oracle_scene *s) { return s ? s->tree.emitter_count : 0u; }
+
 /* ======================================================================= */
 /* the path integrator (src/luisa/path_kernel_pipeline.cpp:73-185 and the  */
 /* stages it calls)                                                        */
@@ -2778,8 +3240,12 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
         const float *ul = sample_4d(&k->smp, sample_index, rng_hash, ps.rng_offset + 1u);
         const v3 u_light = V3(ul[0], ul[1], ul[2]);
         const dist_entry *light = NULL;
-        if (s->distribution_count > 0u)
+        uint32_t light_id = UINT32_MAX;
+        dist_entry tree_light;
+        if (s->distribution_count > 0u && !k->use_tree) {
             light = distribution_sample(s->distribution, s->distribution_count, u_light.z);
+            light_id = (uint32_t)(light - s->distribution);
+        }
         const float u_lightterm = sample_4d(&k->smp, sample_index, rng_hash, ps.rng_offset + 2u)[0];
         const float *ub = sample_4d(&k->smp, sample_index, rng_hash, ps.rng_offset + 3u);
         const v3 u_bsdf = V3(ub[0], ub[1], ub[2]);
@@ -2818,7 +3284,11 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                 }
                 if (found == UINT32_MAX) break;
                 const olamp *l = &s->lamps[found];
-                const float light_pdf = best.pdf * s->light_selection_pdf;
+                const float selection_pdf =
+                    k->use_tree ? lt_pdf(&s->tree, s->emitter_count + found, ray.o, previous_mis_origin_normal,
+                                         had_transmission)
+                                : s->light_selection_pdf;
+                const float light_pdf = best.pdf * selection_pdf;
                 const int competing = ps.bounce > 0u && !previous_delta;
                 const float mis = forward_light_weight(k, previous_bsdf_pdf, light_pdf, competing, light_pdf > 0.0f);
                 const v3 light_radiance = vscale(l->color, l->power * best.eval);
@@ -2870,9 +3340,12 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
             sf.emission_sampling != PSX_EMISSION_SAMPLING_NONE) {
             const int competing = ps.bounce > 0u && !previous_delta;
             const v3 oriented = sf.back_facing ? vneg(sf.ng) : sf.ng;
-            const float selection_pdf =
+            float selection_pdf =
                 0.5f * vlength(vcross(vsub(sf.wp1, sf.wp0), vsub(sf.wp2, sf.wp0))) *
                 s->triangle_area_pdf;
+            if (k->use_tree)
+                selection_pdf = lt_pdf(&s->tree, lt_triangle_emitter(s, sf.object, sf.cprim), ray.o,
+                                       previous_mis_origin_normal, (ps.flag & FLAG_MIS_HAD_TRANSMISSION) != 0u);
             const tri_pdf_t tp = triangle_directional_pdf(ray.o, sf.position, sf.wp0, sf.wp1, sf.wp2);
             const v3 off = vsub(sf.position, ray.o);
             const v3 dir = normalize_or(off, V3(0, 0, 1));
@@ -2914,11 +3387,23 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
             if (writes) ps.flag |= FLAG_SINGLE_PASS_DONE;
         }
         const uint32_t runtime_flags = surface_runtime_flags(&cs, sf.back_facing);
+        /* light tree selection (src/luisa/path_kernel_pipeline.cpp:140-150) */
+        if (k->use_tree) {
+            float tree_pdf = 0.0f;
+            const uint32_t id = lt_sample(&s->tree, u_light.z, sf.position, sf.ns,
+                                          (runtime_flags & RT_BSDF_HAS_TRANSMISSION) != 0u, &tree_pdf);
+            if (id != LT_INVALID && id < s->distribution_count) {
+                tree_light = s->distribution[id];
+                tree_light.selection_pdf = tree_pdf;
+                light = &tree_light;
+                light_id = id;
+            }
+        }
         /* next-event estimation: emissive mesh component
          * (src/luisa/path_kernel_nee_emissive_mesh.cpp:30-298) */
         if (k->nee && (runtime_flags & RT_BSDF_HAS_EVAL) != 0u && light && light->kind == 0u) {
             log_flag(log, step, 16u);
-            log_event(log, step, PSX_TRACE_LIGHT, (uint32_t)(light - s->distribution));
+            log_event(log, step, PSX_TRACE_LIGHT, light_id);
             const emitter *em = &s->emitters[light->index];
             const oinstance *einst = &s->instances[em->instance];
             const omesh *emesh = &s->meshes[einst->mesh];
@@ -3003,7 +3488,7 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
          * (src/luisa/path_kernel_nee_analytic.cpp:98-410) */
         if (k->nee && (runtime_flags & RT_BSDF_HAS_EVAL) != 0u && light && light->kind == 1u) {
             log_flag(log, step, 16u);
-            log_event(log, step, PSX_TRACE_LIGHT, (uint32_t)(light - s->distribution));
+            log_event(log, step, PSX_TRACE_LIGHT, light_id);
             const olamp *l = &s->lamps[light->index];
             const int reached_max_bounces = ps.bounce > l->max_bounces;
             const v2 rnd = {u_light.x, u_light.y};

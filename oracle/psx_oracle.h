@@ -96,6 +96,52 @@ int oracle_intersect_lamp(const psx_light *light, const float origin[3],
                           const float previous_normal[3], int had_transmission,
                           float out[4]);
 
+/* --- light tree (src/sampling/light_tree.cpp, built by
+ *     psx_oracle_tree.cpp) ------------------------------------------------- */
+enum { ORACLE_TREE_INNER = 0, ORACLE_TREE_LEAF = 1, ORACLE_TREE_DISTANT = 2 };
+
+/* LightTreeEmitter input (include/psycles/sampling/light_tree.h:15-40). */
+typedef struct oracle_tree_emitter {
+    float bounds_min[3], bounds_max[3];
+    int bounds_empty;
+    float axis[3], theta_o, theta_e;
+    int orientation_empty;
+    float energy;
+    float centroid[3];
+    uint32_t emitter_id;
+    int distant;
+} oracle_tree_emitter;
+
+/* measure[12] = bounds min (3), bounds max (3), axis (3), theta_o, theta_e,
+ * energy; flags[2] = bounds empty, orientation empty. */
+typedef struct oracle_tree_node {
+    float measure[12];
+    uint32_t flags[2];
+    uint32_t parent, left, right, first, count, kind;
+} oracle_tree_node;
+
+typedef struct oracle_tree_record {
+    float measure[12];
+    uint32_t flags[2];
+    float centroid[3];
+    uint32_t emitter_id, distant;
+} oracle_tree_record;
+
+typedef struct oracle_light_tree {
+    oracle_tree_node *nodes;
+    oracle_tree_record *emitters;       /* tree (reordered) emitter order */
+    uint32_t *emitter_to_tree;          /* indexed by emitter id          */
+    uint32_t *emitter_to_leaf;
+    uint32_t node_count, emitter_count, root;
+} oracle_light_tree;
+
+/* Returns 0, or -1 when identities are not a dense permutation. */
+int oracle_light_tree_build(const oracle_tree_emitter *emitters, uint32_t count,
+                            uint32_t max_emitters_per_leaf,
+                            oracle_light_tree *out);
+void oracle_light_tree_free(oracle_light_tree *tree);
+
+
 /* --- render ------------------------------------------------------------- */
 typedef struct oracle_stats {
     uint64_t rays;
@@ -105,6 +151,18 @@ typedef struct oracle_stats {
 } oracle_stats;
 
 typedef struct oracle_scene oracle_scene;
+
+/* Scene light tree (built by oracle_scene_create): one selection at (point,
+ * normal) for random number u, with the sample's pdf and the reverse pdf of
+ * the chosen emitter; and the reverse pdf of any emitter id. */
+int oracle_light_tree_query(const oracle_scene *scene, const float point[3],
+                            const float normal[3], int has_transmission, float u,
+                            uint32_t *emitter_id, float *pdf_sample,
+                            float *pdf_reverse);
+float oracle_light_tree_pdf(const oracle_scene *scene, uint32_t emitter_id,
+                            const float point[3], const float normal[3],
+                            int has_transmission);
+uint32_t oracle_light_tree_emitters(const oracle_scene *scene);
 
 /* Build the oracle's own scene tables (binary SAH BVH with 16 bins and
  * leaves of at most 4 triangles, light distribution).  `brute_force`

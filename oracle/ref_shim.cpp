@@ -9,13 +9,33 @@
 //   sampling::make_pixel_filter_table (src/sampling/pixel_filter.cpp:30-86)
 //   sampling::build_cycles_light_distribution
 //       (src/sampling/light_distribution.cpp:25-138)
+//   sampling::build_cycles_light_tree (src/sampling/light_tree.cpp:403-408),
+//       emitted in the oracle_light_tree layout of psx_oracle.h
 #include <psycles/sampling/light_distribution.h>
+#include <psycles/sampling/light_tree.h>
 #include <psycles/sampling/pixel_filter.h>
 #include <psycles/sampling/tabulated_sobol.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <span>
+#include <vector>
+
+#include "psx_oracle.h"
+
+namespace {
+
+void put_measure(const psycles::sampling::LightTreeMeasure &m, float *f, std::uint32_t *u) {
+    const float v[12] = {m.bounds.minimum.x, m.bounds.minimum.y, m.bounds.minimum.z, m.bounds.maximum.x,
+                         m.bounds.maximum.y, m.bounds.maximum.z, m.orientation.axis.x, m.orientation.axis.y,
+                         m.orientation.axis.z, m.orientation.theta_o, m.orientation.theta_e, m.energy};
+    std::memcpy(f, v, sizeof(v));
+    u[0] = m.bounds.empty ? 1u : 0u;
+    u[1] = m.orientation.empty ? 1u : 0u;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -58,6 +78,74 @@ std::uint32_t ref_light_distribution(const float *areas, std::uint32_t count,
     *triangle_area_pdf = d.triangle_area_pdf;
     *light_selection_pdf = d.light_selection_pdf;
     return d.emitter_count;
+}
+
+int ref_light_tree_build(const oracle_tree_emitter *in, std::uint32_t count, std::uint32_t max_leaf,
+                         oracle_light_tree *out) {
+    using namespace psycles::sampling;
+    std::memset(out, 0, sizeof(*out));
+    std::vector<LightTreeEmitter> emitters(count);
+    for (std::uint32_t i = 0; i < count; ++i) {
+        const oracle_tree_emitter &s = in[i];
+        LightTreeEmitter &e = emitters[i];
+        e.measure.bounds.minimum = {s.bounds_min[0], s.bounds_min[1], s.bounds_min[2]};
+        e.measure.bounds.maximum = {s.bounds_max[0], s.bounds_max[1], s.bounds_max[2]};
+        e.measure.bounds.empty = s.bounds_empty != 0;
+        e.measure.orientation.axis = {s.axis[0], s.axis[1], s.axis[2]};
+        e.measure.orientation.theta_o = s.theta_o;
+        e.measure.orientation.theta_e = s.theta_e;
+        e.measure.orientation.empty = s.orientation_empty != 0;
+        e.measure.energy = s.energy;
+        e.centroid = {s.centroid[0], s.centroid[1], s.centroid[2]};
+        e.emitter_id = s.emitter_id;
+        e.distant = s.distant != 0;
+    }
+    CyclesLightTree tree;
+    try {
+        tree = build_cycles_light_tree(emitters, max_leaf);
+    } catch (...) {
+        return -1;
+    }
+    out->root = tree.root;
+    out->node_count = static_cast<std::uint32_t>(tree.nodes.size());
+    out->emitter_count = static_cast<std::uint32_t>(tree.emitters.size());
+    out->nodes = static_cast<oracle_tree_node *>(std::calloc(out->node_count + 1u, sizeof(oracle_tree_node)));
+    out->emitters =
+        static_cast<oracle_tree_record *>(std::calloc(out->emitter_count + 1u, sizeof(oracle_tree_record)));
+    out->emitter_to_tree = static_cast<std::uint32_t *>(std::calloc(out->emitter_count + 1u, 4u));
+    out->emitter_to_leaf = static_cast<std::uint32_t *>(std::calloc(out->emitter_count + 1u, 4u));
+    for (std::uint32_t i = 0; i < out->node_count; ++i) {
+        const auto &n = tree.nodes[i];
+        auto &o = out->nodes[i];
+        put_measure(n.measure, o.measure, o.flags);
+        o.parent = n.parent;
+        o.left = n.left;
+        o.right = n.right;
+        o.first = n.first_emitter;
+        o.count = n.emitter_count;
+        o.kind = static_cast<std::uint32_t>(n.kind);
+    }
+    for (std::uint32_t i = 0; i < out->emitter_count; ++i) {
+        const auto &e = tree.emitters[i];
+        auto &o = out->emitters[i];
+        put_measure(e.measure, o.measure, o.flags);
+        o.centroid[0] = e.centroid.x;
+        o.centroid[1] = e.centroid.y;
+        o.centroid[2] = e.centroid.z;
+        o.emitter_id = e.emitter_id;
+        o.distant = e.distant ? 1u : 0u;
+        out->emitter_to_tree[i] = tree.emitter_to_tree[i];
+        out->emitter_to_leaf[i] = tree.emitter_to_leaf[i];
+    }
+    return 0;
+}
+
+void ref_light_tree_free(oracle_light_tree *tree) {
+    std::free(tree->nodes);
+    std::free(tree->emitters);
+    std::free(tree->emitter_to_tree);
+    std::free(tree->emitter_to_leaf);
+    std::memset(tree, 0, sizeof(*tree));
 }
 
 }  // extern "C"

@@ -142,6 +142,27 @@ struct Lamp {
 constexpr uint32_t kLampNormalize = 1u, kLampSphere = 4u, kLampUseMis = 8u, kLampForward = 32u;
 constexpr uint32_t kLampPoint = 0u, kLampSpot = 1u, kLampDistant = 3u;
 
+// Light tree records (LightTreeNodeGpu / LightTreeEmitterGpu,
+// src/luisa/path_tracer_types.h:202-225): measure packed as bounds min +
+// energy, bounds max + theta_o, cone axis + theta_e.
+constexpr uint32_t kTreeHasBounds = 1u, kTreeHasOrientation = 2u, kTreeDistant = 4u;
+constexpr uint32_t kTreeInner = 0u, kTreeLeaf = 1u, kTreeDistantNode = 2u;
+
+struct TreeNode {
+    float4 bounds_min_energy;
+    float4 bounds_max_theta_o;
+    float4 cone_axis_theta_e;
+    uint4 topology;  // parent, left, right, kind
+    uint4 emitters;  // first, count, measure flags, 0
+};
+
+struct TreeEmitter {
+    float4 bounds_min_energy;
+    float4 bounds_max_theta_o;
+    float4 cone_axis_theta_e;
+    uint4 identity;  // emitter id, measure flags, 0, 0
+};
+
 struct DistEntry {
     float cumulative;
     float selection_pdf;
@@ -174,6 +195,11 @@ struct SceneView {
     const Lamp *lamps;
     uint32_t lamp_count;
     float light_selection_pdf;        // flat-CDF probability of one lamp
+    const TreeNode *tree_nodes;
+    const TreeEmitter *tree_emitters;
+    const uint2 *tree_mapping;        // emitter id -> (tree emitter, leaf node)
+    const uint32_t *emitter_cprim;    // Cycles primitive of each emissive triangle
+    uint32_t tree_node_count, tree_emitter_count, tree_root;
     uint32_t tlas_root;
     uint32_t instance_count;
     uint32_t material_count;
@@ -194,7 +220,7 @@ struct KernelParams {
     uint32_t full_width, full_height, seed, sequence_size;
     uint32_t max_bounces, min_bounces, max_diffuse, max_glossy;
     uint32_t max_transmission, transparent_min, transparent_max, max_path_steps;
-    uint32_t transparent_background, nee, mis_mode, pad0;
+    uint32_t transparent_background, nee, mis_mode, use_tree;
     float clamp_direct, clamp_indirect, light_inv_rr_threshold, pass_alpha_threshold;
     float tan_h, tan_v, shift_x, shift_y;
     float near_clip, far_clip, pad1, pad2;

@@ -24,6 +24,7 @@
 #include "../../include/psx.h"
 #include "psx_bvh.h"
 #include "psx_device.h"
+#include "psx_light_tree.h"
 
 namespace psx {
 
@@ -433,6 +434,8 @@ struct Scene {
     uint32_t tlas_depth = 0, blas_depth = 0;
     // device buffers
     std::vector<Lamp> lamp_list;
+    std::vector<uint32_t> mesh_prim_offsets;
+    DeviceBuffer tree_nodes, tree_emitters, tree_mapping, emitter_cprim;
     DeviceBuffer nodes, tris, instances_buf, meshes_buf, positions, indices, normals, smooth, slots, tri_slots,
         overrides_buf, materials_buf, leaves, params, emitters, distribution, material_shadow, material_emission, lamps;
     SceneView view{};
@@ -500,10 +503,108 @@ void Scene::refresh_material_tables() {
     hip_check(hipDeviceSynchronize());
 }
 
+// light-tree measures of an emissive triangle and of a lamp
+// (make_triangle_light_tree_emitter / make_analytic_light_tree_emitter,
+//  src/luisa/path_tracer_light_tree_scene.cpp:130-250)
+constexpr float kTreePi = 3.14159265358979323846f;
+constexpr float kTreeHalfPi = 0.5f * kTreePi;
+
+Vec3 tree_unit_or(Vec3 v, Vec3 fallback) {
+    const float n = std::sqrt(std::max(dot(v, v), 0.0f));
+    return n > 1.0e-20f ? v3(v.x * (1.0f / n), v.y * (1.0f / n), v.z * (1.0f / n)) : fallback;
+}
+float tree_average_abs(float x, float y, float z) { return (std::abs(x) + std::abs(y) + std::abs(z)) / 3.0f; }
+
+TreeEmitterIn triangle_tree_emitter(uint32_t id, const Vec3 q[3], const float *o2w, const float estimate[3],
+                                    uint32_t emission_sampling) {
+    const Vec3 c = cross(sub(q[1], q[0]), sub(q[2], q[0]));
+    const float area = 0.5f * std::sqrt(std::max(dot(c, c), 0.0f));
+    const bool front = emission_sampling == PSX_EMISSION_SAMPLING_FRONT;
+    const bool back = emission_sampling == PSX_EMISSION_SAMPLING_BACK;
+    Vec3 axis = (front || back) ? tree_unit_or(c, v3(0.0f, 0.0f, 1.0f)) : tree_unit_or(sub(q[0], q[1]), v3(1.0f, 0.0f, 0.0f));
+    if (back) axis = v3(axis.x * -1.0f, axis.y * -1.0f, axis.z * -1.0f);
+    if (front || back) {
+        const Vec3 a0 = v3(o2w[0], o2w[1], o2w[2]), a1 = v3(o2w[4], o2w[5], o2w[6]), a2 = v3(o2w[8], o2w[9], o2w[10]);
+        if (dot(a0, cross(a1, a2)) < 0.0f) axis = v3(axis.x * -1.0f, axis.y * -1.0f, axis.z * -1.0f);
+    }
+    TreeEmitterIn e;
+    e.measure.bounds_min[0] = std::min(std::min(q[0].x, q[1].x), q[2].x);
+    e.measure.bounds_min[1] = std::min(std::min(q[0].y, q[1].y), q[2].y);
+    e.measure.bounds_min[2] = std::min(std::min(q[0].z, q[1].z), q[2].z);
+    e.measure.bounds_max[0] = std::max(std::max(q[0].x, q[1].x), q[2].x);
+    e.measure.bounds_max[1] = std::max(std::max(q[0].y, q[1].y), q[2].y);
+    e.measure.bounds_max[2] = std::max(std::max(q[0].z, q[1].z), q[2].z);
+    e.measure.bounds_empty = false;
+    e.measure.axis[0] = axis.x;
+    e.measure.axis[1] = axis.y;
+    e.measure.axis[2] = axis.z;
+    e.measure.theta_o = (front || back) ? 0.0f : kTreeHalfPi;
+    e.measure.theta_e = kTreeHalfPi;
+    e.measure.orientation_empty = false;
+    e.measure.energy = area * tree_average_abs(estimate[0], estimate[1], estimate[2]);
+    const float third = 1.0f / 3.0f;
+    e.centroid[0] = ((q[0].x + q[1].x) + q[2].x) * third;
+    e.centroid[1] = ((q[0].y + q[1].y) + q[2].y) * third;
+    e.centroid[2] = ((q[0].z + q[1].z) + q[2].z) * third;
+    e.emitter_id = id;
+    e.distant = false;
+    return e;
+}
+
+TreeEmitterIn lamp_tree_emitter(uint32_t id, const Lamp &l) {
+    float strength[3] = {l.color[0] * l.power, l.color[1] * l.power, l.color[2] * l.power};
+    if ((l.flags & kLampNormalize) == 0u) {
+        float area = 1.0f;
+        if (l.type == kLampPoint || l.type == kLampSpot) {
+            area = l.radius > 0.0f ? 4.0f * kTreePi * l.radius * l.radius : 4.0f;
+        } else if (l.type == kLampDistant && l.angle > 0.0f) {
+            const float s = std::sin(0.5f * l.angle);
+            area = kTreePi * s * s;
+        }
+        for (float &v : strength) v = v * area;
+    }
+    TreeEmitterIn e;
+    e.measure.axis[0] = l.axis_z[0] * -1.0f;
+    e.measure.axis[1] = l.axis_z[1] * -1.0f;
+    e.measure.axis[2] = l.axis_z[2] * -1.0f;
+    e.measure.orientation_empty = false;
+    e.measure.bounds_empty = true;
+    for (int i = 0; i < 3; ++i) e.measure.bounds_min[i] = e.measure.bounds_max[i] = 0.0f;
+    if (l.type == kLampPoint) {
+        e.measure.theta_o = kTreePi;
+        e.measure.theta_e = kTreeHalfPi;
+    } else if (l.type == kLampSpot) {
+        float theta_e = std::min(0.5f * l.spot_angle, kTreeHalfPi);
+        if (std::abs(kTreeHalfPi - theta_e) >= 1.0e-6f)
+            theta_e = std::atan(std::tan(theta_e) * std::max(l.axis_scale[0], l.axis_scale[1]) /
+                                std::max(l.axis_scale[2], 1.0e-20f));
+        e.measure.theta_e = theta_e;
+    } else {
+        e.measure.theta_e = 0.5f * l.angle;
+        e.distant = true;
+    }
+    if (l.type == kLampPoint || l.type == kLampSpot) {
+        for (int i = 0; i < 3; ++i) {
+            const float a = l.position[i] - l.radius, b = l.position[i] + l.radius;
+            e.measure.bounds_min[i] = std::min(a, b);
+            e.measure.bounds_max[i] = std::max(a, b);
+        }
+        e.measure.bounds_empty = false;
+        const float k = 0.25f / kTreePi;
+        for (float &v : strength) v = v * k;
+    }
+    e.measure.energy = tree_average_abs(strength[0], strength[1], strength[2]);
+    for (int i = 0; i < 3; ++i) e.centroid[i] = l.position[i];
+    e.emitter_id = id;
+    return e;
+}
+
 // emissive triangles (src/luisa/path_tracer_scene.cpp:1548-1640)
 void Scene::rebuild_emitters() {
     std::vector<Emitter> em;
     std::vector<float> areas;
+    std::vector<TreeEmitterIn> tree_in;
+    std::vector<uint32_t> cprims;
     for (uint32_t i = 0; i < instances.size(); ++i) {
         const psx_instance &inst = instances[i];
         const bool light_visible =
@@ -530,9 +631,31 @@ void Scene::rebuild_emitters() {
             }
             const Vec3 n = cross(sub(q[1], q[0]), sub(q[2], q[0]));
             areas.push_back(0.5f * std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z));
+            float estimate[3] = {0.0f, 0.0f, 0.0f};
+            if (!materials[mat].instr.empty()) emission_estimate(materials[mat], materials[mat].root, estimate, 0);
+            tree_in.push_back(triangle_tree_emitter(static_cast<uint32_t>(em.size()), q, inst.transform, estimate,
+                                                    materials[mat].emission_sampling));
+            cprims.push_back(mesh_prim_offsets[mi] + p);
             em.push_back(Emitter{i, p, mat, materials[mat].emission_sampling});
         }
     }
+    for (size_t j = 0; j < lamp_list.size(); ++j)
+        tree_in.push_back(lamp_tree_emitter(static_cast<uint32_t>(em.size() + j), lamp_list[j]));
+    // light tree over triangles, then lamps
+    // (src/luisa/path_tracer_light_sampling_scene.cpp:150-245)
+    LightTreeTables tree;
+    if (!build_light_tree(tree_in, 8u, tree) || !tree.usable) tree = LightTreeTables{};
+    tree_nodes.upload(tree.nodes);
+    tree_emitters.upload(tree.emitters);
+    tree_mapping.upload(tree.mapping);
+    emitter_cprim.upload(cprims);
+    view.tree_nodes = tree_nodes.as<TreeNode>();
+    view.tree_emitters = tree_emitters.as<TreeEmitter>();
+    view.tree_mapping = tree_mapping.as<uint2>();
+    view.emitter_cprim = emitter_cprim.as<uint32_t>();
+    view.tree_node_count = static_cast<uint32_t>(tree.nodes.size());
+    view.tree_emitter_count = static_cast<uint32_t>(tree.emitters.size());
+    view.tree_root = tree.root;
     std::vector<DistEntry> dist;
     float tri_pdf = 0.0f;
     float lamp_pdf = 0.0f;
@@ -678,6 +801,7 @@ void initialize_session(Session &s, const psx_render_settings &rs) {
     const bool nee = s.scene->backend->options.next_event_estimation != 0u &&
                      in.direct_light_sampling != PSX_DIRECT_LIGHT_FORWARD && available;
     k.nee = nee ? 1u : 0u;
+    k.use_tree = in.use_light_tree ? 1u : 0u;
     k.mis_mode = nee ? in.direct_light_sampling : PSX_DIRECT_LIGHT_FORWARD;
     const psx_camera &cam = s.scene->camera;
     const float aspect = static_cast<float>(k.full_width) / static_cast<float>(k.full_height);
@@ -969,6 +1093,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         mesh_prim_offset.push_back(prim_offset);
         prim_offset += src.triangle_count;
     }
+    scene->mesh_prim_offsets = mesh_prim_offset;
     // BLASes and instances (build_cycles_instance_intersection_plan,
     // src/luisa/path_tracer_scene_geometry.cpp:272-318)
     std::vector<Node4> nodes(1);  // node 0 reserved for the TLAS root
@@ -1307,7 +1432,8 @@ int psx_scene_update_parameters(psx_scene *handle, uint32_t material, const floa
         g.may_emit = m.may_emit ? 1u : 0u;
         hip_check(hipMemcpy(s->params.as<float>() + g.param_offset, p, sizeof(float) * count, hipMemcpyHostToDevice));
         hip_check(hipMemcpy(s->materials_buf.as<Material>() + material, &g, sizeof(Material), hipMemcpyHostToDevice));
-        if (emitted != m.may_emit) s->rebuild_emitters();
+        // emitter set, flat CDF and light-tree energies follow the emission
+        if (emitted || m.may_emit) s->rebuild_emitters();
         s->refresh_material_tables();
         return static_cast<int>(PSX_OK);
     });

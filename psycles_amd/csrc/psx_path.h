@@ -1519,6 +1519,309 @@ __device__ LampHit intersect_lamp(const Lamp &l, f3 ro, f3 rd, float tmin, float
 }
 
 // ===========================================================================
+// light tree selection and reverse pdf
+// (src/luisa/path_tracer_light_tree.cpp:14-826, surface variants)
+// ===========================================================================
+constexpr float LT_PI = 3.14159265358979323846f;
+constexpr float LT_SAFE = 1.0e-20f;
+constexpr uint32_t LT_INVALID = 0xffffffffu;
+
+struct TreeImportance {
+    float max, min;
+};
+
+__device__ __forceinline__ float lt_sin_from_cos(float c) { return sqrtf(fmaxf(1.0f - c * c, 0.0f)); }
+
+__device__ __forceinline__ f3 lt_safe_normalize(f3 v, f3 fallback) {
+    const float l2 = dot(v, v);
+    return l2 > LT_SAFE ? v * (1.0f / sqrtf(fmaxf(l2, LT_SAFE))) : fallback;
+}
+
+// bounded_importance (:83-180)
+__device__ TreeImportance lt_bounded(f3 n, bool has_transmission, f3 p2c, float cos_theta_u, f3 cone_axis,
+                                     float theta_o, float theta_e, float distance, float energy) {
+    cos_theta_u = clampf(cos_theta_u, -1.0f, 1.0f);
+    const float sin_theta_u = lt_sin_from_cos(cos_theta_u);
+    bool visible = energy > 0.0f;
+    const float ic = has_transmission ? fabsf(dot(p2c, n)) : dot(p2c, n);
+    const float is = lt_sin_from_cos(clampf(ic, -1.0f, 1.0f));
+    const float minimum_incidence = ic >= cos_theta_u ? 1.0f : ic * cos_theta_u + is * sin_theta_u;
+    const float maximum_incidence = fmaxf(ic * cos_theta_u - is * sin_theta_u, 0.0f);
+    visible = visible && (has_transmission || minimum_incidence >= 0.0f);
+    const float tc = clampf(dot(cone_axis, neg(p2c)), -1.0f, 1.0f);
+    const float ts = lt_sin_from_cos(tc);
+    const float tmu_c = tc * cos_theta_u + ts * sin_theta_u;
+    const float to_c = cosf(theta_o), to_s = sinf(theta_o);
+    const bool fully = (tc >= cos_theta_u) || (tmu_c >= to_c);
+    const bool partially = !fully && ((theta_o + theta_e > LT_PI) || (tmu_c > cosf(theta_o + theta_e)));
+    const float tmu_s = lt_sin_from_cos(clampf(tmu_c, -1.0f, 1.0f));
+    const float minimum_outgoing = fully ? 1.0f : (partially ? tmu_c * to_c + tmu_s * to_s : 0.0f);
+    visible = visible && (fully || partially);
+    const float d = fmaxf(distance, LT_SAFE);
+    const float geometric = 1.0f / (d * d);
+    TreeImportance r;
+    r.max = visible ? fabsf(minimum_incidence * energy * minimum_outgoing * geometric) : 0.0f;
+    const float tpu_c = tc * cos_theta_u - ts * sin_theta_u;
+    const bool minimum_is_zero = (theta_e - theta_o < 0.0f) || (tc < 0.0f) || (cos_theta_u < 0.0f) ||
+                                 (tpu_c < cosf(theta_e - theta_o));
+    const float tpu_s = lt_sin_from_cos(clampf(tpu_c, -1.0f, 1.0f));
+    const float maximum_outgoing = tpu_c * to_c - tpu_s * to_s;
+    r.min = (minimum_is_zero || !visible) ? 0.0f : fabsf(maximum_incidence * energy * maximum_outgoing / (d * d));
+    return r;
+}
+
+// measure_importance (:185-283)
+__device__ TreeImportance lt_measure_importance(float4 bmin_energy, float4 bmax_theta_o, float4 axis_theta_e,
+                                                uint32_t flags, f3 point, f3 n, bool has_transmission) {
+    TreeImportance r;
+    r.max = r.min = 0.0f;
+    const float energy = bmin_energy.w;
+    if (!((flags & kTreeHasOrientation) != 0u && energy > 0.0f)) return r;
+    const f3 cone_axis = lt_safe_normalize(V(axis_theta_e.x, axis_theta_e.y, axis_theta_e.z), V(0.0f, 0.0f, 1.0f));
+    const float theta_o = bmax_theta_o.w, theta_e = axis_theta_e.w;
+    f3 p2c = neg(cone_axis);
+    float cos_theta_u = cosf(theta_o + theta_e);
+    float distance = 1.0f;
+    bool visible = true;
+    if ((flags & kTreeDistant) == 0u) {
+        const f3 bmin = V(bmin_energy.x, bmin_energy.y, bmin_energy.z);
+        const f3 bmax = V(bmax_theta_o.x, bmax_theta_o.y, bmax_theta_o.z);
+        const f3 centroid = (bmin + bmax) * 0.5f;
+        const f3 extent = bmax - centroid;
+        visible = has_transmission ||
+                  (dot(n, centroid - point) +
+                       dot(V(fabsf(n.x), fabsf(n.y), fabsf(n.z)), V(fabsf(extent.x), fabsf(extent.y), fabsf(extent.z))) >
+                   0.0f);
+        const f3 delta = centroid - point;
+        distance = sqrtf(dot(delta, delta));
+        p2c = lt_safe_normalize(delta, neg(cone_axis));
+        // cos_bound_subtended_angle (:40-53)
+        const f3 pc = point - centroid, bc = bmax - centroid;
+        const float d2 = dot(pc, pc), r2 = dot(bc, bc);
+        cos_theta_u = d2 <= r2 ? -1.0f : sqrtf(fmaxf(1.0f - r2 / fmaxf(d2, LT_SAFE), 0.0f));
+        const f3 cb = centroid - bmax;
+        distance = fmaxf(0.5f * sqrtf(dot(cb, cb)), distance);
+    }
+    if (visible) r = lt_bounded(n, has_transmission, p2c, cos_theta_u, cone_axis, theta_o, theta_e, distance, energy);
+    return r;
+}
+
+__device__ __forceinline__ TreeImportance lt_emitter_importance(const SceneView &s, uint32_t i, f3 point, f3 n,
+                                                               bool has_transmission) {
+    const TreeEmitter &e = s.tree_emitters[i];
+    return lt_measure_importance(e.bounds_min_energy, e.bounds_max_theta_o, e.cone_axis_theta_e, e.identity.y, point,
+                                 n, has_transmission);
+}
+
+// child_importance (:317-350)
+__device__ TreeImportance lt_child_importance(const SceneView &s, uint32_t index, f3 point, f3 n,
+                                              bool has_transmission) {
+    const TreeNode &node = s.tree_nodes[index];
+    TreeImportance r;
+    r.max = r.min = 0.0f;
+    if (node.topology.w == kTreeInner || node.emitters.y > 1u)
+        r = lt_measure_importance(node.bounds_min_energy, node.bounds_max_theta_o, node.cone_axis_theta_e,
+                                  node.emitters.z, point, n, has_transmission);
+    else if (node.emitters.y == 1u)
+        r = lt_emitter_importance(s, node.emitters.x, point, n, has_transmission);
+    return r;
+}
+
+// left_probability (:352-386)
+__device__ float lt_left_probability(const SceneView &s, uint32_t left, uint32_t right, f3 point, f3 n,
+                                      bool has_transmission, bool &usable) {
+    const TreeImportance li = lt_child_importance(s, left, point, n, has_transmission);
+    const TreeImportance ri = lt_child_importance(s, right, point, n, has_transmission);
+    const float max_total = li.max + ri.max;
+    const float min_total = li.min + ri.min;
+    const float max_p = li.max / fmaxf(max_total, LT_SAFE);
+    const float min_p = min_total > 0.0f ? li.min / fmaxf(min_total, LT_SAFE)
+                                         : 0.5f * ((li.max > 0.0f ? 1.0f : 0.0f) + (ri.max == 0.0f ? 1.0f : 0.0f));
+    usable = max_total > 0.0f;
+    return 0.5f * (max_p + min_p);
+}
+
+// select_leaf_emitter (:388-535) with reservoir_update (:388-416)
+__device__ void lt_select_leaf(const SceneView &s, const TreeNode &node, f3 point, f3 n, bool has_transmission,
+                               float &random, uint32_t &selected, float &pdf) {
+    float sel_max = 0.0f, sel_min = 0.0f, tot_max = 0.0f, tot_min = 0.0f;
+    uint32_t positive = 0u;
+    const bool sample_maximum = random > 0.5f;
+    const uint32_t first = node.emitters.x, count = node.emitters.y;
+    if (count > 1u) random = random * 2.0f - (sample_maximum ? 1.0f : 0.0f);
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t idx = first + i;
+        const TreeImportance imp = lt_emitter_importance(s, idx, point, n, has_transmission);
+        tot_max += imp.max;
+        tot_min += imp.min;
+        positive += imp.max > 0.0f ? 1u : 0u;
+        const float primary = sample_maximum ? imp.max : imp.min;
+        const float primary_total = sample_maximum ? tot_max : tot_min;
+        if (primary > 0.0f) {
+            if (selected == LT_INVALID) {
+                selected = idx;
+                sel_max = imp.max;
+                sel_min = imp.min;
+            } else {
+                const float threshold = primary / primary_total;
+                if (random <= threshold) {
+                    selected = idx;
+                    sel_max = imp.max;
+                    sel_min = imp.min;
+                    random = random / threshold;
+                } else {
+                    random = (random - threshold) / fmaxf(1.0f - threshold, LT_SAFE);
+                }
+                random = clampf(random, 0.0f, 1.0f);
+            }
+        }
+    }
+    if (positive > 0u && tot_min == 0.0f) {
+        if (sample_maximum) {
+            sel_min = 1.0f;
+            tot_min = (float)positive;
+        } else {
+            selected = LT_INVALID;
+            float sel_uniform = 0.0f, tot_uniform = 0.0f;
+            for (uint32_t i = 0; i < count; ++i) {
+                const uint32_t idx = first + i;
+                const TreeImportance imp = lt_emitter_importance(s, idx, point, n, has_transmission);
+                const float weight = imp.max > 0.0f ? 1.0f : 0.0f;
+                if (weight > 0.0f) {
+                    tot_uniform += weight;
+                    if (selected == LT_INVALID) {
+                        selected = idx;
+                        sel_uniform = weight;
+                    } else {
+                        const float threshold = weight / tot_uniform;
+                        if (random <= threshold) {
+                            selected = idx;
+                            sel_uniform = weight;
+                            random = random / threshold;
+                        } else {
+                            random = (random - threshold) / fmaxf(1.0f - threshold, LT_SAFE);
+                        }
+                        random = clampf(random, 0.0f, 1.0f);
+                    }
+                }
+                if (selected == idx) {
+                    sel_max = imp.max;
+                    sel_min = 1.0f;
+                }
+            }
+            tot_min = tot_uniform;
+        }
+    }
+    if (selected != LT_INVALID)
+        pdf *= 0.5f * (sel_max / fmaxf(tot_max, LT_SAFE) + sel_min / fmaxf(tot_min, LT_SAFE));
+}
+
+// make_sample_callable (:551-637): emitter id or LT_INVALID, and its pdf
+__device__ uint32_t lt_sample(const SceneView &s, float u, f3 point, f3 n, bool has_transmission, float &pdf) {
+    pdf = 0.0f;
+    if (s.tree_node_count == 0u || s.tree_emitter_count == 0u || s.tree_root >= s.tree_node_count)
+        return LT_INVALID;
+    float random = clampf(u, 0.0f, 1.0f);
+    float selection_pdf = 1.0f;
+    uint32_t index = s.tree_root, selected = LT_INVALID, steps = 0u;
+    bool active = true;
+    while (active && steps <= s.tree_node_count) {
+        steps += 1u;
+        const TreeNode &node = s.tree_nodes[index];
+        if (node.topology.w != kTreeInner) {
+            lt_select_leaf(s, node, point, n, has_transmission, random, selected, selection_pdf);
+            active = false;
+        } else {
+            bool usable;
+            const float p_left =
+                lt_left_probability(s, node.topology.y, node.topology.z, point, n, has_transmission, usable);
+            if (!usable) {
+                active = false;
+            } else {
+                const float p_right = 1.0f - p_left;
+                const bool choose_right = random <= p_right;
+                index = choose_right ? node.topology.z : node.topology.y;
+                selection_pdf *= choose_right ? p_right : p_left;
+                random = choose_right ? random / fmaxf(p_right, LT_SAFE) : (random - p_right) / fmaxf(p_left, LT_SAFE);
+                random = clampf(random, 0.0f, 1.0f);
+            }
+        }
+    }
+    if (selected == LT_INVALID) return LT_INVALID;
+    pdf = selection_pdf;
+    return s.tree_emitters[selected].identity.x;
+}
+
+// make_pdf_callable (:639-744)
+__device__ float lt_pdf(const SceneView &s, uint32_t emitter_id, f3 point, f3 n, bool has_transmission) {
+    if (s.tree_node_count == 0u || s.tree_emitter_count == 0u || emitter_id >= s.tree_emitter_count) return 0.0f;
+    const uint2 mapping = s.tree_mapping[emitter_id];
+    const uint32_t target = mapping.x;
+    uint32_t current = mapping.y;
+    if (target >= s.tree_emitter_count || current >= s.tree_node_count) return 0.0f;
+    const TreeNode &leaf = s.tree_nodes[current];
+    float tmax = 0.0f, tmin = 0.0f, tot_max = 0.0f, tot_min = 0.0f;
+    uint32_t positive = 0u;
+    for (uint32_t i = 0; i < leaf.emitters.y; ++i) {
+        const uint32_t idx = leaf.emitters.x + i;
+        const TreeImportance imp = lt_emitter_importance(s, idx, point, n, has_transmission);
+        tot_max += imp.max;
+        tot_min += imp.min;
+        positive += imp.max > 0.0f ? 1u : 0u;
+        if (idx == target) {
+            tmax = imp.max;
+            tmin = imp.min;
+        }
+    }
+    if (!(tmax > 0.0f)) return 0.0f;
+    float result = 0.5f * (tmax / fmaxf(tot_max, LT_SAFE) +
+                           (tot_min > 0.0f ? tmin / fmaxf(tot_min, LT_SAFE) : 1.0f / fmaxf((float)positive, 1.0f)));
+    uint32_t steps = 0u;
+    bool valid = true;
+    while (valid && current != s.tree_root && steps <= s.tree_node_count) {
+        steps += 1u;
+        const uint32_t parent_index = s.tree_nodes[current].topology.x;
+        if (parent_index >= s.tree_node_count) {
+            valid = false;
+            result = 0.0f;
+        } else {
+            const TreeNode &parent = s.tree_nodes[parent_index];
+            bool usable;
+            const float p_left =
+                lt_left_probability(s, parent.topology.y, parent.topology.z, point, n, has_transmission, usable);
+            if (!usable) {
+                valid = false;
+                result = 0.0f;
+            } else {
+                result *= current == parent.topology.y ? p_left : 1.0f - p_left;
+                current = parent_index;
+            }
+        }
+    }
+    if (current != s.tree_root) result = 0.0f;
+    return result;
+}
+
+// make_triangle_emitter_callable (:746-790)
+__device__ uint32_t lt_triangle_emitter(const SceneView &s, uint32_t object, uint32_t cprim) {
+    uint32_t first = 0u, length = s.emitter_count;
+    while (length > 0u) {
+        const uint32_t half = length >> 1u;
+        const uint32_t middle = first + half;
+        const uint32_t eo = s.emitters[middle].instance, ep = s.emitter_cprim[middle];
+        if (eo < object || (eo == object && ep < cprim)) {
+            first = middle + 1u;
+            length = length - half - 1u;
+        } else {
+            length = half;
+        }
+    }
+    if (first < s.emitter_count && s.emitters[first].instance == object && s.emitter_cprim[first] == cprim)
+        return first;
+    return LT_INVALID;
+}
+
+// ===========================================================================
 // per-material tables evaluated once per parameter set
 // ===========================================================================
 
@@ -1688,7 +1991,10 @@ __device__ bool path_step(const SceneView &s, const KernelParams &k, const float
             }
             if (found == 0xffffffffu) break;
             const Lamp &l = s.lamps[found];
-            const float light_pdf = best.pdf * s.light_selection_pdf;
+            const float selection_pdf =
+                k.use_tree ? lt_pdf(s, s.emitter_count + found, p.ray.o, p.mis_normal, had_transmission)
+                           : s.light_selection_pdf;
+            const float light_pdf = best.pdf * selection_pdf;
             const bool competing = p.ps.bounce > 0u && (p.bits & PATH_PREV_DELTA) == 0u;
             const float mis = forward_light_weight(k, p.prev_pdf, light_pdf, competing, light_pdf > 0.0f);
             const f3 light_radiance = lamp3(l.color) * (l.power * best.eval);
@@ -1734,7 +2040,10 @@ __device__ bool path_step(const SceneView &s, const KernelParams &k, const float
         const bool competing = p.ps.bounce > 0u && (p.bits & PATH_PREV_DELTA) == 0u;
         const f3 oriented = sf.back_facing ? neg(sf.ng) : sf.ng;
         const float area = 0.5f * length(cross(sf.wp1 - sf.wp0, sf.wp2 - sf.wp0));
-        const float selection_pdf = area * s.triangle_area_pdf;
+        float selection_pdf = area * s.triangle_area_pdf;
+        if (k.use_tree)
+            selection_pdf = lt_pdf(s, lt_triangle_emitter(s, sf.object, sf.cprim), p.ray.o, p.mis_normal,
+                                   (p.ps.flag & FLAG_MIS_HAD_TRANSMISSION) != 0u);
         float tpdf;
         const bool tvalid = triangle_pdf(p.ray.o, sf.position, sf.wp0, sf.wp1, sf.wp2, tpdf);
         const f3 dir = normalize_or(sf.position - p.ray.o, V(0.0f, 0.0f, 1.0f));
@@ -1785,21 +2094,38 @@ __device__ bool path_step(const SceneView &s, const KernelParams &k, const float
     // NEE: emissive mesh component (src/luisa/path_kernel_nee_emissive_mesh.cpp:30-298)
     if (k.nee && (rflags & RT_BSDF_HAS_EVAL) != 0u) {
         const float4 ul = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 1u);
-        // flat-CDF light choice (src/luisa/path_tracer_light_distribution.cpp:12-52)
-        uint32_t first = 0u, len = s.distribution_count + 1u;
-        while (len > 0u) {
-            const uint32_t half = len >> 1u;
-            const uint32_t mid = first + half;
-            if (ul.z < s.distribution[mid].cumulative) {
-                len = half;
+        uint32_t light;
+        DistEntry entry;
+        if (k.use_tree) {
+            // light tree (src/luisa/path_kernel_pipeline.cpp:140-150)
+            float tree_pdf;
+            light = lt_sample(s, ul.z, sf.position, sf.ns, (rflags & RT_BSDF_HAS_TRANSMISSION) != 0u, tree_pdf);
+            if (light != LT_INVALID && light < s.distribution_count) {
+                entry = s.distribution[light];
+                entry.selection_pdf = tree_pdf;
             } else {
-                first = mid + 1u;
-                len = len - half - 1u;
+                light = LT_INVALID;
+                entry.kind = 3u;  // sentinel: nothing sampled
+                entry.index = 0u;
+                entry.selection_pdf = entry.cumulative = 0.0f;
             }
+        } else {
+            // flat-CDF light choice (src/luisa/path_tracer_light_distribution.cpp:12-52)
+            uint32_t first = 0u, len = s.distribution_count + 1u;
+            while (len > 0u) {
+                const uint32_t half = len >> 1u;
+                const uint32_t mid = first + half;
+                if (ul.z < s.distribution[mid].cumulative) {
+                    len = half;
+                } else {
+                    first = mid + 1u;
+                    len = len - half - 1u;
+                }
+            }
+            light = first > 0u ? first - 1u : 0u;
+            if (light > s.distribution_count - 1u) light = s.distribution_count - 1u;
+            entry = s.distribution[light];
         }
-        uint32_t light = first > 0u ? first - 1u : 0u;
-        if (light > s.distribution_count - 1u) light = s.distribution_count - 1u;
-        const DistEntry entry = s.distribution[light];
         if (entry.kind == 0u) {
             if (TRACE && ev) {
                 ev[0] |= 16u;

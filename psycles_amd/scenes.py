@@ -571,3 +571,83 @@ def instanced_settings(width=1920, height=1080) -> abi.RenderSettings:
     s.seed = 0
     s.integrator.max_bounces = 8
     return s
+
+
+def many_lights(frequency: int = 224, instances: int = 64, emissive: int = 10000, points: int = 256,
+                spots: int = 64, box: float = 40.0) -> SceneSnapshot:
+    """The instanced icospheres plus emissive triangles and analytic lamps.
+
+    `emissive` random (instance, triangle) pairs (SplitMix64 seed 3) are
+    copied to world space, offset 1e-4 along their geometric normal into one
+    extra mesh and given add(emission, diffuse(0.6)) materials: strength is
+    log-uniform in [1, 100], bucketed into 16 log-spaced materials of one
+    structure.  `points` sphere point lamps, `spots` sphere spot lamps and one
+    distant sun get random positions, directions, sizes and colours
+    (SplitMix64 seed 33).  Meant to be rendered with the light tree."""
+    snap = instanced_icospheres(frequency, instances, box)
+    sphere = snap.meshes[0]
+    spheres = [i for i in snap.instances if i.mesh == 0]
+    rng = SplitMix64(3)
+    buckets = 16
+    first_material = len(snap.materials)
+    for b in range(buckets):
+        strength = 100.0 ** ((b + 0.5) / buckets)
+        snap.materials.append(emission_diffuse_material((1.0, 0.85, 0.7), strength, (0.6, 0.6, 0.6),
+                                                        name=f"glow{b}"))
+    pos = np.empty((emissive, 3, 3), dtype=np.float64)
+    slots = np.empty(emissive, dtype=np.uint32)
+    tri_count = sphere.triangles.shape[0]
+    for e in range(emissive):
+        inst = spheres[min(int(rng.uniform() * len(spheres)), len(spheres) - 1)]
+        tri = sphere.triangles[min(int(rng.uniform() * tri_count), tri_count - 1)]
+        m = inst.transform.astype(np.float64)
+        local = sphere.positions[tri].astype(np.float64)
+        world = local @ m[:3, :3].T + m[:3, 3]
+        n = np.cross(world[1] - world[0], world[2] - world[0])
+        n /= np.linalg.norm(n)
+        pos[e] = world + 1.0e-4 * n
+        u = rng.uniform()
+        slots[e] = min(int(u * buckets), buckets - 1)
+    glow = MeshData(pos.reshape(-1, 3).astype(np.float32),
+                    np.arange(3 * emissive, dtype=np.uint32).reshape(-1, 3),
+                    list(range(first_material, first_material + buckets)),
+                    triangle_material_slots=slots, name="glow")
+    snap.meshes.append(glow)
+    snap.instances.append(InstanceData(len(snap.meshes) - 1, identity(), name="glow"))
+    rng = SplitMix64(33)
+    half = box / 2.0
+
+    def colour():
+        return tuple(0.3 + 0.7 * rng.uniform() for _ in range(3))
+
+    lights = []
+    for _ in range(points):
+        p = [(rng.uniform() * 2.0 - 1.0) * half * 1.2 for _ in range(3)]
+        c = colour()
+        power = 50.0 * 10.0 ** rng.uniform()
+        size = 0.05 + 0.45 * rng.uniform()
+        lights.append(lamp(abi.LIGHT_POINT, translation(*p), c, power, size=size))
+    for _ in range(spots):
+        p = [(rng.uniform() * 2.0 - 1.0) * half, 5.0 + 20.0 * rng.uniform(), (rng.uniform() * 2.0 - 1.0) * half]
+        d = (rng.uniform() - 0.5, -1.0, rng.uniform() - 0.5)
+        c = colour()
+        power = 200.0 * 10.0 ** rng.uniform()
+        size = 0.05 + 0.25 * rng.uniform()
+        angle = 0.3 + 0.9 * rng.uniform()
+        smooth = 0.4 * rng.uniform()
+        # the spot shines along -z of its frame
+        lights.append(lamp(abi.LIGHT_SPOT, translation(*p) @ frame_z((-d[0], -d[1], -d[2])), c, power,
+                           size=size, spot_angle=angle, spot_smooth=smooth))
+    lights.append(lamp(abi.LIGHT_DISTANT, frame_z((0.3, 1.0, 0.2)), (1.0, 0.95, 0.85), 1.5, angle=0.05))
+    snap.lights = lights
+    snap.name = "many_lights"
+    return snap
+
+
+def many_light_settings(width=1920, height=1080) -> abi.RenderSettings:
+    """32 spp, max 6 bounces, seed 0, light-tree NEE."""
+    s = abi.default_settings(width, height)
+    s.seed = 0
+    s.integrator.max_bounces = 6
+    s.integrator.use_light_tree = 1
+    return s

@@ -96,6 +96,13 @@ def load():
     lib.oracle_primary_hits.argtypes = [vp, P(abi.RenderSettings), u32, u32, P(u32)]
     lib.oracle_trace_events.argtypes = [vp, P(abi.RenderSettings), u32, u32, P(u32)]
     lib.oracle_sampler_probe.argtypes = [P(abi.RenderSettings), u32, u32, u32, u32, P(u32)]
+    lib.oracle_light_tree_build.argtypes = [P(TreeEmitter), u32, u32, P(LightTree)]
+    lib.oracle_light_tree_query.argtypes = [vp, P(f32), P(f32), C.c_int, f32, P(u32), P(f32), P(f32)]
+    lib.oracle_light_tree_pdf.argtypes = [vp, u32, P(f32), P(f32), C.c_int]
+    lib.oracle_light_tree_pdf.restype = f32
+    lib.oracle_light_tree_emitters.argtypes = [vp]
+    lib.oracle_light_tree_emitters.restype = u32
+    lib.oracle_light_tree_free.argtypes = [P(LightTree)]
     lib.oracle_sample_lamp.argtypes = [P(abi.Light), P(f32), P(f32), C.c_int, P(f32), P(f32)]
     lib.oracle_intersect_lamp.argtypes = [P(abi.Light), P(f32), P(f32), f32, f32, P(f32), C.c_int,
                                           P(f32)]
@@ -162,6 +169,19 @@ class OracleScene:
         assert rc == 0
         return film, stats
 
+    def tree_query(self, point, normal, has_transmission, u):
+        eid, ps, pr = u32(), f32(), f32()
+        load().oracle_light_tree_query(self.handle, _vec(point), _vec(normal), int(has_transmission), u,
+                                         C.byref(eid), C.byref(ps), C.byref(pr))
+        return eid.value, ps.value, pr.value
+
+    def tree_pdf(self, emitter_id, point, normal, has_transmission):
+        return load().oracle_light_tree_pdf(self.handle, emitter_id, _vec(point), _vec(normal),
+                                              int(has_transmission))
+
+    def tree_emitters(self):
+        return load().oracle_light_tree_emitters(self.handle)
+
     def primary_hits(self, settings, sample_index, total):
         w = settings.window_width or settings.width
         h = settings.window_height or settings.height
@@ -205,6 +225,55 @@ def sampler_probe(settings, sample_index, total, first_dim, dim_count) -> np.nda
     return out
 
 
+class TreeEmitter(C.Structure):
+    _fields_ = [("bounds_min", f32 * 3), ("bounds_max", f32 * 3), ("bounds_empty", C.c_int),
+                ("axis", f32 * 3), ("theta_o", f32), ("theta_e", f32), ("orientation_empty", C.c_int),
+                ("energy", f32), ("centroid", f32 * 3), ("emitter_id", u32), ("distant", C.c_int)]
+
+
+class TreeNode(C.Structure):
+    _fields_ = [("measure", f32 * 12), ("flags", u32 * 2), ("parent", u32), ("left", u32),
+                ("right", u32), ("first", u32), ("count", u32), ("kind", u32)]
+
+
+class TreeRecord(C.Structure):
+    _fields_ = [("measure", f32 * 12), ("flags", u32 * 2), ("centroid", f32 * 3), ("emitter_id", u32),
+                ("distant", u32)]
+
+
+class LightTree(C.Structure):
+    _fields_ = [("nodes", P(TreeNode)), ("emitters", P(TreeRecord)), ("emitter_to_tree", P(u32)),
+                ("emitter_to_leaf", P(u32)), ("node_count", u32), ("emitter_count", u32), ("root", u32)]
+
+
+def tree_arrays(tree: LightTree) -> dict:
+    """Raw words of a built tree (nodes, emitters, mappings, root)."""
+    nodes = np.ctypeslib.as_array(C.cast(tree.nodes, P(u32)), shape=(tree.node_count * 20,)).copy() \
+        if tree.node_count else np.zeros(0, np.uint32)
+    emitters = np.ctypeslib.as_array(C.cast(tree.emitters, P(u32)), shape=(tree.emitter_count * 19,)).copy() \
+        if tree.emitter_count else np.zeros(0, np.uint32)
+    to_tree = np.ctypeslib.as_array(tree.emitter_to_tree, shape=(tree.emitter_count,)).copy() \
+        if tree.emitter_count else np.zeros(0, np.uint32)
+    to_leaf = np.ctypeslib.as_array(tree.emitter_to_leaf, shape=(tree.emitter_count,)).copy() \
+        if tree.emitter_count else np.zeros(0, np.uint32)
+    return {"nodes": nodes, "emitters": emitters, "to_tree": to_tree, "to_leaf": to_leaf, "root": tree.root}
+
+
+def light_tree(emitters, max_leaf=8, library=None):
+    """Build with the oracle (or, given `library`, the reference); returns
+    tree_arrays or None when the build rejects the input."""
+    lib = library or load()
+    arr = (TreeEmitter * max(len(emitters), 1))(*emitters)
+    tree = LightTree()
+    build = lib.ref_light_tree_build if library else lib.oracle_light_tree_build
+    free = lib.ref_light_tree_free if library else lib.oracle_light_tree_free
+    if build(arr, len(emitters), max_leaf, C.byref(tree)) != 0:
+        return None
+    out = tree_arrays(tree)
+    free(C.byref(tree))
+    return out
+
+
 def ref_library():
     """The reference's own sampling sources compiled in place (may be absent)."""
     if not REF_LIB.exists():
@@ -216,6 +285,9 @@ def ref_library():
     lib.ref_pixel_filter_table.argtypes = [u32, f32, P(f32)]
     lib.ref_light_distribution.argtypes = [P(f32), u32, u32, C.c_int, P(u32), P(f32), P(f32)]
     lib.ref_light_distribution.restype = u32
+    if hasattr(lib, "ref_light_tree_build"):
+        lib.ref_light_tree_build.argtypes = [P(TreeEmitter), u32, u32, P(LightTree)]
+        lib.ref_light_tree_free.argtypes = [P(LightTree)]
     return lib
 
 
