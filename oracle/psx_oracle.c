@@ -1901,6 +1901,25 @@ uint32_t oracle_scene_emissive_triangles(const oracle_scene *s) {
     return s ? s->emitter_count : 0u;
 }
 
+uint32_t oracle_scene_distribution(const oracle_scene *s, float *cumulative, float *selection_pdf, uint32_t *kind,
+                                   uint32_t *index, uint32_t capacity) {
+    if (!s) return 0u;
+    const uint32_t n = s->distribution_count;
+    for (uint32_t i = 0; i < n && i < capacity; ++i) {
+        cumulative[i] = s->distribution[i].cumulative;
+        selection_pdf[i] = s->distribution[i].selection_pdf;
+        kind[i] = s->distribution[i].kind;
+        index[i] = s->distribution[i].index;
+    }
+    return n;
+}
+
+int oracle_scene_light_tree(const oracle_scene *s, oracle_light_tree *view) {
+    if (!s || !view) return -1;
+    *view = s->tree;
+    return s->tree.node_count > 0u ? 0 : 1;
+}
+
 /* ======================================================================= */
 /* closest hit (src/luisa/path_kernel_scene_traversal.cpp:34-68, 212-493)  */
 /* ======================================================================= */
@@ -3134,6 +3153,24 @@ float oracle_light_tree_pdf(const oracle_scene *s, uint32_t emitter_id, const fl
 }
 
 uint32_t oracle_light_tree_emitters(const oracle_scene *s) { return s ? s->tree.emitter_count : 0u; }
+
+int oracle_light_tree_sample(const oracle_light_tree *tree, const float point[3], const float normal[3],
+                             int has_transmission, float u, uint32_t *emitter_id, float *pdf_sample,
+                             float *pdf_reverse) {
+    if (!tree) return -1;
+    const v3 p = load3(point), n = load3(normal);
+    float pdf = 0.0f;
+    const uint32_t id = lt_sample(tree, u, p, n, has_transmission, &pdf);
+    *emitter_id = id;
+    *pdf_sample = pdf;
+    *pdf_reverse = id == LT_INVALID ? 0.0f : lt_pdf(tree, id, p, n, has_transmission);
+    return 0;
+}
+
+float oracle_light_tree_reverse_pdf(const oracle_light_tree *tree, uint32_t emitter_id, const float point[3],
+                                    const float normal[3], int has_transmission) {
+    return tree ? lt_pdf(tree, emitter_id, load3(point), load3(normal), has_transmission) : 0.0f;
+}
 
 /* ======================================================================= */
 /* the path integrator (src/luisa/path_kernel_pipeline.cpp:73-185 and the  */

@@ -163,6 +163,13 @@ float oracle_light_tree_pdf(const oracle_scene *scene, uint32_t emitter_id,
                             const float point[3], const float normal[3],
                             int has_transmission);
 uint32_t oracle_light_tree_emitters(const oracle_scene *scene);
+/* The same descent and reverse pdf on a tree built by oracle_light_tree_build
+ * (emitter ids as given to the build). */
+int oracle_light_tree_sample(const oracle_light_tree *tree, const float point[3], const float normal[3],
+                             int has_transmission, float u, uint32_t *emitter_id, float *pdf_sample,
+                             float *pdf_reverse);
+float oracle_light_tree_reverse_pdf(const oracle_light_tree *tree, uint32_t emitter_id, const float point[3],
+                                    const float normal[3], int has_transmission);
 
 /* Build the oracle's own scene tables (binary SAH BVH with 16 bins and
  * leaves of at most 4 triangles, light distribution).  `brute_force`
@@ -172,6 +179,13 @@ void oracle_scene_destroy(oracle_scene *scene);
 int oracle_scene_update_parameters(oracle_scene *scene, uint32_t material,
                                    const float *parameters, uint32_t count);
 uint32_t oracle_scene_emissive_triangles(const oracle_scene *scene);
+/* Flat light distribution entries (count returned; at most `capacity`
+ * written); kind 0 triangle, 1 lamp. */
+uint32_t oracle_scene_distribution(const oracle_scene *scene, float *cumulative, float *selection_pdf,
+                                   uint32_t *kind, uint32_t *index, uint32_t capacity);
+/* A non-owning view of the scene's light tree: 0 when a usable tree exists,
+ * 1 when the scene has none.  Never pass the view to oracle_light_tree_free. */
+int oracle_scene_light_tree(const oracle_scene *scene, oracle_light_tree *view);
 
 /* Accumulate samples [first, first + count) of a `total`-sample render into
  * the window film (row-major, same layout as psx_session_read_film).  The

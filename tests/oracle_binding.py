@@ -103,6 +103,12 @@ def load():
     lib.oracle_light_tree_emitters.argtypes = [vp]
     lib.oracle_light_tree_emitters.restype = u32
     lib.oracle_light_tree_free.argtypes = [P(LightTree)]
+    lib.oracle_scene_distribution.argtypes = [vp, P(f32), P(f32), P(u32), P(u32), u32]
+    lib.oracle_scene_distribution.restype = u32
+    lib.oracle_scene_light_tree.argtypes = [vp, P(LightTree)]
+    lib.oracle_light_tree_sample.argtypes = [P(LightTree), P(f32), P(f32), C.c_int, f32, P(u32), P(f32), P(f32)]
+    lib.oracle_light_tree_reverse_pdf.argtypes = [P(LightTree), u32, P(f32), P(f32), C.c_int]
+    lib.oracle_light_tree_reverse_pdf.restype = f32
     lib.oracle_sample_lamp.argtypes = [P(abi.Light), P(f32), P(f32), C.c_int, P(f32), P(f32)]
     lib.oracle_intersect_lamp.argtypes = [P(abi.Light), P(f32), P(f32), f32, f32, P(f32), C.c_int,
                                           P(f32)]
@@ -181,6 +187,21 @@ class OracleScene:
 
     def tree_emitters(self):
         return load().oracle_light_tree_emitters(self.handle)
+
+    def distribution(self):
+        """(cumulative, selection_pdf, kind, index) arrays of the flat CDF."""
+        n = load().oracle_scene_distribution(self.handle, None, None, None, None, 0)
+        c, p = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        k, i = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        load().oracle_scene_distribution(self.handle, fptr(c), fptr(p), uptr(k), uptr(i), n)
+        return c, p, k, i
+
+    def light_tree(self):
+        """tree_arrays of the scene's light tree, or None."""
+        view = LightTree()
+        if load().oracle_scene_light_tree(self.handle, C.byref(view)) != 0:
+            return None
+        return tree_arrays(view)
 
     def primary_hits(self, settings, sample_index, total):
         w = settings.window_width or settings.width
@@ -272,6 +293,37 @@ def light_tree(emitters, max_leaf=8, library=None):
     out = tree_arrays(tree)
     free(C.byref(tree))
     return out
+
+
+class BuiltTree:
+    """An oracle tree kept alive for descent and reverse-pdf queries."""
+
+    def __init__(self, emitters, max_leaf=8):
+        self._arr = (TreeEmitter * max(len(emitters), 1))(*emitters)
+        self.tree = LightTree()
+        if load().oracle_light_tree_build(self._arr, len(emitters), max_leaf, C.byref(self.tree)) != 0:
+            raise ValueError("light tree build rejected the emitters")
+
+    def sample(self, point, normal, has_transmission, u):
+        eid, ps, pr = u32(), f32(), f32()
+        load().oracle_light_tree_sample(C.byref(self.tree), _vec(point), _vec(normal), int(has_transmission), u,
+                                        C.byref(eid), C.byref(ps), C.byref(pr))
+        return eid.value, ps.value, pr.value
+
+    def pdf(self, emitter_id, point, normal, has_transmission):
+        return load().oracle_light_tree_reverse_pdf(C.byref(self.tree), emitter_id, _vec(point), _vec(normal),
+                                                    int(has_transmission))
+
+    def close(self):
+        if self.tree.nodes or self.tree.emitters:
+            load().oracle_light_tree_free(C.byref(self.tree))
+            self.tree = LightTree()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ref_library():

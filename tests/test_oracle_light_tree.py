@@ -116,3 +116,35 @@ def test_matches_reference_build(n, leaf):
     assert mine["root"] == theirs["root"]
     for key in ("nodes", "emitters", "to_tree", "to_leaf"):
         assert np.array_equal(mine[key], theirs[key]), key
+
+
+def test_scene_upload_contract():
+    """tests/test_luisa_light_tree.cpp:49-161: one front-sampled emissive
+    triangle (0,0,3) (2,0,3) (0,2,3) whose instance moves it +10 in x and
+    overrides its material with emission (0.25, 0.5, 1) x 4, plus one point
+    lamp.  The tree is usable, the flat distribution has two entries of
+    selection pdf 0.5, and the triangle's tree record carries world bounds
+    x in [10, 12], z = 3 and energy area x mean emission = 2 x 7/3."""
+    from psycles_amd import abi, scenes
+    b = scenes.ProgramBuilder()
+    base = b.build(b.emission((1.0, 1.0, 1.0), 0.25), emission_sampling=abi.EMISSION_SAMPLING_FRONT)
+    b = scenes.ProgramBuilder()
+    override = b.build(b.emission((0.25, 0.5, 1.0), 4.0), emission_sampling=abi.EMISSION_SAMPLING_FRONT)
+    mesh = scenes.MeshData(np.array([[0, 0, 3], [2, 0, 3], [0, 2, 3]], np.float32), np.array([[0, 1, 2]], np.uint32),
+                           [0])
+    inst = scenes.InstanceData(0, scenes.translation(10.0, 0.0, 0.0), material_overrides=[1])
+    cam = scenes.CameraData(scenes.translation(0.0, 0.0, 10.0))
+    snap = scenes.SceneSnapshot([base, override], [mesh], [inst], cam,
+                                lights=[scenes.lamp(abi.LIGHT_POINT, scenes.translation(20.0, 0.0, 2.0))])
+    osc = ob.OracleScene(snap)
+    _, pdf, kind, _ = osc.distribution()
+    assert list(kind) == [0, 1]
+    assert np.allclose(pdf, [0.5, 0.5], atol=1e-6)
+    tree = osc.light_tree()
+    assert tree is not None and osc.tree_emitters() == 2
+    records = tree["emitters"].reshape(-1, 19)
+    tri = records[records[:, 17] == 0][0]
+    m = tri[:12].view(np.float32)
+    assert abs(m[0] - 10.0) <= 1e-6 and abs(m[2] - 3.0) <= 1e-6
+    assert abs(m[3] - 12.0) <= 1e-6 and abs(m[5] - 3.0) <= 1e-6
+    assert abs(m[11] - 2.0 * (1.0 + 2.0 + 4.0) / 3.0) <= 1e-5
