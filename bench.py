@@ -9,6 +9,10 @@ The timed path is the batched wavefront integrator (trace and shade kernels
 over path state in HBM, 8 samples of every pixel per dispatch).  The plain
 megakernel (one lane runs whole paths) is timed on one frame beside it.
 
+`--workload many_lights` times the many-light scene instead (the instanced
+scene plus 10,000 emissive triangles and 321 lamps, light-tree NEE,
+1920x1080 at 32 spp, max 6 bounces); the default line is the instanced one.
+
 Prints one JSON line with samples/s (value), Mrays/s, the HBM roofline of the
 dominant kernel (the wavefront trace kernel, timed with HIP events around
 each of its launches on the session stream) and the CPU oracle timed on a
@@ -28,45 +32,68 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s & samples/s (batched vs plain) at 1 GPU; p50/p99 request latency"
 
-# Algorithmic bytes per ray, B_ray = 48 + 32 * N_node + 36 * N_tri, with the
-# mean binary-SAH node visits and triangle tests per ray measured by the CPU
-# oracle's BVH (16 bins, leaves <= 4) over the full frame of this workload.
-ORACLE_NODES_PER_RAY = 46.83
-ORACLE_TRIS_PER_RAY = 8.59
-BYTES_PER_RAY = 48.0 + 32.0 * ORACLE_NODES_PER_RAY + 36.0 * ORACLE_TRIS_PER_RAY
 FILM_BYTES_PER_PIXEL_DISPATCH = 104.0
 HBM_PEAK_GBS = 8000.0
-# HBM bytes the trace kernel moves per ray on this workload: rocprofv3
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale factors from a 1 GiB copy),
-# divided by the rays of the profiled dispatch
-# (profiles/wf_trace_kernel_instanced_pmc.md)
-TRACE_TRAFFIC_BYTES_PER_RAY = 1091.0
 SAMPLES_PER_DISPATCH = 8
 
 
-def cpu_baseline(snap, settings, seconds_target=15.0):
+def bytes_per_ray(nodes: float, tris: float) -> float:
+    """Algorithmic bytes per ray, B_ray = 48 + 32 * N_node + 36 * N_tri: ray in
+    and hit out, one binary node, one triangle."""
+    return 48.0 + 32.0 * nodes + 36.0 * tris
+
+
+# Per workload: the scene, its settings, and the mean binary-SAH node visits
+# and triangle tests per ray measured by the CPU oracle's BVH (16 bins,
+# leaves <= 4) over one full-frame sample of that workload.  `traffic` is the
+# HBM bytes the trace kernel moves per ray: rocprofv3 FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0 (scale factors from a 1 GiB copy) divided by the rays of
+# the profiled dispatch (profiles/wf_trace_kernel_instanced_pmc.md), or None
+# where no counter profile was taken.
+WORKLOADS = {
+    "instanced": {
+        "scene": "instanced_icospheres", "settings": "instanced_settings", "spp": 64,
+        "nodes": 46.83, "tris": 8.59, "traffic": 1091.0,
+        "describe": "instanced icospheres: 1,003,520-tri displaced icosphere x 64 instances, "
+                    "mix(diffuse, transparent) materials, ground quad, constant background; "
+                    "1920x1080, {spp} spp, max 8 bounces, seed 0",
+    },
+    "many_lights": {
+        "scene": "many_lights", "settings": "many_light_settings", "spp": 32,
+        "nodes": 94.81, "tris": 15.06, "traffic": None,
+        "describe": "many lights: the instanced icospheres plus 10,000 emissive add(emission, diffuse) "
+                    "triangles, 256 point, 64 spot and 1 distant lamp, light-tree NEE; "
+                    "1920x1080, {spp} spp, max 6 bounces, seed 0",
+    },
+}
+
+
+def cpu_baseline(snap, settings, total, seconds_target=15.0):
     """Time the CPU oracle on the full frame of the same workload: a 1 spp
-    calibration pass, then as many consecutive samples as fill the target
-    duration, on the host cores."""
+    calibration pass (which also yields the oracle BVH's node visits and
+    triangle tests per ray behind B_ray), then as many consecutive samples as
+    fill the target duration, on the host cores."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_binding as ob
 
     threads = min(os.cpu_count() or 1, 16)
     scene = ob.OracleScene(snap)
     t0 = time.perf_counter()
-    scene.render(settings, 0, 1, 64, threads=threads)
+    _, st0 = scene.render(settings, 0, 1, total, threads=threads)
     dt = max(time.perf_counter() - t0, 1e-3)
-    spp = int(max(1, min(63, round(seconds_target / dt))))
+    spp = int(max(1, min(total - 1, round(seconds_target / dt))))
     t0 = time.perf_counter()
-    _, st = scene.render(settings, 1, spp, 64, threads=threads)
+    _, st = scene.render(settings, 1, spp, total, threads=threads)
     dt = time.perf_counter() - t0
     return {
         "value": st.samples / dt,
         "unit": "samples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"full 1920x1080 frame, samples 1-{spp} of 64 ({st.samples} samples) in "
-                  f"{dt:.1f} s, {st.rays / max(dt, 1e-9) / 1e6:.2f} Mrays/s",
+        "sample": f"full {settings.width}x{settings.height} frame, samples 1-{spp} of {total} "
+                  f"({st.samples} samples) in {dt:.1f} s, {st.rays / max(dt, 1e-9) / 1e6:.2f} Mrays/s",
+        "oracle_nodes_per_ray": st0.node_visits / max(st0.rays, 1),
+        "oracle_tris_per_ray": st0.triangle_tests / max(st0.rays, 1),
     }
 
 
@@ -96,7 +123,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="instanced")
+    ap.add_argument("--spp", type=int, default=None, help="samples per pixel (default: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the megakernel frame")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -110,9 +138,11 @@ def main() -> int:
     from psycles_amd import abi, scenes
     from psycles_amd.render import RendererBackend
 
-    print("bench: building instanced scene (1,003,520 tris x 64 instances)", file=sys.stderr, flush=True)
-    snap = scenes.instanced_icospheres()
-    settings = scenes.instanced_settings()
+    w = WORKLOADS[args.workload]
+    print(f"bench: building the {args.workload} scene", file=sys.stderr, flush=True)
+    snap = getattr(scenes, w["scene"])()
+    settings = getattr(scenes, w["settings"])()
+    b_ray = bytes_per_ray(w["nodes"], w["tris"])
     backend = RendererBackend(0, integrator=abi.INTEGRATOR_WAVEFRONT,
                               max_samples_per_dispatch=SAMPLES_PER_DISPATCH)
     t0 = time.perf_counter()
@@ -122,7 +152,7 @@ def main() -> int:
         return 1
     compile_s = time.perf_counter() - t0
     print(f"bench: scene compiled in {compile_s:.2f} s", file=sys.stderr, flush=True)
-    spp = args.spp
+    spp = args.spp or w["spp"]
     pixels = settings.width * settings.height
 
     session = backend.create_session(comp.scene, settings)
@@ -154,7 +184,7 @@ def main() -> int:
     mrays = rays / wall / 1e6
     trace_launch_ms = trace_ms / max(trace_launches, 1)
     rays_per_launch = rays / max(trace_launches, 1)
-    achieved = rays_per_launch * BYTES_PER_RAY / (trace_launch_ms * 1e-3) / 1e9
+    achieved = rays_per_launch * b_ray / (trace_launch_ms * 1e-3) / 1e9
     result = {
         "metric": METRIC,
         "value": value,
@@ -169,9 +199,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "instanced icospheres: 1,003,520-tri displaced icosphere x 64 instances, "
-                        "mix(diffuse, transparent) materials, ground quad, constant background; "
-                        f"1920x1080, {spp} spp, max 8 bounces, seed 0",
+            "workload": w["describe"].format(spp=spp),
             "integrator": "wavefront (batched)",
             "samples_per_dispatch": SAMPLES_PER_DISPATCH,
         },
@@ -183,11 +211,11 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": rays_per_launch * TRACE_TRAFFIC_BYTES_PER_RAY,
+            "traffic": rays_per_launch * w["traffic"] if w["traffic"] else None,
             "kernel": "wf_trace_kernel",
             "launch_ms": trace_launch_ms,
             "launches": trace_launches,
-            "bytes_per_ray": BYTES_PER_RAY,
+            "bytes_per_ray": b_ray,
         },
     }
     if not args.no_plain:
@@ -201,7 +229,7 @@ def main() -> int:
         pwall = time.perf_counter() - t1
         film_bytes = pixels * FILM_BYTES_PER_PIXEL_DISPATCH
         pl_ms = pst.kernel_ms / max(pst.dispatches, 1)
-        pl_achieved = (pst.rays / max(pst.dispatches, 1) * BYTES_PER_RAY + film_bytes) / (pl_ms * 1e-3) / 1e9
+        pl_achieved = (pst.rays / max(pst.dispatches, 1) * b_ray + film_bytes) / (pl_ms * 1e-3) / 1e9
         result["plain"] = {
             "integrator": "megakernel",
             "samples_per_s": pixels * spp / pwall,
@@ -215,7 +243,7 @@ def main() -> int:
         del psess, pcomp, plain
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle baseline", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline(snap, settings)
+        result["cpu_baseline"] = cpu_baseline(snap, settings, spp)
     print(json.dumps(result))
     if args.calibrate_copy:
         calibration_copy()

@@ -140,6 +140,9 @@ struct Lamp {
     uint32_t pad;
 };
 constexpr uint32_t kLampNormalize = 1u, kLampSphere = 4u, kLampUseMis = 8u, kLampForward = 32u;
+// lamp_bits: visibility mask in bits 0-7, forward-intersectable (not distant,
+// radius > 0) in bit 8, effective MIS in bit 9
+constexpr uint32_t kLampBitIntersectable = 256u, kLampBitMis = 512u;
 constexpr uint32_t kLampPoint = 0u, kLampSpot = 1u, kLampDistant = 3u;
 
 // Light tree records (LightTreeNodeGpu / LightTreeEmitterGpu,
@@ -148,10 +151,13 @@ constexpr uint32_t kLampPoint = 0u, kLampSpot = 1u, kLampDistant = 3u;
 constexpr uint32_t kTreeHasBounds = 1u, kTreeHasOrientation = 2u, kTreeDistant = 4u;
 constexpr uint32_t kTreeInner = 0u, kTreeLeaf = 1u, kTreeDistantNode = 2u;
 
+// `trig` holds cos(theta_o), sin(theta_o), cos(theta_o + theta_e) and
+// cos(theta_e - theta_o) of the measure, evaluated once on the host.
 struct TreeNode {
     float4 bounds_min_energy;
     float4 bounds_max_theta_o;
     float4 cone_axis_theta_e;
+    float4 trig;
     uint4 topology;  // parent, left, right, kind
     uint4 emitters;  // first, count, measure flags, 0
 };
@@ -160,6 +166,7 @@ struct TreeEmitter {
     float4 bounds_min_energy;
     float4 bounds_max_theta_o;
     float4 cone_axis_theta_e;
+    float4 trig;
     uint4 identity;  // emitter id, measure flags, 0, 0
 };
 
@@ -193,6 +200,8 @@ struct SceneView {
     const float4 *material_shadow;    // per material: transparent extinction for shadow rays
     const float4 *material_emission;  // per material: constant emission
     const Lamp *lamps;
+    const float4 *lamp_sphere;        // per lamp: centre, radius (forward-hit culling)
+    const uint32_t *lamp_bits;
     uint32_t lamp_count;
     float light_selection_pdf;        // flat-CDF probability of one lamp
     const TreeNode *tree_nodes;

@@ -437,7 +437,8 @@ struct Scene {
     std::vector<uint32_t> mesh_prim_offsets;
     DeviceBuffer tree_nodes, tree_emitters, tree_mapping, emitter_cprim;
     DeviceBuffer nodes, tris, instances_buf, meshes_buf, positions, indices, normals, smooth, slots, tri_slots,
-        overrides_buf, materials_buf, leaves, params, emitters, distribution, material_shadow, material_emission, lamps;
+        overrides_buf, materials_buf, leaves, params, emitters, distribution, material_shadow, material_emission, lamps,
+        lamp_spheres, lamp_bits;
     SceneView view{};
     uint64_t triangles = 0;
 
@@ -1312,6 +1313,20 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     v.world_visibility = d->world_visibility_mask;
     s.lamps.upload(s.lamp_list);
     v.lamps = s.lamps.as<Lamp>();
+    {
+        std::vector<float4> spheres;
+        std::vector<uint32_t> bits;
+        for (const Lamp &l : s.lamp_list) {
+            spheres.push_back(make_float4(l.position[0], l.position[1], l.position[2], l.radius));
+            const bool intersectable = l.type != kLampDistant && l.radius > 0.0f;
+            bits.push_back((l.visibility & 0xffu) | (intersectable ? kLampBitIntersectable : 0u) |
+                           ((l.flags & kLampUseMis) != 0u ? kLampBitMis : 0u));
+        }
+        s.lamp_spheres.upload(spheres);
+        s.lamp_bits.upload(bits);
+        v.lamp_sphere = s.lamp_spheres.as<float4>();
+        v.lamp_bits = s.lamp_bits.as<uint32_t>();
+    }
     v.lamp_count = static_cast<uint32_t>(s.lamp_list.size());
     s.rebuild_emitters();
     s.refresh_material_tables();

@@ -22,9 +22,9 @@ struct PathResult {
 // Runs one path to completion.  A single traversal call site serves the
 // continuation ray and every shadow query so the loop inlines once.
 template <bool COUNT, bool TRACE>
-__device__ PathResult integrate(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
-                                uint32_t fx, uint32_t fy, uint32_t sample, Stack &st, uint32_t &rays, uint32_t &nodes,
-                                uint32_t &tris, uint32_t *trace) {
+__device__ PathResult integrate(const SceneView &s, const LampView &lv, const KernelParams &k, const float4 *sobol,
+                                const float *filter, uint32_t fx, uint32_t fy, uint32_t sample, Stack &st,
+                                uint32_t &rays, uint32_t &nodes, uint32_t &tris, uint32_t *trace) {
     PathVars p;
     init_path(k, sobol, filter, fx, fy, sample, p);
     ShadowRay sr;
@@ -45,7 +45,7 @@ __device__ PathResult integrate(const SceneView &s, const KernelParams &k, const
             continue;
         }
         uint32_t *ev = (TRACE && p.step < 8u) ? trace + p.step * 8u : nullptr;
-        cont = path_step<TRACE>(s, k, sobol, p, hit, sr, ev);
+        cont = path_step<TRACE>(s, lv, k, sobol, p, hit, sr, ev);
         if (sr.active) {
             shadow = true;
             transmittance = V(1.0f, 1.0f, 1.0f);
@@ -97,6 +97,9 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView s, KernelParam
                                                          uint32_t count, Counters *counters, uint32_t *spill,
                                                          uint32_t *cursor) {
     __shared__ uint32_t lds[kShortStack * kBlock];
+    __shared__ float4 lamp_sphere[kLampCache];
+    __shared__ uint32_t lamp_bits[kLampCache];
+    const LampView lv = stage_lamps(s, lamp_sphere, lamp_bits);
     Stack st = block_stack(lds, spill);
     uint32_t rays = 0u, nodes = 0u, tris = 0u;
     const uint32_t tiles = tile_count(k);
@@ -110,8 +113,8 @@ __global__ __launch_bounds__(kBlock) void render_kernel(SceneView s, KernelParam
         float4 a = reinterpret_cast<float4 *>(film.albedo)[index];
         uint32_t completed = film.sample_count[index];
         for (uint32_t i = 0; i < count; ++i) {
-            const PathResult r = integrate<COUNT, false>(s, k, sobol, filter, px + k.window_x, py + k.window_y,
-                                                         first + i, st, rays, nodes, tris, nullptr);
+            const PathResult r = integrate<COUNT, false>(s, lv, k, sobol, filter, px + k.window_x,
+                                                         py + k.window_y, first + i, st, rays, nodes, tris, nullptr);
             f3 rad = r.radiance;
             if (any_nan(rad)) rad = V(0.0f, 0.0f, 0.0f);
             c.x += rad.x;
@@ -145,6 +148,9 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(SceneView s, KernelParams
                                                         const float *filter, uint32_t sample, uint32_t *records,
                                                         uint32_t *spill) {
     __shared__ uint32_t lds[kShortStack * kBlock];
+    __shared__ float4 lamp_sphere[kLampCache];
+    __shared__ uint32_t lamp_bits[kLampCache];
+    const LampView lv = stage_lamps(s, lamp_sphere, lamp_bits);
     Stack st = block_stack(lds, spill);
     const uint32_t tiles = tile_count(k);
     const uint32_t waves = gridDim.x * (kBlock / 64u);
@@ -154,8 +160,8 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(SceneView s, KernelParams
         uint32_t *rec = records + ((size_t)py * k.window_width + px) * 64u;
         for (int i = 0; i < 64; ++i) rec[i] = 0u;
         uint32_t rays = 0u, nodes = 0u, tris = 0u;
-        integrate<false, true>(s, k, sobol, filter, px + k.window_x, py + k.window_y, sample, st, rays, nodes, tris,
-                               rec);
+        integrate<false, true>(s, lv, k, sobol, filter, px + k.window_x, py + k.window_y, sample, st, rays, nodes,
+                               tris, rec);
     }
 }
 

@@ -242,10 +242,13 @@ uint32_t measure_flags(const Measure &m, bool distant) {
            (distant ? kTreeDistant : 0u);
 }
 
-void pack(const Measure &m, float4 &min_energy, float4 &max_theta_o, float4 &axis_theta_e) {
+void pack(const Measure &m, float4 &min_energy, float4 &max_theta_o, float4 &axis_theta_e, float4 &trig) {
     min_energy = make_float4(m.box.lo.x, m.box.lo.y, m.box.lo.z, m.energy);
     max_theta_o = make_float4(m.box.hi.x, m.box.hi.y, m.box.hi.z, m.cone.theta_o);
     axis_theta_e = make_float4(m.cone.axis.x, m.cone.axis.y, m.cone.axis.z, m.cone.theta_e);
+    // the angle terms of the importance bound depend on the measure only
+    const float o = m.cone.theta_o, e = m.cone.theta_e;
+    trig = make_float4(::cosf(o), ::sinf(o), ::cosf(o + e), ::cosf(e - o));
 }
 
 }  // namespace
@@ -298,7 +301,7 @@ bool build_light_tree(const std::vector<TreeEmitterIn> &emitters, uint32_t max_l
     for (size_t i = 0; i < b.nodes.size(); ++i) {
         const BuildNode &n = b.nodes[i];
         TreeNode &g = out.nodes[i];
-        pack(n.m, g.bounds_min_energy, g.bounds_max_theta_o, g.cone_axis_theta_e);
+        pack(n.m, g.bounds_min_energy, g.bounds_max_theta_o, g.cone_axis_theta_e, g.trig);
         g.topology = make_uint4(n.parent, n.left, n.right, n.kind);
         g.emitters = make_uint4(n.first, n.count, measure_flags(n.m, n.kind == kTreeDistantNode), 0u);
     }
@@ -306,7 +309,7 @@ bool build_light_tree(const std::vector<TreeEmitterIn> &emitters, uint32_t max_l
     for (uint32_t i = 0; i < count; ++i) {
         const Item &it = b.items[i];
         TreeEmitter &g = out.emitters[i];
-        pack(it.m, g.bounds_min_energy, g.bounds_max_theta_o, g.cone_axis_theta_e);
+        pack(it.m, g.bounds_min_energy, g.bounds_max_theta_o, g.cone_axis_theta_e, g.trig);
         g.identity = make_uint4(it.id, measure_flags(it.m, it.distant), 0u, 0u);
     }
     out.mapping.resize(count);

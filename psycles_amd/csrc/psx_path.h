@@ -1474,48 +1474,98 @@ __device__ __forceinline__ LampSample sample_distant_light(const Lamp &l, float 
     return s;
 }
 
-struct LampHit {
-    bool valid;
-    float distance, pdf, eval;
-};
-
-// intersect_point / intersect_spot (analytic_light_intersection.h:33-260)
-__device__ LampHit intersect_lamp(const Lamp &l, f3 ro, f3 rd, float tmin, float tmax, f3 previous_normal,
-                                  bool had_transmission) {
+// intersect_point / intersect_spot (analytic_light_intersection.h:33-260),
+// split in two: lamp_candidate decides whether the ray hits the lamp nearer
+// than `tmax` and with non-zero emission (the part every lamp of the forward
+// scan needs); lamp_forward_pdf evaluates the sampling pdf for the one lamp
+// the scan keeps.  Both compute the same expressions, in the same order, as
+// the single-pass intersection.
+__device__ __forceinline__ bool lamp_candidate(const Lamp &l, f3 ro, f3 rd, float tmin, float tmax, float &distance,
+                                               float &eval) {
+    if (!(l.radius > 0.0f)) return false;
     const bool spot = l.type == kLampSpot;
     const bool sphere = (l.flags & kLampSphere) != 0u;
     const f3 center = lamp3(l.position);
     const f3 co = ro - center;
-    const float cd2 = dot(co, co);
-    const float r2 = l.radius * l.radius;
-    const float sphere_cap = sin2_to_one_minus_cos(r2 / fmaxf(cd2, 1.0e-30f));
-    const float outside_cap = spot ? fminf(sphere_cap, spot_omc_larger_spread(l)) : sphere_cap;
     PrimHit g;
-    float pdf;
     if (sphere) {
         g = intersect_sphere(ro, rd, tmin, tmax, center, l.radius);
-        pdf = sphere_light_pdf(cd2, r2, previous_normal, rd, had_transmission, outside_cap);
     } else {
         const f3 light_normal = lamp_normalize(co, neg(rd));
         g = intersect_disk(ro, rd, tmin, tmax, center, light_normal, l.radius);
-        const float light_cosine = dot(light_normal, neg(rd));
-        pdf = point_disk_pdf(g.distance * g.distance, light_cosine, l.radius);
     }
-    LampHit h;
-    h.valid = g.valid && (l.radius > 0.0f);
-    h.distance = g.distance;
-    h.pdf = pdf;
-    h.eval = point_eval_factor(l.radius, (l.flags & kLampNormalize) != 0u);
+    if (!g.valid) return false;
+    distance = g.distance;
+    eval = point_eval_factor(l.radius, (l.flags & kLampNormalize) != 0u);
     if (spot) {
+        const float cd2 = dot(co, co);
+        const float r2 = l.radius * l.radius;
         const f3 local = spot_light_to_local(l, neg(rd));
         const bool use_attenuation = !sphere || (cd2 > r2);
         const float attenuation = spot_light_attenuation(local, l.spot_angle, l.spot_smooth);
-        h.eval = h.eval * (use_attenuation ? attenuation : 1.0f);
-        h.valid = h.valid && (dot(rd, co) < 0.0f) && (h.eval > 0.0f);
-    } else {
-        h.valid = h.valid && (h.eval > 0.0f);
+        eval = eval * (use_attenuation ? attenuation : 1.0f);
+        return (dot(rd, co) < 0.0f) && (eval > 0.0f);
     }
-    return h;
+    return eval > 0.0f;
+}
+
+__device__ __forceinline__ float lamp_forward_pdf(const Lamp &l, f3 ro, f3 rd, float distance, f3 previous_normal,
+                                                  bool had_transmission) {
+    const bool spot = l.type == kLampSpot;
+    const f3 center = lamp3(l.position);
+    const f3 co = ro - center;
+    const float cd2 = dot(co, co);
+    const float r2 = l.radius * l.radius;
+    if ((l.flags & kLampSphere) != 0u) {
+        const float sphere_cap = sin2_to_one_minus_cos(r2 / fmaxf(cd2, 1.0e-30f));
+        const float outside_cap = spot ? fminf(sphere_cap, spot_omc_larger_spread(l)) : sphere_cap;
+        return sphere_light_pdf(cd2, r2, previous_normal, rd, had_transmission, outside_cap);
+    }
+    const f3 light_normal = lamp_normalize(co, neg(rd));
+    const float light_cosine = dot(light_normal, neg(rd));
+    return point_disk_pdf(distance * distance, light_cosine, l.radius);
+}
+
+// Forward-hit culling records of the first kLampCache lamps, staged in LDS
+// once per block; lamps beyond them are read from HBM.
+constexpr uint32_t kLampCache = 512u;
+
+struct LampView {
+    const float4 *sphere;  // centre, radius
+    const uint32_t *bits;  // kLampBit* and the visibility mask
+    uint32_t cached;       // records held by `sphere` / `bits`
+};
+
+// Every thread of the block must call this (it synchronises the block).
+__device__ __forceinline__ LampView stage_lamps(const SceneView &s, float4 *lds_sphere, uint32_t *lds_bits) {
+    const uint32_t n = s.lamp_count < kLampCache ? s.lamp_count : kLampCache;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        lds_sphere[i] = s.lamp_sphere[i];
+        lds_bits[i] = s.lamp_bits[i];
+    }
+    __syncthreads();
+    LampView v;
+    v.sphere = lds_sphere;
+    v.bits = lds_bits;
+    v.cached = n;
+    return v;
+}
+
+// Conservative reject for the forward scan: a sphere lamp and a disk lamp of
+// radius r both lie in the ball of radius r about the lamp centre, so a ray
+// that hits either in (tmin, tmax) passes within r of the centre at a ray
+// parameter in [pd - r, pd + r].  The radius is widened by 1 % plus a margin
+// proportional to the coordinate magnitudes, which covers the rounding of
+// the exact intersection routines.
+__device__ __forceinline__ bool lamp_may_hit(float4 sph, f3 ro, f3 rd, float tmin, float tmax) {
+    const f3 co = V(sph.x - ro.x, sph.y - ro.y, sph.z - ro.z);
+    const float pd = dot(co, rd);
+    const f3 perp = co - rd * pd;
+    const float perp2 = dot(perp, perp);
+    const float scale = fabsf(ro.x) + fabsf(ro.y) + fabsf(ro.z) + fabsf(sph.x) + fabsf(sph.y) + fabsf(sph.z) +
+                        fabsf(pd);
+    const float r = sph.w * 1.01f + 2.0e-5f * scale;
+    return (perp2 <= r * r) && (pd + r > tmin) && (pd - r < tmax);
 }
 
 // ===========================================================================
@@ -1539,7 +1589,7 @@ __device__ __forceinline__ f3 lt_safe_normalize(f3 v, f3 fallback) {
 
 // bounded_importance (:83-180)
 __device__ TreeImportance lt_bounded(f3 n, bool has_transmission, f3 p2c, float cos_theta_u, f3 cone_axis,
-                                     float theta_o, float theta_e, float distance, float energy) {
+                                     float theta_o, float theta_e, float4 trig, float distance, float energy) {
     cos_theta_u = clampf(cos_theta_u, -1.0f, 1.0f);
     const float sin_theta_u = lt_sin_from_cos(cos_theta_u);
     bool visible = energy > 0.0f;
@@ -1551,9 +1601,9 @@ __device__ TreeImportance lt_bounded(f3 n, bool has_transmission, f3 p2c, float 
     const float tc = clampf(dot(cone_axis, neg(p2c)), -1.0f, 1.0f);
     const float ts = lt_sin_from_cos(tc);
     const float tmu_c = tc * cos_theta_u + ts * sin_theta_u;
-    const float to_c = cosf(theta_o), to_s = sinf(theta_o);
+    const float to_c = trig.x, to_s = trig.y;
     const bool fully = (tc >= cos_theta_u) || (tmu_c >= to_c);
-    const bool partially = !fully && ((theta_o + theta_e > LT_PI) || (tmu_c > cosf(theta_o + theta_e)));
+    const bool partially = !fully && ((theta_o + theta_e > LT_PI) || (tmu_c > trig.z));
     const float tmu_s = lt_sin_from_cos(clampf(tmu_c, -1.0f, 1.0f));
     const float minimum_outgoing = fully ? 1.0f : (partially ? tmu_c * to_c + tmu_s * to_s : 0.0f);
     visible = visible && (fully || partially);
@@ -1563,7 +1613,7 @@ __device__ TreeImportance lt_bounded(f3 n, bool has_transmission, f3 p2c, float 
     r.max = visible ? fabsf(minimum_incidence * energy * minimum_outgoing * geometric) : 0.0f;
     const float tpu_c = tc * cos_theta_u - ts * sin_theta_u;
     const bool minimum_is_zero = (theta_e - theta_o < 0.0f) || (tc < 0.0f) || (cos_theta_u < 0.0f) ||
-                                 (tpu_c < cosf(theta_e - theta_o));
+                                 (tpu_c < trig.w);
     const float tpu_s = lt_sin_from_cos(clampf(tpu_c, -1.0f, 1.0f));
     const float maximum_outgoing = tpu_c * to_c - tpu_s * to_s;
     r.min = (minimum_is_zero || !visible) ? 0.0f : fabsf(maximum_incidence * energy * maximum_outgoing / (d * d));
@@ -1572,7 +1622,7 @@ __device__ TreeImportance lt_bounded(f3 n, bool has_transmission, f3 p2c, float 
 
 // measure_importance (:185-283)
 __device__ TreeImportance lt_measure_importance(float4 bmin_energy, float4 bmax_theta_o, float4 axis_theta_e,
-                                                uint32_t flags, f3 point, f3 n, bool has_transmission) {
+                                                float4 trig, uint32_t flags, f3 point, f3 n, bool has_transmission) {
     TreeImportance r;
     r.max = r.min = 0.0f;
     const float energy = bmin_energy.w;
@@ -1580,7 +1630,7 @@ __device__ TreeImportance lt_measure_importance(float4 bmin_energy, float4 bmax_
     const f3 cone_axis = lt_safe_normalize(V(axis_theta_e.x, axis_theta_e.y, axis_theta_e.z), V(0.0f, 0.0f, 1.0f));
     const float theta_o = bmax_theta_o.w, theta_e = axis_theta_e.w;
     f3 p2c = neg(cone_axis);
-    float cos_theta_u = cosf(theta_o + theta_e);
+    float cos_theta_u = trig.z;
     float distance = 1.0f;
     bool visible = true;
     if ((flags & kTreeDistant) == 0u) {
@@ -1602,15 +1652,16 @@ __device__ TreeImportance lt_measure_importance(float4 bmin_energy, float4 bmax_
         const f3 cb = centroid - bmax;
         distance = fmaxf(0.5f * sqrtf(dot(cb, cb)), distance);
     }
-    if (visible) r = lt_bounded(n, has_transmission, p2c, cos_theta_u, cone_axis, theta_o, theta_e, distance, energy);
+    if (visible)
+        r = lt_bounded(n, has_transmission, p2c, cos_theta_u, cone_axis, theta_o, theta_e, trig, distance, energy);
     return r;
 }
 
 __device__ __forceinline__ TreeImportance lt_emitter_importance(const SceneView &s, uint32_t i, f3 point, f3 n,
                                                                bool has_transmission) {
     const TreeEmitter &e = s.tree_emitters[i];
-    return lt_measure_importance(e.bounds_min_energy, e.bounds_max_theta_o, e.cone_axis_theta_e, e.identity.y, point,
-                                 n, has_transmission);
+    return lt_measure_importance(e.bounds_min_energy, e.bounds_max_theta_o, e.cone_axis_theta_e, e.trig, e.identity.y,
+                                 point, n, has_transmission);
 }
 
 // child_importance (:317-350)
@@ -1620,7 +1671,7 @@ __device__ TreeImportance lt_child_importance(const SceneView &s, uint32_t index
     TreeImportance r;
     r.max = r.min = 0.0f;
     if (node.topology.w == kTreeInner || node.emitters.y > 1u)
-        r = lt_measure_importance(node.bounds_min_energy, node.bounds_max_theta_o, node.cone_axis_theta_e,
+        r = lt_measure_importance(node.bounds_min_energy, node.bounds_max_theta_o, node.cone_axis_theta_e, node.trig,
                                   node.emitters.z, point, n, has_transmission);
     else if (node.emitters.y == 1u)
         r = lt_emitter_importance(s, node.emitters.x, point, n, has_transmission);
@@ -1948,8 +1999,8 @@ __device__ __forceinline__ void apply_shadow(const KernelParams &k, PathVars &p,
 // One path step.  Returns true when the path continues with p.ray; sr.active
 // is set when a shadow ray must be traced and applied before the next step.
 template <bool TRACE>
-__device__ bool path_step(const SceneView &s, const KernelParams &k, const float4 *sobol, PathVars &p,
-                          const Hit &hit, ShadowRay &sr, uint32_t *ev) {
+__device__ bool path_step(const SceneView &s, const LampView &lv, const KernelParams &k, const float4 *sobol,
+                          PathVars &p, const Hit &hit, ShadowRay &sr, uint32_t *ev) {
     sr.active = false;
     if (TRACE && ev) {
         ev[0] = 1u;
@@ -1973,35 +2024,38 @@ __device__ bool path_step(const SceneView &s, const KernelParams &k, const float
         for (;;) {
             float event_distance = surface_distance;
             uint32_t found = 0xffffffffu;
-            LampHit best;
-            best.distance = best.pdf = best.eval = 0.0f;
+            float best_eval = 0.0f;
+            const uint32_t required = kLampBitIntersectable | (camera_path ? 0u : kLampBitMis);
             for (uint32_t li = 0; li < s.lamp_count; ++li) {
-                const Lamp &l = s.lamps[li];
-                if (l.type == kLampDistant) continue;
-                const bool eligible = (l.visibility & ray_visibility) != 0u &&
-                                      (camera_path || (l.flags & kLampUseMis) != 0u) && li != previous;
+                const bool cached = li < lv.cached;
+                const uint32_t bits = cached ? lv.bits[li] : s.lamp_bits[li];
+                const bool eligible =
+                    (bits & required) == required && (bits & ray_visibility & 0xffu) != 0u && li != previous;
                 if (!eligible) continue;
-                const LampHit c =
-                    intersect_lamp(l, p.ray.o, p.ray.d, p.ray.tmin, event_distance, p.mis_normal, had_transmission);
-                if (c.valid) {
+                const float4 sph = cached ? lv.sphere[li] : s.lamp_sphere[li];
+                if (!lamp_may_hit(sph, p.ray.o, p.ray.d, p.ray.tmin, event_distance)) continue;
+                float distance, eval;
+                if (lamp_candidate(s.lamps[li], p.ray.o, p.ray.d, p.ray.tmin, event_distance, distance, eval)) {
                     found = li;
-                    best = c;
-                    event_distance = c.distance;
+                    best_eval = eval;
+                    event_distance = distance;
                 }
             }
             if (found == 0xffffffffu) break;
             const Lamp &l = s.lamps[found];
+            const float lamp_pdf = lamp_forward_pdf(l, p.ray.o, p.ray.d, event_distance, p.mis_normal,
+                                                    had_transmission);
             const float selection_pdf =
                 k.use_tree ? lt_pdf(s, s.emitter_count + found, p.ray.o, p.mis_normal, had_transmission)
                            : s.light_selection_pdf;
-            const float light_pdf = best.pdf * selection_pdf;
+            const float light_pdf = lamp_pdf * selection_pdf;
             const bool competing = p.ps.bounce > 0u && (p.bits & PATH_PREV_DELTA) == 0u;
             const float mis = forward_light_weight(k, p.prev_pdf, light_pdf, competing, light_pdf > 0.0f);
-            const f3 light_radiance = lamp3(l.color) * (l.power * best.eval);
+            const f3 light_radiance = lamp3(l.color) * (l.power * best_eval);
             p.radiance = p.radiance + film_clamp((p.throughput * light_radiance) * mis, p.ps.bounce > 1u ? 1u : 0u,
                                                  k.clamp_direct, k.clamp_indirect);
             p.ps.transparent_bounce += 1u;
-            p.ray.tmin = next_float_up(best.distance);
+            p.ray.tmin = next_float_up(event_distance);
             previous = found;
             if (p.ps.transparent_bounce >= k.transparent_max) return false;
         }

@@ -250,6 +250,9 @@ __global__ __launch_bounds__(kTraceBlock) void wf_trace_kernel(SceneView s, Kern
 
 __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, KernelParams k, const float4 *sobol,
                                                                WavefrontView w, uint32_t it) {
+    __shared__ float4 lamp_sphere[kLampCache];
+    __shared__ uint32_t lamp_bits[kLampCache];
+    const LampView lv = stage_lamps(s, lamp_sphere, lamp_bits);
     const WavefrontCounters *c = w.counters + it;
     WavefrontCounters *next = w.counters + it + 1u;
     const uint32_t count = c->shade_count;
@@ -284,7 +287,7 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, Kern
                 h.u = uv.x;
                 h.v = uv.y;
                 ShadowRay sr;
-                cont = path_step<false>(s, k, sobol, p, h, sr, nullptr);
+                cont = path_step<false>(s, lv, k, sobol, p, h, sr, nullptr);
                 shadow = sr.active;
                 if (shadow) {
                     w.shadow_o[path] = make_float4(sr.ray.o.x, sr.ray.o.y, sr.ray.o.z, sr.ray.tmax);
