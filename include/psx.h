@@ -422,6 +422,94 @@ int psx_session_sampler_probe(psx_session *session, uint32_t sample_index,
                               uint32_t total, uint32_t first_dimension,
                               uint32_t dimension_count, uint32_t *out);
 
+/* ---- render server ------------------------------------------------------
+ *
+ * A request loop over resident compiled scenes (the serving form of
+ * RenderSession::render_samples; the reference has no server, so these entry
+ * points have no single counterpart: each request is a SampleRange of one
+ * RenderSettings on one CompiledScene, with render_samples' result contract).
+ *
+ * Requests are taken in submission order by one worker thread.  A request
+ * binds the scene's material parameters as they are when the worker takes it,
+ * so psx_server_update_parameters affects exactly the requests submitted
+ * after it.  Admitted requests (film allocated within the HBM budget) run as
+ * chunks of their sample range inside batches: one wavefront launch sequence
+ * over the paths of every request in the batch that shares its scene,
+ * parameter version and integrator settings.  Batches run on several streams
+ * at once; viewport requests (window pixels and sample count within the
+ * options' limits) are batched ahead of the others and have a stream of their
+ * own.  A request's film is accumulated in sample order, one chunk at a time,
+ * so it is bit-identical to a session rendering the same range alone.
+ * All entry points are thread-safe. */
+
+typedef struct psx_server psx_server;
+
+typedef struct psx_server_options {
+    uint32_t path_capacity;        /* paths per batch (0: 4,194,304)        */
+    uint32_t streams;              /* concurrent batches, >= 2 (0: 3); the  */
+                                   /* last stream takes viewport batches    */
+    uint64_t hbm_budget;           /* bytes for films + batch state (0: 200 GB) */
+    uint32_t viewport_max_pixels;  /* (0: 256,000)                          */
+    uint32_t viewport_max_samples; /* (0: 4)                                */
+    uint32_t reserved[4];
+} psx_server_options;
+
+enum psx_request_state {
+    PSX_REQUEST_QUEUED = 0,    /* waiting for HBM admission                 */
+    PSX_REQUEST_RUNNING = 1,   /* admitted; chunks in progress              */
+    PSX_REQUEST_DONE = 2,
+    PSX_REQUEST_CANCELLED = 3,
+    PSX_REQUEST_FAILED = 4
+};
+
+typedef struct psx_request_status {
+    uint32_t state;            /* psx_request_state                         */
+    uint32_t samples_done;     /* samples accumulated so far                */
+    uint32_t batches;          /* batches the request took part in          */
+    uint32_t viewport;         /* 1 when batched as a viewport request      */
+    double submit_ms;          /* server clock (ms since server creation)   */
+    double admit_ms;
+    double first_launch_ms;
+    double done_ms;            /* latency = done_ms - submit_ms             */
+} psx_request_status;
+
+typedef struct psx_serving_stats {
+    uint64_t submitted, completed, batches, paths, rays;
+    uint64_t edits;            /* parameter versions created               */
+    uint64_t hbm_in_use;       /* films + batch state, bytes                */
+    uint64_t peak_hbm;
+    double busy_ms;            /* wall time with at least one batch running */
+} psx_serving_stats;
+
+int psx_server_create(psx_backend *backend, const psx_server_options *options,
+                      psx_server **out);
+/* Cancels queued work, waits for running batches, frees every request. */
+void psx_server_destroy(psx_server *server);
+/* Queues samples [first, first+count) of `total` (SampleRange, offset 0) of
+ * `settings` on `scene`.  Validation as psx_session_create and
+ * render_samples: PSX_ERROR_INVALID_ARGUMENT / PSX_ERROR_RANGE. */
+int psx_server_submit(psx_server *server, psx_scene *scene,
+                      const psx_render_settings *settings, uint32_t first,
+                      uint32_t count, uint32_t total, uint64_t *out_id);
+/* Parameter-only material edit ordered with submissions (see above). */
+int psx_server_update_parameters(psx_server *server, psx_scene *scene,
+                                 uint32_t material, const float *parameters,
+                                 uint32_t parameter_count);
+int psx_server_status(psx_server *server, uint64_t id, psx_request_status *out);
+/* Blocks until the request is done, cancelled or failed, or timeout_ms
+ * passes (negative: no limit); returns PSX_OK when done. */
+int psx_server_wait(psx_server *server, uint64_t id, double timeout_ms);
+/* A done request's film: raw sums as psx_session_read_film, or passes
+ * through a sink as psx_session_render (begin, one tile per pass, end). */
+int psx_server_read_film(psx_server *server, uint64_t id, float *combined,
+                         float *normal, float *albedo, uint32_t *counts);
+int psx_server_write_passes(psx_server *server, uint64_t id,
+                            const psx_sink *sink);
+int psx_server_cancel(psx_server *server, uint64_t id);
+/* Frees a finished request and its film. */
+int psx_server_release(psx_server *server, uint64_t id);
+int psx_server_stats(psx_server *server, psx_serving_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

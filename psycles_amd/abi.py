@@ -200,6 +200,26 @@ class RenderStats(C.Structure):
                 ("trace_ms", C.c_double), ("trace_launches", C.c_uint64)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("path_capacity", u32), ("streams", u32), ("hbm_budget", C.c_uint64),
+                ("viewport_max_pixels", u32), ("viewport_max_samples", u32), ("reserved", u32 * 4)]
+
+
+REQUEST_QUEUED, REQUEST_RUNNING, REQUEST_DONE, REQUEST_CANCELLED, REQUEST_FAILED = range(5)
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("state", u32), ("samples_done", u32), ("batches", u32), ("viewport", u32),
+                ("submit_ms", C.c_double), ("admit_ms", C.c_double), ("first_launch_ms", C.c_double),
+                ("done_ms", C.c_double)]
+
+
+class ServingStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
+                ("paths", C.c_uint64), ("rays", C.c_uint64), ("edits", C.c_uint64),
+                ("hbm_in_use", C.c_uint64), ("peak_hbm", C.c_uint64), ("busy_ms", C.c_double)]
+
+
 def default_settings(width: int, height: int) -> RenderSettings:
     """contract::RenderSettings defaults (include/psycles/contract/render.h:81-117)."""
     s = RenderSettings()
@@ -241,6 +261,10 @@ EXPORTED_SYMBOLS = [
     "psx_session_render_device", "psx_session_read_film",
     "psx_session_primary_hits", "psx_session_trace_events",
     "psx_sobol_table", "psx_session_sampler_probe",
+    "psx_server_create", "psx_server_destroy", "psx_server_submit",
+    "psx_server_update_parameters", "psx_server_status", "psx_server_wait",
+    "psx_server_read_film", "psx_server_write_passes", "psx_server_cancel",
+    "psx_server_release", "psx_server_stats",
 ]
 
 _lib = None
@@ -278,6 +302,23 @@ def load_library(path: os.PathLike | None = None):
     lib.psx_session_trace_events.argtypes = [vp, u32, u32, P(u32)]
     lib.psx_sobol_table.argtypes = [u32, P(f32), P(u32)]
     lib.psx_session_sampler_probe.argtypes = [vp, u32, u32, u32, u32, P(u32)]
+    lib.psx_server_create.argtypes = [vp, P(ServerOptions), P(vp)]
+    lib.psx_server_destroy.argtypes = [vp]
+    lib.psx_server_submit.argtypes = [vp, vp, P(RenderSettings), u32, u32, u32, P(C.c_uint64)]
+    lib.psx_server_update_parameters.argtypes = [vp, vp, u32, P(f32), u32]
+    lib.psx_server_status.argtypes = [vp, C.c_uint64, P(RequestStatus)]
+    lib.psx_server_wait.argtypes = [vp, C.c_uint64, C.c_double]
+    lib.psx_server_read_film.argtypes = [vp, C.c_uint64, P(f32), P(f32), P(f32), P(u32)]
+    lib.psx_server_write_passes.argtypes = [vp, C.c_uint64, P(Sink)]
+    lib.psx_server_cancel.argtypes = [vp, C.c_uint64]
+    lib.psx_server_release.argtypes = [vp, C.c_uint64]
+    lib.psx_server_stats.argtypes = [vp, P(ServingStats)]
+    for name in ("psx_server_create", "psx_server_submit", "psx_server_update_parameters",
+                 "psx_server_status", "psx_server_wait", "psx_server_read_film",
+                 "psx_server_write_passes", "psx_server_cancel", "psx_server_release",
+                 "psx_server_stats"):
+        getattr(lib, name).restype = C.c_int
+    lib.psx_server_destroy.restype = None
     for name in ("psx_backend_create", "psx_scene_compile",
                  "psx_scene_update_parameters", "psx_session_create",
                  "psx_session_reset", "psx_session_render",

@@ -261,6 +261,21 @@ struct WavefrontCounters {
     uint32_t pad;
 };
 
+// One request's share of a server batch: samples [first, first + count) of
+// every pixel of its window, held at paths
+// [path_base, path_base + count * padded) in window-tile order, and
+// accumulated by lanes [pixel_base, pixel_base + padded).  `padded`,
+// `path_base` and `pixel_base` are multiples of 64, so a wave never spans
+// two items.
+struct BatchItem {
+    KernelParams k;
+    FilmView film;
+    const float *filter;
+    const float4 *sobol;
+    uint32_t first, count, padded, path_base;
+    uint32_t pixel_base, pad0, pad1, pad2;
+};
+
 struct WavefrontView {
     float4 *ray_o;       // origin.xyz, tmin
     float4 *ray_d;       // direction.xyz, tmax

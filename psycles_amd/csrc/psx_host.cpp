@@ -10,6 +10,8 @@
 #include <array>
 #include <atomic>
 #include <cfloat>
+#include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -19,6 +21,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/psx.h"
@@ -417,6 +420,20 @@ struct Backend {
     psx_backend_options options{};
 };
 
+// Light-sampling tables that follow material emission: emitter list, flat
+// CDF, light tree.  Shared by parameter versions whose edits left them valid.
+struct EmitterTables {
+    DeviceBuffer emitters, distribution, tree_nodes, tree_emitters, tree_mapping, emitter_cprim;
+};
+
+// Everything on the device that depends on material parameter blocks.  A
+// parameter-only edit builds a new version and leaves the old one intact for
+// work still using it; the last holder frees it.
+struct ParamTables {
+    DeviceBuffer params, materials_buf, material_shadow, material_emission;
+    std::shared_ptr<EmitterTables> light;
+};
+
 struct Scene {
     Backend *backend = nullptr;
     // host copies needed after compile
@@ -435,15 +452,28 @@ struct Scene {
     // device buffers
     std::vector<Lamp> lamp_list;
     std::vector<uint32_t> mesh_prim_offsets;
-    DeviceBuffer tree_nodes, tree_emitters, tree_mapping, emitter_cprim;
     DeviceBuffer nodes, tris, instances_buf, meshes_buf, positions, indices, normals, smooth, slots, tri_slots,
-        overrides_buf, materials_buf, leaves, params, emitters, distribution, material_shadow, material_emission, lamps,
-        lamp_spheres, lamp_bits;
-    SceneView view{};
+        overrides_buf, leaves, lamps, lamp_spheres, lamp_bits;
+    std::shared_ptr<ParamTables> tables;  // current parameter version
+    SceneView view{};                     // geometry + the current version
     uint64_t triangles = 0;
 
-    void rebuild_emitters();
-    void refresh_material_tables();
+    void upload_materials(ParamTables &t);
+    // a new parameter version from the host materials; the emitter tables
+    // are rebuilt when `emission_changed`, shared otherwise
+    void apply_parameter_version(bool emission_changed) {
+        auto fresh = std::make_shared<ParamTables>();
+        upload_materials(*fresh);
+        if (emission_changed) {
+            rebuild_emitters(*fresh);
+        } else {
+            fresh->light = tables->light;
+        }
+        refresh_material_tables(*fresh);
+        tables = std::move(fresh);
+    }
+    void rebuild_emitters(ParamTables &t);
+    void refresh_material_tables(ParamTables &t);
 };
 
 // LightDesc -> LightGpu for point, spot and distant lamps
@@ -492,16 +522,28 @@ Lamp make_lamp(const psx_light &l) {
 
 // per-material shadow extinction and constant emission, evaluated on the
 // device by the same closure code the kernels run
-void Scene::refresh_material_tables() {
+void Scene::refresh_material_tables(ParamTables &t) {
     const uint32_t n = static_cast<uint32_t>(gpu_materials.size());
-    if (material_shadow.bytes() < std::max<size_t>(n * sizeof(float4), 16u)) {
-        material_shadow.allocate(n * sizeof(float4));
-        material_emission.allocate(n * sizeof(float4));
+    if (t.material_shadow.bytes() < std::max<size_t>(n * sizeof(float4), 16u)) {
+        t.material_shadow.allocate(n * sizeof(float4));
+        t.material_emission.allocate(n * sizeof(float4));
     }
-    view.material_shadow = material_shadow.as<float4>();
-    view.material_emission = material_emission.as<float4>();
-    hip_check(launch_material_tables(view, material_shadow.as<float4>(), material_emission.as<float4>(), n, nullptr));
+    view.material_shadow = t.material_shadow.as<float4>();
+    view.material_emission = t.material_emission.as<float4>();
+    hip_check(launch_material_tables(view, t.material_shadow.as<float4>(), t.material_emission.as<float4>(), n,
+                                     nullptr));
     hip_check(hipDeviceSynchronize());
+}
+
+// parameter blocks (concatenated in material order, at the offsets fixed by
+// compilation) and the material records
+void Scene::upload_materials(ParamTables &t) {
+    std::vector<float> params;
+    for (const HostMaterial &m : materials) params.insert(params.end(), m.params.begin(), m.params.end());
+    t.params.upload(params);
+    t.materials_buf.upload(gpu_materials);
+    view.params = t.params.as<float>();
+    view.materials = t.materials_buf.as<Material>();
 }
 
 // light-tree measures of an emissive triangle and of a lamp
@@ -601,7 +643,9 @@ TreeEmitterIn lamp_tree_emitter(uint32_t id, const Lamp &l) {
 }
 
 // emissive triangles (src/luisa/path_tracer_scene.cpp:1548-1640)
-void Scene::rebuild_emitters() {
+void Scene::rebuild_emitters(ParamTables &t) {
+    t.light = std::make_shared<EmitterTables>();
+    EmitterTables &lt = *t.light;
     std::vector<Emitter> em;
     std::vector<float> areas;
     std::vector<TreeEmitterIn> tree_in;
@@ -646,14 +690,14 @@ void Scene::rebuild_emitters() {
     // (src/luisa/path_tracer_light_sampling_scene.cpp:150-245)
     LightTreeTables tree;
     if (!build_light_tree(tree_in, 8u, tree) || !tree.usable) tree = LightTreeTables{};
-    tree_nodes.upload(tree.nodes);
-    tree_emitters.upload(tree.emitters);
-    tree_mapping.upload(tree.mapping);
-    emitter_cprim.upload(cprims);
-    view.tree_nodes = tree_nodes.as<TreeNode>();
-    view.tree_emitters = tree_emitters.as<TreeEmitter>();
-    view.tree_mapping = tree_mapping.as<uint2>();
-    view.emitter_cprim = emitter_cprim.as<uint32_t>();
+    lt.tree_nodes.upload(tree.nodes);
+    lt.tree_emitters.upload(tree.emitters);
+    lt.tree_mapping.upload(tree.mapping);
+    lt.emitter_cprim.upload(cprims);
+    view.tree_nodes = lt.tree_nodes.as<TreeNode>();
+    view.tree_emitters = lt.tree_emitters.as<TreeEmitter>();
+    view.tree_mapping = lt.tree_mapping.as<uint2>();
+    view.emitter_cprim = lt.emitter_cprim.as<uint32_t>();
     view.tree_node_count = static_cast<uint32_t>(tree.nodes.size());
     view.tree_emitter_count = static_cast<uint32_t>(tree.emitters.size());
     view.tree_root = tree.root;
@@ -661,11 +705,11 @@ void Scene::rebuild_emitters() {
     float tri_pdf = 0.0f;
     float lamp_pdf = 0.0f;
     const uint32_t count = build_distribution(areas, static_cast<uint32_t>(lamp_list.size()), dist, tri_pdf, lamp_pdf);
-    emitters.upload(em);
-    distribution.upload(dist);
+    lt.emitters.upload(em);
+    lt.distribution.upload(dist);
     view.light_selection_pdf = lamp_pdf;
-    view.emitters = emitters.as<Emitter>();
-    view.distribution = distribution.as<DistEntry>();
+    view.emitters = lt.emitters.as<Emitter>();
+    view.distribution = lt.distribution.as<DistEntry>();
     view.emitter_count = static_cast<uint32_t>(em.size());
     view.distribution_count = count;
     view.triangle_area_pdf = tri_pdf;
@@ -762,18 +806,12 @@ struct Session {
 
 namespace {
 
-// LuisaRenderSession::initialize (src/luisa/path_tracer_kernel.cpp:44-439)
-void initialize_session(Session &s, const psx_render_settings &rs) {
-    s.settings = rs;
-    s.passes.assign(rs.passes, rs.passes + rs.pass_count);
-    if (rs.pass_channels)
-        s.pass_channels.assign(rs.pass_channels, rs.pass_channels + rs.pass_count);
-    else
-        s.pass_channels.assign(rs.pass_count, 4u);
-    s.settings.passes = s.passes.data();
-    s.settings.pass_channels = s.pass_channels.data();
-    KernelParams &k = s.params;
-    k = KernelParams{};
+// Kernel parameters of one render request: the host part of
+// LuisaRenderSession::initialize (src/luisa/path_tracer_kernel.cpp:44-439).
+// `view` is the scene version the request renders (NEE needs a non-empty
+// light distribution).
+KernelParams kernel_params(const Scene &scene, const SceneView &view, const psx_render_settings &rs) {
+    KernelParams k{};
     k.window_x = rs.window_x;
     k.window_y = rs.window_y;
     k.window_width = rs.window_width ? rs.window_width : rs.width;
@@ -798,13 +836,13 @@ void initialize_session(Session &s, const psx_render_settings &rs) {
     k.light_inv_rr_threshold =
         (!in.use_light_tree && in.light_sampling_threshold > 0.0f) ? in.film_exposure / in.light_sampling_threshold
                                                                    : 0.0f;
-    const bool available = s.scene->view.distribution_count > 0u;
-    const bool nee = s.scene->backend->options.next_event_estimation != 0u &&
+    const bool available = view.distribution_count > 0u;
+    const bool nee = scene.backend->options.next_event_estimation != 0u &&
                      in.direct_light_sampling != PSX_DIRECT_LIGHT_FORWARD && available;
     k.nee = nee ? 1u : 0u;
     k.use_tree = in.use_light_tree ? 1u : 0u;
     k.mis_mode = nee ? in.direct_light_sampling : PSX_DIRECT_LIGHT_FORWARD;
-    const psx_camera &cam = s.scene->camera;
+    const psx_camera &cam = scene.camera;
     const float aspect = static_cast<float>(k.full_width) / static_cast<float>(k.full_height);
     const bool hfit = cam.sensor_fit == 0u || (cam.sensor_fit == 2u && aspect >= 1.0f);
     k.tan_v = hfit ? std::tan(cam.horizontal_field_of_view * 0.5f) / aspect : std::tan(cam.field_of_view * 0.5f);
@@ -815,6 +853,26 @@ void initialize_session(Session &s, const psx_render_settings &rs) {
     k.far_clip = cam.far_clip;
     k.pass_alpha_threshold = std::min(std::max(rs.pass_alpha_threshold, 0.0f), 1.0f);
     std::memcpy(k.camera, cam.transform, sizeof(k.camera));
+    return k;
+}
+
+// Copies the settings (with their pass arrays) into the session.
+void keep_settings(Session &s, const psx_render_settings &rs) {
+    s.settings = rs;
+    s.passes.assign(rs.passes, rs.passes + rs.pass_count);
+    if (rs.pass_channels)
+        s.pass_channels.assign(rs.pass_channels, rs.pass_channels + rs.pass_count);
+    else
+        s.pass_channels.assign(rs.pass_count, 4u);
+    s.settings.passes = s.passes.data();
+    s.settings.pass_channels = s.pass_channels.data();
+}
+
+// LuisaRenderSession::initialize (src/luisa/path_tracer_kernel.cpp:44-439)
+void initialize_session(Session &s, const psx_render_settings &rs) {
+    keep_settings(s, rs);
+    s.params = kernel_params(*s.scene, s.scene->view, rs);
+    const KernelParams &k = s.params;
     s.window_w = k.window_width;
     s.window_h = k.window_height;
     const size_t pixels = static_cast<size_t>(s.window_w) * s.window_h;
@@ -1270,9 +1328,9 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     s.slots.upload(slots);
     s.tri_slots.upload(tri_slots);
     s.overrides_buf.upload(overrides);
-    s.materials_buf.upload(scene->gpu_materials);
     s.leaves.upload(leaves);
-    s.params.upload(params);
+    s.tables = std::make_shared<ParamTables>();
+    s.upload_materials(*s.tables);
     SceneView &v = s.view;
     v.nodes = s.nodes.as<Node4>();
     v.tris = s.tris.as<LeafTri>();
@@ -1285,9 +1343,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     v.slots = s.slots.as<uint32_t>();
     v.tri_slots = s.tri_slots.as<uint32_t>();
     v.overrides = s.overrides_buf.as<uint32_t>();
-    v.materials = s.materials_buf.as<Material>();
     v.leaves = s.leaves.as<Leaf>();
-    v.params = s.params.as<float>();
     v.tlas_root = 0u;
     v.has_tlas = has_tlas ? 1u : 0u;
     v.instance_count = d->instance_count;
@@ -1328,8 +1384,8 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         v.lamp_bits = s.lamp_bits.as<uint32_t>();
     }
     v.lamp_count = static_cast<uint32_t>(s.lamp_list.size());
-    s.rebuild_emitters();
-    s.refresh_material_tables();
+    s.rebuild_emitters(*s.tables);
+    s.refresh_material_tables(*s.tables);
     *out = scene.release();
     return PSX_OK;
 }
@@ -1348,6 +1404,392 @@ int guarded(F &&f) {
 }
 
 }  // namespace
+
+// ===========================================================================
+// render server: continuous batching of requests over resident scenes
+// ===========================================================================
+hipError_t launch_wavefront_batch(const SceneView &s, const KernelParams &k, const float4 *sobol,
+                                  const BatchItem *items, uint32_t n_items, uint32_t paths, uint32_t pixels,
+                                  const WavefrontView &w, Counters *counters, uint32_t *spill, hipStream_t stream);
+
+namespace {
+
+using ServerClock = std::chrono::steady_clock;
+
+// Requests can share a batch when they render the same scene version with
+// the same integrator fields (everything the trace and shade kernels read).
+struct BatchKey {
+    const Scene *scene = nullptr;
+    const ParamTables *tables = nullptr;
+    uint32_t u[13] = {};
+    float f[4] = {};
+    bool operator==(const BatchKey &o) const {
+        return scene == o.scene && tables == o.tables && std::memcmp(u, o.u, sizeof(u)) == 0 &&
+               std::memcmp(f, o.f, sizeof(f)) == 0;
+    }
+};
+
+BatchKey batch_key(const Scene *scene, const ParamTables *tables, const KernelParams &k) {
+    BatchKey b;
+    b.scene = scene;
+    b.tables = tables;
+    const uint32_t u[13] = {k.max_bounces,      k.min_bounces,    k.max_diffuse,     k.max_glossy, k.max_transmission,
+                            k.transparent_min,  k.transparent_max, k.max_path_steps, k.transparent_background,
+                            k.nee,              k.mis_mode,        k.use_tree,        k.sequence_size};
+    std::memcpy(b.u, u, sizeof(u));
+    const float f[4] = {k.clamp_direct, k.clamp_indirect, k.light_inv_rr_threshold, k.pass_alpha_threshold};
+    std::memcpy(b.f, f, sizeof(f));
+    return b;
+}
+
+struct ServerRequest {
+    uint64_t id = 0;
+    Scene *scene = nullptr;
+    std::shared_ptr<ParamTables> tables;  // parameter version bound when the worker took the request
+    SceneView view{};
+    Session film;                         // film, settings, kernel parameters, filter table
+    uint32_t first = 0, count = 0, total = 0;
+    uint32_t next = 0;                    // samples handed to batches
+    uint32_t done = 0;                    // samples accumulated
+    uint32_t padded = 0;
+    uint32_t batches = 0;
+    bool in_flight = false, viewport = false, admitted = false;
+    std::atomic<bool> cancel{false};
+    uint32_t state = PSX_REQUEST_QUEUED;
+    BatchKey key;
+    size_t film_bytes = 0;
+    double submit_ms = 0.0, admit_ms = -1.0, first_launch_ms = -1.0, done_ms = -1.0;
+};
+
+struct ServerOp {
+    std::shared_ptr<ServerRequest> request;  // a submission, or
+    Scene *scene = nullptr;                  // an edit of `material` on `scene`
+    uint32_t material = 0;
+    std::vector<float> params;
+};
+
+// One stream with its own path state: a batch in flight at a time.
+struct Arena {
+    WavefrontState state;
+    DeviceBuffer items, spill, counters;
+    BatchItem *host_items = nullptr;  // pinned staging for the item table
+    Counters *host_counters = nullptr;
+    uint32_t item_capacity = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    bool viewport_only = false;
+    std::vector<std::pair<std::shared_ptr<ServerRequest>, uint32_t>> members;
+    std::shared_ptr<ParamTables> hold;
+    uint64_t paths = 0;
+    size_t bytes = 0;
+
+    ~Arena() {
+        if (host_items) (void)hipHostFree(host_items);
+        if (host_counters) (void)hipHostFree(host_counters);
+        if (done) (void)hipEventDestroy(done);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+}  // namespace
+
+struct Server {
+    Backend *backend = nullptr;
+    psx_server_options options{};
+    std::vector<std::unique_ptr<Arena>> arenas;
+    std::map<uint32_t, std::unique_ptr<DeviceBuffer>> sobol;  // by sequence size
+    hipStream_t control = nullptr;                            // film clears, edits
+
+    std::mutex mutex;
+    std::condition_variable wake, finished;
+    std::vector<ServerOp> ops;                                // submission order
+    std::map<uint64_t, std::shared_ptr<ServerRequest>> requests;
+    std::vector<std::shared_ptr<ServerRequest>> active;       // taken, not finished (worker only)
+    uint64_t next_id = 1;
+    bool stopping = false;
+    std::thread worker;
+    int worker_error = PSX_OK;
+
+    ServerClock::time_point epoch = ServerClock::now();
+    psx_serving_stats stats{};
+    size_t fixed_bytes = 0, film_bytes = 0;
+    ServerClock::time_point busy_since{};
+    uint32_t busy_arenas = 0;
+
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(ServerClock::now() - epoch).count();
+    }
+
+    ~Server() {
+        {
+            // running batches finish; everything else is cancelled
+            std::lock_guard<std::mutex> lock(mutex);
+            stopping = true;
+            for (auto &entry : requests) entry.second->cancel.store(true);
+        }
+        wake.notify_all();
+        if (worker.joinable()) worker.join();
+        if (control) (void)hipStreamDestroy(control);
+    }
+
+    const float4 *sobol_table_for(uint32_t size) {
+        auto it = sobol.find(size);
+        if (it == sobol.end()) {
+            auto buf = std::make_unique<DeviceBuffer>();
+            buf->upload(sobol_table(size));
+            it = sobol.emplace(size, std::move(buf)).first;
+        }
+        return it->second->as<float4>();
+    }
+
+    // -- worker ---------------------------------------------------------------
+    void take(ServerOp &op) {
+        if (op.request) {
+            ServerRequest &r = *op.request;
+            r.tables = r.scene->tables;
+            r.view = r.scene->view;
+            r.film.scene = r.scene;
+            KernelParams k = kernel_params(*r.scene, r.view, r.film.settings);
+            k.sequence_size = sequence_size_for(r.total);
+            r.film.params = k;
+            r.film.window_w = k.window_width;
+            r.film.window_h = k.window_height;
+            r.padded = wavefront_padded_pixels(k);
+            r.key = batch_key(r.scene, r.tables.get(), k);
+            const size_t pixels = static_cast<size_t>(k.window_width) * k.window_height;
+            r.viewport = pixels <= options.viewport_max_pixels && r.count <= options.viewport_max_samples;
+            r.film_bytes = pixels * 52u + 4096u;
+            active.push_back(op.request);
+            return;
+        }
+        // parameter-only edit: a new version for the requests taken from now on
+        HostMaterial &m = op.scene->materials[op.material];
+        m.params = op.params;
+        const bool emitted = m.may_emit;
+        refresh_may_emit(m);
+        op.scene->gpu_materials[op.material].may_emit = m.may_emit ? 1u : 0u;
+        op.scene->apply_parameter_version(emitted || m.may_emit);
+        stats.edits += 1u;
+    }
+
+    void finish(ServerRequest &r, uint32_t state) {
+        std::lock_guard<std::mutex> lock(mutex);
+        r.state = state;
+        r.done_ms = now_ms();
+        if (r.admitted) film_bytes -= r.film_bytes;
+        if (state != PSX_REQUEST_DONE) {
+            // a cancelled or failed request keeps no film
+            r.film.combined.reset();
+            r.film.normal.reset();
+            r.film.albedo.reset();
+            r.film.sample_count.reset();
+            r.film.filter.reset();
+        }
+        if (state == PSX_REQUEST_DONE) stats.completed += 1u;
+        finished.notify_all();
+    }
+
+    bool retire() {
+        bool progress = false;
+        for (auto &a : arenas) {
+            if (!a->busy) continue;
+            const hipError_t q = hipEventQuery(a->done);
+            if (q == hipErrorNotReady) continue;
+            hip_check(q);
+            progress = true;
+            a->busy = false;
+            busy_arenas -= 1u;
+            if (busy_arenas == 0u)
+                stats.busy_ms +=
+                    std::chrono::duration<double, std::milli>(ServerClock::now() - busy_since).count();
+            stats.rays += a->host_counters->rays;
+            for (auto &[r, n] : a->members) {
+                {
+                    std::lock_guard<std::mutex> lock(mutex);
+                    r->in_flight = false;
+                    r->done += n;
+                    r->film.rendered = r->done;
+                }
+                if (r->done == r->count) finish(*r, PSX_REQUEST_DONE);
+            }
+            a->members.clear();
+            a->hold.reset();
+        }
+        // cancelled requests leave as soon as no chunk of theirs is running
+        for (auto &r : active)
+            if (r->cancel.load() && !r->in_flight && r->state < PSX_REQUEST_DONE) finish(*r, PSX_REQUEST_CANCELLED);
+        const size_t before = active.size();
+        active.erase(std::remove_if(active.begin(), active.end(),
+                                    [](const std::shared_ptr<ServerRequest> &r) {
+                                        return r->state >= PSX_REQUEST_DONE && !r->in_flight;
+                                    }),
+                     active.end());
+        return progress || active.size() != before;
+    }
+
+    bool admit() {
+        bool progress = false;
+        for (auto &rp : active) {
+            ServerRequest &r = *rp;
+            if (r.admitted || r.state != PSX_REQUEST_QUEUED) continue;
+            size_t in_use;
+            {
+                std::lock_guard<std::mutex> lock(mutex);
+                in_use = fixed_bytes + film_bytes;
+            }
+            // admission in submission order: a request that does not fit
+            // holds back the ones after it
+            if (in_use + r.film_bytes > options.hbm_budget) break;
+            const size_t pixels = static_cast<size_t>(r.film.window_w) * r.film.window_h;
+            r.film.combined.allocate(pixels * 16u);
+            r.film.normal.allocate(pixels * 16u);
+            r.film.albedo.allocate(pixels * 16u);
+            r.film.sample_count.allocate(pixels * 4u);
+            r.film.filter.upload(pixel_filter_table(r.film.settings.pixel_filter, r.film.settings.filter_width));
+            hip_check(hipMemsetAsync(r.film.combined.as<void>(), 0, pixels * 16u, control));
+            hip_check(hipMemsetAsync(r.film.normal.as<void>(), 0, pixels * 16u, control));
+            hip_check(hipMemsetAsync(r.film.albedo.as<void>(), 0, pixels * 16u, control));
+            hip_check(hipMemsetAsync(r.film.sample_count.as<void>(), 0, pixels * 4u, control));
+            hip_check(hipStreamSynchronize(control));
+            std::lock_guard<std::mutex> lock(mutex);
+            r.admitted = true;
+            r.state = PSX_REQUEST_RUNNING;
+            r.admit_ms = now_ms();
+            film_bytes += r.film_bytes;
+            stats.hbm_in_use = fixed_bytes + film_bytes;
+            stats.peak_hbm = std::max(stats.peak_hbm, stats.hbm_in_use);
+            progress = true;
+        }
+        return progress;
+    }
+
+    static bool runnable(const ServerRequest &r) {
+        return r.admitted && r.state == PSX_REQUEST_RUNNING && !r.in_flight && !r.cancel.load() && r.next < r.count;
+    }
+
+    // Fills `a` with chunks of runnable requests sharing the leader's key:
+    // viewport requests first (the whole remainder of each), then the others
+    // in submission order, each as many samples as the capacity leaves.
+    bool launch(Arena &a) {
+        ServerRequest *leader = nullptr;
+        for (auto &r : active)
+            if (runnable(*r) && r->viewport) {
+                leader = r.get();
+                break;
+            }
+        if (!leader && !a.viewport_only)
+            for (auto &r : active)
+                if (runnable(*r)) {
+                    leader = r.get();
+                    break;
+                }
+        if (!leader) return false;
+        const BatchKey key = leader->key;
+        const uint32_t capacity = a.state.view.capacity;
+        uint32_t paths = 0, pixels = 0, n = 0;
+        for (int pass = 0; pass < 2 && n < a.item_capacity; ++pass) {
+            const bool want_viewport = pass == 0;
+            if (!want_viewport && a.viewport_only) break;
+            for (auto &rp : active) {
+                ServerRequest &r = *rp;
+                if (n >= a.item_capacity) break;
+                if (!runnable(r) || r.viewport != want_viewport || !(r.key == key)) continue;
+                const uint32_t room = (capacity - paths) / r.padded;
+                const uint32_t chunk = std::min(r.count - r.next, room);
+                if (chunk == 0u) continue;
+                BatchItem &it = a.host_items[n];
+                it.k = r.film.params;
+                it.film = FilmView{r.film.combined.as<float>(), r.film.normal.as<float>(), r.film.albedo.as<float>(),
+                                   r.film.sample_count.as<uint32_t>()};
+                it.filter = r.film.filter.as<float>();
+                it.sobol = sobol_table_for(r.film.params.sequence_size);
+                it.first = r.first + r.next;
+                it.count = chunk;
+                it.padded = r.padded;
+                it.path_base = paths;
+                it.pixel_base = pixels;
+                it.pad0 = it.pad1 = it.pad2 = 0u;
+                paths += chunk * r.padded;
+                pixels += r.padded;
+                r.next += chunk;
+                r.in_flight = true;
+                r.batches += 1u;
+                if (r.first_launch_ms < 0.0) r.first_launch_ms = now_ms();
+                a.members.emplace_back(rp, chunk);
+                ++n;
+            }
+        }
+        if (n == 0u) return false;
+        hip_check(hipMemcpyAsync(a.items.as<void>(), a.host_items, n * sizeof(BatchItem), hipMemcpyHostToDevice,
+                                 a.stream));
+        hip_check(hipMemsetAsync(a.counters.as<void>(), 0, sizeof(Counters), a.stream));
+        const KernelParams &k = a.host_items[0].k;
+        hip_check(launch_wavefront_batch(leader->view, k, sobol_table_for(k.sequence_size), a.items.as<BatchItem>(),
+                                         n, paths, pixels, a.state.view, a.counters.as<Counters>(),
+                                         a.spill.as<uint32_t>(), a.stream));
+        hip_check(hipMemcpyAsync(a.host_counters, a.counters.as<void>(), sizeof(Counters), hipMemcpyDeviceToHost,
+                                 a.stream));
+        hip_check(hipEventRecord(a.done, a.stream));
+        a.hold = leader->tables;
+        a.paths = paths;
+        a.busy = true;
+        if (busy_arenas == 0u) busy_since = ServerClock::now();
+        busy_arenas += 1u;
+        stats.batches += 1u;
+        stats.paths += paths;
+        return true;
+    }
+
+    void run() {
+        try {
+            hip_check(hipSetDevice(backend->device));
+            for (;;) {
+                std::vector<ServerOp> taken;
+                {
+                    std::unique_lock<std::mutex> lock(mutex);
+                    taken.swap(ops);
+                    if (stopping && taken.empty() && active.empty() && busy_arenas == 0u) break;
+                }
+                for (ServerOp &op : taken) take(op);
+                bool progress = !taken.empty();
+                progress |= retire();
+                progress |= admit();
+                for (auto &a : arenas)
+                    if (!a->busy) progress |= launch(*a);
+                if (stopping && !progress && busy_arenas == 0u) {
+                    // nothing left that can run (requests that never fit)
+                    for (auto &r : active)
+                        if (r->state < PSX_REQUEST_DONE) finish(*r, PSX_REQUEST_CANCELLED);
+                    active.clear();
+                }
+                if (!progress) {
+                    std::unique_lock<std::mutex> lock(mutex);
+                    wake.wait_for(lock, std::chrono::microseconds(busy_arenas ? 50 : 2000),
+                                  [&] { return !ops.empty() || stopping; });
+                }
+            }
+        } catch (const DeviceError &e) {
+            std::lock_guard<std::mutex> lock(mutex);
+            worker_error = e.status;
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(mutex);
+            worker_error = PSX_ERROR_DEVICE;
+        }
+        if (worker_error != PSX_OK) {
+            // wait for what is on the device before the films can go
+            for (auto &a : arenas)
+                if (a->busy) (void)hipEventSynchronize(a->done);
+            std::lock_guard<std::mutex> lock(mutex);
+            for (auto &r : active)
+                if (r->state < PSX_REQUEST_DONE) r->state = PSX_REQUEST_FAILED;
+            finished.notify_all();
+        }
+    }
+};
+
+namespace {
+}  // namespace
 }  // namespace psx
 
 // ===========================================================================
@@ -1356,6 +1798,7 @@ int guarded(F &&f) {
 using namespace psx;
 
 struct psx_backend : Backend {};
+struct psx_server : Server {};
 struct psx_scene : Scene {};
 struct psx_session : Session {};
 
@@ -1445,11 +1888,7 @@ int psx_scene_update_parameters(psx_scene *handle, uint32_t material, const floa
         refresh_may_emit(m);
         Material &g = s->gpu_materials[material];
         g.may_emit = m.may_emit ? 1u : 0u;
-        hip_check(hipMemcpy(s->params.as<float>() + g.param_offset, p, sizeof(float) * count, hipMemcpyHostToDevice));
-        hip_check(hipMemcpy(s->materials_buf.as<Material>() + material, &g, sizeof(Material), hipMemcpyHostToDevice));
-        // emitter set, flat CDF and light-tree energies follow the emission
-        if (emitted || m.may_emit) s->rebuild_emitters();
-        s->refresh_material_tables();
+        s->apply_parameter_version(emitted || m.may_emit);
         return static_cast<int>(PSX_OK);
     });
 }
@@ -1623,3 +2062,221 @@ int psx_session_sampler_probe(psx_session *handle, uint32_t sample, uint32_t tot
 }
 
 }  // extern "C"
+
+// ---- render server ---------------------------------------------------------
+int psx_server_create(psx_backend *b, const psx_server_options *options, psx_server **out) {
+    if (!b || !out) return PSX_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    psx_server_options o = options ? *options : psx_server_options{};
+    if (o.path_capacity == 0u) o.path_capacity = 4194304u;
+    o.path_capacity = (o.path_capacity + 63u) & ~63u;
+    if (o.streams == 0u) o.streams = 3u;
+    if (o.streams < 2u || o.streams > 8u || o.path_capacity > (1u << 30)) return PSX_ERROR_INVALID_ARGUMENT;
+    if (o.hbm_budget == 0u) o.hbm_budget = 200ull << 30;
+    if (o.viewport_max_pixels == 0u) o.viewport_max_pixels = 256000u;
+    if (o.viewport_max_samples == 0u) o.viewport_max_samples = 4u;
+    return guarded([&] {
+        hip_check(hipSetDevice(b->device));
+        auto server = std::make_unique<psx_server>();
+        server->backend = b;
+        server->options = o;
+        hip_check(hipStreamCreateWithFlags(&server->control, hipStreamNonBlocking));
+        const size_t lanes = wavefront_lanes();
+        for (uint32_t i = 0; i < o.streams; ++i) {
+            auto a = std::make_unique<Arena>();
+            a->state.allocate(o.path_capacity, 0u, 1025u);
+            a->item_capacity = 256u;
+            a->items.allocate(a->item_capacity * sizeof(BatchItem));
+            a->spill.allocate(lanes * kTraversalStackDepth * sizeof(uint32_t));
+            a->counters.allocate(sizeof(Counters));
+            hip_check(hipHostMalloc(reinterpret_cast<void **>(&a->host_items), a->item_capacity * sizeof(BatchItem),
+                                    hipHostMallocDefault));
+            hip_check(hipHostMalloc(reinterpret_cast<void **>(&a->host_counters), sizeof(Counters),
+                                    hipHostMallocDefault));
+            std::memset(a->host_counters, 0, sizeof(Counters));
+            hip_check(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
+            hip_check(hipEventCreateWithFlags(&a->done, hipEventDisableTiming));
+            a->viewport_only = i + 1u == o.streams;
+            a->bytes = static_cast<size_t>(o.path_capacity) * 288u + a->items.bytes() + a->spill.bytes();
+            server->fixed_bytes += a->bytes;
+            server->arenas.push_back(std::move(a));
+        }
+        if (server->fixed_bytes > o.hbm_budget) return static_cast<int>(PSX_ERROR_OUT_OF_MEMORY);
+        server->stats.hbm_in_use = server->stats.peak_hbm = server->fixed_bytes;
+        psx_server *raw = server.get();
+        raw->worker = std::thread([raw] { raw->run(); });
+        *out = server.release();
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+void psx_server_destroy(psx_server *server) { delete server; }
+
+int psx_server_submit(psx_server *server, psx_scene *handle, const psx_render_settings *rs, uint32_t first,
+                      uint32_t count, uint32_t total, uint64_t *out_id) {
+    Scene *scene = reinterpret_cast<Scene *>(handle);
+    if (!server || !scene || !out_id || !valid_settings(rs)) return PSX_ERROR_INVALID_ARGUMENT;
+    if (scene->backend != server->backend) return PSX_ERROR_INVALID_ARGUMENT;
+    if (count == 0u || total == 0u || static_cast<uint64_t>(first) + count > total) return PSX_ERROR_RANGE;
+    const uint32_t ww = rs->window_width ? rs->window_width : rs->width;
+    const uint32_t wh = rs->window_height ? rs->window_height : rs->height;
+    const uint64_t padded = static_cast<uint64_t>((ww + 7u) / 8u) * ((wh + 7u) / 8u) * 64u;
+    if (padded > server->options.path_capacity) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        auto r = std::make_shared<ServerRequest>();
+        r->scene = scene;
+        keep_settings(r->film, *rs);
+        r->first = first;
+        r->count = count;
+        r->total = total;
+        {
+            std::lock_guard<std::mutex> lock(server->mutex);
+            if (server->worker_error != PSX_OK) return server->worker_error;
+            r->id = server->next_id++;
+            r->submit_ms = server->now_ms();
+            server->requests.emplace(r->id, r);
+            ServerOp op;
+            op.request = r;
+            server->ops.push_back(std::move(op));
+            server->stats.submitted += 1u;
+            *out_id = r->id;
+        }
+        server->wake.notify_all();
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+int psx_server_update_parameters(psx_server *server, psx_scene *handle, uint32_t material, const float *p,
+                                 uint32_t count) {
+    Scene *scene = reinterpret_cast<Scene *>(handle);
+    if (!server || !scene || !p || material >= scene->materials.size()) return PSX_ERROR_INVALID_ARGUMENT;
+    if (count != scene->materials[material].params.size()) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        {
+            std::lock_guard<std::mutex> lock(server->mutex);
+            if (server->worker_error != PSX_OK) return server->worker_error;
+            ServerOp op;
+            op.scene = scene;
+            op.material = material;
+            op.params.assign(p, p + count);
+            server->ops.push_back(std::move(op));
+        }
+        server->wake.notify_all();
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+namespace {
+std::shared_ptr<ServerRequest> find_request(psx_server *server, uint64_t id) {
+    auto it = server->requests.find(id);
+    return it == server->requests.end() ? nullptr : it->second;
+}
+}  // namespace
+
+int psx_server_status(psx_server *server, uint64_t id, psx_request_status *out) {
+    if (!server || !out) return PSX_ERROR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(server->mutex);
+    auto r = find_request(server, id);
+    if (!r) return PSX_ERROR_INVALID_ARGUMENT;
+    out->state = r->state;
+    out->samples_done = r->done;
+    out->batches = r->batches;
+    out->viewport = r->viewport ? 1u : 0u;
+    out->submit_ms = r->submit_ms;
+    out->admit_ms = r->admit_ms;
+    out->first_launch_ms = r->first_launch_ms;
+    out->done_ms = r->done_ms;
+    return PSX_OK;
+}
+
+int psx_server_wait(psx_server *server, uint64_t id, double timeout_ms) {
+    if (!server) return PSX_ERROR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lock(server->mutex);
+    auto r = find_request(server, id);
+    if (!r) return PSX_ERROR_INVALID_ARGUMENT;
+    auto ended = [&] { return r->state >= PSX_REQUEST_DONE || server->worker_error != PSX_OK; };
+    if (timeout_ms < 0.0) {
+        server->finished.wait(lock, ended);
+    } else if (!server->finished.wait_for(lock, std::chrono::duration<double, std::milli>(timeout_ms), ended)) {
+        return PSX_ERROR_INCOMPLETE;
+    }
+    if (r->state == PSX_REQUEST_DONE) return PSX_OK;
+    if (r->state == PSX_REQUEST_CANCELLED) return PSX_ERROR_CANCELLED;
+    return server->worker_error != PSX_OK ? server->worker_error : static_cast<int>(PSX_ERROR_DEVICE);
+}
+
+int psx_server_read_film(psx_server *server, uint64_t id, float *combined, float *normal, float *albedo,
+                         uint32_t *counts) {
+    if (!server) return PSX_ERROR_INVALID_ARGUMENT;
+    std::shared_ptr<ServerRequest> r;
+    {
+        std::lock_guard<std::mutex> lock(server->mutex);
+        r = find_request(server, id);
+        if (!r) return PSX_ERROR_INVALID_ARGUMENT;
+        if (r->state != PSX_REQUEST_DONE) return PSX_ERROR_INCOMPLETE;
+    }
+    return guarded([&] {
+        hip_check(hipSetDevice(server->backend->device));
+        const Session &f = r->film;
+        const size_t n = static_cast<size_t>(f.window_w) * f.window_h;
+        if (combined) hip_check(hipMemcpy(combined, f.combined.as<void>(), n * 16u, hipMemcpyDeviceToHost));
+        if (normal) hip_check(hipMemcpy(normal, f.normal.as<void>(), n * 16u, hipMemcpyDeviceToHost));
+        if (albedo) hip_check(hipMemcpy(albedo, f.albedo.as<void>(), n * 16u, hipMemcpyDeviceToHost));
+        if (counts) hip_check(hipMemcpy(counts, f.sample_count.as<void>(), n * 4u, hipMemcpyDeviceToHost));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+int psx_server_write_passes(psx_server *server, uint64_t id, const psx_sink *sink) {
+    if (!server) return PSX_ERROR_INVALID_ARGUMENT;
+    std::shared_ptr<ServerRequest> r;
+    {
+        std::lock_guard<std::mutex> lock(server->mutex);
+        r = find_request(server, id);
+        if (!r) return PSX_ERROR_INVALID_ARGUMENT;
+        if (r->state != PSX_REQUEST_DONE) return PSX_ERROR_INCOMPLETE;
+    }
+    return guarded([&] {
+        hip_check(hipSetDevice(server->backend->device));
+        return write_passes(r->film, sink);
+    });
+}
+
+int psx_server_cancel(psx_server *server, uint64_t id) {
+    if (!server) return PSX_ERROR_INVALID_ARGUMENT;
+    {
+        std::lock_guard<std::mutex> lock(server->mutex);
+        auto r = find_request(server, id);
+        if (!r) return PSX_ERROR_INVALID_ARGUMENT;
+        r->cancel.store(true);
+    }
+    server->wake.notify_all();
+    return PSX_OK;
+}
+
+int psx_server_release(psx_server *server, uint64_t id) {
+    if (!server) return PSX_ERROR_INVALID_ARGUMENT;
+    std::shared_ptr<ServerRequest> r;
+    {
+        std::lock_guard<std::mutex> lock(server->mutex);
+        auto it = server->requests.find(id);
+        if (it == server->requests.end()) return PSX_ERROR_INVALID_ARGUMENT;
+        if (it->second->state < PSX_REQUEST_DONE || it->second->in_flight) return PSX_ERROR_INCOMPLETE;
+        r = std::move(it->second);
+        server->requests.erase(it);
+    }
+    // the film is freed here unless the worker still holds the request
+    return guarded([&] {
+        hip_check(hipSetDevice(server->backend->device));
+        r.reset();
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+int psx_server_stats(psx_server *server, psx_serving_stats *out) {
+    if (!server || !out) return PSX_ERROR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(server->mutex);
+    *out = server->stats;
+    out->hbm_in_use = server->fixed_bytes + server->film_bytes;
+    return PSX_OK;
+}

@@ -352,6 +352,110 @@ __global__ __launch_bounds__(256) void accumulate_kernel(KernelParams k, Wavefro
 }
 
 
+// item of a batch-wide index: the last item whose base is <= i (bases rise)
+template <bool PIXELS>
+__device__ __forceinline__ uint32_t batch_item(const BatchItem *items, uint32_t n, uint32_t i) {
+    uint32_t lo = 0u, len = n;
+    while (len > 1u) {
+        const uint32_t half = len >> 1;
+        const uint32_t base = PIXELS ? items[lo + half].pixel_base : items[lo + half].path_base;
+        if (base <= i) {
+            lo += half;
+            len -= half;
+        } else {
+            len = half;
+        }
+    }
+    // every lane of a wave lies in the same item
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(lo)));
+}
+
+// camera rays for every item of a server batch
+__global__ __launch_bounds__(kShadeBlock) void generate_batch_kernel(const BatchItem *items, uint32_t n_items,
+                                                                      WavefrontView w, uint32_t total) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0u) {
+        w.counters[0].trace_count = total;
+        w.counters[0].shade_count = total;
+    }
+    if (i - threadIdx.x % 64u >= total) return;
+    const BatchItem &it = items[batch_item<false>(items, n_items, i)];
+    const KernelParams &k = it.k;
+    const uint32_t local = i - it.path_base;
+    const uint32_t sample = it.first + local / it.padded;
+    const uint32_t rem = local % it.padded;
+    const uint32_t tile = rem >> 6, lane = rem & 63u;
+    const uint32_t tiles_x = (k.window_width + 7u) >> 3;
+    const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
+    const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
+    PathVars p;
+    bool live = false;
+    if (px < k.window_width && py < k.window_height) {
+        init_path(k, it.sobol, it.filter, px + k.window_x, py + k.window_y, sample, p);
+        live = (p.bits & PATH_DONE) == 0u;
+    } else {
+        p.bits = PATH_DONE;
+        p.ray.o = p.ray.d = V(0.0f, 0.0f, 0.0f);
+        p.ray.tmin = p.ray.tmax = 0.0f;
+        p.src_obj = p.src_prim = 0xffffffffu;
+        p.throughput = p.radiance = p.normal = p.albedo = p.mis_normal = V(0.0f, 0.0f, 0.0f);
+        p.prev_pdf = p.transparency = 0.0f;
+        p.ps = PathState{0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+        p.sample = sample;
+        p.rng_hash = 0u;
+        p.step = 0u;
+    }
+    store_path(w, i, p);
+    w.queue[0][i] = live ? (i << 1) : kSkip;
+    w.list[0][i] = live ? i : kSkip;
+}
+
+// in-order film accumulation of every item of a server batch
+__global__ __launch_bounds__(256) void accumulate_batch_kernel(const BatchItem *items, uint32_t n_items,
+                                                               WavefrontView w, uint32_t total_pixels) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i - threadIdx.x % 64u >= total_pixels) return;
+    const BatchItem &it = items[batch_item<true>(items, n_items, i)];
+    const KernelParams &k = it.k;
+    const uint32_t local = i - it.pixel_base;
+    const uint32_t tile = local >> 6, lane = local & 63u;
+    const uint32_t tiles_x = (k.window_width + 7u) >> 3;
+    const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
+    const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
+    if (px >= k.window_width || py >= k.window_height) return;
+    const size_t index = (size_t)py * k.window_width + px;
+    const FilmView &film = it.film;
+    float4 c = reinterpret_cast<float4 *>(film.combined)[index];
+    float4 n = reinterpret_cast<float4 *>(film.normal)[index];
+    float4 a = reinterpret_cast<float4 *>(film.albedo)[index];
+    uint32_t completed = film.sample_count[index];
+    for (uint32_t s = 0; s < it.count; ++s) {
+        const uint32_t path = it.path_base + s * it.padded + local;
+        const float4 r = w.radiance[path];
+        const float4 pn = w.normal[path];
+        const float4 pa = w.albedo[path];
+        f3 rad = V(r.x, r.y, r.z);
+        if (any_nan(rad)) rad = V(0.0f, 0.0f, 0.0f);
+        c.x += rad.x;
+        c.y += rad.y;
+        c.z += rad.z;
+        c.w += r.w;
+        n.x += pn.x;
+        n.y += pn.y;
+        n.z += pn.z;
+        n.w += 1.0f;
+        a.x += pa.x;
+        a.y += pa.y;
+        a.z += pa.z;
+        a.w += 1.0f;
+        completed += 1u;
+    }
+    reinterpret_cast<float4 *>(film.combined)[index] = c;
+    reinterpret_cast<float4 *>(film.normal)[index] = n;
+    reinterpret_cast<float4 *>(film.albedo)[index] = a;
+    film.sample_count[index] = completed;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -398,6 +502,32 @@ hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const flo
     }
     hipLaunchKernelGGL(accumulate_kernel, dim3((w.padded_pixels + 255u) / 256u), dim3(256), 0, stream, k, w, film,
                        count);
+    return hipGetLastError();
+}
+
+// One server batch: `items` (device copy, path bases ascending) share the
+// scene view `s` and the integrator fields of `k`; `paths` and `pixels` are
+// the batch totals of count * padded and of padded over the items.
+hipError_t launch_wavefront_batch(const SceneView &s, const KernelParams &k, const float4 *sobol,
+                                  const BatchItem *items, uint32_t n_items, uint32_t paths, uint32_t pixels,
+                                  const WavefrontView &w, Counters *counters, uint32_t *spill, hipStream_t stream) {
+    if (paths == 0u || n_items == 0u) return hipSuccess;
+    if (paths > w.capacity || (paths & 63u) != 0u || (pixels & 63u) != 0u) return hipErrorInvalidValue;
+    const uint32_t iterations = k.max_path_steps + 1u;
+    hipLaunchKernelGGL(reset_kernel, dim3((iterations + 1u + 63u) / 64u), dim3(64), 0, stream, w.counters,
+                       iterations + 1u);
+    hipLaunchKernelGGL(generate_batch_kernel, dim3((paths + kShadeBlock - 1u) / kShadeBlock), dim3(kShadeBlock), 0,
+                       stream, items, n_items, w, paths);
+    const uint32_t trace_blocks =
+        resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<false>), kTraceBlock);
+    const uint32_t shade_blocks = resident_blocks(reinterpret_cast<const void *>(wf_shade_kernel), kShadeBlock);
+    for (uint32_t it = 0; it < iterations; ++it) {
+        hipLaunchKernelGGL(wf_trace_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, stream, s, k, w, it,
+                           counters, spill);
+        hipLaunchKernelGGL(wf_shade_kernel, dim3(shade_blocks), dim3(kShadeBlock), 0, stream, s, k, sobol, w, it);
+    }
+    hipLaunchKernelGGL(accumulate_batch_kernel, dim3((pixels + 255u) / 256u), dim3(256), 0, stream, items, n_items,
+                       w, pixels);
     return hipGetLastError();
 }
 

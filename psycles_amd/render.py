@@ -108,6 +108,25 @@ class CompiledScene:
             pass
 
 
+def make_sink(output: OutputSink, settings):
+    """psx_sink forwarding to `output`; keep the returned callbacks alive for
+    the duration of the call that uses the sink."""
+    def on_begin(user, settings_ptr):
+        output.begin(settings)
+
+    def on_write(user, kind, channels, wx, wy, ww, wh, fw, fh, pixels):
+        arr = np.ctypeslib.as_array(pixels, shape=(wh * ww * channels,)).copy()
+        output.write(PassTile(kind, channels, (wx, wy, ww, wh), (fw, fh), arr.reshape(wh, ww, channels)))
+
+    def on_end(user, cancelled):
+        output.end(bool(cancelled))
+
+    refs = (abi.SINK_BEGIN(on_begin), abi.SINK_WRITE(on_write), abi.SINK_END(on_end))
+    sink = abi.Sink()
+    sink.begin, sink.write, sink.end = refs
+    return sink, refs
+
+
 class RenderSession:
     def __init__(self, backend: "RendererBackend", scene: CompiledScene, handle, settings):
         self._backend = backend
@@ -128,26 +147,7 @@ class RenderSession:
 
     def render_samples(self, samples: SampleRange, output: OutputSink) -> bool:
         lib = self._backend.lib
-        sink_ref = {}
-
-        def on_begin(user, settings_ptr):
-            output.begin(self.settings)
-
-        def on_write(user, kind, channels, wx, wy, ww, wh, fw, fh, pixels):
-            arr = np.ctypeslib.as_array(pixels, shape=(wh * ww * channels,)).copy()
-            output.write(PassTile(kind, channels, (wx, wy, ww, wh), (fw, fh),
-                                  arr.reshape(wh, ww, channels)))
-
-        def on_end(user, cancelled):
-            output.end(bool(cancelled))
-
-        sink = abi.Sink()
-        sink_ref["b"] = abi.SINK_BEGIN(on_begin)
-        sink_ref["w"] = abi.SINK_WRITE(on_write)
-        sink_ref["e"] = abi.SINK_END(on_end)
-        sink.begin = sink_ref["b"]
-        sink.write = sink_ref["w"]
-        sink.end = sink_ref["e"]
+        sink, _refs = make_sink(output, self.settings)
         rc = lib.psx_session_render(self.handle, samples.first, samples.count, samples.offset,
                                     samples.total, C.byref(sink))
         self.last_status = rc
