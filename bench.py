@@ -118,12 +118,76 @@ def calibration_copy(nbytes: int = 1 << 30) -> None:
     print(f"bench: calibration copy of {nbytes} bytes done", file=sys.stderr, flush=True)
 
 
+def serving_bench(args) -> int:
+    """The request stream over the resident Cornell, instanced and many-light
+    scenes: the render server (continuous batching) against the plain path
+    (one megakernel session per request, in arrival order)."""
+    from psycles_amd import abi, scenes, serving
+    from psycles_amd.render import RendererBackend
+    from psycles_amd.server import RenderServer
+
+    def log(msg):
+        print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+    log("building the Cornell, instanced and many-light scenes")
+    snaps = {"cornell": scenes.cornell_box(), "instanced": scenes.instanced_icospheres(),
+             "many_lights": scenes.many_lights()}
+    stream = serving.make_stream(snaps, args.requests)
+    batched_backend = RendererBackend(0, integrator=abi.INTEGRATOR_WAVEFRONT)
+    compiled = {n: batched_backend.compile_scene(sn).scene for n, sn in snaps.items()}
+    server = RenderServer(batched_backend)
+    # warm the kernels and allocator with one small request per scene
+    warm = []
+    for n in snaps:
+        st = serving.scene_settings(n, 64, 64)
+        warm.append((st, server.submit(compiled[n], st, 0, 1, 1)))
+    for _, h in warm:
+        h.wait(-1.0)
+        h.release()
+    log(f"stream of {len(stream)} requests over {stream[-1].arrival:.2f} s; running the server")
+    batched = serving.run_batched(server, compiled, stream, log)
+    server.close()
+    log(f"server: {batched['samples_per_s'] / 1e6:.1f} M samples/s, viewport p99 "
+        f"{batched['viewport_latency_ms']['p99']} ms")
+    result = {
+        "metric": METRIC,
+        "value": batched["samples_per_s"],
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": batched["makespan_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"serving: {len(stream)} requests, Poisson arrivals at 200 req/s (seed 4) over resident "
+                        "Cornell, instanced and many-light scenes; resolutions 256x256 to 1920x1080, 1-64 spp, "
+                        "30% parameter-only edits",
+            "integrator": "render server (continuous batching of wavefront batches)",
+        },
+        "batched": batched,
+    }
+    if not args.no_plain:
+        plain_backend = RendererBackend(0, integrator=abi.INTEGRATOR_MEGAKERNEL)
+        plain_compiled = {n: plain_backend.compile_scene(sn).scene for n, sn in snaps.items()}
+        log("running the plain per-request path")
+        plain = serving.run_plain(plain_backend, plain_compiled, stream, log)
+        result["plain"] = plain
+        result["batched_vs_plain"] = batched["samples_per_s"] / plain["samples_per_s"]
+    print(json.dumps(result))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="instanced")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving"], default="instanced")
+    ap.add_argument("--requests", type=int, default=120, help="serving: requests in the stream")
     ap.add_argument("--spp", type=int, default=None, help="samples per pixel (default: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the megakernel frame")
@@ -138,6 +202,8 @@ def main() -> int:
     from psycles_amd import abi, scenes
     from psycles_amd.render import RendererBackend
 
+    if args.workload == "serving":
+        return serving_bench(args)
     w = WORKLOADS[args.workload]
     print(f"bench: building the {args.workload} scene", file=sys.stderr, flush=True)
     snap = getattr(scenes, w["scene"])()

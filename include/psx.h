@@ -438,7 +438,10 @@ int psx_session_sampler_probe(psx_session *session, uint32_t sample_index,
  * parameter version and integrator settings.  Batches run on several streams
  * at once; viewport requests (window pixels and sample count within the
  * options' limits) are batched ahead of the others and have a stream of their
- * own.  A request's film is accumulated in sample order, one chunk at a time,
+ * own; while viewport work waits or runs no final-frame batch starts and
+ * running ones launch their kernels on a quarter of the compute units.
+ * Final-frame batches are sized from the measured device time per path of
+ * their scene and settings to last about target_batch_ms.  A request's film is accumulated in sample order, one chunk at a time,
  * so it is bit-identical to a session rendering the same range alone.
  * All entry points are thread-safe. */
 
@@ -446,12 +449,14 @@ typedef struct psx_server psx_server;
 
 typedef struct psx_server_options {
     uint32_t path_capacity;        /* paths per batch (0: 4,194,304)        */
-    uint32_t streams;              /* concurrent batches, >= 2 (0: 3); the  */
-                                   /* last stream takes viewport batches    */
+    uint32_t streams;              /* concurrent batches, 2..8 (0: 6); the  */
+                                   /* upper half take viewport batches only */
     uint64_t hbm_budget;           /* bytes for films + batch state (0: 200 GB) */
     uint32_t viewport_max_pixels;  /* (0: 256,000)                          */
     uint32_t viewport_max_samples; /* (0: 4)                                */
-    uint32_t reserved[4];
+    uint32_t target_batch_ms;      /* final-frame batch length the server   */
+                                   /* sizes batches for (0: 12)             */
+    uint32_t reserved[3];
 } psx_server_options;
 
 enum psx_request_state {

@@ -202,7 +202,8 @@ class RenderStats(C.Structure):
 
 class ServerOptions(C.Structure):
     _fields_ = [("path_capacity", u32), ("streams", u32), ("hbm_budget", C.c_uint64),
-                ("viewport_max_pixels", u32), ("viewport_max_samples", u32), ("reserved", u32 * 4)]
+                ("viewport_max_pixels", u32), ("viewport_max_samples", u32), ("target_batch_ms", u32),
+                ("reserved", u32 * 3)]
 
 
 REQUEST_QUEUED, REQUEST_RUNNING, REQUEST_DONE, REQUEST_CANCELLED, REQUEST_FAILED = range(5)

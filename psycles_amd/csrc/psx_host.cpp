@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <map>
 #include <memory>
@@ -71,20 +72,41 @@ class DeviceBuffer {
     DeviceBuffer &operator=(const DeviceBuffer &) = delete;
     ~DeviceBuffer() { reset(); }
     void reset() {
-        if (ptr_) (void)hipFree(ptr_);
+        if (ptr_) {
+            if (stream_)
+                (void)hipFreeAsync(ptr_, stream_);
+            else
+                (void)hipFree(ptr_);
+        }
         ptr_ = nullptr;
         bytes_ = 0;
     }
     void allocate(size_t bytes) {
         reset();
         if (bytes == 0) bytes = 16;
-        hip_check(hipMalloc(&ptr_, bytes));
+        if (stream_)
+            hip_check(hipMallocAsync(&ptr_, bytes, stream_));
+        else
+            hip_check(hipMalloc(&ptr_, bytes));
         bytes_ = bytes;
+    }
+    // Stream-ordered allocation and release on `stream` from now on: no
+    // device-wide synchronisation when the buffer is freed while other
+    // streams run.  The stream must outlive the buffer's allocation.
+    void set_stream(hipStream_t stream) {
+        reset();
+        stream_ = stream;
     }
     template <typename T>
     void upload(const std::vector<T> &v) {
         allocate(v.size() * sizeof(T));
-        if (!v.empty()) hip_check(hipMemcpy(ptr_, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        if (v.empty()) return;
+        if (stream_) {
+            hip_check(hipMemcpyAsync(ptr_, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
+            hip_check(hipStreamSynchronize(stream_));
+        } else {
+            hip_check(hipMemcpy(ptr_, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        }
     }
     template <typename T>
     T *as() const {
@@ -95,6 +117,7 @@ class DeviceBuffer {
   private:
     void *ptr_ = nullptr;
     size_t bytes_ = 0;
+    hipStream_t stream_ = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -424,6 +447,10 @@ struct Backend {
 // CDF, light tree.  Shared by parameter versions whose edits left them valid.
 struct EmitterTables {
     DeviceBuffer emitters, distribution, tree_nodes, tree_emitters, tree_mapping, emitter_cprim;
+    explicit EmitterTables(hipStream_t stream) {
+        for (DeviceBuffer *b : {&emitters, &distribution, &tree_nodes, &tree_emitters, &tree_mapping, &emitter_cprim})
+            b->set_stream(stream);
+    }
 };
 
 // Everything on the device that depends on material parameter blocks.  A
@@ -432,6 +459,14 @@ struct EmitterTables {
 struct ParamTables {
     DeviceBuffer params, materials_buf, material_shadow, material_emission;
     std::shared_ptr<EmitterTables> light;
+    hipEvent_t ready = nullptr;  // recorded after the last upload and table kernel
+    explicit ParamTables(hipStream_t stream) {
+        for (DeviceBuffer *b : {&params, &materials_buf, &material_shadow, &material_emission}) b->set_stream(stream);
+        hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    }
+    ~ParamTables() {
+        if (ready) (void)hipEventDestroy(ready);
+    }
 };
 
 struct Scene {
@@ -454,15 +489,26 @@ struct Scene {
     std::vector<uint32_t> mesh_prim_offsets;
     DeviceBuffer nodes, tris, instances_buf, meshes_buf, positions, indices, normals, smooth, slots, tri_slots,
         overrides_buf, leaves, lamps, lamp_spheres, lamp_bits;
+    // parameter versions are built, and released, on this stream (see
+    // DeviceBuffer::set_stream)
+    hipStream_t table_stream = nullptr;
     std::shared_ptr<ParamTables> tables;  // current parameter version
     SceneView view{};                     // geometry + the current version
+
+    ~Scene() {
+        tables.reset();
+        if (table_stream) {
+            (void)hipStreamSynchronize(table_stream);
+            (void)hipStreamDestroy(table_stream);
+        }
+    }
     uint64_t triangles = 0;
 
     void upload_materials(ParamTables &t);
     // a new parameter version from the host materials; the emitter tables
     // are rebuilt when `emission_changed`, shared otherwise
     void apply_parameter_version(bool emission_changed) {
-        auto fresh = std::make_shared<ParamTables>();
+        auto fresh = std::make_shared<ParamTables>(table_stream);
         upload_materials(*fresh);
         if (emission_changed) {
             rebuild_emitters(*fresh);
@@ -473,6 +519,18 @@ struct Scene {
         tables = std::move(fresh);
     }
     void rebuild_emitters(ParamTables &t);
+
+    // Triangles whose material can emit for some parameter block (its
+    // program has an emission closure and its emission sampling is not
+    // NONE), on light-visible instances, in (instance, primitive) order: the
+    // order in which the emitter list enumerates emissive triangles.  Built
+    // once, so parameter edits do not rescan every instanced triangle.
+    struct EmissiveCandidate {
+        uint32_t instance, prim, material;
+    };
+    std::vector<EmissiveCandidate> emissive_candidates;
+    bool candidates_indexed = false;
+    void index_emissive_candidates();
     void refresh_material_tables(ParamTables &t);
 };
 
@@ -531,8 +589,10 @@ void Scene::refresh_material_tables(ParamTables &t) {
     view.material_shadow = t.material_shadow.as<float4>();
     view.material_emission = t.material_emission.as<float4>();
     hip_check(launch_material_tables(view, t.material_shadow.as<float4>(), t.material_emission.as<float4>(), n,
-                                     nullptr));
-    hip_check(hipDeviceSynchronize());
+                                     table_stream));
+    // work of this version waits on `ready` (hipStreamWaitEvent), callers
+    // that need the tables on the host side of the call synchronise on it
+    hip_check(hipEventRecord(t.ready, table_stream));
 }
 
 // parameter blocks (concatenated in material order, at the offsets fixed by
@@ -643,13 +703,14 @@ TreeEmitterIn lamp_tree_emitter(uint32_t id, const Lamp &l) {
 }
 
 // emissive triangles (src/luisa/path_tracer_scene.cpp:1548-1640)
-void Scene::rebuild_emitters(ParamTables &t) {
-    t.light = std::make_shared<EmitterTables>();
-    EmitterTables &lt = *t.light;
-    std::vector<Emitter> em;
-    std::vector<float> areas;
-    std::vector<TreeEmitterIn> tree_in;
-    std::vector<uint32_t> cprims;
+void Scene::index_emissive_candidates() {
+    std::vector<bool> can_emit(materials.size(), false);
+    for (size_t m = 0; m < materials.size(); ++m) {
+        bool emission = false;
+        for (const psx_closure_instr &ins : materials[m].instr) emission |= ins.op == PSX_CLOSURE_EMISSION;
+        can_emit[m] = emission && materials[m].emission_sampling != PSX_EMISSION_SAMPLING_NONE;
+    }
+    emissive_candidates.clear();
     for (uint32_t i = 0; i < instances.size(); ++i) {
         const psx_instance &inst = instances[i];
         const bool light_visible =
@@ -659,8 +720,6 @@ void Scene::rebuild_emitters(ParamTables &t) {
         if (!light_visible) continue;
         const uint32_t mi = inst.mesh;
         const Mesh &mesh = meshes[mi];
-        const auto &pos = mesh_positions[mi];
-        const auto &tri = mesh_tris[mi];
         for (uint32_t p = 0; p < mesh.triangle_count; ++p) {
             const uint32_t slot = mesh_tri_slots[mi].empty() ? 0u : mesh_tri_slots[mi][p];
             uint32_t mat = kNone;
@@ -668,7 +727,29 @@ void Scene::rebuild_emitters(ParamTables &t) {
                 mat = overrides[i][slot];
             else if (!mesh_slots[mi].empty())
                 mat = mesh_slots[mi][std::min<size_t>(slot, mesh_slots[mi].size() - 1u)];
-            if (mat >= materials.size() || !materials[mat].may_emit) continue;
+            if (mat >= materials.size() || !can_emit[mat]) continue;
+            emissive_candidates.push_back(EmissiveCandidate{i, p, mat});
+        }
+    }
+    candidates_indexed = true;
+}
+
+void Scene::rebuild_emitters(ParamTables &t) {
+    t.light = std::make_shared<EmitterTables>(table_stream);
+    EmitterTables &lt = *t.light;
+    std::vector<Emitter> em;
+    std::vector<float> areas;
+    std::vector<TreeEmitterIn> tree_in;
+    std::vector<uint32_t> cprims;
+    if (!candidates_indexed) index_emissive_candidates();
+    {
+        for (const EmissiveCandidate &cand : emissive_candidates) {
+            const uint32_t i = cand.instance, p = cand.prim, mat = cand.material;
+            if (!materials[mat].may_emit) continue;
+            const psx_instance &inst = instances[i];
+            const uint32_t mi = inst.mesh;
+            const auto &pos = mesh_positions[mi];
+            const auto &tri = mesh_tris[mi];
             Vec3 q[3];
             for (int c = 0; c < 3; ++c) {
                 const uint32_t vi = tri[3u * p + c];
@@ -1329,7 +1410,8 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     s.tri_slots.upload(tri_slots);
     s.overrides_buf.upload(overrides);
     s.leaves.upload(leaves);
-    s.tables = std::make_shared<ParamTables>();
+    hip_check(hipStreamCreateWithFlags(&s.table_stream, hipStreamNonBlocking));
+    s.tables = std::make_shared<ParamTables>(s.table_stream);
     s.upload_materials(*s.tables);
     SceneView &v = s.view;
     v.nodes = s.nodes.as<Node4>();
@@ -1386,6 +1468,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     v.lamp_count = static_cast<uint32_t>(s.lamp_list.size());
     s.rebuild_emitters(*s.tables);
     s.refresh_material_tables(*s.tables);
+    hip_check(hipEventSynchronize(s.tables->ready));
     *out = scene.release();
     return PSX_OK;
 }
@@ -1408,9 +1491,13 @@ int guarded(F &&f) {
 // ===========================================================================
 // render server: continuous batching of requests over resident scenes
 // ===========================================================================
-hipError_t launch_wavefront_batch(const SceneView &s, const KernelParams &k, const float4 *sobol,
-                                  const BatchItem *items, uint32_t n_items, uint32_t paths, uint32_t pixels,
-                                  const WavefrontView &w, Counters *counters, uint32_t *spill, hipStream_t stream);
+hipError_t launch_batch_begin(const BatchItem *items, uint32_t n_items, uint32_t paths, uint32_t iterations,
+                              const WavefrontView &w, hipStream_t stream);
+hipError_t launch_batch_iteration(const SceneView &s, const KernelParams &k, const float4 *sobol,
+                                  const WavefrontView &w, uint32_t it, Counters *counters, uint32_t *spill,
+                                  float share, hipStream_t stream);
+hipError_t launch_batch_end(const BatchItem *items, uint32_t n_items, uint32_t pixels, const WavefrontView &w,
+                            hipStream_t stream);
 
 namespace {
 
@@ -1456,8 +1543,16 @@ struct ServerRequest {
     bool in_flight = false, viewport = false, admitted = false;
     std::atomic<bool> cancel{false};
     uint32_t state = PSX_REQUEST_QUEUED;
+    uint64_t version = 0;     // the scene version this request renders
+    bool bound = false;       // version tables and kernel parameters known
     BatchKey key;
     size_t film_bytes = 0;
+    hipEvent_t film_ready = nullptr;  // the cleared film, on the control stream
+    const float *filter = nullptr;    // the server's table for the request's pixel filter
+
+    ~ServerRequest() {
+        if (film_ready) (void)hipEventDestroy(film_ready);
+    }
     double submit_ms = 0.0, admit_ms = -1.0, first_launch_ms = -1.0, done_ms = -1.0;
 };
 
@@ -1466,27 +1561,62 @@ struct ServerOp {
     Scene *scene = nullptr;                  // an edit of `material` on `scene`
     uint32_t material = 0;
     std::vector<float> params;
+    uint64_t version = 0;                    // the edit's version number on its scene
+};
+
+// Parameter versions of one scene as the server sees them: version n is the
+// scene after the first n edits taken by the worker.  The edit thread builds
+// them in order; a request binds the version its submission position names.
+struct SceneVersions {
+    uint64_t requested = 0;   // edits taken
+    uint64_t published = 0;   // edits applied and published
+    std::map<uint64_t, std::pair<std::shared_ptr<ParamTables>, SceneView>> history;
 };
 
 // One stream with its own path state: a batch in flight at a time.
+// Iterations of a batch queued on its stream ahead of the device.
+constexpr uint32_t kIterationsAhead = 3u;
+// fraction of the resident blocks a final-frame batch launches with while
+// viewport work waits or runs
+constexpr float kViewportShare = 0.25f;
+constexpr uint32_t kMaxIterations = 1025u;
+
 struct Arena {
     WavefrontState state;
     DeviceBuffer items, spill, counters;
     BatchItem *host_items = nullptr;  // pinned staging for the item table
     Counters *host_counters = nullptr;
+    WavefrontCounters *host_queues = nullptr;  // pinned: queue sizes after each iteration
     uint32_t item_capacity = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t start = nullptr, done = nullptr;
+    hipEvent_t ring[kIterationsAhead + 1u] = {};
     bool busy = false;
+    bool ending = false;          // accumulation queued; `done` ends the batch
+    bool held = false;            // paused for viewport work at least once
+    uint32_t iterations = 0;      // at most k.max_path_steps + 1
+    uint32_t next_it = 0;         // next iteration to queue
+    uint32_t completed_it = 0;    // iterations known to be finished
+    // what the iterations and the end of the batch launch with
+    SceneView view{};
+    KernelParams k{};
+    const float4 *sobol = nullptr;
+    uint32_t n_items = 0, pixels = 0;
     bool viewport_only = false;
+    bool viewport_batch = false;  // the batch in flight serves viewport requests
+    BatchKey key;
     std::vector<std::pair<std::shared_ptr<ServerRequest>, uint32_t>> members;
     std::shared_ptr<ParamTables> hold;
     uint64_t paths = 0;
     size_t bytes = 0;
 
     ~Arena() {
+        for (hipEvent_t e : ring)
+            if (e) (void)hipEventDestroy(e);
+        if (host_queues) (void)hipHostFree(host_queues);
         if (host_items) (void)hipHostFree(host_items);
         if (host_counters) (void)hipHostFree(host_counters);
+        if (start) (void)hipEventDestroy(start);
         if (done) (void)hipEventDestroy(done);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1499,6 +1629,7 @@ struct Server {
     psx_server_options options{};
     std::vector<std::unique_ptr<Arena>> arenas;
     std::map<uint32_t, std::unique_ptr<DeviceBuffer>> sobol;  // by sequence size
+    std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<DeviceBuffer>> filters;  // by (filter, width bits)
     hipStream_t control = nullptr;                            // film clears, edits
 
     std::mutex mutex;
@@ -1510,6 +1641,16 @@ struct Server {
     bool stopping = false;
     std::thread worker;
     int worker_error = PSX_OK;
+    // edits run on their own thread so that host-side rebuilds and table
+    // uploads never delay batch feeding
+    std::thread editor;
+    std::deque<ServerOp> edits;
+    std::condition_variable edit_wake;
+    std::map<Scene *, SceneVersions> versions;  // guarded by `mutex`
+
+    // device time per path of recent batches, per key: final-frame batches
+    // are sized so that one takes about options.target_batch_ms
+    std::vector<std::pair<BatchKey, double>> cost_ns;
 
     ServerClock::time_point epoch = ServerClock::now();
     psx_serving_stats stats{};
@@ -1529,8 +1670,38 @@ struct Server {
             for (auto &entry : requests) entry.second->cancel.store(true);
         }
         wake.notify_all();
+        edit_wake.notify_all();
         if (worker.joinable()) worker.join();
-        if (control) (void)hipStreamDestroy(control);
+        if (editor.joinable()) editor.join();
+        // films are released on the control stream: drop them before it goes
+        ops.clear();
+        active.clear();
+        requests.clear();
+        arenas.clear();
+        if (control) {
+            (void)hipStreamSynchronize(control);
+            (void)hipStreamDestroy(control);
+        }
+    }
+
+    double &cost_of(const BatchKey &key) {
+        for (auto &c : cost_ns)
+            if (c.first == key) return c.second;
+        cost_ns.emplace_back(key, 5.0);  // first guess, refined by every batch
+        return cost_ns.back().second;
+    }
+
+    const float *filter_table_for(uint32_t filter, float width) {
+        uint32_t bits;
+        std::memcpy(&bits, &width, sizeof(bits));
+        const auto key = std::make_pair(filter, bits);
+        auto it = filters.find(key);
+        if (it == filters.end()) {
+            auto buf = std::make_unique<DeviceBuffer>();
+            buf->upload(pixel_filter_table(filter, width));
+            it = filters.emplace(key, std::move(buf)).first;
+        }
+        return it->second->as<float>();
     }
 
     const float4 *sobol_table_for(uint32_t size) {
@@ -1544,33 +1715,107 @@ struct Server {
     }
 
     // -- worker ---------------------------------------------------------------
+    // caller holds `mutex`
+    SceneVersions &versions_of(Scene *scene) {
+        auto it = versions.find(scene);
+        if (it == versions.end()) {
+            // first use: version 0 is the scene as it is now (no edit of
+            // this server can be running on it yet)
+            it = versions.emplace(scene, SceneVersions{}).first;
+            it->second.history[0] = std::make_pair(scene->tables, scene->view);
+        }
+        return it->second;
+    }
+
     void take(ServerOp &op) {
+        std::lock_guard<std::mutex> lock(mutex);
         if (op.request) {
             ServerRequest &r = *op.request;
-            r.tables = r.scene->tables;
-            r.view = r.scene->view;
+            r.version = versions_of(r.scene).requested;
             r.film.scene = r.scene;
-            KernelParams k = kernel_params(*r.scene, r.view, r.film.settings);
-            k.sequence_size = sequence_size_for(r.total);
-            r.film.params = k;
-            r.film.window_w = k.window_width;
-            r.film.window_h = k.window_height;
-            r.padded = wavefront_padded_pixels(k);
-            r.key = batch_key(r.scene, r.tables.get(), k);
-            const size_t pixels = static_cast<size_t>(k.window_width) * k.window_height;
+            const psx_render_settings &rs = r.film.settings;
+            r.film.window_w = rs.window_width ? rs.window_width : rs.width;
+            r.film.window_h = rs.window_height ? rs.window_height : rs.height;
+            r.padded = ((r.film.window_w + 7u) >> 3) * ((r.film.window_h + 7u) >> 3) * 64u;
+            const size_t pixels = static_cast<size_t>(r.film.window_w) * r.film.window_h;
             r.viewport = pixels <= options.viewport_max_pixels && r.count <= options.viewport_max_samples;
             r.film_bytes = pixels * 52u + 4096u;
             active.push_back(op.request);
             return;
         }
-        // parameter-only edit: a new version for the requests taken from now on
-        HostMaterial &m = op.scene->materials[op.material];
-        m.params = op.params;
-        const bool emitted = m.may_emit;
-        refresh_may_emit(m);
-        op.scene->gpu_materials[op.material].may_emit = m.may_emit ? 1u : 0u;
-        op.scene->apply_parameter_version(emitted || m.may_emit);
-        stats.edits += 1u;
+        // parameter-only edit: version n + 1 of its scene, for the requests
+        // taken from now on; built by the edit thread
+        op.version = ++versions_of(op.scene).requested;
+        edits.push_back(std::move(op));
+        edit_wake.notify_all();
+    }
+
+    // binds requests whose scene version is published (worker only)
+    bool bind() {
+        bool progress = false;
+        std::lock_guard<std::mutex> lock(mutex);
+        for (auto &rp : active) {
+            ServerRequest &r = *rp;
+            if (r.bound) continue;
+            SceneVersions &v = versions_of(r.scene);
+            if (v.published < r.version) continue;
+            const auto &entry = v.history.at(r.version);
+            r.tables = entry.first;
+            r.view = entry.second;
+            KernelParams k = kernel_params(*r.scene, r.view, r.film.settings);
+            k.sequence_size = sequence_size_for(r.total);
+            r.film.params = k;
+            r.key = batch_key(r.scene, r.tables.get(), k);
+            r.bound = true;
+            progress = true;
+        }
+        // versions no request can still bind
+        for (auto &sv : versions) {
+            uint64_t keep = sv.second.requested;
+            for (auto &rp : active)
+                if (!rp->bound && rp->scene == sv.first) keep = std::min(keep, rp->version);
+            keep = std::min(keep, sv.second.published);
+            auto &h = sv.second.history;
+            while (!h.empty() && h.begin()->first < keep) h.erase(h.begin());
+        }
+        return progress;
+    }
+
+    void edit_loop() {
+        try {
+            hip_check(hipSetDevice(backend->device));
+            for (;;) {
+                ServerOp op;
+                {
+                    std::unique_lock<std::mutex> lock(mutex);
+                    edit_wake.wait(lock, [&] { return !edits.empty() || stopping; });
+                    if (edits.empty()) return;
+                    op = std::move(edits.front());
+                    edits.pop_front();
+                }
+                HostMaterial &m = op.scene->materials[op.material];
+                m.params = op.params;
+                const bool emitted = m.may_emit;
+                refresh_may_emit(m);
+                op.scene->gpu_materials[op.material].may_emit = m.may_emit ? 1u : 0u;
+                op.scene->apply_parameter_version(emitted || m.may_emit);
+                {
+                    std::lock_guard<std::mutex> lock(mutex);
+                    SceneVersions &v = versions_of(op.scene);
+                    v.history[op.version] = std::make_pair(op.scene->tables, op.scene->view);
+                    v.published = op.version;
+                    stats.edits += 1u;
+                }
+                wake.notify_all();
+            }
+        } catch (const DeviceError &e) {
+            std::lock_guard<std::mutex> lock(mutex);
+            worker_error = e.status;
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(mutex);
+            worker_error = PSX_ERROR_DEVICE;
+        }
+        wake.notify_all();
     }
 
     void finish(ServerRequest &r, uint32_t state) {
@@ -1593,17 +1838,25 @@ struct Server {
     bool retire() {
         bool progress = false;
         for (auto &a : arenas) {
-            if (!a->busy) continue;
+            if (!a->busy || !a->ending) continue;
             const hipError_t q = hipEventQuery(a->done);
             if (q == hipErrorNotReady) continue;
             hip_check(q);
             progress = true;
             a->busy = false;
+            a->ending = false;
             busy_arenas -= 1u;
             if (busy_arenas == 0u)
                 stats.busy_ms +=
                     std::chrono::duration<double, std::milli>(ServerClock::now() - busy_since).count();
             stats.rays += a->host_counters->rays;
+            float elapsed = 0.0f;
+            if (a->paths > 0u && !a->held && hipEventElapsedTime(&elapsed, a->start, a->done) == hipSuccess) {
+                // concurrent batches inflate this, which only errs towards
+                // shorter batches
+                double &c = cost_of(a->key);
+                c = 0.7 * c + 0.3 * (static_cast<double>(elapsed) * 1.0e6 / static_cast<double>(a->paths));
+            }
             for (auto &[r, n] : a->members) {
                 {
                     std::lock_guard<std::mutex> lock(mutex);
@@ -1642,16 +1895,21 @@ struct Server {
             // holds back the ones after it
             if (in_use + r.film_bytes > options.hbm_budget) break;
             const size_t pixels = static_cast<size_t>(r.film.window_w) * r.film.window_h;
+            // stream-ordered allocation and clearing; the request's first
+            // batch waits on film_ready, the worker never blocks here
+            for (DeviceBuffer *b : {&r.film.combined, &r.film.normal, &r.film.albedo, &r.film.sample_count})
+                b->set_stream(control);
             r.film.combined.allocate(pixels * 16u);
             r.film.normal.allocate(pixels * 16u);
             r.film.albedo.allocate(pixels * 16u);
             r.film.sample_count.allocate(pixels * 4u);
-            r.film.filter.upload(pixel_filter_table(r.film.settings.pixel_filter, r.film.settings.filter_width));
+            r.filter = filter_table_for(r.film.settings.pixel_filter, r.film.settings.filter_width);
             hip_check(hipMemsetAsync(r.film.combined.as<void>(), 0, pixels * 16u, control));
             hip_check(hipMemsetAsync(r.film.normal.as<void>(), 0, pixels * 16u, control));
             hip_check(hipMemsetAsync(r.film.albedo.as<void>(), 0, pixels * 16u, control));
             hip_check(hipMemsetAsync(r.film.sample_count.as<void>(), 0, pixels * 4u, control));
-            hip_check(hipStreamSynchronize(control));
+            hip_check(hipEventCreateWithFlags(&r.film_ready, hipEventDisableTiming));
+            hip_check(hipEventRecord(r.film_ready, control));
             std::lock_guard<std::mutex> lock(mutex);
             r.admitted = true;
             r.state = PSX_REQUEST_RUNNING;
@@ -1665,12 +1923,17 @@ struct Server {
     }
 
     static bool runnable(const ServerRequest &r) {
-        return r.admitted && r.state == PSX_REQUEST_RUNNING && !r.in_flight && !r.cancel.load() && r.next < r.count;
+        return r.bound && r.admitted && r.state == PSX_REQUEST_RUNNING && !r.in_flight && !r.cancel.load() &&
+               r.next < r.count;
     }
 
     // Fills `a` with chunks of runnable requests sharing the leader's key:
     // viewport requests first (the whole remainder of each), then the others
     // in submission order, each as many samples as the capacity leaves.
+    //
+    // While viewport work is waiting or running no new final-frame batch
+    // starts, and final-frame batches are sized from the key's measured cost
+    // to last about options.target_batch_ms.
     bool launch(Arena &a) {
         ServerRequest *leader = nullptr;
         for (auto &r : active)
@@ -1678,19 +1941,29 @@ struct Server {
                 leader = r.get();
                 break;
             }
-        if (!leader && !a.viewport_only)
+        const bool viewport = leader != nullptr;
+        if (!leader) {
+            if (a.viewport_only) return false;
+            for (auto &b : arenas)
+                if (b->busy && b->viewport_batch) return false;
             for (auto &r : active)
                 if (runnable(*r)) {
                     leader = r.get();
                     break;
                 }
+        }
         if (!leader) return false;
         const BatchKey key = leader->key;
-        const uint32_t capacity = a.state.view.capacity;
+        uint32_t capacity = a.state.view.capacity;
+        if (!viewport) {
+            const double target = static_cast<double>(options.target_batch_ms) * 1.0e6 / cost_of(key);
+            const uint64_t bounded = std::max<uint64_t>(static_cast<uint64_t>(target), leader->padded);
+            capacity = static_cast<uint32_t>(std::min<uint64_t>(bounded, capacity));
+        }
         uint32_t paths = 0, pixels = 0, n = 0;
         for (int pass = 0; pass < 2 && n < a.item_capacity; ++pass) {
             const bool want_viewport = pass == 0;
-            if (!want_viewport && a.viewport_only) break;
+            if (want_viewport != viewport) continue;
             for (auto &rp : active) {
                 ServerRequest &r = *rp;
                 if (n >= a.item_capacity) break;
@@ -1702,7 +1975,7 @@ struct Server {
                 it.k = r.film.params;
                 it.film = FilmView{r.film.combined.as<float>(), r.film.normal.as<float>(), r.film.albedo.as<float>(),
                                    r.film.sample_count.as<uint32_t>()};
-                it.filter = r.film.filter.as<float>();
+                it.filter = r.filter;
                 it.sobol = sobol_table_for(r.film.params.sequence_size);
                 it.first = r.first + r.next;
                 it.count = chunk;
@@ -1721,24 +1994,85 @@ struct Server {
             }
         }
         if (n == 0u) return false;
+        hip_check(hipEventRecord(a.start, a.stream));
+        // the parameter version's tables and every member's cleared film
+        hip_check(hipStreamWaitEvent(a.stream, leader->tables->ready, 0));
+        for (auto &member : a.members) hip_check(hipStreamWaitEvent(a.stream, member.first->film_ready, 0));
         hip_check(hipMemcpyAsync(a.items.as<void>(), a.host_items, n * sizeof(BatchItem), hipMemcpyHostToDevice,
                                  a.stream));
         hip_check(hipMemsetAsync(a.counters.as<void>(), 0, sizeof(Counters), a.stream));
-        const KernelParams &k = a.host_items[0].k;
-        hip_check(launch_wavefront_batch(leader->view, k, sobol_table_for(k.sequence_size), a.items.as<BatchItem>(),
-                                         n, paths, pixels, a.state.view, a.counters.as<Counters>(),
-                                         a.spill.as<uint32_t>(), a.stream));
-        hip_check(hipMemcpyAsync(a.host_counters, a.counters.as<void>(), sizeof(Counters), hipMemcpyDeviceToHost,
-                                 a.stream));
-        hip_check(hipEventRecord(a.done, a.stream));
+        a.k = a.host_items[0].k;
+        a.view = leader->view;
+        a.sobol = sobol_table_for(a.k.sequence_size);
+        a.n_items = n;
+        a.pixels = pixels;
+        a.iterations = a.k.max_path_steps + 1u;
+        a.next_it = 0u;
+        a.completed_it = 0u;
+        a.ending = false;
+        a.held = false;
+        hip_check(launch_batch_begin(a.items.as<BatchItem>(), n, paths, a.iterations, a.state.view, a.stream));
         a.hold = leader->tables;
         a.paths = paths;
+        a.key = key;
+        a.viewport_batch = viewport;
         a.busy = true;
         if (busy_arenas == 0u) busy_since = ServerClock::now();
         busy_arenas += 1u;
         stats.batches += 1u;
         stats.paths += paths;
+        feed(a, false);
         return true;
+    }
+
+    bool viewport_pending() const {
+        for (auto &r : active)
+            if (runnable(*r) && r->viewport) return true;
+        for (auto &a : arenas)
+            if (a->busy && a->viewport_batch && !a->ending) return true;
+        return false;
+    }
+
+    // Queues the next iterations of a running batch (at most
+    // kIterationsAhead ahead of the device), or its end once the queues ran
+    // dry or every iteration is queued.  While `hold` (viewport work waits or
+    // runs), a final-frame batch's iterations launch on kViewportShare of the
+    // device's resident blocks, which leaves compute units free for the
+    // viewport batches' kernels.
+    bool feed(Arena &a, bool hold) {
+        if (!a.busy || a.ending) return false;
+        bool progress = false;
+        while (a.completed_it < a.next_it) {
+            const hipError_t q = hipEventQuery(a.ring[a.completed_it % (kIterationsAhead + 1u)]);
+            if (q == hipErrorNotReady) break;
+            hip_check(q);
+            a.completed_it += 1u;
+            progress = true;
+        }
+        const WavefrontCounters &after = a.host_queues[a.completed_it];
+        const bool drained = a.completed_it > 0u && after.trace_count == 0u && after.shade_count == 0u;
+        if (drained || a.next_it == a.iterations) {
+            // stream order puts the end after every iteration already queued
+            hip_check(launch_batch_end(a.items.as<BatchItem>(), a.n_items, a.pixels, a.state.view, a.stream));
+            hip_check(hipMemcpyAsync(a.host_counters, a.counters.as<void>(), sizeof(Counters),
+                                     hipMemcpyDeviceToHost, a.stream));
+            hip_check(hipEventRecord(a.done, a.stream));
+            a.ending = true;
+            return true;
+        }
+        const bool shrink = hold && !a.viewport_batch;
+        while (a.next_it < a.iterations && a.next_it - a.completed_it < kIterationsAhead) {
+            const uint32_t it = a.next_it;
+            a.held |= shrink;
+            hip_check(launch_batch_iteration(a.view, a.k, a.sobol, a.state.view, it, a.counters.as<Counters>(),
+                                             a.spill.as<uint32_t>(), shrink ? kViewportShare : 1.0f, a.stream));
+            hip_check(hipMemcpyAsync(&a.host_queues[it + 1u], a.state.view.counters + it + 1u,
+                                     sizeof(WavefrontCounters), hipMemcpyDeviceToHost, a.stream));
+            hip_check(hipEventRecord(a.ring[it % (kIterationsAhead + 1u)], a.stream));
+            a.next_it += 1u;
+            progress = true;
+        }
+        return progress;
     }
 
     void run() {
@@ -1749,14 +2083,18 @@ struct Server {
                 {
                     std::unique_lock<std::mutex> lock(mutex);
                     taken.swap(ops);
+                    if (worker_error != PSX_OK) throw DeviceError{worker_error};
                     if (stopping && taken.empty() && active.empty() && busy_arenas == 0u) break;
                 }
                 for (ServerOp &op : taken) take(op);
                 bool progress = !taken.empty();
+                progress |= bind();
                 progress |= retire();
                 progress |= admit();
                 for (auto &a : arenas)
                     if (!a->busy) progress |= launch(*a);
+                const bool hold = viewport_pending();
+                for (auto &a : arenas) progress |= feed(*a, hold);
                 if (stopping && !progress && busy_arenas == 0u) {
                     // nothing left that can run (requests that never fit)
                     for (auto &r : active)
@@ -1889,6 +2227,7 @@ int psx_scene_update_parameters(psx_scene *handle, uint32_t material, const floa
         Material &g = s->gpu_materials[material];
         g.may_emit = m.may_emit ? 1u : 0u;
         s->apply_parameter_version(emitted || m.may_emit);
+        hip_check(hipEventSynchronize(s->tables->ready));
         return static_cast<int>(PSX_OK);
     });
 }
@@ -2070,11 +2409,12 @@ int psx_server_create(psx_backend *b, const psx_server_options *options, psx_ser
     psx_server_options o = options ? *options : psx_server_options{};
     if (o.path_capacity == 0u) o.path_capacity = 4194304u;
     o.path_capacity = (o.path_capacity + 63u) & ~63u;
-    if (o.streams == 0u) o.streams = 3u;
+    if (o.streams == 0u) o.streams = 6u;
     if (o.streams < 2u || o.streams > 8u || o.path_capacity > (1u << 30)) return PSX_ERROR_INVALID_ARGUMENT;
     if (o.hbm_budget == 0u) o.hbm_budget = 200ull << 30;
     if (o.viewport_max_pixels == 0u) o.viewport_max_pixels = 256000u;
     if (o.viewport_max_samples == 0u) o.viewport_max_samples = 4u;
+    if (o.target_batch_ms == 0u) o.target_batch_ms = 12u;
     return guarded([&] {
         hip_check(hipSetDevice(b->device));
         auto server = std::make_unique<psx_server>();
@@ -2084,7 +2424,7 @@ int psx_server_create(psx_backend *b, const psx_server_options *options, psx_ser
         const size_t lanes = wavefront_lanes();
         for (uint32_t i = 0; i < o.streams; ++i) {
             auto a = std::make_unique<Arena>();
-            a->state.allocate(o.path_capacity, 0u, 1025u);
+            a->state.allocate(o.path_capacity, 0u, kMaxIterations);
             a->item_capacity = 256u;
             a->items.allocate(a->item_capacity * sizeof(BatchItem));
             a->spill.allocate(lanes * kTraversalStackDepth * sizeof(uint32_t));
@@ -2095,8 +2435,14 @@ int psx_server_create(psx_backend *b, const psx_server_options *options, psx_ser
                                     hipHostMallocDefault));
             std::memset(a->host_counters, 0, sizeof(Counters));
             hip_check(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
-            hip_check(hipEventCreateWithFlags(&a->done, hipEventDisableTiming));
-            a->viewport_only = i + 1u == o.streams;
+            hip_check(hipEventCreate(&a->start));
+            hip_check(hipEventCreate(&a->done));
+            for (hipEvent_t &e : a->ring) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            hip_check(hipHostMalloc(reinterpret_cast<void **>(&a->host_queues),
+                                    (kMaxIterations + 1u) * sizeof(WavefrontCounters), hipHostMallocDefault));
+            std::memset(a->host_queues, 0, (kMaxIterations + 1u) * sizeof(WavefrontCounters));
+            // the upper half of the streams takes viewport batches only
+            a->viewport_only = i >= o.streams - std::max(o.streams / 2u, 1u);
             a->bytes = static_cast<size_t>(o.path_capacity) * 288u + a->items.bytes() + a->spill.bytes();
             server->fixed_bytes += a->bytes;
             server->arenas.push_back(std::move(a));
@@ -2105,6 +2451,7 @@ int psx_server_create(psx_backend *b, const psx_server_options *options, psx_ser
         server->stats.hbm_in_use = server->stats.peak_hbm = server->fixed_bytes;
         psx_server *raw = server.get();
         raw->worker = std::thread([raw] { raw->run(); });
+        raw->editor = std::thread([raw] { raw->edit_loop(); });
         *out = server.release();
         return static_cast<int>(PSX_OK);
     });

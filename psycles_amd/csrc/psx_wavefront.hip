@@ -18,6 +18,8 @@
 // bit-identical to the megakernel's.  Queue counts live in device memory, so
 // the host enqueues all iterations without synchronising; an empty iteration
 // costs two near-empty launches.
+#include <algorithm>
+
 #include "psx_launch.h"
 #include "psx_path.h"
 
@@ -505,27 +507,44 @@ hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const flo
     return hipGetLastError();
 }
 
-// One server batch: `items` (device copy, path bases ascending) share the
-// scene view `s` and the integrator fields of `k`; `paths` and `pixels` are
-// the batch totals of count * padded and of padded over the items.
-hipError_t launch_wavefront_batch(const SceneView &s, const KernelParams &k, const float4 *sobol,
-                                  const BatchItem *items, uint32_t n_items, uint32_t paths, uint32_t pixels,
-                                  const WavefrontView &w, Counters *counters, uint32_t *spill, hipStream_t stream) {
-    if (paths == 0u || n_items == 0u) return hipSuccess;
-    if (paths > w.capacity || (paths & 63u) != 0u || (pixels & 63u) != 0u) return hipErrorInvalidValue;
-    const uint32_t iterations = k.max_path_steps + 1u;
+// A server batch, launched in steps so that the host can interleave other
+// batches between its iterations: begin (counters, camera rays of every
+// item), any number of iterations, end (in-order accumulation per item).
+// `items` (device copy, path bases ascending) share the scene view `s` and
+// the integrator fields of `k`; `paths` and `pixels` are the batch totals of
+// count * padded and of padded over the items.
+hipError_t launch_batch_begin(const BatchItem *items, uint32_t n_items, uint32_t paths, uint32_t iterations,
+                              const WavefrontView &w, hipStream_t stream) {
+    if (paths == 0u || n_items == 0u || paths > w.capacity || (paths & 63u) != 0u) return hipErrorInvalidValue;
     hipLaunchKernelGGL(reset_kernel, dim3((iterations + 1u + 63u) / 64u), dim3(64), 0, stream, w.counters,
                        iterations + 1u);
     hipLaunchKernelGGL(generate_batch_kernel, dim3((paths + kShadeBlock - 1u) / kShadeBlock), dim3(kShadeBlock), 0,
                        stream, items, n_items, w, paths);
-    const uint32_t trace_blocks =
+    return hipGetLastError();
+}
+
+// `share` in (0, 1]: the fraction of the device's resident blocks the two
+// kernels launch with, which leaves the rest of the compute units to batches
+// on other streams.
+hipError_t launch_batch_iteration(const SceneView &s, const KernelParams &k, const float4 *sobol,
+                                  const WavefrontView &w, uint32_t it, Counters *counters, uint32_t *spill,
+                                  float share, hipStream_t stream) {
+    static const uint32_t trace_blocks =
         resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<false>), kTraceBlock);
-    const uint32_t shade_blocks = resident_blocks(reinterpret_cast<const void *>(wf_shade_kernel), kShadeBlock);
-    for (uint32_t it = 0; it < iterations; ++it) {
-        hipLaunchKernelGGL(wf_trace_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, stream, s, k, w, it,
-                           counters, spill);
-        hipLaunchKernelGGL(wf_shade_kernel, dim3(shade_blocks), dim3(kShadeBlock), 0, stream, s, k, sobol, w, it);
-    }
+    static const uint32_t shade_blocks =
+        resident_blocks(reinterpret_cast<const void *>(wf_shade_kernel), kShadeBlock);
+    const float f = share > 0.0f && share < 1.0f ? share : 1.0f;
+    const uint32_t tb = std::max(1u, static_cast<uint32_t>(static_cast<float>(trace_blocks) * f));
+    const uint32_t sb = std::max(1u, static_cast<uint32_t>(static_cast<float>(shade_blocks) * f));
+    hipLaunchKernelGGL(wf_trace_kernel<false>, dim3(tb), dim3(kTraceBlock), 0, stream, s, k, w, it, counters,
+                       spill);
+    hipLaunchKernelGGL(wf_shade_kernel, dim3(sb), dim3(kShadeBlock), 0, stream, s, k, sobol, w, it);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_end(const BatchItem *items, uint32_t n_items, uint32_t pixels, const WavefrontView &w,
+                            hipStream_t stream) {
+    if ((pixels & 63u) != 0u) return hipErrorInvalidValue;
     hipLaunchKernelGGL(accumulate_batch_kernel, dim3((pixels + 255u) / 256u), dim3(256), 0, stream, items, n_items,
                        w, pixels);
     return hipGetLastError();

@@ -74,7 +74,8 @@ class ServerRequest:
 
 class RenderServer:
     def __init__(self, backend: RendererBackend, path_capacity: int = 0, streams: int = 0,
-                 hbm_budget: int = 0, viewport_max_pixels: int = 0, viewport_max_samples: int = 0):
+                 hbm_budget: int = 0, viewport_max_pixels: int = 0, viewport_max_samples: int = 0,
+                 target_batch_ms: int = 0):
         self.backend = backend
         self.lib = backend.lib
         opts = abi.ServerOptions()
@@ -83,6 +84,7 @@ class RenderServer:
         opts.hbm_budget = hbm_budget
         opts.viewport_max_pixels = viewport_max_pixels
         opts.viewport_max_samples = viewport_max_samples
+        opts.target_batch_ms = target_batch_ms
         handle = C.c_void_p()
         abi.check("psx_server_create", self.lib.psx_server_create(backend.handle, C.byref(opts), C.byref(handle)))
         self.handle = handle
