@@ -293,4 +293,58 @@ Bvh4 build_bvh4(const std::vector<BuildItem> &items, uint32_t node_base, uint32_
     return out;
 }
 
+QNode4 quantize_node(const Node4 &n, bool tlas) {
+    QNode4 q{};
+    const float *lo[3] = {n.lo_x, n.lo_y, n.lo_z};
+    const float *hi[3] = {n.hi_x, n.hi_y, n.hi_z};
+    uint8_t *qlo[3] = {q.lo_x, q.lo_y, q.lo_z};
+    uint8_t *qhi[3] = {q.hi_x, q.hi_y, q.hi_z};
+    bool any = false;
+    float nlo[3] = {0, 0, 0}, nhi[3] = {0, 0, 0};
+    for (int i = 0; i < 4; ++i) {
+        if (n.count[i] == kEmptySlot) continue;
+        for (int a = 0; a < 3; ++a) {
+            nlo[a] = any ? std::min(nlo[a], lo[a][i]) : lo[a][i];
+            nhi[a] = any ? std::max(nhi[a], hi[a][i]) : hi[a][i];
+        }
+        any = true;
+    }
+    for (int a = 0; a < 3; ++a) {
+        // the smallest power-of-two step that spans the node in 255 steps
+        q.origin[a] = nlo[a];
+        const double extent = static_cast<double>(nhi[a]) - static_cast<double>(nlo[a]);
+        int e = -126;
+        if (extent > 0.0) {
+            e = std::max(-126, static_cast<int>(std::ceil(std::log2(extent / 255.0))));
+            while (e < 127 && std::ceil(extent / std::ldexp(1.0, e)) > 255.0) ++e;
+        }
+        q.exp[a] = static_cast<int8_t>(e);
+        const double step = std::ldexp(1.0, e);
+        for (int i = 0; i < 4; ++i) {
+            if (n.count[i] == kEmptySlot) continue;
+            // outward rounding, checked in exact arithmetic
+            double l = std::floor((static_cast<double>(lo[a][i]) - nlo[a]) / step);
+            double h = std::ceil((static_cast<double>(hi[a][i]) - nlo[a]) / step);
+            l = std::min(std::max(l, 0.0), 255.0);
+            h = std::min(std::max(h, 0.0), 255.0);
+            while (l > 0.0 && static_cast<double>(nlo[a]) + l * step > lo[a][i]) l -= 1.0;
+            while (h < 255.0 && static_cast<double>(nlo[a]) + h * step < hi[a][i]) h += 1.0;
+            qlo[a][i] = static_cast<uint8_t>(l);
+            qhi[a][i] = static_cast<uint8_t>(h);
+        }
+    }
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t c = n.count[i], child = n.child[i];
+        if (c == kEmptySlot)
+            q.ref[i] = kEmptySlot;
+        else if (c == 0u)
+            q.ref[i] = child | (tlas ? kKindTlas : kKindBlas);
+        else if (tlas)
+            q.ref[i] = child | kKindInstance;
+        else
+            q.ref[i] = kKindLeaf | (child << 3) | (c - 1u);
+    }
+    return q;
+}
+
 }  // namespace psx

@@ -25,4 +25,21 @@ struct Bvh4 {
 // max_leaf: largest number of items per leaf (1..4).
 Bvh4 build_bvh4(const std::vector<BuildItem> &items, uint32_t node_base, uint32_t max_leaf);
 
+// The traversal form of a node array: node i becomes QNode4 i with 8-bit
+// outward-rounded child bounds and child references encoded as traversal
+// entries.  `is_tlas(i)` tells whether node i belongs to the instance level
+// (its leaves hold one instance index) or to a mesh level (its leaves hold
+// count triangles from child[i]).
+template <typename IsTlas>
+std::vector<QNode4> quantize_nodes(const std::vector<Node4> &nodes, IsTlas is_tlas);
+
+QNode4 quantize_node(const Node4 &n, bool tlas);
+
+template <typename IsTlas>
+std::vector<QNode4> quantize_nodes(const std::vector<Node4> &nodes, IsTlas is_tlas) {
+    std::vector<QNode4> out(nodes.size());
+    for (size_t i = 0; i < nodes.size(); ++i) out[i] = quantize_node(nodes[i], is_tlas(static_cast<uint32_t>(i)));
+    return out;
+}
+
 }  // namespace psx

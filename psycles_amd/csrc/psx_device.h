@@ -31,6 +31,30 @@ struct alignas(16) Node4 {
 static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
 constexpr uint32_t kEmptySlot = 0xffffffffu;
 
+// Traversal entries (stack entries and quantised child references) carry
+// their kind in the two top bits.
+constexpr uint32_t kKindMask = 0xc0000000u;
+constexpr uint32_t kKindBlas = 0x00000000u;      // BLAS inner node index
+constexpr uint32_t kKindTlas = 0x40000000u;      // TLAS inner node index
+constexpr uint32_t kKindLeaf = 0x80000000u;      // first triangle << 3 | (count - 1)
+constexpr uint32_t kKindInstance = 0xc0000000u;  // instance index
+
+// The node record the kernels traverse: a Node4 quantised to 64 bytes, so a
+// node fetch moves half the bytes and the L2 holds twice the nodes.  Child
+// bound = origin + q * 2^exp per axis, with q an 8-bit integer rounded
+// outward (bounds only grow, so culling stays conservative).  ref[i] is the
+// child's traversal entry (kind | index), kEmptySlot for an unused slot.
+struct alignas(16) QNode4 {
+    float origin[3];
+    int8_t exp[3];
+    uint8_t pad0;
+    uint8_t lo_x[4], hi_x[4], lo_y[4], hi_y[4];
+    uint8_t lo_z[4], hi_z[4];
+    uint32_t pad1[2];
+    uint32_t ref[4];
+};
+static_assert(sizeof(QNode4) == 64, "QNode4 must be 64 bytes");
+
 // Leaf triangle in intersection space (object space for shared meshes, final
 // world space for single-user meshes).  p0.w carries the local primitive
 // index as raw bits.
@@ -181,7 +205,7 @@ struct DistEntry {
 // scene view passed to kernels (device pointers)
 // ---------------------------------------------------------------------------
 struct SceneView {
-    const Node4 *nodes;
+    const QNode4 *nodes;
     const LeafTri *tris;
     const Instance *instances;
     const Mesh *meshes;

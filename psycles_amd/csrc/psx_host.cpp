@@ -1369,6 +1369,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     uint32_t tlas_depth = 0;
     std::vector<uint32_t> tlas_leaf_items;
     bool has_tlas = !tlas_items.empty();
+    uint32_t tlas_begin = static_cast<uint32_t>(nodes.size());  // node 0 and [tlas_begin, end) are TLAS nodes
     if (has_tlas) {
         const uint32_t base = static_cast<uint32_t>(nodes.size());
         Bvh4 t = build_bvh4(tlas_items, base, 1u);
@@ -1398,7 +1399,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
                     std::to_string(kTraversalStackDepth) + ")");
     // upload
     Scene &s = *scene;
-    s.nodes.upload(nodes);
+    s.nodes.upload(quantize_nodes(nodes, [&](uint32_t i) { return has_tlas && (i == 0u || i >= tlas_begin); }));
     s.tris.upload(tris);
     s.instances_buf.upload(instances);
     s.meshes_buf.upload(scene->meshes);
@@ -1414,7 +1415,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     s.tables = std::make_shared<ParamTables>(s.table_stream);
     s.upload_materials(*s.tables);
     SceneView &v = s.view;
-    v.nodes = s.nodes.as<Node4>();
+    v.nodes = s.nodes.as<QNode4>();
     v.tris = s.tris.as<LeafTri>();
     v.instances = s.instances_buf.as<Instance>();
     v.meshes = s.meshes_buf.as<Mesh>();

@@ -276,13 +276,6 @@ __device__ __forceinline__ uint32_t label_from_events(uint32_t e) {
 constexpr int kStackDepth = static_cast<int>(kTraversalStackDepth);
 constexpr uint32_t kShortStack = 16u;  // LDS entries per lane; deeper entries spill to HBM
 
-// Stack entries carry their kind in the two top bits.
-constexpr uint32_t kKindMask = 0xc0000000u;
-constexpr uint32_t kKindBlas = 0x00000000u;      // BLAS inner node index
-constexpr uint32_t kKindTlas = 0x40000000u;      // TLAS inner node index
-constexpr uint32_t kKindLeaf = 0x80000000u;      // first triangle << 3 | (count - 1)
-constexpr uint32_t kKindInstance = 0xc0000000u;  // instance index
-
 struct Ray {
     f3 o, d;
     float tmin, tmax;
@@ -498,30 +491,41 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
     }
     // inner node: test the four child boxes, continue with the nearest hit
     // child and push the others far-to-near
-    const bool tlas = kind == kKindTlas;
-    const Node4 &n = s.nodes[index];
+    const uint4 *rec = reinterpret_cast<const uint4 *>(s.nodes + index);
+    const uint4 head = rec[0], qxy = rec[1], qz = rec[2], refs = rec[3];
     if (COUNT) node_count++;
     const float upper = (t.best.valid ? t.best.t : t.ray.tmax) * 1.00001f + 1.0e-30f;
+    // slab distance of origin + q * 2^e is fma(q, 2^e * inv, origin * inv - o * inv)
+    auto pow2 = [](uint32_t bits, int shift) {
+        const int e = static_cast<int>(static_cast<int8_t>((bits >> shift) & 0xffu));
+        return __uint_as_float(static_cast<uint32_t>(e + 127) << 23);
+    };
+    const float sx = pow2(head.w, 0) * t.inv.x, sy = pow2(head.w, 8) * t.inv.y, sz = pow2(head.w, 16) * t.inv.z;
+    const float bx = fmaf(__uint_as_float(head.x), t.inv.x, -t.oi.x);
+    const float by = fmaf(__uint_as_float(head.y), t.inv.y, -t.oi.y);
+    const float bz = fmaf(__uint_as_float(head.z), t.inv.z, -t.oi.z);
+    const uint32_t ref_arr[4] = {refs.x, refs.y, refs.z, refs.w};
     float key[4];
     uint32_t ent[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         // conservative: boxes are padded by 1e-5 of their coordinate
-        // magnitude and `far` is widened by 1e-5, both far above the
-        // rounding of o * inv
-        const float t0x = fmaf(n.lo_x[i], t.inv.x, -t.oi.x), t1x = fmaf(n.hi_x[i], t.inv.x, -t.oi.x);
-        const float t0y = fmaf(n.lo_y[i], t.inv.y, -t.oi.y), t1y = fmaf(n.hi_y[i], t.inv.y, -t.oi.y);
-        const float t0z = fmaf(n.lo_z[i], t.inv.z, -t.oi.z), t1z = fmaf(n.hi_z[i], t.inv.z, -t.oi.z);
+        // magnitude before quantisation, which rounds outward, and `far` is
+        // widened by 1e-5, all far above the rounding of these slab terms
+        const int sh = 8 * i;
+        const float t0x = fmaf(static_cast<float>((qxy.x >> sh) & 0xffu), sx, bx);
+        const float t1x = fmaf(static_cast<float>((qxy.y >> sh) & 0xffu), sx, bx);
+        const float t0y = fmaf(static_cast<float>((qxy.z >> sh) & 0xffu), sy, by);
+        const float t1y = fmaf(static_cast<float>((qxy.w >> sh) & 0xffu), sy, by);
+        const float t0z = fmaf(static_cast<float>((qz.x >> sh) & 0xffu), sz, bz);
+        const float t1z = fmaf(static_cast<float>((qz.y >> sh) & 0xffu), sz, bz);
         const float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
         const float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
         const float far_safe = far * 1.00001f + 1.0e-30f;
-        const uint32_t count = n.count[i];
-        const bool hit = count != kEmptySlot && near <= far_safe && far_safe >= t.lower && near <= upper;
-        const uint32_t child = n.child[i];
-        const uint32_t inner = child | (tlas ? kKindTlas : kKindBlas);
-        const uint32_t leaf = tlas ? (child | kKindInstance) : (kKindLeaf | (child << 3) | (count - 1u));
+        const uint32_t ref = ref_arr[i];
+        const bool hit = ref != kEmptySlot && near <= far_safe && far_safe >= t.lower && near <= upper;
         key[i] = hit ? near : INFINITY;
-        ent[i] = count == 0u ? inner : leaf;
+        ent[i] = ref;
     }
     sort_pair(key[0], ent[0], key[1], ent[1]);
     sort_pair(key[2], ent[2], key[3], ent[3]);
