@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kShadeBlock) void generate_kernel(KernelParams k, c
 // their traversal state (Aila & Laine, "Understanding the efficiency of ray
 // traversal on GPUs", 2009: persistent threads with ray replacement).
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void wf_trace_kernel(SceneView s, KernelParams k, WavefrontView w,
+__global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, KernelParams k, WavefrontView w,
                                                                uint32_t it, Counters *counters, uint32_t *spill) {
     __shared__ uint32_t lds[kShortStack * kTraceBlock];
     Stack st = make_stack(lds, spill, kTraceBlock, threadIdx.x, blockIdx.x * kTraceBlock + threadIdx.x,
