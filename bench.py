@@ -6,7 +6,7 @@ bounces, mix(diffuse, transparent) materials, constant world background.
 Inputs (scene tables, BVH, film) are resident in HBM before the timed region.
 
 The timed path is the batched wavefront integrator (trace and shade kernels
-over path state in HBM, 8 samples of every pixel per dispatch).  The plain
+over path state in HBM, 32 samples of every pixel per dispatch).  The plain
 megakernel (one lane runs whole paths) is timed on one frame beside it.
 
 `--workload many_lights` times the many-light scene instead (the instanced
@@ -34,7 +34,7 @@ METRIC = "Mrays/s & samples/s (batched vs plain) at 1 GPU; p50/p99 request laten
 
 FILM_BYTES_PER_PIXEL_DISPATCH = 104.0
 HBM_PEAK_GBS = 8000.0
-SAMPLES_PER_DISPATCH = 8
+SAMPLES_PER_DISPATCH = 32
 
 
 def bytes_per_ray(nodes: float, tris: float) -> float:
@@ -53,14 +53,14 @@ def bytes_per_ray(nodes: float, tris: float) -> float:
 WORKLOADS = {
     "instanced": {
         "scene": "instanced_icospheres", "settings": "instanced_settings", "spp": 64,
-        "nodes": 46.83, "tris": 8.59, "traffic": 1025.9,
+        "nodes": 46.83, "tris": 8.59, "traffic": 1028.1,
         "describe": "instanced icospheres: 1,003,520-tri displaced icosphere x 64 instances, "
                     "mix(diffuse, transparent) materials, ground quad, constant background; "
                     "1920x1080, {spp} spp, max 8 bounces, seed 0",
     },
     "many_lights": {
         "scene": "many_lights", "settings": "many_light_settings", "spp": 32,
-        "nodes": 94.81, "tris": 15.06, "traffic": 1694.5,
+        "nodes": 94.81, "tris": 15.06, "traffic": 1703.5,
         "describe": "many lights: the instanced icospheres plus 10,000 emissive add(emission, diffuse) "
                     "triangles, 256 point, 64 spot and 1 distant lamp, light-tree NEE; "
                     "1920x1080, {spp} spp, max 6 bounces, seed 0",
