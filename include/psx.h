@@ -67,9 +67,19 @@ enum psx_pass_kind {
     PSX_PASS_POSITION = 3,
     PSX_PASS_ALBEDO = 4,
     PSX_PASS_EMISSION = 5,
+    PSX_PASS_DIFFUSE_DIRECT = 6,
+    PSX_PASS_DIFFUSE_INDIRECT = 7,
+    PSX_PASS_GLOSSY_DIRECT = 8,
+    PSX_PASS_GLOSSY_INDIRECT = 9,
+    PSX_PASS_TRANSMISSION_DIRECT = 10,
+    PSX_PASS_TRANSMISSION_INDIRECT = 11,
+    PSX_PASS_VOLUME_DIRECT = 12,
+    PSX_PASS_VOLUME_INDIRECT = 13,
     PSX_PASS_DENOISING_NORMAL = 18,
     PSX_PASS_DENOISING_ALBEDO = 19,
     PSX_PASS_SAMPLE_COUNT = 20,
+    PSX_PASS_GLOSSY_COLOR = 23,
+    PSX_PASS_TRANSMISSION_COLOR = 24,
     PSX_PASS_ENVIRONMENT = 25
 };
 

@@ -2697,6 +2697,42 @@ static v3 film_clamp(v3 c, uint32_t depth, float direct, float indirect) {
     return c;
 }
 
+/* light passes (src/luisa/path_kernel_setup.cpp:696-733,
+ * src/luisa/path_tracer_lighting.cpp:107-191) */
+enum { LP_DD, LP_DI, LP_GD, LP_GI, LP_TD, LP_TI, LP_EMISSION, LP_ENVIRONMENT, LP_SUMS };
+
+static v3 component_ratio(v3 n, v3 d) {
+    return V3(fabsf(d.x) > 1.0e-20f ? n.x / d.x : 0.0f, fabsf(d.y) > 1.0e-20f ? n.y / d.y : 0.0f,
+              fabsf(d.z) > 1.0e-20f ? n.z / d.z : 0.0f);
+}
+
+/* split_scattered_light: diffuse, glossy and the rest (transmission) */
+static void lp_scattered(v3 *lp, v3 c, v3 diffuse_w, v3 glossy_w, int direct) {
+    const v3 dc = vmul(c, diffuse_w), gc = vmul(c, glossy_w);
+    const v3 tc = vsub(vsub(c, dc), gc);
+    lp[direct ? LP_DD : LP_DI] = vadd(lp[direct ? LP_DD : LP_DI], dc);
+    lp[direct ? LP_GD : LP_GI] = vadd(lp[direct ? LP_GD : LP_GI], gc);
+    lp[direct ? LP_TD : LP_TI] = vadd(lp[direct ? LP_TD : LP_TI], tc);
+}
+
+/* emission or environment seen directly, otherwise scattered light */
+static void lp_route(v3 *lp, int pass, v3 c, uint32_t flag, uint32_t bounce, v3 diffuse_w, v3 glossy_w) {
+    if ((flag & FLAG_ANY_PASS) == 0u)
+        lp[pass] = vadd(lp[pass], c);
+    else if ((flag & FLAG_SURFACE_PASS) != 0u)
+        lp_scattered(lp, c, diffuse_w, glossy_w, bounce == 1u);
+}
+
+/* split_nee_light: at the first surface by the BSDF's own component
+ * ratios, deeper by the path weights; every scattering closure evaluated
+ * here is diffuse, so diffuse_f is the whole f and glossy_f is zero */
+static void lp_nee(v3 *lp, v3 c, v3 f, uint32_t bounce, v3 path_diffuse_w, v3 path_glossy_w) {
+    const int direct = bounce == 0u;
+    const v3 dw = direct ? component_ratio(f, f) : path_diffuse_w;
+    const v3 gw = direct ? component_ratio(V3(0, 0, 0), f) : path_glossy_w;
+    lp_scattered(lp, c, dw, gw, direct);
+}
+
 static float light_roulette(v3 u, float random, float inv_threshold) {
     const float maximum = fmaxf(fabsf(u.x), fmaxf(fabsf(u.y), fabsf(u.z)));
     const float probability = maximum * inv_threshold;
@@ -3235,6 +3271,7 @@ typedef struct path_out {
     v3 radiance, normal, albedo;
     float transparency;
     uint64_t rays;
+    v3 lp[LP_SUMS];
 } path_out;
 
 static v3 triangle_world_position_mm(const oinstance *inst, v3 p) {
@@ -3251,6 +3288,9 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
     uint32_t ray_visibility = PSX_VISIBILITY_CAMERA;
     v3 radiance = V3(0, 0, 0), throughput = V3(1, 1, 1);
     v3 sample_normal = V3(0, 0, 0), sample_albedo = V3(0, 0, 0);
+    v3 lp[LP_SUMS];
+    for (int i = 0; i < LP_SUMS; ++i) lp[i] = V3(0, 0, 0);
+    v3 path_diffuse_w = V3(0, 0, 0), path_glossy_w = V3(0, 0, 0);
     float transparency = 0.0f;
     int primary_recorded = 0;
     float previous_bsdf_pdf = 0.0f;
@@ -3332,6 +3372,7 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                 const v3 contribution = film_clamp(vscale(vmul(throughput, light_radiance), mis),
                                                    ps.bounce > 1u ? 1u : 0u, k->clamp_direct, k->clamp_indirect);
                 radiance = vadd(radiance, contribution);
+                lp_route(lp, LP_EMISSION, contribution, ps.flag, ps.bounce, path_diffuse_w, path_glossy_w);
                 ps.transparent_bounce += 1u;
                 ray.tmin = bitsf(intersection_t_offset_bits(best.distance));
                 previous_lamp = found;
@@ -3356,6 +3397,8 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
             } else if (world_visible) {
                 radiance = vadd(radiance, contribution);
             }
+            lp_route(lp, LP_ENVIRONMENT, world_visible ? contribution : V3(0, 0, 0), ps.flag, ps.bounce,
+                     path_diffuse_w, path_glossy_w);
             break;
         }
         log_flag(log, step, 2u);
@@ -3398,6 +3441,7 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
             film_clamp(vscale(vmul(throughput, emitted), emission_weight),
                        ps.bounce > 1u ? 1u : 0u, k->clamp_direct, k->clamp_indirect);
         radiance = vadd(radiance, emission_contribution);
+        lp_route(lp, LP_EMISSION, emission_contribution, ps.flag, ps.bounce, path_diffuse_w, path_glossy_w);
         if (terminate_on_next_surface) break;
         /* Russian roulette */
         {
@@ -3515,6 +3559,7 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                                 const v3 c = film_clamp(vmul(vmul(throughput, surviving), transmittance),
                                                         ps.bounce, k->clamp_direct, k->clamp_indirect);
                                 radiance = vadd(radiance, c);
+                                lp_nee(lp, c, ev.f, ps.bounce, path_diffuse_w, path_glossy_w);
                             }
                         }
                     }
@@ -3583,6 +3628,7 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                             const v3 c = film_clamp(vmul(vmul(throughput, surviving), transmittance), ps.bounce,
                                                     k->clamp_direct, k->clamp_indirect);
                             radiance = vadd(radiance, c);
+                            lp_nee(lp, c, ev.f, ps.bounce, path_diffuse_w, path_glossy_w);
                         }
                     }
                 }
@@ -3598,6 +3644,11 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
         const int singular = (bs.eval.events & EV_SINGULAR) != 0u;
         if (transparent) log_flag(log, step, 64u);
         const uint32_t label = label_from_events(bs.eval.events);
+        if (ps.bounce == 0u && !transparent) {
+            /* src/luisa/path_kernel_surface_scatter.cpp:163-176 */
+            path_diffuse_w = component_ratio(bs.eval.f, bs.eval.f);
+            path_glossy_w = component_ratio(V3(0, 0, 0), bs.eval.f);
+        }
         throughput = vmul(throughput, vdivs(bs.eval.f, bs.eval.pdf));
         if (vany_nan(throughput) || throughput.x < 0.0f || throughput.y < 0.0f || throughput.z < 0.0f)
             break;
@@ -3633,6 +3684,7 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
     out->albedo = sample_albedo;
     out->transparency = transparency;
     out->rays = rays;
+    for (int i = 0; i < LP_SUMS; ++i) out->lp[i] = lp[i];
 }
 
 /* ======================================================================= */
@@ -3646,6 +3698,7 @@ typedef struct job {
     uint32_t row_begin, row_end;
     float *combined, *normal, *albedo;
     uint32_t *sample_count;
+    float *light; /* ORACLE_LIGHT_PASSES float4 per pixel, or NULL */
     uint32_t next_row;
     pthread_mutex_t lock;
     uint64_t rays, nodes, tris;
@@ -3667,6 +3720,20 @@ static void render_pixel(job *j, uint32_t px, uint32_t py, trace_counters *tc, u
         c[0] += r.x; c[1] += r.y; c[2] += r.z; c[3] += o.transparency;
         n[0] += o.normal.x; n[1] += o.normal.y; n[2] += o.normal.z; n[3] += 1.0f;
         a[0] += o.albedo.x; a[1] += o.albedo.y; a[2] += o.albedo.z; a[3] += 1.0f;
+        if (j->light) {
+            /* LightPassBuffer order (src/luisa/path_tracer_internal.h:410-424) */
+            const v3 zero = V3(0, 0, 0);
+            const v3 buf[ORACLE_LIGHT_PASSES] = {o.lp[LP_DD], o.lp[LP_DI], o.lp[LP_GD], o.lp[LP_GI],
+                                                 o.lp[LP_TD], o.lp[LP_TI], zero, zero,
+                                                 o.lp[LP_EMISSION], o.lp[LP_ENVIRONMENT], zero, zero};
+            float *l = j->light + 4u * ORACLE_LIGHT_PASSES * index;
+            for (int b = 0; b < ORACLE_LIGHT_PASSES; ++b) {
+                l[4 * b] += buf[b].x;
+                l[4 * b + 1] += buf[b].y;
+                l[4 * b + 2] += buf[b].z;
+                l[4 * b + 3] += 1.0f;
+            }
+        }
         completed += 1u;
     }
     j->sample_count[index] = completed;
@@ -3695,6 +3762,14 @@ int oracle_render_rows(const oracle_scene *s, const psx_render_settings *rs, uin
                        uint32_t count, uint32_t total, int threads, uint32_t row_begin,
                        uint32_t row_end, float *combined, float *normal, float *albedo,
                        uint32_t *sample_count, oracle_stats *stats) {
+    return oracle_render_light(s, rs, first, count, total, threads, row_begin, row_end, combined, normal, albedo,
+                               sample_count, NULL, stats);
+}
+
+int oracle_render_light(const oracle_scene *s, const psx_render_settings *rs, uint32_t first,
+                        uint32_t count, uint32_t total, int threads, uint32_t row_begin,
+                        uint32_t row_end, float *combined, float *normal, float *albedo,
+                        uint32_t *sample_count, float *light, oracle_stats *stats) {
     if (!s || !rs || total == 0u || (uint64_t)first + count > total) return -1;
     const uint32_t seq = oracle_sequence_size_for_samples(total);
     float *table = (float *)malloc(sizeof(float) * 4u * 256u * seq);
@@ -3715,6 +3790,7 @@ int oracle_render_rows(const oracle_scene *s, const psx_render_settings *rs, uin
     j.normal = normal;
     j.albedo = albedo;
     j.sample_count = sample_count;
+    j.light = light;
     pthread_mutex_init(&j.lock, NULL);
     if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
     if (threads < 1) threads = 1;
@@ -3768,6 +3844,79 @@ void oracle_write_pass(uint32_t kind, uint32_t channels, uint32_t pixel_count, f
             break;
         default:
             break;
+        }
+        for (uint32_t c = 0; c < channels; ++c) out[i * channels + c] = v[c < 3 ? c : 3];
+    }
+}
+
+/* Cycles' safe_divide_even_color (src/luisa/path_tracer_common.cpp:516-548):
+ * channels with a zero denominator take the others' quotients */
+static void safe_divide_even_color(const float *n, const float *d, float *out) {
+    float x = d[0] != 0.0f ? n[0] / d[0] : 0.0f;
+    float y = d[1] != 0.0f ? n[1] / d[1] : 0.0f;
+    float z = d[2] != 0.0f ? n[2] / d[2] : 0.0f;
+    if (d[0] == 0.0f) {
+        if (d[1] == 0.0f) {
+            x = z;
+            y = z;
+        } else if (d[2] == 0.0f) {
+            x = y;
+            z = y;
+        } else {
+            x = 0.5f * (y + z);
+        }
+    } else if (d[1] == 0.0f) {
+        if (d[2] == 0.0f) {
+            y = x;
+            z = x;
+        } else {
+            y = 0.5f * (x + z);
+        }
+    } else if (d[2] == 0.0f) {
+        z = 0.5f * (x + y);
+    }
+    out[0] = x;
+    out[1] = y;
+    out[2] = z;
+}
+
+void oracle_write_light_pass(uint32_t kind, uint32_t channels, uint32_t pixel_count, float exposure,
+                             const float *light, const float *albedo, const uint32_t *sample_count,
+                             float *out) {
+    /* pass kind -> LightPassBuffer slot, and the colour it is divided by */
+    int slot = -1, divisor = -2; /* divisor: -1 albedo, >= 0 a light slot, -2 none */
+    int scale = 1;               /* 1: x exposure / N, 0: / N only */
+    switch (kind) {
+    case PSX_PASS_DIFFUSE_DIRECT: slot = 0; divisor = -1; break;
+    case PSX_PASS_DIFFUSE_INDIRECT: slot = 1; divisor = -1; break;
+    case PSX_PASS_GLOSSY_DIRECT: slot = 2; divisor = 10; break;
+    case PSX_PASS_GLOSSY_INDIRECT: slot = 3; divisor = 10; break;
+    case PSX_PASS_TRANSMISSION_DIRECT: slot = 4; divisor = 11; break;
+    case PSX_PASS_TRANSMISSION_INDIRECT: slot = 5; divisor = 11; break;
+    case PSX_PASS_VOLUME_DIRECT: slot = 6; break;
+    case PSX_PASS_VOLUME_INDIRECT: slot = 7; break;
+    case PSX_PASS_EMISSION: slot = 8; break;
+    case PSX_PASS_ENVIRONMENT: slot = 9; break;
+    case PSX_PASS_GLOSSY_COLOR: slot = 10; scale = 0; break;
+    case PSX_PASS_TRANSMISSION_COLOR: slot = 11; scale = 0; break;
+    default: break;
+    }
+    for (uint32_t i = 0; i < pixel_count; ++i) {
+        float v[4] = {0, 0, 0, 0};
+        if (slot >= 0) {
+            const float *l = light + 4u * ORACLE_LIGHT_PASSES * i;
+            const float *sum = l + 4 * slot;
+            const float den = (float)(sample_count[i] > 1u ? sample_count[i] : 1u);
+            if (divisor != -2) {
+                const float *color = divisor == -1 ? albedo + 4 * i : l + 4 * divisor;
+                safe_divide_even_color(sum, color, v);
+                for (int c = 0; c < 3; ++c) v[c] *= exposure;
+                v[3] = 1.0f;
+            } else if (scale) {
+                for (int c = 0; c < 4; ++c) v[c] = sum[c] * (exposure / den);
+            } else {
+                for (int c = 0; c < 4; ++c) v[c] = sum[c] / den;
+            }
         }
         for (uint32_t c = 0; c < channels; ++c) out[i * channels + c] = v[c < 3 ? c : 3];
     }

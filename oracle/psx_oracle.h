@@ -206,6 +206,32 @@ int oracle_render_rows(const oracle_scene *scene,
                        float *normal, float *albedo, uint32_t *sample_count,
                        oracle_stats *stats);
 
+/* Light-pass sums per pixel, in the reference's LightPassBuffer order
+ * (src/luisa/path_tracer_internal.h:410-424): diffuse direct/indirect,
+ * glossy direct/indirect, transmission direct/indirect, volume
+ * direct/indirect, emission, environment, glossy colour, transmission
+ * colour; each a float4 (rgb sum, sample count in w). */
+#define ORACLE_LIGHT_PASSES 12u
+
+/* oracle_render_rows that also accumulates the light passes into `light`
+ * (ORACLE_LIGHT_PASSES float4 per window pixel; NULL skips them). */
+int oracle_render_light(const oracle_scene *scene,
+                        const psx_render_settings *settings, uint32_t first,
+                        uint32_t count, uint32_t total, int threads,
+                        uint32_t row_begin, uint32_t row_end, float *combined,
+                        float *normal, float *albedo, uint32_t *sample_count,
+                        float *light, oracle_stats *stats);
+
+/* A light pass from the accumulated sums, as write_passes computes it
+ * (src/luisa/path_tracer_session.cpp:150-230): emission, environment and
+ * volume passes x exposure / N, colour passes / N, diffuse / glossy /
+ * transmission passes divided by the albedo / glossy colour / transmission
+ * colour sums channel by channel (safe_divide_even_color) x exposure. */
+void oracle_write_light_pass(uint32_t pass_kind, uint32_t channels,
+                             uint32_t pixel_count, float exposure,
+                             const float *light, const float *albedo,
+                             const uint32_t *sample_count, float *out);
+
 /* Normalise accumulated film into a pass, as write_passes does
  * (src/luisa/path_tracer_session.cpp:71-254).  out holds `channels` floats
  * per pixel. */

@@ -35,6 +35,20 @@ PASS_NORMAL = 2
 PASS_POSITION = 3
 PASS_ALBEDO = 4
 PASS_EMISSION = 5
+PASS_DIFFUSE_DIRECT = 6
+PASS_DIFFUSE_INDIRECT = 7
+PASS_GLOSSY_DIRECT = 8
+PASS_GLOSSY_INDIRECT = 9
+PASS_TRANSMISSION_DIRECT = 10
+PASS_TRANSMISSION_INDIRECT = 11
+PASS_VOLUME_DIRECT = 12
+PASS_VOLUME_INDIRECT = 13
+PASS_GLOSSY_COLOR = 23
+PASS_TRANSMISSION_COLOR = 24
+#: the passes accumulated from light-pass sums (LightPassBuffer)
+LIGHT_PASS_KINDS = (PASS_DIFFUSE_DIRECT, PASS_DIFFUSE_INDIRECT, PASS_GLOSSY_DIRECT, PASS_GLOSSY_INDIRECT,
+                    PASS_TRANSMISSION_DIRECT, PASS_TRANSMISSION_INDIRECT, PASS_VOLUME_DIRECT, PASS_VOLUME_INDIRECT,
+                    PASS_EMISSION, 25, PASS_GLOSSY_COLOR, PASS_TRANSMISSION_COLOR)
 PASS_DENOISING_NORMAL = 18
 PASS_DENOISING_ALBEDO = 19
 PASS_SAMPLE_COUNT = 20

@@ -91,6 +91,10 @@ def load():
     lib.oracle_render_rows.argtypes = [vp, P(abi.RenderSettings), u32, u32, u32, C.c_int,
                                        u32, u32, P(f32), P(f32), P(f32), P(u32),
                                        P(OracleStats)]
+    lib.oracle_render_light.argtypes = [vp, P(abi.RenderSettings), u32, u32, u32, C.c_int, u32, u32,
+                                         P(f32), P(f32), P(f32), P(u32), P(f32), P(OracleStats)]
+    lib.oracle_write_light_pass.argtypes = [u32, u32, u32, f32, P(f32), P(f32), P(u32), P(f32)]
+    lib.oracle_write_light_pass.restype = None
     lib.oracle_write_pass.argtypes = [u32, u32, u32, f32, P(f32), P(f32), P(f32), P(u32),
                                       P(f32)]
     lib.oracle_primary_hits.argtypes = [vp, P(abi.RenderSettings), u32, u32, P(u32)]
@@ -155,13 +159,18 @@ class OracleScene:
         p = np.ascontiguousarray(params, dtype=np.float32)
         return load().oracle_scene_update_parameters(self.handle, material, fptr(p), p.size)
 
-    def render(self, settings, first, count, total, threads=0, film=None, rows=None):
+    def render(self, settings, first, count, total, threads=0, film=None, rows=None, light=False):
         w = settings.window_width or settings.width
         h = settings.window_height or settings.height
         if film is None:
-            film = Film(w, h)
+            film = Film(w, h, light)
         stats = OracleStats()
-        if rows is None:
+        if film.light is not None:
+            r0, r1 = rows if rows is not None else (0, h)
+            rc = load().oracle_render_light(self.handle, C.byref(settings), first, count, total, threads, r0, r1,
+                                            fptr(film.combined), fptr(film.normal), fptr(film.albedo),
+                                            uptr(film.sample_count), fptr(film.light), C.byref(stats))
+        elif rows is None:
             rc = load().oracle_render(self.handle, C.byref(settings), first, count, total,
                                       threads, fptr(film.combined), fptr(film.normal),
                                       fptr(film.albedo), uptr(film.sample_count),
@@ -220,14 +229,24 @@ class OracleScene:
         return out
 
 
+LIGHT_PASSES = 12  # diffuse, glossy, transmission, volume (direct, indirect), emission, environment, colours
+
+
 class Film:
-    def __init__(self, w, h):
+    def __init__(self, w, h, light=False):
         self.width = w
         self.height = h
         self.combined = np.zeros((h, w, 4), dtype=np.float32)
         self.normal = np.zeros((h, w, 4), dtype=np.float32)
         self.albedo = np.zeros((h, w, 4), dtype=np.float32)
         self.sample_count = np.zeros((h, w), dtype=np.uint32)
+        self.light = np.zeros((h, w, LIGHT_PASSES, 4), dtype=np.float32) if light else None
+
+    def write_light_pass(self, kind: int, channels: int, exposure: float = 1.0) -> np.ndarray:
+        out = np.zeros((self.height, self.width, channels), dtype=np.float32)
+        load().oracle_write_light_pass(kind, channels, self.width * self.height, exposure, fptr(self.light),
+                                       fptr(self.albedo), uptr(self.sample_count), fptr(out))
+        return out
 
     def write_pass(self, kind: int, channels: int, exposure: float = 1.0) -> np.ndarray:
         out = np.zeros((self.height, self.width, channels), dtype=np.float32)
