@@ -425,6 +425,13 @@ int psx_session_trace_events(psx_session *session, uint32_t sample_index,
  * float4); out may be NULL to query *out_count. */
 int psx_sobol_table(uint32_t total, float *out, uint32_t *out_count);
 
+/* Raw light-pass sums of a session whose settings request any light pass:
+ * per window pixel, 12 float4 in the order diffuse direct/indirect, glossy
+ * direct/indirect, transmission direct/indirect, volume direct/indirect,
+ * emission, environment, glossy colour, transmission colour (rgb sum,
+ * samples in w).  PSX_ERROR_UNSUPPORTED when no light pass was requested. */
+int psx_session_read_light_passes(psx_session *session, float *out);
+
 /* Per-pixel sampler probe on the device: for each pixel of the window and the
  * given sample index, the u32 bit patterns of sample_4d at `dimension_count`
  * consecutive dimensions starting at `first_dimension`, plus rng_hash. */
@@ -502,7 +509,9 @@ int psx_server_create(psx_backend *backend, const psx_server_options *options,
 void psx_server_destroy(psx_server *server);
 /* Queues samples [first, first+count) of `total` (SampleRange, offset 0) of
  * `settings` on `scene`.  Validation as psx_session_create and
- * render_samples: PSX_ERROR_INVALID_ARGUMENT / PSX_ERROR_RANGE. */
+ * render_samples: PSX_ERROR_INVALID_ARGUMENT / PSX_ERROR_RANGE; settings
+ * that request light passes are PSX_ERROR_UNSUPPORTED (sessions render
+ * them). */
 int psx_server_submit(psx_server *server, psx_scene *scene,
                       const psx_render_settings *settings, uint32_t first,
                       uint32_t count, uint32_t total, uint64_t *out_id);

@@ -275,7 +275,7 @@ EXPORTED_SYMBOLS = [
     "psx_session_render", "psx_session_cancel", "psx_session_stats",
     "psx_session_render_device", "psx_session_read_film",
     "psx_session_primary_hits", "psx_session_trace_events",
-    "psx_sobol_table", "psx_session_sampler_probe",
+    "psx_sobol_table", "psx_session_sampler_probe", "psx_session_read_light_passes",
     "psx_server_create", "psx_server_destroy", "psx_server_submit",
     "psx_server_update_parameters", "psx_server_status", "psx_server_wait",
     "psx_server_read_film", "psx_server_write_passes", "psx_server_cancel",
@@ -317,6 +317,8 @@ def load_library(path: os.PathLike | None = None):
     lib.psx_session_trace_events.argtypes = [vp, u32, u32, P(u32)]
     lib.psx_sobol_table.argtypes = [u32, P(f32), P(u32)]
     lib.psx_session_sampler_probe.argtypes = [vp, u32, u32, u32, u32, P(u32)]
+    lib.psx_session_read_light_passes.argtypes = [vp, P(f32)]
+    lib.psx_session_read_light_passes.restype = C.c_int
     lib.psx_server_create.argtypes = [vp, P(ServerOptions), P(vp)]
     lib.psx_server_destroy.argtypes = [vp]
     lib.psx_server_submit.argtypes = [vp, vp, P(RenderSettings), u32, u32, u32, P(C.c_uint64)]

@@ -260,11 +260,21 @@ struct KernelParams {
     float camera[16];
 };
 
+// Light-pass sums per pixel, in the reference's LightPassBuffer order
+// (src/luisa/path_tracer_internal.h:410-424).
+constexpr uint32_t kLightPasses = 12u;
+enum LightPassSlot : uint32_t {
+    kLpDiffuseDirect, kLpDiffuseIndirect, kLpGlossyDirect, kLpGlossyIndirect, kLpTransmissionDirect,
+    kLpTransmissionIndirect, kLpVolumeDirect, kLpVolumeIndirect, kLpEmission, kLpEnvironment, kLpGlossyColor,
+    kLpTransmissionColor
+};
+
 struct FilmView {
     float *combined;   // float4 per pixel
     float *normal;     // float4 per pixel
     float *albedo;     // float4 per pixel
     uint32_t *sample_count;
+    float *light;      // kLightPasses float4 per pixel, or null
 };
 
 struct Counters {
@@ -322,6 +332,8 @@ struct WavefrontView {
     uint32_t *queue[2];  // trace entries: path << 1 | is_shadow
     uint32_t *list[2];   // shade entries: path
     WavefrontCounters *counters;  // one per iteration
+    float4 *lp;            // light passes: 10 float4 per path (LightSums), field-major, or null
+    float4 *shadow_split;  // light passes: 2 float4 per path (shadow dw, gw), field-major, or null
     uint32_t capacity;            // paths
     uint32_t padded_pixels;       // 64 * tiles of the window
 };

@@ -800,11 +800,21 @@ void Scene::rebuild_emitters(ParamTables &t) {
 // for max_samples_per_dispatch samples of every window pixel.
 struct WavefrontState {
     DeviceBuffer ray_o, ray_d, aux, throughput, mis_normal, radiance, normal, albedo, state0, state1, ident, hit, hit_uv,
-        shadow_o, shadow_d, shadow_q, shadow_l, shadow_t, queue0, queue1, list0, list1, counters;
+        shadow_o, shadow_d, shadow_q, shadow_l, shadow_t, queue0, queue1, list0, list1, counters, lp, shadow_split;
     WavefrontView view{};
 
-    void allocate(uint32_t capacity, uint32_t padded_pixels, uint32_t iterations) {
+    bool light = false;  // light-pass arrays allocated
+
+    void allocate(uint32_t capacity, uint32_t padded_pixels, uint32_t iterations, bool light_passes) {
         const size_t n = capacity;
+        light = light_passes;
+        if (light_passes) {
+            lp.allocate(n * 10u * 16u);
+            shadow_split.allocate(n * 2u * 16u);
+        } else {
+            lp.reset();
+            shadow_split.reset();
+        }
         ray_o.allocate(n * 16u);
         ray_d.allocate(n * 16u);
         aux.allocate(n * 16u);
@@ -853,6 +863,8 @@ struct WavefrontState {
         v.list[0] = list0.as<uint32_t>();
         v.list[1] = list1.as<uint32_t>();
         v.counters = counters.as<WavefrontCounters>();
+        v.lp = light_passes ? lp.as<float4>() : nullptr;
+        v.shadow_split = light_passes ? shadow_split.as<float4>() : nullptr;
         v.capacity = capacity;
         v.padded_pixels = padded_pixels;
     }
@@ -865,6 +877,8 @@ struct Session {
     KernelParams params{};
     uint32_t window_w = 0, window_h = 0;
     DeviceBuffer combined, normal, albedo, sample_count, sobol, filter, counters, spill, tile_cursor;
+    DeviceBuffer light;        // light-pass sums (kLightPasses float4 per pixel) when requested
+    bool light_passes = false;
     WavefrontState wavefront;
     uint32_t wavefront_iterations = 0;
     uint32_t sobol_size = 0;
@@ -886,6 +900,57 @@ struct Session {
 };
 
 namespace {
+
+// LightPassBuffer slot of a pass kind, or -1
+// (src/luisa/path_tracer_internal.h:410-424)
+int light_pass_slot(uint32_t kind) {
+    switch (kind) {
+    case PSX_PASS_DIFFUSE_DIRECT: return kLpDiffuseDirect;
+    case PSX_PASS_DIFFUSE_INDIRECT: return kLpDiffuseIndirect;
+    case PSX_PASS_GLOSSY_DIRECT: return kLpGlossyDirect;
+    case PSX_PASS_GLOSSY_INDIRECT: return kLpGlossyIndirect;
+    case PSX_PASS_TRANSMISSION_DIRECT: return kLpTransmissionDirect;
+    case PSX_PASS_TRANSMISSION_INDIRECT: return kLpTransmissionIndirect;
+    case PSX_PASS_VOLUME_DIRECT: return kLpVolumeDirect;
+    case PSX_PASS_VOLUME_INDIRECT: return kLpVolumeIndirect;
+    case PSX_PASS_EMISSION: return kLpEmission;
+    case PSX_PASS_ENVIRONMENT: return kLpEnvironment;
+    case PSX_PASS_GLOSSY_COLOR: return kLpGlossyColor;
+    case PSX_PASS_TRANSMISSION_COLOR: return kLpTransmissionColor;
+    default: return -1;
+    }
+}
+
+// Cycles' safe_divide_even_color (src/luisa/path_tracer_common.cpp:516-548):
+// channels with a zero denominator take the other channels' quotients.
+void divide_even_color(const float *n, const float *d, float *out) {
+    float x = d[0] != 0.0f ? n[0] / d[0] : 0.0f;
+    float y = d[1] != 0.0f ? n[1] / d[1] : 0.0f;
+    float z = d[2] != 0.0f ? n[2] / d[2] : 0.0f;
+    if (d[0] == 0.0f) {
+        if (d[1] == 0.0f) {
+            x = z;
+            y = z;
+        } else if (d[2] == 0.0f) {
+            x = y;
+            z = y;
+        } else {
+            x = 0.5f * (y + z);
+        }
+    } else if (d[1] == 0.0f) {
+        if (d[2] == 0.0f) {
+            y = x;
+            z = x;
+        } else {
+            y = 0.5f * (x + z);
+        }
+    } else if (d[2] == 0.0f) {
+        z = 0.5f * (x + y);
+    }
+    out[0] = x;
+    out[1] = y;
+    out[2] = z;
+}
 
 // Kernel parameters of one render request: the host part of
 // LuisaRenderSession::initialize (src/luisa/path_tracer_kernel.cpp:44-439).
@@ -967,6 +1032,14 @@ void initialize_session(Session &s, const psx_render_settings &rs) {
     hip_check(hipMemset(s.normal.as<void>(), 0, pixels * 16u));
     hip_check(hipMemset(s.albedo.as<void>(), 0, pixels * 16u));
     hip_check(hipMemset(s.sample_count.as<void>(), 0, pixels * 4u));
+    s.light_passes = false;
+    for (uint32_t kind : s.passes) s.light_passes |= light_pass_slot(kind) >= 0;
+    if (s.light_passes) {
+        if (s.light.bytes() != pixels * kLightPasses * 16u) s.light.allocate(pixels * kLightPasses * 16u);
+        hip_check(hipMemset(s.light.as<void>(), 0, pixels * kLightPasses * 16u));
+    } else {
+        s.light.reset();
+    }
     s.filter.upload(pixel_filter_table(rs.pixel_filter, rs.filter_width));
     if (!s.counters.bytes()) s.counters.allocate(sizeof(Counters));
     s.total_aa = 0;
@@ -994,7 +1067,7 @@ bool valid_settings(const psx_render_settings *rs) {
     for (uint32_t i = 0; i < rs->pass_count; ++i) {
         const uint32_t p = rs->passes[i];
         if (p != PSX_PASS_COMBINED && p != PSX_PASS_NORMAL && p != PSX_PASS_ALBEDO && p != PSX_PASS_SAMPLE_COUNT &&
-            p != PSX_PASS_DENOISING_NORMAL && p != PSX_PASS_DENOISING_ALBEDO)
+            p != PSX_PASS_DENOISING_NORMAL && p != PSX_PASS_DENOISING_ALBEDO && light_pass_slot(p) < 0)
             return false;
     }
     return true;
@@ -1011,8 +1084,9 @@ void prepare_wavefront(Session &s, uint32_t per) {
     if (want > 0x7fffffffull) throw DeviceError{PSX_ERROR_INVALID_ARGUMENT};
     const uint32_t iterations = s.params.max_path_steps + 1u;
     const WavefrontView &v = s.wavefront.view;
-    if (v.capacity < want || v.padded_pixels != padded || s.wavefront_iterations < iterations) {
-        s.wavefront.allocate(static_cast<uint32_t>(want), padded, iterations);
+    if (v.capacity < want || v.padded_pixels != padded || s.wavefront_iterations < iterations ||
+        s.wavefront.light != s.light_passes) {
+        s.wavefront.allocate(static_cast<uint32_t>(want), padded, iterations, s.light_passes);
         s.wavefront_iterations = iterations;
     }
     const size_t events = 2u * static_cast<size_t>(iterations);
@@ -1026,10 +1100,13 @@ void prepare_wavefront(Session &s, uint32_t per) {
 int dispatch_samples(Session &s, uint32_t first, uint32_t count) {
     const psx_backend_options &opt = s.scene->backend->options;
     const uint32_t per = std::max(opt.max_samples_per_dispatch, 1u);
-    const bool wavefront = opt.integrator == PSX_INTEGRATOR_WAVEFRONT;
+    // light passes are accumulated by the wavefront kernels (both integrators
+    // give bit-identical films, so the choice is invisible in the result)
+    const bool wavefront = opt.integrator == PSX_INTEGRATOR_WAVEFRONT || s.light_passes;
     if (wavefront) prepare_wavefront(s, std::min(per, count));
     hip_check(hipMemsetAsync(s.counters.as<void>(), 0, sizeof(Counters), s.stream));
-    FilmView film{s.combined.as<float>(), s.normal.as<float>(), s.albedo.as<float>(), s.sample_count.as<uint32_t>()};
+    FilmView film{s.combined.as<float>(), s.normal.as<float>(), s.albedo.as<float>(), s.sample_count.as<uint32_t>(),
+                  s.light_passes ? s.light.as<float>() : nullptr};
     hip_check(hipEventRecord(s.ev_begin, s.stream));
     uint64_t dispatches = 0;
     double trace_ms = 0.0;
@@ -1087,6 +1164,11 @@ int write_passes(Session &s, const psx_sink *sink) {
     hip_check(hipMemcpy(normal.data(), s.normal.as<void>(), n * 16u, hipMemcpyDeviceToHost));
     hip_check(hipMemcpy(albedo.data(), s.albedo.as<void>(), n * 16u, hipMemcpyDeviceToHost));
     hip_check(hipMemcpy(counts.data(), s.sample_count.as<void>(), n * 4u, hipMemcpyDeviceToHost));
+    std::vector<float> light;
+    if (s.light_passes) {
+        light.resize(n * kLightPasses * 4u);
+        hip_check(hipMemcpy(light.data(), s.light.as<void>(), n * kLightPasses * 16u, hipMemcpyDeviceToHost));
+    }
     for (size_t i = 0; i < n; ++i)
         if (counts[i] != s.rendered) return PSX_ERROR_INCOMPLETE;
     if (sink && sink->begin) sink->begin(sink->user, &s.settings);
@@ -1115,8 +1197,28 @@ int write_passes(Session &s, const psx_sink *sink) {
             case PSX_PASS_SAMPLE_COUNT:
                 v[0] = v[1] = v[2] = v[3] = static_cast<float>(counts[i]);
                 break;
-            default:
+            default: {
+                // light passes (src/luisa/path_tracer_session.cpp:150-230)
+                const int slot = light_pass_slot(kind);
+                if (slot < 0 || light.empty()) break;
+                const float *lp = light.data() + i * kLightPasses * 4u;
+                const float *sum = lp + 4 * slot;
+                const float *color = nullptr;
+                if (slot == kLpDiffuseDirect || slot == kLpDiffuseIndirect) color = albedo.data() + 4 * i;
+                if (slot == kLpGlossyDirect || slot == kLpGlossyIndirect) color = lp + 4 * kLpGlossyColor;
+                if (slot == kLpTransmissionDirect || slot == kLpTransmissionIndirect)
+                    color = lp + 4 * kLpTransmissionColor;
+                if (color) {
+                    divide_even_color(sum, color, v);
+                    for (int c = 0; c < 3; ++c) v[c] *= exposure;
+                    v[3] = 1.0f;
+                } else if (slot == kLpGlossyColor || slot == kLpTransmissionColor) {
+                    for (int c = 0; c < 4; ++c) v[c] = sum[c] / den;
+                } else {
+                    for (int c = 0; c < 4; ++c) v[c] = sum[c] * (exposure / den);
+                }
                 break;
+            }
             }
             for (uint32_t c = 0; c < channels; ++c) pixels[i * channels + c] = v[std::min(c, 3u)];
         }
@@ -2335,6 +2437,18 @@ int psx_session_read_film(psx_session *handle, float *combined, float *normal, f
     });
 }
 
+int psx_session_read_light_passes(psx_session *handle, float *out) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s || !out) return PSX_ERROR_INVALID_ARGUMENT;
+    if (!s->light_passes) return PSX_ERROR_UNSUPPORTED;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->scene->backend->device));
+        const size_t n = static_cast<size_t>(s->window_w) * s->window_h;
+        hip_check(hipMemcpy(out, s->light.as<void>(), n * kLightPasses * 16u, hipMemcpyDeviceToHost));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
 int psx_session_primary_hits(psx_session *handle, uint32_t sample, uint32_t total, uint32_t *out) {
     Session *s = reinterpret_cast<Session *>(handle);
     if (!s || !out || total == 0u || sample >= total) return PSX_ERROR_INVALID_ARGUMENT;
@@ -2425,7 +2539,7 @@ int psx_server_create(psx_backend *b, const psx_server_options *options, psx_ser
         const size_t lanes = wavefront_lanes();
         for (uint32_t i = 0; i < o.streams; ++i) {
             auto a = std::make_unique<Arena>();
-            a->state.allocate(o.path_capacity, 0u, kMaxIterations);
+            a->state.allocate(o.path_capacity, 0u, kMaxIterations, false);
             a->item_capacity = 256u;
             a->items.allocate(a->item_capacity * sizeof(BatchItem));
             a->spill.allocate(lanes * kTraversalStackDepth * sizeof(uint32_t));
@@ -2465,6 +2579,8 @@ int psx_server_submit(psx_server *server, psx_scene *handle, const psx_render_se
     Scene *scene = reinterpret_cast<Scene *>(handle);
     if (!server || !scene || !out_id || !valid_settings(rs)) return PSX_ERROR_INVALID_ARGUMENT;
     if (scene->backend != server->backend) return PSX_ERROR_INVALID_ARGUMENT;
+    for (uint32_t i = 0; i < rs->pass_count; ++i)
+        if (light_pass_slot(rs->passes[i]) >= 0) return PSX_ERROR_UNSUPPORTED;
     if (count == 0u || total == 0u || static_cast<uint64_t>(first) + count > total) return PSX_ERROR_RANGE;
     const uint32_t ww = rs->window_width ? rs->window_width : rs->width;
     const uint32_t wh = rs->window_height ? rs->window_height : rs->height;

@@ -40,12 +40,12 @@ __device__ PathResult integrate(const SceneView &s, const LampView &lv, const Ke
         ++rays;
         if (shadow) {
             if (!shadow_hit(s, k, hit, sr.ray, tdepth, transmittance)) continue;
-            apply_shadow(k, p, sr.L, sr.depth, transmittance);
+            apply_shadow<false>(k, p, sr.L, sr.depth, transmittance, sr.dw, sr.gw);
             shadow = false;
             continue;
         }
         uint32_t *ev = (TRACE && p.step < 8u) ? trace + p.step * 8u : nullptr;
-        cont = path_step<TRACE>(s, lv, k, sobol, p, hit, sr, ev);
+        cont = path_step<TRACE, false>(s, lv, k, sobol, p, hit, sr, ev);
         if (sr.active) {
             shadow = true;
             transmittance = V(1.0f, 1.0f, 1.0f);

@@ -1916,6 +1916,14 @@ constexpr uint32_t PATH_SHADOW_PENDING = 2u;   // a shadow ray awaits its result
 constexpr uint32_t PATH_FINISHING = 4u;        // path ended, pending shadow only
 constexpr uint32_t PATH_DONE = 8u;             // path complete
 
+// Light-pass sums of one sample (src/luisa/path_kernel_setup.cpp:696-733)
+// and the first surface's diffuse / glossy weights that split scattered
+// light (src/luisa/path_kernel_surface_scatter.cpp:163-176).
+struct LightSums {
+    f3 dd, di, gd, gi, td, ti, emission, environment;
+    f3 path_dw, path_gw;
+};
+
 struct PathVars {
     Ray ray;
     uint32_t src_obj, src_prim;
@@ -1927,6 +1935,7 @@ struct PathVars {
     f3 normal, albedo;
     PathState ps;
     uint32_t sample, rng_hash, step, bits;
+    LightSums lp;   // only read and written by the light-pass kernels
 };
 
 struct ShadowRay {
@@ -1936,7 +1945,50 @@ struct ShadowRay {
     uint32_t depth;  // bounce at NEE time (clamp selection)
     uint32_t tdepth; // transparent bounces already taken
     bool active;
+    f3 dw, gw;       // light passes: diffuse / glossy split of the contribution
 };
+
+// light-pass routing (src/luisa/path_tracer_lighting.cpp:107-191)
+__device__ __forceinline__ f3 component_ratio(f3 n, f3 d) {
+    return V(fabsf(d.x) > 1.0e-20f ? n.x / d.x : 0.0f, fabsf(d.y) > 1.0e-20f ? n.y / d.y : 0.0f,
+             fabsf(d.z) > 1.0e-20f ? n.z / d.z : 0.0f);
+}
+
+// split_scattered_light: diffuse, glossy and the rest (transmission)
+__device__ __forceinline__ void lp_scattered(LightSums &lp, f3 c, f3 dw, f3 gw, bool direct) {
+    const f3 dc = c * dw, gc = c * gw;
+    const f3 tc = (c - dc) - gc;
+    if (direct) {
+        lp.dd = lp.dd + dc;
+        lp.gd = lp.gd + gc;
+        lp.td = lp.td + tc;
+    } else {
+        lp.di = lp.di + dc;
+        lp.gi = lp.gi + gc;
+        lp.ti = lp.ti + tc;
+    }
+}
+
+// emission or environment seen directly, otherwise scattered light
+__device__ __forceinline__ void lp_route(LightSums &lp, bool environment, f3 c, uint32_t flag, uint32_t bounce) {
+    if ((flag & FLAG_ANY_PASS) == 0u) {
+        if (environment)
+            lp.environment = lp.environment + c;
+        else
+            lp.emission = lp.emission + c;
+    } else if ((flag & FLAG_SURFACE_PASS) != 0u) {
+        lp_scattered(lp, c, lp.path_dw, lp.path_gw, bounce == 1u);
+    }
+}
+
+// split_nee_light weights: at the first surface the BSDF's own component
+// ratios, deeper the path weights.  Every scattering closure evaluated here
+// is diffuse, so diffuse_f is the whole f and glossy_f is zero.
+__device__ __forceinline__ void nee_split(const LightSums &lp, f3 f, uint32_t bounce, f3 &dw, f3 &gw) {
+    const bool direct = bounce == 0u;
+    dw = direct ? component_ratio(f, f) : lp.path_dw;
+    gw = direct ? component_ratio(V(0.0f, 0.0f, 0.0f), f) : lp.path_gw;
+}
 
 __device__ __forceinline__ TraceQuery path_query(const PathVars &p) {
     TraceQuery q;
@@ -1994,15 +2046,19 @@ __device__ __forceinline__ bool shadow_hit(const SceneView &s, const KernelParam
     return true;
 }
 
+template <bool LP>
 __device__ __forceinline__ void apply_shadow(const KernelParams &k, PathVars &p, f3 L, uint32_t depth,
-                                             f3 transmittance) {
-    if (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f)
-        p.radiance = p.radiance + film_clamp(L * transmittance, depth, k.clamp_direct, k.clamp_indirect);
+                                             f3 transmittance, f3 dw, f3 gw) {
+    if (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f) {
+        const f3 c = film_clamp(L * transmittance, depth, k.clamp_direct, k.clamp_indirect);
+        p.radiance = p.radiance + c;
+        if (LP) lp_scattered(p.lp, c, dw, gw, depth == 0u);
+    }
 }
 
 // One path step.  Returns true when the path continues with p.ray; sr.active
 // is set when a shadow ray must be traced and applied before the next step.
-template <bool TRACE>
+template <bool TRACE, bool LP>
 __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelParams &k, const float4 *sobol,
                           PathVars &p, const Hit &hit, ShadowRay &sr, uint32_t *ev) {
     sr.active = false;
@@ -2056,8 +2112,10 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
             const bool competing = p.ps.bounce > 0u && (p.bits & PATH_PREV_DELTA) == 0u;
             const float mis = forward_light_weight(k, p.prev_pdf, light_pdf, competing, light_pdf > 0.0f);
             const f3 light_radiance = lamp3(l.color) * (l.power * best_eval);
-            p.radiance = p.radiance + film_clamp((p.throughput * light_radiance) * mis, p.ps.bounce > 1u ? 1u : 0u,
-                                                 k.clamp_direct, k.clamp_indirect);
+            const f3 lc = film_clamp((p.throughput * light_radiance) * mis, p.ps.bounce > 1u ? 1u : 0u,
+                                     k.clamp_direct, k.clamp_indirect);
+            p.radiance = p.radiance + lc;
+            if (LP) lp_route(p.lp, false, lc, p.ps.flag, p.ps.bounce);
             p.ps.transparent_bounce += 1u;
             p.ray.tmin = next_float_up(event_distance);
             previous = found;
@@ -2078,6 +2136,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
         } else if (world_visible) {
             p.radiance = p.radiance + c;
         }
+        if (LP) lp_route(p.lp, true, world_visible ? c : V(0.0f, 0.0f, 0.0f), p.ps.flag, p.ps.bounce);
         return false;
     }
     if (TRACE && ev) {
@@ -2111,8 +2170,10 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
         const float light_pdf = valid ? value : 0.0f;
         emission_weight = forward_light_weight(k, p.prev_pdf, light_pdf, competing, light_pdf > 0.0f);
     }
-    p.radiance = p.radiance + film_clamp((p.throughput * cs.emission) * emission_weight,
-                                         p.ps.bounce > 1u ? 1u : 0u, k.clamp_direct, k.clamp_indirect);
+    const f3 emitted = film_clamp((p.throughput * cs.emission) * emission_weight, p.ps.bounce > 1u ? 1u : 0u,
+                                  k.clamp_direct, k.clamp_indirect);
+    p.radiance = p.radiance + emitted;
+    if (LP) lp_route(p.lp, false, emitted, p.ps.flag, p.ps.bounce);
     if (terminate_on_next_surface) return false;
     // Russian roulette (cycles_path_state.h:416-446)
     {
@@ -2240,6 +2301,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                             sr.depth = p.ps.bounce;
                             sr.tdepth = p.ps.transparent_bounce;
                             sr.active = true;
+                            if (LP) nee_split(p.lp, evl.f, p.ps.bounce, sr.dw, sr.gw);
                         }
                     }
                 }
@@ -2284,6 +2346,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                         sr.depth = p.ps.bounce;
                         sr.tdepth = p.ps.transparent_bounce;
                         sr.active = true;
+                        if (LP) nee_split(p.lp, evl.f, p.ps.bounce, sr.dw, sr.gw);
                     }
                 }
             }
@@ -2298,6 +2361,10 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
     const bool transparent = (bs.eval.events & EV_TRANSPARENT) != 0u;
     const bool singular = (bs.eval.events & EV_SINGULAR) != 0u;
     if (TRACE && ev && transparent) ev[0] |= 64u;
+    if (LP && p.ps.bounce == 0u && !transparent) {
+        p.lp.path_dw = component_ratio(bs.eval.f, bs.eval.f);
+        p.lp.path_gw = component_ratio(V(0.0f, 0.0f, 0.0f), bs.eval.f);
+    }
     p.throughput = p.throughput * (bs.eval.f / bs.eval.pdf);
     const f3 t = p.throughput;
     if (any_nan(t) || t.x < 0.0f || t.y < 0.0f || t.z < 0.0f) return false;

@@ -63,6 +63,36 @@ __device__ __forceinline__ void store_path(const WavefrontView &w, uint32_t i, c
     w.ident[i] = make_uint2(p.sample, p.rng_hash);
 }
 
+// light-pass sums of a path, field-major: field f of path i at f * capacity + i
+__device__ __forceinline__ void store_lp(const WavefrontView &w, uint32_t i, const LightSums &lp) {
+    const size_t n = w.capacity;
+    const f3 v[10] = {lp.dd, lp.di, lp.gd, lp.gi, lp.td, lp.ti, lp.emission, lp.environment, lp.path_dw, lp.path_gw};
+#pragma unroll
+    for (int f = 0; f < 10; ++f) w.lp[f * n + i] = make_float4(v[f].x, v[f].y, v[f].z, 0.0f);
+}
+
+__device__ __forceinline__ LightSums load_lp(const WavefrontView &w, uint32_t i) {
+    const size_t n = w.capacity;
+    f3 v[10];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) {
+        const float4 q = w.lp[f * n + i];
+        v[f] = V(q.x, q.y, q.z);
+    }
+    LightSums lp;
+    lp.dd = v[0];
+    lp.di = v[1];
+    lp.gd = v[2];
+    lp.gi = v[3];
+    lp.td = v[4];
+    lp.ti = v[5];
+    lp.emission = v[6];
+    lp.environment = v[7];
+    lp.path_dw = v[8];
+    lp.path_gw = v[9];
+    return lp;
+}
+
 __device__ __forceinline__ PathVars load_path(const WavefrontView &w, uint32_t i) {
     PathVars p;
     const float4 o = w.ray_o[i], d = w.ray_d[i];
@@ -138,6 +168,12 @@ __global__ __launch_bounds__(kShadeBlock) void generate_kernel(KernelParams k, c
             p.step = 0u;
         }
         store_path(w, i, p);
+        if (w.lp) {
+            LightSums z;
+            z.dd = z.di = z.gd = z.gi = z.td = z.ti = z.emission = z.environment = z.path_dw = z.path_gw =
+                V(0.0f, 0.0f, 0.0f);
+            store_lp(w, i, z);
+        }
     }
     // iteration 0 lists every path in order; padding lanes are skip markers
     if (i < total) {
@@ -250,6 +286,7 @@ __global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, K
     }
 }
 
+template <bool LP>
 __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, KernelParams k, const float4 *sobol,
                                                                WavefrontView w, uint32_t it) {
     __shared__ float4 lamp_sphere[kLampCache];
@@ -268,10 +305,18 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, Kern
         if (i < count && list[i] != kSkip) {
             path = list[i];
             PathVars p = load_path(w, path);
+            if (LP) p.lp = load_lp(w, path);
             if ((p.bits & PATH_SHADOW_PENDING) != 0u) {
                 const float4 L = w.shadow_l[path];
                 const float4 tr = w.shadow_t[path];
-                apply_shadow(k, p, V(L.x, L.y, L.z), __float_as_uint(w.shadow_d[path].w), V(tr.x, tr.y, tr.z));
+                f3 dw = V(0.0f, 0.0f, 0.0f), gw = dw;
+                if (LP) {
+                    const float4 a = w.shadow_split[path], b = w.shadow_split[w.capacity + path];
+                    dw = V(a.x, a.y, a.z);
+                    gw = V(b.x, b.y, b.z);
+                }
+                apply_shadow<LP>(k, p, V(L.x, L.y, L.z), __float_as_uint(w.shadow_d[path].w), V(tr.x, tr.y, tr.z),
+                                 dw, gw);
                 p.bits &= ~PATH_SHADOW_PENDING;
             }
             if ((p.bits & PATH_FINISHING) != 0u) {
@@ -289,18 +334,23 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, Kern
                 h.u = uv.x;
                 h.v = uv.y;
                 ShadowRay sr;
-                cont = path_step<false>(s, lv, k, sobol, p, h, sr, nullptr);
+                cont = path_step<false, LP>(s, lv, k, sobol, p, h, sr, nullptr);
                 shadow = sr.active;
                 if (shadow) {
                     w.shadow_o[path] = make_float4(sr.ray.o.x, sr.ray.o.y, sr.ray.o.z, sr.ray.tmax);
                     w.shadow_d[path] = make_float4(sr.ray.d.x, sr.ray.d.y, sr.ray.d.z, __uint_as_float(sr.depth));
                     w.shadow_q[path] = make_uint4(sr.q.src_obj, sr.q.src_prim, sr.q.light_obj, sr.q.light_prim);
                     w.shadow_l[path] = make_float4(sr.L.x, sr.L.y, sr.L.z, __uint_as_float(sr.tdepth));
+                    if (LP) {
+                        w.shadow_split[path] = make_float4(sr.dw.x, sr.dw.y, sr.dw.z, 0.0f);
+                        w.shadow_split[w.capacity + path] = make_float4(sr.gw.x, sr.gw.y, sr.gw.z, 0.0f);
+                    }
                     p.bits |= PATH_SHADOW_PENDING;
                 }
                 if (!cont) p.bits |= shadow ? PATH_FINISHING : PATH_DONE;
             }
             store_path(w, path, p);
+            if (LP) store_lp(w, path, p.lp);
         }
         const uint32_t qe = wave_append(&next->trace_count, cont);
         if (cont) next_queue[qe] = path << 1;
@@ -312,6 +362,29 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, Kern
 }
 
 // in-order film accumulation of the batch, one lane per window pixel
+// adds one sample's light passes to a pixel's sums (w counts samples)
+__device__ __forceinline__ void add_light_passes(float4 *sums, const WavefrontView &w, uint32_t path) {
+    const size_t n = w.capacity;
+    // LightSums fields in LightPassBuffer order; volume and colour passes stay zero
+    const int field[kLightPasses] = {0, 1, 2, 3, 4, 5, -1, -1, 6, 7, -1, -1};
+#pragma unroll
+    for (uint32_t b = 0; b < kLightPasses; ++b) {
+        float4 v = sums[b];
+        if (field[b] >= 0) {
+            const float4 q = w.lp[field[b] * n + path];
+            v.x += q.x;
+            v.y += q.y;
+            v.z += q.z;
+        } else {
+            v.x += 0.0f;
+            v.y += 0.0f;
+            v.z += 0.0f;
+        }
+        v.w += 1.0f;
+        sums[b] = v;
+    }
+}
+
 __global__ __launch_bounds__(256) void accumulate_kernel(KernelParams k, WavefrontView w, FilmView film,
                                                          uint32_t count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -326,8 +399,10 @@ __global__ __launch_bounds__(256) void accumulate_kernel(KernelParams k, Wavefro
     float4 n = reinterpret_cast<float4 *>(film.normal)[index];
     float4 a = reinterpret_cast<float4 *>(film.albedo)[index];
     uint32_t completed = film.sample_count[index];
+    float4 *light = film.light ? reinterpret_cast<float4 *>(film.light) + index * kLightPasses : nullptr;
     for (uint32_t s = 0; s < count; ++s) {
         const uint32_t path = s * w.padded_pixels + i;
+        if (light) add_light_passes(light, w, path);
         const float4 r = w.radiance[path];
         const float4 pn = w.normal[path];
         const float4 pa = w.albedo[path];
@@ -490,7 +565,10 @@ hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const flo
     const void *trace_fn = count_nodes ? reinterpret_cast<const void *>(wf_trace_kernel<true>)
                                        : reinterpret_cast<const void *>(wf_trace_kernel<false>);
     const uint32_t trace_blocks = resident_blocks(trace_fn, kTraceBlock);
-    const uint32_t shade_blocks = resident_blocks(reinterpret_cast<const void *>(wf_shade_kernel), kShadeBlock);
+    const bool lp = w.lp != nullptr;
+    const void *shade_fn = lp ? reinterpret_cast<const void *>(wf_shade_kernel<true>)
+                              : reinterpret_cast<const void *>(wf_shade_kernel<false>);
+    const uint32_t shade_blocks = resident_blocks(shade_fn, kShadeBlock);
     for (uint32_t it = 0; it < iterations; ++it) {
         if (events) (void)hipEventRecord(events[2u * it], stream);
         if (count_nodes)
@@ -500,7 +578,12 @@ hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const flo
             hipLaunchKernelGGL(wf_trace_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, stream, s, k, w, it,
                                counters, spill);
         if (events) (void)hipEventRecord(events[2u * it + 1u], stream);
-        hipLaunchKernelGGL(wf_shade_kernel, dim3(shade_blocks), dim3(kShadeBlock), 0, stream, s, k, sobol, w, it);
+        if (lp)
+            hipLaunchKernelGGL(wf_shade_kernel<true>, dim3(shade_blocks), dim3(kShadeBlock), 0, stream, s, k, sobol,
+                               w, it);
+        else
+            hipLaunchKernelGGL(wf_shade_kernel<false>, dim3(shade_blocks), dim3(kShadeBlock), 0, stream, s, k,
+                               sobol, w, it);
     }
     hipLaunchKernelGGL(accumulate_kernel, dim3((w.padded_pixels + 255u) / 256u), dim3(256), 0, stream, k, w, film,
                        count);
@@ -532,13 +615,13 @@ hipError_t launch_batch_iteration(const SceneView &s, const KernelParams &k, con
     static const uint32_t trace_blocks =
         resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<false>), kTraceBlock);
     static const uint32_t shade_blocks =
-        resident_blocks(reinterpret_cast<const void *>(wf_shade_kernel), kShadeBlock);
+        resident_blocks(reinterpret_cast<const void *>(wf_shade_kernel<false>), kShadeBlock);
     const float f = share > 0.0f && share < 1.0f ? share : 1.0f;
     const uint32_t tb = std::max(1u, static_cast<uint32_t>(static_cast<float>(trace_blocks) * f));
     const uint32_t sb = std::max(1u, static_cast<uint32_t>(static_cast<float>(shade_blocks) * f));
     hipLaunchKernelGGL(wf_trace_kernel<false>, dim3(tb), dim3(kTraceBlock), 0, stream, s, k, w, it, counters,
                        spill);
-    hipLaunchKernelGGL(wf_shade_kernel, dim3(sb), dim3(kShadeBlock), 0, stream, s, k, sobol, w, it);
+    hipLaunchKernelGGL(wf_shade_kernel<false>, dim3(sb), dim3(kShadeBlock), 0, stream, s, k, sobol, w, it);
     return hipGetLastError();
 }
 

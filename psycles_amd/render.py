@@ -180,6 +180,14 @@ class RenderSession:
             albedo.ctypes.data_as(f), counts.ctypes.data_as(C.POINTER(C.c_uint32))))
         return combined, normal, albedo, counts
 
+    def read_light_passes(self) -> np.ndarray:
+        """Raw light-pass sums, (H, W, 12, 4) in LightPassBuffer order."""
+        w, h = self.window
+        out = np.zeros((h, w, 12, 4), dtype=np.float32)
+        abi.check("psx_session_read_light_passes", self._backend.lib.psx_session_read_light_passes(
+            self.handle, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
     def primary_hits(self, sample_index: int, total: int) -> np.ndarray:
         w, h = self.window
         out = np.zeros((h, w, 4), dtype=np.uint32)
