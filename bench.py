@@ -181,13 +181,70 @@ def serving_bench(args) -> int:
     return 0
 
 
+def uneven_bench(args) -> int:
+    """The first `--requests` requests of the uneven-path batch (stacked
+    transparent slabs, every request its own scene, side 128-1024, 1-256 spp,
+    seed 5) submitted together: the render server against one megakernel
+    session per request."""
+    from psycles_amd import abi, scenes, serving
+    from psycles_amd.render import RendererBackend
+    from psycles_amd.server import RenderServer
+
+    def log(msg):
+        print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+    batch = scenes.uneven_batch()[: args.requests]
+    log(f"compiling {len(batch)} uneven-batch scenes")
+    backend = RendererBackend(0, integrator=abi.INTEGRATOR_WAVEFRONT)
+    compiled = [backend.compile_scene(r.snapshot).scene for r in batch]
+    server = RenderServer(backend)
+    warm_settings = batch[0].settings(64)
+    warm = server.submit(compiled[0], warm_settings, 0, 1, 1)
+    warm.wait(-1.0)
+    warm.release()
+    batched = serving.run_uneven_batched(server, compiled, batch, log)
+    server.close()
+    result = {
+        "metric": METRIC,
+        "value": batched["samples_per_s"],
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": batched["makespan_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"uneven-path batch: the first {len(batch)} of 512 requests (seed 5), each its own scene "
+                        "of 1-16 stacked transparent slabs (transparency U[0.5, 0.95]) over diffuse geometry, "
+                        "square side U{128..1024}, U{1..256} spp, max 8 bounces with RR, 32 transparent bounces; "
+                        "the optional homogeneous box is not built",
+            "integrator": "render server (wavefront batches)",
+        },
+        "batched": batched,
+    }
+    if not args.no_plain:
+        plain_backend = RendererBackend(0, integrator=abi.INTEGRATOR_MEGAKERNEL)
+        plain_compiled = [plain_backend.compile_scene(r.snapshot).scene for r in batch]
+        log("running the plain per-request path")
+        plain = serving.run_uneven_plain(plain_backend, plain_compiled, batch, log)
+        result["plain"] = plain
+        result["batched_vs_plain"] = batched["samples_per_s"] / plain["samples_per_s"]
+    print(json.dumps(result))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving"], default="instanced")
-    ap.add_argument("--requests", type=int, default=120, help="serving: requests in the stream")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["serving", "uneven"], default="instanced")
+    ap.add_argument("--requests", type=int, default=None,
+                    help="serving: requests in the stream (120); uneven: leading requests of the batch (16)")
     ap.add_argument("--spp", type=int, default=None, help="samples per pixel (default: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the megakernel frame")
@@ -203,7 +260,11 @@ def main() -> int:
     from psycles_amd.render import RendererBackend
 
     if args.workload == "serving":
+        args.requests = args.requests or 120
         return serving_bench(args)
+    if args.workload == "uneven":
+        args.requests = args.requests or 16
+        return uneven_bench(args)
     w = WORKLOADS[args.workload]
     print(f"bench: building the {args.workload} scene", file=sys.stderr, flush=True)
     snap = getattr(scenes, w["scene"])()

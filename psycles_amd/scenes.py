@@ -651,3 +651,91 @@ def many_light_settings(width=1920, height=1080) -> abi.RenderSettings:
     s.integrator.max_bounces = 6
     s.integrator.use_light_tree = 1
     return s
+
+
+# --------------------------------------------------------------------------
+# uneven-path batch: stacked transparent slabs over diffuse geometry
+# --------------------------------------------------------------------------
+
+@dataclass
+class UnevenRequest:
+    """One request of the uneven-path batch: its own scene, a square
+    resolution and a sample count.  `volume` and `sigma_t` describe the
+    optional homogeneous box the request would carry; this render path has no
+    volume transport, so the box is drawn (the stream stays the same) but not
+    built into the scene."""
+
+    index: int
+    snapshot: SceneSnapshot
+    side: int
+    spp: int
+    slabs: int
+    volume: bool
+    sigma_t: float
+
+    def settings(self, side: int | None = None) -> abi.RenderSettings:
+        return uneven_settings(self.side if side is None else side, self.index)
+
+
+def uneven_slabs(transparencies, albedos, boxes) -> SceneSnapshot:
+    """A stack of 4 x 4 slabs at z = 1.5 ... -0.5 facing the camera, each a
+    mix(diffuse(albedo), transparent(white), transparency) quad, in front of
+    a grey 0.6 ground at y = -1 carrying three diffuse boxes; a 0.3 constant
+    world and a distant sun (angle 0.05 rad, strength 3)."""
+    materials = [diffuse_material((0.6, 0.6, 0.6), name="ground")]
+    meshes = []
+    inst = []
+    g = 20.0
+    gp, gt = quad([[-g, -1.0, -g], [-g, -1.0, g], [g, -1.0, g], [g, -1.0, -g]])
+    meshes.append(MeshData(gp, gt, [0], name="ground"))
+    inst.append(InstanceData(0, identity(), name="ground"))
+    bp, bt = box_mesh((0.8, 0.8, 0.8))
+    meshes.append(MeshData(bp, bt, [0], name="box"))
+    for k, (albedo, (x, z, angle)) in enumerate(zip(albedos[:3], boxes)):
+        materials.append(diffuse_material(albedo, name=f"box{k}"))
+        inst.append(InstanceData(1, translation(x, -0.6, z) @ rotation_y(angle),
+                                 material_overrides=[len(materials) - 1], name=f"box{k}"))
+    n = len(transparencies)
+    for k, t in enumerate(transparencies):
+        z = 1.5 - 2.0 * k / max(n - 1, 1) if n > 1 else 0.5
+        materials.append(mix_diffuse_transparent(albedos[3 + k], t, name=f"slab{k}"))
+        sp, st = quad([[-2.0, -0.9, z], [2.0, -0.9, z], [2.0, 2.0, z], [-2.0, 2.0, z]])
+        meshes.append(MeshData(sp, st, [len(materials) - 1], name=f"slab{k}"))
+        inst.append(InstanceData(len(meshes) - 1, identity(), name=f"slab{k}"))
+    camera = CameraData(look_at((0.0, 0.4, 4.2), (0.0, -0.2, -1.5)), math.radians(45.0), 1.0e-3, 100.0)
+    snap = SceneSnapshot(materials, meshes, inst, camera, (0.3, 0.3, 0.3), name="uneven")
+    snap.lights = [lamp(abi.LIGHT_DISTANT, frame_z((0.4, 1.0, 0.6)), (1.0, 0.95, 0.9), 3.0, angle=0.05)]
+    return snap
+
+
+def uneven_batch(count: int = 512, seed: int = 5) -> list:
+    """The uneven-path batch: per request 1-16 slabs with transparency
+    U[0.5, 0.95], slab and box albedos U[0.2, 0.9]^3, box placements, a
+    homogeneous box with probability 0.5 (sigma_t U[0.1, 2]), a square
+    resolution side U{128..1024} and U{1..256} samples (SplitMix64 `seed`)."""
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(count):
+        slabs = 1 + int(rng.uniform() * 16)
+        transparencies = [0.5 + 0.45 * rng.uniform() for _ in range(slabs)]
+        albedos = [tuple(0.2 + 0.7 * rng.uniform() for _ in range(3)) for _ in range(3 + slabs)]
+        boxes = [(-1.5 + 3.0 * rng.uniform(), -2.5 + 1.5 * rng.uniform(), 90.0 * rng.uniform())
+                 for _ in range(3)]
+        volume = rng.uniform() < 0.5
+        sigma_t = 0.1 + 1.9 * rng.uniform()
+        side = 128 + int(rng.uniform() * 897)
+        spp = 1 + int(rng.uniform() * 256)
+        out.append(UnevenRequest(i, uneven_slabs(transparencies, albedos, boxes), side, spp, slabs,
+                                 volume, sigma_t))
+    return out
+
+
+def uneven_settings(side: int, seed: int) -> abi.RenderSettings:
+    """Square `side` x `side`, seed = request index, max 8 bounces with RR
+    (min_bounces 0), 32 transparent bounces so every slab of a 16-slab stack
+    is crossed by camera and shadow rays."""
+    s = abi.default_settings(side, side)
+    s.seed = seed
+    s.integrator.max_bounces = 8
+    s.integrator.transparent_max_bounces = 32
+    return s

@@ -206,3 +206,52 @@ def run_plain(backend, compiled: dict, stream, log=None) -> dict:
     if log:
         log(f"plain: makespan {out['makespan_s']:.2f} s")
     return out
+
+
+def run_uneven_batched(server, compiled: list, batch, log=None) -> dict:
+    """The uneven-path batch submitted at once (every request its own scene)
+    and rendered to completion by the server; samples/s over the makespan
+    on the server clock, per-request latency from submission."""
+    keep, handles = [], []
+    for req, comp in zip(batch, compiled):
+        s = req.settings()
+        keep.append(s)
+        handles.append(server.submit(comp, s, 0, req.spp, req.spp))
+    latencies, first_submit, last_done = [], None, 0.0
+    for req, h in zip(batch, handles):
+        if not h.wait(-1.0):
+            raise RuntimeError(f"request {req.index} did not complete")
+        st = h.status()
+        latencies.append(st.done_ms - st.submit_ms)
+        first_submit = st.submit_ms if first_submit is None else min(first_submit, st.submit_ms)
+        last_done = max(last_done, st.done_ms)
+        h.release()
+    makespan = (last_done - first_submit) * 1e-3
+    stats = server.stats()
+    samples = sum(r.side * r.side * r.spp for r in batch)
+    if log:
+        log(f"uneven batched: {stats.batches} batches, {stats.rays / 1e9:.2f} G rays, {makespan:.2f} s")
+    return {"samples_per_s": samples / makespan, "mrays_per_s": stats.rays / makespan / 1e6,
+            "makespan_s": makespan, "samples": samples, "requests": len(batch), "batches": int(stats.batches),
+            "latency_ms": _percentiles(latencies), "peak_hbm_gib": stats.peak_hbm / 2**30}
+
+
+def run_uneven_plain(backend, compiled: list, batch, log=None) -> dict:
+    """The same batch one request at a time through megakernel sessions."""
+    rays = 0
+    start = time.perf_counter()
+    latencies = []
+    for req, comp in zip(batch, compiled):
+        s = req.settings()
+        sess = backend.create_session(comp, s)
+        sess.render_device(0, req.spp, req.spp)
+        sess.read_film()
+        rays += sess.stats().rays
+        sess.close()
+        latencies.append((time.perf_counter() - start) * 1e3)
+    makespan = time.perf_counter() - start
+    samples = sum(r.side * r.side * r.spp for r in batch)
+    if log:
+        log(f"uneven plain: {makespan:.2f} s")
+    return {"samples_per_s": samples / makespan, "mrays_per_s": rays / makespan / 1e6, "makespan_s": makespan,
+            "samples": samples, "latency_ms": _percentiles(latencies)}
