@@ -288,10 +288,12 @@ struct Counters {
 // wavefront integrator: per-path state in HBM, structure of arrays indexed by
 // path = sample_in_batch * padded_pixels + tile * 64 + lane (8x8 tiles)
 // ---------------------------------------------------------------------------
-struct WavefrontCounters {
+// trace_count and shade_count share one 8-byte word, so that a shade wave
+// appends to both with a single atomic
+struct alignas(16) WavefrontCounters {
     uint32_t trace_count;   // entries in this iteration's trace queue
-    uint32_t trace_fetch;   // work-fetch cursor of the persistent trace kernel
     uint32_t shade_count;   // paths in this iteration's shade list
+    uint32_t trace_fetch;   // work-fetch cursor of the persistent trace kernel
     uint32_t pad;
 };
 

@@ -1540,6 +1540,12 @@ struct LampView {
     uint32_t cached;       // records held by `sphere` / `bits`
 };
 
+// Whether a path step can meet a light: a lamp (forward hits, NEE) or an
+// entry of the light distribution (NEE, emission MIS).
+__host__ __device__ inline bool scene_has_lights(const SceneView &s) {
+    return s.lamp_count > 0u || s.distribution_count > 0u || s.emitter_count > 0u;
+}
+
 // Every thread of the block must call this (it synchronises the block).
 __device__ __forceinline__ LampView stage_lamps(const SceneView &s, float4 *lds_sphere, uint32_t *lds_bits) {
     const uint32_t n = s.lamp_count < kLampCache ? s.lamp_count : kLampCache;
@@ -2058,7 +2064,10 @@ __device__ __forceinline__ void apply_shadow(const KernelParams &k, PathVars &p,
 
 // One path step.  Returns true when the path continues with p.ray; sr.active
 // is set when a shadow ray must be traced and applied before the next step.
-template <bool TRACE, bool LP>
+// LIGHTS = false compiles a step for scenes with no lamp and an empty light
+// distribution (scene_has_lights): no forward lamp scan, no NEE, emission
+// without MIS, which is what the runtime tests select for such scenes.
+template <bool TRACE, bool LP, bool LIGHTS = true>
 __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelParams &k, const float4 *sobol,
                           PathVars &p, const Hit &hit, ShadowRay &sr, uint32_t *ev) {
     sr.active = false;
@@ -2075,7 +2084,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
     // (src/luisa/path_kernel_pipeline.cpp:79-137,
     //  src/luisa/path_kernel_closest_event.cpp:24-266,
     //  src/luisa/path_kernel_forward_light.cpp:59-117)
-    if (s.lamp_count > 0u) {
+    if (LIGHTS && s.lamp_count > 0u) {
         const float surface_distance = hit.valid ? hit.t : p.ray.tmax;
         const uint32_t ray_visibility = contract_visibility(p.ps.visibility);
         const bool camera_path = (ray_visibility & VIS_CAMERA) != 0u;
@@ -2153,7 +2162,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
     const uint32_t lobe_mask = terminate_after_transparent ? EV_TRANSPARENT : 0x3fu;
     // surface emission and forward MIS (src/luisa/path_kernel_surface_shading.cpp:115-181)
     float emission_weight = 1.0f;
-    if (k.nee && s.emitter_count > 0u && sf.emission_sampling != 0u) {
+    if (LIGHTS && k.nee && s.emitter_count > 0u && sf.emission_sampling != 0u) {
         const bool competing = p.ps.bounce > 0u && (p.bits & PATH_PREV_DELTA) == 0u;
         const f3 oriented = sf.back_facing ? neg(sf.ng) : sf.ng;
         const float area = 0.5f * length(cross(sf.wp1 - sf.wp0, sf.wp2 - sf.wp0));
@@ -2211,7 +2220,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
     }
     const uint32_t rflags = runtime_flags(cs, sf.back_facing);
     // NEE: emissive mesh component (src/luisa/path_kernel_nee_emissive_mesh.cpp:30-298)
-    if (k.nee && (rflags & RT_BSDF_HAS_EVAL) != 0u) {
+    if (LIGHTS && k.nee && (rflags & RT_BSDF_HAS_EVAL) != 0u) {
         const float4 ul = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 1u);
         uint32_t light;
         DistEntry entry;

@@ -31,18 +31,6 @@ constexpr uint32_t kShadeBlock = 256;
 constexpr uint32_t kSkip = 0xffffffffu;  // queue/list entry of a padding lane
 constexpr uint32_t kRefill = 16u;        // idle lanes that trigger a refill
 
-// one atomic per wave; returns this lane's slot (valid where pred is true)
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0ull) return 0u;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(mask)) - 1);
-    uint32_t base = 0u;
-    if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, static_cast<int>(leader), 64);
-    return base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-}
-
 __device__ __forceinline__ void add_counter(unsigned long long *dst, uint32_t value) {
     unsigned long long v = value;
     for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_down(v, offset, 64);
@@ -215,7 +203,7 @@ __global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, K
             const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(idle)) - 1);
             uint32_t base = 0u;
             if (lane == leader) base = atomicAdd(&c->trace_fetch, want);
-            base = __shfl(base, static_cast<int>(leader), 64);
+            base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), static_cast<int>(leader)));
             exhausted = base + want >= count;
             const uint32_t qi = base + static_cast<uint32_t>(__popcll(idle & below));
             if (!active && qi < count) {
@@ -286,12 +274,13 @@ __global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, K
     }
 }
 
-template <bool LP>
+template <bool LP, bool LIGHTS>
 __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, KernelParams k, const float4 *sobol,
                                                                WavefrontView w, uint32_t it) {
-    __shared__ float4 lamp_sphere[kLampCache];
-    __shared__ uint32_t lamp_bits[kLampCache];
-    const LampView lv = stage_lamps(s, lamp_sphere, lamp_bits);
+    __shared__ float4 lamp_sphere[LIGHTS ? kLampCache : 1u];
+    __shared__ uint32_t lamp_bits[LIGHTS ? kLampCache : 1u];
+    LampView lv{lamp_sphere, lamp_bits, 0u};
+    if (LIGHTS) lv = stage_lamps(s, lamp_sphere, lamp_bits);
     const WavefrontCounters *c = w.counters + it;
     WavefrontCounters *next = w.counters + it + 1u;
     const uint32_t count = c->shade_count;
@@ -334,7 +323,7 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, Kern
                 h.u = uv.x;
                 h.v = uv.y;
                 ShadowRay sr;
-                cont = path_step<false, LP>(s, lv, k, sobol, p, h, sr, nullptr);
+                cont = path_step<false, LP, LIGHTS>(s, lv, k, sobol, p, h, sr, nullptr);
                 shadow = sr.active;
                 if (shadow) {
                     w.shadow_o[path] = make_float4(sr.ray.o.x, sr.ray.o.y, sr.ray.o.z, sr.ray.tmax);
@@ -352,12 +341,26 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, Kern
             store_path(w, path, p);
             if (LP) store_lp(w, path, p.lp);
         }
-        const uint32_t qe = wave_append(&next->trace_count, cont);
-        if (cont) next_queue[qe] = path << 1;
-        const uint32_t qs = wave_append(&next->trace_count, shadow);
-        if (shadow) next_queue[qs] = (path << 1) | 1u;
-        const uint32_t ls = wave_append(&next->shade_count, cont || shadow);
-        if (cont || shadow) next_list[ls] = path;
+        // one atomic per wave on the (trace_count, shade_count) word: the
+        // wave's continuation entries, then its shadow entries, then its list
+        // entries (no carry crosses into shade_count: counts stay below 2^32)
+        const unsigned long long cm = __ballot(cont), sm = __ballot(shadow), lm = cm | sm;
+        if (lm != 0ull) {
+            const uint32_t lane = __lane_id();
+            const unsigned long long below = (1ull << lane) - 1ull;
+            const uint32_t nc = static_cast<uint32_t>(__popcll(cm)), ns = static_cast<uint32_t>(__popcll(sm));
+            const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(lm)) - 1);
+            unsigned long long old = 0ull;
+            if (lane == leader)
+                old = atomicAdd(reinterpret_cast<unsigned long long *>(&next->trace_count),
+                                static_cast<unsigned long long>(nc + ns) |
+                                    (static_cast<unsigned long long>(__popcll(lm)) << 32));
+            old = __shfl(old, static_cast<int>(leader), 64);
+            const uint32_t qbase = static_cast<uint32_t>(old), lbase = static_cast<uint32_t>(old >> 32);
+            if (cont) next_queue[qbase + static_cast<uint32_t>(__popcll(cm & below))] = path << 1;
+            if (shadow) next_queue[qbase + nc + static_cast<uint32_t>(__popcll(sm & below))] = (path << 1) | 1u;
+            if (cont || shadow) next_list[lbase + static_cast<uint32_t>(__popcll(lm & below))] = path;
+        }
     }
 }
 
@@ -548,6 +551,15 @@ uint32_t wavefront_padded_pixels(const KernelParams &k) {
     return ((k.window_width + 7u) >> 3) * ((k.window_height + 7u) >> 3) * 64u;
 }
 
+using ShadeFn = void (*)(SceneView, KernelParams, const float4 *, WavefrontView, uint32_t);
+
+// The shade variant for a dispatch: light-pass sums or not, lights in the
+// scene or not.
+ShadeFn shade_kernel(bool lp, bool lights) {
+    if (lp) return lights ? wf_shade_kernel<true, true> : wf_shade_kernel<true, false>;
+    return lights ? wf_shade_kernel<false, true> : wf_shade_kernel<false, false>;
+}
+
 // One dispatch of samples [first, first + count).  When `events` is given, it
 // holds 2 * (k.max_path_steps + 1) events recorded around every trace launch.
 hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
@@ -566,9 +578,9 @@ hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const flo
                                        : reinterpret_cast<const void *>(wf_trace_kernel<false>);
     const uint32_t trace_blocks = resident_blocks(trace_fn, kTraceBlock);
     const bool lp = w.lp != nullptr;
-    const void *shade_fn = lp ? reinterpret_cast<const void *>(wf_shade_kernel<true>)
-                              : reinterpret_cast<const void *>(wf_shade_kernel<false>);
-    const uint32_t shade_blocks = resident_blocks(shade_fn, kShadeBlock);
+    const bool lights = scene_has_lights(s);
+    const ShadeFn shade_fn = shade_kernel(lp, lights);
+    const uint32_t shade_blocks = resident_blocks(reinterpret_cast<const void *>(shade_fn), kShadeBlock);
     for (uint32_t it = 0; it < iterations; ++it) {
         if (events) (void)hipEventRecord(events[2u * it], stream);
         if (count_nodes)
@@ -578,12 +590,7 @@ hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const flo
             hipLaunchKernelGGL(wf_trace_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, stream, s, k, w, it,
                                counters, spill);
         if (events) (void)hipEventRecord(events[2u * it + 1u], stream);
-        if (lp)
-            hipLaunchKernelGGL(wf_shade_kernel<true>, dim3(shade_blocks), dim3(kShadeBlock), 0, stream, s, k, sobol,
-                               w, it);
-        else
-            hipLaunchKernelGGL(wf_shade_kernel<false>, dim3(shade_blocks), dim3(kShadeBlock), 0, stream, s, k,
-                               sobol, w, it);
+        hipLaunchKernelGGL(shade_fn, dim3(shade_blocks), dim3(kShadeBlock), 0, stream, s, k, sobol, w, it);
     }
     hipLaunchKernelGGL(accumulate_kernel, dim3((w.padded_pixels + 255u) / 256u), dim3(256), 0, stream, k, w, film,
                        count);
@@ -614,14 +621,18 @@ hipError_t launch_batch_iteration(const SceneView &s, const KernelParams &k, con
                                   float share, hipStream_t stream) {
     static const uint32_t trace_blocks =
         resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<false>), kTraceBlock);
-    static const uint32_t shade_blocks =
-        resident_blocks(reinterpret_cast<const void *>(wf_shade_kernel<false>), kShadeBlock);
+    const ShadeFn shade_fn = shade_kernel(false, scene_has_lights(s));
+    static const uint32_t shade_blocks_lights =
+        resident_blocks(reinterpret_cast<const void *>(shade_kernel(false, true)), kShadeBlock);
+    static const uint32_t shade_blocks_plain =
+        resident_blocks(reinterpret_cast<const void *>(shade_kernel(false, false)), kShadeBlock);
+    const uint32_t shade_blocks = scene_has_lights(s) ? shade_blocks_lights : shade_blocks_plain;
     const float f = share > 0.0f && share < 1.0f ? share : 1.0f;
     const uint32_t tb = std::max(1u, static_cast<uint32_t>(static_cast<float>(trace_blocks) * f));
     const uint32_t sb = std::max(1u, static_cast<uint32_t>(static_cast<float>(shade_blocks) * f));
     hipLaunchKernelGGL(wf_trace_kernel<false>, dim3(tb), dim3(kTraceBlock), 0, stream, s, k, w, it, counters,
                        spill);
-    hipLaunchKernelGGL(wf_shade_kernel<false>, dim3(sb), dim3(kShadeBlock), 0, stream, s, k, sobol, w, it);
+    hipLaunchKernelGGL(shade_fn, dim3(sb), dim3(kShadeBlock), 0, stream, s, k, sobol, w, it);
     return hipGetLastError();
 }
 
