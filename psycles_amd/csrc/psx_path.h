@@ -310,11 +310,15 @@ struct Stack {
     }
     __device__ __forceinline__ uint32_t pop() {
         --sp;
+        // the LDS slot is read unconditionally (the index is always in the
+        // ring), so the common case stays an LDS access instead of one
+        // generic load that selects between LDS and the spill column
+        uint32_t e = lds[(sp & (kShortStack - 1u)) * lanes + lane];
         if (sp < bottom) {
             bottom = sp;
-            return spill[static_cast<size_t>(sp) * stride];
+            e = spill[static_cast<size_t>(sp) * stride];
         }
-        return lds[(sp & (kShortStack - 1u)) * lanes + lane];
+        return e;
     }
 };
 
@@ -505,6 +509,14 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
     const float by = fmaf(__uint_as_float(head.y), t.inv.y, -t.oi.y);
     const float bz = fmaf(__uint_as_float(head.z), t.inv.z, -t.oi.z);
     const uint32_t ref_arr[4] = {refs.x, refs.y, refs.z, refs.w};
+    // Per axis, the plane nearer along the ray: the scale has the sign of
+    // the inverse direction, lo <= hi, and fma rounds monotonically, so
+    // min(t(lo), t(hi)) is exactly t(lo) for a positive scale and t(hi) for a
+    // negative one.  Swapping the byte words once per node replaces a min and
+    // a max per axis and child.
+    const uint32_t nx = sx < 0.0f ? qxy.y : qxy.x, fx = sx < 0.0f ? qxy.x : qxy.y;
+    const uint32_t ny = sy < 0.0f ? qxy.w : qxy.z, fy = sy < 0.0f ? qxy.z : qxy.w;
+    const uint32_t nz = sz < 0.0f ? qz.y : qz.x, fz = sz < 0.0f ? qz.x : qz.y;
     float key[4];
     uint32_t ent[4];
 #pragma unroll
@@ -513,14 +525,14 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
         // magnitude before quantisation, which rounds outward, and `far` is
         // widened by 1e-5, all far above the rounding of these slab terms
         const int sh = 8 * i;
-        const float t0x = fmaf(static_cast<float>((qxy.x >> sh) & 0xffu), sx, bx);
-        const float t1x = fmaf(static_cast<float>((qxy.y >> sh) & 0xffu), sx, bx);
-        const float t0y = fmaf(static_cast<float>((qxy.z >> sh) & 0xffu), sy, by);
-        const float t1y = fmaf(static_cast<float>((qxy.w >> sh) & 0xffu), sy, by);
-        const float t0z = fmaf(static_cast<float>((qz.x >> sh) & 0xffu), sz, bz);
-        const float t1z = fmaf(static_cast<float>((qz.y >> sh) & 0xffu), sz, bz);
-        const float near = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
-        const float far = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+        const float t0x = fmaf(static_cast<float>((nx >> sh) & 0xffu), sx, bx);
+        const float t1x = fmaf(static_cast<float>((fx >> sh) & 0xffu), sx, bx);
+        const float t0y = fmaf(static_cast<float>((ny >> sh) & 0xffu), sy, by);
+        const float t1y = fmaf(static_cast<float>((fy >> sh) & 0xffu), sy, by);
+        const float t0z = fmaf(static_cast<float>((nz >> sh) & 0xffu), sz, bz);
+        const float t1z = fmaf(static_cast<float>((fz >> sh) & 0xffu), sz, bz);
+        const float near = fmaxf(fmaxf(t0x, t0y), t0z);
+        const float far = fminf(fminf(t1x, t1y), t1z);
         const float far_safe = far * 1.00001f + 1.0e-30f;
         const uint32_t ref = ref_arr[i];
         const bool hit = ref != kEmptySlot && near <= far_safe && far_safe >= t.lower && near <= upper;
