@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define PSX_ABI_VERSION 1u
+#define PSX_ABI_VERSION 2u
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -224,9 +224,10 @@ enum psx_light_flags {
     PSX_LIGHT_FLAG_CAST_SHADOW = 16u
 };
 
-/* contract::CameraDesc (scene.h:347-367), perspective subset. */
+/* contract::CameraDesc (scene.h:347-367): perspective and orthographic
+ * projections (CameraProjection, scene.h:335-339); no depth of field. */
 typedef struct psx_camera {
-    uint32_t projection;             /* 0 = perspective                     */
+    uint32_t projection;             /* 0 perspective, 1 orthographic       */
     uint32_t sensor_fit;             /* 0 horizontal, 1 vertical, 2 auto    */
     float transform[16];
     float field_of_view;
@@ -235,6 +236,7 @@ typedef struct psx_camera {
     float lens_shift_y;
     float near_clip;
     float far_clip;
+    float orthographic_scale;        /* view-plane extent of the fitted axis */
 } psx_camera;
 
 typedef struct psx_scene_desc {

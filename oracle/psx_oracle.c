@@ -827,14 +827,17 @@ typedef struct emitter {
 /* Analytic lamp (LightGpu, src/luisa/path_tracer_types.h:142-168, built by
  * src/luisa/path_tracer_scene.cpp:1723-1870): point, spot or distant. */
 #define LAMP_NORMALIZE 1u
+#define LAMP_ELLIPSE 2u
 #define LAMP_SPHERE 4u
 #define LAMP_USE_MIS 8u
 #define LAMP_FORWARD 32u
+#define LAMP_FULL_SPREAD 64u
 
 typedef struct olamp {
     uint32_t type;
     v3 position, axis_x, axis_y, axis_z, axis_scale, color;
     float power, radius, spot_angle, spot_smooth, angle;
+    float size_u, size_v, spread;   /* area lamps: scaled edge lengths, spread angle */
     uint32_t flags, visibility, max_bounces;
 } olamp;
 
@@ -1275,7 +1278,10 @@ static oracle_tree_emitter lamp_tree_emitter(uint32_t id, const olamp *l) {
         float area = 1.0f;
         if (l->type == PSX_LIGHT_POINT || l->type == PSX_LIGHT_SPOT)
             area = l->radius > 0.0f ? 4.0f * LT_PI * l->radius * l->radius : 4.0f;
-        else if (l->type == PSX_LIGHT_DISTANT && l->angle > 0.0f) {
+        else if (l->type == PSX_LIGHT_AREA) {
+            area = fabsf(l->size_u * l->size_v);
+            if ((l->flags & LAMP_ELLIPSE) != 0u) area *= 0.25f * LT_PI;
+        } else if (l->type == PSX_LIGHT_DISTANT && l->angle > 0.0f) {
             const float s = sinf(0.5f * l->angle);
             area = LT_PI * s * s;
         }
@@ -1287,7 +1293,22 @@ static oracle_tree_emitter lamp_tree_emitter(uint32_t id, const olamp *l) {
     const v3 axis = vscale(l->axis_z, -1.0f);
     e.axis[0] = axis.x; e.axis[1] = axis.y; e.axis[2] = axis.z;
     e.orientation_empty = 0;
-    if (l->type == PSX_LIGHT_POINT) {
+    if (l->type == PSX_LIGHT_AREA) {
+        e.theta_e = 0.5f * l->spread;
+        const v3 hu = vscale(l->axis_x, 0.5f * l->size_u), hv = vscale(l->axis_y, 0.5f * l->size_v);
+        const v3 c[4] = {vadd(l->position, vadd(hu, hv)), vadd(l->position, vsub(hu, hv)),
+                         vadd(l->position, vsub(hv, hu)), vsub(l->position, vadd(hu, hv))};
+        e.bounds_min[0] = e.bounds_max[0] = c[0].x;
+        e.bounds_min[1] = e.bounds_max[1] = c[0].y;
+        e.bounds_min[2] = e.bounds_max[2] = c[0].z;
+        for (int i = 1; i < 4; ++i) {
+            e.bounds_min[0] = fminf(e.bounds_min[0], c[i].x); e.bounds_max[0] = fmaxf(e.bounds_max[0], c[i].x);
+            e.bounds_min[1] = fminf(e.bounds_min[1], c[i].y); e.bounds_max[1] = fmaxf(e.bounds_max[1], c[i].y);
+            e.bounds_min[2] = fminf(e.bounds_min[2], c[i].z); e.bounds_max[2] = fmaxf(e.bounds_max[2], c[i].z);
+        }
+        e.bounds_empty = 0;
+        strength = vscale(strength, 1.0f / LT_PI);
+    } else if (l->type == PSX_LIGHT_POINT) {
         e.theta_o = LT_PI;
         e.theta_e = LT_HALF_PI;
     } else if (l->type == PSX_LIGHT_SPOT) {
@@ -1638,12 +1659,23 @@ static void lamp_from_desc(const psx_light *l, olamp *lp) {
     lp->angle = l->angle;
     lp->visibility = l->visibility_mask;
     lp->max_bounces = l->max_bounces;
+    lp->size_u = l->size * len[0];
+    lp->size_v = (l->size_y > 0.0f ? l->size_y : l->size) * len[1];
+    lp->spread = l->spread;
     const int use_mis = (l->flags & PSX_LIGHT_FLAG_USE_MIS) != 0u;
-    const int effective_mis = use_mis && (l->type == PSX_LIGHT_DISTANT ? l->angle > 0.0f : l->size > 0.0f);
+    int effective_mis;
+    if (l->type == PSX_LIGHT_DISTANT)
+        effective_mis = use_mis && l->angle > 0.0f;
+    else if (l->type == PSX_LIGHT_AREA)
+        effective_mis = use_mis && lp->size_u != 0.0f && lp->size_v != 0.0f && l->spread > 0.0f;
+    else
+        effective_mis = use_mis && l->size > 0.0f;
     lp->flags = ((l->flags & PSX_LIGHT_FLAG_NORMALIZE) ? LAMP_NORMALIZE : 0u) |
+                ((l->flags & PSX_LIGHT_FLAG_ELLIPSE) ? LAMP_ELLIPSE : 0u) |
                 ((l->flags & PSX_LIGHT_FLAG_SPHERE) ? LAMP_SPHERE : 0u) |
                 (effective_mis ? LAMP_USE_MIS : 0u) |
-                (effective_mis && l->type != PSX_LIGHT_DISTANT ? LAMP_FORWARD : 0u);
+                (effective_mis && l->type != PSX_LIGHT_DISTANT ? LAMP_FORWARD : 0u) |
+                (l->type == PSX_LIGHT_AREA && l->spread >= 3.14159265358979f - 1.0e-6f ? LAMP_FULL_SPREAD : 0u);
 }
 
 oracle_scene *oracle_scene_create(const psx_scene_desc *d, int brute_force) {
@@ -1665,8 +1697,7 @@ oracle_scene *oracle_scene_create(const psx_scene_desc *d, int brute_force) {
             s->background.z += l->color[2] * l->power;
             continue;
         }
-        if (l->type != PSX_LIGHT_POINT && l->type != PSX_LIGHT_SPOT && l->type != PSX_LIGHT_DISTANT)
-            continue;
+        if (l->type > PSX_LIGHT_DISTANT) continue;
         lamp_from_desc(l, &s->lamps[s->lamp_count++]);
     }
     s->material_count = d->material_count;
@@ -2604,6 +2635,8 @@ typedef struct kparams {
     float clamp_direct, clamp_indirect;
     float light_inv_rr_threshold;
     float tan_h, tan_v, shift_x, shift_y, near_clip, far_clip;
+    uint32_t projection;   /* 0 perspective, 1 orthographic */
+    float ortho_span;      /* vertical view-plane span of the orthographic camera */
     float pass_alpha_threshold;
     int nee;
     int mis_mode; /* psx_direct_light_sampling actually used */
@@ -2653,6 +2686,9 @@ static void setup_params(const oracle_scene *s, const psx_render_settings *rs,
     k->tan_v = hfit ? tanf(s->camera.horizontal_field_of_view * 0.5f) / aspect
                     : tanf(s->camera.field_of_view * 0.5f);
     k->tan_h = k->tan_v * aspect;
+    /* orthographic_viewplane_span (include/psycles/luisa/camera_sampling.h:50-56) */
+    k->projection = s->camera.projection;
+    k->ortho_span = hfit ? s->camera.orthographic_scale / aspect : s->camera.orthographic_scale;
     k->shift_x = s->camera.lens_shift_x * (hfit ? 1.0f : 1.0f / aspect);
     k->shift_y = s->camera.lens_shift_y * (hfit ? aspect : 1.0f);
     k->near_clip = s->camera.near_clip;
@@ -3119,16 +3155,309 @@ static lamp_hit_t intersect_lamp(const olamp *l, v3 ro, v3 rd, float tmin, float
     return h;
 }
 
+/* ---- area lamps (src/luisa/area_light_sampling.cpp,
+ * include/psycles/luisa/analytic_light_sampling.h:605-846) ---------------- */
+static float al_sdiv(float a, float b) { return b != 0.0f ? a / b : 0.0f; }
+
+/* area_spread_attenuation (:796-846) */
+static float area_spread_attenuation(v3 dir_to_light, v3 light_normal, float spread) {
+    const float cosine = fmaxf(vdot(light_normal, vneg(dir_to_light)), 0.0f);
+    const float half_spread = 0.5f * fmaxf(spread, 0.0f);
+    const float sine_angle = sqrtf(fmaxf(1.0f - cosine * cosine, 0.0f));
+    const float tangent_angle = sine_angle / fmaxf(cosine, 1.0e-20f);
+    const float tangent_spread = tanf(half_spread);
+    const float normalization = half_spread > 0.05f
+                                    ? 1.0f / fmaxf(tangent_spread - half_spread, 1.0e-20f)
+                                    : 3.0f / fmaxf(half_spread * half_spread * half_spread, 1.0e-20f);
+    const float finite_spread = fmaxf((tangent_spread - tangent_angle) * normalization, 0.0f);
+    const float zero_spread = tangent_angle > 1.0e-5f ? 0.0f : AL_PI;
+    const float narrowed = half_spread <= 0.0f ? zero_spread : finite_spread;
+    return spread >= AL_PI - 1.0e-6f ? 1.0f : narrowed;
+}
+
+/* rectangle_solid_angle_context (:605-690): the rectangle seen from `ref`
+ * in the frame of its edges (Urena, Fajardo, King, "An area-preserving
+ * parametrization for spherical rectangles", EGSR 2013) */
+typedef struct rect_ctx {
+    v3 z;
+    float z0, x0, x1, y0, y1;
+    float e[4], a[4];
+    float solid_angle, pdf;
+} rect_ctx;
+
+static rect_ctx rect_solid_angle_context(v3 ref, v3 center, v3 au, float lu, v3 av, float lv) {
+    rect_ctx c;
+    v3 z = vcross(au, av);
+    const v3 off = vsub(center, ref);
+    float z0 = vdot(off, z);
+    if (z0 > 0.0f) {
+        z = vneg(z);
+        z0 = -z0;
+    }
+    c.z = z;
+    c.z0 = z0;
+    const float cu = vdot(off, au), cv = vdot(off, av);
+    c.x0 = cu - 0.5f * lu;
+    c.x1 = cu + 0.5f * lu;
+    c.y0 = cv - 0.5f * lv;
+    c.y1 = cv + 0.5f * lv;
+    const float raw[4] = {-c.y0, c.x1, c.y1, -c.x0};
+    for (int i = 0; i < 4; ++i) c.e[i] = raw[i] / sqrtf(raw[i] * raw[i] + z0 * z0);
+    for (int i = 0; i < 4; ++i) {
+        const float p = -c.e[i] * c.e[(i + 1) & 3];
+        c.a[i] = asinf(fminf(fmaxf(p, -1.0f), 1.0f));
+    }
+    c.solid_angle = -(c.a[0] + c.a[1] + c.a[2] + c.a[3]);
+    const float min_e2 = fminf(fminf(c.e[0] * c.e[0], c.e[1] * c.e[1]), fminf(c.e[2] * c.e[2], c.e[3] * c.e[3]));
+    const float dist = vlength(off);
+    const float planar_pdf = al_sdiv(-(dist * dist * dist), z0 * lu * lv);
+    const int planar = (c.solid_angle < 1.0e-5f) || (min_e2 > 0.99999f);
+    c.pdf = planar ? planar_pdf : 1.0f / c.solid_angle;
+    return c;
+}
+
+/* sample_rectangle_solid_angle (:716-794) */
+static v3 sample_rect_solid_angle(v3 ref, v3 center, v3 au, float lu, v3 av, float lv, v2 rnd, float *pdf) {
+    const rect_ctx c = rect_solid_angle_context(ref, center, au, lu, av, lv);
+    const float b0 = c.e[0], b1 = c.e[2];
+    const float b0sq = b0 * b0;
+    const float alpha = rnd.x * c.solid_angle + c.a[2] + c.a[3];
+    const float fu = al_sdiv(cosf(alpha) * b0 + b1, sinf(alpha));
+    float cos_u = copysignf(1.0f / sqrtf(fu * fu + b0sq), fu);
+    cos_u = fminf(fmaxf(cos_u, -1.0f), 1.0f);
+    float su = -(cos_u * c.z0) / fmaxf(sqrtf(fmaxf(1.0f - cos_u * cos_u, 0.0f)), 1.0e-7f);
+    su = fminf(fmaxf(su, c.x0), c.x1);
+    const float du2 = su * su + c.z0 * c.z0;
+    const float h0 = c.y0 / sqrtf(du2 + c.y0 * c.y0);
+    const float h1 = c.y1 / sqrtf(du2 + c.y1 * c.y1);
+    const float hv = h0 + rnd.y * (h1 - h0);
+    const float hv2 = hv * hv;
+    const float sv = hv2 < 1.0f - 1.0e-6f ? hv * sqrtf(du2 / (1.0f - hv2)) : c.y1;
+    *pdf = c.pdf;
+    return vadd(vadd(vadd(ref, vscale(au, su)), vscale(av, sv)), vscale(c.z, c.z0));
+}
+
+static float area_value(const olamp *l) {
+    float area = l->size_u * l->size_v;
+    if ((l->flags & LAMP_ELLIPSE) != 0u) area *= 0.25f * AL_PI;
+    return area;
+}
+
+static float area_inverse_area(const olamp *l) {
+    const float area = area_value(l);
+    return ((l->flags & LAMP_NORMALIZE) != 0u && area != 0.0f) ? al_sdiv(1.0f, area) : 1.0f;
+}
+
+static float area_eval_factor(const olamp *l, v3 direction, v3 normal) {
+    return AL_INV_PI * area_inverse_area(l) * area_spread_attenuation(direction, normal, l->spread);
+}
+
+static float area_tangent_half_spread(float spread) {
+    return spread == AL_PI ? FLT_MAX : tanf(0.5f * fmaxf(spread, 0.0f));
+}
+
+static float area_to_solid_angle(v3 normal, v3 direction, float distance_squared) {
+    const float cosine = vdot(normal, vneg(direction));
+    return cosine > 0.0f ? distance_squared / cosine : 0.0f;
+}
+
+/* clamp_to_spread (area_light_sampling.cpp:179-420): the part of the lamp
+ * that the spread cone about the reference point's foot can see, as a
+ * rectangle, a circle or the original shape */
+typedef struct area_shape {
+    int valid, rectangle;
+    v3 center, au, av;
+    float lu, lv;
+} area_shape;
+
+static area_shape area_original_shape(const olamp *l) {
+    area_shape r;
+    r.valid = 1;
+    r.rectangle = (l->flags & LAMP_ELLIPSE) == 0u;
+    r.center = l->position;
+    r.au = l->axis_x;
+    r.av = l->axis_y;
+    r.lu = l->size_u;
+    r.lv = l->size_v;
+    return r;
+}
+
+static area_shape area_clamp_to_spread(const olamp *l, v3 ref, v3 normal, float tangent) {
+    area_shape r = area_original_shape(l);
+    const float plane_distance = vdot(normal, vsub(ref, l->position));
+    const v3 closest = vsub(ref, vscale(normal, plane_distance));
+    const float spread_radius = plane_distance * tangent;
+    const float spread_u = vdot(l->axis_x, vsub(closest, l->position));
+    const float spread_v = vdot(l->axis_y, vsub(closest, l->position));
+    const int round = !r.rectangle && r.lu == r.lv;
+    int sample_spread = spread_radius == 0.0f;
+    int resolved = 0;
+    if (round && !sample_spread) {
+        const float center_distance = sqrtf(spread_u * spread_u + spread_v * spread_v);
+        const float light_radius = 0.5f * r.lu;
+        if (center_distance >= light_radius + spread_radius) {
+            r.valid = 0;
+        } else {
+            sample_spread = (center_distance <= fabsf(light_radius - spread_radius)) && (spread_radius < light_radius);
+            if (center_distance > fabsf(light_radius - spread_radius)) {
+                const float rect_u = light_radius + spread_radius - center_distance;
+                float rect_v;
+                if (fabsf(light_radius * light_radius - spread_radius * spread_radius) >=
+                    center_distance * center_distance) {
+                    rect_v = 2.0f * fminf(light_radius, spread_radius);
+                } else {
+                    const float offset = center_distance +
+                                         (spread_radius * spread_radius - light_radius * light_radius) /
+                                             center_distance;
+                    rect_v = sqrtf(4.0f * spread_radius * spread_radius - offset * offset);
+                }
+                const float rect_area = rect_u * rect_v;
+                const float light_area = AL_PI * light_radius * light_radius;
+                const float spread_area = AL_PI * spread_radius * spread_radius;
+                if (rect_area < fminf(light_area, spread_area)) {
+                    r.rectangle = 1;
+                    r.au = safe_normalize(vsub(l->position, closest), l->axis_x);
+                    r.av = vcross(normal, r.au);
+                    r.lu = rect_u;
+                    r.lv = rect_v;
+                    r.center = vscale(vadd(vadd(l->position, closest), vscale(r.au, spread_radius - light_radius)),
+                                      0.5f);
+                    resolved = 1;
+                } else {
+                    sample_spread = spread_area < light_area;
+                }
+            }
+        }
+    } else if (!round && !sample_spread) {
+        const float min_u = fmaxf(spread_u - spread_radius, -0.5f * r.lu);
+        const float max_u = fminf(spread_u + spread_radius, 0.5f * r.lu);
+        const float min_v = fmaxf(spread_v - spread_radius, -0.5f * r.lv);
+        const float max_v = fminf(spread_v + spread_radius, 0.5f * r.lv);
+        if (min_u >= max_u || min_v >= max_v) {
+            r.valid = 0;
+        } else {
+            const float rect_u = max_u - min_u, rect_v = max_v - min_v;
+            const float rect_area = rect_u * rect_v;
+            const float ellipse_area = r.rectangle ? FLT_MAX : 0.25f * AL_PI * r.lu * r.lv;
+            const float spread_area = AL_PI * spread_radius * spread_radius;
+            if (r.rectangle || rect_area < fminf(ellipse_area, spread_area)) {
+                const float cu = 0.5f * (min_u + max_u), cv = 0.5f * (min_v + max_v);
+                r.center = vadd(r.center, vadd(vscale(r.au, cu), vscale(r.av, cv)));
+                r.lu = rect_u;
+                r.lv = rect_v;
+                r.rectangle = 1;
+                resolved = 1;
+            } else {
+                r.rectangle = 0;
+                sample_spread = spread_area < ellipse_area;
+            }
+        }
+    }
+    if (r.valid && !resolved && sample_spread) {
+        r.rectangle = 0;
+        r.center = vadd(l->position, vadd(vscale(r.au, spread_u), vscale(r.av, spread_v)));
+        r.lu = 2.0f * spread_radius;
+        r.lv = 2.0f * spread_radius;
+    }
+    return r;
+}
+
+static v3 area_sample_ellipse(v3 center, v3 au, float lu, v3 av, float lv, v2 rnd) {
+    const v2 d = sample_uniform_disk(rnd);
+    return vadd(center, vadd(vscale(au, 0.5f * lu * d.x), vscale(av, 0.5f * lv * d.y)));
+}
+
+/* AreaLightSampling::from_position (:494-646): surface NEE */
+static lamp_sample_t sample_area_light(const olamp *l, v3 ref, v2 rnd) {
+    lamp_sample_t s;
+    memset(&s, 0, sizeof(s));
+    const v3 normal = vneg(l->axis_z);
+    s.position = l->position;
+    s.normal = normal;
+    if (!(vdot(vsub(l->position, ref), normal) <= 0.0f)) return s;
+    const float tangent = area_tangent_half_spread(l->spread);
+    area_shape sh = area_original_shape(l);
+    if ((l->flags & LAMP_FULL_SPREAD) == 0u) sh = area_clamp_to_spread(l, ref, normal, tangent);
+    if (!sh.valid) return s;
+    v3 position = sh.center;
+    float pdf = 0.0f;
+    if (sh.rectangle) {
+        position = sample_rect_solid_angle(ref, sh.center, sh.au, sh.lu, sh.av, sh.lv, rnd, &pdf);
+    } else if (tangent == 0.0f) {
+        pdf = 1.0f;
+    } else {
+        position = area_sample_ellipse(sh.center, sh.au, sh.lu, sh.av, sh.lv, rnd);
+        pdf = 4.0f * AL_INV_PI / (sh.lu * sh.lv);
+    }
+    const v3 off = vsub(position, ref);
+    const float d2 = vdot(off, off);
+    const float d = sqrtf(fmaxf(d2, 0.0f));
+    const v3 dir = vdivs(off, fmaxf(d, 1.0e-30f));
+    if (!sh.rectangle && tangent > 0.0f) pdf *= area_to_solid_angle(normal, dir, d2);
+    const float factor = area_eval_factor(l, dir, normal);
+    const v3 inplane = vsub(position, l->position);
+    const float lu = vdot(inplane, l->axis_x), lv = vdot(inplane, l->axis_y);
+    const float eu = (0.5f + 1.0e-7f) * l->size_u + 1.0e-6f;
+    const float ev = (0.5f + 1.0e-7f) * l->size_v + 1.0e-6f;
+    const int inside_rect = fabsf(lu) <= eu && fabsf(lv) <= ev;
+    const int inside_ellipse = lu * lu / (eu * eu) + lv * lv / (ev * ev) <= 1.0f;
+    s.valid = ((l->flags & LAMP_ELLIPSE) ? inside_ellipse : inside_rect) && factor > 0.0f;
+    s.direction = dir;
+    s.position = position;
+    s.distance = d;
+    s.pdf = pdf;
+    s.eval = factor;
+    return s;
+}
+
+/* intersect_area (analytic_light_intersection.h:271-353) with
+ * AreaLightSampling::from_intersection (area_light_sampling.cpp:648-726) */
+static lamp_hit_t intersect_area_lamp(const olamp *l, v3 ro, v3 rd, float tmin, float tmax) {
+    lamp_hit_t h;
+    const v3 normal = vneg(l->axis_z);
+    const float dn = vdot(rd, normal);
+    const float distance = al_sdiv(vdot(vsub(l->position, ro), normal), dn);
+    const v3 position = vadd(ro, vscale(rd, distance));
+    const v3 inplane = vsub(position, l->position);
+    const float u = al_sdiv(vdot(inplane, l->axis_x), l->size_u);
+    const float v = al_sdiv(vdot(inplane, l->axis_y), l->size_v);
+    const int inside_rect = u >= -0.5f && u <= 0.5f && v >= -0.5f && v <= 0.5f;
+    const int inside = inside_rect && (!(l->flags & LAMP_ELLIPSE) || u * u + v * v <= 0.25f);
+    const float tangent = area_tangent_half_spread(l->spread);
+    area_shape sh = area_original_shape(l);
+    if ((l->flags & LAMP_FULL_SPREAD) == 0u) sh = area_clamp_to_spread(l, ro, normal, tangent);
+    float pdf = 0.0f, factor = 0.0f;
+    int eval_valid = 0;
+    if (sh.valid) {
+        if (sh.rectangle) {
+            pdf = rect_solid_angle_context(ro, sh.center, sh.au, sh.lu, sh.av, sh.lv).pdf;
+        } else if (tangent == 0.0f) {
+            pdf = 1.0f;
+        } else {
+            pdf = 4.0f * AL_INV_PI / (sh.lu * sh.lv);
+            pdf *= area_to_solid_angle(normal, rd, distance * distance);
+        }
+        factor = area_eval_factor(l, rd, normal);
+        eval_valid = area_value(l) > 0.0f && factor > 0.0f;
+    }
+    h.valid = dn < 0.0f && distance > tmin && distance < tmax && inside && eval_valid;
+    h.distance = distance;
+    h.pdf = pdf;
+    h.eval = factor;
+    return h;
+}
+
 int oracle_sample_lamp(const psx_light *light, const float reference[3], const float shading_normal[3],
                        int has_transmission, const float random[2], float out[13]) {
-    if (!light || light->type > PSX_LIGHT_DISTANT || light->type == PSX_LIGHT_AREA) return -1;
+    if (!light || light->type > PSX_LIGHT_DISTANT) return -1;
     olamp l;
     lamp_from_desc(light, &l);
     const v2 rnd = {random[0], random[1]};
-    const lamp_sample_t ls = l.type == PSX_LIGHT_DISTANT
-                                 ? sample_distant_light(&l, rnd)
-                                 : sample_point_or_spot(&l, load3(reference), load3(shading_normal),
-                                                        has_transmission, rnd);
+    const lamp_sample_t ls = l.type == PSX_LIGHT_DISTANT ? sample_distant_light(&l, rnd)
+                             : l.type == PSX_LIGHT_AREA  ? sample_area_light(&l, load3(reference), rnd)
+                                                         : sample_point_or_spot(&l, load3(reference),
+                                                                                load3(shading_normal),
+                                                                                has_transmission, rnd);
     const float v[13] = {ls.direction.x, ls.direction.y, ls.direction.z, ls.pdf, ls.position.x, ls.position.y,
                          ls.position.z, ls.distance, ls.normal.x, ls.normal.y, ls.normal.z, ls.eval,
                          ls.valid ? 1.0f : 0.0f};
@@ -3138,11 +3467,14 @@ int oracle_sample_lamp(const psx_light *light, const float reference[3], const f
 
 int oracle_intersect_lamp(const psx_light *light, const float origin[3], const float direction[3], float tmin,
                           float tmax, const float previous_normal[3], int had_transmission, float out[4]) {
-    if (!light || (light->type != PSX_LIGHT_POINT && light->type != PSX_LIGHT_SPOT)) return -1;
+    if (!light || (light->type != PSX_LIGHT_POINT && light->type != PSX_LIGHT_SPOT && light->type != PSX_LIGHT_AREA))
+        return -1;
     olamp l;
     lamp_from_desc(light, &l);
-    const lamp_hit_t h = intersect_lamp(&l, load3(origin), load3(direction), tmin, tmax, load3(previous_normal),
-                                        had_transmission);
+    const lamp_hit_t h = l.type == PSX_LIGHT_AREA
+                             ? intersect_area_lamp(&l, load3(origin), load3(direction), tmin, tmax)
+                             : intersect_lamp(&l, load3(origin), load3(direction), tmin, tmax,
+                                              load3(previous_normal), had_transmission);
     out[0] = h.valid ? 1.0f : 0.0f;
     out[1] = h.distance;
     out[2] = h.pdf;
@@ -3237,8 +3569,8 @@ typedef struct camera_ray_t {
     uint32_t rng_hash;
 } camera_ray_t;
 
-/* sample_camera_dimensions + construct_camera_ray (perspective)
- * (src/luisa/path_tracer_camera.cpp:10-244) */
+/* sample_camera_dimensions + construct_camera_ray (perspective and
+ * orthographic; src/luisa/path_tracer_camera.cpp:10-244) */
 static camera_ray_t camera_ray(const kparams *k, uint32_t full_x, uint32_t full_y,
                                uint32_t sample_index) {
     camera_ray_t r;
@@ -3253,9 +3585,16 @@ static camera_ray_t camera_ray(const kparams *k, uint32_t full_x, uint32_t full_
     const float height = (float)k->full_height;
     const float sx = 2.0f * ((float)full_x + jx) / width - 1.0f + 2.0f * k->shift_x;
     const float sy = 1.0f - 2.0f * ((float)full_y + jy) / height + 2.0f * k->shift_y;
-    const v3 local_dir = vnormalize(V3(sx * k->tan_h, sy * k->tan_v, -1.0f));
-    const float clip_cos = -local_dir.z;
-    const v3 world_o = xform_point(&k->camera, V3(0, 0, 0));
+    v3 local_o = V3(0, 0, 0), local_dir = V3(0, 0, -1);
+    float clip_cos = 1.0f;
+    if (k->projection == 1u) {
+        /* orthographic (src/luisa/path_tracer_camera.cpp:127-143) */
+        local_o = V3(sx * k->ortho_span * (width / height) * 0.5f, sy * k->ortho_span * 0.5f, 0.0f);
+    } else {
+        local_dir = vnormalize(V3(sx * k->tan_h, sy * k->tan_v, -1.0f));
+        clip_cos = -local_dir.z;
+    }
+    const v3 world_o = xform_point(&k->camera, local_o);
     const v3 world_d = xform_dir(&k->camera, local_dir);
     const v3 d = safe_normalize(world_d, V3(0, 0, -1));
     float clip[2];
@@ -3351,8 +3690,10 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                     const int eligible = (l->visibility & ray_visibility) != 0u &&
                                          (camera_path || (l->flags & LAMP_USE_MIS) != 0u) && li != previous_lamp;
                     if (!eligible) continue;
-                    const lamp_hit_t c = intersect_lamp(l, ray.o, ray.d, ray.tmin, event_distance,
-                                                        previous_mis_origin_normal, had_transmission);
+                    const lamp_hit_t c = l->type == PSX_LIGHT_AREA
+                                             ? intersect_area_lamp(l, ray.o, ray.d, ray.tmin, event_distance)
+                                             : intersect_lamp(l, ray.o, ray.d, ray.tmin, event_distance,
+                                                              previous_mis_origin_normal, had_transmission);
                     if (c.valid) {
                         found = li;
                         best = c;
@@ -3577,7 +3918,9 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
             const int distant = l->type == PSX_LIGHT_DISTANT;
             const int has_transmission = (runtime_flags & RT_BSDF_HAS_TRANSMISSION) != 0u;
             const lamp_sample_t ls = distant ? sample_distant_light(l, rnd)
-                                             : sample_point_or_spot(l, sf.position, sf.ns, has_transmission, rnd);
+                                     : l->type == PSX_LIGHT_AREA
+                                         ? sample_area_light(l, sf.position, rnd)
+                                         : sample_point_or_spot(l, sf.position, sf.ns, has_transmission, rnd);
             const v3 light_radiance = vscale(l->color, l->power * ls.eval);
             const float light_pdf = ls.pdf * light->selection_pdf;
             if (!reached_max_bounces && ls.valid && light_pdf > 0.0f) {

@@ -10,7 +10,7 @@ import ctypes as C
 import os
 from pathlib import Path
 
-PSX_ABI_VERSION = 1
+PSX_ABI_VERSION = 2
 
 PSX_OK = 0
 PSX_ERROR_INVALID_ARGUMENT = -1
@@ -154,7 +154,8 @@ class Camera(C.Structure):
     _fields_ = [("projection", u32), ("sensor_fit", u32),
                 ("transform", f32 * 16), ("field_of_view", f32),
                 ("horizontal_field_of_view", f32), ("lens_shift_x", f32),
-                ("lens_shift_y", f32), ("near_clip", f32), ("far_clip", f32)]
+                ("lens_shift_y", f32), ("near_clip", f32), ("far_clip", f32),
+                ("orthographic_scale", f32)]
 
 
 class SceneDesc(C.Structure):

@@ -161,13 +161,17 @@ struct Lamp {
     uint32_t flags;  // kLamp* bits
     uint32_t visibility;
     uint32_t max_bounces;
-    uint32_t pad;
+    float size_u;    // area lamps: edge lengths along axis_x and axis_y, scaled
+    float size_v;
+    float spread;    // area lamps: emission spread angle
+    uint32_t pad0, pad1;
 };
-constexpr uint32_t kLampNormalize = 1u, kLampSphere = 4u, kLampUseMis = 8u, kLampForward = 32u;
+constexpr uint32_t kLampNormalize = 1u, kLampEllipse = 2u, kLampSphere = 4u, kLampUseMis = 8u, kLampForward = 32u,
+                   kLampFullSpread = 64u;
 // lamp_bits: visibility mask in bits 0-7, forward-intersectable (not distant,
-// radius > 0) in bit 8, effective MIS in bit 9
+// non-zero size) in bit 8, effective MIS in bit 9
 constexpr uint32_t kLampBitIntersectable = 256u, kLampBitMis = 512u;
-constexpr uint32_t kLampPoint = 0u, kLampSpot = 1u, kLampDistant = 3u;
+constexpr uint32_t kLampPoint = 0u, kLampSpot = 1u, kLampArea = 2u, kLampDistant = 3u;
 
 // Light tree records (LightTreeNodeGpu / LightTreeEmitterGpu,
 // src/luisa/path_tracer_types.h:202-225): measure packed as bounds min +
@@ -256,6 +260,8 @@ struct KernelParams {
     uint32_t transparent_background, nee, mis_mode, use_tree;
     float clamp_direct, clamp_indirect, light_inv_rr_threshold, pass_alpha_threshold;
     float tan_h, tan_v, shift_x, shift_y;
+    uint32_t projection;  // 0 perspective, 1 orthographic
+    float ortho_span;     // vertical view-plane span of the orthographic camera
     float near_clip, far_clip, pad1, pad2;
     float camera[16];
 };

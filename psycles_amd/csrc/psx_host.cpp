@@ -569,12 +569,24 @@ Lamp make_lamp(const psx_light &l) {
     g.angle = l.angle;
     g.visibility = l.visibility_mask;
     g.max_bounces = l.max_bounces;
+    g.size_u = l.size * lengths[0];
+    g.size_v = (l.size_y > 0.0f ? l.size_y : l.size) * lengths[1];
+    g.spread = l.spread;
     // the MIS property only holds where a forward technique exists
     const bool use_mis = (l.flags & PSX_LIGHT_FLAG_USE_MIS) != 0u;
-    const bool effective_mis = use_mis && (l.type == PSX_LIGHT_DISTANT ? l.angle > 0.0f : l.size > 0.0f);
+    bool effective_mis;
+    if (l.type == PSX_LIGHT_DISTANT)
+        effective_mis = use_mis && l.angle > 0.0f;
+    else if (l.type == PSX_LIGHT_AREA)
+        effective_mis = use_mis && g.size_u != 0.0f && g.size_v != 0.0f && l.spread > 0.0f;
+    else
+        effective_mis = use_mis && l.size > 0.0f;
+    constexpr float kPi = 3.14159265358979f;
     g.flags = ((l.flags & PSX_LIGHT_FLAG_NORMALIZE) ? kLampNormalize : 0u) |
+              ((l.flags & PSX_LIGHT_FLAG_ELLIPSE) ? kLampEllipse : 0u) |
               ((l.flags & PSX_LIGHT_FLAG_SPHERE) ? kLampSphere : 0u) | (effective_mis ? kLampUseMis : 0u) |
-              (effective_mis && l.type != PSX_LIGHT_DISTANT ? kLampForward : 0u);
+              (effective_mis && l.type != PSX_LIGHT_DISTANT ? kLampForward : 0u) |
+              (l.type == PSX_LIGHT_AREA && l.spread >= kPi - 1.0e-6f ? kLampFullSpread : 0u);
     return g;
 }
 
@@ -660,6 +672,9 @@ TreeEmitterIn lamp_tree_emitter(uint32_t id, const Lamp &l) {
         float area = 1.0f;
         if (l.type == kLampPoint || l.type == kLampSpot) {
             area = l.radius > 0.0f ? 4.0f * kTreePi * l.radius * l.radius : 4.0f;
+        } else if (l.type == kLampArea) {
+            area = std::abs(l.size_u * l.size_v);
+            if ((l.flags & kLampEllipse) != 0u) area *= 0.25f * kTreePi;
         } else if (l.type == kLampDistant && l.angle > 0.0f) {
             const float s = std::sin(0.5f * l.angle);
             area = kTreePi * s * s;
@@ -673,7 +688,29 @@ TreeEmitterIn lamp_tree_emitter(uint32_t id, const Lamp &l) {
     e.measure.orientation_empty = false;
     e.measure.bounds_empty = true;
     for (int i = 0; i < 3; ++i) e.measure.bounds_min[i] = e.measure.bounds_max[i] = 0.0f;
-    if (l.type == kLampPoint) {
+    if (l.type == kLampArea) {
+        e.measure.theta_e = 0.5f * l.spread;
+        float hu[3], hv[3], c[4][3];
+        for (int i = 0; i < 3; ++i) {
+            hu[i] = l.axis_x[i] * (0.5f * l.size_u);
+            hv[i] = l.axis_y[i] * (0.5f * l.size_v);
+        }
+        for (int i = 0; i < 3; ++i) {
+            c[0][i] = l.position[i] + (hu[i] + hv[i]);
+            c[1][i] = l.position[i] + (hu[i] - hv[i]);
+            c[2][i] = l.position[i] + (hv[i] - hu[i]);
+            c[3][i] = l.position[i] - (hu[i] + hv[i]);
+        }
+        for (int i = 0; i < 3; ++i) {
+            e.measure.bounds_min[i] = e.measure.bounds_max[i] = c[0][i];
+            for (int k = 1; k < 4; ++k) {
+                e.measure.bounds_min[i] = std::min(e.measure.bounds_min[i], c[k][i]);
+                e.measure.bounds_max[i] = std::max(e.measure.bounds_max[i], c[k][i]);
+            }
+        }
+        e.measure.bounds_empty = false;
+        for (float &v : strength) v = v * (1.0f / kTreePi);
+    } else if (l.type == kLampPoint) {
         e.measure.theta_o = kTreePi;
         e.measure.theta_e = kTreeHalfPi;
     } else if (l.type == kLampSpot) {
@@ -993,6 +1030,9 @@ KernelParams kernel_params(const Scene &scene, const SceneView &view, const psx_
     const bool hfit = cam.sensor_fit == 0u || (cam.sensor_fit == 2u && aspect >= 1.0f);
     k.tan_v = hfit ? std::tan(cam.horizontal_field_of_view * 0.5f) / aspect : std::tan(cam.field_of_view * 0.5f);
     k.tan_h = k.tan_v * aspect;
+    // orthographic_viewplane_span (include/psycles/luisa/camera_sampling.h:50-56)
+    k.projection = cam.projection;
+    k.ortho_span = hfit ? cam.orthographic_scale / aspect : cam.orthographic_scale;
     k.shift_x = cam.lens_shift_x * (hfit ? 1.0f : 1.0f / aspect);
     k.shift_y = cam.lens_shift_y * (hfit ? aspect : 1.0f);
     k.near_clip = cam.near_clip;
@@ -1238,7 +1278,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         return PSX_ERROR_INVALID_ARGUMENT;
     };
     if (d->abi_version != PSX_ABI_VERSION) return fail("scene ABI version mismatch");
-    if (d->camera.projection != 0u) return fail("only perspective cameras are supported");
+    if (d->camera.projection > 1u) return fail("only perspective and orthographic cameras are supported");
     auto scene = std::make_unique<Scene>();
     scene->backend = backend;
     scene->camera = d->camera;
@@ -1538,12 +1578,12 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     float bg[3] = {d->world_color[0], d->world_color[1], d->world_color[2]};
     for (uint32_t i = 0; i < d->light_count; ++i) {
         const psx_light &l = d->lights[i];
-        if (l.type == PSX_LIGHT_POINT || l.type == PSX_LIGHT_SPOT || l.type == PSX_LIGHT_DISTANT) {
+        if (l.type == PSX_LIGHT_POINT || l.type == PSX_LIGHT_SPOT || l.type == PSX_LIGHT_AREA ||
+            l.type == PSX_LIGHT_DISTANT) {
             s.lamp_list.push_back(make_lamp(l));
             continue;
         }
-        if (l.type != PSX_LIGHT_BACKGROUND)
-            return fail("light " + std::to_string(i) + ": area lights are not supported by this ABI version");
+        if (l.type != PSX_LIGHT_BACKGROUND) return fail("light " + std::to_string(i) + ": unknown light type");
         bg[0] += l.color[0] * l.power;
         bg[1] += l.color[1] * l.power;
         bg[2] += l.color[2] * l.power;
@@ -1558,8 +1598,12 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         std::vector<float4> spheres;
         std::vector<uint32_t> bits;
         for (const Lamp &l : s.lamp_list) {
-            spheres.push_back(make_float4(l.position[0], l.position[1], l.position[2], l.radius));
-            const bool intersectable = l.type != kLampDistant && l.radius > 0.0f;
+            // an area lamp lies in the ball of its half diagonal about the centre
+            const float r = l.type == kLampArea ? 0.5f * std::sqrt(l.size_u * l.size_u + l.size_v * l.size_v)
+                                                : l.radius;
+            spheres.push_back(make_float4(l.position[0], l.position[1], l.position[2], r));
+            const bool intersectable = l.type == kLampArea ? l.size_u * l.size_v != 0.0f
+                                                           : l.type != kLampDistant && l.radius > 0.0f;
             bits.push_back((l.visibility & 0xffu) | (intersectable ? kLampBitIntersectable : 0u) |
                            ((l.flags & kLampUseMis) != 0u ? kLampBitMis : 0u));
         }

@@ -1184,9 +1184,16 @@ __device__ Ray camera_ray(const KernelParams &k, const float4 *sobol, const floa
     const float width = (float)k.full_width, height = (float)k.full_height;
     const float sx = 2.0f * ((float)fx + jx) / width - 1.0f + 2.0f * k.shift_x;
     const float sy = 1.0f - 2.0f * ((float)fy + jy) / height + 2.0f * k.shift_y;
-    const f3 ld = normalize(V(sx * k.tan_h, sy * k.tan_v, -1.0f));
-    const float clip_cos = -ld.z;
-    const f3 wo = cam_point(k.camera, V(0.0f, 0.0f, 0.0f));
+    f3 lo = V(0.0f, 0.0f, 0.0f), ld = V(0.0f, 0.0f, -1.0f);
+    float clip_cos = 1.0f;
+    if (k.projection == 1u) {
+        // orthographic (src/luisa/path_tracer_camera.cpp:127-143)
+        lo = V(sx * k.ortho_span * (width / height) * 0.5f, sy * k.ortho_span * 0.5f, 0.0f);
+    } else {
+        ld = normalize(V(sx * k.tan_h, sy * k.tan_v, -1.0f));
+        clip_cos = -ld.z;
+    }
+    const f3 wo = cam_point(k.camera, lo);
     const f3 d = safe_normalize(cam_dir(k.camera, ld), V(0.0f, 0.0f, -1.0f));
     const float safe = fmaxf(fabsf(clip_cos), 1.0e-20f);
     const float c0 = k.near_clip / safe;
@@ -1490,6 +1497,287 @@ __device__ __forceinline__ LampSample sample_distant_light(const Lamp &l, float 
     return s;
 }
 
+// ---- area lamps (src/luisa/area_light_sampling.cpp;
+// include/psycles/luisa/analytic_light_sampling.h:605-846) ----------------
+__device__ __forceinline__ float al_sdiv(float a, float b) { return b != 0.0f ? a / b : 0.0f; }
+
+// area_spread_attenuation (:796-846)
+__device__ float area_spread_attenuation(f3 dir_to_light, f3 light_normal, float spread) {
+    const float cosine = fmaxf(dot(light_normal, neg(dir_to_light)), 0.0f);
+    const float half_spread = 0.5f * fmaxf(spread, 0.0f);
+    const float sine_angle = sqrtf(fmaxf(1.0f - cosine * cosine, 0.0f));
+    const float tangent_angle = sine_angle / fmaxf(cosine, 1.0e-20f);
+    const float tangent_spread = tanf(half_spread);
+    const float normalization = half_spread > 0.05f
+                                    ? 1.0f / fmaxf(tangent_spread - half_spread, 1.0e-20f)
+                                    : 3.0f / fmaxf(half_spread * half_spread * half_spread, 1.0e-20f);
+    const float finite_spread = fmaxf((tangent_spread - tangent_angle) * normalization, 0.0f);
+    const float zero_spread = tangent_angle > 1.0e-5f ? 0.0f : AL_PI;
+    const float narrowed = half_spread <= 0.0f ? zero_spread : finite_spread;
+    return spread >= AL_PI - 1.0e-6f ? 1.0f : narrowed;
+}
+
+// rectangle_solid_angle_context (:605-690): the rectangle in the frame of
+// its edges as seen from `ref` (Urena, Fajardo, King, "An area-preserving
+// parametrization for spherical rectangles", EGSR 2013)
+struct RectContext {
+    f3 z;
+    float z0, x0, x1, y0, y1;
+    float e[4], a[4];
+    float solid_angle, pdf;
+};
+
+__device__ RectContext rect_solid_angle_context(f3 ref, f3 center, f3 au, float lu, f3 av, float lv) {
+    RectContext c;
+    f3 z = cross(au, av);
+    const f3 off = center - ref;
+    float z0 = dot(off, z);
+    if (z0 > 0.0f) {
+        z = neg(z);
+        z0 = -z0;
+    }
+    c.z = z;
+    c.z0 = z0;
+    const float cu = dot(off, au), cv = dot(off, av);
+    c.x0 = cu - 0.5f * lu;
+    c.x1 = cu + 0.5f * lu;
+    c.y0 = cv - 0.5f * lv;
+    c.y1 = cv + 0.5f * lv;
+    const float raw[4] = {-c.y0, c.x1, c.y1, -c.x0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c.e[i] = raw[i] / sqrtf(raw[i] * raw[i] + z0 * z0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c.a[i] = asinf(fminf(fmaxf(-c.e[i] * c.e[(i + 1) & 3], -1.0f), 1.0f));
+    c.solid_angle = -(c.a[0] + c.a[1] + c.a[2] + c.a[3]);
+    const float min_e2 = fminf(fminf(c.e[0] * c.e[0], c.e[1] * c.e[1]), fminf(c.e[2] * c.e[2], c.e[3] * c.e[3]));
+    const float dist = length(off);
+    const float planar_pdf = al_sdiv(-(dist * dist * dist), z0 * lu * lv);
+    const bool planar = (c.solid_angle < 1.0e-5f) || (min_e2 > 0.99999f);
+    c.pdf = planar ? planar_pdf : 1.0f / c.solid_angle;
+    return c;
+}
+
+// sample_rectangle_solid_angle (:716-794)
+__device__ f3 sample_rect_solid_angle(f3 ref, f3 center, f3 au, float lu, f3 av, float lv, float rx, float ry,
+                                      float &pdf) {
+    const RectContext c = rect_solid_angle_context(ref, center, au, lu, av, lv);
+    const float b0 = c.e[0], b1 = c.e[2];
+    const float b0sq = b0 * b0;
+    const float alpha = rx * c.solid_angle + c.a[2] + c.a[3];
+    const float fu = al_sdiv(cosf(alpha) * b0 + b1, sinf(alpha));
+    float cos_u = copysignf(1.0f / sqrtf(fu * fu + b0sq), fu);
+    cos_u = fminf(fmaxf(cos_u, -1.0f), 1.0f);
+    float su = -(cos_u * c.z0) / fmaxf(sqrtf(fmaxf(1.0f - cos_u * cos_u, 0.0f)), 1.0e-7f);
+    su = fminf(fmaxf(su, c.x0), c.x1);
+    const float du2 = su * su + c.z0 * c.z0;
+    const float h0 = c.y0 / sqrtf(du2 + c.y0 * c.y0);
+    const float h1 = c.y1 / sqrtf(du2 + c.y1 * c.y1);
+    const float hv = h0 + ry * (h1 - h0);
+    const float hv2 = hv * hv;
+    const float sv = hv2 < 1.0f - 1.0e-6f ? hv * sqrtf(du2 / (1.0f - hv2)) : c.y1;
+    pdf = c.pdf;
+    return ((ref + au * su) + av * sv) + c.z * c.z0;
+}
+
+__device__ __forceinline__ float area_value(const Lamp &l) {
+    float area = l.size_u * l.size_v;
+    if ((l.flags & kLampEllipse) != 0u) area *= 0.25f * AL_PI;
+    return area;
+}
+
+__device__ __forceinline__ float area_eval_factor(const Lamp &l, f3 direction, f3 normal) {
+    const float area = area_value(l);
+    const float inverse_area = ((l.flags & kLampNormalize) != 0u && area != 0.0f) ? al_sdiv(1.0f, area) : 1.0f;
+    return AL_INV_PI * inverse_area * area_spread_attenuation(direction, normal, l.spread);
+}
+
+__device__ __forceinline__ float area_tangent_half_spread(float spread) {
+    return spread == AL_PI ? FLT_MAX : tanf(0.5f * fmaxf(spread, 0.0f));
+}
+
+__device__ __forceinline__ float area_to_solid_angle(f3 normal, f3 direction, float distance_squared) {
+    const float cosine = dot(normal, neg(direction));
+    return cosine > 0.0f ? distance_squared / cosine : 0.0f;
+}
+
+// clamp_to_spread (area_light_sampling.cpp:179-420): the part of the lamp
+// the spread cone about the reference point's foot can see, as a
+// rectangle, a circle or the original shape
+struct AreaShape {
+    bool valid, rectangle;
+    f3 center, au, av;
+    float lu, lv;
+};
+
+__device__ AreaShape area_shape(const Lamp &l, f3 ref, f3 normal, float tangent) {
+    AreaShape r;
+    r.valid = true;
+    r.rectangle = (l.flags & kLampEllipse) == 0u;
+    r.center = lamp3(l.position);
+    r.au = lamp3(l.axis_x);
+    r.av = lamp3(l.axis_y);
+    r.lu = l.size_u;
+    r.lv = l.size_v;
+    if ((l.flags & kLampFullSpread) != 0u) return r;
+    const f3 center = r.center;
+    const float plane_distance = dot(normal, ref - center);
+    const f3 closest = ref - normal * plane_distance;
+    const float spread_radius = plane_distance * tangent;
+    const float spread_u = dot(r.au, closest - center);
+    const float spread_v = dot(r.av, closest - center);
+    const bool round = !r.rectangle && r.lu == r.lv;
+    bool sample_spread = spread_radius == 0.0f;
+    bool resolved = false;
+    if (round && !sample_spread) {
+        const float center_distance = sqrtf(spread_u * spread_u + spread_v * spread_v);
+        const float light_radius = 0.5f * r.lu;
+        if (center_distance >= light_radius + spread_radius) {
+            r.valid = false;
+        } else {
+            sample_spread = (center_distance <= fabsf(light_radius - spread_radius)) && (spread_radius < light_radius);
+            if (center_distance > fabsf(light_radius - spread_radius)) {
+                const float rect_u = light_radius + spread_radius - center_distance;
+                float rect_v;
+                if (fabsf(light_radius * light_radius - spread_radius * spread_radius) >=
+                    center_distance * center_distance) {
+                    rect_v = 2.0f * fminf(light_radius, spread_radius);
+                } else {
+                    const float offset = center_distance +
+                                         (spread_radius * spread_radius - light_radius * light_radius) /
+                                             center_distance;
+                    rect_v = sqrtf(4.0f * spread_radius * spread_radius - offset * offset);
+                }
+                const float rect_area = rect_u * rect_v;
+                const float light_area = AL_PI * light_radius * light_radius;
+                const float spread_area = AL_PI * spread_radius * spread_radius;
+                if (rect_area < fminf(light_area, spread_area)) {
+                    r.rectangle = true;
+                    r.au = safe_normalize(center - closest, lamp3(l.axis_x));
+                    r.av = cross(normal, r.au);
+                    r.lu = rect_u;
+                    r.lv = rect_v;
+                    r.center = ((center + closest) + r.au * (spread_radius - light_radius)) * 0.5f;
+                    resolved = true;
+                } else {
+                    sample_spread = spread_area < light_area;
+                }
+            }
+        }
+    } else if (!round && !sample_spread) {
+        const float min_u = fmaxf(spread_u - spread_radius, -0.5f * r.lu);
+        const float max_u = fminf(spread_u + spread_radius, 0.5f * r.lu);
+        const float min_v = fmaxf(spread_v - spread_radius, -0.5f * r.lv);
+        const float max_v = fminf(spread_v + spread_radius, 0.5f * r.lv);
+        if (min_u >= max_u || min_v >= max_v) {
+            r.valid = false;
+        } else {
+            const float rect_u = max_u - min_u, rect_v = max_v - min_v;
+            const float rect_area = rect_u * rect_v;
+            const float ellipse_area = r.rectangle ? FLT_MAX : 0.25f * AL_PI * r.lu * r.lv;
+            const float spread_area = AL_PI * spread_radius * spread_radius;
+            if (r.rectangle || rect_area < fminf(ellipse_area, spread_area)) {
+                const float cu = 0.5f * (min_u + max_u), cv = 0.5f * (min_v + max_v);
+                r.center = r.center + (r.au * cu + r.av * cv);
+                r.lu = rect_u;
+                r.lv = rect_v;
+                r.rectangle = true;
+                resolved = true;
+            } else {
+                r.rectangle = false;
+                sample_spread = spread_area < ellipse_area;
+            }
+        }
+    }
+    if (r.valid && !resolved && sample_spread) {
+        r.rectangle = false;
+        r.center = center + (r.au * spread_u + r.av * spread_v);
+        r.lu = 2.0f * spread_radius;
+        r.lv = 2.0f * spread_radius;
+    }
+    return r;
+}
+
+// AreaLightSampling::from_position (:494-646): surface NEE
+__device__ LampSample sample_area_light(const Lamp &l, f3 ref, float rx, float ry) {
+    LampSample s;
+    s.valid = false;
+    s.direction = V(0.0f, 0.0f, 0.0f);
+    s.position = lamp3(l.position);
+    s.distance = 0.0f;
+    s.pdf = 0.0f;
+    s.eval = 0.0f;
+    const f3 normal = neg(lamp3(l.axis_z));
+    if (!(dot(s.position - ref, normal) <= 0.0f)) return s;
+    const float tangent = area_tangent_half_spread(l.spread);
+    const AreaShape sh = area_shape(l, ref, normal, tangent);
+    if (!sh.valid) return s;
+    f3 position = sh.center;
+    float pdf = 0.0f;
+    if (sh.rectangle) {
+        position = sample_rect_solid_angle(ref, sh.center, sh.au, sh.lu, sh.av, sh.lv, rx, ry, pdf);
+    } else if (tangent == 0.0f) {
+        pdf = 1.0f;
+    } else {
+        float dx, dy;
+        uniform_disk(rx, ry, dx, dy);
+        position = sh.center + (sh.au * (0.5f * sh.lu * dx) + sh.av * (0.5f * sh.lv * dy));
+        pdf = 4.0f * AL_INV_PI / (sh.lu * sh.lv);
+    }
+    const f3 off = position - ref;
+    const float d2 = dot(off, off);
+    const float d = sqrtf(fmaxf(d2, 0.0f));
+    const f3 dir = off / fmaxf(d, 1.0e-30f);
+    if (!sh.rectangle && tangent > 0.0f) pdf *= area_to_solid_angle(normal, dir, d2);
+    const float factor = area_eval_factor(l, dir, normal);
+    const f3 inplane = position - lamp3(l.position);
+    const float u = dot(inplane, lamp3(l.axis_x)), v = dot(inplane, lamp3(l.axis_y));
+    const float eu = (0.5f + 1.0e-7f) * l.size_u + 1.0e-6f;
+    const float ev = (0.5f + 1.0e-7f) * l.size_v + 1.0e-6f;
+    const bool inside_rect = fabsf(u) <= eu && fabsf(v) <= ev;
+    const bool inside_ellipse = u * u / (eu * eu) + v * v / (ev * ev) <= 1.0f;
+    s.valid = ((l.flags & kLampEllipse) != 0u ? inside_ellipse : inside_rect) && factor > 0.0f;
+    s.direction = dir;
+    s.position = position;
+    s.distance = d;
+    s.pdf = pdf;
+    s.eval = factor;
+    return s;
+}
+
+// intersect_area (analytic_light_intersection.h:271-353): the ray meets
+// the lamp's front strictly inside (tmin, tmax), within its shape, with
+// non-zero emission; `eval` from AreaLightSampling::from_intersection
+__device__ bool area_candidate(const Lamp &l, f3 ro, f3 rd, float tmin, float tmax, float &distance, float &eval) {
+    const f3 normal = neg(lamp3(l.axis_z));
+    const f3 center = lamp3(l.position);
+    const float dn = dot(rd, normal);
+    const float t = al_sdiv(dot(center - ro, normal), dn);
+    if (!(dn < 0.0f && t > tmin && t < tmax)) return false;
+    const f3 inplane = (ro + rd * t) - center;
+    const float u = al_sdiv(dot(inplane, lamp3(l.axis_x)), l.size_u);
+    const float v = al_sdiv(dot(inplane, lamp3(l.axis_y)), l.size_v);
+    const bool inside_rect = u >= -0.5f && u <= 0.5f && v >= -0.5f && v <= 0.5f;
+    if (!inside_rect || ((l.flags & kLampEllipse) != 0u && !(u * u + v * v <= 0.25f))) return false;
+    const AreaShape sh = area_shape(l, ro, normal, area_tangent_half_spread(l.spread));
+    if (!sh.valid) return false;
+    const float factor = area_eval_factor(l, rd, normal);
+    if (!(area_value(l) > 0.0f && factor > 0.0f)) return false;
+    distance = t;
+    eval = factor;
+    return true;
+}
+
+// AreaLightSampling::from_intersection (:648-726): the pdf of a forward hit
+__device__ float area_forward_pdf(const Lamp &l, f3 ro, f3 rd, float distance) {
+    const f3 normal = neg(lamp3(l.axis_z));
+    const float tangent = area_tangent_half_spread(l.spread);
+    const AreaShape sh = area_shape(l, ro, normal, tangent);
+    if (!sh.valid) return 0.0f;
+    if (sh.rectangle) return rect_solid_angle_context(ro, sh.center, sh.au, sh.lu, sh.av, sh.lv).pdf;
+    if (tangent == 0.0f) return 1.0f;
+    return 4.0f * AL_INV_PI / (sh.lu * sh.lv) * area_to_solid_angle(normal, rd, distance * distance);
+}
+
 // intersect_point / intersect_spot (analytic_light_intersection.h:33-260),
 // split in two: lamp_candidate decides whether the ray hits the lamp nearer
 // than `tmax` and with non-zero emission (the part every lamp of the forward
@@ -1498,6 +1786,7 @@ __device__ __forceinline__ LampSample sample_distant_light(const Lamp &l, float 
 // the single-pass intersection.
 __device__ __forceinline__ bool lamp_candidate(const Lamp &l, f3 ro, f3 rd, float tmin, float tmax, float &distance,
                                                float &eval) {
+    if (l.type == kLampArea) return area_candidate(l, ro, rd, tmin, tmax, distance, eval);
     if (!(l.radius > 0.0f)) return false;
     const bool spot = l.type == kLampSpot;
     const bool sphere = (l.flags & kLampSphere) != 0u;
@@ -1527,6 +1816,7 @@ __device__ __forceinline__ bool lamp_candidate(const Lamp &l, f3 ro, f3 rd, floa
 
 __device__ __forceinline__ float lamp_forward_pdf(const Lamp &l, f3 ro, f3 rd, float distance, f3 previous_normal,
                                                   bool had_transmission) {
+    if (l.type == kLampArea) return area_forward_pdf(l, ro, rd, distance);
     const bool spot = l.type == kLampSpot;
     const f3 center = lamp3(l.position);
     const f3 co = ro - center;
@@ -2337,8 +2627,10 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
             const bool reached_max_bounces = p.ps.bounce > l.max_bounces;
             const bool distant = l.type == kLampDistant;
             const bool has_transmission = (rflags & RT_BSDF_HAS_TRANSMISSION) != 0u;
-            const LampSample ls = distant ? sample_distant_light(l, ul.x, ul.y)
-                                          : sample_point_or_spot(l, sf.position, sf.ns, has_transmission, ul.x, ul.y);
+            const LampSample ls = distant                 ? sample_distant_light(l, ul.x, ul.y)
+                                  : l.type == kLampArea ? sample_area_light(l, sf.position, ul.x, ul.y)
+                                                        : sample_point_or_spot(l, sf.position, sf.ns,
+                                                                               has_transmission, ul.x, ul.y);
             const f3 light_radiance = lamp3(l.color) * (l.power * ls.eval);
             const float light_pdf = ls.pdf * entry.selection_pdf;
             if (!reached_max_bounces && ls.valid && light_pdf > 0.0f) {

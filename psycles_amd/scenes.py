@@ -222,7 +222,10 @@ class CameraData:
     field_of_view: float = math.radians(45.0)
     near_clip: float = 1.0e-4
     far_clip: float = 1.0e5
-    sensor_fit: int = 1              # vertical
+    sensor_fit: int = 1              # 0 horizontal, 1 vertical, 2 automatic
+    projection: int = 0              # 0 perspective, 1 orthographic
+    orthographic_scale: float = 1.0
+    horizontal_field_of_view: float | None = None   # defaults to field_of_view
 
 
 @dataclass
@@ -303,11 +306,13 @@ class SceneSnapshot:
         d.lights = C.cast(lights, C.POINTER(abi.Light))
         d.light_count = len(self.lights)
         cam = self.camera
-        d.camera.projection = 0
+        d.camera.projection = cam.projection
         d.camera.sensor_fit = cam.sensor_fit
         d.camera.transform[:] = column_major(cam.transform)
         d.camera.field_of_view = cam.field_of_view
-        d.camera.horizontal_field_of_view = cam.field_of_view
+        d.camera.horizontal_field_of_view = (cam.field_of_view if cam.horizontal_field_of_view is None
+                                             else cam.horizontal_field_of_view)
+        d.camera.orthographic_scale = cam.orthographic_scale
         d.camera.near_clip = cam.near_clip
         d.camera.far_clip = cam.far_clip
         d.world_color[:] = [float(c) for c in self.world_color]

@@ -104,12 +104,12 @@ def test_lamp_integrators_bit_identical():
     assert films[0][1] == films[1][1]
 
 
-def test_area_light_is_rejected_with_diagnostic():
+def test_unknown_light_type_is_rejected_with_diagnostic():
     be = RendererBackend(0)
     snap = scenes.cornell_box()
-    area = scenes.lamp(abi.LIGHT_POINT, scenes.identity())
-    area.type = abi.LIGHT_AREA
-    snap.lights = [area]
+    bad = scenes.lamp(abi.LIGHT_POINT, scenes.identity())
+    bad.type = 9
+    snap.lights = [bad]
     comp = be.compile_scene(snap)
     assert not comp.ok()
-    assert "area" in comp.diagnostics[0].message
+    assert "light type" in comp.diagnostics[0].message
