@@ -468,20 +468,24 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
         return true;
     }
     if (kind == kKindLeaf) {
+        // one triangle per step: a wave whose lanes sit on leaves of
+        // different sizes then runs one intersection test per step instead
+        // of the longest leaf's loop; the rest of the leaf is the next step
         const uint32_t first = index >> 3;
         const uint32_t n = (index & 7u) + 1u;
-        for (uint32_t k = 0; k < n; ++k) {
-            const LeafTri &tri = s.tris[first + k];
+        if (n > 1u) t.current = (((first + 1u) << 3) | (n - 2u)) | kKindLeaf;
+        {
+            const LeafTri &tri = s.tris[first];
             if (COUNT) tri_count++;
             float th, bu, bv;
             const bool ok = pluecker(t.o, t.d, t.ray.tmin, t.ray.tmax, V(tri.p0x, tri.p0y, tri.p0z),
                                      V(tri.p1x, tri.p1y, tri.p1z), V(tri.p2x, tri.p2y, tri.p2z), th, bu, bv);
-            if (!ok) continue;
+            if (!ok) return true;
             const uint32_t prim = __float_as_uint(tri.prim_bits);
             const uint32_t cprim = t.inst_prim_offset + prim;
             const bool excluded = (t.inst_index == t.q.src_obj && cprim == t.q.src_prim) ||
                                   (t.inst_index == t.q.light_obj && cprim == t.q.light_prim);
-            if (excluded || !order_accepts(t.best, th, t.inst_index, cprim)) continue;
+            if (excluded || !order_accepts(t.best, th, t.inst_index, cprim)) return true;
             t.best.valid = true;
             t.best.t = th;
             t.best.u = bu;

@@ -29,6 +29,7 @@ namespace {
 constexpr uint32_t kTraceBlock = 256;
 constexpr uint32_t kShadeBlock = 256;
 constexpr uint32_t kSkip = 0xffffffffu;  // queue/list entry of a padding lane
+constexpr uint32_t kChunk = 128u;         // queue entries a trace wave reserves per atomic
 constexpr uint32_t kRefill = 16u;        // idle lanes that trigger a refill
 
 __device__ __forceinline__ void add_counter(unsigned long long *dst, uint32_t value) {
@@ -175,8 +176,8 @@ __global__ __launch_bounds__(kShadeBlock) void generate_kernel(KernelParams k, c
 }
 
 // Persistent trace kernel.  Each lane owns one query at a time; when at least
-// kRefill lanes of a wave have finished, the wave claims that many queue
-// entries with one atomic and the idle lanes start them while the others keep
+// kRefill lanes of a wave have finished, the idle lanes take entries from
+// the wave's reserved chunk of the queue and start them while the others keep
 // their traversal state (Aila & Laine, "Understanding the efficiency of ray
 // traversal on GPUs", 2009: persistent threads with ray replacement).
 template <bool COUNT>
@@ -195,18 +196,32 @@ __global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, K
     bool active = false, exhausted = false, shadow = false;
     uint32_t path = 0u, tdepth = 0u;
     f3 transmittance = V(1.0f, 1.0f, 1.0f);
+    // The wave's reservation [res, res_end) of queue entries.  One atomic on
+    // the shared cursor reserves kChunk entries, which the wave hands to its
+    // idle lanes over several refills: a returning atomic on one address
+    // stalls the whole wave, and claiming per refill made it the largest
+    // cost per ray.  `exhausted`: no chunk is left to reserve.
+    uint32_t res = 0u, res_end = 0u;
     for (;;) {
-        if (!exhausted) {
-            // claim one queue entry per idle lane
+        if (!exhausted || res < res_end) {
             const unsigned long long idle = __ballot(!active);
             const uint32_t want = static_cast<uint32_t>(__popcll(idle));
-            const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(idle)) - 1);
-            uint32_t base = 0u;
-            if (lane == leader) base = atomicAdd(&c->trace_fetch, want);
-            base = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(base), static_cast<int>(leader)));
-            exhausted = base + want >= count;
-            const uint32_t qi = base + static_cast<uint32_t>(__popcll(idle & below));
-            if (!active && qi < count) {
+            if (res == res_end && !exhausted && want > 0u) {
+                const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(idle)) - 1);
+                uint32_t b = 0u;
+                if (lane == leader) b = atomicAdd(&c->trace_fetch, kChunk);
+                b = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b), static_cast<int>(leader)));
+                if (b < count) {
+                    res = b;
+                    res_end = count - b > kChunk ? b + kChunk : count;
+                }
+                exhausted = b >= count || count - b <= kChunk;
+            }
+            const uint32_t avail = res_end - res;
+            const uint32_t rank = static_cast<uint32_t>(__popcll(idle & below));
+            const uint32_t qi = res + rank;
+            res += want < avail ? want : avail;
+            if (!active && rank < avail) {
                 const uint32_t entry = queue[qi];
                 if (entry != kSkip) {
                     path = entry >> 1;
@@ -243,7 +258,7 @@ __global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, K
             }
         }
         if (__ballot(active) == 0ull) {
-            if (exhausted) break;
+            if (exhausted && res == res_end) break;
             continue;
         }
         // traverse until enough lanes are idle to refill (or all are, at the end)
@@ -264,7 +279,7 @@ __global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, K
                 }
             }
             const uint32_t idle = static_cast<uint32_t>(__popcll(__ballot(!active)));
-            if (idle == 64u || (!exhausted && idle >= kRefill)) break;
+            if (idle == 64u || ((!exhausted || res < res_end) && idle >= kRefill)) break;
         }
     }
     add_counter(&counters->rays, rays);
