@@ -53,14 +53,14 @@ def bytes_per_ray(nodes: float, tris: float) -> float:
 WORKLOADS = {
     "instanced": {
         "scene": "instanced_icospheres", "settings": "instanced_settings", "spp": 64,
-        "nodes": 46.83, "tris": 8.59, "traffic": 1028.1,
+        "nodes": 46.83, "tris": 8.59, "traffic": 1117.8,
         "describe": "instanced icospheres: 1,003,520-tri displaced icosphere x 64 instances, "
                     "mix(diffuse, transparent) materials, ground quad, constant background; "
                     "1920x1080, {spp} spp, max 8 bounces, seed 0",
     },
     "many_lights": {
         "scene": "many_lights", "settings": "many_light_settings", "spp": 32,
-        "nodes": 94.81, "tris": 15.06, "traffic": 1703.5,
+        "nodes": 94.81, "tris": 15.06, "traffic": 1848.8,
         "describe": "many lights: the instanced icospheres plus 10,000 emissive add(emission, diffuse) "
                     "triangles, 256 point, 64 spot and 1 distant lamp, light-tree NEE; "
                     "1920x1080, {spp} spp, max 6 bounces, seed 0",
