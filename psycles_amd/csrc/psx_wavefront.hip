@@ -38,6 +38,24 @@ __device__ __forceinline__ void add_counter(unsigned long long *dst, uint32_t va
     if ((threadIdx.x & 63) == 0 && v != 0ull) atomicAdd(dst, v);
 }
 
+// The state a shade step can change: the sample identity never changes
+// after generation, and the primary Normal / Albedo only while the path is
+// at bounce 0 (`aov`).
+__device__ __forceinline__ void store_step(const WavefrontView &w, uint32_t i, const PathVars &p, bool aov) {
+    w.ray_o[i] = make_float4(p.ray.o.x, p.ray.o.y, p.ray.o.z, p.ray.tmin);
+    w.ray_d[i] = make_float4(p.ray.d.x, p.ray.d.y, p.ray.d.z, p.ray.tmax);
+    w.aux[i] = make_uint4(p.src_obj, p.src_prim, p.step, p.bits);
+    w.throughput[i] = make_float4(p.throughput.x, p.throughput.y, p.throughput.z, p.prev_pdf);
+    w.mis_normal[i] = make_float4(p.mis_normal.x, p.mis_normal.y, p.mis_normal.z, 0.0f);
+    w.radiance[i] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, p.transparency);
+    if (aov) {
+        w.normal[i] = make_float4(p.normal.x, p.normal.y, p.normal.z, 0.0f);
+        w.albedo[i] = make_float4(p.albedo.x, p.albedo.y, p.albedo.z, 0.0f);
+    }
+    w.state0[i] = make_uint4(p.ps.flag, p.ps.visibility, p.ps.bounce, p.ps.diffuse_bounce);
+    w.state1[i] = make_uint4(p.ps.glossy_bounce, p.ps.transmission_bounce, p.ps.transparent_bounce, p.ps.rng_offset);
+}
+
 __device__ __forceinline__ void store_path(const WavefrontView &w, uint32_t i, const PathVars &p) {
     w.ray_o[i] = make_float4(p.ray.o.x, p.ray.o.y, p.ray.o.z, p.ray.tmin);
     w.ray_d[i] = make_float4(p.ray.d.x, p.ray.d.y, p.ray.d.z, p.ray.tmax);
@@ -309,6 +327,7 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, Kern
         if (i < count && list[i] != kSkip) {
             path = list[i];
             PathVars p = load_path(w, path);
+            const bool aov = p.ps.bounce == 0u;
             if (LP) p.lp = load_lp(w, path);
             if ((p.bits & PATH_SHADOW_PENDING) != 0u) {
                 const float4 L = w.shadow_l[path];
@@ -353,7 +372,7 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, Kern
                 }
                 if (!cont) p.bits |= shadow ? PATH_FINISHING : PATH_DONE;
             }
-            store_path(w, path, p);
+            store_step(w, path, p, aov);
             if (LP) store_lp(w, path, p.lp);
         }
         // one atomic per wave on the (trace_count, shade_count) word: the
