@@ -48,6 +48,8 @@ hipError_t launch_material_tables(const SceneView &s, float4 *shadow, float4 *em
 // psx_wavefront.hip
 uint32_t wavefront_lanes();
 uint32_t wavefront_padded_pixels(const KernelParams &k);
+uint32_t wavefront_queue_headroom();
+uint32_t wavefront_list_headroom();
 hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
                             const FilmView &film, const WavefrontView &w, uint32_t first, uint32_t count,
                             Counters *counters, bool count_nodes, uint32_t *spill, hipEvent_t *events,
@@ -870,11 +872,13 @@ struct WavefrontState {
         shadow_q.allocate(n * 16u);
         shadow_l.allocate(n * 16u);
         shadow_t.allocate(n * 16u);
-        // a live path queues at most a continuation ray and a shadow ray
-        queue0.allocate(n * 8u);
-        queue1.allocate(n * 8u);
-        list0.allocate(n * 4u);
-        list1.allocate(n * 4u);
+        // a live path queues at most a continuation ray and a shadow ray;
+        // shade waves leave partly used reservations as skip entries
+        const size_t qh = wavefront_queue_headroom(), lh = wavefront_list_headroom();
+        queue0.allocate((2u * n + qh) * 4u);
+        queue1.allocate((2u * n + qh) * 4u);
+        list0.allocate((n + lh) * 4u);
+        list1.allocate((n + lh) * 4u);
         counters.allocate((iterations + 1u) * sizeof(WavefrontCounters));
         WavefrontView &v = view;
         v.ray_o = ray_o.as<float4>();

@@ -31,6 +31,11 @@ constexpr uint32_t kShadeBlock = 256;
 constexpr uint32_t kSkip = 0xffffffffu;  // queue/list entry of a padding lane
 constexpr uint32_t kChunk = 128u;         // queue entries a trace wave reserves per atomic
 constexpr uint32_t kRefill = 16u;        // idle lanes that trigger a refill
+// Slots of the next trace queue and shade list a shade wave reserves per
+// atomic; at most one partly used chunk of each per wave is left as skip
+// entries when the wave ends.
+constexpr uint32_t kQueueChunk = 256u, kListChunk = 128u;
+constexpr uint32_t kMaxWavesPerCu = 32u;
 
 __device__ __forceinline__ void add_counter(unsigned long long *dst, uint32_t value) {
     unsigned long long v = value;
@@ -321,6 +326,12 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, Kern
     uint32_t *next_queue = w.queue[(it + 1u) & 1u];
     uint32_t *next_list = w.list[(it + 1u) & 1u];
     const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    // this wave's reservations [q_pos, q_end) in next_queue and [l_pos, l_end)
+    // in next_list: a returning atomic on the shared counters per step would
+    // stall the wave, as it did the trace kernel's refill
+    uint32_t q_pos = 0u, q_end = 0u, l_pos = 0u, l_end = 0u;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i - threadIdx.x % 64u < count; i += stride) {
         bool cont = false, shadow = false;
         uint32_t path = 0u;
@@ -375,27 +386,49 @@ __global__ __launch_bounds__(kShadeBlock) void wf_shade_kernel(SceneView s, Kern
             store_step(w, path, p, aov);
             if (LP) store_lp(w, path, p.lp);
         }
-        // one atomic per wave on the (trace_count, shade_count) word: the
-        // wave's continuation entries, then its shadow entries, then its list
-        // entries (no carry crosses into shade_count: counts stay below 2^32)
+        // the wave's continuation entries, then its shadow entries, into its
+        // queue reservation, and its live paths into its list reservation;
+        // an append that overflows a reservation continues in a new chunk
         const unsigned long long cm = __ballot(cont), sm = __ballot(shadow), lm = cm | sm;
         if (lm != 0ull) {
-            const uint32_t lane = __lane_id();
-            const unsigned long long below = (1ull << lane) - 1ull;
             const uint32_t nc = static_cast<uint32_t>(__popcll(cm)), ns = static_cast<uint32_t>(__popcll(sm));
-            const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(lm)) - 1);
-            unsigned long long old = 0ull;
-            if (lane == leader)
-                old = atomicAdd(reinterpret_cast<unsigned long long *>(&next->trace_count),
-                                static_cast<unsigned long long>(nc + ns) |
-                                    (static_cast<unsigned long long>(__popcll(lm)) << 32));
-            old = __shfl(old, static_cast<int>(leader), 64);
-            const uint32_t qbase = static_cast<uint32_t>(old), lbase = static_cast<uint32_t>(old >> 32);
-            if (cont) next_queue[qbase + static_cast<uint32_t>(__popcll(cm & below))] = path << 1;
-            if (shadow) next_queue[qbase + nc + static_cast<uint32_t>(__popcll(sm & below))] = (path << 1) | 1u;
-            if (cont || shadow) next_list[lbase + static_cast<uint32_t>(__popcll(lm & below))] = path;
+            const uint32_t nq = nc + ns, nl = static_cast<uint32_t>(__popcll(lm));
+            const uint32_t q_old = q_pos, q_avail = q_end - q_pos;
+            uint32_t q_new = 0u;
+            if (nq > q_avail) {
+                if (lane == 0u) q_new = atomicAdd(&next->trace_count, kQueueChunk);
+                q_new = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(q_new)));
+                q_pos = q_new + (nq - q_avail);
+                q_end = q_new + kQueueChunk;
+            } else {
+                q_pos += nq;
+            }
+            const uint32_t l_old = l_pos, l_avail = l_end - l_pos;
+            uint32_t l_new = 0u;
+            if (nl > l_avail) {
+                if (lane == 0u) l_new = atomicAdd(&next->shade_count, kListChunk);
+                l_new = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(l_new)));
+                l_pos = l_new + (nl - l_avail);
+                l_end = l_new + kListChunk;
+            } else {
+                l_pos += nl;
+            }
+            const uint32_t qr = cont ? static_cast<uint32_t>(__popcll(cm & below))
+                                     : nc + static_cast<uint32_t>(__popcll(sm & below));
+            const uint32_t qs = qr < q_avail ? q_old + qr : q_new + (qr - q_avail);
+            if (cont) next_queue[qs] = path << 1;
+            if (shadow && !cont) next_queue[qs] = (path << 1) | 1u;
+            if (cont && shadow) {
+                const uint32_t r2 = nc + static_cast<uint32_t>(__popcll(sm & below));
+                next_queue[r2 < q_avail ? q_old + r2 : q_new + (r2 - q_avail)] = (path << 1) | 1u;
+            }
+            const uint32_t lr = static_cast<uint32_t>(__popcll(lm & below));
+            if (cont || shadow) next_list[lr < l_avail ? l_old + lr : l_new + (lr - l_avail)] = path;
         }
     }
+    // the unused ends of this wave's reservations become skip entries
+    for (uint32_t j = q_pos + lane; j < q_end; j += 64u) next_queue[j] = kSkip;
+    for (uint32_t j = l_pos + lane; j < l_end; j += 64u) next_list[j] = kSkip;
 }
 
 // in-order film accumulation of the batch, one lane per window pixel
@@ -580,6 +613,11 @@ uint32_t wavefront_lanes() {
     const uint32_t b = resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<true>), kTraceBlock);
     return (a > b ? a : b) * kTraceBlock;
 }
+
+// Entries the next trace queue and shade list need beyond the live ones:
+// one partly used reservation per array and resident wave.
+uint32_t wavefront_queue_headroom() { return static_cast<uint32_t>(device_cus()) * kMaxWavesPerCu * kQueueChunk; }
+uint32_t wavefront_list_headroom() { return static_cast<uint32_t>(device_cus()) * kMaxWavesPerCu * kListChunk; }
 
 uint32_t wavefront_padded_pixels(const KernelParams &k) {
     return ((k.window_width + 7u) >> 3) * ((k.window_height + 7u) >> 3) * 64u;
