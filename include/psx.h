@@ -423,6 +423,25 @@ enum psx_trace_word {
 int psx_session_trace_events(psx_session *session, uint32_t sample_index,
                              uint32_t total, uint32_t *out_records);
 
+/* Diagnostic path trace of one pixel and sample: PSX_PATH_TRACE_SLOTS float4
+ * slots in the "psycles.cycles-path-trace" schema version 2 layout
+ * (include/psycles/luisa/path_trace_schema.h:10-121, written at
+ * src/luisa/path_kernel_setup.cpp:540-560 and
+ * src/luisa/path_kernel_surface_shading.cpp:270-340); replaces
+ * LuisaPathTraceRequest{pixel_x, pixel_y, sample, sink}
+ * (include/psycles/luisa/path_tracer.h:20-43).  `pixel_x` is a column of the
+ * full image, `pixel_y` a row counted from the lower row of the full image
+ * (Cycles' film convention); the pixel must lie in the session's window.
+ * w = 1 marks a written slot.  Written: the global slots 0-6 and, for the
+ * first 4 surface events, event slots 0-13 and 38.  `out_records`
+ * (optional) receives the pixel's PSX_TRACE_EVENTS x PSX_TRACE_WORDS
+ * discrete events. */
+#define PSX_PATH_TRACE_SLOTS 328u
+#define PSX_PATH_TRACE_VERSION 2u
+int psx_session_path_trace(psx_session *session, uint32_t pixel_x, uint32_t pixel_y,
+                           uint32_t sample_index, uint32_t total,
+                           float *out_slots, uint32_t *out_records);
+
 /* Sobol LUT as built for `total` samples (256 patterns x sequence size
  * float4); out may be NULL to query *out_count. */
 int psx_sobol_table(uint32_t total, float *out, uint32_t *out_count);

@@ -3553,7 +3553,28 @@ typedef struct film_pixel {
 typedef struct trace_log {
     uint32_t *records; /* PSX_TRACE_EVENTS * PSX_TRACE_WORDS, or NULL */
     uint32_t events;
+    float *slots;      /* PSX_PATH_TRACE_SLOTS * 4 schema v2 slots, or NULL */
 } trace_log;
+
+/* Diagnostic path trace record, schema "psycles.cycles-path-trace" v2
+ * (include/psycles/luisa/path_trace_schema.h:10-121): 8 global slots, then
+ * 72 slots per surface event for the first 4 events; w = 1 marks a written
+ * slot.  trace_uint32 (src/luisa/path_kernel_setup.cpp:430-433) stores a
+ * word as its two 16-bit halves. */
+#define TRACE_GLOBAL_SLOTS 8u
+#define TRACE_EVENT_SLOTS 72u
+#define TRACE_MAX_EVENTS 4u
+
+static void trace_slot(float *slots, uint32_t index, float x, float y, float z) {
+    float *d = slots + 4u * index;
+    d[0] = x;
+    d[1] = y;
+    d[2] = z;
+    d[3] = 1.0f;
+}
+static void trace_slot3(float *slots, uint32_t index, v3 v) { trace_slot(slots, index, v.x, v.y, v.z); }
+static float lo16(uint32_t v) { return (float)(v & 0xffffu); }
+static float hi16(uint32_t v) { return (float)(v >> 16); }
 
 static void log_event(trace_log *log, uint32_t step, uint32_t word, uint32_t value) {
     if (!log || !log->records || step >= PSX_TRACE_EVENTS) return;
@@ -3623,6 +3644,18 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
     const camera_ray_t cam = camera_ray(k, full_x, full_y, sample_index);
     const uint32_t rng_hash = cam.rng_hash;
     ray_t ray = cam.ray;
+    if (log && log->slots) {
+        /* global slots (src/luisa/path_kernel_setup.cpp:540-560) */
+        const float *f = sample_4d(&k->smp, sample_index, rng_hash, 0u);
+        const float fx = sample_index == 0u ? 0.5f : f[0], fy = sample_index == 0u ? 0.5f : f[1];
+        trace_slot(log->slots, 0u, 2.0f, (float)full_x, (float)(k->full_height - 1u - full_y));
+        trace_slot(log->slots, 1u, (float)sample_index, lo16(rng_hash), hi16(rng_hash));
+        trace_slot(log->slots, 2u, fx, fy, 0.0f);
+        trace_slot(log->slots, 3u, 0.0f, 0.0f, 0.0f);
+        trace_slot3(log->slots, 4u, ray.o);
+        trace_slot3(log->slots, 5u, ray.d);
+        trace_slot(log->slots, 6u, ray.tmin, ray.tmax, 0.5f);
+    }
     uint32_t src_obj = UINT32_MAX, src_prim = UINT32_MAX;
     uint32_t ray_visibility = PSX_VISIBILITY_CAMERA;
     v3 radiance = V3(0, 0, 0), throughput = V3(1, 1, 1);
@@ -3794,6 +3827,27 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
             if (p != 1.0f) {
                 if (u_term >= p) { log_flag(log, step, 8u); break; }
                 throughput = vdivs(throughput, p);
+            }
+            /* surface event slots once the path survived the roulette
+             * (src/luisa/path_kernel_surface_shading.cpp:270-340) */
+            if (log && log->slots && step < TRACE_MAX_EVENTS) {
+                float *e = log->slots + 4u * (TRACE_GLOBAL_SLOTS + step * TRACE_EVENT_SLOTS);
+                trace_slot(e, 0u, (float)step, (float)ps.rng_offset, (float)ps.bounce);
+                trace_slot(e, 1u, (float)ps.transparent_bounce, (float)ps.diffuse_bounce, (float)ps.glossy_bounce);
+                trace_slot(e, 2u, (float)ps.transmission_bounce, lo16(ps.visibility), hi16(ps.visibility));
+                trace_slot(e, 3u, lo16(ps.flag), hi16(ps.flag), p);
+                trace_slot3(e, 4u, throughput);
+                trace_slot3(e, 5u, ray.o);
+                trace_slot3(e, 6u, ray.d);
+                trace_slot(e, 7u, ray.tmin, ray.tmax, 0.5f);
+                trace_slot(e, 8u, hit.t, hit.u, hit.v);
+                trace_slot(e, 9u, (float)hit.object, (float)hit.cprim, 1.0f);
+                trace_slot(e, 10u, lo16(sf.material), hi16(sf.material), (float)cs.count);
+                trace_slot3(e, 11u, sf.position);
+                trace_slot3(e, 12u, sf.ng);
+                trace_slot3(e, 13u, sf.ns);
+                const uint32_t rf = surface_runtime_flags(&cs, sf.back_facing);
+                trace_slot(e, 38u, lo16(rf), hi16(rf), 0.0f);
             }
         }
         if (ps.bounce == 0u) {
@@ -4299,11 +4353,35 @@ int oracle_trace_events(const oracle_scene *s, const psx_render_settings *rs,
     for (uint32_t y = 0; y < k->window_height; ++y)
         for (uint32_t x = 0; x < k->window_width; ++x) {
             trace_log log;
+            log.events = 0u;
+            log.slots = NULL;
             log.records = out + per * ((size_t)y * k->window_width + x);
             memset(log.records, 0, sizeof(uint32_t) * per);
             path_out o;
             integrate(s, k, x + k->window_x, y + k->window_y, sample_index, &o, &log, NULL);
         }
+    free(k);
+    free(table);
+    return 0;
+}
+
+int oracle_path_trace(const oracle_scene *s, const psx_render_settings *rs, uint32_t x, uint32_t y,
+                      uint32_t sample_index, uint32_t total, float *out_slots, uint32_t *out_records) {
+    const uint32_t seq = oracle_sequence_size_for_samples(total);
+    float *table = (float *)malloc(sizeof(float) * 4u * 256u * seq);
+    oracle_sobol_table(seq, table);
+    kparams *k = (kparams *)malloc(sizeof(kparams));
+    setup_params(s, rs, table, seq, k);
+    uint32_t records[PSX_TRACE_EVENTS * PSX_TRACE_WORDS];
+    memset(records, 0, sizeof(records));
+    memset(out_slots, 0, sizeof(float) * 4u * PSX_PATH_TRACE_SLOTS);
+    trace_log log;
+    log.records = records;
+    log.events = 0u;
+    log.slots = out_slots;
+    path_out o;
+    integrate(s, k, x + k->window_x, y + k->window_y, sample_index, &o, &log, NULL);
+    if (out_records) memcpy(out_records, records, sizeof(records));
     free(k);
     free(table);
     return 0;

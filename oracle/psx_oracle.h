@@ -252,6 +252,11 @@ int oracle_trace_events(const oracle_scene *scene,
                         const psx_render_settings *settings,
                         uint32_t sample_index, uint32_t total,
                         uint32_t *out_records);
+/* schema v2 path trace of one pixel (window-relative) and sample:
+ * PSX_PATH_TRACE_SLOTS float4 slots and, optionally, the pixel's events */
+int oracle_path_trace(const oracle_scene *scene, const psx_render_settings *settings,
+                      uint32_t x, uint32_t y, uint32_t sample_index, uint32_t total,
+                      float *out_slots, uint32_t *out_records);
 
 /* Sampler probe with the same layout as psx_session_sampler_probe. */
 int oracle_sampler_probe(const psx_render_settings *settings,

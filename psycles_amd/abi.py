@@ -101,6 +101,8 @@ BACKEND_COUNT_TRAVERSAL = 1
 
 TRACE_EVENTS = 8
 TRACE_WORDS = 8
+PATH_TRACE_SLOTS = 328
+PATH_TRACE_VERSION = 2
 TRACE_FLAGS = 0
 TRACE_RNG_OFFSET = 1
 TRACE_OBJECT = 2
@@ -275,7 +277,7 @@ EXPORTED_SYMBOLS = [
     "psx_session_create", "psx_session_destroy", "psx_session_reset",
     "psx_session_render", "psx_session_cancel", "psx_session_stats",
     "psx_session_render_device", "psx_session_read_film",
-    "psx_session_primary_hits", "psx_session_trace_events",
+    "psx_session_primary_hits", "psx_session_trace_events", "psx_session_path_trace",
     "psx_sobol_table", "psx_session_sampler_probe", "psx_session_read_light_passes",
     "psx_server_create", "psx_server_destroy", "psx_server_submit",
     "psx_server_update_parameters", "psx_server_status", "psx_server_wait",
@@ -316,6 +318,7 @@ def load_library(path: os.PathLike | None = None):
     lib.psx_session_read_film.argtypes = [vp, P(f32), P(f32), P(f32), P(u32)]
     lib.psx_session_primary_hits.argtypes = [vp, u32, u32, P(u32)]
     lib.psx_session_trace_events.argtypes = [vp, u32, u32, P(u32)]
+    lib.psx_session_path_trace.argtypes = [vp, u32, u32, u32, u32, P(f32), P(u32)]
     lib.psx_sobol_table.argtypes = [u32, P(f32), P(u32)]
     lib.psx_session_sampler_probe.argtypes = [vp, u32, u32, u32, u32, P(u32)]
     lib.psx_session_read_light_passes.argtypes = [vp, P(f32)]
@@ -342,7 +345,7 @@ def load_library(path: os.PathLike | None = None):
                  "psx_session_reset", "psx_session_render",
                  "psx_session_stats", "psx_session_render_device",
                  "psx_session_read_film", "psx_session_primary_hits",
-                 "psx_session_trace_events", "psx_sobol_table",
+                 "psx_session_trace_events", "psx_session_path_trace", "psx_sobol_table",
                  "psx_session_sampler_probe"):
         getattr(lib, name).restype = C.c_int
     for name in ("psx_backend_destroy", "psx_scene_destroy",

@@ -39,6 +39,9 @@ hipError_t launch_render(const SceneView &s, const KernelParams &k, const float4
                          uint32_t *spill, uint32_t *cursor, hipStream_t stream);
 hipError_t launch_trace(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
                         uint32_t sample, uint32_t *records, uint32_t *spill, hipStream_t stream);
+hipError_t launch_path_trace(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
+                             uint32_t sample, uint32_t px, uint32_t py, float4 *slots, uint32_t *records,
+                             uint32_t *spill, hipStream_t stream);
 hipError_t launch_primary(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
                           uint32_t sample, uint32_t *records, uint32_t *spill, hipStream_t stream);
 hipError_t launch_sampler(const KernelParams &k, const float4 *sobol, uint32_t sample, uint32_t first_dim,
@@ -2528,6 +2531,35 @@ int psx_session_trace_events(psx_session *handle, uint32_t sample, uint32_t tota
                                rec.as<uint32_t>(), s->spill.as<uint32_t>(), s->stream));
         hip_check(hipStreamSynchronize(s->stream));
         hip_check(hipMemcpy(out, rec.as<void>(), words * 4u, hipMemcpyDeviceToHost));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+int psx_session_path_trace(psx_session *handle, uint32_t pixel_x, uint32_t pixel_y, uint32_t sample,
+                           uint32_t total, float *out_slots, uint32_t *out_records) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s || !out_slots || total == 0u || sample >= total) return PSX_ERROR_INVALID_ARGUMENT;
+    // pixel_y counts from the lower row of the full image (Cycles' film convention)
+    const KernelParams &k = s->params;
+    if (pixel_x >= k.full_width || pixel_y >= k.full_height) return PSX_ERROR_INVALID_ARGUMENT;
+    const uint32_t fy = k.full_height - 1u - pixel_y;
+    if (pixel_x < k.window_x || fy < k.window_y) return PSX_ERROR_INVALID_ARGUMENT;
+    const uint32_t x = pixel_x - k.window_x, y = fy - k.window_y;
+    if (x >= s->window_w || y >= s->window_h) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->scene->backend->device));
+        prepare_sobol(*s, total);
+        DeviceBuffer slots, rec;
+        slots.allocate(PSX_PATH_TRACE_SLOTS * 16u);
+        rec.allocate(PSX_TRACE_EVENTS * PSX_TRACE_WORDS * 4u);
+        hip_check(launch_path_trace(s->scene->view, s->params, s->sobol.as<float4>(), s->filter.as<float>(), sample,
+                                    x, y, slots.as<float4>(), rec.as<uint32_t>(), s->spill.as<uint32_t>(),
+                                    s->stream));
+        hip_check(hipStreamSynchronize(s->stream));
+        hip_check(hipMemcpy(out_slots, slots.as<void>(), PSX_PATH_TRACE_SLOTS * 16u, hipMemcpyDeviceToHost));
+        if (out_records)
+            hip_check(hipMemcpy(out_records, rec.as<void>(), PSX_TRACE_EVENTS * PSX_TRACE_WORDS * 4u,
+                                hipMemcpyDeviceToHost));
         return static_cast<int>(PSX_OK);
     });
 }

@@ -24,9 +24,11 @@ struct PathResult {
 template <bool COUNT, bool TRACE>
 __device__ PathResult integrate(const SceneView &s, const LampView &lv, const KernelParams &k, const float4 *sobol,
                                 const float *filter, uint32_t fx, uint32_t fy, uint32_t sample, Stack &st,
-                                uint32_t &rays, uint32_t &nodes, uint32_t &tris, uint32_t *trace) {
+                                uint32_t &rays, uint32_t &nodes, uint32_t &tris, uint32_t *trace,
+                                float4 *slots = nullptr) {
     PathVars p;
     init_path(k, sobol, filter, fx, fy, sample, p);
+    if (TRACE && slots) trace_globals(k, sobol, fx, fy, sample, p, slots);
     ShadowRay sr;
     sr.active = false;
     bool cont = (p.bits & PATH_DONE) == 0u;
@@ -45,7 +47,7 @@ __device__ PathResult integrate(const SceneView &s, const LampView &lv, const Ke
             continue;
         }
         uint32_t *ev = (TRACE && p.step < 8u) ? trace + p.step * 8u : nullptr;
-        cont = path_step<TRACE, false>(s, lv, k, sobol, p, hit, sr, ev);
+        cont = path_step<TRACE, false>(s, lv, k, sobol, p, hit, sr, ev, slots);
         if (sr.active) {
             shadow = true;
             transmittance = V(1.0f, 1.0f, 1.0f);
@@ -165,6 +167,25 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(SceneView s, KernelParams
     }
 }
 
+// diagnostic path trace of one pixel and sample: the schema v2 slots and the
+// pixel's discrete events; one lane of one block does the work
+__global__ __launch_bounds__(kBlock) void path_trace_kernel(SceneView s, KernelParams k, const float4 *sobol,
+                                                             const float *filter, uint32_t sample, uint32_t px,
+                                                             uint32_t py, float4 *slots, uint32_t *records,
+                                                             uint32_t *spill) {
+    __shared__ uint32_t lds[kShortStack * kBlock];
+    __shared__ float4 lamp_sphere[kLampCache];
+    __shared__ uint32_t lamp_bits[kLampCache];
+    const LampView lv = stage_lamps(s, lamp_sphere, lamp_bits);
+    Stack st = block_stack(lds, spill);
+    if (threadIdx.x != 0u) return;
+    for (uint32_t i = 0; i < kTraceSlotCount; ++i) slots[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < 64; ++i) records[i] = 0u;
+    uint32_t rays = 0u, nodes = 0u, tris = 0u;
+    integrate<false, true>(s, lv, k, sobol, filter, px + k.window_x, py + k.window_y, sample, st, rays, nodes, tris,
+                           records, slots);
+}
+
 // primary hit identity: object, Cycles primitive, t bits, valid
 __global__ __launch_bounds__(kBlock) void primary_kernel(SceneView s, KernelParams k, const float4 *sobol,
                                                           const float *filter, uint32_t sample, uint32_t *records,
@@ -274,6 +295,14 @@ hipError_t launch_trace(const SceneView &s, const KernelParams &k, const float4 
     if (blocks == 0u) return hipSuccess;
     hipLaunchKernelGGL(trace_kernel, dim3(blocks), dim3(kBlock), 0, stream, s, k, sobol, filter, sample, records,
                        spill);
+    return hipGetLastError();
+}
+
+hipError_t launch_path_trace(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
+                             uint32_t sample, uint32_t px, uint32_t py, float4 *slots, uint32_t *records,
+                             uint32_t *spill, hipStream_t stream) {
+    hipLaunchKernelGGL(path_trace_kernel, dim3(1), dim3(kBlock), 0, stream, s, k, sobol, filter, sample, px, py,
+                       slots, records, spill);
     return hipGetLastError();
 }
 

@@ -2368,6 +2368,42 @@ __device__ __forceinline__ void apply_shadow(const KernelParams &k, PathVars &p,
     }
 }
 
+// Diagnostic path trace record, schema "psycles.cycles-path-trace" v2
+// (include/psycles/luisa/path_trace_schema.h:10-121): 328 float4 slots per
+// traced pixel and sample, 8 global slots then 72 per surface event for the
+// first 4 events; w = 1 marks a slot that was written.
+constexpr uint32_t kTraceSchemaVersion = 2u;
+constexpr uint32_t kTraceGlobalSlots = 8u;
+constexpr uint32_t kTraceEventSlots = 72u;
+constexpr uint32_t kTraceMaxEvents = 4u;
+constexpr uint32_t kTraceSlotCount = 328u;
+
+__device__ __forceinline__ void trace_slot(float4 *slots, uint32_t index, f3 v) {
+    slots[index] = make_float4(v.x, v.y, v.z, 1.0f);
+}
+
+// trace_uint32: a 32-bit word as two exact 16-bit halves
+__device__ __forceinline__ f3 trace_halves(uint32_t v, float z) {
+    return V((float)(v & 0xffffu), (float)(v >> 16), z);
+}
+
+// global slots written after the camera ray (src/luisa/path_kernel_setup.cpp:540-560)
+__device__ __forceinline__ void trace_globals(const KernelParams &k, const float4 *sobol, uint32_t fx, uint32_t fy,
+                                              uint32_t sample, const PathVars &p, float4 *slots) {
+    float4 fs = sample4(sobol, k.sequence_size, sample, p.rng_hash, 0u);
+    if (sample == 0u) {
+        fs.x = 0.5f;
+        fs.y = 0.5f;
+    }
+    trace_slot(slots, 0u, V((float)kTraceSchemaVersion, (float)fx, (float)(k.full_height - 1u - fy)));
+    trace_slot(slots, 1u, V((float)sample, (float)(p.rng_hash & 0xffffu), (float)(p.rng_hash >> 16)));
+    trace_slot(slots, 2u, V(fs.x, fs.y, 0.0f));
+    trace_slot(slots, 3u, V(0.0f, 0.0f, 0.0f));
+    trace_slot(slots, 4u, p.ray.o);
+    trace_slot(slots, 5u, p.ray.d);
+    trace_slot(slots, 6u, V(p.ray.tmin, p.ray.tmax, 0.5f));
+}
+
 // One path step.  Returns true when the path continues with p.ray; sr.active
 // is set when a shadow ray must be traced and applied before the next step.
 // LIGHTS = false compiles a step for scenes with no lamp and an empty light
@@ -2375,7 +2411,7 @@ __device__ __forceinline__ void apply_shadow(const KernelParams &k, PathVars &p,
 // without MIS, which is what the runtime tests select for such scenes.
 template <bool TRACE, bool LP, bool LIGHTS = true>
 __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelParams &k, const float4 *sobol,
-                          PathVars &p, const Hit &hit, ShadowRay &sr, uint32_t *ev) {
+                          PathVars &p, const Hit &hit, ShadowRay &sr, uint32_t *ev, float4 *slots = nullptr) {
     sr.active = false;
     if (TRACE && ev) {
         ev[0] = 1u;
@@ -2509,6 +2545,28 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                 return false;
             }
             p.throughput = p.throughput / prob;
+        }
+        // surface event slots, written once the path survived the roulette
+        // (src/luisa/path_kernel_surface_shading.cpp:270-340)
+        if (TRACE && slots && p.step < kTraceMaxEvents) {
+            float4 *e = slots + kTraceGlobalSlots + p.step * kTraceEventSlots;
+            trace_slot(e, 0u, V((float)p.step, (float)p.ps.rng_offset, (float)p.ps.bounce));
+            trace_slot(e, 1u, V((float)p.ps.transparent_bounce, (float)p.ps.diffuse_bounce,
+                                (float)p.ps.glossy_bounce));
+            trace_slot(e, 2u, V((float)p.ps.transmission_bounce, (float)(p.ps.visibility & 0xffffu),
+                                (float)(p.ps.visibility >> 16)));
+            trace_slot(e, 3u, trace_halves(p.ps.flag, prob));
+            trace_slot(e, 4u, p.throughput);
+            trace_slot(e, 5u, p.ray.o);
+            trace_slot(e, 6u, p.ray.d);
+            trace_slot(e, 7u, V(p.ray.tmin, p.ray.tmax, 0.5f));
+            trace_slot(e, 8u, V(hit.t, hit.u, hit.v));
+            trace_slot(e, 9u, V((float)hit.object, (float)hit.cprim, 1.0f));
+            trace_slot(e, 10u, trace_halves(sf.material, (float)cs.count));
+            trace_slot(e, 11u, sf.position);
+            trace_slot(e, 12u, sf.ng);
+            trace_slot(e, 13u, sf.ns);
+            trace_slot(e, 38u, trace_halves(runtime_flags(cs, sf.back_facing), 0.0f));
         }
     }
     if (p.ps.bounce == 0u) {

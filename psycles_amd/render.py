@@ -202,6 +202,17 @@ class RenderSession:
             self.handle, sample_index, total, out.ctypes.data_as(C.POINTER(C.c_uint32))))
         return out
 
+    def path_trace(self, pixel_x: int, pixel_y: int, sample_index: int, total: int):
+        """Schema v2 path trace of one pixel (`pixel_y` counted from the lower
+        row of the full image) and sample: ((328, 4) float32 slots, (8, 8)
+        uint32 discrete events)."""
+        slots = np.zeros((abi.PATH_TRACE_SLOTS, 4), dtype=np.float32)
+        records = np.zeros((abi.TRACE_EVENTS, abi.TRACE_WORDS), dtype=np.uint32)
+        abi.check("psx_session_path_trace", self._backend.lib.psx_session_path_trace(
+            self.handle, pixel_x, pixel_y, sample_index, total, slots.ctypes.data_as(C.POINTER(C.c_float)),
+            records.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return slots, records
+
     def sampler_probe(self, sample_index: int, total: int, first_dim: int, count: int) -> np.ndarray:
         w, h = self.window
         out = np.zeros((h, w, 1 + 4 * count), dtype=np.uint32)

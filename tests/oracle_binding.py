@@ -99,6 +99,7 @@ def load():
                                       P(f32)]
     lib.oracle_primary_hits.argtypes = [vp, P(abi.RenderSettings), u32, u32, P(u32)]
     lib.oracle_trace_events.argtypes = [vp, P(abi.RenderSettings), u32, u32, P(u32)]
+    lib.oracle_path_trace.argtypes = [vp, P(abi.RenderSettings), u32, u32, u32, u32, P(f32), P(u32)]
     lib.oracle_sampler_probe.argtypes = [P(abi.RenderSettings), u32, u32, u32, u32, P(u32)]
     lib.oracle_light_tree_build.argtypes = [P(TreeEmitter), u32, u32, P(LightTree)]
     lib.oracle_light_tree_query.argtypes = [vp, P(f32), P(f32), C.c_int, f32, P(u32), P(f32), P(f32)]
@@ -227,6 +228,14 @@ class OracleScene:
         load().oracle_trace_events(self.handle, C.byref(settings), sample_index, total,
                                    uptr(out))
         return out
+
+    def path_trace(self, settings, x, y, sample_index, total):
+        """((328, 4) float32 schema v2 slots, (8, 8) uint32 events)."""
+        slots = np.zeros((abi.PATH_TRACE_SLOTS, 4), dtype=np.float32)
+        records = np.zeros((abi.TRACE_EVENTS, abi.TRACE_WORDS), dtype=np.uint32)
+        load().oracle_path_trace(self.handle, C.byref(settings), x, y, sample_index, total,
+                                 fptr(slots), uptr(records))
+        return slots, records
 
 
 LIGHT_PASSES = 12  # diffuse, glossy, transmission, volume (direct, indirect), emission, environment, colours

@@ -1,0 +1,124 @@
+"""Schema v2 path trace records from the oracle (CPU).
+
+The record restates include/psycles/luisa/path_trace_schema.h:10-121 as
+written by src/luisa/path_kernel_setup.cpp:540-560 (global slots) and
+src/luisa/path_kernel_surface_shading.cpp:270-340 (surface event slots 0-13
+and 38).  Checks tie the slots to independent quantities: the sampler
+probe's pixel hash, the discrete event log, the camera ray geometry and
+the hit distance along each event's ray.
+"""
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+from psycles_amd import abi, scenes
+
+GLOBAL, EVENT, MAX_EVENTS = 8, 72, 4
+
+
+def event(slots, i, slot):
+    return slots[GLOBAL + i * EVENT + slot]
+
+
+@pytest.fixture(scope="module")
+def cornell():
+    snap = scenes.cornell_box()
+    return ob.OracleScene(snap), scenes.cornell_settings(32, 20)
+
+
+@pytest.mark.parametrize("pixel, sample", [((16, 10), 0), ((3, 17), 5), ((30, 1), 11)])
+def test_globals_match_hash_and_camera(cornell, pixel, sample):
+    scene, s = cornell
+    x, y = pixel
+    slots, _ = scene.path_trace(s, x, y, sample, 16)
+    assert slots.shape == (abi.PATH_TRACE_SLOTS, 4)
+    assert list(slots[0, :3]) == [2.0, float(x), float(s.height - 1 - y)]
+    probe = ob.sampler_probe(s, sample, 16, 0, 1)
+    h = int(probe[y, x, 0])
+    assert list(slots[1, :3]) == [float(sample), float(h & 0xFFFF), float(h >> 16)]
+    fs = probe[y, x, 1:3].view(np.float32)
+    expected = (0.5, 0.5) if sample == 0 else tuple(fs)
+    assert tuple(slots[2, :2]) == expected
+    assert np.all(slots[:7, 3] == 1.0) and slots[7, 3] == 0.0
+    d = slots[5, :3]
+    assert abs(np.linalg.norm(d) - 1.0) < 1e-6
+    # perspective camera at (0, 0, 3.9) looking down -z: the ray starts at
+    # the near clip along its own direction
+    origin = np.array([0.0, 0.0, 3.9], np.float32)
+    along = slots[4, :3] - origin
+    assert np.allclose(np.cross(along, d), 0.0, atol=1e-6) and along @ d > 0.0
+
+
+def test_events_agree_with_event_log_and_geometry(cornell):
+    scene, s = cornell
+    written = 0
+    for y in range(0, 20, 3):
+        for x in range(0, 32, 3):
+            slots, records = scene.path_trace(s, x, y, 3, 16)
+            log = scene.trace_events(s, 3, 16)[y, x]
+            assert np.array_equal(records, log)
+            for i in range(MAX_EVENTS):
+                flags = int(records[i, abi.TRACE_FLAGS])
+                hit = flags & 2 and not flags & 8
+                w = event(slots, i, 0)[3]
+                # slots are written exactly for surface events that survive the roulette
+                assert (w == 1.0) == bool(hit and records[i, 7] != 0), (x, y, i, flags)
+                if w != 1.0:
+                    assert not np.any(slots[GLOBAL + i * EVENT:GLOBAL + (i + 1) * EVENT])
+                    continue
+                written += 1
+                assert event(slots, i, 0)[0] == i
+                assert event(slots, i, 0)[1] == records[i, 1]          # rng offset
+                assert event(slots, i, 9)[0] == float(records[i, 2])   # object
+                assert event(slots, i, 9)[1] == float(records[i, 3])   # primitive
+                assert event(slots, i, 3)[2] == records[i, 7:8].view(np.float32)[0]  # RR probability
+                o, d, t = event(slots, i, 5)[:3], event(slots, i, 6)[:3], event(slots, i, 8)[0]
+                assert np.allclose(o + t * d, event(slots, i, 11)[:3], atol=1e-4)
+                assert abs(np.linalg.norm(event(slots, i, 13)[:3]) - 1.0) < 1e-5
+                assert event(slots, i, 7)[0] <= t <= event(slots, i, 7)[1]
+                if i == 0:
+                    assert np.array_equal(event(slots, 0, 5)[:3], slots[4, :3])
+                    assert np.array_equal(event(slots, 0, 6)[:3], slots[5, :3])
+                    assert list(event(slots, 0, 4)[:3]) == [1.0, 1.0, 1.0]
+                else:
+                    # the next ray leaves the previous surface point
+                    assert np.allclose(event(slots, i, 5)[:3], event(slots, i - 1, 11)[:3], atol=1e-4)
+    assert written > 100
+
+
+def test_state_counters_follow_bounces():
+    """Opaque diffuse walls: every event is one more diffuse bounce."""
+    scene = ob.OracleScene(scenes.cornell_box())
+    s = scenes.cornell_settings(8, 8)
+    s.integrator.min_bounces = 8
+    slots, _ = scene.path_trace(s, 4, 4, 1, 4)
+    for i in range(MAX_EVENTS):
+        if event(slots, i, 0)[3] != 1.0:
+            break
+        depth = event(slots, i, 0)
+        lobes = event(slots, i, 1)
+        assert depth[2] == i and lobes[0] == 0.0 and lobes[1] == i and lobes[2] == 0.0
+        assert depth[1] == 16 * (i + 1)
+        assert event(slots, i, 3)[2] == 1.0        # before the minimum bounce count
+
+
+def test_decode_and_raw_round_trip(cornell, tmp_path):
+    from psycles_amd import path_trace
+    scene, s = cornell
+    slots, records = scene.path_trace(s, 16, 10, 2, 16)
+    trace = path_trace.PathTrace(16, s.height - 1 - 10, 2, slots)
+    path_trace.write_raw_path_trace(trace, tmp_path / "trace.json")
+    back = path_trace.read_raw_path_trace(tmp_path / "trace.json")
+    assert np.array_equal(back.slots, slots) and (back.pixel_x, back.pixel_y, back.sample) == (16, 9, 2)
+    decoded = path_trace.decode(back)
+    assert decoded["global"]["header"] == (2.0, 16.0, 9.0)
+    assert "reserved" not in decoded["global"]
+    events = decoded["events"]
+    assert 1 <= len(events) <= 4
+    for i, e in enumerate(events):
+        assert e["state_depth"][0] == i and e["isect_id"][0] == records[i, abi.TRACE_OBJECT]
+        assert e["path_flag"] == path_trace.uint32_from_halves(*e["state_flags"][:2])
+    # schema index helpers agree with the header's layout
+    assert path_trace.index(0, "state_depth") == 8 and path_trace.index(3, "nee_contribution") == 8 + 3 * 72 + 47
+    assert path_trace.closure_index(1, 7, 2) == 8 + 72 + 48 + 23
+    assert path_trace.shadow_index(3, "shadow_transmittance") == 296 + 3 * 8 + 7 == 327
