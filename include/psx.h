@@ -433,7 +433,7 @@ int psx_session_trace_events(psx_session *session, uint32_t sample_index,
  * full image, `pixel_y` a row counted from the lower row of the full image
  * (Cycles' film convention); the pixel must lie in the session's window.
  * w = 1 marks a written slot.  Written: the global slots 0-6 and, for the
- * first 4 surface events, event slots 0-13 and 38.  `out_records`
+ * first 4 surface events, event slots 0-16, 32-36, 38 and 39.  `out_records`
  * (optional) receives the pixel's PSX_TRACE_EVENTS x PSX_TRACE_WORDS
  * discrete events. */
 #define PSX_PATH_TRACE_SLOTS 328u

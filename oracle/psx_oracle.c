@@ -3848,6 +3848,9 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                 trace_slot3(e, 13u, sf.ns);
                 const uint32_t rf = surface_runtime_flags(&cs, sf.back_facing);
                 trace_slot(e, 38u, lo16(rf), hi16(rf), 0.0f);
+                trace_slot(e, 14u, u_term, k->light_inv_rr_threshold > 0.0f ? u_lightterm : 0.0f, 0.0f);
+                trace_slot3(e, 15u, u_light);
+                trace_slot3(e, 16u, u_bsdf);
             }
         }
         if (ps.bounce == 0u) {
@@ -4073,6 +4076,16 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
         }
         src_obj = sf.object;
         src_prim = sf.cprim;
+        /* post-scatter slots (src/luisa/path_kernel_surface_scatter.cpp:293-320) */
+        if (log && log->slots && step < TRACE_MAX_EVENTS) {
+            float *e = log->slots + 4u * (TRACE_GLOBAL_SLOTS + step * TRACE_EVENT_SLOTS);
+            trace_slot(e, 32u, (float)ps.bounce, (float)ps.transparent_bounce, (float)ps.rng_offset);
+            trace_slot3(e, 33u, throughput);
+            trace_slot3(e, 34u, ray.o);
+            trace_slot3(e, 35u, ray.d);
+            trace_slot(e, 36u, lo16(ps.flag), hi16(ps.flag), 0.0f);
+            trace_slot(e, 39u, lo16(ps.visibility), hi16(ps.visibility), 0.0f);
+        }
     }
     (void)minimum_bsdf_pdf;
     (void)ray_events;

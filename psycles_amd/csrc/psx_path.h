@@ -2567,6 +2567,12 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
             trace_slot(e, 12u, sf.ng);
             trace_slot(e, 13u, sf.ns);
             trace_slot(e, 38u, trace_halves(runtime_flags(cs, sf.back_facing), 0.0f));
+            const float u_lightterm = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 2u).x;
+            trace_slot(e, 14u, V(u_term, k.light_inv_rr_threshold > 0.0f ? u_lightterm : 0.0f, 0.0f));
+            const float4 ul = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 1u);
+            trace_slot(e, 15u, V(ul.x, ul.y, ul.z));
+            const float4 ub = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 3u);
+            trace_slot(e, 16u, V(ub.x, ub.y, ub.z));
         }
     }
     if (p.ps.bounce == 0u) {
@@ -2758,6 +2764,16 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
     }
     p.src_obj = sf.object;
     p.src_prim = sf.cprim;
+    // post-scatter slots (src/luisa/path_kernel_surface_scatter.cpp:293-320)
+    if (TRACE && slots && p.step < kTraceMaxEvents) {
+        float4 *e = slots + kTraceGlobalSlots + p.step * kTraceEventSlots;
+        trace_slot(e, 32u, V((float)p.ps.bounce, (float)p.ps.transparent_bounce, (float)p.ps.rng_offset));
+        trace_slot(e, 33u, p.throughput);
+        trace_slot(e, 34u, p.ray.o);
+        trace_slot(e, 35u, p.ray.d);
+        trace_slot(e, 36u, trace_halves(p.ps.flag, 0.0f));
+        trace_slot(e, 39u, trace_halves(p.ps.visibility, 0.0f));
+    }
     p.step += 1u;
     return p.step < k.max_path_steps;
 }

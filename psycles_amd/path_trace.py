@@ -6,7 +6,7 @@ one pixel (``pixel_y`` counted from the lower row of the full image, Cycles'
 film convention) and one sample, 328 float4 slots laid out as
 include/psycles/luisa/path_trace_schema.h:10-121 defines them.  The device
 writes the global slots 0-6 and, for the first 4 surface events, event slots
-0-13 and 38; w = 1 marks a written slot.  `write_raw_path_trace` writes the
+0-16, 32-36, 38 and 39; w = 1 marks a written slot.  `write_raw_path_trace` writes the
 JSON document of examples/render_blender_scene.cpp:77-112
 ("psycles.cycles-path-trace-raw"); `decode` names the written slots.
 """

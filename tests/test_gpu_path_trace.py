@@ -2,9 +2,9 @@
 
 One pixel and sample per call, as LuisaPathTraceRequest does.  Bars: the
 discrete slots (header, hash, state counters, visibility and flag words,
-hit identity, material / closure count, runtime flags) and which slots are
-written are identical; the float slots (rays, throughput, hit distance,
-barycentrics, surface frame) agree within 1e-4 relative (1e-5 absolute
+hit identity, material / closure count, runtime flags, sampler values,
+post-scatter state) and which slots are written are identical; the float slots (rays, throughput, hit distance,
+barycentrics, surface frame, scattered ray and throughput) agree within 1e-4 relative (1e-5 absolute
 floor), each on >= 99.9 % of the traced pixels; the discrete words of the
 same call's event log (flags, RNG offset, hit, light and closure choice)
 equal the oracle's.
@@ -21,8 +21,10 @@ from psycles_amd.render import RendererBackend
 pytestmark = pytest.mark.gpu
 
 GLOBAL, EVENT = 8, 72
-DISCRETE_EVENT = (0, 1, 2, 9, 10, 38)        # state_depth, lobes, visibility, isect_id, surface_meta, flags
-FLOAT_EVENT = (4, 5, 6, 7, 8, 11, 12, 13)   # throughput, ray, range, isect_coord, surface p / ng / n
+# state counters, hit identity, material, sampler values, post-scatter state and flag words
+DISCRETE_EVENT = (0, 1, 2, 9, 10, 14, 15, 16, 32, 36, 38, 39)
+# throughput, rays, range, hit distance and barycentrics, surface frame
+FLOAT_EVENT = (4, 5, 6, 7, 8, 11, 12, 13, 33, 34, 35)
 PIXEL_FRACTION = 0.999
 
 

@@ -59,6 +59,8 @@ def test_events_agree_with_event_log_and_geometry(cornell):
             assert np.array_equal(records, log)
             for i in range(MAX_EVENTS):
                 flags = int(records[i, abi.TRACE_FLAGS])
+                if event(slots, i, 32)[3] == 1.0:
+                    assert flags & 32               # post slots only after a sampled scatter
                 hit = flags & 2 and not flags & 8
                 w = event(slots, i, 0)[3]
                 # slots are written exactly for surface events that survive the roulette
@@ -76,6 +78,18 @@ def test_events_agree_with_event_log_and_geometry(cornell):
                 assert np.allclose(o + t * d, event(slots, i, 11)[:3], atol=1e-4)
                 assert abs(np.linalg.norm(event(slots, i, 13)[:3]) - 1.0) < 1e-5
                 assert event(slots, i, 7)[0] <= t <= event(slots, i, 7)[1]
+                # sampler dimensions of this event's RNG offset: terminate,
+                # light, light-terminate and BSDF samples
+                dims = ob.sampler_probe(s, 3, 16, int(records[i, 1]), 4)[y, x, 1:].view(np.float32).reshape(4, 4)
+                assert event(slots, i, 14)[0] == dims[0, 0] and event(slots, i, 14)[1] == dims[2, 0]
+                assert np.array_equal(event(slots, i, 15)[:3], dims[1, :3])
+                assert np.array_equal(event(slots, i, 16)[:3], dims[3, :3])
+                post = event(slots, i, 32)
+                if post[3] == 1.0 and i + 1 < MAX_EVENTS and event(slots, i + 1, 0)[3] == 1.0:
+                    # the scattered ray and state are the next event's
+                    assert np.array_equal(event(slots, i, 34)[:3], event(slots, i + 1, 5)[:3])
+                    assert np.array_equal(event(slots, i, 35)[:3], event(slots, i + 1, 6)[:3])
+                    assert post[2] == event(slots, i + 1, 0)[1] and post[0] == event(slots, i + 1, 0)[2]
                 if i == 0:
                     assert np.array_equal(event(slots, 0, 5)[:3], slots[4, :3])
                     assert np.array_equal(event(slots, 0, 6)[:3], slots[5, :3])
