@@ -6,7 +6,7 @@ bounces, mix(diffuse, transparent) materials, constant world background.
 Inputs (scene tables, BVH, film) are resident in HBM before the timed region.
 
 The timed path is the batched wavefront integrator (trace and shade kernels
-over path state in HBM, 32 samples of every pixel per dispatch).  The plain
+over path state in HBM, all 64 samples of every pixel in one dispatch).  The plain
 megakernel (one lane runs whole paths) is timed on one frame beside it.
 
 `--workload many_lights` times the many-light scene instead (the instanced
@@ -34,7 +34,7 @@ METRIC = "Mrays/s & samples/s (batched vs plain) at 1 GPU; p50/p99 request laten
 
 FILM_BYTES_PER_PIXEL_DISPATCH = 104.0
 HBM_PEAK_GBS = 8000.0
-SAMPLES_PER_DISPATCH = 32
+SAMPLES_PER_DISPATCH = 64
 
 
 def bytes_per_ray(nodes: float, tris: float) -> float:
@@ -53,7 +53,7 @@ def bytes_per_ray(nodes: float, tris: float) -> float:
 WORKLOADS = {
     "instanced": {
         "scene": "instanced_icospheres", "settings": "instanced_settings", "spp": 64,
-        "nodes": 46.83, "tris": 8.59, "traffic": 1117.8,
+        "nodes": 46.83, "tris": 8.59, "traffic": 1120.0,
         "describe": "instanced icospheres: 1,003,520-tri displaced icosphere x 64 instances, "
                     "mix(diffuse, transparent) materials, ground quad, constant background; "
                     "1920x1080, {spp} spp, max 8 bounces, seed 0",
