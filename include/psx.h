@@ -460,6 +460,119 @@ int psx_session_sampler_probe(psx_session *session, uint32_t sample_index,
                               uint32_t total, uint32_t first_dimension,
                               uint32_t dimension_count, uint32_t *out);
 
+/* ---- ray queries ----------------------------------------------------------
+ *
+ * Closest-hit and shadow queries against a compiled scene: the traversal
+ * contract SceneTraversalComponent::closest / closest_shadow
+ * (src/luisa/path_kernel_scene_traversal.cpp:34-68, 212-526: closed
+ * interval [tmin, tmax], source and light identities excluded, visibility
+ * mask, equal distances resolved to the greatest (object, primitive)), and
+ * for shadow queries the transparent shadow loop trace_shadow
+ * (src/luisa/path_tracer_geometry.cpp:75-174).  Objects are instance
+ * indices; primitives are scene primitive ids: the sum of the triangle
+ * counts of the earlier meshes plus the triangle's index in its mesh. */
+#define PSX_INVALID_ID 0xffffffffu
+enum psx_query_kind {
+    PSX_QUERY_CLOSEST = 0,  /* closest hit                                  */
+    PSX_QUERY_SHADOW = 1    /* transmittance through transparent surfaces   */
+};
+typedef struct psx_ray_query {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+    uint32_t visibility;        /* psx_visibility bits an instance must share */
+    uint32_t source_object;     /* excluded identity or PSX_INVALID_ID        */
+    uint32_t source_primitive;
+    uint32_t light_object;      /* second excluded identity                   */
+    uint32_t light_primitive;
+    uint32_t kind;              /* enum psx_query_kind                        */
+    uint32_t transparent_depth; /* shadow: transparent bounces already taken  */
+    uint32_t transparent_max;   /* shadow: transparent bounce limit           */
+} psx_ray_query;                /* 64 bytes */
+
+typedef struct psx_ray_hit {
+    float t, u, v;              /* closest: distance and barycentrics         */
+    uint32_t object;            /* closest: instance or PSX_INVALID_ID        */
+    uint32_t primitive;         /* closest: scene primitive id                */
+    uint32_t hit;               /* closest: 1 hit; shadow: 1 fully occluded   */
+    uint32_t traversals;        /* traversal passes the query used            */
+    uint32_t reserved;
+    float transmittance[3];     /* shadow: product of transparent extinctions */
+    uint32_t reserved2;
+} psx_ray_hit;                  /* 48 bytes */
+
+/* Host arrays in and out; synchronous.  A shadow query returns as soon as
+ * any opaque surface lies in its interval (its transmittance is then exactly
+ * zero); transparent surfaces before it are taken in distance order. */
+int psx_scene_intersect(psx_scene *scene, const psx_ray_query *queries, uint32_t count, psx_ray_hit *hits);
+/* The same with device-resident queries and hits (device pointers of the
+ * scene's device); allocates its traversal scratch for the call and returns
+ * when the hits are written. */
+int psx_scene_intersect_device(psx_scene *scene, const psx_ray_query *device_queries, uint32_t count,
+                               psx_ray_hit *device_hits);
+
+/* ---- single-function probes ----------------------------------------------
+ *
+ * Evaluates one device function of the render path on `count` input records
+ * (host arrays, synchronous), so that parity tests can hold the device code
+ * itself against the reference's function fixtures.  Record layouts, all
+ * float (flags 0 or 1, integers as exact floats):
+ *   FILM_CLAMP          in  contribution[3], depth, direct limit,
+ *                           indirect limit (6)
+ *                       out contribution[3] (3)
+ *                       (cycles_film_light::clamp,
+ *                        include/psycles/luisa/cycles_film_light.h:14-50)
+ *   T_OFFSET            in  distance (1) out next tmin (1)
+ *                       (surface_ray::intersection_t_offset)
+ *   SELF_EXCLUSION      in  world origin[3], world geometric normal[3],
+ *                           intersection-space origin[3] and direction[3],
+ *                           p0[3], p1[3], p2[3] (21)
+ *                       out origin[3], certified (4)
+ *                       (origin_with_explicit_self_exclusion)
+ *   SHADOW_TERMINATOR   in  position[3], shading normal[3], geometric
+ *                           normal[3], light direction[3], geometry offset,
+ *                           smooth, transform applied, barycentric[2],
+ *                           p0..p2[9], n0..n2[9], object-to-world affine[12]
+ *                           (columns x, y, z, translation) (47)
+ *                       out position[3], skip_self (4)
+ *                       (shadow_terminator_origin,
+ *                        include/psycles/luisa/surface_ray.h:183-243)
+ *   SURFACE_SHADOW_ORIGIN
+ *                       in  the SHADOW_TERMINATOR record, then the
+ *                           world-to-object affine[12] (59)
+ *                       out position[3], skip_self (4)
+ *                       (surface_shadow_origin: the lift, then the source
+ *                        triangle certificate while skip_self,
+ *                        include/psycles/luisa/surface_ray.h:252-309)
+ *   TRIANGLE_SAMPLE     in  reference point[3], p0..p2[9], random[2] (14)
+ *                       out position[3], direction[3], distance, pdf,
+ *                           valid, solid angle used (10)
+ *                       (triangle_light_sample<false>,
+ *                        include/psycles/luisa/triangle_light_sampling.h)
+ *   TRIANGLE_PDF        in  reference point[3], light point[3], p0..p2[9] (15)
+ *                       out pdf, valid, solid angle used (3)
+ *   DIFFUSE_EVAL        in  normal[3], roughness, colour[3], incoming[3],
+ *                           outgoing[3] (13)
+ *                       out cosine-weighted intensity[3], pdf (4)
+ *                       (Lambert / Oren-Nayar,
+ *                        src/luisa/graph_surface_scattering.cpp:463-506) */
+enum psx_probe_function {
+    PSX_PROBE_FILM_CLAMP = 0,
+    PSX_PROBE_T_OFFSET = 1,
+    PSX_PROBE_SELF_EXCLUSION = 2,
+    PSX_PROBE_SHADOW_TERMINATOR = 3,
+    PSX_PROBE_TRIANGLE_SAMPLE = 4,
+    PSX_PROBE_TRIANGLE_PDF = 5,
+    PSX_PROBE_DIFFUSE_EVAL = 6,
+    PSX_PROBE_SURFACE_SHADOW_ORIGIN = 7
+};
+/* Input and output floats per record of a probe function; 0 if unknown. */
+uint32_t psx_probe_input_width(uint32_t function);
+uint32_t psx_probe_output_width(uint32_t function);
+int psx_backend_probe(psx_backend *backend, uint32_t function, const float *inputs, uint32_t count,
+                      float *outputs);
+
 /* ---- render server ------------------------------------------------------
  *
  * A request loop over resident compiled scenes (the serving form of

@@ -801,6 +801,7 @@ typedef struct oinstance {
     mat4 o2w, w2o, n2w;
     uint32_t *override;
     uint32_t override_count;
+    float terminator_offset; /* shadow_terminator_geometry_offset */
     float lo[3], hi[3];     /* world bounds */
 } oinstance;
 
@@ -1755,6 +1756,7 @@ oracle_scene *oracle_scene_create(const psx_scene_desc *d, int brute_force) {
         inst->override = (uint32_t *)dupmem(src->material_overrides,
                                             sizeof(uint32_t) * src->material_override_count);
         inst->override_count = src->material_override_count;
+        inst->terminator_offset = src->shadow_terminator_geometry_offset;
         const omesh *m = &s->meshes[src->mesh];
         inst->applied = m->users == 1u;
         if (inst->applied) {
@@ -2133,6 +2135,7 @@ typedef struct surface_t {
     uint32_t inst, prim, object, cprim, material;
     int applied, smooth, back_facing;
     v3 p0, p1, p2;        /* intersection-space positions (cycles p) */
+    v3 cn0, cn1, cn2;     /* intersection-space vertex normals (cycles n) */
     v3 wp0, wp1, wp2;     /* world positions */
     v3 position, object_hit_position;
     v3 ng, ns;
@@ -2191,6 +2194,7 @@ static void resolve_surface(const oracle_scene *s, const hit_t *h, const ray_t *
     const v3 cn0 = sf->applied ? wn0 : n0;
     const v3 cn1 = sf->applied ? wn1 : n1;
     const v3 cn2 = sf->applied ? wn2 : n2;
+    sf->cn0 = cn0; sf->cn1 = cn1; sf->cn2 = cn2;
     const v3 osn = sf->smooth ? interp3(h->u, h->v, n0, n1, n2) : ogn;
     const v3 tsn = safe_normalize(xform_dir(&inst->n2w, osn), gn);
     const v3 ssn = safe_normalize(sf->smooth ? interp3(h->u, h->v, cn0, cn1, cn2) : gn, gn);
@@ -2270,14 +2274,93 @@ static v3 surface_ray_origin(const oracle_scene *s, const surface_t *sf, v3 dir)
     return origin_with_self_exclusion(sf->position, sf->ng, co, cd, sf->p0, sf->p1, sf->p2);
 }
 
-/* make_shadow_origin + surface_shadow_origin with a zero shadow-terminator
- * geometry offset (surface_ray.h:268-330): the smooth-surface lift is
- * inactive and the source primitive stays excluded. */
-static v3 surface_shadow_origin(const oracle_scene *s, const surface_t *sf, v3 dir) {
+/* Cycles' parabolic smooth-surface height (smooth_surface_offset,
+ * include/psycles/luisa/surface_ray.h:108-181): positions and normals in
+ * intersection space, the interpolated normal taken to world space with the
+ * instance's FMA affine unless the transform is applied. */
+static v3 smooth_surface_offset(const mat4 *object_to_world, int applied, float bu, float bv, v3 p0, v3 p1,
+                                v3 p2, v3 n0, v3 n1, v3 n2, v3 world_ng) {
+    const float u = 1.0f - bu - bv;
+    const float v = bu;
+    const float w = bv;
+    const v3 object_position = vadd(vadd(vscale(p0, u), vscale(p1, v)), vscale(p2, w));
+    const v3 object_normal = vadd(vadd(vscale(n0, u), vscale(n1, v)), vscale(n2, w));
+    const v3 world_normal = applied ? object_normal : xform_dir(object_to_world, object_normal);
+    const float a = vdot(vsub(n2, n0), vsub(p0, p2));
+    const float b = vdot(vsub(n2, n1), vsub(p1, p2));
+    const float c = vdot(vsub(n1, n0), vsub(p1, p0));
+    const float parabolic = ((a * u) * (u - 1.0f) + ((a + b) + c) * u * v) + (b * v) * (v - 1.0f);
+    float f0 = fmaxf(fmaxf(vdot(vsub(p1, p0), n0), vdot(vsub(p2, p0), n0)), 0.0f);
+    float f1 = fmaxf(fmaxf(vdot(vsub(p0, p1), n1), vdot(vsub(p2, p1), n1)), 0.0f);
+    float f2 = fmaxf(fmaxf(vdot(vsub(p0, p2), n2), vdot(vsub(p1, p2), n2)), 0.0f);
+    f0 = fmaxf(vdot(vsub(p0, object_position), n0) + f0, 0.0f);
+    f1 = fmaxf(vdot(vsub(p1, object_position), n1) + f1, 0.0f);
+    f2 = fmaxf(vdot(vsub(p2, object_position), n2) + f2, 0.0f);
+    const float front = fmaxf(fminf(f0, fminf(f1, f2)), parabolic * 0.5f);
+    float b0 = fmaxf(fmaxf(vdot(vsub(p0, p1), n0), vdot(vsub(p0, p2), n0)), 0.0f);
+    float b1 = fmaxf(fmaxf(vdot(vsub(p1, p0), n1), vdot(vsub(p1, p2), n1)), 0.0f);
+    float b2 = fmaxf(fmaxf(vdot(vsub(p2, p0), n2), vdot(vsub(p2, p1), n2)), 0.0f);
+    b0 = fmaxf(vdot(vsub(object_position, p0), n0) + b0, 0.0f);
+    b1 = fmaxf(vdot(vsub(object_position, p1), n1) + b1, 0.0f);
+    b2 = fmaxf(vdot(vsub(object_position, p2), n2) + b2, 0.0f);
+    const float back = fminf(-fminf(b0, fminf(b1, b2)), parabolic * 0.5f);
+    const float height = vdot(world_normal, world_ng) > 0.0f ? front : back;
+    return vscale(world_normal, height);
+}
+
+typedef struct shadow_origin_t {
+    v3 position;
+    int skip_self;
+} shadow_origin_t;
+
+/* shadow_terminator_origin (include/psycles/luisa/surface_ray.h:183-243):
+ * Cycles' shadow-terminator geometry offset; the source primitive stays
+ * excluded unless the lifted origin faces away from the light. */
+static shadow_origin_t shadow_terminator_origin(v3 world_position, v3 world_ns, v3 world_ng, v3 light_dir,
+                                                float geometry_offset, int smooth, const mat4 *o2w, int applied,
+                                                float bu, float bv, v3 p0, v3 p1, v3 p2, v3 n0, v3 n1, v3 n2) {
+    float normal_light = vdot(world_ns, light_dir);
+    const int transmit = normal_light < 0.0f;
+    normal_light = fabsf(normal_light);
+    const v3 offset_normal = transmit ? vneg(world_ng) : world_ng;
+    const float geometric_light = vdot(offset_normal, light_dir);
+    const float safe_cutoff = fmaxf(geometry_offset, 1.0e-20f);
+    const float near_terminator = fclamp(2.0f - (geometric_light + normal_light) / safe_cutoff, 0.0f, 1.0f);
+    const float regular = fclamp(1.0f - geometric_light / safe_cutoff, 0.0f, 1.0f);
+    const float amount = normal_light < geometry_offset ? near_terminator : regular;
+    const int active = smooth && geometry_offset > 0.0f && amount > 0.0f;
+    shadow_origin_t r;
+    r.position = world_position;
+    r.skip_self = 1;
+    if (active) {
+        const v3 offset = smooth_surface_offset(o2w, applied, bu, bv, p0, p1, p2, n0, n1, n2, offset_normal);
+        r.position = vadd(world_position, vscale(offset, amount));
+        r.skip_self = geometric_light > 0.0f;
+    }
+    return r;
+}
+
+/* make_shadow_origin (src/luisa/path_kernel_surface_geometry.cpp:168-182)
+ * and surface_shadow_origin (include/psycles/luisa/surface_ray.h:252-309):
+ * the terminator lift, then the source-triangle certificate with the robust
+ * offset fallback while the source primitive stays excluded. */
+static shadow_origin_t shadow_origin_core(v3 position, v3 ns, v3 ng, v3 dir, float geometry_offset, int smooth,
+                                          const mat4 *o2w, const mat4 *w2o, int applied, float bu, float bv,
+                                          v3 p0, v3 p1, v3 p2, v3 n0, v3 n1, v3 n2) {
+    shadow_origin_t o = shadow_terminator_origin(position, ns, ng, dir, geometry_offset, smooth, o2w, applied, bu,
+                                                 bv, p0, p1, p2, n0, n1, n2);
+    const v3 oo = applied ? o.position : xform_point(w2o, o.position);
+    const v3 od = applied ? dir : xform_dir(w2o, dir);
+    const v3 offset = origin_with_self_exclusion(o.position, ng, oo, od, p0, p1, p2);
+    if (o.skip_self) o.position = offset;
+    return o;
+}
+
+static shadow_origin_t surface_shadow_origin(const oracle_scene *s, const surface_t *sf, v3 dir) {
     const oinstance *inst = &s->instances[sf->inst];
-    const v3 oo = sf->applied ? sf->position : xform_point(&inst->w2o, sf->position);
-    const v3 od = sf->applied ? dir : xform_dir(&inst->w2o, dir);
-    return origin_with_self_exclusion(sf->position, sf->ng, oo, od, sf->p0, sf->p1, sf->p2);
+    return shadow_origin_core(sf->position, sf->ns, sf->ng, dir, inst->terminator_offset, sf->smooth, &inst->o2w,
+                              &inst->w2o, sf->applied, sf->u, sf->v, sf->p0, sf->p1, sf->p2, sf->cn0, sf->cn1,
+                              sf->cn2);
 }
 
 static uint32_t intersection_t_offset_bits(float t) {
@@ -2377,6 +2460,7 @@ typedef struct bsdf_sample_t {
     eval_t eval;
     uint32_t runtime_flags;
     uint32_t selected;
+    float rescaled;  /* the lobe random rescaled into the chosen closure */
 } bsdf_sample_t;
 
 static uint32_t closure_runtime_flags(const closure_rec *c) {
@@ -2427,9 +2511,9 @@ static bsdf_sample_t sample_closures(const closure_set *cs, const surface_t *sf,
         selected |= choose;
         accumulated = next;
     }
-    (void)rescaled;
     if (!selected) return r;
     r.selected = sel;
+    r.rescaled = rescaled;
     const closure_rec *c = &cs->c[sel];
     v3 direction;
     int transparent = c->kind == KIND_TRANSPARENT;
@@ -2620,6 +2704,42 @@ static int samples_back(uint32_t s) {
 static int sampled_side(uint32_t s, v3 ng, v3 dir) {
     const int back = vdot(ng, vneg(dir)) < 0.0f;
     return back ? samples_back(s) : samples_front(s);
+}
+
+/* trace_shadow (src/luisa/path_tracer_geometry.cpp:75-174): closest hits in
+ * distance order through surfaces whose transparent extinction carries
+ * light, each pass restarting one float past the last hit; the first opaque
+ * surface, or the transparent-bounce limit, ends the query with zero.
+ * `passes` receives the traversals issued. */
+static v3 trace_shadow(const oracle_scene *s, ray_t sray, uint32_t visibility, uint32_t src_obj,
+                       uint32_t src_prim, uint32_t light_obj, uint32_t light_prim, uint32_t tdepth,
+                       uint32_t transparent_max, trace_counters *counters, uint32_t *passes) {
+    v3 transmittance = V3(1, 1, 1);
+    for (;;) {
+        const hit_t sh = closest_hit(s, &sray, visibility, src_obj, src_prim, light_obj, light_prim, counters);
+        (*passes)++;
+        if (!sh.valid) break;
+        if (tdepth >= transparent_max) { transmittance = V3(0, 0, 0); break; }
+        const oinstance *si = &s->instances[sh.inst];
+        const uint32_t smat = material_of(s, si, sh.prim);
+        v3 ext = V3(0, 0, 0);
+        if (smat < s->material_count) {
+            surface_t ssf;
+            resolve_surface(s, &sh, &sray, &ssf);
+            closure_set scs;
+            material_closures(&s->materials[smat], ssf.ns, &scs);
+            ext = vclamp(transparent_extinction(&scs), 0.0f, 1.0f);
+        }
+        if (fmaxf(ext.x, fmaxf(ext.y, ext.z)) > 0.0f) {
+            transmittance = vmul(transmittance, ext);
+            tdepth += 1u;
+            sray.tmin = bitsf(intersection_t_offset_bits(sh.t));
+        } else {
+            transmittance = V3(0, 0, 0);
+            break;
+        }
+    }
+    return transmittance;
 }
 
 /* ======================================================================= */
@@ -3917,7 +4037,10 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                         const float rr = light_roulette(unshadowed, u_lightterm, k->light_inv_rr_threshold);
                         const v3 surviving = vscale(unshadowed, rr);
                         if (vany_nonzero(surviving)) {
-                            const v3 so = surface_shadow_origin(s, &sf, ls.direction);
+                            const shadow_origin_t sso = surface_shadow_origin(s, &sf, ls.direction);
+                            const v3 so = sso.position;
+                            const uint32_t sh_obj = sso.skip_self ? sf.object : UINT32_MAX;
+                            const uint32_t sh_prim = sso.skip_self ? sf.cprim : UINT32_MAX;
                             const v3 soff = vsub(ls.position, so);
                             const float sdist = sqrtf(fmaxf(vdot(soff, soff), 1.0e-20f));
                             ray_t sray;
@@ -3926,33 +4049,11 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                             sray.tmin = 0.0f;
                             sray.tmax = sdist;
                             /* trace_shadow (src/luisa/path_tracer_geometry.cpp:75-174) */
-                            v3 transmittance = V3(1, 1, 1);
-                            uint32_t tdepth = ps.transparent_bounce;
-                            for (;;) {
-                                const hit_t sh = closest_hit(s, &sray, PSX_VISIBILITY_SHADOW, sf.object,
-                                                             sf.cprim, eobj, eprim, counters);
-                                rays++;
-                                if (!sh.valid) break;
-                                if (tdepth >= k->transparent_max) { transmittance = V3(0, 0, 0); break; }
-                                const oinstance *si = &s->instances[sh.inst];
-                                const uint32_t smat = material_of(s, si, sh.prim);
-                                v3 ext = V3(0, 0, 0);
-                                if (smat < s->material_count) {
-                                    surface_t ssf;
-                                    resolve_surface(s, &sh, &sray, &ssf);
-                                    closure_set scs;
-                                    material_closures(&s->materials[smat], ssf.ns, &scs);
-                                    ext = vclamp(transparent_extinction(&scs), 0.0f, 1.0f);
-                                }
-                                if (fmaxf(ext.x, fmaxf(ext.y, ext.z)) > 0.0f) {
-                                    transmittance = vmul(transmittance, ext);
-                                    tdepth += 1u;
-                                    sray.tmin = bitsf(intersection_t_offset_bits(sh.t));
-                                } else {
-                                    transmittance = V3(0, 0, 0);
-                                    break;
-                                }
-                            }
+                            uint32_t passes = 0u;
+                            const v3 transmittance = trace_shadow(s, sray, PSX_VISIBILITY_SHADOW, sh_obj, sh_prim,
+                                                                  eobj, eprim, ps.transparent_bounce,
+                                                                  k->transparent_max, counters, &passes);
+                            rays += passes;
                             if (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f) {
                                 const v3 c = film_clamp(vmul(vmul(throughput, surviving), transmittance),
                                                         ps.bounce, k->clamp_direct, k->clamp_indirect);
@@ -3989,7 +4090,10 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                     const float rr = light_roulette(weighted, u_lightterm, k->light_inv_rr_threshold);
                     const v3 surviving = vscale(weighted, rr);
                     if (vany_nonzero(surviving)) {
-                        const v3 so = surface_shadow_origin(s, &sf, ls.direction);
+                        const shadow_origin_t sso = surface_shadow_origin(s, &sf, ls.direction);
+                        const v3 so = sso.position;
+                        const uint32_t sh_obj = sso.skip_self ? sf.object : UINT32_MAX;
+                        const uint32_t sh_prim = sso.skip_self ? sf.cprim : UINT32_MAX;
                         const v3 foff = vsub(ls.position, so);
                         const float fdist = sqrtf(fmaxf(vdot(foff, foff), 1.0e-20f));
                         ray_t sray;
@@ -3997,33 +4101,11 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                         sray.d = distant ? ls.direction : vdivs(foff, fdist);
                         sray.tmin = 0.0f;
                         sray.tmax = distant ? RAY_MAXIMUM : fdist;
-                        v3 transmittance = V3(1, 1, 1);
-                        uint32_t tdepth = ps.transparent_bounce;
-                        for (;;) {
-                            const hit_t sh = closest_hit(s, &sray, PSX_VISIBILITY_SHADOW, sf.object, sf.cprim,
-                                                         UINT32_MAX, UINT32_MAX, counters);
-                            rays++;
-                            if (!sh.valid) break;
-                            if (tdepth >= k->transparent_max) { transmittance = V3(0, 0, 0); break; }
-                            const oinstance *si = &s->instances[sh.inst];
-                            const uint32_t smat = material_of(s, si, sh.prim);
-                            v3 ext = V3(0, 0, 0);
-                            if (smat < s->material_count) {
-                                surface_t ssf;
-                                resolve_surface(s, &sh, &sray, &ssf);
-                                closure_set scs;
-                                material_closures(&s->materials[smat], ssf.ns, &scs);
-                                ext = vclamp(transparent_extinction(&scs), 0.0f, 1.0f);
-                            }
-                            if (fmaxf(ext.x, fmaxf(ext.y, ext.z)) > 0.0f) {
-                                transmittance = vmul(transmittance, ext);
-                                tdepth += 1u;
-                                sray.tmin = bitsf(intersection_t_offset_bits(sh.t));
-                            } else {
-                                transmittance = V3(0, 0, 0);
-                                break;
-                            }
-                        }
+                        uint32_t passes = 0u;
+                        const v3 transmittance = trace_shadow(s, sray, PSX_VISIBILITY_SHADOW, sh_obj, sh_prim,
+                                                              UINT32_MAX, UINT32_MAX, ps.transparent_bounce,
+                                                              k->transparent_max, counters, &passes);
+                        rays += passes;
                         if (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f) {
                             const v3 c = film_clamp(vmul(vmul(throughput, surviving), transmittance), ps.bounce,
                                                     k->clamp_direct, k->clamp_indirect);
@@ -4422,5 +4504,224 @@ int oracle_sampler_probe(const psx_render_settings *rs, uint32_t sample_index, u
             }
         }
     free(table);
+    return 0;
+}
+
+/* ======================================================================= */
+/* ray queries (SceneTraversalComponent::closest / closest_shadow,          */
+/* src/luisa/path_kernel_scene_traversal.cpp:282-526; trace_shadow,         */
+/* src/luisa/path_tracer_geometry.cpp:75-174)                               */
+/* ======================================================================= */
+
+int oracle_scene_intersect(const oracle_scene *s, const psx_ray_query *queries, uint32_t count,
+                           psx_ray_hit *hits) {
+    if (!s || (count && (!queries || !hits))) return -1;
+    for (uint32_t i = 0; i < count; ++i) {
+        const psx_ray_query *q = &queries[i];
+        psx_ray_hit *h = &hits[i];
+        memset(h, 0, sizeof(*h));
+        ray_t r;
+        r.o = load3(q->origin);
+        r.d = load3(q->direction);
+        r.tmin = q->tmin;
+        r.tmax = q->tmax;
+        if (q->kind == PSX_QUERY_SHADOW) {
+            uint32_t passes = 0u;
+            const v3 tr = trace_shadow(s, r, q->visibility, q->source_object, q->source_primitive,
+                                       q->light_object, q->light_primitive, q->transparent_depth,
+                                       q->transparent_max, NULL, &passes);
+            h->transmittance[0] = tr.x;
+            h->transmittance[1] = tr.y;
+            h->transmittance[2] = tr.z;
+            h->hit = !(tr.x > 0.0f || tr.y > 0.0f || tr.z > 0.0f);
+            h->traversals = passes;
+            h->object = h->primitive = PSX_INVALID_ID;
+            h->t = q->tmax;
+        } else {
+            const hit_t b = closest_hit(s, &r, q->visibility, q->source_object, q->source_primitive,
+                                        q->light_object, q->light_primitive, NULL);
+            h->traversals = 1u;
+            h->hit = b.valid ? 1u : 0u;
+            h->t = b.valid ? b.t : q->tmax;
+            h->u = b.valid ? b.u : 0.0f;
+            h->v = b.valid ? b.v : 0.0f;
+            h->object = b.valid ? b.object : PSX_INVALID_ID;
+            h->primitive = b.valid ? b.cprim : PSX_INVALID_ID;
+        }
+    }
+    return 0;
+}
+
+/* ======================================================================= */
+/* single-function probes: the record layouts of psx_backend_probe          */
+/* ======================================================================= */
+
+static mat4 affine12(const float *a) {
+    mat4 m;
+    memset(&m, 0, sizeof(m));
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 3; ++r) m.m[4 * c + r] = a[3 * c + r];
+    m.m[15] = 1.0f;
+    return m;
+}
+
+uint32_t oracle_probe_input_width(uint32_t function) {
+    static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u};
+    return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
+}
+
+uint32_t oracle_probe_output_width(uint32_t function) {
+    static const uint32_t w[] = {3u, 1u, 4u, 4u, 10u, 3u, 4u, 4u};
+    return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
+}
+
+int oracle_probe(uint32_t function, const float *inputs, uint32_t count, float *outputs) {
+    const uint32_t wi = oracle_probe_input_width(function), wo = oracle_probe_output_width(function);
+    if (!wi || (count && (!inputs || !outputs))) return -1;
+    for (uint32_t i = 0; i < count; ++i) {
+        const float *in = inputs + (size_t)wi * i;
+        float *out = outputs + (size_t)wo * i;
+        switch (function) {
+        case PSX_PROBE_FILM_CLAMP: {
+            const v3 c = film_clamp(load3(in), (uint32_t)in[3], in[4], in[5]);
+            out[0] = c.x; out[1] = c.y; out[2] = c.z;
+            break;
+        }
+        case PSX_PROBE_T_OFFSET:
+            out[0] = bitsf(intersection_t_offset_bits(in[0]));
+            break;
+        case PSX_PROBE_SELF_EXCLUSION: {
+            const v3 o = origin_with_self_exclusion(load3(in), load3(in + 3), load3(in + 6), load3(in + 9),
+                                                    load3(in + 12), load3(in + 15), load3(in + 18));
+            out[0] = o.x; out[1] = o.y; out[2] = o.z;
+            out[3] = source_covers_origin(load3(in + 6), load3(in + 9), load3(in + 12), load3(in + 15),
+                                          load3(in + 18)) ? 1.0f : 0.0f;
+            break;
+        }
+        case PSX_PROBE_SHADOW_TERMINATOR:
+        case PSX_PROBE_SURFACE_SHADOW_ORIGIN: {
+            const mat4 o2w = affine12(in + 35);
+            const int applied = in[14] != 0.0f;
+            const v3 ng = load3(in + 6), dir = load3(in + 9);
+            const v3 p0 = load3(in + 17), p1 = load3(in + 20), p2 = load3(in + 23);
+            shadow_origin_t o;
+            if (function == PSX_PROBE_SURFACE_SHADOW_ORIGIN) {
+                const mat4 w2o = affine12(in + 47);
+                o = shadow_origin_core(load3(in), load3(in + 3), ng, dir, in[12], in[13] != 0.0f, &o2w, &w2o,
+                                       applied, in[15], in[16], p0, p1, p2, load3(in + 26), load3(in + 29),
+                                       load3(in + 32));
+            } else {
+                o = shadow_terminator_origin(load3(in), load3(in + 3), ng, dir, in[12], in[13] != 0.0f, &o2w,
+                                             applied, in[15], in[16], p0, p1, p2, load3(in + 26), load3(in + 29),
+                                             load3(in + 32));
+            }
+            out[0] = o.position.x; out[1] = o.position.y; out[2] = o.position.z;
+            out[3] = o.skip_self ? 1.0f : 0.0f;
+            break;
+        }
+        case PSX_PROBE_TRIANGLE_SAMPLE: {
+            const v2 rnd = {in[12], in[13]};
+            const light_sample_t ls = sample_triangle(load3(in), load3(in + 3), load3(in + 6), load3(in + 9), rnd);
+            out[0] = ls.position.x; out[1] = ls.position.y; out[2] = ls.position.z;
+            out[3] = ls.direction.x; out[4] = ls.direction.y; out[5] = ls.direction.z;
+            out[6] = ls.distance;
+            out[7] = ls.pdf;
+            out[8] = ls.valid ? 1.0f : 0.0f;
+            out[9] = use_solid_angle(load3(in), load3(in + 3), load3(in + 6), load3(in + 9)) ? 1.0f : 0.0f;
+            break;
+        }
+        case PSX_PROBE_TRIANGLE_PDF: {
+            const tri_pdf_t p = triangle_directional_pdf(load3(in), load3(in + 3), load3(in + 6), load3(in + 9),
+                                                         load3(in + 12));
+            out[0] = p.value;
+            out[1] = p.valid ? 1.0f : 0.0f;
+            out[2] = use_solid_angle(load3(in), load3(in + 6), load3(in + 9), load3(in + 12)) ? 1.0f : 0.0f;
+            break;
+        }
+        case PSX_PROBE_DIFFUSE_EVAL: {
+            closure_rec c;
+            memset(&c, 0, sizeof(c));
+            c.kind = KIND_DIFFUSE;
+            c.normal = load3(in);
+            c.roughness = in[3];
+            c.color = load3(in + 4);
+            const v3 value = diffuse_intensity(&c, load3(in + 7), load3(in + 10));
+            out[0] = value.x; out[1] = value.y; out[2] = value.z;
+            out[3] = fmaxf(vdot(c.normal, load3(in + 10)), 0.0f) * INV_PI_F;
+            break;
+        }
+        default:
+            return -1;
+        }
+    }
+    return 0;
+}
+
+/* One diffuse closure of colour `color` at a surface with shading and
+ * geometric normal +z seen from +z: the records of the reference's closure
+ * fixture (tests/test_luisa_cycles_closure.cpp:693-755) that this path
+ * models, out[4 * r + c] for records r = 0..8 and 10: closure count, index,
+ * Cycles type (2, diffuse) and sample weight; weight and validity; normal;
+ * sampled index, type, sample weight and rescaled selection; weight and
+ * validity; normal; pdf twice, label and validity; direction; evaluation;
+ * (record 9, roughness and eta, is not modelled and stays zero); closure and
+ * sample runtime flags and the closure counts for colours 0.5e-5 and 1e-5. */
+static uint32_t diffuse_closure_count(float value) {
+    psx_closure_instr ins;
+    memset(&ins, 0, sizeof(ins));
+    ins.op = PSX_CLOSURE_DIFFUSE;
+    ins.color = 0u;
+    ins.roughness = 3u;
+    float params[4] = {value, value, value, 0.0f};
+    omaterial m;
+    memset(&m, 0, sizeof(m));
+    m.instr = &ins;
+    m.count = 1u;
+    m.params = params;
+    m.param_count = 4u;
+    closure_set cs;
+    material_closures(&m, V3(0, 0, 1), &cs);
+    return cs.count;
+}
+
+int oracle_diffuse_closure_probe(const float color[3], float u_lobe, const float u_dir[2], float *out) {
+    psx_closure_instr ins;
+    memset(&ins, 0, sizeof(ins));
+    ins.op = PSX_CLOSURE_DIFFUSE;
+    ins.color = 0u;
+    ins.roughness = 3u;
+    float params[4] = {color[0], color[1], color[2], 0.0f};
+    omaterial m;
+    memset(&m, 0, sizeof(m));
+    m.instr = &ins;
+    m.count = 1u;
+    m.params = params;
+    m.param_count = 4u;
+    closure_set cs;
+    material_closures(&m, V3(0, 0, 1), &cs);
+    memset(out, 0, sizeof(float) * 44u);
+    const closure_rec *c = &cs.c[0];
+    const float type = 2.0f;
+    out[0] = (float)cs.count; out[1] = 0.0f; out[2] = type; out[3] = cs.count ? c->sample_weight : 0.0f;
+    out[4] = c->weight.x; out[5] = c->weight.y; out[6] = c->weight.z; out[7] = cs.count ? 1.0f : 0.0f;
+    out[8] = c->normal.x; out[9] = c->normal.y; out[10] = c->normal.z;
+    surface_t sf;
+    memset(&sf, 0, sizeof(sf));
+    sf.ng = sf.ns = sf.incoming = V3(0, 0, 1);
+    const v2 ud = {u_dir[0], u_dir[1]};
+    const bsdf_sample_t bs = sample_closures(&cs, &sf, u_lobe, ud, LOBE_ALL);
+    const closure_rec *sc = &cs.c[bs.selected < cs.count ? bs.selected : 0u];
+    out[12] = (float)bs.selected; out[13] = type; out[14] = sc->sample_weight;
+    out[15] = bs.rescaled;
+    out[16] = sc->weight.x; out[17] = sc->weight.y; out[18] = sc->weight.z; out[19] = 1.0f;
+    out[20] = sc->normal.x; out[21] = sc->normal.y; out[22] = sc->normal.z;
+    out[24] = bs.eval.pdf; out[25] = bs.eval.pdf; out[26] = (float)label_from_events(bs.eval.events);
+    out[27] = bs.valid ? 1.0f : 0.0f;
+    out[28] = bs.wi.x; out[29] = bs.wi.y; out[30] = bs.wi.z;
+    out[32] = bs.eval.f.x; out[33] = bs.eval.f.y; out[34] = bs.eval.f.z;
+    out[40] = (float)surface_runtime_flags(&cs, 0);
+    out[41] = (float)bs.runtime_flags;
+    out[42] = (float)diffuse_closure_count(0.5e-5f);
+    out[43] = (float)diffuse_closure_count(1.0e-5f);
     return 0;
 }

@@ -264,6 +264,19 @@ int oracle_sampler_probe(const psx_render_settings *settings,
                          uint32_t first_dimension, uint32_t dimension_count,
                          uint32_t *out);
 
+/* Closest-hit and shadow queries with the layouts of psx_scene_intersect
+ * (the oracle's shadow loop always walks every transparent hit in order). */
+int oracle_scene_intersect(const oracle_scene *scene, const psx_ray_query *queries, uint32_t count,
+                           psx_ray_hit *hits);
+
+/* Single-function probes with the record layouts of psx_backend_probe. */
+uint32_t oracle_probe_input_width(uint32_t function);
+uint32_t oracle_probe_output_width(uint32_t function);
+int oracle_probe(uint32_t function, const float *inputs, uint32_t count, float *outputs);
+
+/* The closure fixture records of one diffuse closure (44 floats). */
+int oracle_diffuse_closure_probe(const float color[3], float u_lobe, const float u_dir[2], float *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -238,6 +238,58 @@ class ServingStats(C.Structure):
                 ("hbm_in_use", C.c_uint64), ("peak_hbm", C.c_uint64), ("busy_ms", C.c_double)]
 
 
+INVALID_ID = 0xFFFFFFFF
+QUERY_CLOSEST = 0
+QUERY_SHADOW = 1
+
+
+class RayQuery(C.Structure):
+    _fields_ = [("origin", f32 * 3), ("tmin", f32), ("direction", f32 * 3), ("tmax", f32),
+                ("visibility", u32), ("source_object", u32), ("source_primitive", u32),
+                ("light_object", u32), ("light_primitive", u32), ("kind", u32),
+                ("transparent_depth", u32), ("transparent_max", u32)]
+
+
+class RayHit(C.Structure):
+    _fields_ = [("t", f32), ("u", f32), ("v", f32), ("object", u32), ("primitive", u32), ("hit", u32),
+                ("traversals", u32), ("reserved", u32), ("transmittance", f32 * 3), ("reserved2", u32)]
+
+
+assert C.sizeof(RayQuery) == 64 and C.sizeof(RayHit) == 48
+
+# psx_probe_function values and record widths (floats in, floats out)
+PROBE_FILM_CLAMP = 0
+PROBE_T_OFFSET = 1
+PROBE_SELF_EXCLUSION = 2
+PROBE_SHADOW_TERMINATOR = 3
+PROBE_TRIANGLE_SAMPLE = 4
+PROBE_TRIANGLE_PDF = 5
+PROBE_DIFFUSE_EVAL = 6
+PROBE_SURFACE_SHADOW_ORIGIN = 7
+PROBE_WIDTHS = {PROBE_FILM_CLAMP: (6, 3), PROBE_T_OFFSET: (1, 1), PROBE_SELF_EXCLUSION: (21, 4),
+                PROBE_SHADOW_TERMINATOR: (47, 4), PROBE_TRIANGLE_SAMPLE: (14, 10), PROBE_TRIANGLE_PDF: (15, 3),
+                PROBE_DIFFUSE_EVAL: (13, 4), PROBE_SURFACE_SHADOW_ORIGIN: (59, 4)}
+
+
+def make_queries(records) -> "C.Array":
+    """RayQuery array from dicts with keys origin, direction, tmin, tmax and
+    optional visibility, source, light, kind, transparent_depth,
+    transparent_max."""
+    arr = (RayQuery * len(records))()
+    for q, r in zip(arr, records):
+        q.origin[:] = [float(x) for x in r["origin"]]
+        q.direction[:] = [float(x) for x in r["direction"]]
+        q.tmin = float(r.get("tmin", 0.0))
+        q.tmax = float(r.get("tmax", 3.4028234663852886e38))
+        q.visibility = int(r.get("visibility", VIS_ALL))
+        q.source_object, q.source_primitive = r.get("source", (INVALID_ID, INVALID_ID))
+        q.light_object, q.light_primitive = r.get("light", (INVALID_ID, INVALID_ID))
+        q.kind = int(r.get("kind", QUERY_CLOSEST))
+        q.transparent_depth = int(r.get("transparent_depth", 0))
+        q.transparent_max = int(r.get("transparent_max", 8))
+    return arr
+
+
 def default_settings(width: int, height: int) -> RenderSettings:
     """contract::RenderSettings defaults (include/psycles/contract/render.h:81-117)."""
     s = RenderSettings()
@@ -283,6 +335,8 @@ EXPORTED_SYMBOLS = [
     "psx_server_update_parameters", "psx_server_status", "psx_server_wait",
     "psx_server_read_film", "psx_server_write_passes", "psx_server_cancel",
     "psx_server_release", "psx_server_stats",
+    "psx_scene_intersect", "psx_scene_intersect_device",
+    "psx_probe_input_width", "psx_probe_output_width", "psx_backend_probe",
 ]
 
 _lib = None
@@ -334,6 +388,16 @@ def load_library(path: os.PathLike | None = None):
     lib.psx_server_cancel.argtypes = [vp, C.c_uint64]
     lib.psx_server_release.argtypes = [vp, C.c_uint64]
     lib.psx_server_stats.argtypes = [vp, P(ServingStats)]
+    lib.psx_scene_intersect.argtypes = [vp, P(RayQuery), u32, P(RayHit)]
+    lib.psx_scene_intersect.restype = C.c_int
+    lib.psx_scene_intersect_device.argtypes = [vp, vp, u32, vp]
+    lib.psx_scene_intersect_device.restype = C.c_int
+    lib.psx_probe_input_width.argtypes = [u32]
+    lib.psx_probe_input_width.restype = u32
+    lib.psx_probe_output_width.argtypes = [u32]
+    lib.psx_probe_output_width.restype = u32
+    lib.psx_backend_probe.argtypes = [vp, u32, P(f32), u32, P(f32)]
+    lib.psx_backend_probe.restype = C.c_int
     for name in ("psx_server_create", "psx_server_submit", "psx_server_update_parameters",
                  "psx_server_status", "psx_server_wait", "psx_server_read_film",
                  "psx_server_write_passes", "psx_server_cancel", "psx_server_release",

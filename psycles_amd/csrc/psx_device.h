@@ -83,6 +83,8 @@ struct alignas(16) Instance {
     uint32_t ipos_offset;  // first intersection-space vertex (float3 index)
     uint32_t override_offset;
     uint32_t override_count;
+    float terminator_offset;  // shadow_terminator_geometry_offset
+    uint32_t pad[3];
 };
 
 struct Mesh {

@@ -48,6 +48,13 @@ hipError_t launch_sampler(const KernelParams &k, const float4 *sobol, uint32_t s
                           uint32_t dim_count, uint32_t *out, hipStream_t stream);
 hipError_t launch_material_tables(const SceneView &s, float4 *shadow, float4 *emission, uint32_t count,
                                   hipStream_t stream);
+// psx_query.hip
+uint32_t query_resident_lanes();
+uint32_t query_lanes(uint32_t count, uint32_t max_lanes);
+hipError_t launch_intersect(const SceneView &s, const psx_ray_query *queries, uint32_t count, psx_ray_hit *hits,
+                            uint32_t *cursor, uint32_t *spill, uint32_t lanes, hipStream_t stream);
+hipError_t launch_probe(uint32_t function, const float *inputs, uint32_t in_width, uint32_t count, float *outputs,
+                        uint32_t out_width, hipStream_t stream);
 // psx_wavefront.hip
 uint32_t wavefront_lanes();
 uint32_t wavefront_padded_pixels(const KernelParams &k);
@@ -497,6 +504,7 @@ struct Scene {
     // parameter versions are built, and released, on this stream (see
     // DeviceBuffer::set_stream)
     hipStream_t table_stream = nullptr;
+    hipStream_t query_stream = nullptr;   // psx_scene_intersect* work
     std::shared_ptr<ParamTables> tables;  // current parameter version
     SceneView view{};                     // geometry + the current version
 
@@ -505,6 +513,10 @@ struct Scene {
         if (table_stream) {
             (void)hipStreamSynchronize(table_stream);
             (void)hipStreamDestroy(table_stream);
+        }
+        if (query_stream) {
+            (void)hipStreamSynchronize(query_stream);
+            (void)hipStreamDestroy(query_stream);
         }
     }
     uint64_t triangles = 0;
@@ -1454,6 +1466,9 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         g.prim_offset = mesh_prim_offset[mi];
         g.override_offset = static_cast<uint32_t>(overrides.size());
         g.override_count = src.material_override_count;
+        if (!(src.shadow_terminator_geometry_offset >= 0.0f) || !std::isfinite(src.shadow_terminator_geometry_offset))
+            return fail("instance " + std::to_string(i) + " has an invalid shadow terminator geometry offset");
+        g.terminator_offset = src.shadow_terminator_geometry_offset;
         overrides.insert(overrides.end(), src.material_overrides, src.material_overrides + src.material_override_count);
         const Mesh &mesh = scene->meshes[mi];
         float lo[3], hi[3];
@@ -1561,6 +1576,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     s.overrides_buf.upload(overrides);
     s.leaves.upload(leaves);
     hip_check(hipStreamCreateWithFlags(&s.table_stream, hipStreamNonBlocking));
+    hip_check(hipStreamCreateWithFlags(&s.query_stream, hipStreamNonBlocking));
     s.tables = std::make_shared<ParamTables>(s.table_stream);
     s.upload_materials(*s.tables);
     SceneView &v = s.view;
@@ -2591,6 +2607,73 @@ int psx_session_sampler_probe(psx_session *handle, uint32_t sample, uint32_t tot
                                  s->stream));
         hip_check(hipStreamSynchronize(s->stream));
         hip_check(hipMemcpy(out, rec.as<void>(), words * 4u, hipMemcpyDeviceToHost));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+// ---- ray queries and probes -------------------------------------------------
+int psx_scene_intersect_device(psx_scene *handle, const psx_ray_query *queries, uint32_t count, psx_ray_hit *hits) {
+    Scene *s = reinterpret_cast<Scene *>(handle);
+    if (!s || (count && (!queries || !hits))) return PSX_ERROR_INVALID_ARGUMENT;
+    if (count == 0u) return PSX_OK;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->backend->device));
+        const uint32_t lanes = query_lanes(count, query_resident_lanes());
+        DeviceBuffer cursor, spill;
+        cursor.allocate(sizeof(uint32_t));
+        spill.allocate(static_cast<size_t>(lanes) * kTraversalStackDepth * sizeof(uint32_t));
+        hip_check(launch_intersect(s->view, queries, count, hits, cursor.as<uint32_t>(), spill.as<uint32_t>(), lanes,
+                                   s->query_stream));
+        hip_check(hipStreamSynchronize(s->query_stream));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+int psx_scene_intersect(psx_scene *handle, const psx_ray_query *queries, uint32_t count, psx_ray_hit *hits) {
+    Scene *s = reinterpret_cast<Scene *>(handle);
+    if (!s || (count && (!queries || !hits))) return PSX_ERROR_INVALID_ARGUMENT;
+    if (count == 0u) return PSX_OK;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->backend->device));
+        DeviceBuffer q, h;
+        q.allocate(static_cast<size_t>(count) * sizeof(psx_ray_query));
+        h.allocate(static_cast<size_t>(count) * sizeof(psx_ray_hit));
+        hip_check(hipMemcpy(q.as<void>(), queries, static_cast<size_t>(count) * sizeof(psx_ray_query),
+                            hipMemcpyHostToDevice));
+        const int rc = psx_scene_intersect_device(handle, q.as<psx_ray_query>(), count, h.as<psx_ray_hit>());
+        if (rc != PSX_OK) return rc;
+        hip_check(hipMemcpy(hits, h.as<void>(), static_cast<size_t>(count) * sizeof(psx_ray_hit),
+                            hipMemcpyDeviceToHost));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+uint32_t psx_probe_input_width(uint32_t function) {
+    static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u};
+    return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
+}
+
+uint32_t psx_probe_output_width(uint32_t function) {
+    static const uint32_t w[] = {3u, 1u, 4u, 4u, 10u, 3u, 4u, 4u};
+    return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
+}
+
+int psx_backend_probe(psx_backend *handle, uint32_t function, const float *inputs, uint32_t count, float *outputs) {
+    Backend *b = reinterpret_cast<Backend *>(handle);
+    const uint32_t wi = psx_probe_input_width(function), wo = psx_probe_output_width(function);
+    if (!b || wi == 0u || (count && (!inputs || !outputs))) return PSX_ERROR_INVALID_ARGUMENT;
+    if (count == 0u) return PSX_OK;
+    return guarded([&] {
+        hip_check(hipSetDevice(b->device));
+        DeviceBuffer in, out;
+        in.allocate(static_cast<size_t>(count) * wi * sizeof(float));
+        out.allocate(static_cast<size_t>(count) * wo * sizeof(float));
+        hip_check(hipMemcpy(in.as<void>(), inputs, static_cast<size_t>(count) * wi * sizeof(float),
+                            hipMemcpyHostToDevice));
+        hip_check(launch_probe(function, in.as<float>(), wi, count, out.as<float>(), wo, nullptr));
+        hip_check(hipDeviceSynchronize());
+        hip_check(hipMemcpy(outputs, out.as<void>(), static_cast<size_t>(count) * wo * sizeof(float),
+                            hipMemcpyDeviceToHost));
         return static_cast<int>(PSX_OK);
     });
 }

@@ -392,6 +392,31 @@ __device__ __forceinline__ void sort_pair(float &ka, uint32_t &ea, float &kb, ui
     eb = e1;
 }
 
+// Material of a triangle of an instance: per-triangle slot, instance
+// override, then the mesh's slot table (clamped to its last slot).
+__device__ __forceinline__ uint32_t material_of(const SceneView &s, const Instance &inst, const Mesh &mesh,
+                                                uint32_t prim) {
+    const uint32_t slot = mesh.tri_slot_offset != kNone ? s.tri_slots[mesh.tri_slot_offset + prim] : 0u;
+    if (slot < inst.override_count) return s.overrides[inst.override_offset + slot];
+    if (mesh.slot_count == 0u) return kNone;
+    return s.slots[mesh.slot_offset + (slot < mesh.slot_count ? slot : mesh.slot_count - 1u)];
+}
+
+__device__ __forceinline__ uint32_t hit_material(const SceneView &s, uint32_t inst_index, uint32_t prim) {
+    const Instance &inst = s.instances[inst_index];
+    const Mesh &mesh = s.meshes[inst.mesh];
+    return material_of(s, inst, mesh, prim);
+}
+
+// A surface whose transparent extinction carries no light (or that has no
+// material) stops a shadow ray (src/luisa/path_tracer_geometry.cpp:128-160).
+__device__ __forceinline__ bool shadow_opaque(const SceneView &s, uint32_t inst_index, uint32_t prim) {
+    const uint32_t material = hit_material(s, inst_index, prim);
+    if (material == kNone) return true;
+    const float4 e = s.material_shadow[material];
+    return !(fmaxf(e.x, fmaxf(e.y, e.z)) > 0.0f);
+}
+
 // Closest hit along [tmin, tmax] under the Cycles order: smallest t, ties to
 // the greatest (object, primitive).  Box culling is conservative (boxes are
 // padded at build time and the far distance is widened), so traversal order
@@ -409,6 +434,11 @@ struct Traversal {
     uint32_t current;
     uint32_t inst_index, inst_prim_offset;
     bool in_blas;
+    // Shadow queries (`anyhit`) end at the first accepted candidate whose
+    // material is opaque to shadow rays (`opaque`): the transparent shadow
+    // loop's transmittance is then exactly zero whatever else lies on the
+    // ray, so the early end changes no result bit.
+    bool anyhit, opaque;
 };
 
 __device__ __forceinline__ f3 clamped_dir(f3 d) {
@@ -430,6 +460,7 @@ __device__ __forceinline__ void traversal_begin(const SceneView &s, Traversal &t
     t.best.u = t.best.v = 0.0f;
     t.lower = t.ray.tmin > 0.0f ? t.ray.tmin * 0.99999f : t.ray.tmin - 1.0e-30f;
     t.inst_index = t.inst_prim_offset = 0u;
+    t.opaque = false;
     traversal_world_space(t);
     st.sp = 0;
     st.bottom = 0;
@@ -485,7 +516,23 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
             const uint32_t cprim = t.inst_prim_offset + prim;
             const bool excluded = (t.inst_index == t.q.src_obj && cprim == t.q.src_prim) ||
                                   (t.inst_index == t.q.light_obj && cprim == t.q.light_prim);
-            if (excluded || !order_accepts(t.best, th, t.inst_index, cprim)) return true;
+            if (excluded) return true;
+            if (t.anyhit && shadow_opaque(s, t.inst_index, prim)) {
+                // any opaque candidate in [tmin, tmax] zeroes the shadow
+                t.opaque = true;
+                t.best.valid = true;
+                t.best.t = th;
+                t.best.u = bu;
+                t.best.v = bv;
+                t.best.inst = t.best.object = t.inst_index;
+                t.best.prim = prim;
+                t.best.cprim = cprim;
+                t.current = kEmptySlot;
+                st.sp = 0;
+                st.bottom = 0;
+                return true;
+            }
+            if (!order_accepts(t.best, th, t.inst_index, cprim)) return true;
             t.best.valid = true;
             t.best.t = th;
             t.best.u = bu;
@@ -557,13 +604,16 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
 
 template <bool COUNT>
 __device__ Hit closest_hit(const SceneView &s, const Ray &ray, const TraceQuery &q, Stack &st,
-                           uint32_t &node_count, uint32_t &tri_count) {
+                           uint32_t &node_count, uint32_t &tri_count, bool anyhit = false,
+                           bool *opaque = nullptr) {
     Traversal t;
     t.ray = ray;
     t.q = q;
+    t.anyhit = anyhit;
     traversal_begin(s, t, st);
     while (traversal_step<COUNT>(s, t, st, node_count, tri_count)) {
     }
+    if (opaque) *opaque = t.opaque;
     return t.best;
 }
 
@@ -578,14 +628,6 @@ struct Surface {
     f3 wp0, wp1, wp2;   // world positions
     f3 position, object_hit_position, ng, ns, incoming;
 };
-
-__device__ __forceinline__ uint32_t material_of(const SceneView &s, const Instance &inst, const Mesh &mesh,
-                                                uint32_t prim) {
-    const uint32_t slot = mesh.tri_slot_offset != kNone ? s.tri_slots[mesh.tri_slot_offset + prim] : 0u;
-    if (slot < inst.override_count) return s.overrides[inst.override_offset + slot];
-    if (mesh.slot_count == 0u) return kNone;
-    return s.slots[mesh.slot_offset + (slot < mesh.slot_count ? slot : mesh.slot_count - 1u)];
-}
 
 __device__ __forceinline__ f3 interp3(float u, float v, f3 a, f3 b, f3 c) {
     return (a * (1.0f - u - v) + b * u) + c * v;
@@ -675,7 +717,10 @@ __device__ __forceinline__ float offset_component(float p, float n) {
 }
 
 // include/psycles/luisa/surface_ray.h:50-106
-__device__ __forceinline__ f3 origin_with_self_exclusion(f3 wo, f3 wn, f3 oo, f3 od, f3 p0, f3 p1, f3 p2) {
+// source_triangle_covers_ray_origin: the three Pluecker edge functions share
+// a strict sign, beyond a scaled epsilon, exactly when the ray from the
+// origin crosses the source triangle's interior
+__device__ __forceinline__ bool source_covers_origin(f3 oo, f3 od, f3 p0, f3 p1, f3 p2) {
     const f3 v0 = p0 - oo, v1 = p1 - oo, v2 = p2 - oo;
     const f3 e0 = v2 - v0, e1 = v0 - v1, e2 = v1 - v2;
     const float u = dot(cross(v2 + v0, e0), od);
@@ -684,8 +729,11 @@ __device__ __forceinline__ f3 origin_with_self_exclusion(f3 wo, f3 wn, f3 oo, f3
     const float eps = FLT_EPSILON * fabsf(u + v + w);
     const float mn = fminf(u, fminf(v, w));
     const float mx = fmaxf(u, fmaxf(v, w));
-    const bool covered = (mn >= eps) || (mx <= -eps);
-    if (covered) return wo;
+    return (mn >= eps) || (mx <= -eps);
+}
+
+__device__ __forceinline__ f3 origin_with_self_exclusion(f3 wo, f3 wn, f3 oo, f3 od, f3 p0, f3 p1, f3 p2) {
+    if (source_covers_origin(oo, od, p0, p1, p2)) return wo;
     return V(offset_component(wo.x, wn.x), offset_component(wo.y, wn.y), offset_component(wo.z, wn.z));
 }
 
@@ -696,11 +744,108 @@ __device__ __forceinline__ f3 surface_ray_origin(const SceneView &s, const Surfa
     return origin_with_self_exclusion(sf.position, sf.ng, co, cd, sf.p0, sf.p1, sf.p2);
 }
 
-__device__ __forceinline__ f3 surface_shadow_origin(const SceneView &s, const Surface &sf, f3 dir) {
+// Cycles' parabolic smooth-surface height (include/psycles/luisa/surface_ray.h:
+// 108-181) in intersection space; the interpolated normal goes to world space
+// with the instance's FMA affine unless the transform is applied.
+__device__ f3 smooth_surface_offset(const Affine &o2w, bool applied, float bu, float bv, f3 p0, f3 p1, f3 p2,
+                                    f3 n0, f3 n1, f3 n2, f3 world_ng) {
+    const float u = 1.0f - bu - bv;
+    const float v = bu;
+    const float w = bv;
+    const f3 object_position = (p0 * u + p1 * v) + p2 * w;
+    const f3 object_normal = (n0 * u + n1 * v) + n2 * w;
+    const f3 world_normal = applied ? object_normal : affine_dir(o2w, object_normal);
+    const float a = dot(n2 - n0, p0 - p2);
+    const float b = dot(n2 - n1, p1 - p2);
+    const float c = dot(n1 - n0, p1 - p0);
+    const float parabolic = ((a * u) * (u - 1.0f) + ((a + b) + c) * u * v) + (b * v) * (v - 1.0f);
+    float f0 = fmaxf(fmaxf(dot(p1 - p0, n0), dot(p2 - p0, n0)), 0.0f);
+    float f1 = fmaxf(fmaxf(dot(p0 - p1, n1), dot(p2 - p1, n1)), 0.0f);
+    float f2 = fmaxf(fmaxf(dot(p0 - p2, n2), dot(p1 - p2, n2)), 0.0f);
+    f0 = fmaxf(dot(p0 - object_position, n0) + f0, 0.0f);
+    f1 = fmaxf(dot(p1 - object_position, n1) + f1, 0.0f);
+    f2 = fmaxf(dot(p2 - object_position, n2) + f2, 0.0f);
+    const float front = fmaxf(fminf(f0, fminf(f1, f2)), parabolic * 0.5f);
+    float b0 = fmaxf(fmaxf(dot(p0 - p1, n0), dot(p0 - p2, n0)), 0.0f);
+    float b1 = fmaxf(fmaxf(dot(p1 - p0, n1), dot(p1 - p2, n1)), 0.0f);
+    float b2 = fmaxf(fmaxf(dot(p2 - p0, n2), dot(p2 - p1, n2)), 0.0f);
+    b0 = fmaxf(dot(object_position - p0, n0) + b0, 0.0f);
+    b1 = fmaxf(dot(object_position - p1, n1) + b1, 0.0f);
+    b2 = fmaxf(dot(object_position - p2, n2) + b2, 0.0f);
+    const float back = fminf(-fminf(b0, fminf(b1, b2)), parabolic * 0.5f);
+    const float height = dot(world_normal, world_ng) > 0.0f ? front : back;
+    return world_normal * height;
+}
+
+struct ShadowOrigin {
+    f3 position;
+    bool skip_self;  // the source primitive stays excluded from the shadow query
+};
+
+// shadow_terminator_origin (include/psycles/luisa/surface_ray.h:183-243)
+__device__ ShadowOrigin shadow_terminator_origin(f3 position, f3 ns, f3 ng, f3 light_dir, float geometry_offset,
+                                                 bool smooth, const Affine &o2w, bool applied, float bu, float bv,
+                                                 f3 p0, f3 p1, f3 p2, f3 n0, f3 n1, f3 n2) {
+    float normal_light = dot(ns, light_dir);
+    const bool transmit = normal_light < 0.0f;
+    normal_light = fabsf(normal_light);
+    const f3 offset_normal = transmit ? neg(ng) : ng;
+    const float geometric_light = dot(offset_normal, light_dir);
+    const float safe_cutoff = fmaxf(geometry_offset, 1.0e-20f);
+    const float near_terminator = clampf(2.0f - (geometric_light + normal_light) / safe_cutoff, 0.0f, 1.0f);
+    const float regular = clampf(1.0f - geometric_light / safe_cutoff, 0.0f, 1.0f);
+    const float amount = normal_light < geometry_offset ? near_terminator : regular;
+    ShadowOrigin r{position, true};
+    if (smooth && geometry_offset > 0.0f && amount > 0.0f) {
+        const f3 offset = smooth_surface_offset(o2w, applied, bu, bv, p0, p1, p2, n0, n1, n2, offset_normal);
+        r.position = position + offset * amount;
+        r.skip_self = geometric_light > 0.0f;
+    }
+    return r;
+}
+
+// The lift, then the source-triangle certificate with the robust offset
+// fallback while the source stays excluded (make_shadow_origin,
+// src/luisa/path_kernel_surface_geometry.cpp:168-182; surface_shadow_origin,
+// include/psycles/luisa/surface_ray.h:252-309).  Vertex normals are reloaded
+// only for smooth triangles of instances with a terminator offset.
+__device__ ShadowOrigin shadow_origin_core(f3 position, f3 ns, f3 ng, f3 dir, float geometry_offset, bool smooth,
+                                           const Affine &o2w, const Affine &w2o, bool applied, float bu, float bv,
+                                           f3 p0, f3 p1, f3 p2, f3 n0, f3 n1, f3 n2) {
+    ShadowOrigin o = shadow_terminator_origin(position, ns, ng, dir, geometry_offset, smooth, o2w, applied, bu, bv,
+                                              p0, p1, p2, n0, n1, n2);
+    if (o.skip_self) {
+        const f3 oo = applied ? o.position : affine_point(w2o, o.position);
+        const f3 od = applied ? dir : affine_dir(w2o, dir);
+        o.position = origin_with_self_exclusion(o.position, ng, oo, od, p0, p1, p2);
+    }
+    return o;
+}
+
+__device__ ShadowOrigin surface_shadow_origin(const SceneView &s, const Surface &sf, const Hit &h, f3 dir) {
     const Instance &inst = s.instances[sf.inst];
-    const f3 oo = sf.applied ? sf.position : affine_point(inst.w2o, sf.position);
-    const f3 od = sf.applied ? dir : affine_dir(inst.w2o, dir);
-    return origin_with_self_exclusion(sf.position, sf.ng, oo, od, sf.p0, sf.p1, sf.p2);
+    const float offset = inst.terminator_offset;
+    bool lift = false;
+    f3 n0 = V(0.0f, 0.0f, 0.0f), n1 = n0, n2 = n0;
+    if (offset > 0.0f) {
+        const Mesh &mesh = s.meshes[inst.mesh];
+        lift = mesh.smooth_offset != kNone && mesh.normal_offset != kNone && s.smooth[mesh.smooth_offset + h.prim] != 0u;
+        if (lift) {
+            const uint32_t *tri = s.indices + 3u * (mesh.index_offset + h.prim);
+            n0 = load3(s.normals + 3u * (mesh.normal_offset + tri[0]));
+            n1 = load3(s.normals + 3u * (mesh.normal_offset + tri[1]));
+            n2 = load3(s.normals + 3u * (mesh.normal_offset + tri[2]));
+            if (sf.applied) {
+                // world vertex normals fall back to the unflipped geometric normal
+                const f3 gn = sf.back_facing ? neg(sf.ng) : sf.ng;
+                n0 = safe_normalize(affine_dir(inst.n2w, n0), gn);
+                n1 = safe_normalize(affine_dir(inst.n2w, n1), gn);
+                n2 = safe_normalize(affine_dir(inst.n2w, n2), gn);
+            }
+        }
+    }
+    return shadow_origin_core(sf.position, sf.ns, sf.ng, dir, offset, lift, inst.o2w, inst.w2o, sf.applied, h.u, h.v,
+                              sf.p0, sf.p1, sf.p2, n0, n1, n2);
 }
 
 __device__ __forceinline__ float next_float_up(float t) {
@@ -2209,12 +2354,6 @@ __device__ __forceinline__ void material_tables(const SceneView &s, uint32_t mat
     emission = make_float4(e.x, e.y, e.z, 0.0f);
 }
 
-__device__ __forceinline__ uint32_t hit_material(const SceneView &s, uint32_t inst_index, uint32_t prim) {
-    const Instance &inst = s.instances[inst_index];
-    const Mesh &mesh = s.meshes[inst.mesh];
-    return material_of(s, inst, mesh, prim);
-}
-
 // ===========================================================================
 // the path integrator as a step function (src/luisa/path_kernel_pipeline.cpp:
 // 73-185).  One step consumes the closest hit of the current ray, adds
@@ -2335,10 +2474,14 @@ __device__ __forceinline__ void init_path(const KernelParams &k, const float4 *s
 // transparent shadow continuation (src/luisa/path_tracer_geometry.cpp:75-174):
 // given the closest hit of the shadow ray, either finish (returns true) or
 // advance past a transparent surface
-__device__ __forceinline__ bool shadow_hit(const SceneView &s, const KernelParams &k, const Hit &sh, Ray &sray,
-                                           uint32_t &tdepth, f3 &transmittance) {
+__device__ __forceinline__ bool shadow_hit(const SceneView &s, uint32_t transparent_max, const Hit &sh, Ray &sray,
+                                           uint32_t &tdepth, f3 &transmittance, bool opaque = false) {
     if (!sh.valid) return true;
-    if (tdepth >= k.transparent_max) {
+    if (opaque) {
+        transmittance = V(0.0f, 0.0f, 0.0f);
+        return true;
+    }
+    if (tdepth >= transparent_max) {
         transmittance = V(0.0f, 0.0f, 0.0f);
         return true;
     }
@@ -2664,7 +2807,8 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                         const float rr = light_roulette(weighted, u_lightterm, k.light_inv_rr_threshold);
                         const f3 surviving = weighted * rr;
                         if (any_nonzero(surviving)) {
-                            const f3 so = surface_shadow_origin(s, sf, ls.direction);
+                            const ShadowOrigin sso = surface_shadow_origin(s, sf, hit, ls.direction);
+                            const f3 so = sso.position;
                             const f3 soff = ls.position - so;
                             const float sdist = sqrtf(fmaxf(dot(soff, soff), 1.0e-20f));
                             sr.ray.o = so;
@@ -2672,8 +2816,8 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                             sr.ray.tmin = 0.0f;
                             sr.ray.tmax = sdist;
                             sr.q.visibility = VIS_SHADOW;
-                            sr.q.src_obj = sf.object;
-                            sr.q.src_prim = sf.cprim;
+                            sr.q.src_obj = sso.skip_self ? sf.object : 0xffffffffu;
+                            sr.q.src_prim = sso.skip_self ? sf.cprim : 0xffffffffu;
                             sr.q.light_obj = eobj;
                             sr.q.light_prim = eprim;
                             sr.L = p.throughput * surviving;
@@ -2712,7 +2856,8 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                     const float rr = light_roulette(weighted, u_lightterm, k.light_inv_rr_threshold);
                     const f3 surviving = weighted * rr;
                     if (any_nonzero(surviving)) {
-                        const f3 so = surface_shadow_origin(s, sf, ls.direction);
+                        const ShadowOrigin sso = surface_shadow_origin(s, sf, hit, ls.direction);
+                        const f3 so = sso.position;
                         const f3 foff = ls.position - so;
                         const float fdist = sqrtf(fmaxf(dot(foff, foff), 1.0e-20f));
                         sr.ray.o = so;
@@ -2720,8 +2865,8 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                         sr.ray.tmin = 0.0f;
                         sr.ray.tmax = distant ? RAY_MAXIMUM : fdist;
                         sr.q.visibility = VIS_SHADOW;
-                        sr.q.src_obj = sf.object;
-                        sr.q.src_prim = sf.cprim;
+                        sr.q.src_obj = sso.skip_self ? sf.object : 0xffffffffu;
+                        sr.q.src_prim = sso.skip_self ? sf.cprim : 0xffffffffu;
                         sr.q.light_obj = sr.q.light_prim = 0xffffffffu;
                         sr.L = p.throughput * surviving;
                         sr.depth = p.ps.bounce;

@@ -1,0 +1,211 @@
+// psx_query.hip - ray queries and single-function probes (gfx950).
+//
+// intersect_kernel answers psx_ray_query records against a compiled scene
+// with the render path's own traversal: closest hits under the Cycles order
+// and the transparent shadow loop (ended by the first opaque surface).  Waves
+// take 64 queries per atomic from a cursor; each lane keeps its stack in an
+// LDS short stack with an HBM spill column, like the trace kernels.
+//
+// probe_kernel evaluates one device function per input record, so tests can
+// hold the device code itself against the reference's function fixtures.
+#include "../../include/psx.h"
+#include "psx_launch.h"
+#include "psx_path.h"
+
+namespace psx {
+namespace {
+
+constexpr uint32_t kQueryBlock = 256;
+
+__global__ void clear_words_kernel(uint32_t *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__device__ void run_query(const SceneView &s, const psx_ray_query &q, Stack &st, psx_ray_hit &h) {
+    Ray r;
+    r.o = V(q.origin[0], q.origin[1], q.origin[2]);
+    r.d = V(q.direction[0], q.direction[1], q.direction[2]);
+    r.tmin = q.tmin;
+    r.tmax = q.tmax;
+    TraceQuery tq;
+    tq.visibility = q.visibility;
+    tq.src_obj = q.source_object;
+    tq.src_prim = q.source_primitive;
+    tq.light_obj = q.light_object;
+    tq.light_prim = q.light_primitive;
+    uint32_t nodes = 0u, tris = 0u;
+    h.u = h.v = 0.0f;
+    h.reserved = h.reserved2 = 0u;
+    if (q.kind == PSX_QUERY_SHADOW) {
+        f3 transmittance = V(1.0f, 1.0f, 1.0f);
+        uint32_t tdepth = q.transparent_depth, passes = 0u;
+        for (;;) {
+            bool opaque = false;
+            const Hit b = closest_hit<false>(s, r, tq, st, nodes, tris, true, &opaque);
+            ++passes;
+            if (shadow_hit(s, q.transparent_max, b, r, tdepth, transmittance, opaque)) break;
+        }
+        h.t = q.tmax;
+        h.object = h.primitive = PSX_INVALID_ID;
+        h.hit = (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f) ? 0u : 1u;
+        h.traversals = passes;
+        h.transmittance[0] = transmittance.x;
+        h.transmittance[1] = transmittance.y;
+        h.transmittance[2] = transmittance.z;
+    } else {
+        const Hit b = closest_hit<false>(s, r, tq, st, nodes, tris);
+        h.hit = b.valid ? 1u : 0u;
+        h.t = b.valid ? b.t : q.tmax;
+        h.u = b.valid ? b.u : 0.0f;
+        h.v = b.valid ? b.v : 0.0f;
+        h.object = b.valid ? b.object : PSX_INVALID_ID;
+        h.primitive = b.valid ? b.cprim : PSX_INVALID_ID;
+        h.traversals = 1u;
+        h.transmittance[0] = h.transmittance[1] = h.transmittance[2] = 0.0f;
+    }
+}
+
+// `cursor` (one word, zero at launch) hands out 64 queries per wave; `spill`
+// holds kTraversalStackDepth words per lane of the grid.
+__global__ __launch_bounds__(kQueryBlock) void intersect_kernel(SceneView s, const psx_ray_query *queries,
+                                                                uint32_t count, psx_ray_hit *hits, uint32_t *cursor,
+                                                                uint32_t *spill) {
+    __shared__ uint32_t lds[kShortStack * kQueryBlock];
+    Stack st = make_stack(lds, spill, kQueryBlock, threadIdx.x, blockIdx.x * kQueryBlock + threadIdx.x,
+                          gridDim.x * kQueryBlock);
+    const uint32_t lane = __lane_id();
+    for (;;) {
+        uint32_t base = 0u;
+        if (lane == 0u) base = atomicAdd(cursor, 64u);
+        base = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(base)));
+        if (base >= count) break;
+        const uint32_t i = base + lane;
+        if (i < count) {
+            const psx_ray_query q = queries[i];
+            psx_ray_hit h;
+            run_query(s, q, st, h);
+            hits[i] = h;
+        }
+    }
+}
+
+__device__ __forceinline__ f3 in3(const float *p) { return V(p[0], p[1], p[2]); }
+
+__device__ __forceinline__ Affine affine12(const float *p) {
+    Affine a;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) a.m[i] = p[i];
+    return a;
+}
+
+__device__ __forceinline__ void out3(float *o, f3 v) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+}
+
+// record layouts of psx_backend_probe (include/psx.h)
+__global__ void probe_kernel(uint32_t function, const float *inputs, uint32_t in_width, uint32_t count,
+                             float *outputs, uint32_t out_width) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const float *in = inputs + static_cast<size_t>(in_width) * i;
+    float *out = outputs + static_cast<size_t>(out_width) * i;
+    switch (function) {
+    case PSX_PROBE_FILM_CLAMP:
+        out3(out, film_clamp(in3(in), static_cast<uint32_t>(in[3]), in[4], in[5]));
+        break;
+    case PSX_PROBE_T_OFFSET:
+        out[0] = next_float_up(in[0]);
+        break;
+    case PSX_PROBE_SELF_EXCLUSION: {
+        const f3 oo = in3(in + 6), od = in3(in + 9), p0 = in3(in + 12), p1 = in3(in + 15), p2 = in3(in + 18);
+        out3(out, origin_with_self_exclusion(in3(in), in3(in + 3), oo, od, p0, p1, p2));
+        out[3] = source_covers_origin(oo, od, p0, p1, p2) ? 1.0f : 0.0f;
+        break;
+    }
+    case PSX_PROBE_SHADOW_TERMINATOR:
+    case PSX_PROBE_SURFACE_SHADOW_ORIGIN: {
+        const Affine o2w = affine12(in + 35);
+        const bool smooth = in[13] != 0.0f, applied = in[14] != 0.0f;
+        ShadowOrigin o;
+        if (function == PSX_PROBE_SURFACE_SHADOW_ORIGIN) {
+            o = shadow_origin_core(in3(in), in3(in + 3), in3(in + 6), in3(in + 9), in[12], smooth, o2w,
+                                   affine12(in + 47), applied, in[15], in[16], in3(in + 17), in3(in + 20),
+                                   in3(in + 23), in3(in + 26), in3(in + 29), in3(in + 32));
+        } else {
+            o = shadow_terminator_origin(in3(in), in3(in + 3), in3(in + 6), in3(in + 9), in[12], smooth, o2w,
+                                         applied, in[15], in[16], in3(in + 17), in3(in + 20), in3(in + 23),
+                                         in3(in + 26), in3(in + 29), in3(in + 32));
+        }
+        out3(out, o.position);
+        out[3] = o.skip_self ? 1.0f : 0.0f;
+        break;
+    }
+    case PSX_PROBE_TRIANGLE_SAMPLE: {
+        const f3 ref = in3(in), p0 = in3(in + 3), p1 = in3(in + 6), p2 = in3(in + 9);
+        const LightSample ls = sample_triangle(ref, p0, p1, p2, in[12], in[13]);
+        out3(out, ls.position);
+        out3(out + 3, ls.direction);
+        out[6] = ls.distance;
+        out[7] = ls.pdf;
+        out[8] = ls.valid ? 1.0f : 0.0f;
+        out[9] = use_solid_angle(ref, p0, p1, p2) ? 1.0f : 0.0f;
+        break;
+    }
+    case PSX_PROBE_TRIANGLE_PDF: {
+        const f3 ref = in3(in), p0 = in3(in + 6), p1 = in3(in + 9), p2 = in3(in + 12);
+        float value = 0.0f;
+        const bool valid = triangle_pdf(ref, in3(in + 3), p0, p1, p2, value);
+        out[0] = value;
+        out[1] = valid ? 1.0f : 0.0f;
+        out[2] = use_solid_angle(ref, p0, p1, p2) ? 1.0f : 0.0f;
+        break;
+    }
+    case PSX_PROBE_DIFFUSE_EVAL: {
+        const f3 n = in3(in);
+        out3(out, diffuse_intensity(n, in[3], in3(in + 4), in3(in + 7), in3(in + 10)));
+        out[3] = fmaxf(dot(n, in3(in + 10)), 0.0f) * INV_PI;
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers called by the host library
+// ---------------------------------------------------------------------------
+uint32_t query_resident_lanes() {
+    return resident_blocks(reinterpret_cast<const void *>(intersect_kernel), kQueryBlock) * kQueryBlock;
+}
+
+// Lanes of a launch for `count` queries, at most `max_lanes` (a multiple of
+// the block); the spill buffer must hold kTraversalStackDepth words per lane.
+uint32_t query_lanes(uint32_t count, uint32_t max_lanes) {
+    const uint32_t want = (count + kQueryBlock - 1u) / kQueryBlock * kQueryBlock;
+    const uint32_t cap = max_lanes / kQueryBlock * kQueryBlock;
+    return want < cap ? want : cap;
+}
+
+hipError_t launch_intersect(const SceneView &s, const psx_ray_query *queries, uint32_t count, psx_ray_hit *hits,
+                            uint32_t *cursor, uint32_t *spill, uint32_t lanes, hipStream_t stream) {
+    hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(64), 0, stream, cursor, 1u);
+    if (count == 0u || lanes < kQueryBlock) return hipGetLastError();
+    hipLaunchKernelGGL(intersect_kernel, dim3(lanes / kQueryBlock), dim3(kQueryBlock), 0, stream, s, queries, count,
+                       hits, cursor, spill);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe(uint32_t function, const float *inputs, uint32_t in_width, uint32_t count, float *outputs,
+                        uint32_t out_width, hipStream_t stream) {
+    if (count == 0u) return hipSuccess;
+    hipLaunchKernelGGL(probe_kernel, dim3((count + 255u) / 256u), dim3(256), 0, stream, function, inputs, in_width,
+                       count, outputs, out_width);
+    return hipGetLastError();
+}
+
+}  // namespace psx

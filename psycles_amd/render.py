@@ -96,6 +96,16 @@ class CompiledScene:
             self.handle, material, p.ctypes.data_as(C.POINTER(C.c_float)), p.size)
         return rc == abi.PSX_OK
 
+    def intersect(self, records) -> list:
+        """psx_ray_hit records for query dicts (abi.make_queries): closest
+        hits and transparent-shadow transmittance through the device
+        traversal."""
+        queries = abi.make_queries(records)
+        hits = (abi.RayHit * len(records))()
+        abi.check("psx_scene_intersect",
+                  self._backend.lib.psx_scene_intersect(self.handle, queries, len(records), hits))
+        return list(hits)
+
     def close(self) -> None:
         if self.handle:
             self._backend.lib.psx_scene_destroy(self.handle)
@@ -272,6 +282,18 @@ class RendererBackend:
             return None
         abi.check("psx_session_create", rc)
         return RenderSession(self, scene, handle, settings)
+
+    def probe(self, function: int, inputs) -> np.ndarray:
+        """psx_backend_probe over rows of `inputs` (widths in abi.PROBE_WIDTHS)."""
+        wi, wo = abi.PROBE_WIDTHS[function]
+        x = np.ascontiguousarray(np.atleast_2d(np.asarray(inputs, dtype=np.float32)))
+        if x.shape[1] != wi or self.lib.psx_probe_input_width(function) != wi:
+            raise ValueError(f"probe {function} takes {wi} floats per record")
+        out = np.zeros((x.shape[0], wo), dtype=np.float32)
+        abi.check("psx_backend_probe", self.lib.psx_backend_probe(
+            self.handle, function, x.ctypes.data_as(C.POINTER(C.c_float)), x.shape[0],
+            out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
 
     def close(self) -> None:
         if self.handle:

@@ -114,6 +114,13 @@ def load():
     lib.oracle_light_tree_sample.argtypes = [P(LightTree), P(f32), P(f32), C.c_int, f32, P(u32), P(f32), P(f32)]
     lib.oracle_light_tree_reverse_pdf.argtypes = [P(LightTree), u32, P(f32), P(f32), C.c_int]
     lib.oracle_light_tree_reverse_pdf.restype = f32
+    lib.oracle_scene_intersect.argtypes = [vp, P(abi.RayQuery), u32, P(abi.RayHit)]
+    lib.oracle_probe_input_width.argtypes = [u32]
+    lib.oracle_probe_input_width.restype = u32
+    lib.oracle_probe_output_width.argtypes = [u32]
+    lib.oracle_probe_output_width.restype = u32
+    lib.oracle_probe.argtypes = [u32, P(f32), u32, P(f32)]
+    lib.oracle_diffuse_closure_probe.argtypes = [P(f32), f32, P(f32), P(f32)]
     lib.oracle_sample_lamp.argtypes = [P(abi.Light), P(f32), P(f32), C.c_int, P(f32), P(f32)]
     lib.oracle_intersect_lamp.argtypes = [P(abi.Light), P(f32), P(f32), f32, f32, P(f32), C.c_int,
                                           P(f32)]
@@ -229,6 +236,14 @@ class OracleScene:
                                    uptr(out))
         return out
 
+    def intersect(self, records):
+        """psx_ray_hit records of the oracle's traversal for query dicts
+        (abi.make_queries)."""
+        queries = abi.make_queries(records)
+        hits = (abi.RayHit * len(records))()
+        assert load().oracle_scene_intersect(self.handle, queries, len(records), hits) == 0
+        return list(hits)
+
     def path_trace(self, settings, x, y, sample_index, total):
         """((328, 4) float32 schema v2 slots, (8, 8) uint32 events)."""
         slots = np.zeros((abi.PATH_TRACE_SLOTS, 4), dtype=np.float32)
@@ -263,6 +278,25 @@ class Film:
                                  fptr(self.combined), fptr(self.normal), fptr(self.albedo),
                                  uptr(self.sample_count), fptr(out))
         return out
+
+
+def probe(function: int, inputs) -> np.ndarray:
+    """oracle_probe over rows of `inputs` (float32, width per abi.PROBE_WIDTHS)."""
+    wi, wo = abi.PROBE_WIDTHS[function]
+    x = np.ascontiguousarray(np.atleast_2d(np.asarray(inputs, dtype=np.float32)))
+    assert x.shape[1] == wi == load().oracle_probe_input_width(function)
+    out = np.zeros((x.shape[0], wo), dtype=np.float32)
+    assert load().oracle_probe(function, fptr(x), x.shape[0], fptr(out)) == 0
+    return out
+
+
+def diffuse_closure_probe(color, u_lobe, u_dir) -> np.ndarray:
+    """(11, 4) records of the reference closure fixture layout."""
+    out = np.zeros((11, 4), dtype=np.float32)
+    c = np.asarray(color, dtype=np.float32)
+    u = np.asarray(u_dir, dtype=np.float32)
+    load().oracle_diffuse_closure_probe(fptr(c), float(u_lobe), fptr(u), fptr(out))
+    return out
 
 
 def sampler_probe(settings, sample_index, total, first_dim, dim_count) -> np.ndarray:

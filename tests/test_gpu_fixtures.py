@@ -1,0 +1,229 @@
+"""The device functions against the reference's own function fixtures,
+through the C ABI: traversal records and ordered shadow hits
+(psx_scene_intersect), the film clamp, tmin offset, self-exclusion and
+shadow-terminator origins, triangle-light sampling and the diffuse closure
+(psx_backend_probe); device ray queries against the oracle on real scenes
+(closest hits bit-identical, any-hit shadow transmittance bit-identical to
+the ordered transparent loop); and films with Oren-Nayar closures and with
+the shadow-terminator offset against the oracle."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+import reference_fixtures as rf
+from psycles_amd import abi, scenes
+from psycles_amd.render import RendererBackend
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def backend():
+    return RendererBackend(0)
+
+
+def compile_scene(backend, snap):
+    comp = backend.compile_scene(snap)
+    assert comp.ok(), [d.message for d in comp.diagnostics]
+    return comp.scene
+
+
+def test_traversal_records(backend):
+    log("case: reference traversal records 10-25 on the device")
+    scene = rf.TraversalScene()
+    dev = compile_scene(backend, scene.snapshot)
+    hits = dev.intersect(scene.queries())
+    assert rf.check_traversal(scene, hits, dev.intersect) == []
+
+
+def test_ordered_shadow_hits(backend):
+    log("case: ordered shadow hits")
+    dev = compile_scene(backend, rf.ordered_hits_snapshot())
+    assert rf.check_ordered_hits(dev.intersect) == []
+
+
+def test_function_probes_match_reference(backend):
+    log("case: function probes vs reference records")
+    v = rf.vectors()
+    out = backend.probe(abi.PROBE_T_OFFSET, np.asarray(v["surface_ray"]["t_offset"]["distances"], np.float32)[:, None])
+    assert out[:, 0].view(np.uint32).tolist() == v["surface_ray"]["t_offset"]["expected_bits"]
+    x, certified = rf.self_exclusion_inputs()
+    out = backend.probe(abi.PROBE_SELF_EXCLUSION, x)
+    assert [bool(c) for c in out[:, 3]] == certified
+    x, tv = rf.terminator_inputs()
+    assert rf.check_terminator(backend.probe(abi.PROBE_SHADOW_TERMINATOR, x), tv) == []
+    assert rf.check_neighbouring(backend.probe(abi.PROBE_SURFACE_SHADOW_ORIGIN, rf.neighbouring_inputs())) == []
+    x, expected = rf.film_clamp_inputs()
+    out = backend.probe(abi.PROBE_FILM_CLAMP, x)
+    assert np.all(np.isfinite(out)) and np.max(np.abs(out - expected)) <= v["film_light"]["tolerance"]
+    x, _, _ = rf.triangle_inputs()
+    sample = backend.probe(abi.PROBE_TRIANGLE_SAMPLE, x)
+    pdf = backend.probe(abi.PROBE_TRIANGLE_PDF, np.asarray([[*x[0, :3], *sample[0, :3], *x[0, 3:12]]], np.float32))[0]
+    assert rf.check_triangle(sample, pdf, rf.far_pdf()) == []
+
+
+def _probe_inputs(function, rng, n):
+    """Random records for a probe function (device vs oracle)."""
+    if function == abi.PROBE_FILM_CLAMP:
+        c = rng.normal(0, 3, (n, 3))
+        return np.concatenate([c, rng.integers(0, 3, (n, 1)), rng.uniform(0, 4, (n, 2))], 1)
+    if function == abi.PROBE_T_OFFSET:
+        return rng.uniform(0, 100, (n, 1))
+    if function == abi.PROBE_DIFFUSE_EVAL:
+        nrm = np.tile([0.0, 0.0, 1.0], (n, 1))
+        wi = rng.normal(size=(n, 3))
+        wi[:, 2] = np.abs(wi[:, 2])
+        wo = rng.normal(size=(n, 3))
+        wo[:, 2] = np.abs(wo[:, 2])
+        wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+        wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+        return np.concatenate([nrm, rng.uniform(0, 1, (n, 1)), rng.uniform(0, 1, (n, 3)), wi, wo], 1)
+    if function in (abi.PROBE_TRIANGLE_SAMPLE,):
+        ref = rng.normal(0, 1, (n, 3))
+        p = rng.normal(0, 1, (n, 9)) + np.array([0, 0, 3] * 3)
+        return np.concatenate([ref, p, rng.uniform(0, 1, (n, 2))], 1)
+    raise ValueError(function)
+
+
+@pytest.mark.parametrize("function,exact", [(abi.PROBE_FILM_CLAMP, True), (abi.PROBE_T_OFFSET, True),
+                                            (abi.PROBE_DIFFUSE_EVAL, False), (abi.PROBE_TRIANGLE_SAMPLE, False)])
+def test_function_probes_match_oracle(backend, function, exact):
+    log(f"case: probe {function} device vs oracle")
+    rng = np.random.default_rng(100 + function)
+    x = _probe_inputs(function, rng, 4096).astype(np.float32)
+    dev = backend.probe(function, x)
+    ora = ob.probe(function, x)
+    if exact:
+        assert np.array_equal(dev.view(np.uint32), ora.view(np.uint32))
+    else:
+        # transcendentals (acos in Oren-Nayar, atan2/sin/cos in the
+        # spherical-triangle sampler) come from ocml and glibc
+        scale = np.maximum(np.maximum(np.abs(dev), np.abs(ora)), 1e-3)
+        frac = float(np.mean(np.all(np.abs(dev - ora) <= 1e-4 * scale, axis=1)))
+        log(f"  within 1e-4 on {frac:.5f} of records")
+        assert frac >= 0.999
+
+
+def _random_queries(snap, rng, n, kind, transparent_max=8):
+    lo = np.array([-1.0, -1.0, -1.0]) if snap.name == "cornell" else np.array([-15.0, -15.0, -15.0])
+    hi = -lo
+    o = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    tmax = rng.uniform(0.5, 40.0, n)
+    recs = []
+    for i in range(n):
+        recs.append({"origin": o[i], "direction": d[i], "tmin": 0.0, "tmax": float(tmax[i]),
+                     "visibility": abi.VIS_ALL if kind == abi.QUERY_CLOSEST else abi.VIS_SHADOW,
+                     "kind": kind, "transparent_max": transparent_max})
+    return recs
+
+
+@pytest.mark.parametrize("name", ["cornell", "instanced"])
+def test_queries_match_oracle(backend, name):
+    log(f"case: random ray queries {name}")
+    snap = scenes.cornell_box() if name == "cornell" else scenes.instanced_icospheres(frequency=10, instances=12,
+                                                                                      box=14.0)
+    dev = compile_scene(backend, snap)
+    ora = ob.OracleScene(snap)
+    rng = np.random.default_rng(11)
+    closest = _random_queries(snap, rng, 20000, abi.QUERY_CLOSEST)
+    a, b = dev.intersect(closest), ora.intersect(closest)
+    for x, y in zip(a, b):
+        assert (x.hit, x.object, x.primitive) == (y.hit, y.object, y.primitive)
+        assert np.float32(x.t).view(np.uint32) == np.float32(y.t).view(np.uint32)
+    hits = sum(x.hit for x in a)
+    log(f"  closest: {hits} of {len(a)} hit, identical")
+    for tmax_depth in (8, 2):
+        shadow = _random_queries(snap, rng, 20000, abi.QUERY_SHADOW, tmax_depth)
+        a, b = dev.intersect(shadow), ora.intersect(shadow)
+        occluded = 0
+        partial = 0
+        for x, y in zip(a, b):
+            assert list(np.float32(x.transmittance).view(np.uint32)) == \
+                list(np.float32(y.transmittance).view(np.uint32))
+            assert x.hit == y.hit
+            assert x.traversals <= y.traversals  # the opaque early end only saves passes
+            occluded += x.hit
+            partial += (not x.hit) and y.traversals > 1
+        log(f"  shadow (limit {tmax_depth}): {occluded} occluded, {partial} through transparent surfaces, "
+            f"passes {sum(x.traversals for x in a)} device / {sum(y.traversals for y in b)} ordered")
+
+
+REL_TOL = 1.0e-4
+
+
+def relative_ok(a, b, tol=REL_TOL):
+    diff = np.abs(a - b).max(axis=-1)
+    scale = np.maximum(np.maximum(np.abs(a), np.abs(b)).max(axis=-1), 1e-3)
+    return diff <= tol * scale
+
+
+def _film_parity(backend, snap, settings, spp, label):
+    dev = compile_scene(backend, snap)
+    sess = backend.create_session(dev, settings)
+    sess.render_device(0, spp, spp)
+    comb, norm, alb, cnt = sess.read_film()
+    film, _ = ob.OracleScene(snap).render(settings, 0, spp, spp)
+    assert np.array_equal(cnt, film.sample_count)
+    for nm, g, c in (("combined", comb, film.combined), ("normal", norm, film.normal),
+                     ("albedo", alb, film.albedo)):
+        frac = float(np.mean(relative_ok(g, c)))
+        log(f"  {label} {nm}: within 1e-4 on {frac:.5f}")
+        assert frac >= 0.999, (label, nm)
+    return comb
+
+
+def test_oren_nayar_film(backend):
+    log("case: cornell with Oren-Nayar walls and boxes, 160x100x8")
+    snap = scenes.cornell_box()
+    for i, sigma in ((0, 0.5), (1, 0.8), (2, 0.3)):
+        m = snap.materials[i]
+        snap.materials[i] = scenes.diffuse_material(tuple(m.parameters[:3]), roughness=sigma)
+    settings = scenes.cornell_settings(160, 100)
+    comb = _film_parity(backend, snap, settings, 8, "oren-nayar")
+    lam = scenes.cornell_box()
+    dev = compile_scene(backend, lam)
+    sess = backend.create_session(dev, settings)
+    sess.render_device(0, 8, 8)
+    assert not np.allclose(sess.read_film()[0], comb)  # the roughness is seen
+
+
+def smooth_sphere_scene(offset):
+    """A coarse smooth-shaded icosphere on a floor under a point lamp and a
+    grazing sun: the shadow terminator runs across flat facets."""
+    pos, tris = scenes.geodesic_icosphere(3, displacement=0.0)
+    normals = pos / np.linalg.norm(pos, axis=1, keepdims=True)
+    sphere = scenes.MeshData(pos, tris, [0], normals=normals.astype(np.float32),
+                             smooth=np.ones(tris.shape[0], np.uint8))
+    fp, ft = scenes.quad([[-4, -1.0, -4], [4, -1.0, -4], [4, -1.0, 4], [-4, -1.0, 4]])
+    floor = scenes.MeshData(fp, ft, [1])
+    mats = [scenes.diffuse_material((0.8, 0.8, 0.8)), scenes.diffuse_material((0.5, 0.5, 0.5))]
+    insts = [scenes.InstanceData(0, scenes.translation(0.0, 0.0, 0.0), shadow_terminator_geometry_offset=offset),
+             scenes.InstanceData(0, scenes.translation(2.2, 0.0, -0.5) @ scenes.scale(0.7, 0.7, 0.7),
+                                 shadow_terminator_geometry_offset=offset),
+             scenes.InstanceData(1, scenes.identity())]
+    cam = scenes.CameraData(scenes.look_at((0.5, 1.0, 5.0), (0.5, -0.2, 0.0)), math.radians(45.0), 1e-3, 100.0)
+    snap = scenes.SceneSnapshot(mats, [sphere, floor], insts, cam, (0.02, 0.02, 0.02), name="terminator")
+    snap.lights = [scenes.lamp(abi.LIGHT_POINT, scenes.translation(3.0, 2.0, 1.0), (1, 1, 1), 60.0, size=0.05),
+                   scenes.lamp(abi.LIGHT_DISTANT, scenes.frame_z((1.0, 0.15, 0.2)), (1, 0.95, 0.9), 2.0,
+                               angle=0.02)]
+    return snap
+
+
+def test_shadow_terminator_film(backend):
+    log("case: shadow terminator offset 0.1 vs oracle, 160x120x8")
+    settings = scenes.cornell_settings(160, 120)
+    lifted = _film_parity(backend, smooth_sphere_scene(0.1), settings, 8, "terminator 0.1")
+    plain = _film_parity(backend, smooth_sphere_scene(0.0), settings, 8, "terminator 0")
+    changed = float(np.mean(np.any(lifted != plain, axis=-1)))
+    log(f"  the offset changes {changed:.4f} of pixels")
+    assert changed > 0.001
