@@ -103,13 +103,22 @@ def test_function_probes_match_oracle(backend, function, exact):
     ora = ob.probe(function, x)
     if exact:
         assert np.array_equal(dev.view(np.uint32), ora.view(np.uint32))
-    else:
-        # transcendentals (acos in Oren-Nayar, atan2/sin/cos in the
-        # spherical-triangle sampler) come from ocml and glibc
-        scale = np.maximum(np.maximum(np.abs(dev), np.abs(ora)), 1e-3)
-        frac = float(np.mean(np.all(np.abs(dev - ora) <= 1e-4 * scale, axis=1)))
-        log(f"  within 1e-4 on {frac:.5f} of records")
-        assert frac >= 0.999
+        return
+    # Transcendentals (acos in Oren-Nayar, atan2/sin/cos/acos in the
+    # spherical-triangle sampler) come from ocml on the device and glibc in
+    # the oracle and may differ in the last ulp.  The comparison is held on
+    # the records the oracle itself reproduces within 1e-5 when every input
+    # moves by one ulp: elsewhere the function amplifies an ulp beyond the
+    # bar (the triangle sampler does so for about 8 % of random triangles).
+    def close(a, b, tol):
+        scale = np.maximum(np.maximum(np.abs(a), np.abs(b)), 1e-3)
+        return np.all(np.abs(a - b) <= tol * scale, axis=1)
+
+    nudged = ob.probe(function, np.nextafter(x, np.float32(np.inf)))
+    stable = close(ora, nudged, 1e-5)
+    frac = float(np.mean(close(dev, ora, 1e-4)[stable]))
+    log(f"  {int(stable.sum())} of {len(x)} records well conditioned; within 1e-4 on {frac:.5f} of them")
+    assert stable.mean() > 0.5 and frac >= 0.999
 
 
 def _random_queries(snap, rng, n, kind, transparent_max=8):
