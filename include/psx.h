@@ -193,12 +193,17 @@ typedef struct psx_instance {
     float transform[16];
     const uint32_t *material_overrides;
     uint32_t material_override_count;
+    /* Cycles' shadow-terminator geometry offset (>= 0, finite): shadow
+     * origins on smooth triangles are lifted onto the parabolic surface
+     * through the vertex normals (include/psycles/luisa/surface_ray.h:108-309). */
     float shadow_terminator_geometry_offset;
 } psx_instance;
 
-/* contract::LightDesc (scene.h:377-403).  Only BACKGROUND lights are
- * rendered by this ABI version; they fold into the constant world radiance
- * exactly like src/luisa/path_tracer_scene.cpp:1728-1733. */
+/* contract::LightDesc (scene.h:377-403): point (sphere or disk), spot,
+ * distant, area (rectangle or ellipse, with spread) and background lights.
+ * Background lights fold into the constant world radiance exactly like
+ * src/luisa/path_tracer_scene.cpp:1728-1733; the others are sampled by NEE
+ * through the flat CDF or the light tree and hit by forward rays. */
 typedef struct psx_light {
     uint32_t type;
     uint32_t visibility_mask;

@@ -66,5 +66,16 @@ def build_library(verbose: bool = False) -> Path:
     return out
 
 
+def build_contract_driver() -> Path:
+    """Compile tests/contract_driver.cpp (a host program on the C++ contract
+    mirror include/psx_contract.hpp) against libpsx.so into _build/."""
+    BUILD.mkdir(exist_ok=True)
+    out = BUILD / "contract_driver"
+    src = PKG.parent / "tests" / "contract_driver.cpp"
+    _run(["g++", "-std=c++20", "-O1", "-Wall", f"-I{ROCM / 'include'}", "-D__HIP_PLATFORM_AMD__", str(src),
+          "-o", str(out), f"-L{PKG}", "-lpsx", "-Wl,-rpath,$ORIGIN/../psycles_amd", f"-Wl,-rpath,{ROCM / 'lib'}"])
+    return out
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))

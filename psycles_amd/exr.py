@@ -27,10 +27,12 @@ import numpy as np
 from . import abi
 from .render import OutputSink, PassTile
 
-# Cycles / Blender pass names (the names in Cycles' multilayer EXR files)
+# Cycles / Blender pass names (the names in Cycles' multilayer EXR files); the
+# albedo pass is "DiffCol" as the reference's Blender example names it
+# (examples/render_blender_scene.cpp:430-433), distinct from Denoising Albedo
 PASS_NAMES = {
     abi.PASS_COMBINED: "Combined", abi.PASS_DEPTH: "Depth", abi.PASS_NORMAL: "Normal",
-    abi.PASS_POSITION: "Position", abi.PASS_ALBEDO: "Denoising Albedo", abi.PASS_EMISSION: "Emit",
+    abi.PASS_POSITION: "Position", abi.PASS_ALBEDO: "DiffCol", abi.PASS_EMISSION: "Emit",
     abi.PASS_DIFFUSE_DIRECT: "DiffDir", abi.PASS_DIFFUSE_INDIRECT: "DiffInd",
     abi.PASS_GLOSSY_DIRECT: "GlossDir", abi.PASS_GLOSSY_INDIRECT: "GlossInd",
     abi.PASS_TRANSMISSION_DIRECT: "TransDir", abi.PASS_TRANSMISSION_INDIRECT: "TransInd",

@@ -118,23 +118,51 @@ class CompiledScene:
             pass
 
 
+class SinkCallbacks:
+    """The ctypes callbacks of one psx_sink.  ctypes prints and drops an
+    exception raised inside a callback, so each callback stores the first one
+    here and `reraise()` raises it once the C call has returned."""
+
+    def __init__(self, output: OutputSink, settings):
+        self.error: BaseException | None = None
+
+        def guard(fn):
+            def call(*args):
+                if self.error is not None:
+                    return
+                try:
+                    fn(*args)
+                except BaseException as e:  # re-raised by reraise()
+                    self.error = e
+            return call
+
+        def on_begin(user, settings_ptr):
+            output.begin(settings)
+
+        def on_write(user, kind, channels, wx, wy, ww, wh, fw, fh, pixels):
+            arr = np.ctypeslib.as_array(pixels, shape=(wh * ww * channels,)).copy()
+            output.write(PassTile(kind, channels, (wx, wy, ww, wh), (fw, fh), arr.reshape(wh, ww, channels)))
+
+        def on_end(user, cancelled):
+            output.end(bool(cancelled))
+
+        self.refs = (abi.SINK_BEGIN(guard(on_begin)), abi.SINK_WRITE(guard(on_write)),
+                     abi.SINK_END(guard(on_end)))
+
+    def reraise(self) -> None:
+        if self.error is not None:
+            err, self.error = self.error, None
+            raise err
+
+
 def make_sink(output: OutputSink, settings):
     """psx_sink forwarding to `output`; keep the returned callbacks alive for
-    the duration of the call that uses the sink."""
-    def on_begin(user, settings_ptr):
-        output.begin(settings)
-
-    def on_write(user, kind, channels, wx, wy, ww, wh, fw, fh, pixels):
-        arr = np.ctypeslib.as_array(pixels, shape=(wh * ww * channels,)).copy()
-        output.write(PassTile(kind, channels, (wx, wy, ww, wh), (fw, fh), arr.reshape(wh, ww, channels)))
-
-    def on_end(user, cancelled):
-        output.end(bool(cancelled))
-
-    refs = (abi.SINK_BEGIN(on_begin), abi.SINK_WRITE(on_write), abi.SINK_END(on_end))
+    the duration of the call that uses the sink and call their `reraise()`
+    after it, so an exception raised by the sink reaches the caller."""
+    callbacks = SinkCallbacks(output, settings)
     sink = abi.Sink()
-    sink.begin, sink.write, sink.end = refs
-    return sink, refs
+    sink.begin, sink.write, sink.end = callbacks.refs
+    return sink, callbacks
 
 
 class RenderSession:
@@ -157,10 +185,11 @@ class RenderSession:
 
     def render_samples(self, samples: SampleRange, output: OutputSink) -> bool:
         lib = self._backend.lib
-        sink, _refs = make_sink(output, self.settings)
+        sink, callbacks = make_sink(output, self.settings)
         rc = lib.psx_session_render(self.handle, samples.first, samples.count, samples.offset,
                                     samples.total, C.byref(sink))
         self.last_status = rc
+        callbacks.reraise()
         if rc in (abi.PSX_ERROR_DEVICE, abi.PSX_ERROR_OUT_OF_MEMORY, abi.PSX_ERROR_NO_DEVICE):
             raise abi.PsxError("psx_session_render", rc)
         return rc == abi.PSX_OK

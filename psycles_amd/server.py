@@ -61,9 +61,10 @@ class ServerRequest:
         return combined, normal, albedo, counts
 
     def write_passes(self, output: OutputSink) -> None:
-        sink, _refs = make_sink(output, self.settings)
-        abi.check("psx_server_write_passes",
-                  self.server.lib.psx_server_write_passes(self.server.handle, self.id, C.byref(sink)))
+        sink, callbacks = make_sink(output, self.settings)
+        rc = self.server.lib.psx_server_write_passes(self.server.handle, self.id, C.byref(sink))
+        callbacks.reraise()
+        abi.check("psx_server_write_passes", rc)
 
     def cancel(self) -> None:
         abi.check("psx_server_cancel", self.server.lib.psx_server_cancel(self.server.handle, self.id))

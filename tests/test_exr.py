@@ -92,3 +92,44 @@ def test_sink_writes_the_render_passes(tmp_path):
     ch, _ = exr.read_exr(tmp_path / "sink.exr")
     assert np.array_equal(ch["ViewLayer.Emit.G"], px[..., 1])
     assert np.array_equal(ch["ViewLayer.Combined.A"], px[..., 3])
+
+
+def test_albedo_and_denoising_albedo_are_distinct_channels(tmp_path):
+    """The albedo pass is DiffCol (examples/render_blender_scene.cpp:430-433),
+    so a render requesting both albedo passes writes both."""
+    sink = exr.ExrOutputSink(tmp_path / "both.exr")
+    px = np.full((2, 2, 3), 0.5, np.float32)
+    sink.begin(None)
+    sink.write(PassTile(abi.PASS_ALBEDO, 3, (0, 0, 2, 2), (2, 2), px))
+    sink.write(PassTile(abi.PASS_DENOISING_ALBEDO, 3, (0, 0, 2, 2), (2, 2), px * 2))
+    sink.end(False)
+    ch, _ = exr.read_exr(tmp_path / "both.exr")
+    assert np.all(ch["ViewLayer.DiffCol.R"] == 0.5)
+    assert np.all(ch["ViewLayer.Denoising Albedo.R"] == 1.0)
+
+
+def test_sink_exceptions_reach_the_caller():
+    """An exception raised inside a sink callback is held and re-raised after
+    the C call, not printed and dropped by ctypes."""
+    import ctypes as C
+
+    from psycles_amd.render import OutputSink, make_sink
+
+    class Failing(OutputSink):
+        def begin(self, settings):
+            pass
+
+        def write(self, tile):
+            raise ValueError("disk full")
+
+        def end(self, cancelled):
+            raise AssertionError("end must not run after a failed write")
+
+    sink, callbacks = make_sink(Failing(), None)
+    pixels = (C.c_float * 4)(1, 2, 3, 4)
+    sink.begin(None, None)
+    sink.write(None, abi.PASS_COMBINED, 4, 0, 0, 1, 1, 1, 1, pixels)
+    sink.end(None, 0)
+    with pytest.raises(ValueError, match="disk full"):
+        callbacks.reraise()
+    callbacks.reraise()  # raised once

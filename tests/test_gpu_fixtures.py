@@ -236,3 +236,11 @@ def test_shadow_terminator_film(backend):
     changed = float(np.mean(np.any(lifted != plain, axis=-1)))
     log(f"  the offset changes {changed:.4f} of pixels")
     assert changed > 0.001
+
+
+def test_invalid_terminator_offset_is_rejected(backend):
+    log("case: negative and non-finite terminator offsets")
+    for bad in (-0.1, float("nan"), float("inf")):
+        comp = backend.compile_scene(smooth_sphere_scene(bad))
+        assert not comp.ok()
+        assert any("shadow terminator" in d.message for d in comp.diagnostics)
