@@ -11,6 +11,7 @@ import sys
 import numpy as np
 import pytest
 
+import event_flips
 import oracle_binding as ob
 from psycles_amd import abi, scenes
 from psycles_amd.render import RendererBackend
@@ -78,15 +79,7 @@ def test_cornell_lamps_path_events():
     comp = be.compile_scene(snap)
     session = be.create_session(comp.scene, s)
     osc = ob.OracleScene(snap)
-    discrete = [abi.TRACE_FLAGS, abi.TRACE_RNG_OFFSET, abi.TRACE_OBJECT, abi.TRACE_PRIMITIVE,
-                abi.TRACE_LIGHT, abi.TRACE_CLOSURE, abi.TRACE_TERMINATE_BITS]
-    for sample in (0, 5):
-        gpu = session.trace_events(sample, 8)
-        cpu = osc.trace_events(s, sample, 8)
-        valid = (cpu[..., 0] & 1) != 0
-        same = np.all(gpu[..., discrete] == cpu[..., discrete], axis=-1)
-        log(f"  sample {sample}: {valid.sum()} events, discrete identical {same[valid].mean():.6f}")
-        assert same[valid].mean() >= PIXEL_FRACTION
+    event_flips.check("cornell lamps", session, osc, s, (0, 5), 8, log)
 
 
 def test_lamp_integrators_bit_identical():

@@ -13,6 +13,7 @@ import sys
 import numpy as np
 import pytest
 
+import event_flips
 import oracle_binding as ob
 from psycles_amd import abi, scenes
 from psycles_amd.render import RendererBackend
@@ -106,13 +107,7 @@ def test_tree_path_events(snap_name):
     comp = be.compile_scene(snap)
     session = be.create_session(comp.scene, s)
     osc = ob.OracleScene(snap)
-    for sample in (0, 3):
-        gpu = session.trace_events(sample, 6)
-        cpu = osc.trace_events(s, sample, 6)
-        valid = (cpu[..., 0] & 1) != 0
-        same = np.all(gpu[..., DISCRETE] == cpu[..., DISCRETE], axis=-1)
-        log(f"  sample {sample}: {valid.sum()} events, discrete identical {same[valid].mean():.6f}")
-        assert same[valid].mean() >= PIXEL_FRACTION
+    event_flips.check(f"light tree {snap_name}", session, osc, s, (0, 3), 6, log)
 
 
 def test_tree_integrators_bit_identical():
