@@ -16,7 +16,12 @@ scene plus 10,000 emissive triangles and 321 lamps, light-tree NEE,
 Prints one JSON line with samples/s (value), Mrays/s, the HBM roofline of the
 dominant kernel (the wavefront trace kernel, timed with HIP events around
 each of its launches on the session stream) and the CPU oracle timed on a
-bounded sample of the same workload on the host cores.
+bounded sample of the same workload on the host cores.  The default line
+also carries `serving`, the metric's latency half: the first 60 requests of
+the serving stream (Poisson arrivals at 200 req/s, seed 4, 30 % parameter
+edits over the resident Cornell, instanced and many-light scenes) through the
+render server and through the plain per-request path, with p50 / p99 request
+latency, viewport p99 and the batched-vs-plain ratio.
 """
 from __future__ import annotations
 
@@ -118,21 +123,18 @@ def calibration_copy(nbytes: int = 1 << 30) -> None:
     print(f"bench: calibration copy of {nbytes} bytes done", file=sys.stderr, flush=True)
 
 
-def serving_bench(args) -> int:
+def serving_leg(requests: int, with_plain: bool, log) -> dict:
     """The request stream over the resident Cornell, instanced and many-light
-    scenes: the render server (continuous batching) against the plain path
-    (one megakernel session per request, in arrival order)."""
+    scenes: the render server (continuous batching) and, optionally, the
+    plain path (one megakernel session per request, in arrival order)."""
     from psycles_amd import abi, scenes, serving
     from psycles_amd.render import RendererBackend
     from psycles_amd.server import RenderServer
 
-    def log(msg):
-        print(f"bench: {msg}", file=sys.stderr, flush=True)
-
     log("building the Cornell, instanced and many-light scenes")
     snaps = {"cornell": scenes.cornell_box(), "instanced": scenes.instanced_icospheres(),
              "many_lights": scenes.many_lights()}
-    stream = serving.make_stream(snaps, args.requests)
+    stream = serving.make_stream(snaps, requests)
     batched_backend = RendererBackend(0, integrator=abi.INTEGRATOR_WAVEFRONT)
     compiled = {n: batched_backend.compile_scene(sn).scene for n, sn in snaps.items()}
     server = RenderServer(batched_backend)
@@ -147,8 +149,45 @@ def serving_bench(args) -> int:
     log(f"stream of {len(stream)} requests over {stream[-1].arrival:.2f} s; running the server")
     batched = serving.run_batched(server, compiled, stream, log)
     server.close()
+    del compiled, batched_backend
     log(f"server: {batched['samples_per_s'] / 1e6:.1f} M samples/s, viewport p99 "
         f"{batched['viewport_latency_ms']['p99']} ms")
+    out = {"workload": f"serving: {len(stream)} requests, Poisson arrivals at 200 req/s (seed 4) over resident "
+                       "Cornell, instanced and many-light scenes; resolutions 256x256 to 1920x1080, 1-64 spp, "
+                       "30% parameter-only edits",
+           "batched": batched}
+    if with_plain:
+        plain_backend = RendererBackend(0, integrator=abi.INTEGRATOR_MEGAKERNEL)
+        plain_compiled = {n: plain_backend.compile_scene(sn).scene for n, sn in snaps.items()}
+        log("running the plain per-request path")
+        out["plain"] = serving.run_plain(plain_backend, plain_compiled, stream, log)
+        out["batched_vs_plain"] = batched["samples_per_s"] / out["plain"]["samples_per_s"]
+    return out
+
+
+def serving_summary(leg: dict) -> dict:
+    """The serving block of the default line: throughput, latency
+    percentiles and the batched-vs-plain ratio."""
+    b = leg["batched"]
+    out = {"workload": leg["workload"], "samples_per_s": b["samples_per_s"], "makespan_s": b["makespan_s"],
+           "latency_p50_ms": b["latency_ms"]["p50"], "latency_p99_ms": b["latency_ms"]["p99"],
+           "viewport_p50_ms": b["viewport_latency_ms"]["p50"], "viewport_p99_ms": b["viewport_latency_ms"]["p99"],
+           "viewport_requests": b["viewport_latency_ms"]["count"], "requests": b["requests"], "edits": b["edits"]}
+    if "plain" in leg:
+        p = leg["plain"]
+        out.update({"plain_samples_per_s": p["samples_per_s"], "plain_latency_p50_ms": p["latency_ms"]["p50"],
+                    "plain_viewport_p99_ms": p["viewport_latency_ms"]["p99"],
+                    "batched_vs_plain": leg["batched_vs_plain"]})
+    return out
+
+
+def serving_bench(args) -> int:
+    """`--workload serving`: the stream alone on the line."""
+    def log(msg):
+        print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+    leg = serving_leg(args.requests, not args.no_plain, log)
+    batched = leg["batched"]
     result = {
         "metric": METRIC,
         "value": batched["samples_per_s"],
@@ -162,21 +201,13 @@ def serving_bench(args) -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": f"serving: {len(stream)} requests, Poisson arrivals at 200 req/s (seed 4) over resident "
-                        "Cornell, instanced and many-light scenes; resolutions 256x256 to 1920x1080, 1-64 spp, "
-                        "30% parameter-only edits",
-            "integrator": "render server (continuous batching of wavefront batches)",
-        },
+        "config": {"workload": leg["workload"],
+                   "integrator": "render server (continuous batching of wavefront batches)"},
         "batched": batched,
     }
-    if not args.no_plain:
-        plain_backend = RendererBackend(0, integrator=abi.INTEGRATOR_MEGAKERNEL)
-        plain_compiled = {n: plain_backend.compile_scene(sn).scene for n, sn in snaps.items()}
-        log("running the plain per-request path")
-        plain = serving.run_plain(plain_backend, plain_compiled, stream, log)
-        result["plain"] = plain
-        result["batched_vs_plain"] = batched["samples_per_s"] / plain["samples_per_s"]
+    if "plain" in leg:
+        result["plain"] = leg["plain"]
+        result["batched_vs_plain"] = leg["batched_vs_plain"]
     print(json.dumps(result))
     return 0
 
@@ -248,6 +279,8 @@ def main() -> int:
     ap.add_argument("--spp", type=int, default=None, help="samples per pixel (default: the workload's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the megakernel frame")
+    ap.add_argument("--serving-requests", type=int, default=60,
+                    help="requests of the serving leg on the default line (0 skips it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -339,6 +372,16 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": rays_per_launch * w["traffic"] if w["traffic"] else None,
+            # `achieved` prices each ray at the algorithmic binary-BVH bytes
+            # B_ray; the measured HBM bytes of the kernel (FETCH_SIZE and
+            # WRITE_SIZE per ray) over the same launch time give the
+            # bandwidth the kernel really draws
+            "basis": "achieved: algorithmic B_ray = 48 + 32 N_node + 36 N_tri bytes per ray (binary SAH BVH); "
+                     "traffic: measured HBM bytes per launch",
+            "traffic_gbs": (rays_per_launch * w["traffic"] / (trace_launch_ms * 1e-3) / 1e9
+                            if w["traffic"] else None),
+            "traffic_frac": (rays_per_launch * w["traffic"] / (trace_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
+                             if w["traffic"] else None),
             "kernel": "wf_trace_kernel",
             "launch_ms": trace_launch_ms,
             "launches": trace_launches,
@@ -371,6 +414,12 @@ def main() -> int:
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(snap, settings, spp)
+    if args.serving_requests > 0 and args.workload == "instanced":
+        # the metric's latency half: the serving stream, batched and plain
+        del session, comp, backend
+        result["serving"] = serving_summary(serving_leg(
+            args.serving_requests, not args.no_plain,
+            lambda msg: print(f"bench: {msg}", file=sys.stderr, flush=True)))
     print(json.dumps(result))
     if args.calibrate_copy:
         calibration_copy()
