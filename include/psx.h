@@ -517,6 +517,19 @@ int psx_scene_intersect(psx_scene *scene, const psx_ray_query *queries, uint32_t
 int psx_scene_intersect_device(psx_scene *scene, const psx_ray_query *device_queries, uint32_t count,
                                psx_ray_hit *device_hits);
 
+/* psx_scene_intersect_device without any allocation.  No counterpart in
+ * the reference: this entry point is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  Its traversal
+ * scratch (a work cursor and a stack spill column per lane, for at most
+ * 65,536 lanes) is one static device array of 16 MiB + 256 B per device,
+ * resolved when a backend is created on that device; nothing is allocated per
+ * call.  Every call on a device shares that array: calls that overlap on one
+ * device race.  More than PSX_INTERSECT_NOALLOC_MAX_QUERIES queries is
+ * PSX_ERROR_INVALID_ARGUMENT. */
+#define PSX_INTERSECT_NOALLOC_MAX_QUERIES (1u << 30)
+int psx_scene_intersect_device_noalloc(psx_scene *scene, const psx_ray_query *device_queries, uint32_t count,
+                                       psx_ray_hit *device_hits);
+
 /* ---- single-function probes ----------------------------------------------
  *
  * Evaluates one device function of the render path on `count` input records

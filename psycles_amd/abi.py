@@ -239,6 +239,7 @@ class ServingStats(C.Structure):
 
 
 INVALID_ID = 0xFFFFFFFF
+INTERSECT_NOALLOC_MAX_QUERIES = 1 << 30
 QUERY_CLOSEST = 0
 QUERY_SHADOW = 1
 
@@ -335,7 +336,7 @@ EXPORTED_SYMBOLS = [
     "psx_server_update_parameters", "psx_server_status", "psx_server_wait",
     "psx_server_read_film", "psx_server_write_passes", "psx_server_cancel",
     "psx_server_release", "psx_server_stats",
-    "psx_scene_intersect", "psx_scene_intersect_device",
+    "psx_scene_intersect", "psx_scene_intersect_device", "psx_scene_intersect_device_noalloc",
     "psx_probe_input_width", "psx_probe_output_width", "psx_backend_probe",
 ]
 
@@ -392,6 +393,8 @@ def load_library(path: os.PathLike | None = None):
     lib.psx_scene_intersect.restype = C.c_int
     lib.psx_scene_intersect_device.argtypes = [vp, vp, u32, vp]
     lib.psx_scene_intersect_device.restype = C.c_int
+    lib.psx_scene_intersect_device_noalloc.argtypes = [vp, vp, u32, vp]
+    lib.psx_scene_intersect_device_noalloc.restype = C.c_int
     lib.psx_probe_input_width.argtypes = [u32]
     lib.psx_probe_input_width.restype = u32
     lib.psx_probe_output_width.argtypes = [u32]

@@ -21,6 +21,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -55,6 +56,9 @@ hipError_t launch_intersect(const SceneView &s, const psx_ray_query *queries, ui
                             uint32_t *cursor, uint32_t *spill, uint32_t lanes, hipStream_t stream);
 hipError_t launch_probe(uint32_t function, const float *inputs, uint32_t in_width, uint32_t count, float *outputs,
                         uint32_t out_width, hipStream_t stream);
+hipError_t setup_noalloc_scratch();
+hipError_t launch_intersect_noalloc(const SceneView &s, const psx_ray_query *queries, uint32_t count,
+                                    psx_ray_hit *hits, hipStream_t stream);
 // psx_wavefront.hip
 uint32_t wavefront_lanes();
 uint32_t wavefront_padded_pixels(const KernelParams &k);
@@ -459,6 +463,10 @@ struct Backend {
 // CDF, light tree.  Shared by parameter versions whose edits left them valid.
 struct EmitterTables {
     DeviceBuffer emitters, distribution, tree_nodes, tree_emitters, tree_mapping, emitter_cprim;
+    size_t bytes() const {
+        return emitters.bytes() + distribution.bytes() + tree_nodes.bytes() + tree_emitters.bytes() +
+               tree_mapping.bytes() + emitter_cprim.bytes();
+    }
     explicit EmitterTables(hipStream_t stream) {
         for (DeviceBuffer *b : {&emitters, &distribution, &tree_nodes, &tree_emitters, &tree_mapping, &emitter_cprim})
             b->set_stream(stream);
@@ -471,6 +479,10 @@ struct EmitterTables {
 struct ParamTables {
     DeviceBuffer params, materials_buf, material_shadow, material_emission;
     std::shared_ptr<EmitterTables> light;
+    // the version's own tables; `light` may be shared with other versions
+    size_t bytes() const {
+        return params.bytes() + materials_buf.bytes() + material_shadow.bytes() + material_emission.bytes();
+    }
     hipEvent_t ready = nullptr;  // recorded after the last upload and table kernel
     explicit ParamTables(hipStream_t stream) {
         for (DeviceBuffer *b : {&params, &materials_buf, &material_shadow, &material_emission}) b->set_stream(stream);
@@ -520,6 +532,16 @@ struct Scene {
         }
     }
     uint64_t triangles = 0;
+
+    // device bytes of the geometry and lamp tables (parameter versions apart)
+    size_t geometry_bytes() const {
+        size_t n = 0;
+        for (const DeviceBuffer *b : {&nodes, &tris, &instances_buf, &meshes_buf, &positions, &indices, &normals,
+                                      &smooth, &slots, &tri_slots, &overrides_buf, &leaves, &lamps, &lamp_spheres,
+                                      &lamp_bits})
+            n += b->bytes();
+        return n;
+    }
 
     void upload_materials(ParamTables &t);
     // a new parameter version from the host materials; the emitter tables
@@ -1854,6 +1876,27 @@ struct Server {
         }
     }
 
+    // HBM the server's work keeps resident besides arenas and films: the
+    // geometry of every scene it renders and each distinct parameter version
+    // (and emitter table) still reachable from the scene, the version history
+    // or a bound request.  Caller holds `mutex`.
+    size_t resident_bytes() const {
+        std::set<const void *> seen;
+        size_t n = 0;
+        auto add = [&](const std::shared_ptr<ParamTables> &t) {
+            if (!t || !seen.insert(t.get()).second) return;
+            n += t->bytes();
+            if (t->light && seen.insert(t->light.get()).second) n += t->light->bytes();
+        };
+        for (const auto &sv : versions) {
+            n += sv.first->geometry_bytes();
+            add(sv.first->tables);
+            for (const auto &h : sv.second.history) add(h.second.first);
+        }
+        for (const auto &rp : active) add(rp->tables);
+        return n;
+    }
+
     double &cost_of(const BatchKey &key) {
         for (auto &c : cost_ns)
             if (c.first == key) return c.second;
@@ -2043,11 +2086,16 @@ struct Server {
         for (auto &r : active)
             if (r->cancel.load() && !r->in_flight && r->state < PSX_REQUEST_DONE) finish(*r, PSX_REQUEST_CANCELLED);
         const size_t before = active.size();
-        active.erase(std::remove_if(active.begin(), active.end(),
-                                    [](const std::shared_ptr<ServerRequest> &r) {
-                                        return r->state >= PSX_REQUEST_DONE && !r->in_flight;
-                                    }),
-                     active.end());
+        {
+            // the worker is the only writer of `active`; other threads read
+            // it under the mutex (resident_bytes)
+            std::lock_guard<std::mutex> lock(mutex);
+            active.erase(std::remove_if(active.begin(), active.end(),
+                                        [](const std::shared_ptr<ServerRequest> &r) {
+                                            return r->state >= PSX_REQUEST_DONE && !r->in_flight;
+                                        }),
+                         active.end());
+        }
         return progress || active.size() != before;
     }
 
@@ -2056,14 +2104,23 @@ struct Server {
         for (auto &rp : active) {
             ServerRequest &r = *rp;
             if (r.admitted || r.state != PSX_REQUEST_QUEUED) continue;
-            size_t in_use;
+            size_t in_use, films_in_use;
             {
                 std::lock_guard<std::mutex> lock(mutex);
-                in_use = fixed_bytes + film_bytes;
+                films_in_use = film_bytes;
+                in_use = fixed_bytes + film_bytes + resident_bytes();
             }
             // admission in submission order: a request that does not fit
-            // holds back the ones after it
-            if (in_use + r.film_bytes > options.hbm_budget) break;
+            // holds back the ones after it; one that cannot fit even with no
+            // other film resident fails
+            if (in_use + r.film_bytes > options.hbm_budget) {
+                if (films_in_use == 0u) {
+                    finish(r, PSX_REQUEST_FAILED);
+                    progress = true;
+                    continue;
+                }
+                break;
+            }
             const size_t pixels = static_cast<size_t>(r.film.window_w) * r.film.window_h;
             // stream-ordered allocation and clearing; the request's first
             // batch waits on film_ready, the worker never blocks here
@@ -2085,7 +2142,7 @@ struct Server {
             r.state = PSX_REQUEST_RUNNING;
             r.admit_ms = now_ms();
             film_bytes += r.film_bytes;
-            stats.hbm_in_use = fixed_bytes + film_bytes;
+            stats.hbm_in_use = fixed_bytes + film_bytes + resident_bytes();
             stats.peak_hbm = std::max(stats.peak_hbm, stats.hbm_in_use);
             progress = true;
         }
@@ -2269,6 +2326,7 @@ struct Server {
                     // nothing left that can run (requests that never fit)
                     for (auto &r : active)
                         if (r->state < PSX_REQUEST_DONE) finish(*r, PSX_REQUEST_CANCELLED);
+                    std::lock_guard<std::mutex> lock(mutex);
                     active.clear();
                 }
                 if (!progress) {
@@ -2350,6 +2408,8 @@ int psx_backend_create(int device, const psx_backend_options *options, psx_backe
         if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return static_cast<int>(PSX_ERROR_NO_DEVICE);
         if (device < 0 || device >= n) return static_cast<int>(PSX_ERROR_INVALID_ARGUMENT);
         hip_check(hipSetDevice(device));
+        // address of the allocation-free query scratch on this device
+        hip_check(setup_noalloc_scratch());
         auto *b = new psx_backend();
         b->device = device;
         b->options.next_event_estimation = 1u;
@@ -2648,6 +2708,20 @@ int psx_scene_intersect(psx_scene *handle, const psx_ray_query *queries, uint32_
     });
 }
 
+int psx_scene_intersect_device_noalloc(psx_scene *handle, const psx_ray_query *queries, uint32_t count,
+                                       psx_ray_hit *hits) {
+    Scene *s = reinterpret_cast<Scene *>(handle);
+    if (!s || (count && (!queries || !hits)) || count > PSX_INTERSECT_NOALLOC_MAX_QUERIES)
+        return PSX_ERROR_INVALID_ARGUMENT;
+    if (count == 0u) return PSX_OK;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->backend->device));
+        hip_check(launch_intersect_noalloc(s->view, queries, count, hits, s->query_stream));
+        hip_check(hipStreamSynchronize(s->query_stream));
+        return static_cast<int>(PSX_OK);
+    });
+}
+
 uint32_t psx_probe_input_width(uint32_t function) {
     static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u};
     return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
@@ -2904,6 +2978,6 @@ int psx_server_stats(psx_server *server, psx_serving_stats *out) {
     if (!server || !out) return PSX_ERROR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lock(server->mutex);
     *out = server->stats;
-    out->hbm_in_use = server->fixed_bytes + server->film_bytes;
+    out->hbm_in_use = server->fixed_bytes + server->film_bytes + server->resident_bytes();
     return PSX_OK;
 }

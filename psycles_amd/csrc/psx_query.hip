@@ -8,6 +8,8 @@
 //
 // probe_kernel evaluates one device function per input record, so tests can
 // hold the device code itself against the reference's function fixtures.
+#include <mutex>
+
 #include "../../include/psx.h"
 #include "psx_launch.h"
 #include "psx_path.h"
@@ -177,6 +179,23 @@ __global__ void probe_kernel(uint32_t function, const float *inputs, uint32_t in
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Allocation-free queries: the traversal scratch (the work cursor, padded to
+// 256 bytes, then one spill column of kTraversalStackDepth words per lane for
+// kNoAllocLanes lanes) lives in this file-scope device array, 16 MiB + 256 B
+// on each device the library runs on.  Host code never passes the array's
+// name to a kernel: setup_noalloc_scratch resolves its address on the current
+// device once and keeps it per device.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoAllocLanes = 65536u;
+constexpr uint32_t kNoAllocCursorWords = 64u;
+constexpr size_t kNoAllocWords = kNoAllocCursorWords + static_cast<size_t>(kNoAllocLanes) * kTraversalStackDepth;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mutex;
+static uint32_t *g_noalloc_scratch[kMaxDevices] = {};  // the array's address per device
+
+// ---------------------------------------------------------------------------
 // launchers called by the host library
 // ---------------------------------------------------------------------------
 uint32_t query_resident_lanes() {
@@ -198,6 +217,42 @@ hipError_t launch_intersect(const SceneView &s, const psx_ray_query *queries, ui
     hipLaunchKernelGGL(intersect_kernel, dim3(lanes / kQueryBlock), dim3(kQueryBlock), 0, stream, s, queries, count,
                        hits, cursor, spill);
     return hipGetLastError();
+}
+
+// Resolves the scratch array's address on the current device (once per
+// device); called by psx_backend_create with its device current.
+hipError_t setup_noalloc_scratch() {
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc_scratch[device]) return hipSuccess;
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_noalloc_scratch[device] = static_cast<uint32_t *>(ptr);
+    return hipSuccess;
+}
+
+uint32_t noalloc_max_lanes() { return kNoAllocLanes; }
+
+// The queries on the current device with the static scratch; nothing is
+// allocated.  Fails with hipErrorNotInitialized before setup on this device.
+hipError_t launch_intersect_noalloc(const SceneView &s, const psx_ray_query *queries, uint32_t count,
+                                    psx_ray_hit *hits, hipStream_t stream) {
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    uint32_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        if (device >= 0 && device < kMaxDevices) base = g_noalloc_scratch[device];
+    }
+    if (!base) return hipErrorNotInitialized;
+    const uint32_t resident = query_resident_lanes();
+    const uint32_t lanes = query_lanes(count, resident < kNoAllocLanes ? resident : kNoAllocLanes);
+    return launch_intersect(s, queries, count, hits, base, base + kNoAllocCursorWords, lanes, stream);
 }
 
 hipError_t launch_probe(uint32_t function, const float *inputs, uint32_t in_width, uint32_t count, float *outputs,
