@@ -229,6 +229,7 @@ struct SceneView {
     const DistEntry *distribution;
     const float4 *material_shadow;    // per material: transparent extinction for shadow rays
     const float4 *material_emission;  // per material: constant emission
+    const uint32_t *instance_opaque;  // per instance: 1 when every material it can use stops shadow rays
     const Lamp *lamps;
     const float4 *lamp_sphere;        // per lamp: centre, radius (forward-hit culling)
     const uint32_t *lamp_bits;

@@ -49,6 +49,7 @@ hipError_t launch_sampler(const KernelParams &k, const float4 *sobol, uint32_t s
                           uint32_t dim_count, uint32_t *out, hipStream_t stream);
 hipError_t launch_material_tables(const SceneView &s, float4 *shadow, float4 *emission, uint32_t count,
                                   hipStream_t stream);
+hipError_t launch_instance_opacity(const SceneView &s, uint32_t *flags, uint32_t count, hipStream_t stream);
 // psx_query.hip
 uint32_t query_resident_lanes();
 uint32_t query_lanes(uint32_t count, uint32_t max_lanes);
@@ -477,15 +478,17 @@ struct EmitterTables {
 // parameter-only edit builds a new version and leaves the old one intact for
 // work still using it; the last holder frees it.
 struct ParamTables {
-    DeviceBuffer params, materials_buf, material_shadow, material_emission;
+    DeviceBuffer params, materials_buf, material_shadow, material_emission, instance_opaque;
     std::shared_ptr<EmitterTables> light;
     // the version's own tables; `light` may be shared with other versions
     size_t bytes() const {
-        return params.bytes() + materials_buf.bytes() + material_shadow.bytes() + material_emission.bytes();
+        return params.bytes() + materials_buf.bytes() + material_shadow.bytes() + material_emission.bytes() +
+               instance_opaque.bytes();
     }
     hipEvent_t ready = nullptr;  // recorded after the last upload and table kernel
     explicit ParamTables(hipStream_t stream) {
-        for (DeviceBuffer *b : {&params, &materials_buf, &material_shadow, &material_emission}) b->set_stream(stream);
+        for (DeviceBuffer *b : {&params, &materials_buf, &material_shadow, &material_emission, &instance_opaque})
+            b->set_stream(stream);
         hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
     }
     ~ParamTables() {
@@ -641,6 +644,12 @@ void Scene::refresh_material_tables(ParamTables &t) {
     view.material_emission = t.material_emission.as<float4>();
     hip_check(launch_material_tables(view, t.material_shadow.as<float4>(), t.material_emission.as<float4>(), n,
                                      table_stream));
+    // the shadow-ray early end of each instance follows its materials
+    const uint32_t ni = static_cast<uint32_t>(instances.size());
+    if (t.instance_opaque.bytes() < std::max<size_t>(ni * sizeof(uint32_t), 16u))
+        t.instance_opaque.allocate(ni * sizeof(uint32_t));
+    view.instance_opaque = t.instance_opaque.as<uint32_t>();
+    hip_check(launch_instance_opacity(view, t.instance_opaque.as<uint32_t>(), ni, table_stream));
     // work of this version waits on `ready` (hipStreamWaitEvent), callers
     // that need the tables on the host side of the call synchronise on it
     hip_check(hipEventRecord(t.ready, table_stream));

@@ -246,6 +246,25 @@ __global__ void material_table_kernel(SceneView s, float4 *shadow, float4 *emiss
 }
 
 
+// per instance: 1 when every material its triangles can take (the mesh's
+// slot table and the instance's overrides; no slot at all means no material)
+// has no transparent extinction, so any hit on it zeroes a shadow ray
+__global__ void instance_opacity_kernel(SceneView s, uint32_t *flags, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const Instance &inst = s.instances[i];
+    const Mesh &mesh = s.meshes[inst.mesh];
+    auto opaque = [&](uint32_t material) {
+        if (material == kNone) return true;
+        const float4 e = s.material_shadow[material];
+        return !(fmaxf(e.x, fmaxf(e.y, e.z)) > 0.0f);
+    };
+    bool all = true;
+    for (uint32_t o = 0; o < inst.override_count; ++o) all = all && opaque(s.overrides[inst.override_offset + o]);
+    for (uint32_t k = 0; k < mesh.slot_count; ++k) all = all && opaque(s.slots[mesh.slot_offset + k]);
+    flags[i] = all ? 1u : 0u;
+}
+
 uint32_t tile_grid(const KernelParams &k, const void *kernel) {
     const uint32_t tiles = ((k.window_width + 7u) >> 3) * ((k.window_height + 7u) >> 3);
     const uint32_t blocks = (tiles + (kBlock / 64u) - 1u) / (kBlock / 64u);
@@ -330,6 +349,13 @@ hipError_t launch_material_tables(const SceneView &s, float4 *shadow, float4 *em
     if (count == 0u) return hipSuccess;
     hipLaunchKernelGGL(material_table_kernel, dim3((count + 63u) / 64u), dim3(64), 0, stream, s, shadow, emission,
                        count);
+    return hipGetLastError();
+}
+
+// after launch_material_tables on the same stream (reads s.material_shadow)
+hipError_t launch_instance_opacity(const SceneView &s, uint32_t *flags, uint32_t count, hipStream_t stream) {
+    if (count == 0u) return hipSuccess;
+    hipLaunchKernelGGL(instance_opacity_kernel, dim3((count + 63u) / 64u), dim3(64), 0, stream, s, flags, count);
     return hipGetLastError();
 }
 

@@ -408,15 +408,6 @@ __device__ __forceinline__ uint32_t hit_material(const SceneView &s, uint32_t in
     return material_of(s, inst, mesh, prim);
 }
 
-// A surface whose transparent extinction carries no light (or that has no
-// material) stops a shadow ray (src/luisa/path_tracer_geometry.cpp:128-160).
-__device__ __forceinline__ bool shadow_opaque(const SceneView &s, uint32_t inst_index, uint32_t prim) {
-    const uint32_t material = hit_material(s, inst_index, prim);
-    if (material == kNone) return true;
-    const float4 e = s.material_shadow[material];
-    return !(fmaxf(e.x, fmaxf(e.y, e.z)) > 0.0f);
-}
-
 // Closest hit along [tmin, tmax] under the Cycles order: smallest t, ties to
 // the greatest (object, primitive).  Box culling is conservative (boxes are
 // padded at build time and the far distance is widened), so traversal order
@@ -434,11 +425,12 @@ struct Traversal {
     uint32_t current;
     uint32_t inst_index, inst_prim_offset;
     bool in_blas;
-    // Shadow queries (`anyhit`) end at the first accepted candidate whose
-    // material is opaque to shadow rays (`opaque`): the transparent shadow
+    // Shadow queries (`anyhit`) end at the first accepted candidate on an
+    // instance whose every material is opaque to shadow rays (`stop` while
+    // inside such an instance; `opaque` once ended): the transparent shadow
     // loop's transmittance is then exactly zero whatever else lies on the
     // ray, so the early end changes no result bit.
-    bool anyhit, opaque;
+    bool anyhit, stop, opaque;
 };
 
 __device__ __forceinline__ f3 clamped_dir(f3 d) {
@@ -460,7 +452,7 @@ __device__ __forceinline__ void traversal_begin(const SceneView &s, Traversal &t
     t.best.u = t.best.v = 0.0f;
     t.lower = t.ray.tmin > 0.0f ? t.ray.tmin * 0.99999f : t.ray.tmin - 1.0e-30f;
     t.inst_index = t.inst_prim_offset = 0u;
-    t.opaque = false;
+    t.stop = t.opaque = false;
     traversal_world_space(t);
     st.sp = 0;
     st.bottom = 0;
@@ -489,6 +481,7 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
         t.inst_index = index;
         t.inst_prim_offset = inst.prim_offset;
         t.in_blas = true;
+        t.stop = t.anyhit && s.instance_opaque[index] != 0u;
         if (!inst.applied) {
             t.o = affine_point(inst.w2o, t.ray.o);
             t.d = clamped_dir(affine_dir(inst.w2o, t.ray.d));
@@ -517,7 +510,7 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
             const bool excluded = (t.inst_index == t.q.src_obj && cprim == t.q.src_prim) ||
                                   (t.inst_index == t.q.light_obj && cprim == t.q.light_prim);
             if (excluded) return true;
-            if (t.anyhit && shadow_opaque(s, t.inst_index, prim)) {
+            if (t.stop) {
                 // any opaque candidate in [tmin, tmax] zeroes the shadow
                 t.opaque = true;
                 t.best.valid = true;
