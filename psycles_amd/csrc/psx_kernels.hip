@@ -38,11 +38,10 @@ __device__ PathResult integrate(const SceneView &s, const LampView &lv, const Ke
     while (cont || shadow) {
         const Ray ray = shadow ? sr.ray : p.ray;
         const TraceQuery q = shadow ? sr.q : path_query(p);
-        bool opaque = false;
-        const Hit hit = closest_hit<COUNT>(s, ray, q, st, nodes, tris, shadow, &opaque);
+        const Hit hit = closest_hit<COUNT>(s, ray, q, st, nodes, tris);
         ++rays;
         if (shadow) {
-            if (!shadow_hit(s, k.transparent_max, hit, sr.ray, tdepth, transmittance, opaque)) continue;
+            if (!shadow_hit(s, k.transparent_max, hit, sr.ray, tdepth, transmittance)) continue;
             apply_shadow<false>(k, p, sr.L, sr.depth, transmittance, sr.dw, sr.gw);
             shadow = false;
             continue;

@@ -425,12 +425,13 @@ struct Traversal {
     uint32_t current;
     uint32_t inst_index, inst_prim_offset;
     bool in_blas;
-    // Shadow queries (`anyhit`) end at the first accepted candidate on an
-    // instance whose every material is opaque to shadow rays (`stop` while
-    // inside such an instance; `opaque` once ended): the transparent shadow
-    // loop's transmittance is then exactly zero whatever else lies on the
-    // ray, so the early end changes no result bit.
-    bool anyhit, stop, opaque;
+    // Any-hit traversal (traversal_step<COUNT, true>, for shadow queries)
+    // ends at the first accepted candidate on an instance whose every
+    // material is opaque to shadow rays (`stop` while inside such an
+    // instance; `opaque` once ended): the transparent shadow loop's
+    // transmittance is then exactly zero whatever else lies on the ray, so
+    // the early end changes no result bit.
+    bool stop, opaque;
 };
 
 __device__ __forceinline__ f3 clamped_dir(f3 d) {
@@ -461,7 +462,7 @@ __device__ __forceinline__ void traversal_begin(const SceneView &s, Traversal &t
 
 // One traversal iteration: an instance entry, a leaf or an inner node.
 // Returns false once the query is complete (t.best holds the result).
-template <bool COUNT>
+template <bool COUNT, bool ANYHIT = false>
 __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t, Stack &st, uint32_t &node_count,
                                                uint32_t &tri_count) {
     if (t.current == kEmptySlot) {
@@ -481,7 +482,7 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
         t.inst_index = index;
         t.inst_prim_offset = inst.prim_offset;
         t.in_blas = true;
-        t.stop = t.anyhit && s.instance_opaque[index] != 0u;
+        if (ANYHIT) t.stop = s.instance_opaque[index] != 0u;
         if (!inst.applied) {
             t.o = affine_point(inst.w2o, t.ray.o);
             t.d = clamped_dir(affine_dir(inst.w2o, t.ray.d));
@@ -510,7 +511,7 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
             const bool excluded = (t.inst_index == t.q.src_obj && cprim == t.q.src_prim) ||
                                   (t.inst_index == t.q.light_obj && cprim == t.q.light_prim);
             if (excluded) return true;
-            if (t.stop) {
+            if (ANYHIT && t.stop) {
                 // any opaque candidate in [tmin, tmax] zeroes the shadow
                 t.opaque = true;
                 t.best.valid = true;
@@ -595,16 +596,14 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
     return true;
 }
 
-template <bool COUNT>
+template <bool COUNT, bool ANYHIT = false>
 __device__ Hit closest_hit(const SceneView &s, const Ray &ray, const TraceQuery &q, Stack &st,
-                           uint32_t &node_count, uint32_t &tri_count, bool anyhit = false,
-                           bool *opaque = nullptr) {
+                           uint32_t &node_count, uint32_t &tri_count, bool *opaque = nullptr) {
     Traversal t;
     t.ray = ray;
     t.q = q;
-    t.anyhit = anyhit;
     traversal_begin(s, t, st);
-    while (traversal_step<COUNT>(s, t, st, node_count, tri_count)) {
+    while (traversal_step<COUNT, ANYHIT>(s, t, st, node_count, tri_count)) {
     }
     if (opaque) *opaque = t.opaque;
     return t.best;

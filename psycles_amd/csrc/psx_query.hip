@@ -2,7 +2,8 @@
 //
 // intersect_kernel answers psx_ray_query records against a compiled scene
 // with the render path's own traversal: closest hits under the Cycles order
-// and the transparent shadow loop (ended by the first opaque surface).  Waves
+// and the transparent shadow loop, whose any-hit traversal ends at the first
+// candidate on an instance opaque to shadow rays.  Waves
 // take 64 queries per atomic from a cursor; each lane keeps its stack in an
 // LDS short stack with an HBM spill column, like the trace kernels.
 //
@@ -44,7 +45,7 @@ __device__ void run_query(const SceneView &s, const psx_ray_query &q, Stack &st,
         uint32_t tdepth = q.transparent_depth, passes = 0u;
         for (;;) {
             bool opaque = false;
-            const Hit b = closest_hit<false>(s, r, tq, st, nodes, tris, true, &opaque);
+            const Hit b = closest_hit<false, true>(s, r, tq, st, nodes, tris, &opaque);
             ++passes;
             if (shadow_hit(s, q.transparent_max, b, r, tdepth, transmittance, opaque)) break;
         }

@@ -4,8 +4,7 @@
 // advances them in lock-step iterations of two kernels:
 //
 //   trace  persistent waves run closest-hit traversal for continuation rays,
-//          or the whole transparent shadow loop for NEE shadow rays (ended
-//          by the first opaque surface met in any order); lanes
+//          or the whole transparent shadow loop for NEE shadow rays; lanes
 //          whose query finished are refilled from the queue (one atomic per
 //          refill) while the rest of the wave keeps traversing
 //   shade  consumes the hit of each live path: pending shadow result, then one
@@ -250,7 +249,6 @@ __global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, K
                 if (entry != kSkip) {
                     path = entry >> 1;
                     shadow = (entry & 1u) != 0u;
-                    t.anyhit = shadow;
                     if (shadow) {
                         const float4 o = w.shadow_o[path], d = w.shadow_d[path];
                         const uint4 sq = w.shadow_q[path];
@@ -295,7 +293,7 @@ __global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, K
                     w.hit[path] = make_uint4(h.valid ? h.inst : kNone, h.prim, h.cprim, __float_as_uint(h.t));
                     w.hit_uv[path] = make_float2(h.u, h.v);
                     active = false;
-                } else if (shadow_hit(s, k.transparent_max, t.best, t.ray, tdepth, transmittance, t.opaque)) {
+                } else if (shadow_hit(s, k.transparent_max, t.best, t.ray, tdepth, transmittance)) {
                     w.shadow_t[path] = make_float4(transmittance.x, transmittance.y, transmittance.z, 0.0f);
                     active = false;
                 } else {
