@@ -1451,7 +1451,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         }
         const uint32_t base = static_cast<uint32_t>(nodes.size());
         const uint32_t tri_base = static_cast<uint32_t>(tris.size());
-        Bvh4 b = build_bvh4(items, base, 4u);
+        Bvh4 b = build_bvh4(items, base, 2u);
         for (auto &n : b.nodes)
             for (int i = 0; i < 4; ++i)
                 if (n.count[i] != kEmptySlot && n.count[i] > 0u) n.child[i] += tri_base;
