@@ -1595,6 +1595,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     // upload
     Scene &s = *scene;
     s.nodes.upload(quantize_nodes(nodes, [&](uint32_t i) { return has_tlas && (i == 0u || i >= tlas_begin); }));
+    tris.push_back(LeafTri{});  // padding: a leaf step fetches 64 bytes from its 48-byte triangle
     s.tris.upload(tris);
     s.instances_buf.upload(instances);
     s.meshes_buf.upload(scene->meshes);

@@ -492,7 +492,15 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
         t.current = inst.blas_root | kKindBlas;
         return true;
     }
-    if (kind == kKindLeaf) {
+    // One 64-byte fetch serves both kinds of step: a node record, or a leaf
+    // triangle (48 bytes, the next 16 read and ignored; the triangle array
+    // carries one record of padding).  Lanes of a wave that sit on nodes and
+    // on triangles then wait on one memory round trip per step, not two.
+    const bool is_leaf = kind == kKindLeaf;
+    const uint4 *rec = is_leaf ? reinterpret_cast<const uint4 *>(s.tris + (index >> 3))
+                               : reinterpret_cast<const uint4 *>(s.nodes + index);
+    const uint4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3];
+    if (is_leaf) {
         // one triangle per step: a wave whose lanes sit on leaves of
         // different sizes then runs one intersection test per step instead
         // of the longest leaf's loop; the rest of the leaf is the next step
@@ -500,13 +508,14 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
         const uint32_t n = (index & 7u) + 1u;
         if (n > 1u) t.current = (((first + 1u) << 3) | (n - 2u)) | kKindLeaf;
         {
-            const LeafTri &tri = s.tris[first];
             if (COUNT) tri_count++;
             float th, bu, bv;
-            const bool ok = pluecker(t.o, t.d, t.ray.tmin, t.ray.tmax, V(tri.p0x, tri.p0y, tri.p0z),
-                                     V(tri.p1x, tri.p1y, tri.p1z), V(tri.p2x, tri.p2y, tri.p2z), th, bu, bv);
+            const bool ok = pluecker(t.o, t.d, t.ray.tmin, t.ray.tmax,
+                                     V(__uint_as_float(r0.x), __uint_as_float(r0.y), __uint_as_float(r0.z)),
+                                     V(__uint_as_float(r1.x), __uint_as_float(r1.y), __uint_as_float(r1.z)),
+                                     V(__uint_as_float(r2.x), __uint_as_float(r2.y), __uint_as_float(r2.z)), th, bu, bv);
             if (!ok) return true;
-            const uint32_t prim = __float_as_uint(tri.prim_bits);
+            const uint32_t prim = r0.w;
             const uint32_t cprim = t.inst_prim_offset + prim;
             const bool excluded = (t.inst_index == t.q.src_obj && cprim == t.q.src_prim) ||
                                   (t.inst_index == t.q.light_obj && cprim == t.q.light_prim);
@@ -540,8 +549,7 @@ __device__ __forceinline__ bool traversal_step(const SceneView &s, Traversal &t,
     }
     // inner node: test the four child boxes, continue with the nearest hit
     // child and push the others far-to-near
-    const uint4 *rec = reinterpret_cast<const uint4 *>(s.nodes + index);
-    const uint4 head = rec[0], qxy = rec[1], qz = rec[2], refs = rec[3];
+    const uint4 head = r0, qxy = r1, qz = r2, refs = r3;
     if (COUNT) node_count++;
     const float upper = (t.best.valid ? t.best.t : t.ray.tmax) * 1.00001f + 1.0e-30f;
     // slab distance of origin + q * 2^e is fma(q, 2^e * inv, origin * inv - o * inv)
