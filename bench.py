@@ -53,19 +53,19 @@ def bytes_per_ray(nodes: float, tris: float) -> float:
 # leaves <= 4) over one full-frame sample of that workload.  `traffic` is the
 # HBM bytes the trace kernel moves per ray: rocprofv3 FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0 (scale factors from a 1 GiB copy) divided by the rays of
-# the profiled dispatch (profiles/wf_trace_kernel_pmc.md), or None where no
+# the profiled dispatch (profiles/wf_kernels_pmc.md), or None where no
 # counter profile was taken.
 WORKLOADS = {
     "instanced": {
         "scene": "instanced_icospheres", "settings": "instanced_settings", "spp": 64,
-        "nodes": 46.83, "tris": 8.59, "traffic": 1120.0,
+        "nodes": 46.83, "tris": 8.59, "traffic": 1111.3,
         "describe": "instanced icospheres: 1,003,520-tri displaced icosphere x 64 instances, "
                     "mix(diffuse, transparent) materials, ground quad, constant background; "
                     "1920x1080, {spp} spp, max 8 bounces, seed 0",
     },
     "many_lights": {
         "scene": "many_lights", "settings": "many_light_settings", "spp": 32,
-        "nodes": 94.81, "tris": 15.06, "traffic": 1848.8,
+        "nodes": 94.81, "tris": 15.06, "traffic": 1847.9,
         "describe": "many lights: the instanced icospheres plus 10,000 emissive add(emission, diffuse) "
                     "triangles, 256 point, 64 spot and 1 distant lamp, light-tree NEE; "
                     "1920x1080, {spp} spp, max 6 bounces, seed 0",
