@@ -79,9 +79,17 @@ def describe(label: str, census: dict) -> str:
 
 # Flips observed on MI355X per (test case, sample), the bound each event test
 # holds: the device and the oracle are deterministic, so a count above the
-# observed one is a new divergence.  A case without an entry is held to
-# 0.1 % of its events.
-FLIP_BUDGET: dict = {}
+# observed one is a new divergence.  Every case below was observed with no
+# flip at all (0 of 3,572 to 10,182 events per sample), so each is held to
+# exactly zero; a case without an entry is held to 0.1 % of its events.
+FLIP_BUDGET: dict = {
+    ("path events cornell", 0): 0, ("path events cornell", 7): 0,
+    ("path events instanced", 0): 0, ("path events instanced", 7): 0,
+    ("cornell lamps", 0): 0, ("cornell lamps", 5): 0,
+    ("light tree cornell_lamps", 0): 0, ("light tree cornell_lamps", 3): 0,
+    ("light tree many_lights", 0): 0, ("light tree many_lights", 3): 0,
+    ("cornell area lamp", 0): 0, ("cornell area lamp", 5): 0,
+}
 
 
 def check(label: str, session, oracle, settings, samples, total, log=print) -> list:

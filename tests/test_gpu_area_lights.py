@@ -9,7 +9,7 @@
   narrow-spread rectangle and an ellipse: Combined / Normal / Albedo within
   1e-4 relative of the oracle on >= 99.9 % of pixels, discrete path events
   (flags, RNG offset, hit identity, light and closure choice, termination
-  bits) identical on >= 99.9 % of events, in MIS, NEE-only and forward-only
+  bits) identical on every event, in MIS, NEE-only and forward-only
   modes, flat CDF and light tree.
 * Orthographic primary hits identical to the oracle's.
 """

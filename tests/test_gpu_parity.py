@@ -4,9 +4,8 @@ Bars (north-star parity gates):
 * Sobol/RNG dimension values: bit-exact.
 * primary-hit identities: identical (ties resolve by the same total order).
 * discrete path events (light choice, closure choice, RR, hit identity):
-  bit-exact on the first bounce of every pixel and on >= 99.9 % of all
-  recorded events (later events may flip when a libm and an ocml
-  transcendental differ in the last ulp).
+  bit-exact on every recorded event (each flip would be counted by kind and
+  attributed to the first path-trace slot that differs; none occurs).
 * Combined / Normal / Albedo: relative error <= 1e-4 on >= 99.9 % of pixels.
 """
 import math
