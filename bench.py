@@ -11,7 +11,13 @@ megakernel (one lane runs whole paths) is timed on one frame beside it.
 
 `--workload many_lights` times the many-light scene instead (the instanced
 scene plus 10,000 emissive triangles and 321 lamps, light-tree NEE,
-1920x1080 at 32 spp, max 6 bounces); the default line is the instanced one.
+1920x1080 at 32 spp, max 6 bounces), `--workload cornell` the Cornell box
+(640x400 at 16 spp, max 4 bounces); the default line is the instanced one.
+Before the timed region the session's wavefront launch shape is autotuned
+(probe dispatches of 4 samples; the film does not depend on the shape).
+Trace launches are timed with HIP events, which this runtime cannot time
+inside a graph, so the timed dispatches launch from the host;
+`--untimed-trace` replays each dispatch as a graph instead (no roofline).
 
 Prints one JSON line with samples/s (value), Mrays/s, the HBM roofline of the
 dominant kernel (the wavefront trace kernel, timed with HIP events around
@@ -56,6 +62,12 @@ def bytes_per_ray(nodes: float, tris: float) -> float:
 # the profiled dispatch (profiles/wf_kernels_pmc.md), or None where no
 # counter profile was taken.
 WORKLOADS = {
+    "cornell": {
+        "scene": "cornell_box", "settings": "cornell_settings", "spp": 16,
+        "nodes": 11.17, "tris": 7.26, "traffic": None,
+        "describe": "Cornell box: 34 triangles, emissive ceiling quad, diffuse walls and boxes; "
+                    "640x400, {spp} spp, max 4 bounces, seed 0",
+    },
     "instanced": {
         "scene": "instanced_icospheres", "settings": "instanced_settings", "spp": 64,
         "nodes": 46.83, "tris": 8.59, "traffic": 1111.3,
@@ -281,6 +293,17 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip the megakernel frame")
     ap.add_argument("--serving-requests", type=int, default=60,
                     help="requests of the serving leg on the default line (0 skips it)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default wavefront launch shape instead of timing candidates first")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel of a dispatch from the host instead of replaying a graph")
+    ap.add_argument("--tuning", default=None,
+                    help="explicit wavefront launch shape instead of the autotuner, e.g. tile_width=16,sort_shading=1")
+    ap.add_argument("--generated-programs", action="store_true",
+                    help="shade with the scene's hipRTC-generated surface programs instead of the interpreter")
+    ap.add_argument("--untimed-trace", action="store_true",
+                    help="do not time each trace launch with HIP events (no roofline; dispatches may then "
+                         "replay as graphs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -304,7 +327,8 @@ def main() -> int:
     settings = getattr(scenes, w["settings"])()
     b_ray = bytes_per_ray(w["nodes"], w["tris"])
     backend = RendererBackend(0, integrator=abi.INTEGRATOR_WAVEFRONT,
-                              max_samples_per_dispatch=SAMPLES_PER_DISPATCH)
+                              max_samples_per_dispatch=SAMPLES_PER_DISPATCH, graphs=not args.no_graphs,
+                              time_trace=not args.untimed_trace, generated_programs=args.generated_programs)
     t0 = time.perf_counter()
     comp = backend.compile_scene(snap)
     if not comp.ok():
@@ -316,6 +340,20 @@ def main() -> int:
     pixels = settings.width * settings.height
 
     session = backend.create_session(comp.scene, settings)
+    tuning = None
+    if args.tuning:
+        # an explicit launch shape, e.g. "tile_width=16,sort_shading=1"
+        session.set_tuning({k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))})
+    elif not args.no_autotune:
+        # session set-up, like the compile: time candidate launch shapes on a
+        # 4-sample probe and keep the fastest (the film does not depend on it)
+        t0 = time.perf_counter()
+        rep = session.autotune(4)
+        tuning = {"chosen": rep.chosen.as_dict(), "probe_ms_default": rep.default_ms,
+                  "probe_ms_chosen": rep.chosen_ms, "candidates": rep.candidates,
+                  "probe_samples": rep.probe_samples, "seconds": time.perf_counter() - t0}
+        print(f"bench: autotune chose {tuning['chosen']} ({rep.chosen_ms:.2f} ms vs default "
+              f"{rep.default_ms:.2f} ms per probe) in {tuning['seconds']:.2f} s", file=sys.stderr, flush=True)
 
     def step(sess):
         # a fresh frame: reset zeroes the resident film, then the whole
@@ -331,11 +369,13 @@ def main() -> int:
     rays = 0
     trace_ms = 0.0
     trace_launches = 0
+    graph_launches = 0
     for i in range(args.steps):
         st = step(session)
         rays += st.rays
         trace_ms += st.trace_ms
         trace_launches += st.trace_launches
+        graph_launches += st.graph_launches
         print(f"bench: step {i} {st.kernel_ms:.1f} ms (trace {st.trace_ms:.1f} ms), {st.rays / 1e6:.1f} Mrays",
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
@@ -344,7 +384,10 @@ def main() -> int:
     mrays = rays / wall / 1e6
     trace_launch_ms = trace_ms / max(trace_launches, 1)
     rays_per_launch = rays / max(trace_launches, 1)
-    achieved = rays_per_launch * b_ray / (trace_launch_ms * 1e-3) / 1e9
+    timed = trace_launches > 0 and trace_launch_ms > 0.0
+    achieved = rays_per_launch * b_ray / (trace_launch_ms * 1e-3) / 1e9 if timed else 0.0
+    traffic_gbs = (rays_per_launch * w["traffic"] / (trace_launch_ms * 1e-3) / 1e9
+                   if timed and w["traffic"] else None)
     result = {
         "metric": METRIC,
         "value": value,
@@ -362,6 +405,9 @@ def main() -> int:
             "workload": w["describe"].format(spp=spp),
             "integrator": "wavefront (batched)",
             "samples_per_dispatch": SAMPLES_PER_DISPATCH,
+            "launch": "graph replay per dispatch" if graph_launches else "host launches",
+            "shading": "generated surface programs (hipRTC)" if args.generated_programs else "closure interpreter",
+            "tuning": tuning if tuning else session.tuning.as_dict(),
         },
         "mrays_per_s": mrays,
         "rays_per_sample": rays / samples,
@@ -378,16 +424,16 @@ def main() -> int:
             # bandwidth the kernel really draws
             "basis": "achieved: algorithmic B_ray = 48 + 32 N_node + 36 N_tri bytes per ray (binary SAH BVH); "
                      "traffic: measured HBM bytes per launch",
-            "traffic_gbs": (rays_per_launch * w["traffic"] / (trace_launch_ms * 1e-3) / 1e9
-                            if w["traffic"] else None),
-            "traffic_frac": (rays_per_launch * w["traffic"] / (trace_launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
-                             if w["traffic"] else None),
+            "traffic_gbs": traffic_gbs,
+            "traffic_frac": traffic_gbs / HBM_PEAK_GBS if traffic_gbs else None,
             "kernel": "wf_trace_kernel",
             "launch_ms": trace_launch_ms,
             "launches": trace_launches,
             "bytes_per_ray": b_ray,
         },
     }
+    if not timed:
+        result["roofline"] = None
     if not args.no_plain:
         # the plain per-request path on one frame of the same workload
         plain = RendererBackend(0, integrator=abi.INTEGRATOR_MEGAKERNEL, max_samples_per_dispatch=2)

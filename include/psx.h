@@ -334,7 +334,24 @@ enum psx_integrator {
 };
 
 enum psx_backend_flags {
-    PSX_BACKEND_COUNT_TRAVERSAL = 1u /* count node visits and triangle tests */
+    PSX_BACKEND_COUNT_TRAVERSAL = 1u, /* count node visits and triangle tests */
+    /* Launch every kernel of a wavefront dispatch from the host.  By default
+     * a session captures its dispatch (counter reset, camera rays, every
+     * bounce iteration's trace and shade, accumulation) into a graph once
+     * and replays it per dispatch; the film is the same either way. */
+    PSX_BACKEND_NO_GRAPHS = 2u,
+    /* Time every wavefront trace launch with HIP events (psx_render_stats
+     * trace_ms and trace_launches).  Events recorded inside a graph cannot be
+     * timed on this runtime, so timed dispatches launch from the host and the
+     * host waits for each dispatch to read its events. */
+    PSX_BACKEND_TIME_TRACE = 4u,
+    /* Shade wavefront dispatches with generated surface programs: one
+     * straight-line closure program per distinct closure-leaf layout of the
+     * scene's materials, compiled with hipRTC on the first dispatch that
+     * needs it (one module per layout set and shade variant, cached by the
+     * backend; parameter-only rebinds reuse it).  Films are bit-identical to
+     * the built-in interpreter's. */
+    PSX_BACKEND_GENERATED_PROGRAMS = 8u
 };
 
 typedef struct psx_backend_options {
@@ -346,10 +363,39 @@ typedef struct psx_backend_options {
 
 const char *psx_version(void);
 int psx_device_count(void);
+/* The most recent HIP failure behind a PSX_ERROR_DEVICE or
+ * PSX_ERROR_OUT_OF_MEMORY on the calling thread ("<HIP error name> at
+ * <source>:<line>"), or "" if there was none.  Valid until the thread's next
+ * call into the library. */
+const char *psx_last_error(void);
 
 int psx_backend_create(int device_ordinal, const psx_backend_options *options,
                        psx_backend **out_backend);
 void psx_backend_destroy(psx_backend *backend);
+
+/* ---- generated surface programs (PSX_BACKEND_GENERATED_PROGRAMS) ---------
+ * The reference JIT-compiles each SurfaceProgram per structure signature
+ * and keeps socket values in parameter buffers (docs/architecture.md:63-110,
+ * include/psycles/compiler/surface_program.h:512-568).  These entry points
+ * expose the generator without a device: the HIP source a set of materials
+ * generates for a shade variant, and a hipRTC build of it. */
+enum psx_program_variant {
+    PSX_PROGRAM_LIGHT_PASSES = 1u, /* shade kernel that sums light passes */
+    PSX_PROGRAM_LIGHTS = 2u        /* shade kernel for a scene with lights */
+};
+/* Writes the NUL-terminated source into `source` (may be NULL), sets
+ * `length` (without the NUL) and `programs` (distinct closure-leaf layouts;
+ * may be NULL).  PSX_ERROR_INVALID_ARGUMENT for an invalid material, a
+ * variant above 3, or a capacity <= length with a non-NULL `source`. */
+int psx_surface_program_source(const psx_material *materials, uint32_t material_count, uint32_t variant,
+                               char *source, size_t capacity, size_t *length, uint32_t *programs);
+/* Compiles that source with hipRTC for `arch` (NULL: "gfx950").  PSX_OK and
+ * the code-object size on success; PSX_ERROR_UNSUPPORTED when it does not
+ * compile.  The compiler log goes to `log` (may be NULL). */
+int psx_surface_program_build(const psx_material *materials, uint32_t material_count, uint32_t variant,
+                              const char *arch, char *log, size_t log_capacity, uint64_t *code_bytes);
+/* Generated modules the backend has compiled so far. */
+uint32_t psx_backend_generated_programs(const psx_backend *backend);
 
 int psx_scene_compile(psx_backend *backend, const psx_scene_desc *desc,
                       psx_diag_fn diagnostics, void *diagnostics_user,
@@ -384,8 +430,11 @@ typedef struct psx_render_stats {
     double kernel_ms;         /* device time of the render kernels          */
     uint64_t node_visits;     /* BVH node records fetched (0 if disabled)   */
     uint64_t triangle_tests;  /* triangle records tested (0 if disabled)    */
-    double trace_ms;          /* wavefront: device time of the trace kernel */
-    uint64_t trace_launches;  /* wavefront: trace kernel launches           */
+    double trace_ms;          /* wavefront with PSX_BACKEND_TIME_TRACE: device
+                                 time of the trace kernel (else 0)          */
+    uint64_t trace_launches;  /* ... and its launches (else 0)              */
+    uint64_t graph_launches;  /* wavefront: dispatches replayed as a graph  */
+    uint64_t graph_builds;    /* wavefront: graphs captured by this call    */
 } psx_render_stats;
 
 /* Statistics of the most recent psx_session_render. */
@@ -396,6 +445,52 @@ int psx_session_stats(const psx_session *session, psx_render_stats *stats);
  * keeps accumulating exactly like psx_session_render. */
 int psx_session_render_device(psx_session *session, uint32_t first,
                               uint32_t count, uint32_t total);
+
+/* ---- wavefront launch shape and its autotuner ----------------------------
+ * The wavefront integrator's launch shape.  Every value in range gives the
+ * same film, bit for bit; only the speed changes.  The closest the reference
+ * has is its fixed dispatch policy (backend_max_pixel_samples_per_dispatch,
+ * src/luisa/path_tracer_backend_policy.h:13-22); these are the MI355X
+ * kernels' own knobs. */
+typedef struct psx_wavefront_tuning {
+    uint32_t tile_width;   /* 8, 16, 32 or 64: a wave's 64 camera paths cover
+                              tile_width x (64 / tile_width) pixels (8)      */
+    uint32_t refill_lanes; /* 1..64: idle lanes of a trace wave that make it
+                              take new rays from the queue (16)              */
+    uint32_t trace_chunk;  /* 16..4096: queue entries a trace wave reserves
+                              per atomic (128)                               */
+    uint32_t queue_chunk;  /* 128 or 256: next-queue slots a shade wave
+                              reserves per atomic (256)                      */
+    uint32_t list_chunk;   /* 64, 128 or 256: shade-list slots a shade wave
+                              reserves per atomic (128)                      */
+    uint32_t sort_shading; /* 0 or 1: each shade block takes its 256 paths
+                              sorted by closure structure of the hit (0)    */
+} psx_wavefront_tuning;
+
+void psx_wavefront_tuning_default(psx_wavefront_tuning *tuning);
+/* PSX_ERROR_INVALID_ARGUMENT for a value out of range. */
+int psx_session_set_tuning(psx_session *session, const psx_wavefront_tuning *tuning);
+int psx_session_get_tuning(const psx_session *session, psx_wavefront_tuning *tuning);
+
+typedef struct psx_tuning_report {
+    psx_wavefront_tuning chosen;
+    float default_ms;        /* probe dispatch time with the default shape  */
+    float chosen_ms;         /* probe dispatch time with the chosen shape   */
+    uint32_t candidates;     /* shapes timed                                */
+    uint32_t probe_samples;  /* samples per probe dispatch                  */
+} psx_tuning_report;
+
+/* Times probe dispatches of `probe_samples` samples (1..64) of the session's
+ * window for candidate shapes, one knob at a time from the default (tile
+ * width, refill lanes, trace chunk, shade chunks, shade sorting), keeps a
+ * candidate only if
+ * it is at least 1 % faster, and sets the winner on the session.  The probes
+ * render into scratch memory: the session's film, sample count and stats are
+ * untouched.  A session whose renders run on the megakernel (integrator
+ * PSX_INTEGRATOR_MEGAKERNEL without light passes) returns
+ * PSX_ERROR_UNSUPPORTED. */
+int psx_session_autotune(psx_session *session, uint32_t probe_samples,
+                         psx_tuning_report *report);
 
 /* Raw accumulated film, row-major over the window: combined float4 (rgb sum,
  * transparency sum), normal float4, albedo float4 and the u32 sample count.

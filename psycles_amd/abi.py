@@ -98,6 +98,11 @@ LIGHT_FLAG_CAST_SHADOW = 16
 INTEGRATOR_MEGAKERNEL = 0
 INTEGRATOR_WAVEFRONT = 1
 BACKEND_COUNT_TRAVERSAL = 1
+BACKEND_NO_GRAPHS = 2
+BACKEND_TIME_TRACE = 4
+BACKEND_GENERATED_PROGRAMS = 8
+PROGRAM_LIGHT_PASSES = 1
+PROGRAM_LIGHTS = 2
 
 TRACE_EVENTS = 8
 TRACE_WORDS = 8
@@ -214,7 +219,21 @@ class RenderStats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("samples", C.c_uint64),
                 ("dispatches", C.c_uint64), ("kernel_ms", C.c_double),
                 ("node_visits", C.c_uint64), ("triangle_tests", C.c_uint64),
-                ("trace_ms", C.c_double), ("trace_launches", C.c_uint64)]
+                ("trace_ms", C.c_double), ("trace_launches", C.c_uint64),
+                ("graph_launches", C.c_uint64), ("graph_builds", C.c_uint64)]
+
+
+class WavefrontTuning(C.Structure):
+    _fields_ = [("tile_width", u32), ("refill_lanes", u32), ("trace_chunk", u32),
+                ("queue_chunk", u32), ("list_chunk", u32), ("sort_shading", u32)]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
+class TuningReport(C.Structure):
+    _fields_ = [("chosen", WavefrontTuning), ("default_ms", f32), ("chosen_ms", f32),
+                ("candidates", u32), ("probe_samples", u32)]
 
 
 class ServerOptions(C.Structure):
@@ -324,7 +343,7 @@ LIBRARY_PATH = PACKAGE_DIR / "libpsx.so"
 
 #: every symbol include/psx.h declares
 EXPORTED_SYMBOLS = [
-    "psx_version", "psx_device_count", "psx_render_settings_default",
+    "psx_version", "psx_device_count", "psx_last_error", "psx_render_settings_default",
     "psx_backend_create", "psx_backend_destroy", "psx_scene_compile",
     "psx_scene_destroy", "psx_scene_update_parameters",
     "psx_session_create", "psx_session_destroy", "psx_session_reset",
@@ -338,6 +357,9 @@ EXPORTED_SYMBOLS = [
     "psx_server_release", "psx_server_stats",
     "psx_scene_intersect", "psx_scene_intersect_device", "psx_scene_intersect_device_noalloc",
     "psx_probe_input_width", "psx_probe_output_width", "psx_backend_probe",
+    "psx_wavefront_tuning_default", "psx_session_set_tuning", "psx_session_get_tuning",
+    "psx_session_autotune", "psx_surface_program_source", "psx_surface_program_build",
+    "psx_backend_generated_programs",
 ]
 
 _lib = None
@@ -357,6 +379,7 @@ def load_library(path: os.PathLike | None = None):
     vp = C.c_void_p
     lib.psx_version.restype = C.c_char_p
     lib.psx_device_count.restype = C.c_int
+    lib.psx_last_error.restype = C.c_char_p
     lib.psx_render_settings_default.argtypes = [P(RenderSettings)]
     lib.psx_backend_create.argtypes = [C.c_int, P(BackendOptions), P(vp)]
     lib.psx_backend_destroy.argtypes = [vp]
@@ -401,6 +424,22 @@ def load_library(path: os.PathLike | None = None):
     lib.psx_probe_output_width.restype = u32
     lib.psx_backend_probe.argtypes = [vp, u32, P(f32), u32, P(f32)]
     lib.psx_backend_probe.restype = C.c_int
+    lib.psx_wavefront_tuning_default.argtypes = [P(WavefrontTuning)]
+    lib.psx_wavefront_tuning_default.restype = None
+    lib.psx_session_set_tuning.argtypes = [vp, P(WavefrontTuning)]
+    lib.psx_session_set_tuning.restype = C.c_int
+    lib.psx_session_get_tuning.argtypes = [vp, P(WavefrontTuning)]
+    lib.psx_session_get_tuning.restype = C.c_int
+    lib.psx_session_autotune.argtypes = [vp, u32, P(TuningReport)]
+    lib.psx_session_autotune.restype = C.c_int
+    lib.psx_surface_program_source.argtypes = [P(Material), u32, u32, C.c_char_p, C.c_size_t,
+                                               P(C.c_size_t), P(u32)]
+    lib.psx_surface_program_source.restype = C.c_int
+    lib.psx_surface_program_build.argtypes = [P(Material), u32, u32, C.c_char_p, C.c_char_p, C.c_size_t,
+                                              P(C.c_uint64)]
+    lib.psx_surface_program_build.restype = C.c_int
+    lib.psx_backend_generated_programs.argtypes = [vp]
+    lib.psx_backend_generated_programs.restype = u32
     for name in ("psx_server_create", "psx_server_submit", "psx_server_update_parameters",
                  "psx_server_status", "psx_server_wait", "psx_server_read_film",
                  "psx_server_write_passes", "psx_server_cancel", "psx_server_release",
@@ -426,7 +465,11 @@ def load_library(path: os.PathLike | None = None):
 
 class PsxError(RuntimeError):
     def __init__(self, where: str, status: int):
-        super().__init__(f"{where} failed: {STATUS_NAMES.get(status, status)} ({status})")
+        detail = ""
+        if status in (PSX_ERROR_DEVICE, PSX_ERROR_OUT_OF_MEMORY) and _lib is not None:
+            detail = (_lib.psx_last_error() or b"").decode(errors="replace")
+        super().__init__(f"{where} failed: {STATUS_NAMES.get(status, status)} ({status})"
+                         + (f": {detail}" if detail else ""))
         self.status = status
 
 

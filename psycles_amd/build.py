@@ -40,12 +40,41 @@ def _run(cmd):
     return r
 
 
+# device headers that generated surface programs include (psx_rtc.cpp
+# compiles them with hipRTC at run time)
+EMBEDDED_HEADERS = ["psx_device.h", "psx_path.h", "psx_wavefront_kernels.h"]
+
+
+def write_embedded_headers() -> Path:
+    """Write _build/psx_embedded.cpp: the headers as string constants."""
+    delim = "PSXSRC"
+    parts = []
+    for name in EMBEDDED_HEADERS:
+        text = (CSRC / name).read_text()
+        if f"){delim}\"" in text:
+            raise RuntimeError(f"{name} contains the raw-string delimiter")
+        parts.append(f'R"{delim}({text}){delim}"')
+    out = BUILD / "psx_embedded.cpp"
+    body = ("// written by psycles_amd/build.py from psycles_amd/csrc; do not edit\n"
+            "namespace psx {\n"
+            f"extern const unsigned kEmbeddedCount = {len(EMBEDDED_HEADERS)};\n"
+            "extern const char *const kEmbeddedNames[] = {"
+            + ", ".join(f'"{n}"' for n in EMBEDDED_HEADERS) + "};\n"
+            "extern const char *const kEmbeddedSources[] = {\n" + ",\n".join(parts) + "};\n"
+            "}  // namespace psx\n")
+    out.write_text(body)
+    return out
+
+
 def build_library(verbose: bool = False) -> Path:
     """Compile every source and link libpsx.so; returns its path."""
     hipcc = shutil.which("hipcc") or str(ROCM / "bin" / "hipcc")
     BUILD.mkdir(exist_ok=True)
     jobs = []
     objects = []
+    embedded = write_embedded_headers()
+    jobs.append(["g++", *HOST_FLAGS, "-c", str(embedded), "-o", str(BUILD / "psx_embedded.o")])
+    objects.append(BUILD / "psx_embedded.o")
     for src in sorted(CSRC.glob("*.hip")):
         obj = BUILD / (src.stem + ".hip.o")
         jobs.append([hipcc, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
@@ -61,7 +90,7 @@ def build_library(verbose: bool = False) -> Path:
     out = PKG / "libpsx.so"
     tmp = PKG / "libpsx.so.tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-          *map(str, objects), f"-L{ROCM / 'lib'}", "-lamdhip64"])
+          *map(str, objects), f"-L{ROCM / 'lib'}", "-lamdhip64", "-lhiprtc"])
     os.replace(tmp, out)
     return out
 

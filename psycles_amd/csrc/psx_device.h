@@ -5,9 +5,24 @@
 // BVH4 nodes are one 128-byte line, leaf triangles are three float4 (48 B).
 #pragma once
 
+#ifdef __HIPCC_RTC__
+// Run-time compilation (hipRTC) of generated surface programs: hipRTC
+// provides the HIP runtime and vector types but no C library headers.
+typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::int8_t int8_t;
+typedef __hip_internal::uint16_t uint16_t;
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+#define FLT_MAX __FLT_MAX__
+#define FLT_MIN __FLT_MIN__
+#define FLT_EPSILON __FLT_EPSILON__
+#define INFINITY __builtin_huge_valf()
+#else
 #include <hip/hip_vector_types.h>
 
 #include <cstdint>
+#endif
 
 namespace psx {
 
@@ -84,8 +99,15 @@ struct alignas(16) Instance {
     uint32_t override_offset;
     uint32_t override_count;
     float terminator_offset;  // shadow_terminator_geometry_offset
-    uint32_t pad[3];
+    uint32_t shade_class;     // sort key of the materials it can use, 1..kShadeKeys-1 (0: miss)
+    uint32_t pad[2];
 };
+
+// Sort keys of shading work: 0 for a miss, then one per closure structure
+// (materials whose closure trees have the same shape run the same code), the
+// last one for instances whose materials differ in structure and for
+// structures beyond the first kShadeKeys - 2.
+constexpr uint32_t kShadeKeys = 8u;
 
 struct Mesh {
     uint32_t vertex_offset;    // object-space positions (float3 index)
@@ -131,7 +153,8 @@ struct Material {
     uint32_t param_offset;
     uint32_t emission_sampling;
     uint32_t may_emit;
-    uint32_t pad[3];
+    uint32_t program;  // generated surface program: one per distinct closure-leaf layout
+    uint32_t pad[2];
 };
 
 // ---------------------------------------------------------------------------
@@ -295,7 +318,8 @@ struct Counters {
 
 // ---------------------------------------------------------------------------
 // wavefront integrator: per-path state in HBM, structure of arrays indexed by
-// path = sample_in_batch * padded_pixels + tile * 64 + lane (8x8 tiles)
+// path = sample_in_batch * padded_pixels + tile * 64 + lane (tiles of 64
+// pixels, 8x8 unless the view's tile_shift says otherwise)
 // ---------------------------------------------------------------------------
 // trace_count and shade_count share one 8-byte word, so that a shade wave
 // appends to both with a single atomic
@@ -347,6 +371,16 @@ struct WavefrontView {
     float4 *shadow_split;  // light passes: 2 float4 per path (shadow dw, gw), field-major, or null
     uint32_t capacity;            // paths
     uint32_t padded_pixels;       // 64 * tiles of the window
+    // launch shape (any values in range give the same film):
+    uint32_t tile_shift;    // a wave's 64 paths cover (1 << tile_shift) x (64 >> tile_shift) pixels
+    uint32_t refill;        // idle lanes of a trace wave that trigger a refill, 1..64
+    uint32_t trace_chunk;   // queue entries a trace wave reserves per atomic, >= 1
+    uint32_t queue_chunk;   // next-queue slots a shade wave reserves per atomic, 128..kMaxQueueChunk
+    uint32_t list_chunk;    // shade-list slots a shade wave reserves per atomic, 64..kMaxListChunk
+    uint32_t shade_sort;    // 1: each shade block takes its 256 paths sorted by shade_class
 };
+
+// Largest shade reservations; the queues' headroom is sized for them.
+constexpr uint32_t kMaxQueueChunk = 256u, kMaxListChunk = 256u;
 
 }  // namespace psx

@@ -30,6 +30,7 @@
 #include "psx_bvh.h"
 #include "psx_device.h"
 #include "psx_light_tree.h"
+#include "psx_rtc.h"
 
 namespace psx {
 
@@ -62,13 +63,15 @@ hipError_t launch_intersect_noalloc(const SceneView &s, const psx_ray_query *que
                                     psx_ray_hit *hits, hipStream_t stream);
 // psx_wavefront.hip
 uint32_t wavefront_lanes();
-uint32_t wavefront_padded_pixels(const KernelParams &k);
+uint32_t wavefront_padded_pixels(const KernelParams &k, uint32_t tile_shift);
 uint32_t wavefront_queue_headroom();
 uint32_t wavefront_list_headroom();
 hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const float4 *sobol, const float *filter,
                             const FilmView &film, const WavefrontView &w, uint32_t first, uint32_t count,
-                            Counters *counters, bool count_nodes, uint32_t *spill, hipEvent_t *events,
-                            hipStream_t stream);
+                            const uint32_t *first_at, Counters *counters, bool count_nodes, uint32_t *spill,
+                            hipEvent_t *events, const GeneratedShade *generated, hipStream_t stream);
+hipError_t launch_store_word(uint32_t *dst, uint32_t value, hipStream_t stream);
+uint32_t shade_variant(const SceneView &s, bool light_passes);
 
 namespace {
 
@@ -76,9 +79,14 @@ struct DeviceError {
     int status;
 };
 
-void hip_check(hipError_t e) {
+// the last HIP failure on this thread, as "<error name> at psx_host.cpp:<line>"
+thread_local std::string g_last_error;
+
+void hip_check(hipError_t e, int line = __builtin_LINE()) {
+    if (e == hipSuccess) return;
+    g_last_error = std::string(hipGetErrorName(e)) + " at psx_host.cpp:" + std::to_string(line);
     if (e == hipErrorOutOfMemory) throw DeviceError{PSX_ERROR_OUT_OF_MEMORY};
-    if (e != hipSuccess) throw DeviceError{PSX_ERROR_DEVICE};
+    throw DeviceError{PSX_ERROR_DEVICE};
 }
 
 // RAII device allocation
@@ -405,6 +413,30 @@ bool flatten_program(const HostMaterial &m, uint32_t id, Leaf path, std::vector<
     }
 }
 
+// The material's closure tree as flat leaves, within the supported size.
+bool lower_leaves(const HostMaterial &m, std::vector<Leaf> &leaves, std::string &error) {
+    leaves.clear();
+    if (m.instr.empty()) return true;
+    Leaf root{};
+    if (!flatten_program(m, m.root, root, leaves, 0, error)) return false;
+    uint32_t scattering = 0;
+    for (const auto &l : leaves) scattering += l.op != kLeafEmission ? 1u : 0u;
+    if (leaves.size() > kMaxLeaves || scattering > 4u) {
+        error = "closure tree larger than supported";
+        return false;
+    }
+    return true;
+}
+
+HostMaterial host_material(const psx_material &src) {
+    HostMaterial m;
+    m.instr.assign(src.instructions, src.instructions + src.instruction_count);
+    m.root = src.root;
+    m.params.assign(src.parameters, src.parameters + src.parameter_count);
+    m.emission_sampling = src.emission_sampling;
+    return m;
+}
+
 // estimate_surface_emission (src/compiler/surface_program.cpp:244-330)
 void emission_estimate(const HostMaterial &m, uint32_t id, float out[3], int guard) {
     out[0] = out[1] = out[2] = 0.0f;
@@ -426,6 +458,20 @@ void refresh_may_emit(HostMaterial &m) {
     float e[3] = {0, 0, 0};
     if (!m.instr.empty()) emission_estimate(m, m.root, e, 0);
     m.may_emit = (e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f) && m.emission_sampling != PSX_EMISSION_SAMPLING_NONE;
+}
+
+// Shade sort key of an instance: its materials' common key, or the last key
+// when they differ in structure (or it has none).
+uint32_t instance_shade_class(const psx_instance &src, const Mesh &mesh, const std::vector<uint32_t> &slots,
+                              const std::vector<uint32_t> &material_class) {
+    uint32_t key = 0u;
+    auto add = [&](uint32_t material) {
+        const uint32_t c = material < material_class.size() ? material_class[material] : kShadeKeys - 1u;
+        key = key == 0u || key == c ? c : kShadeKeys - 1u;
+    };
+    for (uint32_t o = 0; o < src.material_override_count; ++o) add(src.material_overrides[o]);
+    for (uint32_t k = 0; k < mesh.slot_count; ++k) add(slots[mesh.slot_offset + k]);
+    return key == 0u ? kShadeKeys - 1u : key;
 }
 
 uint64_t structure_signature(const HostMaterial &m) {
@@ -458,6 +504,7 @@ uint64_t structure_signature(const HostMaterial &m) {
 struct Backend {
     int device = 0;
     psx_backend_options options{};
+    ProgramCache programs;  // generated shade kernels (PSX_BACKEND_GENERATED_PROGRAMS)
 };
 
 // Light-sampling tables that follow material emission: emitter list, flat
@@ -501,6 +548,7 @@ struct Scene {
     // host copies needed after compile
     std::vector<HostMaterial> materials;
     std::vector<Material> gpu_materials;
+    std::vector<LeafLayout> programs;  // closure-leaf layout of each generated surface program
     std::vector<psx_instance> instances;
     std::vector<std::vector<uint32_t>> overrides;
     std::vector<Mesh> meshes;
@@ -954,7 +1002,42 @@ struct WavefrontState {
         v.shadow_split = light_passes ? shadow_split.as<float4>() : nullptr;
         v.capacity = capacity;
         v.padded_pixels = padded_pixels;
+        psx_wavefront_tuning t;
+        psx_wavefront_tuning_default(&t);
+        set_tuning(t);
     }
+
+    // launch shape (a valid psx_wavefront_tuning) into the view
+    void set_tuning(const psx_wavefront_tuning &t) {
+        WavefrontView &v = view;
+        v.tile_shift = static_cast<uint32_t>(__builtin_ctz(t.tile_width));
+        v.refill = t.refill_lanes;
+        v.trace_chunk = t.trace_chunk;
+        v.queue_chunk = t.queue_chunk;
+        v.list_chunk = t.list_chunk;
+        v.shade_sort = t.sort_shading;
+    }
+};
+
+bool valid_tuning(const psx_wavefront_tuning &t) {
+    const bool tile = t.tile_width == 8u || t.tile_width == 16u || t.tile_width == 32u || t.tile_width == 64u;
+    const bool list = t.list_chunk == 64u || t.list_chunk == 128u || t.list_chunk == 256u;
+    return tile && t.refill_lanes >= 1u && t.refill_lanes <= 64u && t.trace_chunk >= 16u && t.trace_chunk <= 4096u &&
+           (t.queue_chunk == 128u || t.queue_chunk == 256u) && list && t.sort_shading <= 1u;
+}
+
+// A wavefront dispatch captured as a graph, replayed while every kernel
+// argument it was captured with (`key`) is unchanged.
+struct DispatchGraph {
+    hipGraphExec_t exec = nullptr;
+    std::vector<unsigned char> key;
+
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+    ~DispatchGraph() { reset(); }
 };
 
 struct Session {
@@ -977,8 +1060,14 @@ struct Session {
     std::vector<hipEvent_t> trace_events;  // pairs around wavefront trace launches
     psx_render_stats stats{};
     bool count_nodes = false;
+    psx_wavefront_tuning tuning{8u, 16u, 128u, 256u, 128u, 0u};
+    DispatchGraph graph[2];       // full dispatches of max_samples_per_dispatch, and the remainder
+    bool graph_failed = false;    // capture failed once: launch from the host
+    DeviceBuffer dispatch_first;  // first sample of the dispatch a replayed graph renders
 
     ~Session() {
+        graph[0].reset();
+        graph[1].reset();
         for (hipEvent_t e : trace_events) (void)hipEventDestroy(e);
         if (ev_begin) (void)hipEventDestroy(ev_begin);
         if (ev_end) (void)hipEventDestroy(ev_end);
@@ -1169,22 +1258,67 @@ bool valid_settings(const psx_render_settings *rs) {
 // Wavefront buffers for `per` samples per dispatch; kept across dispatches
 // and sessions resets while the window and limits fit.
 void prepare_wavefront(Session &s, uint32_t per) {
-    const uint32_t padded = wavefront_padded_pixels(s.params);
+    const uint32_t shift = static_cast<uint32_t>(__builtin_ctz(s.tuning.tile_width));
+    const uint32_t padded = wavefront_padded_pixels(s.params, shift);
     const uint64_t want = static_cast<uint64_t>(padded) * per;
     if (want > 0x7fffffffull) throw DeviceError{PSX_ERROR_INVALID_ARGUMENT};
     const uint32_t iterations = s.params.max_path_steps + 1u;
     const WavefrontView &v = s.wavefront.view;
-    if (v.capacity < want || v.padded_pixels != padded || s.wavefront_iterations < iterations ||
-        s.wavefront.light != s.light_passes) {
+    if (v.capacity < want || s.wavefront_iterations < iterations || s.wavefront.light != s.light_passes) {
         s.wavefront.allocate(static_cast<uint32_t>(want), padded, iterations, s.light_passes);
         s.wavefront_iterations = iterations;
     }
+    s.wavefront.view.padded_pixels = padded;
+    s.wavefront.set_tuning(s.tuning);
+    if (!s.dispatch_first.bytes()) s.dispatch_first.allocate(sizeof(uint32_t));
     const size_t events = 2u * static_cast<size_t>(iterations);
     while (s.trace_events.size() < events) {
         hipEvent_t e = nullptr;
         hip_check(hipEventCreate(&e));
         s.trace_events.push_back(e);
     }
+}
+
+// The scene's generated shade kernel for `variant`, compiled on first use.
+GeneratedShade generated_program(Scene &scene, uint32_t variant) {
+    try {
+        return scene.backend->programs.get(scene.programs, variant, scene.backend->device);
+    } catch (const std::runtime_error &e) {
+        g_last_error = e.what();
+        throw DeviceError{PSX_ERROR_DEVICE};
+    }
+}
+
+// appends the bytes of `v` to a graph key
+template <class T>
+void key_bytes(std::vector<unsigned char> &key, const T &v) {
+    const auto *b = reinterpret_cast<const unsigned char *>(&v);
+    key.insert(key.end(), b, b + sizeof(T));
+}
+
+// Captures one wavefront dispatch of `count` samples (every kernel launch of
+// launch_wavefront) into a graph slot.  The dispatch
+// reads its first sample from `dispatch_first`, so one graph serves every
+// range of the same size.  Returns false, with no graph, when the capture
+// or instantiation fails; the session then launches from the host.
+bool capture_dispatch(Session &s, DispatchGraph &slot, const FilmView &film, uint32_t count,
+                      const GeneratedShade *generated) {
+    slot.reset();
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+    const hipError_t launched =
+        launch_wavefront(s.scene->view, s.params, s.sobol.as<float4>(), s.filter.as<float>(), film, s.wavefront.view,
+                         0u, count, s.dispatch_first.as<uint32_t>(), s.counters.as<Counters>(), s.count_nodes,
+                         s.spill.as<uint32_t>(), nullptr, generated, s.stream);
+    const hipError_t ended = hipStreamEndCapture(s.stream, &g);
+    hipGraphExec_t exec = nullptr;
+    const bool ok = launched == hipSuccess && ended == hipSuccess && g &&
+                    hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    if (!ok) return false;
+    slot.exec = exec;
+    return true;
 }
 
 int dispatch_samples(Session &s, uint32_t first, uint32_t count) {
@@ -1198,7 +1332,18 @@ int dispatch_samples(Session &s, uint32_t first, uint32_t count) {
     FilmView film{s.combined.as<float>(), s.normal.as<float>(), s.albedo.as<float>(), s.sample_count.as<uint32_t>(),
                   s.light_passes ? s.light.as<float>() : nullptr};
     hip_check(hipEventRecord(s.ev_begin, s.stream));
-    uint64_t dispatches = 0;
+    uint64_t dispatches = 0, graph_launches = 0, graph_builds = 0;
+    // events time every trace launch; they cannot be timed inside a graph
+    // (hipEventElapsedTime rejects events recorded by graph nodes), so timed
+    // dispatches launch from the host
+    const bool time_trace = (opt.flags & PSX_BACKEND_TIME_TRACE) != 0u;
+    const bool graphs = (opt.flags & PSX_BACKEND_NO_GRAPHS) == 0u && !time_trace && !s.graph_failed;
+    GeneratedShade generated_shade{};
+    const GeneratedShade *generated = nullptr;
+    if (wavefront && (opt.flags & PSX_BACKEND_GENERATED_PROGRAMS) != 0u) {
+        generated_shade = generated_program(*s.scene, shade_variant(s.scene->view, s.light_passes));
+        generated = &generated_shade;
+    }
     double trace_ms = 0.0;
     uint64_t trace_launches = 0;
     for (uint32_t done = 0; done < count;) {
@@ -1208,16 +1353,49 @@ int dispatch_samples(Session &s, uint32_t first, uint32_t count) {
         }
         const uint32_t n = std::min(per, count - done);
         if (wavefront) {
-            hip_check(launch_wavefront(s.scene->view, s.params, s.sobol.as<float4>(), s.filter.as<float>(), film,
-                                       s.wavefront.view, first + done, n, s.counters.as<Counters>(), s.count_nodes,
-                                       s.spill.as<uint32_t>(), s.trace_events.data(), s.stream));
-            // the event pool is reused by the next dispatch: read it now
-            hip_check(hipStreamSynchronize(s.stream));
-            for (uint32_t it = 0; it < s.params.max_path_steps + 1u; ++it) {
-                float ms = 0.0f;
-                hip_check(hipEventElapsedTime(&ms, s.trace_events[2u * it], s.trace_events[2u * it + 1u]));
-                trace_ms += ms;
-                ++trace_launches;
+            bool replayed = false;
+            if (graphs) {
+                std::vector<unsigned char> key;
+                key_bytes(key, s.scene->view);
+                key_bytes(key, s.params);
+                key_bytes(key, film);
+                key_bytes(key, s.wavefront.view);
+                const void *ptrs[4] = {s.sobol.as<void>(), s.filter.as<void>(), s.counters.as<void>(),
+                                       s.spill.as<void>()};
+                key_bytes(key, ptrs);
+                key_bytes(key, n);
+                key_bytes(key, s.count_nodes);
+                key_bytes(key, generated_shade);
+                DispatchGraph &slot = s.graph[n == per ? 0 : 1];
+                if (!slot.exec || key != slot.key) {
+                    if (capture_dispatch(s, slot, film, n, generated)) {
+                        slot.key = std::move(key);
+                        ++graph_builds;
+                    } else {
+                        s.graph_failed = true;
+                    }
+                }
+                if (slot.exec) {
+                    hip_check(launch_store_word(s.dispatch_first.as<uint32_t>(), first + done, s.stream));
+                    hip_check(hipGraphLaunch(slot.exec, s.stream));
+                    ++graph_launches;
+                    replayed = true;
+                }
+            }
+            if (!replayed)
+                hip_check(launch_wavefront(s.scene->view, s.params, s.sobol.as<float4>(), s.filter.as<float>(), film,
+                                           s.wavefront.view, first + done, n, nullptr, s.counters.as<Counters>(),
+                                           s.count_nodes, s.spill.as<uint32_t>(),
+                                           time_trace ? s.trace_events.data() : nullptr, generated, s.stream));
+            if (time_trace) {
+                // the event pool is reused by the next dispatch: read it now
+                hip_check(hipStreamSynchronize(s.stream));
+                for (uint32_t it = 0; it < s.params.max_path_steps + 1u; ++it) {
+                    float ms = 0.0f;
+                    hip_check(hipEventElapsedTime(&ms, s.trace_events[2u * it], s.trace_events[2u * it + 1u]));
+                    trace_ms += ms;
+                    ++trace_launches;
+                }
             }
         } else {
             hip_check(launch_render(s.scene->view, s.params, s.sobol.as<float4>(), s.filter.as<float>(), film,
@@ -1241,6 +1419,8 @@ int dispatch_samples(Session &s, uint32_t first, uint32_t count) {
     s.stats.kernel_ms = ms;
     s.stats.trace_ms = trace_ms;
     s.stats.trace_launches = trace_launches;
+    s.stats.graph_launches = graph_launches;
+    s.stats.graph_builds = graph_builds;
     s.rendered += count;
     return PSX_OK;
 }
@@ -1337,28 +1517,20 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     std::vector<float> params;
     for (uint32_t i = 0; i < d->material_count; ++i) {
         const psx_material &src = d->materials[i];
-        HostMaterial m;
         if (src.instruction_count && !src.instructions) return fail("material without instructions");
-        m.instr.assign(src.instructions, src.instructions + src.instruction_count);
-        m.root = src.root;
         if (src.parameter_count && !src.parameters) return fail("material without parameters");
-        m.params.assign(src.parameters, src.parameters + src.parameter_count);
-        m.emission_sampling = src.emission_sampling;
+        HostMaterial m = host_material(src);
         m.signature = structure_signature(m);
         Material g{};
         g.leaf_offset = static_cast<uint32_t>(leaves.size());
         g.param_offset = static_cast<uint32_t>(params.size());
-        if (!m.instr.empty()) {
+        {
             std::vector<Leaf> ml;
-            Leaf root{};
             std::string error;
-            if (!flatten_program(m, m.root, root, ml, 0, error)) return fail("material " + std::to_string(i) + ": " + error);
-            uint32_t scattering = 0;
-            for (const auto &l : ml) scattering += l.op != kLeafEmission ? 1u : 0u;
-            if (ml.size() > kMaxLeaves || scattering > 4u)
-                return fail("material " + std::to_string(i) + ": closure tree larger than supported");
+            if (!lower_leaves(m, ml, error)) return fail("material " + std::to_string(i) + ": " + error);
             leaves.insert(leaves.end(), ml.begin(), ml.end());
             g.leaf_count = static_cast<uint32_t>(ml.size());
+            g.program = program_index(scene->programs, ml);
         }
         params.insert(params.end(), m.params.begin(), m.params.end());
         refresh_may_emit(m);
@@ -1366,6 +1538,19 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         g.may_emit = m.may_emit ? 1u : 0u;
         scene->gpu_materials.push_back(g);
         scene->materials.push_back(std::move(m));
+    }
+    // shade sort key per material: one per closure structure, in order of
+    // first use (structures beyond the first kShadeKeys - 2 share the last key)
+    std::vector<uint32_t> material_class(d->material_count);
+    {
+        std::vector<uint64_t> seen;
+        for (uint32_t i = 0; i < d->material_count; ++i) {
+            const uint64_t sig = scene->materials[i].signature;
+            const auto it = std::find(seen.begin(), seen.end(), sig);
+            if (it == seen.end()) seen.push_back(sig);
+            const uint32_t c = static_cast<uint32_t>(it == seen.end() ? seen.size() : (it - seen.begin()) + 1);
+            material_class[i] = std::min<uint32_t>(c, kShadeKeys - 1u);
+        }
     }
     // meshes
     std::vector<float> positions;
@@ -1500,6 +1685,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         if (!(src.shadow_terminator_geometry_offset >= 0.0f) || !std::isfinite(src.shadow_terminator_geometry_offset))
             return fail("instance " + std::to_string(i) + " has an invalid shadow terminator geometry offset");
         g.terminator_offset = src.shadow_terminator_geometry_offset;
+        g.shade_class = instance_shade_class(src, scene->meshes[mi], slots, material_class);
         overrides.insert(overrides.end(), src.material_overrides, src.material_overrides + src.material_override_count);
         const Mesh &mesh = scene->meshes[mi];
         float lo[3], hi[3];
@@ -1697,7 +1883,7 @@ hipError_t launch_batch_begin(const BatchItem *items, uint32_t n_items, uint32_t
                               const WavefrontView &w, hipStream_t stream);
 hipError_t launch_batch_iteration(const SceneView &s, const KernelParams &k, const float4 *sobol,
                                   const WavefrontView &w, uint32_t it, Counters *counters, uint32_t *spill,
-                                  float share, hipStream_t stream);
+                                  float share, const GeneratedShade *generated, hipStream_t stream);
 hipError_t launch_batch_end(const BatchItem *items, uint32_t n_items, uint32_t pixels, const WavefrontView &w,
                             hipStream_t stream);
 
@@ -1793,6 +1979,7 @@ struct Arena {
     hipStream_t stream = nullptr;
     hipEvent_t start = nullptr, done = nullptr;
     hipEvent_t ring[kIterationsAhead + 1u] = {};
+    GeneratedShade generated{};   // the batch scene's generated shade kernel, if the backend uses them
     bool busy = false;
     bool ending = false;          // accumulation queued; `done` ends the batch
     bool held = false;            // paused for viewport work at least once
@@ -2240,6 +2427,9 @@ struct Server {
         hip_check(hipMemsetAsync(a.counters.as<void>(), 0, sizeof(Counters), a.stream));
         a.k = a.host_items[0].k;
         a.view = leader->view;
+        a.generated = (leader->scene->backend->options.flags & PSX_BACKEND_GENERATED_PROGRAMS) != 0u
+                          ? generated_program(*leader->scene, shade_variant(a.view, false))
+                          : GeneratedShade{};
         a.sobol = sobol_table_for(a.k.sequence_size);
         a.n_items = n;
         a.pixels = pixels;
@@ -2302,7 +2492,8 @@ struct Server {
             const uint32_t it = a.next_it;
             a.held |= shrink;
             hip_check(launch_batch_iteration(a.view, a.k, a.sobol, a.state.view, it, a.counters.as<Counters>(),
-                                             a.spill.as<uint32_t>(), shrink ? kViewportShare : 1.0f, a.stream));
+                                             a.spill.as<uint32_t>(), shrink ? kViewportShare : 1.0f,
+                                             a.generated.fn ? &a.generated : nullptr, a.stream));
             hip_check(hipMemcpyAsync(&a.host_queues[it + 1u], a.state.view.counters + it + 1u,
                                      sizeof(WavefrontCounters), hipMemcpyDeviceToHost, a.stream));
             hip_check(hipEventRecord(a.ring[it % (kIterationsAhead + 1u)], a.stream));
@@ -2557,6 +2748,214 @@ int psx_session_render_device(psx_session *handle, uint32_t first, uint32_t coun
         s->total_aa = total;
         prepare_sobol(*s, total);
         return dispatch_samples(*s, first, count);
+    });
+}
+
+const char *psx_last_error(void) { return g_last_error.c_str(); }
+
+namespace {
+
+// Closure-leaf layouts of a material array, as scene compilation lowers them.
+int material_layouts(const psx_material *materials, uint32_t count, std::vector<LeafLayout> &layouts) {
+    if (count && !materials) return PSX_ERROR_INVALID_ARGUMENT;
+    for (uint32_t i = 0; i < count; ++i) {
+        const psx_material &src = materials[i];
+        if ((src.instruction_count && !src.instructions) || (src.parameter_count && !src.parameters))
+            return PSX_ERROR_INVALID_ARGUMENT;
+        std::vector<Leaf> ml;
+        std::string error;
+        if (!lower_leaves(host_material(src), ml, error)) return PSX_ERROR_INVALID_ARGUMENT;
+        (void)program_index(layouts, ml);
+    }
+    return PSX_OK;
+}
+
+void copy_text(const std::string &text, char *out, size_t capacity) {
+    if (!out || capacity == 0u) return;
+    const size_t n = std::min(capacity - 1u, text.size());
+    std::memcpy(out, text.data(), n);
+    out[n] = '\0';
+}
+
+}  // namespace
+
+int psx_surface_program_source(const psx_material *materials, uint32_t material_count, uint32_t variant,
+                               char *source, size_t capacity, size_t *length, uint32_t *programs) {
+    if (variant > 3u || !length) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        std::vector<LeafLayout> layouts;
+        const int rc = material_layouts(materials, material_count, layouts);
+        if (rc != PSX_OK) return rc;
+        const std::string text = surface_program_source(layouts, variant);
+        *length = text.size();
+        if (programs) *programs = static_cast<uint32_t>(layouts.size());
+        if (source && capacity <= text.size()) return static_cast<int>(PSX_ERROR_INVALID_ARGUMENT);
+        copy_text(text, source, capacity);
+        return static_cast<int>(PSX_OK);
+    });
+}
+
+int psx_surface_program_build(const psx_material *materials, uint32_t material_count, uint32_t variant,
+                              const char *arch, char *log, size_t log_capacity, uint64_t *code_bytes) {
+    if (variant > 3u) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        std::vector<LeafLayout> layouts;
+        const int rc = material_layouts(materials, material_count, layouts);
+        if (rc != PSX_OK) return rc;
+        const RtcResult r = compile_surface_program(surface_program_source(layouts, variant), arch ? arch : "gfx950");
+        copy_text(r.log, log, log_capacity);
+        if (code_bytes) *code_bytes = r.code.size();
+        return static_cast<int>(r.ok ? PSX_OK : PSX_ERROR_UNSUPPORTED);
+    });
+}
+
+uint32_t psx_backend_generated_programs(const psx_backend *handle) {
+    const Backend *b = reinterpret_cast<const Backend *>(handle);
+    return b ? b->programs.compiled() : 0u;
+}
+
+void psx_wavefront_tuning_default(psx_wavefront_tuning *t) {
+    if (t) *t = psx_wavefront_tuning{8u, 16u, 128u, 256u, 128u, 0u};
+}
+
+int psx_session_set_tuning(psx_session *handle, const psx_wavefront_tuning *t) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s || !t || !valid_tuning(*t)) return PSX_ERROR_INVALID_ARGUMENT;
+    s->tuning = *t;
+    return PSX_OK;
+}
+
+int psx_session_get_tuning(const psx_session *handle, psx_wavefront_tuning *t) {
+    const Session *s = reinterpret_cast<const Session *>(handle);
+    if (!s || !t) return PSX_ERROR_INVALID_ARGUMENT;
+    *t = s->tuning;
+    return PSX_OK;
+}
+
+namespace {
+
+// Device time of one probe dispatch of samples [0, count) into `film`.
+float time_probe(Session &s, const FilmView &film, uint32_t count, const GeneratedShade *generated, hipEvent_t a,
+                 hipEvent_t b) {
+    prepare_wavefront(s, count);
+    hip_check(hipEventRecord(a, s.stream));
+    hip_check(launch_wavefront(s.scene->view, s.params, s.sobol.as<float4>(), s.filter.as<float>(), film,
+                               s.wavefront.view, 0u, count, nullptr, s.counters.as<Counters>(), s.count_nodes,
+                               s.spill.as<uint32_t>(), nullptr, generated, s.stream));
+    hip_check(hipEventRecord(b, s.stream));
+    hip_check(hipEventSynchronize(b));
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+// One warm-up dispatch, then the faster of two timed ones.
+float time_shape(Session &s, const psx_wavefront_tuning &t, const FilmView &film, uint32_t count,
+                 const GeneratedShade *generated, hipEvent_t a, hipEvent_t b) {
+    s.tuning = t;
+    (void)time_probe(s, film, count, generated, a, b);
+    return std::min(time_probe(s, film, count, generated, a, b), time_probe(s, film, count, generated, a, b));
+}
+
+int autotune(Session &s, uint32_t probe, psx_tuning_report *report) {
+    const psx_backend_options &opt = s.scene->backend->options;
+    if (opt.integrator != PSX_INTEGRATOR_WAVEFRONT && !s.light_passes) return PSX_ERROR_UNSUPPORTED;
+    if (s.total_aa != 0u) {
+        probe = std::min(probe, s.total_aa);
+    } else {
+        prepare_sobol(s, probe);
+    }
+    // scratch film: the probes never touch the session's
+    const size_t pixels = static_cast<size_t>(s.window_w) * s.window_h;
+    DeviceBuffer combined, normal, albedo, count, light;
+    combined.allocate(pixels * 16u);
+    normal.allocate(pixels * 16u);
+    albedo.allocate(pixels * 16u);
+    count.allocate(pixels * 4u);
+    if (s.light_passes) light.allocate(pixels * kLightPasses * 16u);
+    const FilmView film{combined.as<float>(), normal.as<float>(), albedo.as<float>(), count.as<uint32_t>(),
+                        s.light_passes ? light.as<float>() : nullptr};
+    hipEvent_t a = nullptr, b = nullptr;
+    hip_check(hipEventCreate(&a));
+    hip_check(hipEventCreate(&b));
+    psx_tuning_report r{};
+    r.probe_samples = probe;
+    GeneratedShade generated_shade{};
+    const GeneratedShade *generated = nullptr;
+    if ((opt.flags & PSX_BACKEND_GENERATED_PROGRAMS) != 0u) {
+        generated_shade = generated_program(*s.scene, shade_variant(s.scene->view, s.light_passes));
+        generated = &generated_shade;
+    }
+    try {
+        psx_wavefront_tuning best;
+        psx_wavefront_tuning_default(&best);
+        r.default_ms = time_shape(s, best, film, probe, generated, a, b);
+        float best_ms = r.default_ms;
+        uint32_t candidates = 1u;
+        // one knob at a time; a shape replaces the best only if >= 1 % faster
+        auto consider = [&](psx_wavefront_tuning t) {
+            if (std::memcmp(&t, &best, sizeof(t)) == 0) return;
+            const float ms = time_shape(s, t, film, probe, generated, a, b);
+            ++candidates;
+            if (ms < 0.99f * best_ms) {
+                best = t;
+                best_ms = ms;
+            }
+        };
+        const psx_wavefront_tuning base_tile = best;
+        for (uint32_t tw : {16u, 32u, 64u}) {
+            psx_wavefront_tuning t = base_tile;
+            t.tile_width = tw;
+            consider(t);
+        }
+        const psx_wavefront_tuning base_refill = best;
+        for (uint32_t rl : {8u, 24u, 32u, 48u}) {
+            psx_wavefront_tuning t = base_refill;
+            t.refill_lanes = rl;
+            consider(t);
+        }
+        const psx_wavefront_tuning base_chunk = best;
+        for (uint32_t ch : {64u, 256u, 512u}) {
+            psx_wavefront_tuning t = base_chunk;
+            t.trace_chunk = ch;
+            consider(t);
+        }
+        const psx_wavefront_tuning base_append = best;
+        const uint32_t appends[3][2] = {{128u, 64u}, {128u, 128u}, {256u, 256u}};
+        for (const auto &qa : appends) {
+            psx_wavefront_tuning t = base_append;
+            t.queue_chunk = qa[0];
+            t.list_chunk = qa[1];
+            consider(t);
+        }
+        {
+            psx_wavefront_tuning t = best;
+            t.sort_shading = 1u;
+            consider(t);
+        }
+        r.chosen = best;
+        r.chosen_ms = best_ms;
+        r.candidates = candidates;
+        s.tuning = best;
+    } catch (...) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        throw;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (report) *report = r;
+    return PSX_OK;
+}
+
+}  // namespace
+
+int psx_session_autotune(psx_session *handle, uint32_t probe_samples, psx_tuning_report *report) {
+    Session *s = reinterpret_cast<Session *>(handle);
+    if (!s || probe_samples == 0u || probe_samples > 64u) return PSX_ERROR_INVALID_ARGUMENT;
+    return guarded([&] {
+        hip_check(hipSetDevice(s->scene->backend->device));
+        return autotune(*s, probe_samples, report);
     });
 }
 

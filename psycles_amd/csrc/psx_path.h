@@ -14,10 +14,12 @@
 //   M * (v, w)   = ((c0*x + c1*y) + c2*z) + c3*w
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
 #include <cstdint>
+#endif
 
 #include "psx_device.h"
 
@@ -875,6 +877,30 @@ struct ClosureSet {
 
 __device__ __forceinline__ float avg3(f3 v) { return (v.x + v.y + v.z) / 3.0f; }
 
+// Appends a closure to the set; slot writes through constant indices keep
+// the set in registers.
+__device__ __forceinline__ void append_closure(ClosureSet &cs, uint32_t kind, f3 wt, float sw, float roughness,
+                                               f3 color) {
+#pragma unroll
+    for (uint32_t c = 0; c < kMaxClosures; ++c) {
+        if (c == cs.count) {
+            cs.kind[c] = kind;
+            cs.weight[c] = wt;
+            cs.sw[c] = sw;
+            cs.roughness[c] = roughness;
+            cs.color[c] = color;
+        }
+    }
+    cs.count += 1u;
+}
+
+#ifdef PSX_GENERATED_SURFACE_PROGRAMS
+// Defined by the generated source compiled at run time: the closure set of
+// surface program `program` over the parameter block `p`, statement for
+// statement what the interpreter below computes for that leaf layout.
+__device__ void generated_closures(uint32_t program, const float *p, ClosureSet &cs);
+#endif
+
 __device__ void material_closures(const SceneView &s, uint32_t material, f3 ns, ClosureSet &cs) {
     cs.count = 0u;
     cs.emission = V(0.0f, 0.0f, 0.0f);
@@ -882,6 +908,11 @@ __device__ void material_closures(const SceneView &s, uint32_t material, f3 ns, 
     if (material == kNone) return;
     const Material &m = s.materials[material];
     const float *p = s.params + m.param_offset;
+#ifdef PSX_GENERATED_SURFACE_PROGRAMS
+    generated_closures(m.program, p, cs);
+    return;
+#endif
+    // the interpreter: one pass over the material's flattened closure leaves
     for (uint32_t l = 0; l < m.leaf_count; ++l) {
         const Leaf &leaf = s.leaves[m.leaf_offset + l];
         float w = 1.0f;
@@ -912,18 +943,7 @@ __device__ void material_closures(const SceneView &s, uint32_t material, f3 ns, 
         const uint32_t kind = diffuse ? KIND_DIFFUSE : KIND_TRANSPARENT;
         const float roughness = diffuse ? p[leaf.aux] : 0.0f;
         const f3 ccolor = diffuse ? color : V(1.0f, 1.0f, 1.0f);
-        // slot writes through constant indices keep the set in registers
-#pragma unroll
-        for (uint32_t c = 0; c < kMaxClosures; ++c) {
-            if (c == cs.count) {
-                cs.kind[c] = kind;
-                cs.weight[c] = wt;
-                cs.sw[c] = sw;
-                cs.roughness[c] = roughness;
-                cs.color[c] = ccolor;
-            }
-        }
-        cs.count += 1u;
+        append_closure(cs, kind, wt, sw, roughness, ccolor);
     }
 }
 

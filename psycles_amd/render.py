@@ -202,6 +202,29 @@ class RenderSession:
     def cancel(self) -> None:
         self._backend.lib.psx_session_cancel(self.handle)
 
+    @property
+    def tuning(self) -> abi.WavefrontTuning:
+        """The wavefront launch shape (tile width, refill lanes, chunks)."""
+        t = abi.WavefrontTuning()
+        abi.check("psx_session_get_tuning", self._backend.lib.psx_session_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    def set_tuning(self, tuning: abi.WavefrontTuning | dict) -> None:
+        if isinstance(tuning, dict):
+            base = self.tuning
+            for name, value in tuning.items():
+                setattr(base, name, value)
+            tuning = base
+        abi.check("psx_session_set_tuning", self._backend.lib.psx_session_set_tuning(self.handle, C.byref(tuning)))
+
+    def autotune(self, probe_samples: int = 4) -> abi.TuningReport:
+        """Times probe dispatches of candidate launch shapes and keeps the
+        fastest (the film does not depend on the shape)."""
+        rep = abi.TuningReport()
+        abi.check("psx_session_autotune",
+                  self._backend.lib.psx_session_autotune(self.handle, probe_samples, C.byref(rep)))
+        return rep
+
     def stats(self) -> abi.RenderStats:
         st = abi.RenderStats()
         abi.check("psx_session_stats", self._backend.lib.psx_session_stats(self.handle, C.byref(st)))
@@ -277,15 +300,27 @@ class RendererBackend:
 
     def __init__(self, device: int = 0, next_event_estimation: bool = True,
                  max_samples_per_dispatch: int = 4, integrator: int = abi.INTEGRATOR_MEGAKERNEL,
-                 count_traversal: bool = False):
+                 count_traversal: bool = False, graphs: bool = True, time_trace: bool = False,
+                 generated_programs: bool = False):
         self.lib = abi.load_library()
         flags = abi.BACKEND_COUNT_TRAVERSAL if count_traversal else 0
+        if generated_programs:
+            flags |= abi.BACKEND_GENERATED_PROGRAMS
+        if not graphs:
+            flags |= abi.BACKEND_NO_GRAPHS
+        if time_trace:
+            flags |= abi.BACKEND_TIME_TRACE
         opts = abi.BackendOptions(1 if next_event_estimation else 0, max_samples_per_dispatch,
                                   integrator, flags)
         handle = C.c_void_p()
         abi.check("psx_backend_create", self.lib.psx_backend_create(device, C.byref(opts),
                                                                     C.byref(handle)))
         self.handle = handle
+
+    @property
+    def generated_programs(self) -> int:
+        """Generated shade modules this backend has compiled."""
+        return int(self.lib.psx_backend_generated_programs(self.handle))
 
     def compile_scene(self, snapshot) -> SceneCompilation:
         desc, keep = snapshot.to_desc()

@@ -228,6 +228,26 @@ class CameraData:
     horizontal_field_of_view: float | None = None   # defaults to field_of_view
 
 
+def material_structs(materials, keep: list):
+    """psx_material array of `materials`; the buffers it points to go to `keep`."""
+    mats = (abi.Material * max(len(materials), 1))()
+    for i, m in enumerate(materials):
+        instr = (abi.ClosureInstr * max(len(m.instructions), 1))()
+        for j, t in enumerate(m.instructions):
+            instr[j] = abi.ClosureInstr(*t, 0)
+        keep.append(instr)
+        mats[i].instructions = C.cast(instr, C.POINTER(abi.ClosureInstr))
+        mats[i].instruction_count = len(m.instructions)
+        mats[i].root = m.root
+        params = np.ascontiguousarray(m.parameters.astype(np.float32))
+        keep.append(params)
+        mats[i].parameters = params.ctypes.data_as(C.POINTER(C.c_float))
+        mats[i].parameter_count = len(m.parameters)
+        mats[i].emission_sampling = m.emission_sampling
+    keep.append(mats)
+    return mats
+
+
 @dataclass
 class SceneSnapshot:
     materials: list
@@ -251,19 +271,7 @@ class SceneSnapshot:
             keep.append(a)
             return a.ctypes.data_as(C.POINTER(ctype))
 
-        mats = (abi.Material * max(len(self.materials), 1))()
-        for i, m in enumerate(self.materials):
-            instr = (abi.ClosureInstr * max(len(m.instructions), 1))()
-            for j, t in enumerate(m.instructions):
-                instr[j] = abi.ClosureInstr(*t, 0)
-            keep.append(instr)
-            mats[i].instructions = C.cast(instr, C.POINTER(abi.ClosureInstr))
-            mats[i].instruction_count = len(m.instructions)
-            mats[i].root = m.root
-            mats[i].parameters = arr(m.parameters.astype(np.float32), C.c_float)
-            mats[i].parameter_count = len(m.parameters)
-            mats[i].emission_sampling = m.emission_sampling
-        keep.append(mats)
+        mats = material_structs(self.materials, keep)
         meshes = (abi.Mesh * max(len(self.meshes), 1))()
         for i, m in enumerate(self.meshes):
             meshes[i].positions = arr(m.positions.astype(np.float32).reshape(-1), C.c_float)

@@ -10,6 +10,9 @@ if str(ROOT) not in sys.path:
 TESTS = Path(__file__).resolve().parent
 if str(TESTS) not in sys.path:
     sys.path.insert(0, str(TESTS))
+# hipRTC builds of generated surface programs keep their code-object cache
+# inside the build tree, not in the user's home
+os.environ.setdefault("AMD_COMGR_CACHE_DIR", str(ROOT / "_build" / "comgr_cache"))
 
 
 def pytest_configure(config):
