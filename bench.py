@@ -13,6 +13,9 @@ megakernel (one lane runs whole paths) is timed on one frame beside it.
 scene plus 10,000 emissive triangles and 321 lamps, light-tree NEE,
 1920x1080 at 32 spp, max 6 bounces), `--workload cornell` the Cornell box
 (640x400 at 16 spp, max 4 bounces); the default line is the instanced one.
+Shading runs the scene's surface programs, generated per closure-leaf layout
+and compiled with hipRTC on the first (untimed) dispatch; `--interpreter`
+shades with the built-in closure interpreter instead (the same film).
 Before the timed region the session's wavefront launch shape is autotuned
 (probe dispatches of 4 samples; the film does not depend on the shape).
 Trace launches are timed with HIP events, which this runtime cannot time
@@ -299,8 +302,9 @@ def main() -> int:
                     help="launch every kernel of a dispatch from the host instead of replaying a graph")
     ap.add_argument("--tuning", default=None,
                     help="explicit wavefront launch shape instead of the autotuner, e.g. tile_width=16,sort_shading=1")
-    ap.add_argument("--generated-programs", action="store_true",
-                    help="shade with the scene's hipRTC-generated surface programs instead of the interpreter")
+    ap.add_argument("--interpreter", action="store_true",
+                    help="shade with the closure interpreter instead of the scene's hipRTC-generated surface "
+                         "programs")
     ap.add_argument("--untimed-trace", action="store_true",
                     help="do not time each trace launch with HIP events (no roofline; dispatches may then "
                          "replay as graphs)")
@@ -328,7 +332,7 @@ def main() -> int:
     b_ray = bytes_per_ray(w["nodes"], w["tris"])
     backend = RendererBackend(0, integrator=abi.INTEGRATOR_WAVEFRONT,
                               max_samples_per_dispatch=SAMPLES_PER_DISPATCH, graphs=not args.no_graphs,
-                              time_trace=not args.untimed_trace, generated_programs=args.generated_programs)
+                              time_trace=not args.untimed_trace, generated_programs=not args.interpreter)
     t0 = time.perf_counter()
     comp = backend.compile_scene(snap)
     if not comp.ok():
@@ -406,7 +410,7 @@ def main() -> int:
             "integrator": "wavefront (batched)",
             "samples_per_dispatch": SAMPLES_PER_DISPATCH,
             "launch": "graph replay per dispatch" if graph_launches else "host launches",
-            "shading": "generated surface programs (hipRTC)" if args.generated_programs else "closure interpreter",
+            "shading": "closure interpreter" if args.interpreter else "generated surface programs (hipRTC)",
             "tuning": tuning if tuning else session.tuning.as_dict(),
         },
         "mrays_per_s": mrays,
