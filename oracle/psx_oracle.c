@@ -833,6 +833,7 @@ typedef struct emitter {
 #define LAMP_USE_MIS 8u
 #define LAMP_FORWARD 32u
 #define LAMP_FULL_SPREAD 64u
+#define LAMP_CAST_SHADOW 128u
 
 typedef struct olamp {
     uint32_t type;
@@ -1676,7 +1677,8 @@ static void lamp_from_desc(const psx_light *l, olamp *lp) {
                 ((l->flags & PSX_LIGHT_FLAG_SPHERE) ? LAMP_SPHERE : 0u) |
                 (effective_mis ? LAMP_USE_MIS : 0u) |
                 (effective_mis && l->type != PSX_LIGHT_DISTANT ? LAMP_FORWARD : 0u) |
-                (l->type == PSX_LIGHT_AREA && l->spread >= 3.14159265358979f - 1.0e-6f ? LAMP_FULL_SPREAD : 0u);
+                (l->type == PSX_LIGHT_AREA && l->spread >= 3.14159265358979f - 1.0e-6f ? LAMP_FULL_SPREAD : 0u) |
+                ((l->flags & PSX_LIGHT_FLAG_CAST_SHADOW) ? LAMP_CAST_SHADOW : 0u);
 }
 
 oracle_scene *oracle_scene_create(const psx_scene_desc *d, int brute_force) {
@@ -2711,15 +2713,33 @@ static int sampled_side(uint32_t s, v3 ng, v3 dir) {
  * light, each pass restarting one float past the last hit; the first opaque
  * surface, or the transparent-bounce limit, ends the query with zero.
  * `passes` receives the traversals issued. */
+/* The first surface a shadow ray evaluated (trace_shadow's first_hit,
+ * src/luisa/path_tracer_geometry.cpp:136-143), for path traces. */
+typedef struct shadow_first_t {
+    int hit;
+    uint32_t object, cprim;
+    float t, u, v;
+} shadow_first_t;
+
 static v3 trace_shadow(const oracle_scene *s, ray_t sray, uint32_t visibility, uint32_t src_obj,
                        uint32_t src_prim, uint32_t light_obj, uint32_t light_prim, uint32_t tdepth,
-                       uint32_t transparent_max, trace_counters *counters, uint32_t *passes) {
+                       uint32_t transparent_max, trace_counters *counters, uint32_t *passes,
+                       shadow_first_t *first) {
     v3 transmittance = V3(1, 1, 1);
+    if (first) first->hit = 0;
     for (;;) {
         const hit_t sh = closest_hit(s, &sray, visibility, src_obj, src_prim, light_obj, light_prim, counters);
         (*passes)++;
         if (!sh.valid) break;
         if (tdepth >= transparent_max) { transmittance = V3(0, 0, 0); break; }
+        if (first && !first->hit) {
+            first->hit = 1;
+            first->object = sh.object;
+            first->cprim = sh.cprim;
+            first->t = sh.t;
+            first->u = sh.u;
+            first->v = sh.v;
+        }
         const oinstance *si = &s->instances[sh.inst];
         const uint32_t smat = material_of(s, si, sh.prim);
         v3 ext = V3(0, 0, 0);
@@ -3695,6 +3715,67 @@ static void trace_slot(float *slots, uint32_t index, float x, float y, float z) 
 static void trace_slot3(float *slots, uint32_t index, v3 v) { trace_slot(slots, index, v.x, v.y, v.z); }
 static float lo16(uint32_t v) { return (float)(v & 0xffffu); }
 static float hi16(uint32_t v) { return (float)(v >> 16); }
+#define TRACE_SHADOW_BASE 296u
+#define TRACE_SHADOW_SLOTS 8u
+
+/* slots of surface event `step`, or NULL past the traced events */
+static float *event_slots(trace_log *log, uint32_t step) {
+    return log && log->slots && step < TRACE_MAX_EVENTS
+               ? log->slots + 4u * (TRACE_GLOBAL_SLOTS + step * TRACE_EVENT_SLOTS)
+               : NULL;
+}
+/* shadow-event slots of event `step`, or NULL */
+static float *shadow_event_slots(trace_log *log, uint32_t step) {
+    return log && log->slots && step < TRACE_MAX_EVENTS
+               ? log->slots + 4u * (TRACE_SHADOW_BASE + step * TRACE_SHADOW_SLOTS)
+               : NULL;
+}
+/* an object or primitive identity, -1 when invalid
+ * (src/luisa/path_kernel_direct_light_trace.cpp:6-11) */
+static float trace_identity(uint32_t v) { return v != UINT32_MAX ? (float)v : -1.0f; }
+/* Cycles' exclusion flags of an object's ray visibility, not a shadow
+ * catcher (src/luisa/cycles_shader_identity.h:64-97) */
+static uint32_t object_visibility_flags(uint32_t v) {
+    return ((v & PSX_VISIBILITY_CAMERA) ? 0u : (1u << 24)) | ((v & PSX_VISIBILITY_DIFFUSE) ? 0u : (1u << 27)) |
+           ((v & PSX_VISIBILITY_GLOSSY) ? 0u : (1u << 26)) |
+           ((v & PSX_VISIBILITY_TRANSMISSION) ? 0u : (1u << 25)) |
+           ((v & PSX_VISIBILITY_VOLUME_SCATTER) ? 0u : (1u << 23)) | (1u << 22);
+}
+/* Cycles KernelLightType (src/luisa/cycles_shader_identity.h:37-53) */
+static uint32_t cycles_light_type(uint32_t type) {
+    return type == PSX_LIGHT_POINT ? 0u : type == PSX_LIGHT_DISTANT ? 1u : type == PSX_LIGHT_AREA ? 3u : 4u;
+}
+/* Cycles ClosureType (src/luisa/graph_surface_scattering.cpp:373-420,
+ * include/psycles/luisa/cycles_closure.h:18-32) */
+static uint32_t cycles_closure_type(const closure_rec *c) {
+    if (c->kind == KIND_TRANSPARENT) return 30u;
+    return c->roughness < 1.0e-5f ? 2u : 3u;
+}
+/* record_shadow and record_contribution
+ * (src/luisa/path_kernel_direct_light_trace.cpp:152-228): the traced shadow
+ * ray, its identities, first hit and transmittance; the unclamped
+ * contribution when light arrives.  `unshadowed` is throughput x surviving. */
+static void trace_shadow_record(trace_log *log, uint32_t step, const ray_t *r, int cast_shadow, uint32_t src_obj,
+                                uint32_t src_prim, int skip_self, uint32_t light_obj, uint32_t light_prim,
+                                const shadow_first_t *first, v3 transmittance, v3 unshadowed) {
+    float *e = shadow_event_slots(log, step);
+    if (!e) return;
+    trace_slot3(e, 0u, r->o);
+    trace_slot3(e, 1u, r->d);
+    trace_slot(e, 2u, r->tmin, r->tmax, cast_shadow ? 1.0f : 0.0f);
+    trace_slot(e, 3u, trace_identity(src_obj), trace_identity(src_prim), skip_self ? 1.0f : 0.0f);
+    trace_slot(e, 4u, trace_identity(light_obj), trace_identity(light_prim), 0.0f);
+    if (first->hit) {
+        trace_slot(e, 5u, trace_identity(first->object), trace_identity(first->cprim), 0.0f);
+        trace_slot(e, 6u, first->t, first->u, first->v);
+    } else {
+        trace_slot(e, 5u, -1.0f, -1.0f, -1.0f);
+        trace_slot(e, 6u, 0.0f, 0.0f, 0.0f);
+    }
+    trace_slot3(e, 7u, transmittance);
+    if (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f)
+        trace_slot3(event_slots(log, step), 47u, vmul(unshadowed, transmittance));
+}
 
 static void log_event(trace_log *log, uint32_t step, uint32_t word, uint32_t value) {
     if (!log || !log->records || step >= PSX_TRACE_EVENTS) return;
@@ -3938,10 +4019,12 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
         lp_route(lp, LP_EMISSION, emission_contribution, ps.flag, ps.bounce, path_diffuse_w, path_glossy_w);
         if (terminate_on_next_surface) break;
         /* Russian roulette */
+        float continuation = 1.0f;
         {
             const float th[3] = {throughput.x, throughput.y, throughput.z};
             const float p = oracle_continuation_probability(ps.flag, ps.bounce, ps.transparent_bounce,
                                                             k->min_bounces, k->transparent_min, th);
+            continuation = p;
             log_event(log, step, PSX_TRACE_RR_BITS, fbits(p));
             if (p <= 0.0f) { log_flag(log, step, 8u); break; }
             if (p != 1.0f) {
@@ -3971,6 +4054,12 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                 trace_slot(e, 14u, u_term, k->light_inv_rr_threshold > 0.0f ? u_lightterm : 0.0f, 0.0f);
                 trace_slot3(e, 15u, u_light);
                 trace_slot3(e, 16u, u_bsdf);
+                /* the shading point's closures (src/luisa/path_kernel_surface_shading.cpp:350-380) */
+                for (uint32_t c = 0; c < cs.count && c < 8u; ++c) {
+                    trace_slot(e, 48u + 3u * c, (float)c, (float)cycles_closure_type(&cs.c[c]), cs.c[c].sample_weight);
+                    trace_slot3(e, 49u + 3u * c, cs.c[c].weight);
+                    trace_slot3(e, 50u + 3u * c, cs.c[c].normal);
+                }
             }
         }
         if (ps.bounce == 0u) {
@@ -4019,6 +4108,25 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
             const int side = sampled_side(em->emission_sampling, egn, ls.direction);
             const float lpdf = ls.pdf * light->selection_pdf;
             const int lvalid = ls.valid && side && earea > 0.0f && lpdf > 0.0f;
+            float *te = event_slots(log, step);
+            if (te) {
+                /* record_sample / record_failed_sample
+                 * (src/luisa/path_kernel_direct_light_trace.cpp:55-105) */
+                if (lvalid) {
+                    const int esmooth = emesh->smooth && emesh->smooth[em->prim] != 0u;
+                    const uint32_t shader = em->material | (1u << 30) | (esmooth ? (1u << 31) : 0u) | (1u << 28) |
+                                            object_visibility_flags(einst->visibility & 0x3fu);
+                    trace_slot(te, 17u, 5.0f, (float)light_id, (float)(emesh->prim_offset + em->prim));
+                    trace_slot(te, 18u, (float)em->instance, 0.0f, 0.0f);
+                    trace_slot(te, 40u, lo16(shader), hi16(shader), 0.0f);
+                    trace_slot(te, 19u, lpdf, light->selection_pdf, 1.0f);
+                    trace_slot3(te, 20u, ls.direction);
+                    trace_slot3(te, 21u, ls.position);
+                    trace_slot3(te, 22u, egn);
+                } else {
+                    trace_slot(te, 17u, -1.0f, -1.0f, -1.0f);
+                }
+            }
             if (lvalid) {
                 const uint32_t eobj = em->instance;
                 const uint32_t eprim = emesh->prim_offset + em->prim;
@@ -4032,10 +4140,21 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                                                        lobe_mask, UINT32_MAX);
                     const float mis = nee_light_weight(k, lpdf, ev.pdf);
                     const v3 weighted = vscale(vmul(ev.f, le), mis / lpdf);
+                    if (te) {
+                        trace_slot(te, 23u, ls.distance, ev.pdf, mis);
+                        trace_slot3(te, 41u, ev.f);
+                        trace_slot3(te, 42u, ev.f);
+                        trace_slot(te, 43u, 0.0f, 0.0f, 0.0f);
+                        trace_slot3(te, 44u, weighted);
+                    }
                     if (vany_nonzero(weighted)) {
                         const v3 unshadowed = weighted;
                         const float rr = light_roulette(unshadowed, u_lightterm, k->light_inv_rr_threshold);
                         const v3 surviving = vscale(unshadowed, rr);
+                        if (te) {
+                            trace_slot(te, 45u, 1.0f, 1.0f, 1.0f);
+                            trace_slot3(te, 46u, vmul(throughput, surviving));
+                        }
                         if (vany_nonzero(surviving)) {
                             const shadow_origin_t sso = surface_shadow_origin(s, &sf, ls.direction);
                             const v3 so = sso.position;
@@ -4050,10 +4169,13 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                             sray.tmax = sdist;
                             /* trace_shadow (src/luisa/path_tracer_geometry.cpp:75-174) */
                             uint32_t passes = 0u;
+                            shadow_first_t first;
                             const v3 transmittance = trace_shadow(s, sray, PSX_VISIBILITY_SHADOW, sh_obj, sh_prim,
                                                                   eobj, eprim, ps.transparent_bounce,
-                                                                  k->transparent_max, counters, &passes);
+                                                                  k->transparent_max, counters, &passes, &first);
                             rays += passes;
+                            trace_shadow_record(log, step, &sray, 1, sh_obj, sh_prim, sso.skip_self, eobj, eprim,
+                                                &first, transmittance, vmul(throughput, surviving));
                             if (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f) {
                                 const v3 c = film_clamp(vmul(vmul(throughput, surviving), transmittance),
                                                         ps.bounce, k->clamp_direct, k->clamp_indirect);
@@ -4081,14 +4203,41 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                                          : sample_point_or_spot(l, sf.position, sf.ns, has_transmission, rnd);
             const v3 light_radiance = vscale(l->color, l->power * ls.eval);
             const float light_pdf = ls.pdf * light->selection_pdf;
+            float *te = event_slots(log, step);
+            if (te) {
+                /* record_sample / record_failed_sample
+                 * (src/luisa/path_kernel_nee_analytic.cpp:233-249, 413) */
+                if (!reached_max_bounces && ls.valid && light_pdf > 0.0f) {
+                    trace_slot(te, 17u, (float)cycles_light_type(l->type), (float)light_id, (float)light->index);
+                    trace_slot(te, 18u, (float)UINT32_MAX, 0.0f, 0.0f);
+                    trace_slot(te, 40u, lo16(UINT32_MAX), hi16(UINT32_MAX), 0.0f);
+                    trace_slot(te, 19u, light_pdf, light->selection_pdf, ls.eval);
+                    trace_slot3(te, 20u, ls.direction);
+                    trace_slot3(te, 21u, ls.position);
+                    trace_slot3(te, 22u, ls.normal);
+                } else {
+                    trace_slot(te, 17u, -1.0f, -1.0f, -1.0f);
+                }
+            }
             if (!reached_max_bounces && ls.valid && light_pdf > 0.0f) {
                 const eval_t ev = evaluate_mixture(&cs, sf.incoming, sf.ns, ls.direction, lobe_mask, UINT32_MAX);
                 const float cycles_mis = nee_light_weight(k, light_pdf, ev.pdf);
                 const float mis = (l->flags & LAMP_FORWARD) != 0u ? cycles_mis : 1.0f;
                 const v3 weighted = vscale(vmul(ev.f, light_radiance), mis / fmaxf(light_pdf, 1.0e-20f));
+                if (te) {
+                    trace_slot(te, 23u, ls.distance, ev.pdf, cycles_mis);
+                    trace_slot3(te, 41u, ev.f);
+                    trace_slot3(te, 42u, ev.f);
+                    trace_slot(te, 43u, 0.0f, 0.0f, 0.0f);
+                    trace_slot3(te, 44u, weighted);
+                }
                 if (vany_nonzero(weighted)) {
                     const float rr = light_roulette(weighted, u_lightterm, k->light_inv_rr_threshold);
                     const v3 surviving = vscale(weighted, rr);
+                    if (te) {
+                        trace_slot(te, 45u, 1.0f, 1.0f, 1.0f);
+                        trace_slot3(te, 46u, vmul(throughput, surviving));
+                    }
                     if (vany_nonzero(surviving)) {
                         const shadow_origin_t sso = surface_shadow_origin(s, &sf, ls.direction);
                         const v3 so = sso.position;
@@ -4102,10 +4251,14 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
                         sray.tmin = 0.0f;
                         sray.tmax = distant ? RAY_MAXIMUM : fdist;
                         uint32_t passes = 0u;
+                        shadow_first_t first;
                         const v3 transmittance = trace_shadow(s, sray, PSX_VISIBILITY_SHADOW, sh_obj, sh_prim,
                                                               UINT32_MAX, UINT32_MAX, ps.transparent_bounce,
-                                                              k->transparent_max, counters, &passes);
+                                                              k->transparent_max, counters, &passes, &first);
                         rays += passes;
+                        trace_shadow_record(log, step, &sray, (l->flags & LAMP_CAST_SHADOW) != 0u, sh_obj, sh_prim,
+                                            sso.skip_self, UINT32_MAX, UINT32_MAX, &first, transmittance,
+                                            vmul(throughput, surviving));
                         if (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f) {
                             const v3 c = film_clamp(vmul(vmul(throughput, surviving), transmittance), ps.bounce,
                                                     k->clamp_direct, k->clamp_indirect);
@@ -4120,6 +4273,23 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
         const v2 udir = {u_bsdf.x, u_bsdf.y};
         const bsdf_sample_t bs = sample_closures(&cs, &sf, u_bsdf.z, udir, lobe_mask);
         log_event(log, step, PSX_TRACE_CLOSURE, bs.selected);
+        {
+            /* the sampled closure and BSDF sample
+             * (src/luisa/path_kernel_surface_scatter.cpp:95-130) */
+            float *te = event_slots(log, step);
+            if (te && bs.selected < cs.count) {
+                const closure_rec *c = &cs.c[bs.selected];
+                trace_slot(te, 24u, (float)bs.selected, (float)cycles_closure_type(c), c->sample_weight);
+                trace_slot(te, 25u, u_bsdf.x, u_bsdf.y, bs.rescaled);
+                trace_slot3(te, 26u, c->weight);
+                trace_slot3(te, 27u, c->normal);
+                trace_slot(te, 28u, bs.eval.pdf, bs.eval.pdf, (float)label_from_events(bs.eval.events));
+                trace_slot3(te, 29u, bs.wi);
+                trace_slot3(te, 30u, bs.eval.f);
+                const float r = bs.valid && c->kind == KIND_DIFFUSE ? 1.0f : 0.0f;
+                trace_slot(te, 31u, r, r, 1.0f);
+            }
+        }
         if (!bs.valid || bs.eval.pdf <= 0.0f) break;
         log_flag(log, step, 32u);
         const int transparent = (bs.eval.events & EV_TRANSPARENT) != 0u;
@@ -4166,6 +4336,7 @@ static void integrate(const oracle_scene *s, const kparams *k, uint32_t full_x,
             trace_slot3(e, 34u, ray.o);
             trace_slot3(e, 35u, ray.d);
             trace_slot(e, 36u, lo16(ps.flag), hi16(ps.flag), 0.0f);
+            trace_slot(e, 37u, previous_bsdf_pdf, minimum_bsdf_pdf, continuation);
             trace_slot(e, 39u, lo16(ps.visibility), hi16(ps.visibility), 0.0f);
         }
     }
@@ -4529,7 +4700,7 @@ int oracle_scene_intersect(const oracle_scene *s, const psx_ray_query *queries, 
             uint32_t passes = 0u;
             const v3 tr = trace_shadow(s, r, q->visibility, q->source_object, q->source_primitive,
                                        q->light_object, q->light_primitive, q->transparent_depth,
-                                       q->transparent_max, NULL, &passes);
+                                       q->transparent_max, NULL, &passes, NULL);
             h->transmittance[0] = tr.x;
             h->transmittance[1] = tr.y;
             h->transmittance[2] = tr.z;

@@ -192,7 +192,7 @@ struct Lamp {
     uint32_t pad0, pad1;
 };
 constexpr uint32_t kLampNormalize = 1u, kLampEllipse = 2u, kLampSphere = 4u, kLampUseMis = 8u, kLampForward = 32u,
-                   kLampFullSpread = 64u;
+                   kLampFullSpread = 64u, kLampCastShadow = 128u;
 // lamp_bits: visibility mask in bits 0-7, forward-intersectable (not distant,
 // non-zero size) in bit 8, effective MIS in bit 9
 constexpr uint32_t kLampBitIntersectable = 256u, kLampBitMis = 512u;
@@ -222,6 +222,16 @@ struct TreeEmitter {
     float4 trig;
     uint4 identity;  // emitter id, measure flags, 0, 0
 };
+
+// Node of the binary BVH over lamp bounding spheres that the forward-hit
+// search walks: depth-first order, the left child follows its parent, and
+// `miss` is the node after the subtree, so no stack is needed.  `leaf`:
+// 0 for an inner node, else first | count << 24 into SceneView::lamp_order.
+struct LampNode {
+    float lo[3], hi[3];
+    uint32_t miss, leaf;
+};
+constexpr uint32_t kLampBvhMinLamps = 9u;  // scenes with fewer lamps scan them in order
 
 struct DistEntry {
     float cumulative;
@@ -256,6 +266,9 @@ struct SceneView {
     const Lamp *lamps;
     const float4 *lamp_sphere;        // per lamp: centre, radius (forward-hit culling)
     const uint32_t *lamp_bits;
+    const LampNode *lamp_nodes;       // forward-hit BVH over the lamps, or null
+    const uint32_t *lamp_order;       // lamp indices of the BVH leaves
+    uint32_t lamp_node_count;
     uint32_t lamp_count;
     float light_selection_pdf;        // flat-CDF probability of one lamp
     const TreeNode *tree_nodes;

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <array>
 #include <atomic>
 #include <cfloat>
@@ -460,6 +461,69 @@ void refresh_may_emit(HostMaterial &m) {
     m.may_emit = (e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f) && m.emission_sampling != PSX_EMISSION_SAMPLING_NONE;
 }
 
+// Binary BVH over lamp bounding spheres for the forward-hit search (nodes in
+// depth-first order with skip indices; LampNode).  Leaves hold at most two
+// lamps; inner nodes split their lamps at the median centre along the widest
+// centre extent.  Boxes are padded well beyond float rounding of the sphere
+// bounds and of the device's slab test, so a lamp the exact hit test accepts
+// is never culled.
+void build_lamp_bvh(const std::vector<float4> &spheres, std::vector<LampNode> &nodes, std::vector<uint32_t> &order) {
+    constexpr uint32_t kLeaf = 2u;
+    order.resize(spheres.size());
+    for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+    nodes.clear();
+    std::vector<uint32_t> right;  // right child of each inner node (0 for leaves)
+    auto centre = [&](uint32_t i, int a) { return a == 0 ? spheres[i].x : a == 1 ? spheres[i].y : spheres[i].z; };
+    std::function<uint32_t(uint32_t, uint32_t)> build = [&](uint32_t first, uint32_t count) -> uint32_t {
+        LampNode n{};
+        float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (uint32_t k = first; k < first + count; ++k) {
+            const float4 sp = spheres[order[k]];
+            const float r = std::max(sp.w, 0.0f);
+            const float pad = 0.05f * r + 1.0e-3f * (std::fabs(sp.x) + std::fabs(sp.y) + std::fabs(sp.z) + r) + 1.0e-6f;
+            for (int a = 0; a < 3; ++a) {
+                const float c = centre(order[k], a);
+                lo[a] = std::min(lo[a], c - r - pad);
+                hi[a] = std::max(hi[a], c + r + pad);
+                clo[a] = std::min(clo[a], c);
+                chi[a] = std::max(chi[a], c);
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            n.lo[a] = lo[a];
+            n.hi[a] = hi[a];
+        }
+        const uint32_t index = static_cast<uint32_t>(nodes.size());
+        nodes.push_back(n);
+        right.push_back(0u);
+        if (count <= kLeaf) {
+            nodes[index].leaf = first | (count << 24);
+            return index;
+        }
+        int axis = 0;
+        for (int a = 1; a < 3; ++a)
+            if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+        const uint32_t half = count / 2u;
+        std::nth_element(order.begin() + first, order.begin() + first + half, order.begin() + first + count,
+                         [&](uint32_t x, uint32_t y) {
+                             const float cx = centre(x, axis), cy = centre(y, axis);
+                             return cx < cy || (cx == cy && x < y);
+                         });
+        (void)build(first, half);  // the left child is index + 1
+        right[index] = build(first + half, count - half);
+        return index;
+    };
+    if (!order.empty()) (void)build(0u, static_cast<uint32_t>(order.size()));
+    std::function<void(uint32_t, uint32_t)> link = [&](uint32_t i, uint32_t miss) {
+        nodes[i].miss = miss;
+        if (nodes[i].leaf != 0u) return;
+        link(i + 1u, right[i]);
+        link(right[i], miss);
+    };
+    if (!nodes.empty()) link(0u, static_cast<uint32_t>(nodes.size()));
+}
+
 // Shade sort key of an instance: its materials' common key, or the last key
 // when they differ in structure (or it has none).
 uint32_t instance_shade_class(const psx_instance &src, const Mesh &mesh, const std::vector<uint32_t> &slots,
@@ -563,7 +627,7 @@ struct Scene {
     std::vector<Lamp> lamp_list;
     std::vector<uint32_t> mesh_prim_offsets;
     DeviceBuffer nodes, tris, instances_buf, meshes_buf, positions, indices, normals, smooth, slots, tri_slots,
-        overrides_buf, leaves, lamps, lamp_spheres, lamp_bits;
+        overrides_buf, leaves, lamps, lamp_spheres, lamp_bits, lamp_nodes, lamp_order;
     // parameter versions are built, and released, on this stream (see
     // DeviceBuffer::set_stream)
     hipStream_t table_stream = nullptr;
@@ -589,7 +653,7 @@ struct Scene {
         size_t n = 0;
         for (const DeviceBuffer *b : {&nodes, &tris, &instances_buf, &meshes_buf, &positions, &indices, &normals,
                                       &smooth, &slots, &tri_slots, &overrides_buf, &leaves, &lamps, &lamp_spheres,
-                                      &lamp_bits})
+                                      &lamp_bits, &lamp_nodes, &lamp_order})
             n += b->bytes();
         return n;
     }
@@ -676,7 +740,8 @@ Lamp make_lamp(const psx_light &l) {
               ((l.flags & PSX_LIGHT_FLAG_ELLIPSE) ? kLampEllipse : 0u) |
               ((l.flags & PSX_LIGHT_FLAG_SPHERE) ? kLampSphere : 0u) | (effective_mis ? kLampUseMis : 0u) |
               (effective_mis && l.type != PSX_LIGHT_DISTANT ? kLampForward : 0u) |
-              (l.type == PSX_LIGHT_AREA && l.spread >= kPi - 1.0e-6f ? kLampFullSpread : 0u);
+              (l.type == PSX_LIGHT_AREA && l.spread >= kPi - 1.0e-6f ? kLampFullSpread : 0u) |
+              ((l.flags & PSX_LIGHT_FLAG_CAST_SHADOW) ? kLampCastShadow : 0u);
     return g;
 }
 
@@ -1852,6 +1917,19 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         s.lamp_bits.upload(bits);
         v.lamp_sphere = s.lamp_spheres.as<float4>();
         v.lamp_bits = s.lamp_bits.as<uint32_t>();
+        v.lamp_nodes = nullptr;
+        v.lamp_order = nullptr;
+        v.lamp_node_count = 0u;
+        if (spheres.size() >= kLampBvhMinLamps) {
+            std::vector<LampNode> lnodes;
+            std::vector<uint32_t> order;
+            build_lamp_bvh(spheres, lnodes, order);
+            s.lamp_nodes.upload(lnodes);
+            s.lamp_order.upload(order);
+            v.lamp_nodes = s.lamp_nodes.as<LampNode>();
+            v.lamp_order = s.lamp_order.as<uint32_t>();
+            v.lamp_node_count = static_cast<uint32_t>(lnodes.size());
+        }
     }
     v.lamp_count = static_cast<uint32_t>(s.lamp_list.size());
     s.rebuild_emitters(*s.tables);

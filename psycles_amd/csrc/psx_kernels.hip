@@ -35,13 +35,31 @@ __device__ PathResult integrate(const SceneView &s, const LampView &lv, const Ke
     bool shadow = false;
     f3 transmittance = V(1.0f, 1.0f, 1.0f);
     uint32_t tdepth = 0u;
+    bool first_hit = false;  // path traces: the shadow ray's first evaluated surface
     while (cont || shadow) {
         const Ray ray = shadow ? sr.ray : p.ray;
         const TraceQuery q = shadow ? sr.q : path_query(p);
         const Hit hit = closest_hit<COUNT>(s, ray, q, st, nodes, tris);
         ++rays;
         if (shadow) {
+            float4 *se = TRACE ? shadow_slots(slots, sr.event) : nullptr;
+            if (TRACE && se && hit.valid && !first_hit && tdepth < k.transparent_max) {
+                // trace_shadow's first hit (src/luisa/path_tracer_geometry.cpp:136-143)
+                first_hit = true;
+                trace_slot(se, 5u, V(trace_identity(hit.object), trace_identity(hit.cprim), 0.0f));
+                trace_slot(se, 6u, V(hit.t, hit.u, hit.v));
+            }
             if (!shadow_hit(s, k.transparent_max, hit, sr.ray, tdepth, transmittance)) continue;
+            if (TRACE && se) {
+                if (!first_hit) {
+                    trace_slot(se, 5u, V(-1.0f, -1.0f, -1.0f));
+                    trace_slot(se, 6u, V(0.0f, 0.0f, 0.0f));
+                }
+                trace_slot(se, 7u, transmittance);
+                float4 *te = event_slots(slots, sr.event);
+                if (te && (transmittance.x > 0.0f || transmittance.y > 0.0f || transmittance.z > 0.0f))
+                    trace_slot(te, 47u, sr.L * transmittance);
+            }
             apply_shadow<false>(k, p, sr.L, sr.depth, transmittance, sr.dw, sr.gw);
             shadow = false;
             continue;
@@ -52,6 +70,7 @@ __device__ PathResult integrate(const SceneView &s, const LampView &lv, const Ke
             shadow = true;
             transmittance = V(1.0f, 1.0f, 1.0f);
             tdepth = sr.tdepth;
+            first_hit = false;
         }
     }
     PathResult out;

@@ -877,23 +877,6 @@ struct ClosureSet {
 
 __device__ __forceinline__ float avg3(f3 v) { return (v.x + v.y + v.z) / 3.0f; }
 
-// Appends a closure to the set; slot writes through constant indices keep
-// the set in registers.
-__device__ __forceinline__ void append_closure(ClosureSet &cs, uint32_t kind, f3 wt, float sw, float roughness,
-                                               f3 color) {
-#pragma unroll
-    for (uint32_t c = 0; c < kMaxClosures; ++c) {
-        if (c == cs.count) {
-            cs.kind[c] = kind;
-            cs.weight[c] = wt;
-            cs.sw[c] = sw;
-            cs.roughness[c] = roughness;
-            cs.color[c] = color;
-        }
-    }
-    cs.count += 1u;
-}
-
 #ifdef PSX_GENERATED_SURFACE_PROGRAMS
 // Defined by the generated source compiled at run time: the closure set of
 // surface program `program` over the parameter block `p`, statement for
@@ -943,7 +926,18 @@ __device__ void material_closures(const SceneView &s, uint32_t material, f3 ns, 
         const uint32_t kind = diffuse ? KIND_DIFFUSE : KIND_TRANSPARENT;
         const float roughness = diffuse ? p[leaf.aux] : 0.0f;
         const f3 ccolor = diffuse ? color : V(1.0f, 1.0f, 1.0f);
-        append_closure(cs, kind, wt, sw, roughness, ccolor);
+        // slot writes through constant indices keep the set in registers
+#pragma unroll
+        for (uint32_t c = 0; c < kMaxClosures; ++c) {
+            if (c == cs.count) {
+                cs.kind[c] = kind;
+                cs.weight[c] = wt;
+                cs.sw[c] = sw;
+                cs.roughness[c] = roughness;
+                cs.color[c] = ccolor;
+            }
+        }
+        cs.count += 1u;
     }
 }
 
@@ -1080,6 +1074,7 @@ struct BsdfSample {
     Eval eval;
     uint32_t runtime_flags;
     uint32_t selected;
+    float rescaled;  // the lobe random rescaled into the chosen closure (path traces)
 };
 
 __device__ BsdfSample sample_closures(const ClosureSet &cs, const Surface &sf, float u_lobe, float ux, float uy,
@@ -1087,6 +1082,7 @@ __device__ BsdfSample sample_closures(const ClosureSet &cs, const Surface &sf, f
     BsdfSample r;
     r.valid = false;
     r.selected = 0xffffffffu;
+    r.rescaled = 0.0f;
     r.eval.f = V(0.0f, 0.0f, 0.0f);
     r.eval.pdf = 0.0f;
     r.eval.events = 0u;
@@ -1114,7 +1110,10 @@ __device__ BsdfSample sample_closures(const ClosureSet &cs, const Surface &sf, f
         if (i >= cs.count) break;
         const float next = accumulated + weights[i];
         const bool choose = !selected && weights[i] > 0.0f && target < next;
-        if (choose) sel = i;
+        if (choose) {
+            sel = i;
+            r.rescaled = cs.count > 1u ? (target - accumulated) / fmaxf(weights[i], 1.0e-20f) : random;
+        }
         selected |= choose;
         accumulated = next;
     }
@@ -1543,7 +1542,7 @@ __device__ __forceinline__ float point_eval_factor(float r, bool normalize) {
 
 struct LampSample {
     bool valid;
-    f3 direction, position;
+    f3 direction, position, normal;
     float distance, pdf, eval;
 };
 
@@ -1611,6 +1610,7 @@ __device__ LampSample sample_point_or_spot(const Lamp &l, f3 ref, f3 sn, bool ha
             pos = ref + dir * dist;
         }
         const f3 nrm = lamp_normalize(pos - center, neg(dir));
+        s.normal = nrm;
         s.position = nrm * r + center;
         s.direction = dir;
         s.distance = dist;
@@ -1633,6 +1633,7 @@ __device__ LampSample sample_point_or_spot(const Lamp &l, f3 ref, f3 sn, bool ha
         s.valid = (cd2 > 0.0f) && (cosine > 0.0f);
         s.direction = dir;
         s.position = pos;
+        s.normal = neg(dir);
         s.distance = d;
         s.pdf = point_disk_pdf(d2, cosine, r);
     }
@@ -1661,6 +1662,7 @@ __device__ __forceinline__ LampSample sample_distant_light(const Lamp &l, float 
     s.pdf = ns.pdf;
     s.eval = ((l.flags & kLampNormalize) != 0u && finite) ? 1.0f / fmaxf(disk_area, 1.0e-20f) : 1.0f;
     s.position = neg(s.direction);
+    s.normal = neg(s.direction);
     s.distance = RAY_MAXIMUM;
     s.valid = true;
     return s;
@@ -1876,6 +1878,7 @@ __device__ LampSample sample_area_light(const Lamp &l, f3 ref, float rx, float r
     s.pdf = 0.0f;
     s.eval = 0.0f;
     const f3 normal = neg(lamp3(l.axis_z));
+    s.normal = normal;
     if (!(dot(s.position - ref, normal) <= 0.0f)) return s;
     const float tangent = area_tangent_half_spread(l.spread);
     const AreaShape sh = area_shape(l, ref, normal, tangent);
@@ -2047,6 +2050,66 @@ __device__ __forceinline__ bool lamp_may_hit(float4 sph, f3 ro, f3 rd, float tmi
                         fabsf(pd);
     const float r = sph.w * 1.01f + 2.0e-5f * scale;
     return (perp2 <= r * r) && (pd + r > tmin) && (pd - r < tmax);
+}
+
+// The lamp the forward-hit scan below would find among the lamps eligible
+// for this ray (required bits, ray visibility, not `previous`), walking the
+// lamp BVH instead of every lamp: the nearest candidate strictly before
+// `best`, ties to the lowest lamp index (the scan keeps the first of equal
+// distances, since each candidate must lie strictly before the current
+// one).  Candidates are tested against one float beyond the current best so
+// that an equal distance is seen and the tie rule applied.  Returns the lamp
+// or 0xffffffff; `best` and `best_eval` follow the winner.
+__device__ __forceinline__ uint32_t nearest_lamp_bvh(const SceneView &s, const LampView &lv, f3 ro, f3 rd, float tmin,
+                                                     uint32_t required, uint32_t ray_visibility, uint32_t previous,
+                                                     float &best, float &best_eval) {
+    // zero direction components nudged so every slab distance is finite
+    const f3 inv = V(1.0f / (fabsf(rd.x) > 1.0e-30f ? rd.x : copysignf(1.0e-30f, rd.x)),
+                     1.0f / (fabsf(rd.y) > 1.0e-30f ? rd.y : copysignf(1.0e-30f, rd.y)),
+                     1.0f / (fabsf(rd.z) > 1.0e-30f ? rd.z : copysignf(1.0e-30f, rd.z)));
+    uint32_t found = 0xffffffffu;
+    uint32_t node = 0u;
+    while (node < s.lamp_node_count) {
+        const LampNode n = s.lamp_nodes[node];
+        const float limit = best == INFINITY ? best : __uint_as_float(__float_as_uint(best) + 1u);
+        const float x0 = (n.lo[0] - ro.x) * inv.x, x1 = (n.hi[0] - ro.x) * inv.x;
+        const float y0 = (n.lo[1] - ro.y) * inv.y, y1 = (n.hi[1] - ro.y) * inv.y;
+        const float z0 = (n.lo[2] - ro.z) * inv.z, z1 = (n.hi[2] - ro.z) * inv.z;
+        const float t_near = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+        const float t_far = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+        // loose on both ends: rounding of the slab distances never culls
+        const float slack = 1.0e-4f * (fabsf(t_near) + fabsf(t_far)) + 1.0e-6f;
+        const bool overlap = t_near <= t_far + slack && t_far + slack >= 0.0f && t_near - slack <= limit;
+        if (!overlap) {
+            node = n.miss;
+            continue;
+        }
+        if (n.leaf == 0u) {
+            node += 1u;
+            continue;
+        }
+        const uint32_t first = n.leaf & 0xffffffu, count = n.leaf >> 24;
+        for (uint32_t k = first; k < first + count; ++k) {
+            const uint32_t li = s.lamp_order[k];
+            const bool cached = li < lv.cached;
+            const uint32_t bits = cached ? lv.bits[li] : s.lamp_bits[li];
+            const bool eligible =
+                (bits & required) == required && (bits & ray_visibility & 0xffu) != 0u && li != previous;
+            if (!eligible) continue;
+            const float lim = best == INFINITY ? best : __uint_as_float(__float_as_uint(best) + 1u);
+            const float4 sph = cached ? lv.sphere[li] : s.lamp_sphere[li];
+            if (!lamp_may_hit(sph, ro, rd, tmin, lim)) continue;
+            float distance, eval;
+            if (!lamp_candidate(s.lamps[li], ro, rd, tmin, lim, distance, eval)) continue;
+            if (distance < best || (distance == best && found != 0xffffffffu && li < found)) {
+                found = li;
+                best = distance;
+                best_eval = eval;
+            }
+        }
+        node = n.miss;
+    }
+    return found;
 }
 
 // ===========================================================================
@@ -2407,6 +2470,7 @@ struct PathVars {
     PathState ps;
     uint32_t sample, rng_hash, step, bits;
     LightSums lp;   // only read and written by the light-pass kernels
+    float min_pdf;  // path traces only: smallest non-transparent BSDF pdf so far
 };
 
 struct ShadowRay {
@@ -2417,6 +2481,7 @@ struct ShadowRay {
     uint32_t tdepth; // transparent bounces already taken
     bool active;
     f3 dw, gw;       // light passes: diffuse / glossy split of the contribution
+    uint32_t event;  // path traces only: the surface event that sampled the light
 };
 
 // light-pass routing (src/luisa/path_tracer_lighting.cpp:107-191)
@@ -2489,6 +2554,7 @@ __device__ __forceinline__ void init_path(const KernelParams &k, const float4 *s
     p.sample = sample;
     p.step = 0u;
     p.bits = PATH_PREV_DELTA | (k.max_path_steps == 0u ? PATH_DONE : 0u);
+    p.min_pdf = FLT_MAX;
 }
 
 // transparent shadow continuation (src/luisa/path_tracer_geometry.cpp:75-174):
@@ -2540,6 +2606,7 @@ constexpr uint32_t kTraceGlobalSlots = 8u;
 constexpr uint32_t kTraceEventSlots = 72u;
 constexpr uint32_t kTraceMaxEvents = 4u;
 constexpr uint32_t kTraceSlotCount = 328u;
+constexpr uint32_t kTraceShadowBase = 296u, kTraceShadowSlots = 8u;
 
 __device__ __forceinline__ void trace_slot(float4 *slots, uint32_t index, f3 v) {
     slots[index] = make_float4(v.x, v.y, v.z, 1.0f);
@@ -2548,6 +2615,51 @@ __device__ __forceinline__ void trace_slot(float4 *slots, uint32_t index, f3 v) 
 // trace_uint32: a 32-bit word as two exact 16-bit halves
 __device__ __forceinline__ f3 trace_halves(uint32_t v, float z) {
     return V((float)(v & 0xffffu), (float)(v >> 16), z);
+}
+
+// surface-event slots of event `step`, or null past the traced events
+__device__ __forceinline__ float4 *event_slots(float4 *slots, uint32_t step) {
+    return slots && step < kTraceMaxEvents ? slots + kTraceGlobalSlots + step * kTraceEventSlots : nullptr;
+}
+
+// shadow-event slots of event `step`, or null past the traced events
+__device__ __forceinline__ float4 *shadow_slots(float4 *slots, uint32_t step) {
+    return slots && step < kTraceMaxEvents ? slots + kTraceShadowBase + step * kTraceShadowSlots : nullptr;
+}
+
+// an object or primitive identity, -1 when invalid
+// (src/luisa/path_kernel_direct_light_trace.cpp:6-11)
+__device__ __forceinline__ float trace_identity(uint32_t v) { return v != 0xffffffffu ? (float)v : -1.0f; }
+
+// Cycles' exclusion flags of an object's ray visibility, not a shadow catcher
+// (src/luisa/cycles_shader_identity.h:64-97)
+__device__ __forceinline__ uint32_t object_visibility_flags(uint32_t v) {
+    return ((v & VIS_CAMERA) ? 0u : (1u << 24)) | ((v & VIS_DIFFUSE) ? 0u : (1u << 27)) |
+           ((v & VIS_GLOSSY) ? 0u : (1u << 26)) | ((v & VIS_TRANSMISSION) ? 0u : (1u << 25)) |
+           ((v & VIS_VOLUME_SCATTER) ? 0u : (1u << 23)) | (1u << 22);
+}
+
+// Cycles shader identity of an emissive triangle: shader index, cast shadow,
+// smooth normal, use MIS and the object's visibility exclusions
+// (src/luisa/cycles_shader_identity.h:123-147)
+__device__ __forceinline__ uint32_t emissive_shader_id(const SceneView &s, const Emitter &em) {
+    const Instance &inst = s.instances[em.instance];
+    const Mesh &mesh = s.meshes[inst.mesh];
+    const bool smooth = mesh.smooth_offset != kNone && s.smooth[mesh.smooth_offset + em.prim] != 0u;
+    return em.material | (1u << 30) | (smooth ? (1u << 31) : 0u) | (1u << 28) |
+           object_visibility_flags(inst.visibility & 0x3fu);
+}
+
+// Cycles KernelLightType of a lamp (src/luisa/cycles_shader_identity.h:37-53)
+__device__ __forceinline__ uint32_t cycles_light_type(uint32_t type) {
+    return type == kLampPoint ? 0u : type == kLampDistant ? 1u : type == kLampArea ? 3u : 4u;
+}
+
+// Cycles ClosureType of closure c (src/luisa/graph_surface_scattering.cpp:373-420;
+// include/psycles/luisa/cycles_closure.h:18-32)
+__device__ __forceinline__ uint32_t cycles_closure_type(const ClosureSet &cs, uint32_t c) {
+    if (cs.kind[c] == KIND_TRANSPARENT) return 30u;
+    return cs.roughness[c] < 1.0e-5f ? 2u : 3u;
 }
 
 // global slots written after the camera ray (src/luisa/path_kernel_setup.cpp:540-560)
@@ -2565,6 +2677,21 @@ __device__ __forceinline__ void trace_globals(const KernelParams &k, const float
     trace_slot(slots, 4u, p.ray.o);
     trace_slot(slots, 5u, p.ray.d);
     trace_slot(slots, 6u, V(p.ray.tmin, p.ray.tmax, 0.5f));
+}
+
+// The shadow ray of a NEE sample in the path trace: ray, range, source and
+// light identities (record_shadow, src/luisa/path_kernel_direct_light_trace.cpp:152-208);
+// the first hit and transmittance follow once it is traced.
+__device__ __forceinline__ void trace_shadow_ray(float4 *slots, uint32_t step, ShadowRay &sr, bool skip_self,
+                                                 bool cast_shadow) {
+    sr.event = step;
+    float4 *e = shadow_slots(slots, step);
+    if (!e) return;
+    trace_slot(e, 0u, sr.ray.o);
+    trace_slot(e, 1u, sr.ray.d);
+    trace_slot(e, 2u, V(sr.ray.tmin, sr.ray.tmax, cast_shadow ? 1.0f : 0.0f));
+    trace_slot(e, 3u, V(trace_identity(sr.q.src_obj), trace_identity(sr.q.src_prim), skip_self ? 1.0f : 0.0f));
+    trace_slot(e, 4u, V(trace_identity(sr.q.light_obj), trace_identity(sr.q.light_prim), 0.0f));
 }
 
 // One path step.  Returns true when the path continues with p.ray; sr.active
@@ -2600,7 +2727,10 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
             uint32_t found = 0xffffffffu;
             float best_eval = 0.0f;
             const uint32_t required = kLampBitIntersectable | (camera_path ? 0u : kLampBitMis);
-            for (uint32_t li = 0; li < s.lamp_count; ++li) {
+            if (s.lamp_node_count > 0u)
+                found = nearest_lamp_bvh(s, lv, p.ray.o, p.ray.d, p.ray.tmin, required, ray_visibility, previous,
+                                         event_distance, best_eval);
+            for (uint32_t li = 0; li < (s.lamp_node_count > 0u ? 0u : s.lamp_count); ++li) {
                 const bool cached = li < lv.cached;
                 const uint32_t bits = cached ? lv.bits[li] : s.lamp_bits[li];
                 const bool eligible =
@@ -2690,6 +2820,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
     if (LP) lp_route(p.lp, false, emitted, p.ps.flag, p.ps.bounce);
     if (terminate_on_next_surface) return false;
     // Russian roulette (cycles_path_state.h:416-446)
+    float continuation = 1.0f;
     {
         const bool transparent = (p.ps.flag & FLAG_TRANSPARENT) != 0u;
         const bool before_minimum =
@@ -2697,6 +2828,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
         const f3 t = p.throughput;
         const float roulette = fminf(sqrtf(fmaxf(fabsf(t.x), fmaxf(fabsf(t.y), fabsf(t.z)))), 1.0f);
         const float prob = before_minimum ? 1.0f : roulette;
+        continuation = prob;
         if (TRACE && ev) ev[7] = __float_as_uint(prob);
         if (prob <= 0.0f) {
             if (TRACE && ev) ev[0] |= 8u;
@@ -2736,6 +2868,12 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
             trace_slot(e, 15u, V(ul.x, ul.y, ul.z));
             const float4 ub = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 3u);
             trace_slot(e, 16u, V(ub.x, ub.y, ub.z));
+            // the shading point's closures (src/luisa/path_kernel_surface_shading.cpp:350-380)
+            for (uint32_t c = 0; c < cs.count && c < kMaxClosures; ++c) {
+                trace_slot(e, 48u + 3u * c, V((float)c, (float)cycles_closure_type(cs, c), cs.sw[c]));
+                trace_slot(e, 49u + 3u * c, cs.weight[c]);
+                trace_slot(e, 50u + 3u * c, cs.normal);
+            }
         }
     }
     if (p.ps.bounce == 0u) {
@@ -2808,6 +2946,22 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
             const LightSample ls = sample_triangle(sf.position, p0, p1, p2, ul.x, ul.y);
             const bool side = sampled_side(em.emission_sampling, egn, ls.direction);
             const float lpdf = ls.pdf * entry.selection_pdf;
+            float4 *te = TRACE ? event_slots(slots, p.step) : nullptr;
+            if (TRACE && te) {
+                // record_sample / record_failed_sample
+                // (src/luisa/path_kernel_direct_light_trace.cpp:55-105)
+                if (ls.valid && side && earea > 0.0f && lpdf > 0.0f) {
+                    trace_slot(te, 17u, V(5.0f, (float)light, (float)(einst.prim_offset + em.prim)));
+                    trace_slot(te, 18u, V((float)em.instance, 0.0f, 0.0f));
+                    trace_slot(te, 40u, trace_halves(emissive_shader_id(s, em), 0.0f));
+                    trace_slot(te, 19u, V(lpdf, entry.selection_pdf, 1.0f));
+                    trace_slot(te, 20u, ls.direction);
+                    trace_slot(te, 21u, ls.position);
+                    trace_slot(te, 22u, egn);
+                } else {
+                    trace_slot(te, 17u, V(-1.0f, -1.0f, -1.0f));
+                }
+            }
             if (ls.valid && side && earea > 0.0f && lpdf > 0.0f) {
                 const uint32_t eobj = em.instance;
                 const uint32_t eprim = einst.prim_offset + em.prim;
@@ -2821,11 +2975,22 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                     const Eval evl = evaluate_mixture(cs, sf.incoming, sf.ns, ls.direction, lobe_mask);
                     const float mis = nee_light_weight(k, lpdf, evl.pdf);
                     const f3 weighted = (evl.f * le) * (mis / lpdf);
+                    if (TRACE && te) {
+                        trace_slot(te, 23u, V(ls.distance, evl.pdf, mis));
+                        trace_slot(te, 41u, evl.f);
+                        trace_slot(te, 42u, evl.f);
+                        trace_slot(te, 43u, V(0.0f, 0.0f, 0.0f));
+                        trace_slot(te, 44u, weighted);
+                    }
                     if (any_nonzero(weighted)) {
                         const float u_lightterm =
                             sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 2u).x;
                         const float rr = light_roulette(weighted, u_lightterm, k.light_inv_rr_threshold);
                         const f3 surviving = weighted * rr;
+                        if (TRACE && te) {
+                            trace_slot(te, 45u, V(1.0f, 1.0f, 1.0f));
+                            trace_slot(te, 46u, p.throughput * surviving);
+                        }
                         if (any_nonzero(surviving)) {
                             const ShadowOrigin sso = surface_shadow_origin(s, sf, hit, ls.direction);
                             const f3 so = sso.position;
@@ -2845,6 +3010,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                             sr.tdepth = p.ps.transparent_bounce;
                             sr.active = true;
                             if (LP) nee_split(p.lp, evl.f, p.ps.bounce, sr.dw, sr.gw);
+                            if (TRACE) trace_shadow_ray(slots, p.step, sr, sso.skip_self, true);
                         }
                     }
                 }
@@ -2865,16 +3031,43 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                                                                                has_transmission, ul.x, ul.y);
             const f3 light_radiance = lamp3(l.color) * (l.power * ls.eval);
             const float light_pdf = ls.pdf * entry.selection_pdf;
+            float4 *te = TRACE ? event_slots(slots, p.step) : nullptr;
+            if (TRACE && te) {
+                // record_sample / record_failed_sample
+                // (src/luisa/path_kernel_nee_analytic.cpp:233-249, 413)
+                if (!reached_max_bounces && ls.valid && light_pdf > 0.0f) {
+                    trace_slot(te, 17u, V((float)cycles_light_type(l.type), (float)light, (float)entry.index));
+                    trace_slot(te, 18u, V((float)0xffffffffu, 0.0f, 0.0f));
+                    trace_slot(te, 40u, trace_halves(0xffffffffu, 0.0f));
+                    trace_slot(te, 19u, V(light_pdf, entry.selection_pdf, ls.eval));
+                    trace_slot(te, 20u, ls.direction);
+                    trace_slot(te, 21u, ls.position);
+                    trace_slot(te, 22u, ls.normal);
+                } else {
+                    trace_slot(te, 17u, V(-1.0f, -1.0f, -1.0f));
+                }
+            }
             if (!reached_max_bounces && ls.valid && light_pdf > 0.0f) {
                 const Eval evl = evaluate_mixture(cs, sf.incoming, sf.ns, ls.direction, lobe_mask);
                 const float cycles_mis = nee_light_weight(k, light_pdf, evl.pdf);
                 const float mis = (l.flags & kLampForward) != 0u ? cycles_mis : 1.0f;
                 const f3 weighted = (evl.f * light_radiance) * (mis / fmaxf(light_pdf, 1.0e-20f));
+                if (TRACE && te) {
+                    trace_slot(te, 23u, V(ls.distance, evl.pdf, cycles_mis));
+                    trace_slot(te, 41u, evl.f);
+                    trace_slot(te, 42u, evl.f);
+                    trace_slot(te, 43u, V(0.0f, 0.0f, 0.0f));
+                    trace_slot(te, 44u, weighted);
+                }
                 if (any_nonzero(weighted)) {
                     const float u_lightterm =
                         sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 2u).x;
                     const float rr = light_roulette(weighted, u_lightterm, k.light_inv_rr_threshold);
                     const f3 surviving = weighted * rr;
+                    if (TRACE && te) {
+                        trace_slot(te, 45u, V(1.0f, 1.0f, 1.0f));
+                        trace_slot(te, 46u, p.throughput * surviving);
+                    }
                     if (any_nonzero(surviving)) {
                         const ShadowOrigin sso = surface_shadow_origin(s, sf, hit, ls.direction);
                         const f3 so = sso.position;
@@ -2893,6 +3086,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
                         sr.tdepth = p.ps.transparent_bounce;
                         sr.active = true;
                         if (LP) nee_split(p.lp, evl.f, p.ps.bounce, sr.dw, sr.gw);
+                        if (TRACE) trace_shadow_ray(slots, p.step, sr, sso.skip_self, (l.flags & kLampCastShadow) != 0u);
                     }
                 }
             }
@@ -2902,6 +3096,23 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
     const float4 ub = sample4(sobol, k.sequence_size, p.sample, p.rng_hash, p.ps.rng_offset + 3u);
     const BsdfSample bs = sample_closures(cs, sf, ub.z, ub.x, ub.y, lobe_mask);
     if (TRACE && ev) ev[5] = bs.selected;
+    if (TRACE && bs.selected < cs.count) {
+        // the sampled closure and BSDF sample (src/luisa/path_kernel_surface_scatter.cpp:95-130)
+        float4 *te = event_slots(slots, p.step);
+        if (te) {
+            const uint32_t c = bs.selected;
+            trace_slot(te, 24u, V((float)c, (float)cycles_closure_type(cs, c), cs.sw[c]));
+            trace_slot(te, 25u, V(ub.x, ub.y, bs.rescaled));
+            trace_slot(te, 26u, cs.weight[c]);
+            trace_slot(te, 27u, cs.normal);
+            trace_slot(te, 28u, V(bs.eval.pdf, bs.eval.pdf, (float)label_from_events(bs.eval.events)));
+            trace_slot(te, 29u, bs.wi);
+            trace_slot(te, 30u, bs.eval.f);
+            const bool sampled = bs.valid;
+            const float r = sampled && cs.kind[c] == KIND_DIFFUSE ? 1.0f : 0.0f;
+            trace_slot(te, 31u, V(r, r, 1.0f));
+        }
+    }
     if (!bs.valid || bs.eval.pdf <= 0.0f) return false;
     if (TRACE && ev) ev[0] |= 32u;
     const bool transparent = (bs.eval.events & EV_TRANSPARENT) != 0u;
@@ -2917,6 +3128,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
     next_surface(p.ps, label_from_events(bs.eval.events), bs.runtime_flags, k);
     if (!transparent) {
         p.prev_pdf = bs.eval.pdf;
+        if (TRACE) p.min_pdf = fminf(p.min_pdf, bs.eval.pdf);
         p.bits = singular ? (p.bits | PATH_PREV_DELTA) : (p.bits & ~PATH_PREV_DELTA);
         p.mis_normal = sf.ns;
         const f3 nd = normalize(bs.wi);
@@ -2937,6 +3149,7 @@ __device__ bool path_step(const SceneView &s, const LampView &lv, const KernelPa
         trace_slot(e, 34u, p.ray.o);
         trace_slot(e, 35u, p.ray.d);
         trace_slot(e, 36u, trace_halves(p.ps.flag, 0.0f));
+        trace_slot(e, 37u, V(p.prev_pdf, p.min_pdf, continuation));
         trace_slot(e, 39u, trace_halves(p.ps.visibility, 0.0f));
     }
     p.step += 1u;

@@ -22,6 +22,23 @@ constexpr uint32_t kShadeThreads = 256u;  // kShadeBlock of the wavefront kernel
 
 bool same_leaf(const Leaf &a, const Leaf &b) { return std::memcmp(&a, &b, sizeof(Leaf)) == 0; }
 
+// Appends a closure the way the interpreter does: slot writes through
+// constant indices, which keep the set in registers.
+void emit_append(std::ostringstream &o, const char *kind, const char *weight, const char *sw,
+                 const std::string &roughness, const char *color) {
+    o << "#pragma unroll\n"
+      << "            for (uint32_t c = 0; c < kMaxClosures; ++c) {\n"
+      << "                if (c == cs.count) {\n"
+      << "                    cs.kind[c] = " << kind << ";\n"
+      << "                    cs.weight[c] = " << weight << ";\n"
+      << "                    cs.sw[c] = " << sw << ";\n"
+      << "                    cs.roughness[c] = " << roughness << ";\n"
+      << "                    cs.color[c] = " << color << ";\n"
+      << "                }\n"
+      << "            }\n"
+      << "            cs.count += 1u;\n";
+}
+
 // One leaf's statements: the body of the interpreter's leaf loop
 // (material_closures in psx_path.h) with the leaf's fields as constants.
 void emit_leaf(std::ostringstream &o, const Leaf &leaf, uint32_t index) {
@@ -48,13 +65,15 @@ void emit_leaf(std::ostringstream &o, const Leaf &leaf, uint32_t index) {
           << "        if (!(avg3(wt) >= 1.0e-5f)) wt = V(0.0f, 0.0f, 0.0f);\n"
           << "        const float sw = fabsf(avg3(vmax(wt, V(0.0f, 0.0f, 0.0f))));\n"
           << "        wt = vmax(wt, V(0.0f, 0.0f, 0.0f));\n"
-          << "        if (sw >= 1.0e-5f && cs.count < kMaxClosures)\n"
-          << "            append_closure(cs, KIND_DIFFUSE, wt, sw, p[" << leaf.aux << "u], color);\n";
+          << "        if (sw >= 1.0e-5f && cs.count < kMaxClosures) {\n";
+        emit_append(o, "KIND_DIFFUSE", "wt", "sw", "p[" + std::to_string(leaf.aux) + "u]", "color");
+        o << "        }\n";
     } else {
         o << "        const f3 wt = color * w;\n"
           << "        const float sw = fabsf(avg3(wt));\n"
-          << "        if (sw >= 1.0e-5f && cs.count < kMaxClosures)\n"
-          << "            append_closure(cs, KIND_TRANSPARENT, wt, sw, 0.0f, V(1.0f, 1.0f, 1.0f));\n";
+          << "        if (sw >= 1.0e-5f && cs.count < kMaxClosures) {\n";
+        emit_append(o, "KIND_TRANSPARENT", "wt", "sw", "0.0f", "V(1.0f, 1.0f, 1.0f)");
+        o << "        }\n";
     }
     o << "    }\n";
 }

@@ -5,8 +5,17 @@
 one pixel (``pixel_y`` counted from the lower row of the full image, Cycles'
 film convention) and one sample, 328 float4 slots laid out as
 include/psycles/luisa/path_trace_schema.h:10-121 defines them.  The device
-writes the global slots 0-6 and, for the first 4 surface events, event slots
-0-16, 32-36, 38 and 39; w = 1 marks a written slot.  `write_raw_path_trace` writes the
+writes the global slots 0-6 and, for the first 4 surface events, every event
+slot 0-47 the path reaches (state, hit and surface; light sample, NEE
+evaluation, light shader and contribution,
+src/luisa/path_kernel_direct_light_trace.cpp; closure pick and BSDF sample,
+post-scatter state and MIS, src/luisa/path_kernel_surface_scatter.cpp), the
+closure slots 48-71 (index, Cycles closure type and sample weight; weight;
+normal) and the shadow-event slots 296-327 (shadow ray, range and
+cast-shadow, source and light identities, first hit, transmittance); w = 1
+marks a written slot.  Lamps have no object or shader index in this ABI:
+their light_id object is 4294967295 and their light_shader the halves of
+0xffffffff, what the reference writes for a light without them.  `write_raw_path_trace` writes the
 JSON document of examples/render_blender_scene.cpp:77-112
 ("psycles.cycles-path-trace-raw"); `decode` names the written slots.
 """

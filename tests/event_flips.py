@@ -89,6 +89,7 @@ FLIP_BUDGET: dict = {
     ("light tree cornell_lamps", 0): 0, ("light tree cornell_lamps", 3): 0,
     ("light tree many_lights", 0): 0, ("light tree many_lights", 3): 0,
     ("cornell area lamp", 0): 0, ("cornell area lamp", 5): 0,
+    ("lamp grid", 0): 0, ("lamp grid", 3): 0,
 }
 
 

@@ -136,3 +136,114 @@ def test_decode_and_raw_round_trip(cornell, tmp_path):
     assert path_trace.index(0, "state_depth") == 8 and path_trace.index(3, "nee_contribution") == 8 + 3 * 72 + 47
     assert path_trace.closure_index(1, 7, 2) == 8 + 72 + 48 + 23
     assert path_trace.shadow_index(3, "shadow_transmittance") == 296 + 3 * 8 + 7 == 327
+
+
+def shadow(slots, i, slot):
+    return slots[296 + i * 8 + slot]
+
+
+def traced_events(scene, s, pixels, sample=2, total=16):
+    for x, y in pixels:
+        slots, records = scene.path_trace(s, x, y, sample, total)
+        for i in range(MAX_EVENTS):
+            if event(slots, i, 0)[3] == 1.0:
+                yield slots, records, i
+
+
+@pytest.mark.parametrize("case", ["cornell", "cornell_lamps"])
+def test_light_sample_and_shadow_slots(case):
+    """The NEE record ties to the event log, the sampled point and the
+    shadow ray: the shadow ray runs from its origin to the sampled light
+    position (emissive triangles), its transmittance lies in [0, 1], and the
+    contribution is the unshadowed estimate times that transmittance."""
+    snap = scenes.cornell_box() if case == "cornell" else scenes.cornell_lamps()
+    scene, s = ob.OracleScene(snap), scenes.cornell_settings(24, 16)
+    sampled = shadows = contributions = 0
+    for slots, records, i in traced_events(scene, s, [(x, y) for y in range(0, 16, 3) for x in range(0, 24, 3)]):
+        meta = event(slots, i, 17)
+        if meta[3] != 1.0 or meta[0] < 0.0:
+            continue
+        sampled += 1
+        assert int(meta[1]) == records[i, abi.TRACE_LIGHT]        # the light the event log chose
+        pdf = event(slots, i, 19)
+        assert pdf[0] > 0.0 and 0.0 < pdf[1] <= 1.0
+        d = event(slots, i, 20)[:3]
+        assert abs(np.linalg.norm(d) - 1.0) < 1e-5
+        nee_eval = event(slots, i, 23)
+        if nee_eval[3] == 1.0:
+            assert 0.0 <= nee_eval[2] <= 1.0                      # MIS weight
+        sh = shadow(slots, i, 0)
+        if sh[3] != 1.0:
+            continue
+        shadows += 1
+        o, dr, rng = sh[:3], shadow(slots, i, 1)[:3], shadow(slots, i, 2)
+        t = shadow(slots, i, 7)[:3]
+        assert np.all((t >= 0.0) & (t <= 1.0))
+        if meta[0] == 5.0:                                         # emissive triangle: ray ends on the sample
+            assert np.allclose(o + dr * rng[1], event(slots, i, 21)[:3], atol=2e-4)
+            assert shadow(slots, i, 4)[0] == event(slots, i, 18)[0]   # the light's own object is excluded
+            assert shadow(slots, i, 4)[1] == meta[2]                  # and its primitive
+        if np.any(t > 0.0):
+            contributions += 1
+            c = event(slots, i, 47)[:3]
+            assert np.allclose(c, event(slots, i, 46)[:3] * t, rtol=1e-6, atol=0.0)
+        else:
+            assert event(slots, i, 47)[3] == 0.0
+    assert sampled > 20 and shadows > 10 and contributions > 5
+
+
+def test_closure_and_scatter_slots():
+    """Closure slots follow the closure count of slot 10; the sampled
+    closure is one of them, with its weight and normal; the BSDF pdf is the
+    one post_mis keeps and the minimum never exceeds it."""
+    snap = scenes.instanced_icospheres(frequency=4, instances=4, box=10.0)
+    scene, s = ob.OracleScene(snap), scenes.instanced_settings(24, 16)
+    picked = 0
+    for slots, records, i in traced_events(scene, s, [(x, y) for y in range(0, 16, 2) for x in range(0, 24, 2)]):
+        count = int(event(slots, i, 10)[2])
+        for c in range(8):
+            written = event(slots, i, 48 + 3 * c)[3] == 1.0
+            assert written == (c < count)
+            if written:
+                assert event(slots, i, 48 + 3 * c)[0] == c and event(slots, i, 48 + 3 * c)[1] in (2.0, 3.0, 30.0)
+        pick = event(slots, i, 24)
+        if pick[3] != 1.0:
+            continue
+        picked += 1
+        c = int(pick[0])
+        assert c == records[i, abi.TRACE_CLOSURE] and c < count
+        assert pick[1] == event(slots, i, 48 + 3 * c)[1] and pick[2] == event(slots, i, 48 + 3 * c)[2]
+        assert np.array_equal(event(slots, i, 26), event(slots, i, 49 + 3 * c))
+        assert 0.0 <= event(slots, i, 25)[2] < 1.0                # rescaled lobe random
+        meta = event(slots, i, 28)
+        assert meta[0] == meta[1]
+        r = event(slots, i, 31)[:3]
+        assert tuple(r) == ((1.0, 1.0, 1.0) if pick[1] in (2.0, 3.0) else (0.0, 0.0, 1.0))
+        post = event(slots, i, 37)
+        if post[3] == 1.0:
+            assert post[1] <= post[0] or post[1] == np.float32(3.4028235e38)
+            assert 0.0 < post[2] <= 1.0                           # continuation probability
+            if pick[1] != 30.0:                                   # a non-transparent scatter sets the pdf
+                assert post[0] == meta[0]
+    assert picked > 30
+
+
+def test_analytic_lamp_record():
+    """Lamps carry Cycles' light type, no object and no shader id, and the
+    light normal of their sampled point."""
+    snap = scenes.cornell_lamps(with_quad=False)
+    scene, s = ob.OracleScene(snap), scenes.cornell_settings(24, 16)
+    types = set()
+    for slots, records, i in traced_events(scene, s, [(x, y) for y in range(0, 16, 2) for x in range(0, 24, 2)]):
+        meta = event(slots, i, 17)
+        if meta[3] != 1.0 or meta[0] < 0.0:
+            continue
+        types.add(int(meta[0]))
+        assert event(slots, i, 18)[0] == np.float32(4294967295.0)
+        assert tuple(event(slots, i, 40)[:3]) == (65535.0, 65535.0, 0.0)
+        n = event(slots, i, 22)[:3]
+        assert abs(np.linalg.norm(n) - 1.0) < 1e-4
+        sh = shadow(slots, i, 4)
+        if sh[3] == 1.0:
+            assert sh[0] == -1.0 and sh[1] == -1.0
+    assert types and types <= {0, 1, 3, 4}
