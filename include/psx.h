@@ -669,7 +669,15 @@ int psx_scene_intersect_device_noalloc(psx_scene *scene, const psx_ray_query *de
  *                           outgoing[3] (13)
  *                       out cosine-weighted intensity[3], pdf (4)
  *                       (Lambert / Oren-Nayar,
- *                        src/luisa/graph_surface_scattering.cpp:463-506) */
+ *                        src/luisa/graph_surface_scattering.cpp:463-506)
+ *   AREA_SAMPLE         in  lamp centre[3], unit axes x[3], y[3], z[3],
+ *                           edge lengths u and v, spread, shape (bit 0
+ *                           ellipse, bit 1 normalized power), reference
+ *                           point[3], random[2] (21)
+ *                       out position[3], direction[3], distance, pdf,
+ *                           evaluation factor, valid (10)
+ *                       (AreaLightSampling::from_position,
+ *                        src/luisa/area_light_sampling.cpp) */
 enum psx_probe_function {
     PSX_PROBE_FILM_CLAMP = 0,
     PSX_PROBE_T_OFFSET = 1,
@@ -678,7 +686,8 @@ enum psx_probe_function {
     PSX_PROBE_TRIANGLE_SAMPLE = 4,
     PSX_PROBE_TRIANGLE_PDF = 5,
     PSX_PROBE_DIFFUSE_EVAL = 6,
-    PSX_PROBE_SURFACE_SHADOW_ORIGIN = 7
+    PSX_PROBE_SURFACE_SHADOW_ORIGIN = 7,
+    PSX_PROBE_AREA_SAMPLE = 8
 };
 /* Input and output floats per record of a probe function; 0 if unknown. */
 uint32_t psx_probe_input_width(uint32_t function);

@@ -4737,12 +4737,12 @@ static mat4 affine12(const float *a) {
 }
 
 uint32_t oracle_probe_input_width(uint32_t function) {
-    static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u};
+    static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u, 21u};
     return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
 }
 
 uint32_t oracle_probe_output_width(uint32_t function) {
-    static const uint32_t w[] = {3u, 1u, 4u, 4u, 10u, 3u, 4u, 4u};
+    static const uint32_t w[] = {3u, 1u, 4u, 4u, 10u, 3u, 4u, 4u, 10u};
     return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
 }
 
@@ -4819,6 +4819,30 @@ int oracle_probe(uint32_t function, const float *inputs, uint32_t count, float *
             const v3 value = diffuse_intensity(&c, load3(in + 7), load3(in + 10));
             out[0] = value.x; out[1] = value.y; out[2] = value.z;
             out[3] = fmaxf(vdot(c.normal, load3(in + 10)), 0.0f) * INV_PI_F;
+            break;
+        }
+        case PSX_PROBE_AREA_SAMPLE: {
+            olamp l;
+            memset(&l, 0, sizeof(l));
+            l.type = PSX_LIGHT_AREA;
+            l.position = load3(in);
+            l.axis_x = load3(in + 3);
+            l.axis_y = load3(in + 6);
+            l.axis_z = load3(in + 9);
+            l.size_u = in[12];
+            l.size_v = in[13];
+            l.spread = in[14];
+            const uint32_t shape = (uint32_t)in[15];
+            l.flags = ((shape & 1u) ? LAMP_ELLIPSE : 0u) | ((shape & 2u) ? LAMP_NORMALIZE : 0u) |
+                      (l.spread >= 3.14159265358979f - 1.0e-6f ? LAMP_FULL_SPREAD : 0u);
+            const v2 rnd = {in[19], in[20]};
+            const lamp_sample_t s = sample_area_light(&l, load3(in + 16), rnd);
+            out[0] = s.position.x; out[1] = s.position.y; out[2] = s.position.z;
+            out[3] = s.direction.x; out[4] = s.direction.y; out[5] = s.direction.z;
+            out[6] = s.distance;
+            out[7] = s.pdf;
+            out[8] = s.eval;
+            out[9] = s.valid ? 1.0f : 0.0f;
             break;
         }
         default:

@@ -172,6 +172,37 @@ __global__ void probe_kernel(uint32_t function, const float *inputs, uint32_t in
         out[3] = fmaxf(dot(n, in3(in + 10)), 0.0f) * INV_PI;
         break;
     }
+    case PSX_PROBE_AREA_SAMPLE: {
+        Lamp l{};
+        l.type = kLampArea;
+        const f3 c = in3(in), ax = in3(in + 3), ay = in3(in + 6), az = in3(in + 9);
+        l.position[0] = c.x;
+        l.position[1] = c.y;
+        l.position[2] = c.z;
+        l.axis_x[0] = ax.x;
+        l.axis_x[1] = ax.y;
+        l.axis_x[2] = ax.z;
+        l.axis_y[0] = ay.x;
+        l.axis_y[1] = ay.y;
+        l.axis_y[2] = ay.z;
+        l.axis_z[0] = az.x;
+        l.axis_z[1] = az.y;
+        l.axis_z[2] = az.z;
+        l.size_u = in[12];
+        l.size_v = in[13];
+        l.spread = in[14];
+        const uint32_t shape = static_cast<uint32_t>(in[15]);
+        l.flags = ((shape & 1u) ? kLampEllipse : 0u) | ((shape & 2u) ? kLampNormalize : 0u) |
+                  (l.spread >= 3.14159265358979f - 1.0e-6f ? kLampFullSpread : 0u);
+        const LampSample s = sample_area_light(l, in3(in + 16), in[19], in[20]);
+        out3(out, s.position);
+        out3(out + 3, s.direction);
+        out[6] = s.distance;
+        out[7] = s.pdf;
+        out[8] = s.eval;
+        out[9] = s.valid ? 1.0f : 0.0f;
+        break;
+    }
     default:
         break;
     }

@@ -104,6 +104,25 @@ def _probe_inputs(function, rng, n):
             if area2 > 0.25 * longest and facing > 0.3:
                 rows.append([*ref, *p.reshape(-1), *rng.uniform(0, 1, 2)])
         return np.asarray(rows)
+    if function == abi.PROBE_AREA_SAMPLE:
+        # rectangles and ellipses of 0.2 to 1.5 units, 1.5 to 5 units from
+        # the shading point, facing it at any angle down to nearly edge-on,
+        # with full or narrowed spread
+        rows = []
+        while len(rows) < n:
+            ref = rng.normal(0, 0.2, 3)
+            c = rng.normal(0, 1, 3)
+            c *= rng.uniform(1.5, 5.0) / np.linalg.norm(c)
+            q = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+            ax, ay, az = q[:, 0], q[:, 1], q[:, 2]
+            view = (c - ref) / np.linalg.norm(c - ref)
+            if np.dot(view, az) < 0.0:
+                az = -az
+                ay = -ay
+            spread = np.pi if rng.random() < 0.5 else rng.uniform(0.3, 3.0)
+            shape = float(int(rng.random() < 0.3) | (int(rng.random() < 0.5) << 1))
+            rows.append([*c, *ax, *ay, *az, *rng.uniform(0.2, 1.5, 2), spread, shape, *ref, *rng.uniform(0, 1, 2)])
+        return np.asarray(rows)
     if function == abi.PROBE_SELF_EXCLUSION:
         p = rng.normal(0, 1, (n, 9))
         bary = rng.uniform(-0.2, 1.0, (n, 2))
@@ -143,7 +162,8 @@ def _probe_inputs(function, rng, n):
 @pytest.mark.parametrize("function,exact", [(abi.PROBE_FILM_CLAMP, True), (abi.PROBE_T_OFFSET, True),
                                             (abi.PROBE_SELF_EXCLUSION, True), (abi.PROBE_SHADOW_TERMINATOR, True),
                                             (abi.PROBE_SURFACE_SHADOW_ORIGIN, True),
-                                            (abi.PROBE_DIFFUSE_EVAL, False), (abi.PROBE_TRIANGLE_SAMPLE, False)])
+                                            (abi.PROBE_DIFFUSE_EVAL, False), (abi.PROBE_TRIANGLE_SAMPLE, False),
+                                            (abi.PROBE_AREA_SAMPLE, False)])
 def test_function_probes_match_oracle(backend, function, exact):
     log(f"case: probe {function} device vs oracle")
     rng = np.random.default_rng(100 + function)
@@ -168,7 +188,8 @@ def test_function_probes_match_oracle(backend, function, exact):
         sign = np.where(rng.random(x.shape) < 0.5, -np.inf, np.inf).astype(np.float32)
         stable &= close(ora, ob.probe(function, np.nextafter(x, sign)), 1e-5)
     frac = float(np.mean(close(dev, ora, 1e-4)[stable]))
-    log(f"  {int(stable.sum())} of {len(x)} records well conditioned; within 1e-4 on {frac:.5f} of them")
+    log(f"  {int(stable.sum())} of {len(x)} records well conditioned; within 1e-4 on {frac:.5f} of them; "
+        f"all records within 1e-4: {float(np.mean(close(dev, ora, 1e-4))):.5f}")
     assert stable.mean() > 0.5 and frac >= 0.999
 
 

@@ -146,10 +146,11 @@ def test_lamp_bvh_forward_hits():
     film is within the bar of the oracle's linear scan; in MIS mode the same
     holds without the area lamps, and the discrete events are identical with
     all 60 lamps.  (A build that scans the lamps in order gives a
-    bit-identical film; the area lamps' NEE, whose spherical-rectangle
-    sampler is ill-conditioned where the ceiling sees them edge-on, leaves
-    about 2 % of pixels between 1e-4 and 1e-2 of the oracle, which is why
-    the MIS film is held without them.)"""
+    bit-identical film.  The area lamps' NEE leaves about 2 % of pixels
+    between 1e-4 and 1e-2 of the oracle: their spherical-rectangle sampler
+    amplifies last-ulp differences of asinf / acosf where the ceiling sees
+    them edge-on, which the area-sample probe in test_gpu_fixtures.py
+    measures; so the MIS film is held without them.)"""
     snap = lamp_grid()
     log("case: lamp grid (60 lamps), forward only")
     film_vs_oracle(RendererBackend(0, integrator=abi.INTEGRATOR_WAVEFRONT), snap,
