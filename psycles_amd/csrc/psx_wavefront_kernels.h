@@ -35,13 +35,16 @@ __device__ __forceinline__ void add_counter(unsigned long long *dst, uint32_t va
 
 // The state a shade step can change: the sample identity never changes
 // after generation, and the primary Normal / Albedo only while the path is
-// at bounce 0 (`aov`).
+// at bounce 0 (`aov`).  The MIS normal is read only by light sampling and
+// forward-hit MIS, so a scene without lights (MIS = false) neither loads nor
+// stores it.
+template <bool MIS>
 __device__ __forceinline__ void store_step(const WavefrontView &w, uint32_t i, const PathVars &p, bool aov) {
     w.ray_o[i] = make_float4(p.ray.o.x, p.ray.o.y, p.ray.o.z, p.ray.tmin);
     w.ray_d[i] = make_float4(p.ray.d.x, p.ray.d.y, p.ray.d.z, p.ray.tmax);
     w.aux[i] = make_uint4(p.src_obj, p.src_prim, p.step, p.bits);
     w.throughput[i] = make_float4(p.throughput.x, p.throughput.y, p.throughput.z, p.prev_pdf);
-    w.mis_normal[i] = make_float4(p.mis_normal.x, p.mis_normal.y, p.mis_normal.z, 0.0f);
+    if (MIS) w.mis_normal[i] = make_float4(p.mis_normal.x, p.mis_normal.y, p.mis_normal.z, 0.0f);
     w.radiance[i] = make_float4(p.radiance.x, p.radiance.y, p.radiance.z, p.transparency);
     if (aov) {
         w.normal[i] = make_float4(p.normal.x, p.normal.y, p.normal.z, 0.0f);
@@ -95,6 +98,10 @@ __device__ __forceinline__ LightSums load_lp(const WavefrontView &w, uint32_t i)
     return lp;
 }
 
+// A path's state for one shade step.  The primary Normal / Albedo are read
+// only while the path is at bounce 0, the only steps that change them, and
+// the MIS normal only with MIS (see store_step).
+template <bool MIS>
 __device__ __forceinline__ PathVars load_path(const WavefrontView &w, uint32_t i) {
     PathVars p;
     const float4 o = w.ray_o[i], d = w.ray_d[i];
@@ -110,16 +117,24 @@ __device__ __forceinline__ PathVars load_path(const WavefrontView &w, uint32_t i
     const float4 t = w.throughput[i];
     p.throughput = V(t.x, t.y, t.z);
     p.prev_pdf = t.w;
-    const float4 mn = w.mis_normal[i];
-    p.mis_normal = V(mn.x, mn.y, mn.z);
+    if (MIS) {
+        const float4 mn = w.mis_normal[i];
+        p.mis_normal = V(mn.x, mn.y, mn.z);
+    } else {
+        p.mis_normal = V(0.0f, 0.0f, 0.0f);
+    }
     const float4 r = w.radiance[i];
     p.radiance = V(r.x, r.y, r.z);
     p.transparency = r.w;
-    const float4 n = w.normal[i];
-    p.normal = V(n.x, n.y, n.z);
-    const float4 al = w.albedo[i];
-    p.albedo = V(al.x, al.y, al.z);
     const uint4 s0 = w.state0[i], s1 = w.state1[i];
+    if (s0.z == 0u) {
+        const float4 n = w.normal[i];
+        p.normal = V(n.x, n.y, n.z);
+        const float4 al = w.albedo[i];
+        p.albedo = V(al.x, al.y, al.z);
+    } else {
+        p.normal = p.albedo = V(0.0f, 0.0f, 0.0f);
+    }
     p.ps.flag = s0.x;
     p.ps.visibility = s0.y;
     p.ps.bounce = s0.z;
@@ -369,7 +384,7 @@ __device__ __forceinline__ void shade_launch(const SceneView &s, const KernelPar
         uint32_t path = 0u;
         if (entry != kSkip) {
             path = entry;
-            PathVars p = load_path(w, path);
+            PathVars p = load_path<LIGHTS>(w, path);
             const bool aov = p.ps.bounce == 0u;
             if (LP) p.lp = load_lp(w, path);
             if ((p.bits & PATH_SHADOW_PENDING) != 0u) {
@@ -415,7 +430,7 @@ __device__ __forceinline__ void shade_launch(const SceneView &s, const KernelPar
                 }
                 if (!cont) p.bits |= shadow ? PATH_FINISHING : PATH_DONE;
             }
-            store_step(w, path, p, aov);
+            store_step<LIGHTS>(w, path, p, aov);
             if (LP) store_lp(w, path, p.lp);
         }
         // the wave's continuation entries, then its shadow entries, into its
