@@ -465,6 +465,9 @@ typedef struct psx_wavefront_tuning {
                               reserves per atomic (128)                      */
     uint32_t sort_shading; /* 0 or 1: each shade block takes its 256 paths
                               sorted by closure structure of the hit (0)    */
+    uint32_t xcd_segments; /* 1 or 8: trace waves take rays from one queue, or
+                              from 8 segments, one per XCD, stealing from the
+                              others when theirs is drained (1)             */
 } psx_wavefront_tuning;
 
 void psx_wavefront_tuning_default(psx_wavefront_tuning *tuning);
@@ -482,8 +485,8 @@ typedef struct psx_tuning_report {
 
 /* Times probe dispatches of `probe_samples` samples (1..64) of the session's
  * window for candidate shapes, one knob at a time from the default (tile
- * width, refill lanes, trace chunk, shade chunks, shade sorting), keeps a
- * candidate only if
+ * width, refill lanes, trace chunk, shade chunks, shade sorting, XCD
+ * segments), keeps a candidate only if
  * it is at least 1 % faster, and sets the winner on the session.  The probes
  * render into scratch memory: the session's film, sample count and stats are
  * untouched.  A session whose renders run on the megakernel (integrator

@@ -225,7 +225,7 @@ class RenderStats(C.Structure):
 
 class WavefrontTuning(C.Structure):
     _fields_ = [("tile_width", u32), ("refill_lanes", u32), ("trace_chunk", u32),
-                ("queue_chunk", u32), ("list_chunk", u32), ("sort_shading", u32)]
+                ("queue_chunk", u32), ("list_chunk", u32), ("sort_shading", u32), ("xcd_segments", u32)]
 
     def as_dict(self) -> dict:
         return {name: int(getattr(self, name)) for name, _ in self._fields_}

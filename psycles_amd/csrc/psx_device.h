@@ -341,6 +341,7 @@ struct alignas(16) WavefrontCounters {
     uint32_t shade_count;   // paths in this iteration's shade list
     uint32_t trace_fetch;   // work-fetch cursor of the persistent trace kernel
     uint32_t pad;
+    uint32_t segment_fetch[8];  // per-segment cursors when the queue is split by XCD
 };
 
 // One request's share of a server batch: samples [first, first + count) of
@@ -391,6 +392,8 @@ struct WavefrontView {
     uint32_t queue_chunk;   // next-queue slots a shade wave reserves per atomic, 128..kMaxQueueChunk
     uint32_t list_chunk;    // shade-list slots a shade wave reserves per atomic, 64..kMaxListChunk
     uint32_t shade_sort;    // 1: each shade block takes its 256 paths sorted by shade_class
+    uint32_t segments;      // 1, or 8: the trace queue is split in 8 segments, one per XCD
+    uint32_t pad1[3];
 };
 
 // Largest shade reservations; the queues' headroom is sized for them.

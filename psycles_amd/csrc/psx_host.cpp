@@ -1081,6 +1081,7 @@ struct WavefrontState {
         v.queue_chunk = t.queue_chunk;
         v.list_chunk = t.list_chunk;
         v.shade_sort = t.sort_shading;
+        v.segments = t.xcd_segments;
     }
 };
 
@@ -1088,7 +1089,8 @@ bool valid_tuning(const psx_wavefront_tuning &t) {
     const bool tile = t.tile_width == 8u || t.tile_width == 16u || t.tile_width == 32u || t.tile_width == 64u;
     const bool list = t.list_chunk == 64u || t.list_chunk == 128u || t.list_chunk == 256u;
     return tile && t.refill_lanes >= 1u && t.refill_lanes <= 64u && t.trace_chunk >= 16u && t.trace_chunk <= 4096u &&
-           (t.queue_chunk == 128u || t.queue_chunk == 256u) && list && t.sort_shading <= 1u;
+           (t.queue_chunk == 128u || t.queue_chunk == 256u) && list && t.sort_shading <= 1u &&
+           (t.xcd_segments == 1u || t.xcd_segments == 8u);
 }
 
 // A wavefront dispatch captured as a graph, replayed while every kernel
@@ -1125,7 +1127,7 @@ struct Session {
     std::vector<hipEvent_t> trace_events;  // pairs around wavefront trace launches
     psx_render_stats stats{};
     bool count_nodes = false;
-    psx_wavefront_tuning tuning{8u, 16u, 128u, 256u, 128u, 0u};
+    psx_wavefront_tuning tuning{8u, 16u, 128u, 256u, 128u, 0u, 1u};
     DispatchGraph graph[2];       // full dispatches of max_samples_per_dispatch, and the remainder
     bool graph_failed = false;    // capture failed once: launch from the host
     DeviceBuffer dispatch_first;  // first sample of the dispatch a replayed graph renders
@@ -2893,7 +2895,7 @@ uint32_t psx_backend_generated_programs(const psx_backend *handle) {
 }
 
 void psx_wavefront_tuning_default(psx_wavefront_tuning *t) {
-    if (t) *t = psx_wavefront_tuning{8u, 16u, 128u, 256u, 128u, 0u};
+    if (t) *t = psx_wavefront_tuning{8u, 16u, 128u, 256u, 128u, 0u, 1u};
 }
 
 int psx_session_set_tuning(psx_session *handle, const psx_wavefront_tuning *t) {
@@ -3009,6 +3011,11 @@ int autotune(Session &s, uint32_t probe, psx_tuning_report *report) {
         {
             psx_wavefront_tuning t = best;
             t.sort_shading = 1u;
+            consider(t);
+        }
+        {
+            psx_wavefront_tuning t = best;
+            t.xcd_segments = 8u;
             consider(t);
         }
         r.chosen = best;

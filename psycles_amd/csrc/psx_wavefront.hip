@@ -32,7 +32,9 @@ namespace psx {
 uint32_t wavefront_lanes() {
     const uint32_t a = resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<false>), kTraceBlock);
     const uint32_t b = resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<true>), kTraceBlock);
-    return (a > b ? a : b) * kTraceBlock;
+    const uint32_t c = resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<false, true>), kTraceBlock);
+    const uint32_t d = resident_blocks(reinterpret_cast<const void *>(wf_trace_kernel<true, true>), kTraceBlock);
+    return std::max(std::max(a, b), std::max(c, d)) * kTraceBlock;
 }
 
 // Entries the next trace queue and shade list need beyond the live ones:
@@ -87,16 +89,18 @@ hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const flo
     if (total == 0u) return hipSuccess;
     if (total > w.capacity || w.refill == 0u || w.refill > 64u || w.trace_chunk == 0u || w.queue_chunk < 128u ||
         w.queue_chunk > kMaxQueueChunk || w.list_chunk < 64u || w.list_chunk > kMaxListChunk || w.tile_shift < 3u ||
-        w.tile_shift > 6u)
+        w.tile_shift > 6u || (w.segments != 1u && w.segments != 8u))
         return hipErrorInvalidValue;
     const uint32_t iterations = k.max_path_steps + 1u;
     hipLaunchKernelGGL(reset_kernel, dim3((iterations + 1u + 63u) / 64u), dim3(64), 0, stream, w.counters,
                        iterations + 1u);
     hipLaunchKernelGGL(generate_kernel, dim3((total + kShadeBlock - 1u) / kShadeBlock), dim3(kShadeBlock), 0, stream, k,
                        sobol, filter, w, first, count, first_at);
-    const void *trace_fn = count_nodes ? reinterpret_cast<const void *>(wf_trace_kernel<true>)
-                                       : reinterpret_cast<const void *>(wf_trace_kernel<false>);
-    const uint32_t trace_blocks = resident_blocks(trace_fn, kTraceBlock);
+    using TraceFn = void (*)(SceneView, KernelParams, WavefrontView, uint32_t, Counters *, uint32_t *);
+    const bool segmented = w.segments == 8u;
+    const TraceFn trace_fn = count_nodes ? (segmented ? wf_trace_kernel<true, true> : wf_trace_kernel<true, false>)
+                                         : (segmented ? wf_trace_kernel<false, true> : wf_trace_kernel<false, false>);
+    const uint32_t trace_blocks = resident_blocks(reinterpret_cast<const void *>(trace_fn), kTraceBlock);
     const bool lp = w.lp != nullptr;
     const bool lights = scene_has_lights(s);
     const ShadeFn shade_fn = shade_kernel(lp, lights);
@@ -106,12 +110,7 @@ hipError_t launch_wavefront(const SceneView &s, const KernelParams &k, const flo
             const hipError_t e = hipEventRecord(events[2u * it], stream);
             if (e != hipSuccess) return e;
         }
-        if (count_nodes)
-            hipLaunchKernelGGL(wf_trace_kernel<true>, dim3(trace_blocks), dim3(kTraceBlock), 0, stream, s, k, w, it,
-                               counters, spill);
-        else
-            hipLaunchKernelGGL(wf_trace_kernel<false>, dim3(trace_blocks), dim3(kTraceBlock), 0, stream, s, k, w, it,
-                               counters, spill);
+        hipLaunchKernelGGL(trace_fn, dim3(trace_blocks), dim3(kTraceBlock), 0, stream, s, k, w, it, counters, spill);
         if (events) {
             const hipError_t e = hipEventRecord(events[2u * it + 1u], stream);
             if (e != hipSuccess) return e;

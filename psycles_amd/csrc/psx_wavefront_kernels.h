@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kShadeBlock) void generate_kernel(KernelParams k, c
 // the wave's reserved chunk of the queue and start them while the others keep
 // their traversal state (Aila & Laine, "Understanding the efficiency of ray
 // traversal on GPUs", 2009: persistent threads with ray replacement).
-template <bool COUNT>
+template <bool COUNT, bool SEGMENTED = false>
 __global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, KernelParams k, WavefrontView w,
                                                                uint32_t it, Counters *counters, uint32_t *spill) {
     __shared__ uint32_t lds[kShortStack * kTraceBlock];
@@ -231,20 +231,54 @@ __global__ __launch_bounds__(kTraceBlock, 4) void wf_trace_kernel(SceneView s, K
     // cost per ray.  `exhausted`: no chunk is left to reserve.
     uint32_t res = 0u, res_end = 0u;
     const uint32_t chunk = w.trace_chunk, refill = w.refill;
+    // SEGMENTED (w.segments = 8): the queue is cut into 8 contiguous
+    // segments, each with its own cursor.  A wave starts on the segment of
+    // its block's XCD (workgroups are dispatched to the 8 XCDs round-robin),
+    // so each XCD's L2 sees the rays of one region of the queue, and moves to
+    // the next segment when its own is drained.
+    constexpr uint32_t segments = SEGMENTED ? 8u : 1u;
+    uint32_t seg = blockIdx.x % segments, tried = 0u;
     for (;;) {
         if (!exhausted || res < res_end) {
             const unsigned long long idle = __ballot(!active);
             const uint32_t want = static_cast<uint32_t>(__popcll(idle));
             if (res == res_end && !exhausted && want > 0u) {
                 const uint32_t leader = static_cast<uint32_t>(__ffsll(static_cast<long long>(idle)) - 1);
-                uint32_t b = 0u;
-                if (lane == leader) b = atomicAdd(&c->trace_fetch, chunk);
-                b = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b), static_cast<int>(leader)));
-                if (b < count) {
-                    res = b;
-                    res_end = count - b > chunk ? b + chunk : count;
+                if (!SEGMENTED) {
+                    uint32_t b = 0u;
+                    if (lane == leader) b = atomicAdd(&c->trace_fetch, chunk);
+                    b = static_cast<uint32_t>(
+                        __builtin_amdgcn_readlane(static_cast<int>(b), static_cast<int>(leader)));
+                    if (b < count) {
+                        res = b;
+                        res_end = count - b > chunk ? b + chunk : count;
+                    }
+                    exhausted = b >= count || count - b <= chunk;
+                } else {
+                    uint32_t b = 0u, e = 0u;
+                    if (lane == leader) {
+                        for (;;) {
+                            const uint32_t begin = static_cast<uint32_t>(
+                                static_cast<unsigned long long>(count) * seg / segments);
+                            const uint32_t end = static_cast<uint32_t>(
+                                static_cast<unsigned long long>(count) * (seg + 1u) / segments);
+                            const uint32_t got = atomicAdd(&c->segment_fetch[seg], chunk);
+                            if (got < end - begin) {
+                                b = begin + got;
+                                e = end - b > chunk ? b + chunk : end;
+                                break;
+                            }
+                            if (++tried == segments) break;
+                            seg = seg + 1u == segments ? 0u : seg + 1u;
+                        }
+                    }
+                    const int l = static_cast<int>(leader);
+                    res = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b), l));
+                    res_end = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(e), l));
+                    seg = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(seg), l));
+                    tried = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(tried), l));
+                    exhausted = tried == segments;
                 }
-                exhausted = b >= count || count - b <= chunk;
             }
             const uint32_t avail = res_end - res;
             const uint32_t rank = static_cast<uint32_t>(__popcll(idle & below));

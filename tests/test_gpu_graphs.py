@@ -119,6 +119,7 @@ SHAPES = [
     {"trace_chunk": 16}, {"trace_chunk": 4096}, {"trace_chunk": 200},
     {"queue_chunk": 128, "list_chunk": 64}, {"queue_chunk": 256, "list_chunk": 256},
     {"sort_shading": 1}, {"sort_shading": 1, "tile_width": 16, "list_chunk": 64},
+    {"xcd_segments": 8}, {"xcd_segments": 8, "trace_chunk": 16, "refill_lanes": 1},
     {"tile_width": 64, "refill_lanes": 64, "trace_chunk": 16, "queue_chunk": 128, "list_chunk": 64},
 ]
 
@@ -154,10 +155,11 @@ def test_invalid_shapes_rejected(graphs):
     default = abi.WavefrontTuning()
     lib.psx_wavefront_tuning_default(C.byref(default))
     assert default.as_dict() == {"tile_width": 8, "refill_lanes": 16, "trace_chunk": 128,
-                                 "queue_chunk": 256, "list_chunk": 128, "sort_shading": 0}
+                                 "queue_chunk": 256, "list_chunk": 128, "sort_shading": 0, "xcd_segments": 1}
     for bad in ({"tile_width": 4}, {"tile_width": 12}, {"tile_width": 128}, {"refill_lanes": 0},
                 {"refill_lanes": 65}, {"trace_chunk": 8}, {"trace_chunk": 8192}, {"queue_chunk": 64},
-                {"queue_chunk": 512}, {"list_chunk": 32}, {"list_chunk": 512}, {"sort_shading": 2}):
+                {"queue_chunk": 512}, {"list_chunk": 32}, {"list_chunk": 512}, {"sort_shading": 2},
+                {"xcd_segments": 0}, {"xcd_segments": 4}, {"xcd_segments": 16}):
         t = abi.WavefrontTuning.from_buffer_copy(default)
         for k, v in bad.items():
             setattr(t, k, v)
@@ -176,7 +178,7 @@ def test_autotune_keeps_the_film(graphs):
     rep = tuned.autotune(2)
     log(f"  chosen {rep.chosen.as_dict()} {rep.chosen_ms:.3f} ms vs default {rep.default_ms:.3f} ms "
         f"over {rep.candidates} shapes")
-    assert rep.candidates == 15 and rep.probe_samples == 2
+    assert rep.candidates == 16 and rep.probe_samples == 2
     assert 0.0 < rep.chosen_ms <= rep.default_ms
     assert tuned.tuning.as_dict() == rep.chosen.as_dict()
     # the probes left the film alone; the next range renders with the new shape
