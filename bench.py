@@ -243,7 +243,8 @@ def uneven_bench(args) -> int:
     log(f"compiling {len(batch)} uneven-batch scenes")
     backend = RendererBackend(0, integrator=abi.INTEGRATOR_WAVEFRONT)
     compiled = [backend.compile_scene(r.snapshot).scene for r in batch]
-    server = RenderServer(backend)
+    server = RenderServer(backend, path_capacity=args.server_paths, streams=args.server_streams,
+                          target_batch_ms=args.server_batch_ms)
     warm_settings = batch[0].settings(64)
     warm = server.submit(compiled[0], warm_settings, 0, 1, 1)
     warm.wait(-1.0)
@@ -269,6 +270,9 @@ def uneven_bench(args) -> int:
                         "square side U{128..1024}, U{1..256} spp, max 8 bounces with RR, 32 transparent bounces; "
                         "the optional homogeneous box is not built",
             "integrator": "render server (wavefront batches)",
+            "server_streams": args.server_streams or 6,
+            "server_paths": args.server_paths or 8388608,
+            "server_batch_ms": args.server_batch_ms or 12,
         },
         "batched": batched,
     }
@@ -300,6 +304,12 @@ def main() -> int:
                     help="keep the default wavefront launch shape instead of timing candidates first")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel of a dispatch from the host instead of replaying a graph")
+    ap.add_argument("--server-streams", type=int, default=0,
+                    help="uneven: render-server streams (2-8; 0: the server's default of 6)")
+    ap.add_argument("--server-paths", type=int, default=0,
+                    help="uneven: render-server paths per batch (0: the server's default of 8,388,608)")
+    ap.add_argument("--server-batch-ms", type=int, default=0,
+                    help="uneven: render-server target batch duration (0: the server's default of 12 ms)")
     ap.add_argument("--tuning", default=None,
                     help="explicit wavefront launch shape instead of the autotuner, e.g. tile_width=16,sort_shading=1")
     ap.add_argument("--interpreter", action="store_true",

@@ -338,7 +338,11 @@ enum psx_backend_flags {
     /* Launch every kernel of a wavefront dispatch from the host.  By default
      * a session captures its dispatch (counter reset, camera rays, every
      * bounce iteration's trace and shade, accumulation) into a graph once
-     * and replays it per dispatch; the film is the same either way. */
+     * and replays it per dispatch; the film is the same either way.  A
+     * capture and the render server's device calls on the same backend never
+     * overlap.  Device calls that other threads of the caller make during a
+     * capture (allocations, synchronous copies) can invalidate it; callers
+     * that drive sessions from several threads set this flag. */
     PSX_BACKEND_NO_GRAPHS = 2u,
     /* Time every wavefront trace launch with HIP events (psx_render_stats
      * trace_ms and trace_launches).  Events recorded inside a graph cannot be
@@ -716,22 +720,26 @@ int psx_backend_probe(psx_backend *backend, uint32_t function, const float *inpu
  * options' limits) are batched ahead of the others and have a stream of their
  * own; while viewport work waits or runs no final-frame batch starts and
  * running ones launch their kernels on a quarter of the compute units.
- * Final-frame batches are sized from the measured device time per path of
- * their scene and settings to last about target_batch_ms.  A request's film is accumulated in sample order, one chunk at a time,
- * so it is bit-identical to a session rendering the same range alone.
+ * While viewport requests were seen within the last second, final-frame
+ * batches are sized from the measured device time per path of their scene
+ * and settings to last about target_batch_ms; otherwise they fill the
+ * path capacity.  A request's film is accumulated in sample order, one chunk
+ * at a time, so it is bit-identical to a session rendering the same range
+ * alone.
  * All entry points are thread-safe. */
 
 typedef struct psx_server psx_server;
 
 typedef struct psx_server_options {
-    uint32_t path_capacity;        /* paths per batch (0: 4,194,304)        */
+    uint32_t path_capacity;        /* paths per batch (0: 8,388,608)        */
     uint32_t streams;              /* concurrent batches, 2..8 (0: 6); the  */
                                    /* upper half take viewport batches only */
     uint64_t hbm_budget;           /* bytes for films + batch state (0: 200 GB) */
     uint32_t viewport_max_pixels;  /* (0: 256,000)                          */
     uint32_t viewport_max_samples; /* (0: 4)                                */
     uint32_t target_batch_ms;      /* final-frame batch length the server   */
-                                   /* sizes batches for (0: 12)             */
+                                   /* sizes batches for while viewport work */
+                                   /* is around (0: 12)                     */
     uint32_t reserved[3];
 } psx_server_options;
 
@@ -764,7 +772,9 @@ typedef struct psx_serving_stats {
 
 int psx_server_create(psx_backend *backend, const psx_server_options *options,
                       psx_server **out);
-/* Cancels queued work, waits for running batches, frees every request. */
+/* Cancels queued work, waits for running batches, frees every request.
+ * Every scene a request or edit named must still exist: destroy the server
+ * before those scenes. */
 void psx_server_destroy(psx_server *server);
 /* Queues samples [first, first+count) of `total` (SampleRange, offset 0) of
  * `settings` on `scene`.  Validation as psx_session_create and
@@ -780,7 +790,10 @@ int psx_server_update_parameters(psx_server *server, psx_scene *scene,
                                  uint32_t parameter_count);
 int psx_server_status(psx_server *server, uint64_t id, psx_request_status *out);
 /* Blocks until the request is done, cancelled or failed, or timeout_ms
- * passes (negative: no limit); returns PSX_OK when done. */
+ * passes (negative: no limit); returns PSX_OK when done.  A request whose
+ * film cannot fit the HBM budget even with no other film resident fails
+ * with PSX_ERROR_OUT_OF_MEMORY; a device error on the server's threads fails
+ * every request with its status.  psx_last_error then names the cause. */
 int psx_server_wait(psx_server *server, uint64_t id, double timeout_ms);
 /* A done request's film: raw sums as psx_session_read_film, or passes
  * through a sink as psx_session_render (begin, one tile per pass, end). */

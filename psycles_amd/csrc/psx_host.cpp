@@ -569,6 +569,11 @@ struct Backend {
     int device = 0;
     psx_backend_options options{};
     ProgramCache programs;  // generated shade kernels (PSX_BACKEND_GENERATED_PROGRAMS)
+    // Held while a session captures a dispatch graph, and by the render
+    // server's threads while they make device calls: an allocation, upload
+    // or module load on another thread during a capture invalidates it
+    // (session streams synchronise with the null stream).
+    std::mutex capture_gate;
 };
 
 // Light-sampling tables that follow material emission: emitter list, flat
@@ -1372,6 +1377,7 @@ bool capture_dispatch(Session &s, DispatchGraph &slot, const FilmView &film, uin
                       const GeneratedShade *generated) {
     slot.reset();
     hipGraph_t g = nullptr;
+    std::lock_guard<std::mutex> gate(s.scene->backend->capture_gate);
     if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
     const hipError_t launched =
         launch_wavefront(s.scene->view, s.params, s.sobol.as<float4>(), s.filter.as<float>(), film, s.wavefront.view,
@@ -1941,8 +1947,29 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     return PSX_OK;
 }
 
+// While a stream capture in global mode runs anywhere in the process, HIP
+// refuses allocations, frees, event queries and synchronous copies from
+// every thread whose own mode is global (the default).  The library's
+// captures use thread-local mode, and its calls only need to stay out of
+// captures on their own thread: this scope switches the calling thread to
+// thread-local mode and restores the caller's mode on exit.
+class ThreadLocalCaptureMode {
+  public:
+    ThreadLocalCaptureMode() { ok_ = hipThreadExchangeStreamCaptureMode(&saved_) == hipSuccess; }
+    ~ThreadLocalCaptureMode() {
+        if (ok_) (void)hipThreadExchangeStreamCaptureMode(&saved_);
+    }
+    ThreadLocalCaptureMode(const ThreadLocalCaptureMode &) = delete;
+    ThreadLocalCaptureMode &operator=(const ThreadLocalCaptureMode &) = delete;
+
+  private:
+    hipStreamCaptureMode saved_ = hipStreamCaptureModeThreadLocal;
+    bool ok_ = false;
+};
+
 template <typename F>
 int guarded(F &&f) {
+    ThreadLocalCaptureMode mode;
     try {
         return f();
     } catch (const DeviceError &e) {
@@ -2017,6 +2044,8 @@ struct ServerRequest {
     size_t film_bytes = 0;
     hipEvent_t film_ready = nullptr;  // the cleared film, on the control stream
     const float *filter = nullptr;    // the server's table for the request's pixel filter
+    int error = PSX_OK;               // why a failed request failed
+    std::string message;
 
     ~ServerRequest() {
         if (film_ready) (void)hipEventDestroy(film_ready);
@@ -2047,6 +2076,9 @@ constexpr uint32_t kIterationsAhead = 3u;
 // fraction of the resident blocks a final-frame batch launches with while
 // viewport work waits or runs
 constexpr float kViewportShare = 0.25f;
+// final-frame batches fill their arena once no viewport request has been
+// taken, waited or run for this long
+constexpr double kViewportQuietMs = 1000.0;
 constexpr uint32_t kMaxIterations = 1025u;
 
 struct Arena {
@@ -2110,6 +2142,7 @@ struct Server {
     bool stopping = false;
     std::thread worker;
     int worker_error = PSX_OK;
+    std::string worker_message;  // the failing call, as psx_last_error reports it
     // edits run on their own thread so that host-side rebuilds and table
     // uploads never delay batch feeding
     std::thread editor;
@@ -2122,6 +2155,7 @@ struct Server {
     std::vector<std::pair<BatchKey, double>> cost_ns;
 
     ServerClock::time_point epoch = ServerClock::now();
+    double last_viewport_ms = -1.0e30;  // viewport work last taken, waiting or running (worker only)
     psx_serving_stats stats{};
     size_t fixed_bytes = 0, film_bytes = 0;
     ServerClock::time_point busy_since{};
@@ -2229,6 +2263,7 @@ struct Server {
             r.padded = ((r.film.window_w + 7u) >> 3) * ((r.film.window_h + 7u) >> 3) * 64u;
             const size_t pixels = static_cast<size_t>(r.film.window_w) * r.film.window_h;
             r.viewport = pixels <= options.viewport_max_pixels && r.count <= options.viewport_max_samples;
+            if (r.viewport) last_viewport_ms = now_ms();
             r.film_bytes = pixels * 52u + 4096u;
             active.push_back(op.request);
             return;
@@ -2272,6 +2307,7 @@ struct Server {
     }
 
     void edit_loop() {
+        ThreadLocalCaptureMode mode;
         try {
             hip_check(hipSetDevice(backend->device));
             for (;;) {
@@ -2288,7 +2324,10 @@ struct Server {
                 const bool emitted = m.may_emit;
                 refresh_may_emit(m);
                 op.scene->gpu_materials[op.material].may_emit = m.may_emit ? 1u : 0u;
-                op.scene->apply_parameter_version(emitted || m.may_emit);
+                {
+                    std::lock_guard<std::mutex> gate(backend->capture_gate);
+                    op.scene->apply_parameter_version(emitted || m.may_emit);
+                }
                 {
                     std::lock_guard<std::mutex> lock(mutex);
                     SceneVersions &v = versions_of(op.scene);
@@ -2301,9 +2340,11 @@ struct Server {
         } catch (const DeviceError &e) {
             std::lock_guard<std::mutex> lock(mutex);
             worker_error = e.status;
+            worker_message = "server edit thread: " + g_last_error;
         } catch (...) {
             std::lock_guard<std::mutex> lock(mutex);
             worker_error = PSX_ERROR_DEVICE;
+            worker_message = "server edit thread: unexpected exception";
         }
         wake.notify_all();
     }
@@ -2392,6 +2433,10 @@ struct Server {
             // other film resident fails
             if (in_use + r.film_bytes > options.hbm_budget) {
                 if (films_in_use == 0u) {
+                    r.error = PSX_ERROR_OUT_OF_MEMORY;
+                    r.message = "request film of " + std::to_string(r.film_bytes) + " bytes does not fit the HBM budget (" +
+                                std::to_string(options.hbm_budget) + " bytes, " + std::to_string(in_use) +
+                                " in use with no film resident)";
                     finish(r, PSX_REQUEST_FAILED);
                     progress = true;
                     continue;
@@ -2436,8 +2481,9 @@ struct Server {
     // in submission order, each as many samples as the capacity leaves.
     //
     // While viewport work is waiting or running no new final-frame batch
-    // starts, and final-frame batches are sized from the key's measured cost
-    // to last about options.target_batch_ms.
+    // starts.  Final-frame batches are sized from the key's measured cost to
+    // last about options.target_batch_ms while viewport work was seen within
+    // kViewportQuietMs, and fill the arena otherwise.
     bool launch(Arena &a) {
         ServerRequest *leader = nullptr;
         for (auto &r : active)
@@ -2459,7 +2505,7 @@ struct Server {
         if (!leader) return false;
         const BatchKey key = leader->key;
         uint32_t capacity = a.state.view.capacity;
-        if (!viewport) {
+        if (!viewport && now_ms() - last_viewport_ms < kViewportQuietMs) {
             const double target = static_cast<double>(options.target_batch_ms) * 1.0e6 / cost_of(key);
             const uint64_t bounded = std::max<uint64_t>(static_cast<uint64_t>(target), leader->padded);
             capacity = static_cast<uint32_t>(std::min<uint64_t>(bounded, capacity));
@@ -2584,6 +2630,7 @@ struct Server {
     }
 
     void run() {
+        ThreadLocalCaptureMode mode;
         try {
             hip_check(hipSetDevice(backend->device));
             for (;;) {
@@ -2594,15 +2641,19 @@ struct Server {
                     if (worker_error != PSX_OK) throw DeviceError{worker_error};
                     if (stopping && taken.empty() && active.empty() && busy_arenas == 0u) break;
                 }
-                for (ServerOp &op : taken) take(op);
                 bool progress = !taken.empty();
-                progress |= bind();
-                progress |= retire();
-                progress |= admit();
-                for (auto &a : arenas)
-                    if (!a->busy) progress |= launch(*a);
-                const bool hold = viewport_pending();
-                for (auto &a : arenas) progress |= feed(*a, hold);
+                {
+                    std::lock_guard<std::mutex> gate(backend->capture_gate);
+                    for (ServerOp &op : taken) take(op);
+                    progress |= bind();
+                    progress |= retire();
+                    progress |= admit();
+                    for (auto &a : arenas)
+                        if (!a->busy) progress |= launch(*a);
+                    const bool hold = viewport_pending();
+                    if (hold) last_viewport_ms = now_ms();
+                    for (auto &a : arenas) progress |= feed(*a, hold);
+                }
                 if (stopping && !progress && busy_arenas == 0u) {
                     // nothing left that can run (requests that never fit)
                     for (auto &r : active)
@@ -2619,9 +2670,11 @@ struct Server {
         } catch (const DeviceError &e) {
             std::lock_guard<std::mutex> lock(mutex);
             worker_error = e.status;
+            if (worker_message.empty()) worker_message = "server worker: " + g_last_error;
         } catch (...) {
             std::lock_guard<std::mutex> lock(mutex);
             worker_error = PSX_ERROR_DEVICE;
+            if (worker_message.empty()) worker_message = "server worker: unexpected exception";
         }
         if (worker_error != PSX_OK) {
             // wait for what is on the device before the films can go
@@ -2707,7 +2760,10 @@ int psx_backend_create(int device, const psx_backend_options *options, psx_backe
     });
 }
 
-void psx_backend_destroy(psx_backend *b) { delete b; }
+void psx_backend_destroy(psx_backend *b) {
+    ThreadLocalCaptureMode mode;
+    delete b;
+}
 
 int psx_scene_compile(psx_backend *b, const psx_scene_desc *d, psx_diag_fn diag, void *user, psx_scene **out) {
     if (!b || !d || !out) return PSX_ERROR_INVALID_ARGUMENT;
@@ -2723,7 +2779,10 @@ int psx_scene_compile(psx_backend *b, const psx_scene_desc *d, psx_diag_fn diag,
     });
 }
 
-void psx_scene_destroy(psx_scene *s) { delete reinterpret_cast<Scene *>(s); }
+void psx_scene_destroy(psx_scene *s) {
+    ThreadLocalCaptureMode mode;
+    delete reinterpret_cast<Scene *>(s);
+}
 
 int psx_scene_update_parameters(psx_scene *handle, uint32_t material, const float *p, uint32_t count) {
     Scene *s = reinterpret_cast<Scene *>(handle);
@@ -2766,7 +2825,10 @@ int psx_session_create(psx_scene *handle, const psx_render_settings *rs, psx_ses
     });
 }
 
-void psx_session_destroy(psx_session *s) { delete reinterpret_cast<Session *>(s); }
+void psx_session_destroy(psx_session *s) {
+    ThreadLocalCaptureMode mode;
+    delete reinterpret_cast<Session *>(s);
+}
 
 int psx_session_reset(psx_session *handle, const psx_render_settings *rs) {
     Session *s = reinterpret_cast<Session *>(handle);
@@ -3253,7 +3315,7 @@ int psx_server_create(psx_backend *b, const psx_server_options *options, psx_ser
     if (!b || !out) return PSX_ERROR_INVALID_ARGUMENT;
     *out = nullptr;
     psx_server_options o = options ? *options : psx_server_options{};
-    if (o.path_capacity == 0u) o.path_capacity = 4194304u;
+    if (o.path_capacity == 0u) o.path_capacity = 8388608u;
     o.path_capacity = (o.path_capacity + 63u) & ~63u;
     if (o.streams == 0u) o.streams = 6u;
     if (o.streams < 2u || o.streams > 8u || o.path_capacity > (1u << 30)) return PSX_ERROR_INVALID_ARGUMENT;
@@ -3303,7 +3365,10 @@ int psx_server_create(psx_backend *b, const psx_server_options *options, psx_ser
     });
 }
 
-void psx_server_destroy(psx_server *server) { delete server; }
+void psx_server_destroy(psx_server *server) {
+    ThreadLocalCaptureMode mode;
+    delete server;
+}
 
 int psx_server_submit(psx_server *server, psx_scene *handle, const psx_render_settings *rs, uint32_t first,
                       uint32_t count, uint32_t total, uint64_t *out_id) {
@@ -3397,6 +3462,11 @@ int psx_server_wait(psx_server *server, uint64_t id, double timeout_ms) {
     }
     if (r->state == PSX_REQUEST_DONE) return PSX_OK;
     if (r->state == PSX_REQUEST_CANCELLED) return PSX_ERROR_CANCELLED;
+    if (r->error != PSX_OK) {
+        g_last_error = r->message;
+        return r->error;
+    }
+    g_last_error = server->worker_message;
     return server->worker_error != PSX_OK ? server->worker_error : static_cast<int>(PSX_ERROR_DEVICE);
 }
 

@@ -11,12 +11,25 @@ include/psx.h.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
 from . import abi
 from .render import CompiledScene, OutputSink, RendererBackend, make_sink
+
+
+# servers still open at interpreter exit are closed before module teardown,
+# while the scenes they reference are alive and no worker thread outlives them
+_OPEN_SERVERS: "weakref.WeakSet[RenderServer]" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_open_servers() -> None:
+    for server in list(_OPEN_SERVERS):
+        server.close()
 
 
 class ServerRequest:
@@ -89,17 +102,23 @@ class RenderServer:
         handle = C.c_void_p()
         abi.check("psx_server_create", self.lib.psx_server_create(backend.handle, C.byref(opts), C.byref(handle)))
         self.handle = handle
+        # the scenes of submitted requests and edits: a scene must outlive
+        # every request on it, so the server holds them until it closes
+        self._scenes = {}
+        _OPEN_SERVERS.add(self)
 
     def submit(self, scene: CompiledScene, settings, first: int, count: int, total: int) -> ServerRequest:
         rid = C.c_uint64()
         abi.check("psx_server_submit", self.lib.psx_server_submit(
             self.handle, scene.handle, C.byref(settings), first, count, total, C.byref(rid)))
+        self._scenes[id(scene)] = scene
         return ServerRequest(self, rid.value, settings)
 
     def update_parameters(self, scene: CompiledScene, material: int, parameters) -> None:
         p = np.ascontiguousarray(parameters, dtype=np.float32)
         abi.check("psx_server_update_parameters", self.lib.psx_server_update_parameters(
             self.handle, scene.handle, material, p.ctypes.data_as(C.POINTER(C.c_float)), p.size))
+        self._scenes[id(scene)] = scene
 
     def stats(self) -> abi.ServingStats:
         st = abi.ServingStats()
@@ -110,6 +129,7 @@ class RenderServer:
         if self.handle:
             self.lib.psx_server_destroy(self.handle)
             self.handle = None
+            self._scenes = {}
 
     def __del__(self):
         try:

@@ -218,9 +218,11 @@ def run_uneven_batched(server, compiled: list, batch, log=None) -> dict:
         keep.append(s)
         handles.append(server.submit(comp, s, 0, req.spp, req.spp))
     latencies, first_submit, last_done = [], None, 0.0
-    for req, h in zip(batch, handles):
+    for n, (req, h) in enumerate(zip(batch, handles)):
         if not h.wait(-1.0):
             raise RuntimeError(f"request {req.index} did not complete")
+        if log and (n + 1) % 32 == 0:
+            log(f"uneven batched: {n + 1} of {len(batch)} requests done")
         st = h.status()
         latencies.append(st.done_ms - st.submit_ms)
         first_submit = st.submit_ms if first_submit is None else min(first_submit, st.submit_ms)
@@ -241,7 +243,7 @@ def run_uneven_plain(backend, compiled: list, batch, log=None) -> dict:
     rays = 0
     start = time.perf_counter()
     latencies = []
-    for req, comp in zip(batch, compiled):
+    for n, (req, comp) in enumerate(zip(batch, compiled)):
         s = req.settings()
         sess = backend.create_session(comp, s)
         sess.render_device(0, req.spp, req.spp)
@@ -249,6 +251,8 @@ def run_uneven_plain(backend, compiled: list, batch, log=None) -> dict:
         rays += sess.stats().rays
         sess.close()
         latencies.append((time.perf_counter() - start) * 1e3)
+        if log and (n + 1) % 32 == 0:
+            log(f"uneven plain: {n + 1} of {len(batch)} requests done, {time.perf_counter() - start:.1f} s")
     makespan = time.perf_counter() - start
     samples = sum(r.side * r.side * r.spp for r in batch)
     if log:

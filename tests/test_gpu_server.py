@@ -148,6 +148,28 @@ def test_budget_queues_requests(backend):
     server.close()
 
 
+def test_server_beside_graph_captures(backend):
+    log("case: server admissions while the caller's sessions capture dispatch graphs")
+    # HIP refuses allocations and event queries on every thread in global
+    # capture mode while any stream captures; the worker must not be one
+    comp = backend.compile_scene(scenes.cornell_box()).scene
+    s = scenes.cornell_settings(160, 100)
+    server = RenderServer(backend, path_capacity=1 << 15, streams=2)
+    reference = standalone(backend, comp, s, 0, 2, 2)
+    reqs = []
+    for _ in range(24):
+        reqs.append(server.submit(comp, s, 0, 2, 2))
+        sess = backend.create_session(comp, s)
+        sess.render_device(0, 2, 2)
+        assert sess.stats().graph_builds >= 1
+        sess.close()
+    for r in reqs:
+        assert r.wait(60000)
+        assert_same(r.read_film(), reference)
+        r.release()
+    server.close()
+
+
 def test_cancel_and_validation(backend):
     log("case: cancel and invalid submissions")
     comp = backend.compile_scene(small_instanced()).scene
