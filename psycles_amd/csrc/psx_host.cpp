@@ -570,9 +570,10 @@ struct Backend {
     psx_backend_options options{};
     ProgramCache programs;  // generated shade kernels (PSX_BACKEND_GENERATED_PROGRAMS)
     // Held while a session captures a dispatch graph, and by the render
-    // server's threads while they make device calls: an allocation, upload
-    // or module load on another thread during a capture invalidates it
-    // (session streams synchronise with the null stream).
+    // server's worker while it makes device calls: a synchronous upload
+    // (null stream) or module load on another thread during a capture
+    // invalidates it, since session streams synchronise with the null
+    // stream.
     std::mutex capture_gate;
 };
 
@@ -2079,6 +2080,8 @@ constexpr float kViewportShare = 0.25f;
 // final-frame batches fill their arena once no viewport request has been
 // taken, waited or run for this long
 constexpr double kViewportQuietMs = 1000.0;
+// edit threads: parameter versions of this many scenes build at once
+constexpr uint32_t kEditThreads = 3u;
 constexpr uint32_t kMaxIterations = 1025u;
 
 struct Arena {
@@ -2143,10 +2146,12 @@ struct Server {
     std::thread worker;
     int worker_error = PSX_OK;
     std::string worker_message;  // the failing call, as psx_last_error reports it
-    // edits run on their own thread so that host-side rebuilds and table
-    // uploads never delay batch feeding
-    std::thread editor;
+    // edits run on their own threads so that host-side rebuilds and table
+    // uploads never delay batch feeding; edits of different scenes build at
+    // once, those of one scene in submission order
+    std::vector<std::thread> editors;
     std::deque<ServerOp> edits;
+    std::set<const Scene *> editing;  // scenes with an edit being built
     std::condition_variable edit_wake;
     std::map<Scene *, SceneVersions> versions;  // guarded by `mutex`
 
@@ -2175,7 +2180,8 @@ struct Server {
         wake.notify_all();
         edit_wake.notify_all();
         if (worker.joinable()) worker.join();
-        if (editor.joinable()) editor.join();
+        for (std::thread &t : editors)
+            if (t.joinable()) t.join();
         // films are released on the control stream: drop them before it goes
         ops.clear();
         active.clear();
@@ -2313,29 +2319,45 @@ struct Server {
             for (;;) {
                 ServerOp op;
                 {
+                    // the oldest edit whose scene has no earlier edit
+                    // pending or being built
                     std::unique_lock<std::mutex> lock(mutex);
-                    edit_wake.wait(lock, [&] { return !edits.empty() || stopping; });
-                    if (edits.empty()) return;
-                    op = std::move(edits.front());
-                    edits.pop_front();
+                    auto ready = [&] {
+                        std::set<const Scene *> earlier;
+                        for (auto it = edits.begin(); it != edits.end(); ++it) {
+                            if (!editing.count(it->scene) && !earlier.count(it->scene)) return it;
+                            earlier.insert(it->scene);
+                        }
+                        return edits.end();
+                    };
+                    // a closing server cancels every request, so pending
+                    // edits are dropped
+                    edit_wake.wait(lock, [&] { return stopping || ready() != edits.end(); });
+                    if (stopping) return;
+                    const auto it = ready();
+                    op = std::move(*it);
+                    edits.erase(it);
+                    editing.insert(op.scene);
                 }
                 HostMaterial &m = op.scene->materials[op.material];
                 m.params = op.params;
                 const bool emitted = m.may_emit;
                 refresh_may_emit(m);
                 op.scene->gpu_materials[op.material].may_emit = m.may_emit ? 1u : 0u;
-                {
-                    std::lock_guard<std::mutex> gate(backend->capture_gate);
-                    op.scene->apply_parameter_version(emitted || m.may_emit);
-                }
+                // stream-ordered allocations and copies on the scene's
+                // table stream, which never synchronises with the null
+                // stream, so edits need not wait for the capture gate
+                op.scene->apply_parameter_version(emitted || m.may_emit);
                 {
                     std::lock_guard<std::mutex> lock(mutex);
                     SceneVersions &v = versions_of(op.scene);
                     v.history[op.version] = std::make_pair(op.scene->tables, op.scene->view);
                     v.published = op.version;
                     stats.edits += 1u;
+                    editing.erase(op.scene);
                 }
                 wake.notify_all();
+                edit_wake.notify_all();
             }
         } catch (const DeviceError &e) {
             std::lock_guard<std::mutex> lock(mutex);
@@ -2347,6 +2369,7 @@ struct Server {
             worker_message = "server edit thread: unexpected exception";
         }
         wake.notify_all();
+        edit_wake.notify_all();
     }
 
     void finish(ServerRequest &r, uint32_t state) {
@@ -3359,7 +3382,7 @@ int psx_server_create(psx_backend *b, const psx_server_options *options, psx_ser
         server->stats.hbm_in_use = server->stats.peak_hbm = server->fixed_bytes;
         psx_server *raw = server.get();
         raw->worker = std::thread([raw] { raw->run(); });
-        raw->editor = std::thread([raw] { raw->edit_loop(); });
+        for (uint32_t i = 0; i < kEditThreads; ++i) raw->editors.emplace_back([raw] { raw->edit_loop(); });
         *out = server.release();
         return static_cast<int>(PSX_OK);
     });

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <future>
 #include <limits>
 
 namespace psx {
@@ -146,11 +147,31 @@ struct Slot {
     uint32_t count = 0;
 };
 
+// Split costs along one axis: bucket measures united in item order, one cost
+// per bucket boundary (infinite where a side is empty).
+struct AxisCosts {
+    std::array<float, kBucketCount - 1> cost;
+    std::array<uint32_t, kBucketCount - 1> below;
+};
+
+// Nodes with at least kParallelItems emitters unite their measure and
+// evaluate their three axes on separate threads.  Down to kParallelDepth
+// levels, a node whose children both hold at least kSubtreeItems emitters
+// builds its right subtree on another thread while the left one is built
+// here.  Every float operation happens in the same order as in a serial
+// build, and subtrees are numbered as the serial pre-order numbers them, so
+// the tables are identical.
+constexpr uint32_t kParallelItems = 4096u;
+constexpr uint32_t kParallelDepth = 5u;
+constexpr uint32_t kSubtreeItems = 512u;
+
 class TreeBuilder {
   public:
-    std::vector<Item> items;
-    std::vector<BuildNode> nodes;
-    std::vector<uint32_t> tree_of, leaf_of;
+    // shared by the builders of disjoint subtrees: items [begin, end) of a
+    // subtree, and the entries of tree_of / leaf_of for their ids
+    std::vector<Item> *items = nullptr;
+    std::vector<uint32_t> *tree_of = nullptr, *leaf_of = nullptr;
+    std::vector<BuildNode> nodes;  // this builder's nodes, numbered from 0
     uint32_t leaf_size = 8;
 
     uint32_t new_node(uint32_t parent) {
@@ -160,50 +181,86 @@ class TreeBuilder {
         return static_cast<uint32_t>(nodes.size()) - 1u;
     }
 
-    uint32_t local(uint32_t begin, uint32_t end, uint32_t parent) {
+    AxisCosts axis_costs(uint32_t begin, uint32_t end, size_t a, const Box &centres, float widest) const {
+        AxisCosts r;
+        r.cost.fill(std::numeric_limits<float>::infinity());
+        r.below.fill(0u);
+        const V3 extent = centres.hi - centres.lo;
+        const float span = axis_value(extent, a);
+        if (!(span > 0.0f)) return r;
+        const std::vector<Item> &it = *items;
+        std::array<Slot, kBucketCount> slots{};
+        for (uint32_t i = begin; i < end; ++i) {
+            const float rel = (axis_value(it[i].c, a) - axis_value(centres.lo, a)) / span;
+            const size_t s = std::min<size_t>(
+                static_cast<size_t>(std::max(rel, 0.0f) * static_cast<float>(kBucketCount)), kBucketCount - 1);
+            slots[s].m = unite(slots[s].m, it[i].m);
+            ++slots[s].count;
+        }
+        std::array<Slot, kBucketCount - 1> below{}, above{};
+        below[0] = slots[0];
+        for (size_t s = 1; s + 1 < kBucketCount; ++s)
+            below[s] = Slot{unite(below[s - 1].m, slots[s].m), below[s - 1].count + slots[s].count};
+        above[kBucketCount - 2] = slots[kBucketCount - 1];
+        for (size_t s = kBucketCount - 2; s-- > 0;)
+            above[s] = Slot{unite(above[s + 1].m, slots[s + 1].m), above[s + 1].count + slots[s + 1].count};
+        const float stretch = widest / span;
+        for (size_t s = 0; s + 1 < kBucketCount; ++s) {
+            if (below[s].count == 0u || above[s].count == 0u) continue;
+            r.cost[s] = stretch * (saoh_cost(below[s].m) + saoh_cost(above[s].m));
+            r.below[s] = below[s].count;
+        }
+        return r;
+    }
+
+    uint32_t local(uint32_t begin, uint32_t end, uint32_t parent, uint32_t depth) {
+        std::vector<Item> &it = *items;
         const uint32_t self = new_node(parent);
         const uint32_t n = end - begin;
         if (n == 0u) return self;
-        Measure all;
         Box centres;
-        for (uint32_t i = begin; i < end; ++i) {
-            all = unite(all, items[i].m);
-            centres = unite(centres, Box{items[i].c, items[i].c, false});
+        for (uint32_t i = begin; i < end; ++i) centres = unite(centres, Box{it[i].c, it[i].c, false});
+        const V3 extent = centres.hi - centres.lo;
+        const float widest = std::max({extent.x, extent.y, extent.z});
+        auto unite_all = [&] {
+            Measure all;
+            for (uint32_t i = begin; i < end; ++i) all = unite(all, it[i].m);
+            return all;
+        };
+        Measure all;
+        std::array<AxisCosts, 3> axes;
+        bool costed = false;
+        if (n >= kParallelItems && widest > 0.0f) {
+            // the axes' costs do not depend on the node's measure: all four
+            // passes at once, the parent-cost test afterwards
+            auto m = std::async(std::launch::async, unite_all);
+            auto y = std::async(std::launch::async, [&] { return axis_costs(begin, end, 1, centres, widest); });
+            auto z = std::async(std::launch::async, [&] { return axis_costs(begin, end, 2, centres, widest); });
+            axes[0] = axis_costs(begin, end, 0, centres, widest);
+            axes[1] = y.get();
+            axes[2] = z.get();
+            all = m.get();
+            costed = true;
+        } else {
+            all = unite_all();
         }
         nodes[self].m = all;
         size_t split_axis = std::numeric_limits<size_t>::max();
         uint32_t middle = begin + n / 2u;
-        float best = std::numeric_limits<float>::infinity();
         const float parent_cost = saoh_cost(all);
-        const V3 extent = centres.hi - centres.lo;
-        const float widest = std::max({extent.x, extent.y, extent.z});
         if (n > 1u && parent_cost > 0.0f && widest > 0.0f) {
+            if (!costed)
+                for (size_t a = 0; a < 3; ++a) axes[a] = axis_costs(begin, end, a, centres, widest);
+            // axes and bucket boundaries in order, each replacing the best
+            // only when strictly cheaper
+            float best = std::numeric_limits<float>::infinity();
             for (size_t a = 0; a < 3; ++a) {
-                const float span = axis_value(extent, a);
-                if (!(span > 0.0f)) continue;
-                std::array<Slot, kBucketCount> slots{};
-                for (uint32_t i = begin; i < end; ++i) {
-                    const float rel = (axis_value(items[i].c, a) - axis_value(centres.lo, a)) / span;
-                    const size_t s = std::min<size_t>(
-                        static_cast<size_t>(std::max(rel, 0.0f) * static_cast<float>(kBucketCount)), kBucketCount - 1);
-                    slots[s].m = unite(slots[s].m, items[i].m);
-                    ++slots[s].count;
-                }
-                std::array<Slot, kBucketCount - 1> below{}, above{};
-                below[0] = slots[0];
-                for (size_t s = 1; s + 1 < kBucketCount; ++s)
-                    below[s] = Slot{unite(below[s - 1].m, slots[s].m), below[s - 1].count + slots[s].count};
-                above[kBucketCount - 2] = slots[kBucketCount - 1];
-                for (size_t s = kBucketCount - 2; s-- > 0;)
-                    above[s] = Slot{unite(above[s + 1].m, slots[s + 1].m), above[s + 1].count + slots[s + 1].count};
-                const float stretch = widest / span;
-                for (size_t s = 0; s + 1 < kBucketCount; ++s) {
-                    if (below[s].count == 0u || above[s].count == 0u) continue;
-                    const float c = stretch * (saoh_cost(below[s].m) + saoh_cost(above[s].m));
+                for (size_t b = 0; b + 1 < kBucketCount; ++b) {
+                    const float c = axes[a].cost[b];
                     if (c < parent_cost && c < best) {
                         best = c;
                         split_axis = a;
-                        middle = begin + below[s].count;
+                        middle = begin + axes[a].below[b];
                     }
                 }
             }
@@ -215,22 +272,46 @@ class TreeBuilder {
             leaf.first = begin;
             leaf.count = n;
             for (uint32_t i = begin; i < end; ++i) {
-                tree_of[items[i].id] = i;
-                leaf_of[items[i].id] = self;
+                (*tree_of)[it[i].id] = i;
+                (*leaf_of)[it[i].id] = self;
             }
             return self;
         }
         if (have_split) {
             const size_t a = split_axis;
-            std::nth_element(items.begin() + begin, items.begin() + middle, items.begin() + end,
+            std::nth_element(it.begin() + begin, it.begin() + middle, it.begin() + end,
                              [a](const Item &p, const Item &q) {
                                  const float pv = axis_value(p.c, a), qv = axis_value(q.c, a);
                                  return pv < qv || (pv == qv && p.id < q.id);
                              });
         }
         nodes[self].kind = kTreeInner;
-        const uint32_t l = local(begin, middle, self);
-        const uint32_t r = local(middle, end, self);
+        if (depth < kParallelDepth && end - middle >= kSubtreeItems && middle - begin >= kSubtreeItems) {
+            // the right subtree on its own builder, numbered from 0, then
+            // appended after the left subtree with its indices shifted
+            TreeBuilder right;
+            right.items = items;
+            right.tree_of = tree_of;
+            right.leaf_of = leaf_of;
+            right.leaf_size = leaf_size;
+            auto done = std::async(std::launch::async, [&] { right.local(middle, end, kInvalid, depth + 1u); });
+            const uint32_t l = local(begin, middle, self, depth + 1u);
+            done.get();
+            const uint32_t offset = static_cast<uint32_t>(nodes.size());
+            auto shift = [offset](uint32_t v) { return v == kInvalid ? kInvalid : v + offset; };
+            for (BuildNode node : right.nodes) {
+                node.parent = node.parent == kInvalid ? self : node.parent + offset;
+                node.left = shift(node.left);
+                node.right = shift(node.right);
+                nodes.push_back(node);
+            }
+            for (uint32_t i = middle; i < end; ++i) (*leaf_of)[it[i].id] += offset;
+            nodes[self].left = l;
+            nodes[self].right = offset;
+            return self;
+        }
+        const uint32_t l = local(begin, middle, self, depth + 1u);
+        const uint32_t r = local(middle, end, self, depth + 1u);
         nodes[self].left = l;
         nodes[self].right = r;
         return self;
@@ -256,7 +337,12 @@ void pack(const Measure &m, float4 &min_energy, float4 &max_theta_o, float4 &axi
 bool build_light_tree(const std::vector<TreeEmitterIn> &emitters, uint32_t max_leaf, LightTreeTables &out) {
     out = LightTreeTables{};
     const uint32_t count = static_cast<uint32_t>(emitters.size());
+    std::vector<Item> items;
+    std::vector<uint32_t> tree_of, leaf_of;
     TreeBuilder b;
+    b.items = &items;
+    b.tree_of = &tree_of;
+    b.leaf_of = &leaf_of;
     b.leaf_size = std::max(max_leaf, 1u);
     std::vector<bool> seen(count, false);
     for (const TreeEmitterIn &e : emitters) {
@@ -272,25 +358,25 @@ bool build_light_tree(const std::vector<TreeEmitterIn> &emitters, uint32_t max_l
         if (!all_finite(it.c)) it.c = V3{};
         it.id = e.emitter_id;
         it.distant = e.distant;
-        b.items.push_back(it);
+        items.push_back(it);
     }
     if (count == 0u) return true;
-    b.tree_of.assign(count, kInvalid);
-    b.leaf_of.assign(count, kInvalid);
+    tree_of.assign(count, kInvalid);
+    leaf_of.assign(count, kInvalid);
     const uint32_t distant_begin = static_cast<uint32_t>(
-        std::stable_partition(b.items.begin(), b.items.end(), [](const Item &it) { return !it.distant; }) -
-        b.items.begin());
+        std::stable_partition(items.begin(), items.end(), [](const Item &it) { return !it.distant; }) -
+        items.begin());
     const uint32_t root = b.new_node(kInvalid);
     b.nodes[root].kind = kTreeInner;
-    const uint32_t near = b.local(0u, distant_begin, root);
+    const uint32_t near = b.local(0u, distant_begin, root, 0u);
     const uint32_t far = b.new_node(root);
     b.nodes[far].kind = kTreeDistantNode;
     b.nodes[far].first = distant_begin;
     b.nodes[far].count = count - distant_begin;
     for (uint32_t i = distant_begin; i < count; ++i) {
-        b.nodes[far].m = unite(b.nodes[far].m, b.items[i].m);
-        b.tree_of[b.items[i].id] = i;
-        b.leaf_of[b.items[i].id] = far;
+        b.nodes[far].m = unite(b.nodes[far].m, items[i].m);
+        tree_of[items[i].id] = i;
+        leaf_of[items[i].id] = far;
     }
     b.nodes[root].left = near;
     b.nodes[root].right = far;
@@ -307,13 +393,13 @@ bool build_light_tree(const std::vector<TreeEmitterIn> &emitters, uint32_t max_l
     }
     out.emitters.resize(count);
     for (uint32_t i = 0; i < count; ++i) {
-        const Item &it = b.items[i];
+        const Item &it = items[i];
         TreeEmitter &g = out.emitters[i];
         pack(it.m, g.bounds_min_energy, g.bounds_max_theta_o, g.cone_axis_theta_e, g.trig);
         g.identity = make_uint4(it.id, measure_flags(it.m, it.distant), 0u, 0u);
     }
     out.mapping.resize(count);
-    for (uint32_t id = 0; id < count; ++id) out.mapping[id] = make_uint2(b.tree_of[id], b.leaf_of[id]);
+    for (uint32_t id = 0; id < count; ++id) out.mapping[id] = make_uint2(tree_of[id], leaf_of[id]);
     out.usable = b.nodes[root].m.energy > 0.0f;
     return true;
 }
