@@ -684,7 +684,42 @@ int psx_scene_intersect_device_noalloc(psx_scene *scene, const psx_ray_query *de
  *                       out position[3], direction[3], distance, pdf,
  *                           evaluation factor, valid (10)
  *                       (AreaLightSampling::from_position,
- *                        src/luisa/area_light_sampling.cpp) */
+ *                        src/luisa/area_light_sampling.cpp)
+ *   VOLUME              homogeneous volume transport (24 in, 24 out; unused
+ *                       floats are zero).  in[0] selects the function; a
+ *                       coefficient block C is sigma_t[3], sigma_s[3],
+ *                       emission[3], flags (bit 0 extinction, bit 1
+ *                       scatter, bit 2 emission):
+ *                       0 transmittance: sigma_t[3], distance -> [3]
+ *                       1 emission integral: C, distance -> [3]
+ *                       2 channel pdf: albedo[3], throughput[3] -> [3]
+ *                       3 channel sample: albedo[3], throughput[3], random
+ *                         -> channel, rescaled random, matched, pdf[3]
+ *                       4 bounded exponential sample: random, rate, min,
+ *                         max -> distance
+ *                       5 bounded exponential pdf: distance, rate[3], min,
+ *                         max -> [3]
+ *                       6 sample: C, distance, throughput[3], scatter
+ *                         random, channel random, terminate;
+ *                       7 the same with scatter probability[3] appended
+ *                         -> transmittance[3], emission[3], throughput[3],
+ *                         scatter probability[3], channel pdf[3],
+ *                         distance, event pdf, scatter random, reservoir
+ *                         random, channel, scattered, active
+ *                       8 scatter probability: C, distance, terminate,
+ *                         scattered radiance[3], transmitted radiance[3],
+ *                         majorant optical depth, guiding enabled -> [3]
+ *                       9 direct sampling preparation: method, random,
+ *                         enabled -> method, random, use MIS, enabled
+ *                       10 equiangular sample: origin[3], direction[3],
+ *                         light position[3], min, max, random -> distance,
+ *                         pdf, valid
+ *                       11 equiangular pdf: as 10 with distance for random
+ *                         -> pdf
+ *                       12 power heuristic: pdf, other pdf -> weight
+ *                       (src/luisa/homogeneous_volume_transport.cpp,
+ *                        src/luisa/volume_scatter_probability.cpp:20-98,
+ *                        src/luisa/volume_direct_sampling.cpp) */
 enum psx_probe_function {
     PSX_PROBE_FILM_CLAMP = 0,
     PSX_PROBE_T_OFFSET = 1,
@@ -694,7 +729,8 @@ enum psx_probe_function {
     PSX_PROBE_TRIANGLE_PDF = 5,
     PSX_PROBE_DIFFUSE_EVAL = 6,
     PSX_PROBE_SURFACE_SHADOW_ORIGIN = 7,
-    PSX_PROBE_AREA_SAMPLE = 8
+    PSX_PROBE_AREA_SAMPLE = 8,
+    PSX_PROBE_VOLUME = 9
 };
 /* Input and output floats per record of a probe function; 0 if unknown. */
 uint32_t psx_probe_input_width(uint32_t function);

@@ -4736,13 +4736,289 @@ static mat4 affine12(const float *a) {
     return m;
 }
 
+/* ---- homogeneous volume transport ---------------------------------------
+ * Restatement of the reference's analytic homogeneous-medium estimator
+ * (HomogeneousVolumeTransport, src/luisa/homogeneous_volume_transport.cpp),
+ * its scatter probability guide (src/luisa/volume_scatter_probability.cpp:
+ * 20-98) and direct-light distance proposals (src/luisa/
+ * volume_direct_sampling.cpp), in the reference's operation order.  Pinned by
+ * records 0-45 of tests/test_luisa_homogeneous_volume.cpp:1052-1100. */
+typedef struct {
+    v3 sigma_t, sigma_s, emission;
+    int has_extinction, has_scatter, has_emission;
+} vol_coeffs;
+
+static v3 vol_exp(v3 v) { return V3(expf(v.x), expf(v.y), expf(v.z)); }
+static float vol_channel(v3 v, uint32_t c) { return c < 2u ? (c == 0u ? v.x : v.y) : v.z; }
+static int vol_any_nonzero(v3 v) { return v.x != 0.0f || v.y != 0.0f || v.z != 0.0f; }
+static v3 vol_safe_divide3(v3 n, v3 d) {
+    return V3(d.x != 0.0f ? n.x / d.x : 0.0f, d.y != 0.0f ? n.y / d.y : 0.0f, d.z != 0.0f ? n.z / d.z : 0.0f);
+}
+static v3 vol_safe_divide1(v3 n, float d) { return d != 0.0f ? vdivs(n, d) : V3(0.0f, 0.0f, 0.0f); }
+static float clampf_(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+
+/* transmittance (homogeneous_volume_transport.cpp:28-32) */
+static v3 vol_transmittance(v3 sigma_t, float distance) { return vol_exp(vscale(vneg(sigma_t), distance)); }
+
+/* emission_integral: (1 - e^-tau) / sigma_t, its Taylor form below 1e-5
+ * optical depth, or the plain length without extinction (:33-57) */
+static v3 vol_emission_integral(const vol_coeffs *c, float distance) {
+    const v3 depth = vscale(c->sigma_t, distance);
+    const float d[3] = {depth.x, depth.y, depth.z}, s[3] = {c->sigma_t.x, c->sigma_t.y, c->sigma_t.z};
+    float a[3];
+    for (int i = 0; i < 3; ++i)
+        a[i] = d[i] > 1.0e-5f ? (1.0f - expf(-d[i])) / s[i] : distance * (1.0f - 0.5f * d[i]);
+    return vmul(c->emission, c->has_extinction ? V3(a[0], a[1], a[2]) : V3(distance, distance, distance));
+}
+
+/* channel_pdf: |throughput * albedo| normalised, uniform when the sum fails
+ * Cycles' (1 - sum) < 1 test (:58-74) */
+static v3 vol_channel_pdf(v3 albedo, v3 throughput) {
+    const v3 w = vmul(throughput, albedo);
+    const v3 weights = V3(fabsf(w.x), fabsf(w.y), fabsf(w.z));
+    const float sum = weights.x + weights.y + weights.z;
+    const float third = 1.0f / 3.0f;
+    return (1.0f - sum) < 1.0f ? vdivs(weights, sum) : V3(third, third, third);
+}
+
+typedef struct { uint32_t channel; v3 pdf; float random; int matched; } vol_channel_sample;
+
+/* sample_channel: inverse CDF over the channel pdf, random rescaled (:75-113) */
+static vol_channel_sample vol_sample_channel(v3 albedo, v3 throughput, float random) {
+    vol_channel_sample r;
+    r.pdf = vol_channel_pdf(albedo, throughput);
+    const int first = random < r.pdf.x;
+    const int second = !first && random < r.pdf.x + r.pdf.y;
+    const int third = !first && !second && random < r.pdf.x + r.pdf.y + r.pdf.z;
+    r.matched = first || second || third;
+    r.channel = first ? 0u : (second ? 1u : 2u);
+    const float lower = first ? 0.0f : (second ? r.pdf.x : r.pdf.x + r.pdf.y);
+    const float selected = first ? r.pdf.x : (second ? r.pdf.y : r.pdf.z);
+    r.random = r.matched ? (random - lower) / selected : random;
+    return r;
+}
+
+/* bounded_exponential_sample: inverse CDF of rate * e^(-rate t) on
+ * [minimum, maximum] (:114-133) */
+static float vol_bexp_sample(float random, float rate, float minimum, float maximum) {
+    const float attenuation = 1.0f - expf(rate * (minimum - maximum));
+    const float safe_rate = rate != 0.0f ? rate : 1.0f;
+    return clampf_(minimum - logf(1.0f - random * attenuation) / safe_rate, minimum, maximum);
+}
+
+/* bounded_exponential_pdf per channel rate (:134-152) */
+static v3 vol_bexp_pdf(float distance, v3 rate, float minimum, float maximum) {
+    const v3 neg = vneg(rate);
+    const v3 attenuation = vsub(vol_exp(vscale(neg, minimum)), vol_exp(vscale(neg, maximum)));
+    const v3 numerator = vmul(rate, vol_exp(vscale(neg, clampf_(distance, minimum, maximum))));
+    return vol_safe_divide3(numerator, attenuation);
+}
+
+static int vol_scatters(const vol_coeffs *c) { return c->has_scatter && vol_any_nonzero(c->sigma_s); }
+
+typedef struct {
+    v3 transmittance, emission, throughput, scatter_probability, channel_pdf;
+    float distance, event_pdf, scatter_random, reservoir_random;
+    uint32_t channel;
+    int scattered, active;
+} vol_sample_t;
+
+/* sample_with_probability (:201-341); sample() passes the unguided
+ * probability 1 - T (:153-200) */
+static vol_sample_t vol_sample_with_probability(const vol_coeffs *c, float distance, v3 throughput,
+                                                float scatter_random, float channel_random, v3 probability,
+                                                int terminate) {
+    const v3 one = V3(1.0f, 1.0f, 1.0f);
+    vol_sample_t r;
+    r.transmittance = vol_transmittance(c->sigma_t, distance);
+    r.emission = vmul(throughput, vol_emission_integral(c, distance));
+    const int eligible = !terminate && vol_scatters(c) && distance > 0.0f;
+    r.scatter_probability = eligible ? probability : V3(0.0f, 0.0f, 0.0f);
+    const v3 albedo = vol_safe_divide3(c->sigma_s, c->sigma_t);
+    const v3 ms_albedo = vmul(vmul(albedo, vsub(one, r.transmittance)), throughput);
+    const vol_channel_sample ch = vol_sample_channel(vadd(ms_albedo, r.transmittance), throughput, channel_random);
+    const float channel_sp = vol_channel(r.scatter_probability, ch.channel);
+    r.scattered = eligible && scatter_random < channel_sp;
+    const float rescaled = scatter_random / (r.scattered ? channel_sp : 1.0f);
+    const float sampled = vol_bexp_sample(rescaled, vol_channel(c->sigma_t, ch.channel), 0.0f, distance);
+    const float scatter_pdf =
+        vdot(vmul(vol_bexp_pdf(sampled, c->sigma_t, 0.0f, distance), r.scatter_probability), ch.pdf);
+    const v3 scatter_tp =
+        vmul(throughput, vol_safe_divide1(vmul(c->sigma_s, vol_transmittance(c->sigma_t, sampled)), scatter_pdf));
+    const float transmit_pdf = vdot(vsub(one, r.scatter_probability), ch.pdf);
+    const v3 transmit_tp = vmul(throughput, vol_safe_divide1(r.transmittance, transmit_pdf));
+    const float transmit_rescaled =
+        (scatter_random - channel_sp) / ((!r.scattered && eligible) ? 1.0f - channel_sp : 1.0f);
+    r.throughput = r.scattered ? scatter_tp : transmit_tp;
+    r.channel_pdf = ch.pdf;
+    r.distance = r.scattered ? sampled : distance;
+    r.event_pdf = r.scattered ? scatter_pdf : transmit_pdf;
+    r.scatter_random = r.scattered ? rescaled : transmit_rescaled;
+    r.reservoir_random = ch.random;
+    r.channel = ch.channel;
+    r.active = c->has_extinction || c->has_scatter || c->has_emission;
+    return r;
+}
+
+static vol_sample_t vol_sample(const vol_coeffs *c, float distance, v3 throughput, float scatter_random,
+                               float channel_random, int terminate) {
+    const v3 p = (!terminate && vol_scatters(c)) ? vsub(V3(1.0f, 1.0f, 1.0f), vol_transmittance(c->sigma_t, distance))
+                                                 : V3(0.0f, 0.0f, 0.0f);
+    return vol_sample_with_probability(c, distance, throughput, scatter_random, channel_random, p, terminate);
+}
+
+/* HomogeneousVolumeScatterProbability::evaluate
+ * (volume_scatter_probability.cpp:20-98) */
+static v3 vol_scatter_probability(const vol_coeffs *c, float distance, int terminate, v3 scattered_radiance,
+                                  v3 transmitted_radiance, float majorant, int enabled) {
+    const int attenuation_only = terminate || !c->has_scatter || !vol_any_nonzero(c->sigma_s);
+    const v3 attenuation = vsub(V3(1.0f, 1.0f, 1.0f), vol_exp(vscale(vneg(c->sigma_t), distance)));
+    const v3 radiance = vadd(transmitted_radiance, scattered_radiance);
+    const int history_empty = !vol_any_nonzero(radiance);
+    const v3 empty_p = V3(c->sigma_t.x > 0.0f ? 0.5f : 0.0f, c->sigma_t.y > 0.0f ? 0.5f : 0.0f,
+                          c->sigma_t.z > 0.0f ? 0.5f : 0.0f);
+    const float optical = 1.0f - expf(-majorant);
+    const float scale = fmaxf(attenuation.x, fmaxf(attenuation.y, attenuation.z)) / (optical != 0.0f ? optical : 1.0f);
+    const v3 h = vscale(vol_safe_divide3(scattered_radiance, radiance), scale);
+    const v3 history_p = V3(clampf_(h.x, 0.0f, 1.0f), clampf_(h.y, 0.0f, 1.0f), clampf_(h.z, 0.0f, 1.0f));
+    const v3 guided = history_empty ? empty_p : history_p;
+    const v3 defensive = vadd(vscale(attenuation, 0.25f), vscale(guided, 0.75f));
+    if (attenuation_only) return V3(0.0f, 0.0f, 0.0f);
+    return enabled ? defensive : attenuation;
+}
+
+/* VolumeDirectSampling::prepare (volume_direct_sampling.cpp): MIS splits the
+ * random number between distance (< 0.5) and equiangular sampling */
+static void vol_direct_prepare(uint32_t requested, float random, int enabled, float *out) {
+    const int use_mis = requested == 3u;
+    const int choose_distance = random < 0.5f;
+    const uint32_t method = use_mis ? (choose_distance ? 1u : 2u) : requested;
+    const float prepared = use_mis ? (choose_distance ? random * 2.0f : (random - 0.5f) * 2.0f) : random;
+    const int active = enabled && requested != 0u;
+    out[0] = (float)(active ? method : 0u);
+    out[1] = prepared;
+    out[2] = active && use_mis ? 1.0f : 0.0f;
+    out[3] = active ? 1.0f : 0.0f;
+}
+
+typedef struct {
+    float delta, perpendicular, theta_min, theta_max, theta_range;
+    int valid, uniform;
+} vol_equiangular_geometry;
+
+static vol_equiangular_geometry vol_equiangular(v3 origin, v3 direction, v3 light, float minimum, float maximum) {
+    vol_equiangular_geometry g;
+    const v3 offset = vsub(light, origin);
+    g.delta = vdot(offset, direction);
+    const v3 perp = vsub(offset, vscale(direction, g.delta));
+    g.perpendicular = sqrtf(vdot(perp, perp));
+    g.valid = minimum < maximum && g.perpendicular != 0.0f;
+    const float safe = g.valid ? g.perpendicular : 1.0f;
+    g.theta_min = atan2f(minimum - g.delta, safe);
+    g.theta_max = atan2f(maximum - g.delta, safe);
+    g.theta_range = g.theta_max - g.theta_min;
+    g.uniform = g.valid && g.theta_range < 1.0e-6f;
+    return g;
+}
+
+static float vol_uniform_pdf(float minimum, float maximum) {
+    const float length = maximum - minimum;
+    return length != 0.0f ? 1.0f / length : 0.0f;
+}
+
+static void vol_sample_equiangular(v3 origin, v3 direction, v3 light, float minimum, float maximum, float random,
+                                   float *out) {
+    const vol_equiangular_geometry g = vol_equiangular(origin, direction, light, minimum, maximum);
+    const float uniform_distance = minimum + random * (maximum - minimum);
+    const float angle = random * g.theta_max + (1.0f - random) * g.theta_min;
+    const float offset = g.perpendicular * tanf(angle);
+    const float sampled = clampf_(g.delta + offset, minimum, maximum);
+    const float denominator = g.theta_range * (g.perpendicular * g.perpendicular + offset * offset);
+    const float analytic = g.perpendicular / (denominator != 0.0f ? denominator : 1.0f);
+    const float pdf = g.uniform ? vol_uniform_pdf(minimum, maximum) : analytic;
+    out[0] = g.valid ? (g.uniform ? uniform_distance : sampled) : 0.0f;
+    out[1] = g.valid ? pdf : 0.0f;
+    out[2] = g.valid ? 1.0f : 0.0f;
+}
+
+static float vol_equiangular_pdf(v3 origin, v3 direction, v3 light, float minimum, float maximum, float distance) {
+    const vol_equiangular_geometry g = vol_equiangular(origin, direction, light, minimum, maximum);
+    const float offset = distance - g.delta;
+    const float denominator = g.theta_range * (g.perpendicular * g.perpendicular + offset * offset);
+    const float analytic = g.perpendicular / (denominator != 0.0f ? denominator : 1.0f);
+    const float pdf = g.uniform ? vol_uniform_pdf(minimum, maximum) : analytic;
+    return g.valid ? pdf : 0.0f;
+}
+
+static float vol_power_heuristic(float a, float b) {
+    const float a2 = a * a, b2 = b * b, sum = a2 + b2;
+    return sum > 0.0f ? a2 / sum : 0.0f;
+}
+
+static vol_coeffs vol_block(const float *in) {
+    vol_coeffs c;
+    const uint32_t flags = (uint32_t)in[9];
+    c.sigma_t = load3(in);
+    c.sigma_s = load3(in + 3);
+    c.emission = load3(in + 6);
+    c.has_extinction = (flags & 1u) != 0u;
+    c.has_scatter = (flags & 2u) != 0u;
+    c.has_emission = (flags & 4u) != 0u;
+    return c;
+}
+
+static void store3(float *o, v3 v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
+
+/* PSX_PROBE_VOLUME record (layouts in include/psx.h) */
+static void volume_probe(const float *in, float *out) {
+    switch ((uint32_t)in[0]) {
+    case 0u: store3(out, vol_transmittance(load3(in + 1), in[4])); break;
+    case 1u: { const vol_coeffs c = vol_block(in + 1); store3(out, vol_emission_integral(&c, in[11])); break; }
+    case 2u: store3(out, vol_channel_pdf(load3(in + 1), load3(in + 4))); break;
+    case 3u: {
+        const vol_channel_sample s = vol_sample_channel(load3(in + 1), load3(in + 4), in[7]);
+        out[0] = (float)s.channel; out[1] = s.random; out[2] = s.matched ? 1.0f : 0.0f;
+        store3(out + 3, s.pdf);
+        break;
+    }
+    case 4u: out[0] = vol_bexp_sample(in[1], in[2], in[3], in[4]); break;
+    case 5u: store3(out, vol_bexp_pdf(in[1], load3(in + 2), in[5], in[6])); break;
+    case 6u:
+    case 7u: {
+        const vol_coeffs c = vol_block(in + 1);
+        const vol_sample_t h = (uint32_t)in[0] == 6u
+            ? vol_sample(&c, in[11], load3(in + 12), in[15], in[16], in[17] != 0.0f)
+            : vol_sample_with_probability(&c, in[11], load3(in + 12), in[15], in[16], load3(in + 18), in[17] != 0.0f);
+        store3(out, h.transmittance);
+        store3(out + 3, h.emission);
+        store3(out + 6, h.throughput);
+        store3(out + 9, h.scatter_probability);
+        store3(out + 12, h.channel_pdf);
+        out[15] = h.distance; out[16] = h.event_pdf; out[17] = h.scatter_random; out[18] = h.reservoir_random;
+        out[19] = (float)h.channel; out[20] = h.scattered ? 1.0f : 0.0f; out[21] = h.active ? 1.0f : 0.0f;
+        break;
+    }
+    case 8u: {
+        const vol_coeffs c = vol_block(in + 1);
+        store3(out, vol_scatter_probability(&c, in[11], in[12] != 0.0f, load3(in + 13), load3(in + 16), in[19],
+                                            in[20] != 0.0f));
+        break;
+    }
+    case 9u: vol_direct_prepare((uint32_t)in[1], in[2], in[3] != 0.0f, out); break;
+    case 10u: vol_sample_equiangular(load3(in + 1), load3(in + 4), load3(in + 7), in[10], in[11], in[12], out); break;
+    case 11u: out[0] = vol_equiangular_pdf(load3(in + 1), load3(in + 4), load3(in + 7), in[10], in[11], in[12]); break;
+    case 12u: out[0] = vol_power_heuristic(in[1], in[2]); break;
+    default: break;
+    }
+}
+
 uint32_t oracle_probe_input_width(uint32_t function) {
-    static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u, 21u};
+    static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u, 21u, 24u};
     return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
 }
 
 uint32_t oracle_probe_output_width(uint32_t function) {
-    static const uint32_t w[] = {3u, 1u, 4u, 4u, 10u, 3u, 4u, 4u, 10u};
+    static const uint32_t w[] = {3u, 1u, 4u, 4u, 10u, 3u, 4u, 4u, 10u, 24u};
     return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
 }
 
@@ -4845,6 +5121,10 @@ int oracle_probe(uint32_t function, const float *inputs, uint32_t count, float *
             out[9] = s.valid ? 1.0f : 0.0f;
             break;
         }
+        case PSX_PROBE_VOLUME:
+            memset(out, 0, sizeof(float) * wo);
+            volume_probe(in, out);
+            break;
         default:
             return -1;
         }

@@ -14,6 +14,7 @@
 #include "../../include/psx.h"
 #include "psx_launch.h"
 #include "psx_path.h"
+#include "psx_volume.h"
 
 namespace psx {
 namespace {
@@ -106,6 +107,94 @@ __device__ __forceinline__ void out3(float *o, f3 v) {
     o[0] = v.x;
     o[1] = v.y;
     o[2] = v.z;
+}
+
+__device__ __forceinline__ VolumeCoefficients volume_block(const float *in) {
+    const uint32_t flags = static_cast<uint32_t>(in[9]);
+    return VolumeCoefficients{in3(in), in3(in + 3), in3(in + 6), (flags & 1u) != 0u, (flags & 2u) != 0u,
+                              (flags & 4u) != 0u};
+}
+
+// PSX_PROBE_VOLUME: in[0] names the function, the rest its arguments
+// (layouts in include/psx.h)
+__device__ void volume_probe(const float *in, float *out) {
+    switch (static_cast<uint32_t>(in[0])) {
+    case 0u:
+        out3(out, volume_transmittance(in3(in + 1), in[4]));
+        break;
+    case 1u:
+        out3(out, volume_emission_integral(volume_block(in + 1), in[11]));
+        break;
+    case 2u:
+        out3(out, volume_channel_pdf(in3(in + 1), in3(in + 4)));
+        break;
+    case 3u: {
+        const VolumeChannelSample c = volume_sample_channel(in3(in + 1), in3(in + 4), in[7]);
+        out[0] = static_cast<float>(c.channel);
+        out[1] = c.random;
+        out[2] = c.matched ? 1.0f : 0.0f;
+        out3(out + 3, c.pdf);
+        break;
+    }
+    case 4u:
+        out[0] = volume_bounded_exponential_sample(in[1], in[2], in[3], in[4]);
+        break;
+    case 5u:
+        out3(out, volume_bounded_exponential_pdf(in[1], in3(in + 2), in[5], in[6]));
+        break;
+    case 6u:
+    case 7u: {
+        const VolumeCoefficients c = volume_block(in + 1);
+        const HomogeneousSample h =
+            static_cast<uint32_t>(in[0]) == 6u
+                ? volume_sample(c, in[11], in3(in + 12), in[15], in[16], in[17] != 0.0f)
+                : volume_sample_with_probability(c, in[11], in3(in + 12), in[15], in[16], in3(in + 18),
+                                                 in[17] != 0.0f);
+        out3(out, h.transmittance);
+        out3(out + 3, h.emission);
+        out3(out + 6, h.throughput);
+        out3(out + 9, h.scatter_probability);
+        out3(out + 12, h.channel_pdf);
+        out[15] = h.distance;
+        out[16] = h.event_pdf;
+        out[17] = h.scatter_random;
+        out[18] = h.reservoir_random;
+        out[19] = static_cast<float>(h.channel);
+        out[20] = h.scattered ? 1.0f : 0.0f;
+        out[21] = h.active ? 1.0f : 0.0f;
+        break;
+    }
+    case 8u: {
+        const ScatterGuide g{in3(in + 13), in3(in + 16), in[19], in[20] != 0.0f};
+        out3(out, volume_scatter_probability(volume_block(in + 1), in[11], in[12] != 0.0f, g));
+        break;
+    }
+    case 9u: {
+        const DirectSamplingState d =
+            volume_direct_prepare(static_cast<uint32_t>(in[1]), in[2], in[3] != 0.0f);
+        out[0] = static_cast<float>(d.method);
+        out[1] = d.random;
+        out[2] = d.use_mis ? 1.0f : 0.0f;
+        out[3] = d.enabled ? 1.0f : 0.0f;
+        break;
+    }
+    case 10u: {
+        const EquiangularSample e =
+            volume_sample_equiangular(in3(in + 1), in3(in + 4), in3(in + 7), in[10], in[11], in[12]);
+        out[0] = e.distance;
+        out[1] = e.pdf;
+        out[2] = e.valid ? 1.0f : 0.0f;
+        break;
+    }
+    case 11u:
+        out[0] = volume_equiangular_pdf(in3(in + 1), in3(in + 4), in3(in + 7), in[10], in[11], in[12]);
+        break;
+    case 12u:
+        out[0] = volume_power_heuristic(in[1], in[2]);
+        break;
+    default:
+        break;
+    }
 }
 
 // record layouts of psx_backend_probe (include/psx.h)
@@ -203,6 +292,9 @@ __global__ void probe_kernel(uint32_t function, const float *inputs, uint32_t in
         out[9] = s.valid ? 1.0f : 0.0f;
         break;
     }
+    case PSX_PROBE_VOLUME:
+        volume_probe(in, out);
+        break;
     default:
         break;
     }

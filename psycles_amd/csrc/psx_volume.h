@@ -1,0 +1,279 @@
+// psx_volume.h - homogeneous volume transport primitives (device).
+//
+// The analytic homogeneous-medium estimator of Cycles as the reference's
+// HomogeneousVolumeTransport states it (src/luisa/homogeneous_volume_transport.cpp):
+// segment transmittance and emission, the throughput-weighted channel pick,
+// the bounded free-flight distance sampler and its pdf, the
+// scatter-versus-transmit decision with an optional guided scatter
+// probability; the scatter probability guide
+// (HomogeneousVolumeScatterProbability::evaluate,
+// src/luisa/volume_scatter_probability.cpp:20-98); and the direct-light
+// distance proposals (VolumeDirectSampling, src/luisa/volume_direct_sampling.cpp):
+// technique preparation, equiangular sampling and pdf, the power heuristic.
+//
+// Each function keeps the reference's operation order, so the float results
+// follow its arithmetic.  Include after psx_path.h (f3 and its operators).
+#pragma once
+
+namespace psx {
+
+// volume_sample_* (include/psycles/luisa/volume_stack.h:19-25)
+constexpr uint32_t kVolumeSampleNone = 0u, kVolumeSampleDistance = 1u, kVolumeSampleEquiangular = 2u,
+                   kVolumeSampleMis = 3u;
+
+struct VolumeCoefficients {
+    f3 sigma_t, sigma_s, emission;
+    bool has_extinction, has_scatter, has_emission;
+};
+
+struct VolumeChannelSample {
+    uint32_t channel;
+    f3 pdf;
+    float random;
+    bool matched;
+};
+
+struct HomogeneousSample {
+    f3 transmittance, emission, throughput, scatter_probability, channel_pdf;
+    float distance, event_pdf, scatter_random, reservoir_random;
+    uint32_t channel;
+    bool scattered, active;
+};
+
+struct ScatterGuide {
+    f3 scattered_radiance, transmitted_radiance;
+    float majorant_optical_depth;
+    bool enabled;
+};
+
+struct DirectSamplingState {
+    uint32_t method;
+    float random;
+    bool use_mis, enabled;
+};
+
+struct EquiangularSample {
+    float distance, pdf;
+    bool valid;
+};
+
+__device__ __forceinline__ f3 vol_exp(f3 v) { return V(expf(v.x), expf(v.y), expf(v.z)); }
+__device__ __forceinline__ f3 vol_neg(f3 v) { return V(-v.x, -v.y, -v.z); }
+__device__ __forceinline__ float vol_channel(f3 v, uint32_t c) { return c < 2u ? (c == 0u ? v.x : v.y) : v.z; }
+__device__ __forceinline__ bool vol_any_nonzero(f3 v) { return v.x != 0.0f || v.y != 0.0f || v.z != 0.0f; }
+
+// numerator / denominator per component, `fallback` where the denominator is 0
+__device__ __forceinline__ f3 vol_safe_divide(f3 n, f3 d, float fallback = 0.0f) {
+    return V(d.x != 0.0f ? n.x / d.x : fallback, d.y != 0.0f ? n.y / d.y : fallback,
+             d.z != 0.0f ? n.z / d.z : fallback);
+}
+__device__ __forceinline__ f3 vol_safe_divide(f3 n, float d) {
+    return d != 0.0f ? n / d : V(0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ f3 volume_transmittance(f3 sigma_t, float distance) {
+    return vol_exp(vol_neg(sigma_t) * distance);
+}
+
+__device__ inline f3 volume_emission_integral(const VolumeCoefficients &c, float distance) {
+    const f3 depth = c.sigma_t * distance;
+    f3 attenuated;
+    const float d[3] = {depth.x, depth.y, depth.z}, s[3] = {c.sigma_t.x, c.sigma_t.y, c.sigma_t.z};
+    float a[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const bool analytic = d[i] > 1.0e-5f;
+        a[i] = analytic ? (1.0f - expf(-d[i])) / s[i] : distance * (1.0f - 0.5f * d[i]);
+    }
+    attenuated = V(a[0], a[1], a[2]);
+    return c.emission * (c.has_extinction ? attenuated : V(distance, distance, distance));
+}
+
+__device__ inline f3 volume_channel_pdf(f3 albedo, f3 throughput) {
+    const f3 w = throughput * albedo;
+    const f3 weights = V(fabsf(w.x), fabsf(w.y), fabsf(w.z));
+    const float sum = weights.x + weights.y + weights.z;
+    // (1 - sum) < 1 rather than sum > 0: denormal sums count as empty
+    const bool usable = (1.0f - sum) < 1.0f;
+    const float third = 1.0f / 3.0f;
+    return usable ? weights / sum : V(third, third, third);
+}
+
+__device__ inline VolumeChannelSample volume_sample_channel(f3 albedo, f3 throughput, float random) {
+    const f3 pdf = volume_channel_pdf(albedo, throughput);
+    const bool first = random < pdf.x;
+    const bool second = !first && random < pdf.x + pdf.y;
+    const bool third = !first && !second && random < pdf.x + pdf.y + pdf.z;
+    const bool matched = first || second || third;
+    const uint32_t channel = (first || second) ? (first ? 0u : 1u) : 2u;
+    const float lower = (first || second) ? (first ? 0.0f : pdf.x) : pdf.x + pdf.y;
+    const float selected = (first || second) ? (first ? pdf.x : pdf.y) : pdf.z;
+    const float rescaled = (random - lower) / (matched ? selected : 1.0f);
+    return VolumeChannelSample{channel, pdf, matched ? rescaled : random, matched};
+}
+
+__device__ inline float volume_bounded_exponential_sample(float random, float rate, float minimum, float maximum) {
+    const float attenuation = 1.0f - expf(rate * (minimum - maximum));
+    const float safe_rate = rate != 0.0f ? rate : 1.0f;
+    return fminf(fmaxf(minimum - logf(1.0f - random * attenuation) / safe_rate, minimum), maximum);
+}
+
+__device__ inline f3 volume_bounded_exponential_pdf(float distance, f3 rate, float minimum, float maximum) {
+    const f3 neg = vol_neg(rate);
+    const f3 attenuation = vol_exp(neg * minimum) - vol_exp(neg * maximum);
+    const float t = fminf(fmaxf(distance, minimum), maximum);
+    const f3 numerator = rate * vol_exp(neg * t);
+    return vol_safe_divide(numerator, attenuation);
+}
+
+__device__ __forceinline__ bool volume_scatters(const VolumeCoefficients &c) {
+    return c.has_scatter && vol_any_nonzero(c.sigma_s);
+}
+
+__device__ inline f3 volume_unguided_scatter_probability(const VolumeCoefficients &c, float distance,
+                                                         bool terminate) {
+    return (!terminate && volume_scatters(c)) ? V(1.0f, 1.0f, 1.0f) - volume_transmittance(c.sigma_t, distance)
+                                              : V(0.0f, 0.0f, 0.0f);
+}
+
+// The scatter-versus-transmit estimator over one segment of `distance` with
+// per-channel scatter probability `scatter_probability`.
+__device__ inline HomogeneousSample volume_sample_with_probability(const VolumeCoefficients &c, float distance,
+                                                                   f3 throughput, float scatter_random,
+                                                                   float channel_random, f3 scatter_probability,
+                                                                   bool terminate) {
+    const f3 one = V(1.0f, 1.0f, 1.0f);
+    const f3 segment_t = volume_transmittance(c.sigma_t, distance);
+    const f3 segment_emission = throughput * volume_emission_integral(c, distance);
+    const bool eligible = !terminate && volume_scatters(c) && distance > 0.0f;
+    const f3 sp = eligible ? scatter_probability : V(0.0f, 0.0f, 0.0f);
+    const f3 albedo = vol_safe_divide(c.sigma_s, c.sigma_t);
+    const f3 ms_albedo = albedo * (one - segment_t) * throughput;
+    const VolumeChannelSample ch = volume_sample_channel(ms_albedo + segment_t, throughput, channel_random);
+    const float channel_sp = vol_channel(sp, ch.channel);
+    const bool scattered = eligible && scatter_random < channel_sp;
+    const float rescaled_scatter = scatter_random / (scattered ? channel_sp : 1.0f);
+    const float rate = vol_channel(c.sigma_t, ch.channel);
+    const float sampled = volume_bounded_exponential_sample(rescaled_scatter, rate, 0.0f, distance);
+    const float scatter_pdf = dot(volume_bounded_exponential_pdf(sampled, c.sigma_t, 0.0f, distance) * sp, ch.pdf);
+    const f3 scatter_tp =
+        throughput * vol_safe_divide(c.sigma_s * volume_transmittance(c.sigma_t, sampled), scatter_pdf);
+    const float transmit_pdf = dot(one - sp, ch.pdf);
+    const f3 transmit_tp = throughput * vol_safe_divide(segment_t, transmit_pdf);
+    const float transmit_rescaled = (scatter_random - channel_sp) / ((!scattered && eligible) ? 1.0f - channel_sp : 1.0f);
+    HomogeneousSample r;
+    r.transmittance = segment_t;
+    r.emission = segment_emission;
+    r.throughput = scattered ? scatter_tp : transmit_tp;
+    r.scatter_probability = sp;
+    r.channel_pdf = ch.pdf;
+    r.distance = scattered ? sampled : distance;
+    r.event_pdf = scattered ? scatter_pdf : transmit_pdf;
+    r.scatter_random = scattered ? rescaled_scatter : transmit_rescaled;
+    r.reservoir_random = ch.random;
+    r.channel = ch.channel;
+    r.scattered = scattered;
+    r.active = c.has_extinction || c.has_scatter || c.has_emission;
+    return r;
+}
+
+__device__ inline HomogeneousSample volume_sample(const VolumeCoefficients &c, float distance, f3 throughput,
+                                                  float scatter_random, float channel_random, bool terminate) {
+    return volume_sample_with_probability(c, distance, throughput, scatter_random, channel_random,
+                                          volume_unguided_scatter_probability(c, distance, terminate), terminate);
+}
+
+// Scatter probability of one segment from the primary ray's guide: the
+// attenuation alone without guiding, a defensive mix of attenuation and the
+// guide's scattered share otherwise.
+__device__ inline f3 volume_scatter_probability(const VolumeCoefficients &c, float distance, bool terminate,
+                                                const ScatterGuide &g) {
+    const bool attenuation_only = terminate || !c.has_scatter || !vol_any_nonzero(c.sigma_s);
+    const f3 one = V(1.0f, 1.0f, 1.0f);
+    const f3 attenuation = one - vol_exp(vol_neg(c.sigma_t) * distance);
+    const f3 radiance = g.transmitted_radiance + g.scattered_radiance;
+    const bool history_empty = !vol_any_nonzero(radiance);
+    const f3 empty_p = V(c.sigma_t.x > 0.0f ? 0.5f : 0.0f, c.sigma_t.y > 0.0f ? 0.5f : 0.0f,
+                         c.sigma_t.z > 0.0f ? 0.5f : 0.0f);
+    const float optical = 1.0f - expf(-g.majorant_optical_depth);
+    const float safe_optical = optical != 0.0f ? optical : 1.0f;
+    const float scale = fmaxf(attenuation.x, fmaxf(attenuation.y, attenuation.z)) / safe_optical;
+    const f3 h = vol_safe_divide(g.scattered_radiance, radiance) * scale;
+    const f3 history_p = V(fminf(fmaxf(h.x, 0.0f), 1.0f), fminf(fmaxf(h.y, 0.0f), 1.0f), fminf(fmaxf(h.z, 0.0f), 1.0f));
+    const f3 guided = history_empty ? empty_p : history_p;
+    const f3 defensive = attenuation * 0.25f + guided * 0.75f;
+    const f3 probability = g.enabled ? defensive : attenuation;
+    return attenuation_only ? V(0.0f, 0.0f, 0.0f) : probability;
+}
+
+__device__ inline DirectSamplingState volume_direct_prepare(uint32_t requested, float random, bool enabled) {
+    const bool use_mis = requested == kVolumeSampleMis;
+    const bool choose_distance = random < 0.5f;
+    const uint32_t mis_method = choose_distance ? kVolumeSampleDistance : kVolumeSampleEquiangular;
+    const float mis_random = choose_distance ? random * 2.0f : (random - 0.5f) * 2.0f;
+    const uint32_t method = use_mis ? mis_method : requested;
+    const bool active = enabled && requested != kVolumeSampleNone;
+    return DirectSamplingState{active ? method : kVolumeSampleNone, use_mis ? mis_random : random,
+                               active && use_mis, active};
+}
+
+struct EquiangularGeometry {
+    float delta, perpendicular, theta_min, theta_max, theta_range;
+    bool valid, uniform;
+};
+
+__device__ inline EquiangularGeometry equiangular_geometry(f3 origin, f3 direction, f3 light, float minimum,
+                                                           float maximum) {
+    const f3 offset = light - origin;
+    const float delta = dot(offset, direction);
+    const f3 perp = offset - direction * delta;
+    const float pd = sqrtf(dot(perp, perp));
+    const bool valid = minimum < maximum && pd != 0.0f;
+    const float safe = valid ? pd : 1.0f;
+    EquiangularGeometry g;
+    g.delta = delta;
+    g.perpendicular = pd;
+    g.theta_min = atan2f(minimum - delta, safe);
+    g.theta_max = atan2f(maximum - delta, safe);
+    g.theta_range = g.theta_max - g.theta_min;
+    g.valid = valid;
+    g.uniform = valid && g.theta_range < 1.0e-6f;
+    return g;
+}
+
+__device__ __forceinline__ float uniform_interval_pdf(float minimum, float maximum) {
+    const float length = maximum - minimum;
+    return length != 0.0f ? 1.0f / length : 0.0f;
+}
+
+__device__ inline EquiangularSample volume_sample_equiangular(f3 origin, f3 direction, f3 light, float minimum,
+                                                              float maximum, float random) {
+    const EquiangularGeometry g = equiangular_geometry(origin, direction, light, minimum, maximum);
+    const float uniform_distance = minimum + random * (maximum - minimum);
+    const float angle = random * g.theta_max + (1.0f - random) * g.theta_min;
+    const float offset = g.perpendicular * tanf(angle);
+    const float sampled = fminf(fmaxf(g.delta + offset, minimum), maximum);
+    const float denominator = g.theta_range * (g.perpendicular * g.perpendicular + offset * offset);
+    const float analytic = g.perpendicular / (denominator != 0.0f ? denominator : 1.0f);
+    const float pdf = g.uniform ? uniform_interval_pdf(minimum, maximum) : analytic;
+    return EquiangularSample{g.valid ? (g.uniform ? uniform_distance : sampled) : 0.0f, g.valid ? pdf : 0.0f,
+                             g.valid};
+}
+
+__device__ inline float volume_equiangular_pdf(f3 origin, f3 direction, f3 light, float minimum, float maximum,
+                                               float distance) {
+    const EquiangularGeometry g = equiangular_geometry(origin, direction, light, minimum, maximum);
+    const float offset = distance - g.delta;
+    const float denominator = g.theta_range * (g.perpendicular * g.perpendicular + offset * offset);
+    const float analytic = g.perpendicular / (denominator != 0.0f ? denominator : 1.0f);
+    const float pdf = g.uniform ? uniform_interval_pdf(minimum, maximum) : analytic;
+    return g.valid ? pdf : 0.0f;
+}
+
+__device__ __forceinline__ float volume_power_heuristic(float selected, float competing) {
+    const float a = selected * selected, b = competing * competing;
+    const float sum = a + b;
+    return sum > 0.0f ? a / sum : 0.0f;
+}
+
+}  // namespace psx

@@ -1,0 +1,96 @@
+"""Homogeneous volume transport in the oracle (PSX_PROBE_VOLUME).
+
+Pinned by records 0-45 of the reference's fixture
+(tests/test_luisa_homogeneous_volume.cpp:1052-1100, transcribed with their
+inputs in tests/golden/homogeneous_volume.json) under the reference's own
+tolerance, and by what the mathematics fixes: the bounded free-flight
+sampler inverts its CDF and its pdf integrates to one on the interval, the
+equiangular pdf integrates to one and is the derivative of its sampler's
+inverse, channel pdfs sum to one, and the power heuristic is symmetric.
+"""
+import numpy as np
+import pytest
+
+import oracle_binding as ob
+import volume_cases as vc
+from psycles_amd import abi
+
+
+def test_reference_records():
+    doc = vc.golden()
+    tol = doc["tolerance"]
+    bad = []
+    for r in doc["records"]:
+        out = ob.probe(abi.PROBE_VOLUME, [r["inputs"]])[0]
+        got = vc.picked(out, r["outputs"])
+        for g, e in zip(got, r["expected"]):
+            if not vc.approximately_equal(g, e, tol):
+                bad.append((r["record"], got.tolist(), r["expected"]))
+                break
+    assert not bad, bad
+
+
+def _row(op, *args):
+    x = np.zeros(24, np.float32)
+    x[0] = op
+    x[1:1 + len(args)] = args
+    return x
+
+
+@pytest.mark.parametrize("rate,lo,hi", [(1.7, 0.2, 2.3), (0.05, 0.0, 10.0), (6.0, 1.0, 1.5)])
+def test_bounded_exponential_is_normalised_and_inverted(rate, lo, hi):
+    ts = np.linspace(lo, hi, 20001)
+    pdf = ob.probe(abi.PROBE_VOLUME, [_row(5, t, rate, rate, rate, lo, hi) for t in ts])[:, 0].astype(np.float64)
+    integral = np.trapezoid(pdf, ts)
+    assert abs(integral - 1.0) < 1e-4
+    for u in (0.1, 0.5, 0.9):
+        t = float(ob.probe(abi.PROBE_VOLUME, [_row(4, u, rate, lo, hi)])[0, 0])
+        cdf = (np.exp(-rate * lo) - np.exp(-rate * t)) / (np.exp(-rate * lo) - np.exp(-rate * hi))
+        assert abs(cdf - u) < 1e-5
+
+
+def test_equiangular_pdf_is_normalised_and_matches_sampler():
+    origin, direction, light, lo, hi = (0.3, -0.2, 0.0), (0.0, 0.6, 0.8), (1.0, 2.0, 0.5), -1.0, 4.0
+    ts = np.linspace(lo, hi, 40001)
+    pdf = ob.probe(abi.PROBE_VOLUME, [_row(11, *origin, *direction, *light, lo, hi, t) for t in ts])[:, 0]
+    assert abs(np.trapezoid(pdf.astype(np.float64), ts) - 1.0) < 1e-4
+    us = np.linspace(0.05, 0.95, 19)
+    samples = ob.probe(abi.PROBE_VOLUME, [_row(10, *origin, *direction, *light, lo, hi, u) for u in us])
+    assert np.all(samples[:, 2] == 1.0)
+    # the sampler's pdf is the pdf at its sample, and dt/du = 1 / pdf
+    at = ob.probe(abi.PROBE_VOLUME, [_row(11, *origin, *direction, *light, lo, hi, t) for t in samples[:, 0]])[:, 0]
+    assert np.allclose(samples[:, 1], at, rtol=1e-5)
+    h = 1e-3
+    up = ob.probe(abi.PROBE_VOLUME, [_row(10, *origin, *direction, *light, lo, hi, u + h) for u in us])[:, 0]
+    dn = ob.probe(abi.PROBE_VOLUME, [_row(10, *origin, *direction, *light, lo, hi, u - h) for u in us])[:, 0]
+    assert np.allclose((up - dn) / (2 * h), 1.0 / samples[:, 1], rtol=2e-3)
+
+
+def test_channel_pdf_sums_to_one_and_power_heuristic_is_symmetric():
+    rows = vc.random_records(2, 500, 1)
+    out = ob.probe(abi.PROBE_VOLUME, rows)
+    assert np.allclose(out[:, :3].sum(axis=1), 1.0, atol=1e-6)
+    a = ob.probe(abi.PROBE_VOLUME, [_row(12, 2.0, 5.0), _row(12, 5.0, 2.0)])[:, 0]
+    assert abs(a[0] + a[1] - 1.0) < 1e-6 and abs(a[0] - 4.0 / 29.0) < 1e-7
+
+
+def test_transmit_and_scatter_estimators_are_unbiased():
+    """E[throughput] over the scatter/transmit choice: transmitting weighs
+    T / P(transmit), scattering weighs sigma_s T(t) / p(t); their sum over
+    many random numbers estimates T + integral of sigma_s T(t) dt."""
+    block = [0.5, 1.0, 2.0, 0.25, 0.2, 1.5, 0.0, 0.0, 0.0, 3.0]
+    d = 1.3
+    rng = np.random.default_rng(3)
+    n = 200000
+    rows = np.zeros((n, 24), np.float32)
+    rows[:, 0] = 6
+    rows[:, 1:11] = block
+    rows[:, 11] = d
+    rows[:, 12:15] = 1.0
+    rows[:, 15] = rng.random(n)
+    rows[:, 16] = rng.random(n)
+    out = ob.probe(abi.PROBE_VOLUME, rows)
+    st, ss = np.array(block[0:3]), np.array(block[3:6])
+    expected = np.exp(-st * d) + ss / st * (1.0 - np.exp(-st * d))
+    mean = out[:, 6:9].astype(np.float64).mean(axis=0)
+    assert np.allclose(mean, expected, rtol=1e-2)
