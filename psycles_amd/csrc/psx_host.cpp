@@ -1439,7 +1439,9 @@ int dispatch_samples(Session &s, uint32_t first, uint32_t count) {
                 key_bytes(key, ptrs);
                 key_bytes(key, n);
                 key_bytes(key, s.count_nodes);
-                key_bytes(key, generated_shade);
+                // members one by one: the struct's tail padding is not initialised
+                key_bytes(key, generated_shade.fn);
+                key_bytes(key, generated_shade.blocks);
                 DispatchGraph &slot = s.graph[n == per ? 0 : 1];
                 if (!slot.exec || key != slot.key) {
                     if (capture_dispatch(s, slot, film, n, generated)) {
