@@ -717,6 +717,15 @@ int psx_scene_intersect_device_noalloc(psx_scene *scene, const psx_ray_query *de
  *                       11 equiangular pdf: as 10 with distance for random
  *                         -> pdf
  *                       12 power heuristic: pdf, other pdf -> weight
+ *                       13 direct-light scatter point on a ray from the
+ *                         origin along +z: C, distance, throughput[3],
+ *                         scatter random, reservoir random, requested
+ *                         method (0 none, 1 distance, 2 equiangular, 3
+ *                         MIS), technique random, light position[3],
+ *                         min, max -> transmittance[3], throughput[3],
+ *                         channel pdf[3], distance, distance pdf,
+ *                         equiangular pdf, MIS weight, channel, method,
+ *                         use MIS, scattered
  *                       (src/luisa/homogeneous_volume_transport.cpp,
  *                        src/luisa/volume_scatter_probability.cpp:20-98,
  *                        src/luisa/volume_direct_sampling.cpp) */

@@ -4955,6 +4955,54 @@ static float vol_power_heuristic(float a, float b) {
     return sum > 0.0f ? a2 / sum : 0.0f;
 }
 
+static void store3(float *o, v3 v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
+
+/* HomogeneousVolumeTransport::sample_direct (:386-566) on a ray from the
+ * origin along +z, after VolumeDirectSampling::prepare(method, technique
+ * random, enabled); out as PSX_PROBE_VOLUME operation 13 */
+static void vol_sample_direct(const vol_coeffs *c, float distance, v3 throughput, float scatter_random,
+                              float reservoir_random, uint32_t requested, float technique_random, v3 light,
+                              float minimum, float maximum, float *out) {
+    float st[4];
+    vol_direct_prepare(requested, technique_random, 1, st);
+    const uint32_t method = (uint32_t)st[0];
+    const int use_mis = st[2] != 0.0f, enabled = st[3] != 0.0f;
+    const v3 origin = V3(0.0f, 0.0f, 0.0f), direction = V3(0.0f, 0.0f, 1.0f), one = V3(1.0f, 1.0f, 1.0f);
+    const v3 segment_t = vol_transmittance(c->sigma_t, distance);
+    const int scattered = enabled && vol_scatters(c) && distance > 0.0f && minimum < maximum;
+    const v3 albedo = vol_safe_divide3(c->sigma_s, c->sigma_t);
+    const v3 volume_albedo = vmul(vmul(albedo, vsub(one, segment_t)), throughput);
+    const vol_channel_sample ch = vol_sample_channel(volume_albedo, throughput, reservoir_random);
+    const float rate = vol_channel(c->sigma_t, ch.channel);
+    const float distance_sample = vol_bexp_sample(scatter_random, rate, minimum, maximum);
+    const float distance_sample_pdf = vdot(vol_bexp_pdf(distance_sample, c->sigma_t, minimum, maximum), ch.pdf);
+    float eq[3];
+    vol_sample_equiangular(origin, direction, light, minimum, maximum, st[1], eq);
+    const int equiangular = method == 2u;
+    const float sampled = equiangular ? eq[0] : distance_sample;
+    const float distance_pdf =
+        equiangular ? vdot(vol_bexp_pdf(eq[0], c->sigma_t, minimum, maximum), ch.pdf) : distance_sample_pdf;
+    const float equiangular_pdf =
+        equiangular ? eq[1] : vol_equiangular_pdf(origin, direction, light, minimum, maximum, distance_sample);
+    const float selected = equiangular ? equiangular_pdf : distance_pdf;
+    const float competing = equiangular ? distance_pdf : equiangular_pdf;
+    const float mis = use_mis ? 2.0f * vol_power_heuristic(selected, competing) : 1.0f;
+    const int valid = scattered && (!equiangular || eq[2] != 0.0f) && selected > 0.0f;
+    const v3 sampled_t = vol_transmittance(c->sigma_t, sampled);
+    const v3 direct_tp = vscale(vmul(throughput, vol_safe_divide1(vmul(c->sigma_s, sampled_t), selected)), mis);
+    store3(out, valid ? sampled_t : one);
+    store3(out + 3, valid ? direct_tp : V3(0.0f, 0.0f, 0.0f));
+    store3(out + 6, ch.pdf);
+    out[9] = valid ? sampled : 0.0f;
+    out[10] = valid ? distance_pdf : 0.0f;
+    out[11] = valid ? equiangular_pdf : 0.0f;
+    out[12] = valid ? mis : 0.0f;
+    out[13] = (float)ch.channel;
+    out[14] = (float)method;
+    out[15] = use_mis ? 1.0f : 0.0f;
+    out[16] = valid ? 1.0f : 0.0f;
+}
+
 static vol_coeffs vol_block(const float *in) {
     vol_coeffs c;
     const uint32_t flags = (uint32_t)in[9];
@@ -4967,7 +5015,6 @@ static vol_coeffs vol_block(const float *in) {
     return c;
 }
 
-static void store3(float *o, v3 v) { o[0] = v.x; o[1] = v.y; o[2] = v.z; }
 
 /* PSX_PROBE_VOLUME record (layouts in include/psx.h) */
 static void volume_probe(const float *in, float *out) {
@@ -5008,6 +5055,12 @@ static void volume_probe(const float *in, float *out) {
     case 10u: vol_sample_equiangular(load3(in + 1), load3(in + 4), load3(in + 7), in[10], in[11], in[12], out); break;
     case 11u: out[0] = vol_equiangular_pdf(load3(in + 1), load3(in + 4), load3(in + 7), in[10], in[11], in[12]); break;
     case 12u: out[0] = vol_power_heuristic(in[1], in[2]); break;
+    case 13u: {
+        const vol_coeffs c = vol_block(in + 1);
+        vol_sample_direct(&c, in[11], load3(in + 12), in[15], in[16], (uint32_t)in[17], in[18], load3(in + 19), in[22],
+                          in[23], out);
+        break;
+    }
     default: break;
     }
 }

@@ -192,6 +192,24 @@ __device__ void volume_probe(const float *in, float *out) {
     case 12u:
         out[0] = volume_power_heuristic(in[1], in[2]);
         break;
+    case 13u: {
+        const DirectSamplingState st = volume_direct_prepare(static_cast<uint32_t>(in[17]), in[18], true);
+        const HomogeneousDirectSample d =
+            volume_sample_direct(volume_block(in + 1), in[11], in3(in + 12), in[15], in[16], st, V(0.0f, 0.0f, 0.0f),
+                                 V(0.0f, 0.0f, 1.0f), in3(in + 19), in[22], in[23]);
+        out3(out, d.transmittance);
+        out3(out + 3, d.throughput);
+        out3(out + 6, d.channel_pdf);
+        out[9] = d.distance;
+        out[10] = d.distance_pdf;
+        out[11] = d.equiangular_pdf;
+        out[12] = d.mis_weight;
+        out[13] = static_cast<float>(d.channel);
+        out[14] = static_cast<float>(d.method);
+        out[15] = d.use_mis ? 1.0f : 0.0f;
+        out[16] = d.scattered ? 1.0f : 0.0f;
+        break;
+    }
     default:
         break;
     }

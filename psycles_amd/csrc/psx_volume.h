@@ -276,4 +276,63 @@ __device__ __forceinline__ float volume_power_heuristic(float selected, float co
     return sum > 0.0f ? a / sum : 0.0f;
 }
 
+struct HomogeneousDirectSample {
+    f3 transmittance, throughput, channel_pdf;
+    float distance, distance_pdf, equiangular_pdf, mis_weight;
+    uint32_t channel, method;
+    bool use_mis, scattered;
+};
+
+// The direct-light scatter point on a segment [minimum, maximum] of a ray
+// towards a light at `light`: free-flight distance sampling, equiangular
+// sampling, or both under the power heuristic, as `sampling` (from
+// volume_direct_prepare) selects (HomogeneousVolumeTransport::sample_direct,
+// src/luisa/homogeneous_volume_transport.cpp:386-566).  `distance` is the
+// segment length whose transmittance weights the channel pick.
+__device__ inline HomogeneousDirectSample volume_sample_direct(const VolumeCoefficients &c, float distance,
+                                                               f3 throughput, float scatter_random,
+                                                               float reservoir_random,
+                                                               const DirectSamplingState &sampling, f3 origin,
+                                                               f3 direction, f3 light, float minimum,
+                                                               float maximum) {
+    const f3 one = V(1.0f, 1.0f, 1.0f);
+    const f3 segment_t = volume_transmittance(c.sigma_t, distance);
+    const bool scattered = sampling.enabled && volume_scatters(c) && distance > 0.0f && minimum < maximum;
+    const f3 albedo = vol_safe_divide(c.sigma_s, c.sigma_t);
+    // the second throughput factor is part of Cycles' channel measure
+    const f3 volume_albedo = albedo * (one - segment_t) * throughput;
+    const VolumeChannelSample ch = volume_sample_channel(volume_albedo, throughput, reservoir_random);
+    const float rate = vol_channel(c.sigma_t, ch.channel);
+    const float distance_sample = volume_bounded_exponential_sample(scatter_random, rate, minimum, maximum);
+    const float distance_sample_pdf =
+        dot(volume_bounded_exponential_pdf(distance_sample, c.sigma_t, minimum, maximum), ch.pdf);
+    const EquiangularSample eq = volume_sample_equiangular(origin, direction, light, minimum, maximum, sampling.random);
+    const bool equiangular = sampling.method == kVolumeSampleEquiangular;
+    const float sampled = equiangular ? eq.distance : distance_sample;
+    const float distance_pdf =
+        equiangular ? dot(volume_bounded_exponential_pdf(eq.distance, c.sigma_t, minimum, maximum), ch.pdf)
+                    : distance_sample_pdf;
+    const float equiangular_pdf =
+        equiangular ? eq.pdf : volume_equiangular_pdf(origin, direction, light, minimum, maximum, distance_sample);
+    const float selected_pdf = equiangular ? equiangular_pdf : distance_pdf;
+    const float competing_pdf = equiangular ? distance_pdf : equiangular_pdf;
+    const float mis = sampling.use_mis ? 2.0f * volume_power_heuristic(selected_pdf, competing_pdf) : 1.0f;
+    const bool valid = scattered && (!equiangular || eq.valid) && selected_pdf > 0.0f;
+    const f3 sampled_t = volume_transmittance(c.sigma_t, sampled);
+    const f3 direct_tp = throughput * vol_safe_divide(c.sigma_s * sampled_t, selected_pdf) * mis;
+    HomogeneousDirectSample r;
+    r.transmittance = valid ? sampled_t : one;
+    r.throughput = valid ? direct_tp : V(0.0f, 0.0f, 0.0f);
+    r.channel_pdf = ch.pdf;
+    r.distance = valid ? sampled : 0.0f;
+    r.distance_pdf = valid ? distance_pdf : 0.0f;
+    r.equiangular_pdf = valid ? equiangular_pdf : 0.0f;
+    r.mis_weight = valid ? mis : 0.0f;
+    r.channel = ch.channel;
+    r.method = sampling.method;
+    r.use_mis = sampling.use_mis;
+    r.scattered = valid;
+    return r;
+}
+
 }  // namespace psx

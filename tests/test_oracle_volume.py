@@ -94,3 +94,35 @@ def test_transmit_and_scatter_estimators_are_unbiased():
     expected = np.exp(-st * d) + ss / st * (1.0 - np.exp(-st * d))
     mean = out[:, 6:9].astype(np.float64).mean(axis=0)
     assert np.allclose(mean, expected, rtol=1e-2)
+
+
+@pytest.mark.parametrize("method", [1, 2, 3])
+def test_direct_estimators_are_unbiased(method):
+    """Distance, equiangular and MIS direct sampling all estimate the
+    in-scattered integral of sigma_s T(t) over [min, max]: their mean
+    weights agree with sigma_s / sigma_t (e^-sigma_t min - e^-sigma_t max)."""
+    block = [0.5, 1.0, 2.0, 0.25, 0.2, 1.5, 0.0, 0.0, 0.0, 3.0]
+    lo, hi = 0.2, 2.5
+    rng = np.random.default_rng(11 + method)
+    n = 400000
+    rows = np.zeros((n, 24), np.float32)
+    rows[:, 0] = 13
+    rows[:, 1:11] = block
+    rows[:, 11] = hi
+    rows[:, 12:15] = 1.0
+    rows[:, 15] = rng.random(n)
+    rows[:, 16] = rng.random(n)
+    rows[:, 17] = method
+    rows[:, 18] = rng.random(n)
+    rows[:, 19:22] = (1.0, 0.5, 1.2)
+    rows[:, 22] = lo
+    rows[:, 23] = hi
+    out = ob.probe(abi.PROBE_VOLUME, rows)
+    assert np.all(out[:, 16] == 1.0)
+    st, ss = np.array(block[0:3]), np.array(block[3:6])
+    expected = ss / st * (np.exp(-st * lo) - np.exp(-st * hi))
+    mean = out[:, 3:6].astype(np.float64).mean(axis=0)
+    assert np.allclose(mean, expected, rtol=1e-2), (mean, expected)
+    if method == 3:
+        # the technique random picks distance below 0.5, equiangular above
+        assert set(np.unique(out[:, 14])) == {1.0, 2.0}

@@ -11,8 +11,8 @@ GOLDEN = Path(__file__).resolve().parent / "golden" / "homogeneous_volume.json"
 # outputs each operation writes: discrete ones are compared exactly
 OUTPUTS = {0: [0, 1, 2], 1: [0, 1, 2], 2: [0, 1, 2], 3: [0, 1, 2, 3, 4, 5], 4: [0], 5: [0, 1, 2],
            6: list(range(22)), 7: list(range(22)), 8: [0, 1, 2], 9: [0, 1, 2, 3], 10: [0, 1, 2], 11: [0],
-           12: [0]}
-DISCRETE = {3: [0, 2], 6: [19, 20, 21], 7: [19, 20, 21], 9: [0, 2, 3], 10: [2]}
+           12: [0], 13: list(range(17))}
+DISCRETE = {3: [0, 2], 6: [19, 20, 21], 7: [19, 20, 21], 9: [0, 2, 3], 10: [2], 13: [13, 14, 15, 16]}
 
 
 def golden():
@@ -92,4 +92,15 @@ def random_records(op: int, n: int, seed: int) -> np.ndarray:
         x[:, 12] = u(n) if op == 10 else rng.uniform(x[:, 10], np.maximum(x[:, 11], x[:, 10]))
     elif op == 12:
         x[:, 1:3] = rng.uniform(0, 3, (n, 2)) * (rng.random((n, 2)) > 0.1)
+    elif op == 13:
+        x[:, 1:11] = _block(rng, n)
+        x[:, 11] = rng.uniform(0, 4, n) * (rng.random(n) > 0.05)
+        x[:, 12:15] = rng.uniform(0, 2, (n, 3))
+        x[:, 15] = u(n)
+        x[:, 16] = u(n)
+        x[:, 17] = rng.integers(0, 4, n)
+        x[:, 18] = u(n)
+        x[:, 19:22] = rng.normal(0, 3, (n, 3))
+        x[:, 22] = rng.uniform(-1, 2, n)
+        x[:, 23] = x[:, 22] + rng.uniform(0, 4, n) * (rng.random(n) > 0.05)
     return x.astype(np.float32)
