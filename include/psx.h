@@ -491,7 +491,9 @@ typedef struct psx_tuning_report {
  * window for candidate shapes, one knob at a time from the default (tile
  * width, refill lanes, trace chunk, shade chunks, shade sorting, XCD
  * segments), keeps a candidate only if
- * it is at least 1 % faster, and sets the winner on the session.  The probes
+ * it is at least 1 % faster, then times the winner and the default shape
+ * three more times in turns and keeps the winner only if it stays at least
+ * 1 % faster; the result is set on the session.  The probes
  * render into scratch memory: the session's film, sample count and stats are
  * untouched.  A session whose renders run on the megakernel (integrator
  * PSX_INTEGRATOR_MEGAKERNEL without light passes) returns

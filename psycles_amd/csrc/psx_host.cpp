@@ -3059,7 +3059,9 @@ int autotune(Session &s, uint32_t probe, psx_tuning_report *report) {
         r.default_ms = time_shape(s, best, film, probe, generated, a, b);
         float best_ms = r.default_ms;
         uint32_t candidates = 1u;
-        // one knob at a time; a shape replaces the best only if >= 1 % faster
+        // one knob at a time; a shape replaces the best only if >= 1 % faster,
+        // and the final choice only if it stays >= 1 % faster than the
+        // default when both are timed again in turns
         auto consider = [&](psx_wavefront_tuning t) {
             if (std::memcmp(&t, &best, sizeof(t)) == 0) return;
             const float ms = time_shape(s, t, film, probe, generated, a, b);
@@ -3104,6 +3106,23 @@ int autotune(Session &s, uint32_t probe, psx_tuning_report *report) {
             psx_wavefront_tuning t = best;
             t.xcd_segments = 8u;
             consider(t);
+        }
+        psx_wavefront_tuning fallback;
+        psx_wavefront_tuning_default(&fallback);
+        if (std::memcmp(&best, &fallback, sizeof(best)) != 0) {
+            // confirm the winner against the default, timed in turns, so a
+            // one-off fast probe does not replace the default shape
+            float chosen = best_ms, plain = r.default_ms;
+            for (int round = 0; round < 3; ++round) {
+                plain = std::min(plain, time_shape(s, fallback, film, probe, generated, a, b));
+                chosen = std::min(chosen, time_shape(s, best, film, probe, generated, a, b));
+            }
+            r.default_ms = plain;
+            best_ms = chosen;
+            if (!(chosen < 0.99f * plain)) {
+                best = fallback;
+                best_ms = plain;
+            }
         }
         r.chosen = best;
         r.chosen_ms = best_ms;
