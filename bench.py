@@ -340,9 +340,20 @@ def main() -> int:
     snap = getattr(scenes, w["scene"])()
     settings = getattr(scenes, w["settings"])()
     b_ray = bytes_per_ray(w["nodes"], w["tris"])
+    bvh = None
+    if not args.no_autotune and not args.tuning:
+        # scene set-up: the BVH branching factor, by compiling the scene at
+        # each width and timing the same 4-sample probe (films are the same)
+        from psycles_amd.tuning import tune_bvh_width
+        t0 = time.perf_counter()
+        bvh = tune_bvh_width(snap, settings, probe_samples=4)
+        bvh["seconds"] = time.perf_counter() - t0
+        print(f"bench: BVH width {bvh['chosen']} (probe ms by width {bvh['ms']}) in {bvh['seconds']:.1f} s",
+              file=sys.stderr, flush=True)
     backend = RendererBackend(0, integrator=abi.INTEGRATOR_WAVEFRONT,
                               max_samples_per_dispatch=SAMPLES_PER_DISPATCH, graphs=not args.no_graphs,
-                              time_trace=not args.untimed_trace, generated_programs=not args.interpreter)
+                              time_trace=not args.untimed_trace, generated_programs=not args.interpreter,
+                              bvh_width=bvh["chosen"] if bvh else 4)
     t0 = time.perf_counter()
     comp = backend.compile_scene(snap)
     if not comp.ok():
@@ -422,6 +433,7 @@ def main() -> int:
             "launch": "graph replay per dispatch" if graph_launches else "host launches",
             "shading": "closure interpreter" if args.interpreter else "generated surface programs (hipRTC)",
             "tuning": tuning if tuning else session.tuning.as_dict(),
+            "bvh_width": bvh if bvh else 4,
         },
         "mrays_per_s": mrays,
         "rays_per_sample": rays / samples,

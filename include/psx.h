@@ -355,7 +355,17 @@ enum psx_backend_flags {
      * needs it (one module per layout set and shade variant, cached by the
      * backend; parameter-only rebinds reuse it).  Films are bit-identical to
      * the built-in interpreter's. */
-    PSX_BACKEND_GENERATED_PROGRAMS = 8u
+    PSX_BACKEND_GENERATED_PROGRAMS = 8u,
+    /* Branching factor of the BVHs the backend's scene compiles build: two
+     * or three children per node instead of four (the binary SAH tree is
+     * collapsed to that width; records stay 64-byte 4-wide nodes with unused
+     * slots empty).  Setting both is PSX_ERROR_INVALID_ARGUMENT.  Hits,
+     * films and events are the same at every width (closest hits resolve
+     * ties by primitive order, not by the tree); only traversal time
+     * changes.  psycles_amd.tuning.tune_bvh_width picks the width by timing
+     * probe dispatches. */
+    PSX_BACKEND_BVH_WIDTH_2 = 16u,
+    PSX_BACKEND_BVH_WIDTH_3 = 32u
 };
 
 typedef struct psx_backend_options {

@@ -101,6 +101,8 @@ BACKEND_COUNT_TRAVERSAL = 1
 BACKEND_NO_GRAPHS = 2
 BACKEND_TIME_TRACE = 4
 BACKEND_GENERATED_PROGRAMS = 8
+BACKEND_BVH_WIDTH_2 = 16
+BACKEND_BVH_WIDTH_3 = 32
 PROGRAM_LIGHT_PASSES = 1
 PROGRAM_LIGHTS = 2
 

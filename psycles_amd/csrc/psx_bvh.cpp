@@ -199,7 +199,8 @@ class BinaryBuilder {
 
 }  // namespace
 
-Bvh4 build_bvh4(const std::vector<BuildItem> &items, uint32_t node_base, uint32_t max_leaf) {
+Bvh4 build_bvh4(const std::vector<BuildItem> &items, uint32_t node_base, uint32_t max_leaf, uint32_t width) {
+    width = std::min(std::max(width, 2u), 4u);
     Bvh4 out;
     if (items.empty()) {
         Node4 n{};
@@ -217,8 +218,9 @@ Bvh4 build_bvh4(const std::vector<BuildItem> &items, uint32_t node_base, uint32_
     builder.build();
     const auto &bn = builder.nodes();
     out.order = builder.order();
-    // collapse: each 4-wide node absorbs up to four binary descendants,
-    // always opening the inner child with the largest surface area.
+    // collapse: each node absorbs up to `width` binary descendants, always
+    // opening the inner child with the largest surface area; unused slots
+    // of the 4-wide record stay empty.
     struct Task {
         uint32_t binary, slot_node, depth;
     };
@@ -237,7 +239,7 @@ Bvh4 build_bvh4(const std::vector<BuildItem> &items, uint32_t node_base, uint32_
         } else {
             kids[k++] = bn[t.binary].left;
             kids[k++] = bn[t.binary].right;
-            while (k < 4) {
+            while (k < static_cast<int>(width)) {
                 int best = -1;
                 float best_area = -1.0f;
                 for (int i = 0; i < k; ++i) {

@@ -22,8 +22,9 @@ struct Bvh4 {
     uint32_t depth = 0;           // number of 4-wide levels
 };
 
-// max_leaf: largest number of items per leaf (1..4).
-Bvh4 build_bvh4(const std::vector<BuildItem> &items, uint32_t node_base, uint32_t max_leaf);
+// max_leaf: largest number of items per leaf (1..4); width: children per
+// node (2..4; the 4-wide record keeps unused slots empty).
+Bvh4 build_bvh4(const std::vector<BuildItem> &items, uint32_t node_base, uint32_t max_leaf, uint32_t width = 4u);
 
 // The traversal form of a node array: node i becomes QNode4 i with 8-bit
 // outward-rounded child bounds and child references encoded as traversal

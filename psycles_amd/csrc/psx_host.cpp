@@ -1578,6 +1578,13 @@ int write_passes(Session &s, const psx_sink *sink) {
 
 // Scene compilation (LuisaPathTracerBackend::compile_scene,
 // src/luisa/path_tracer_scene.cpp:50-1995, triangle subset)
+// Children per BVH node the backend builds with (PSX_BACKEND_BVH_WIDTH_*).
+uint32_t bvh_width(const psx_backend_options &o) {
+    if (o.flags & PSX_BACKEND_BVH_WIDTH_2) return 2u;
+    if (o.flags & PSX_BACKEND_BVH_WIDTH_3) return 3u;
+    return 4u;
+}
+
 int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *diag_user, Scene **out) {
     auto fail = [&](const std::string &msg) {
         if (diag) diag(diag_user, msg.c_str());
@@ -1712,7 +1719,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
         }
         const uint32_t base = static_cast<uint32_t>(nodes.size());
         const uint32_t tri_base = static_cast<uint32_t>(tris.size());
-        Bvh4 b = build_bvh4(items, base, 2u);
+        Bvh4 b = build_bvh4(items, base, 2u, bvh_width(backend->options));
         for (auto &n : b.nodes)
             for (int i = 0; i < 4; ++i)
                 if (n.count[i] != kEmptySlot && n.count[i] > 0u) n.child[i] += tri_base;
@@ -1829,7 +1836,7 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     uint32_t tlas_begin = static_cast<uint32_t>(nodes.size());  // node 0 and [tlas_begin, end) are TLAS nodes
     if (has_tlas) {
         const uint32_t base = static_cast<uint32_t>(nodes.size());
-        Bvh4 t = build_bvh4(tlas_items, base, 1u);
+        Bvh4 t = build_bvh4(tlas_items, base, 1u, bvh_width(backend->options));
         // the TLAS root must live at node 0: place it there and append the rest
         for (uint32_t idx : t.order) tlas_leaf_items.push_back(tlas_instance[idx]);
         // inner child indices point at base + local index; local 0 moves to slot 0
@@ -1845,9 +1852,10 @@ int compile(Backend *backend, const psx_scene_desc *d, psx_diag_fn diag, void *d
     }
     scene->tlas_depth = tlas_depth;
     scene->blas_depth = max_blas_depth;
-    // stack bound: a TLAS node pushes at most four entries (instance markers
-    // and inner children), a BLAS node at most three
-    scene->max_stack = 4u * tlas_depth + 3u * max_blas_depth + 2u;
+    // stack bound: a TLAS node of w children pushes at most w entries
+    // (instance markers and inner children), a BLAS node at most w - 1
+    const uint32_t width = bvh_width(backend->options);
+    scene->max_stack = width * tlas_depth + (width - 1u) * max_blas_depth + 2u;
     if (tris.size() >= kMaxLeafTriangles || nodes.size() >= (1ull << 30) || d->instance_count >= (1u << 30))
         return fail("scene exceeds the traversal entry encoding (" + std::to_string(tris.size()) +
                     " leaf triangles, " + std::to_string(nodes.size()) + " nodes)");
@@ -2775,7 +2783,8 @@ int psx_backend_create(int device, const psx_backend_options *options, psx_backe
         b->options.max_samples_per_dispatch = 4u;
         if (options) {
             b->options = *options;
-            if (b->options.max_samples_per_dispatch == 0u) {
+            const uint32_t widths = PSX_BACKEND_BVH_WIDTH_2 | PSX_BACKEND_BVH_WIDTH_3;
+            if (b->options.max_samples_per_dispatch == 0u || (b->options.flags & widths) == widths) {
                 delete b;
                 return static_cast<int>(PSX_ERROR_INVALID_ARGUMENT);
             }

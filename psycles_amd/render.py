@@ -301,9 +301,16 @@ class RendererBackend:
     def __init__(self, device: int = 0, next_event_estimation: bool = True,
                  max_samples_per_dispatch: int = 4, integrator: int = abi.INTEGRATOR_MEGAKERNEL,
                  count_traversal: bool = False, graphs: bool = True, time_trace: bool = False,
-                 generated_programs: bool = False):
+                 generated_programs: bool = False, bvh_width: int = 4):
         self.lib = abi.load_library()
+        if bvh_width not in (2, 3, 4):
+            raise ValueError("bvh_width is 2, 3 or 4")
+        self.bvh_width = bvh_width
         flags = abi.BACKEND_COUNT_TRAVERSAL if count_traversal else 0
+        if bvh_width == 2:
+            flags |= abi.BACKEND_BVH_WIDTH_2
+        elif bvh_width == 3:
+            flags |= abi.BACKEND_BVH_WIDTH_3
         if generated_programs:
             flags |= abi.BACKEND_GENERATED_PROGRAMS
         if not graphs:
