@@ -697,7 +697,7 @@ int psx_scene_intersect_device_noalloc(psx_scene *scene, const psx_ray_query *de
  *                           evaluation factor, valid (10)
  *                       (AreaLightSampling::from_position,
  *                        src/luisa/area_light_sampling.cpp)
- *   VOLUME              homogeneous volume transport (24 in, 24 out; unused
+ *   VOLUME              homogeneous volume transport (32 in, 24 out; unused
  *                       floats are zero).  in[0] selects the function; a
  *                       coefficient block C is sigma_t[3], sigma_s[3],
  *                       emission[3], flags (bit 0 extinction, bit 1
@@ -738,9 +738,18 @@ int psx_scene_intersect_device_noalloc(psx_scene *scene, const psx_ray_query *de
  *                         channel pdf[3], distance, distance pdf,
  *                         equiangular pdf, MIS weight, channel, method,
  *                         use MIS, scattered
+ *                       14 spot light interval: ray origin[3], direction[3],
+ *                         min, max, lamp centre[3], axes x[3], y[3], z[3],
+ *                         axis scale[3], radius, spot angle -> min, max,
+ *                         valid
+ *                       15 area light interval: ray origin[3], direction[3],
+ *                         min, max, centre[3], axis u[3], v[3], z[3],
+ *                         length u, length v, spread, ellipse -> min,
+ *                         max, valid
  *                       (src/luisa/homogeneous_volume_transport.cpp,
  *                        src/luisa/volume_scatter_probability.cpp:20-98,
- *                        src/luisa/volume_direct_sampling.cpp) */
+ *                        src/luisa/volume_direct_sampling.cpp,
+ *                        src/luisa/volume_light_interval.cpp) */
 enum psx_probe_function {
     PSX_PROBE_FILM_CLAMP = 0,
     PSX_PROBE_T_OFFSET = 1,

@@ -5003,6 +5003,147 @@ static void vol_sample_direct(const vol_coeffs *c, float distance, v3 throughput
     out[16] = valid ? 1.0f : 0.0f;
 }
 
+/* ---- light intervals (VolumeLightInterval, src/luisa/volume_light_interval.cpp) */
+typedef struct { float minimum, maximum; int valid; } light_interval;
+
+static float iv_min(float a, float b) { return b < a ? b : a; }
+static float iv_max(float a, float b) { return b > a ? b : a; }
+
+/* solve_quadratic (:53-120): the stable root pair, or the linear root */
+static light_interval iv_quadratic(float a, float b, float c) {
+    light_interval r;
+    const int valid_linear = a == 0.0f && b != 0.0f;
+    const float linear_root = -c / (b != 0.0f ? b : 1.0f);
+    const float discriminant = b * b - 4.0f * a * c;
+    const int valid_quadratic = a != 0.0f && discriminant > -1.0e-5f;
+    const float root = sqrtf(iv_max(discriminant, 0.0f));
+    const float temporary = -0.5f * (b + copysignf(root, b));
+    const float r1 = temporary / (a != 0.0f ? a : 1.0f);
+    const float r2 = temporary != 0.0f ? c / temporary : r1;
+    r.minimum = valid_quadratic ? (r2 < r1 ? r2 : r1) : linear_root;
+    r.maximum = valid_quadratic ? (r2 > r1 ? r2 : r1) : linear_root;
+    r.valid = valid_linear || valid_quadratic;
+    return r;
+}
+
+static light_interval iv_and(float lo0, float hi0, float lo1, float hi1, int valid) {
+    light_interval r = {iv_max(lo0, lo1), iv_min(hi0, hi1), 0};
+    r.valid = valid && r.minimum < r.maximum;
+    return r;
+}
+
+static light_interval iv_plane(v3 normal, v3 plane_to_origin, v3 direction, float lo, float hi) {
+    const float cosine = vdot(direction, normal);
+    const float distance = -vdot(plane_to_origin, normal) / cosine;
+    light_interval r = {cosine > 0.0f ? iv_max(lo, distance) : lo, cosine > 0.0f ? hi : iv_min(hi, distance), 0};
+    r.valid = r.minimum < r.maximum;
+    return r;
+}
+
+static light_interval iv_slab(float blo, float bhi, float origin, float direction, float lo, float hi) {
+    const int moving = direction != 0.0f;
+    const float safe = moving ? direction : 1.0f;
+    const float a = (blo - origin) / safe, b = (bhi - origin) / safe;
+    if (moving) return iv_and(lo, hi, iv_min(a, b), iv_max(a, b), 1);
+    light_interval r = {lo, hi, origin >= blo && origin <= bhi && lo < hi};
+    return r;
+}
+
+static light_interval iv_box(v3 blo, v3 bhi, v3 o, v3 d, float lo, float hi) {
+    const light_interval x = iv_slab(blo.x, bhi.x, o.x, d.x, lo, hi);
+    const light_interval y = iv_slab(blo.y, bhi.y, o.y, d.y, x.minimum, x.maximum);
+    light_interval z = iv_slab(blo.z, bhi.z, o.z, d.z, y.minimum, y.maximum);
+    z.valid = x.valid && y.valid && z.valid;
+    return z;
+}
+
+static light_interval iv_elliptic_cylinder(v3 origin, v3 direction, float ru, float rv, float lo, float hi) {
+    const int radii = ru > 0.0f && rv > 0.0f;
+    const float su = radii ? ru : 1.0f, sv = radii ? rv : 1.0f;
+    float ox = origin.x / su, oy = origin.y / sv;
+    const float dx = direction.x / su, dy = direction.y / sv;
+    const float a = dx * dx + dy * dy;
+    float b = ox * dx + oy * dy;
+    const int movable = a != 0.0f;
+    const float middle = -b / (movable ? a : 1.0f);
+    ox = ox + dx * middle;
+    oy = oy + dy * middle;
+    b = ox * dx + oy * dy;
+    const float c = (ox * ox + oy * oy) - 1.0f;
+    const light_interval q = iv_quadratic(a, 2.0f * b, c);
+    light_interval r = iv_and(lo, hi, q.minimum + middle, q.maximum + middle, q.valid);
+    r.valid = radii && movable && r.valid;
+    return r;
+}
+
+/* intersect_cone (:325-440): the nappe around `axis`, or the plane when the
+ * cone opens to a half space */
+static light_interval iv_cone(v3 axis, v3 apex_to_origin, v3 direction, float cos2, float lo, float hi) {
+    const light_interval plane = iv_plane(axis, apex_to_origin, direction, lo, hi);
+    if (cos2 < 1.0e-4f) return plane;
+    const float length2 = vdot(direction, direction);
+    const float inverse_length = 1.0f / sqrtf(iv_max(length2, 1.0e-30f));
+    const v3 nd = vscale(direction, inverse_length);
+    const float ad = vdot(axis, nd), ao = vdot(axis, apex_to_origin);
+    const float a = ad * ad - cos2;
+    const float b = 2.0f * (ad * ao - cos2 * vdot(nd, apex_to_origin));
+    const float c = ao * ao - cos2 * vdot(apex_to_origin, apex_to_origin);
+    const light_interval q = iv_quadratic(a, b, c);
+    const int min_valid = ao + q.minimum * ad > 0.0f, max_valid = ao + q.maximum * ad > 0.0f;
+    const int neither = !max_valid, only_max = max_valid && !min_valid;
+    const float hmin = only_max ? q.maximum : (neither ? 0.0f : q.minimum);
+    const float hmax = only_max ? FLT_MAX : (neither ? q.minimum : q.maximum);
+    return iv_and(lo, hi, hmin * inverse_length, hmax * inverse_length,
+                  length2 > 0.0f && q.valid && (min_valid || max_valid));
+}
+
+/* VolumeLightInterval::spot (:441-512) */
+static light_interval iv_spot(v3 origin, v3 direction, float lo, float hi, v3 center, v3 ax, v3 ay, v3 az, v3 scale,
+                              float radius, float spot_angle) {
+    const v3 cx = vscale(ax, scale.x), cy = vscale(ay, scale.y), cz = vscale(az, scale.z);
+    const v3 adj_x = vcross(cy, cz);
+    const float det = vdot(cx, adj_x);
+    const int transform_valid = fabsf(det) > 1.0e-20f;
+    const float inv_det = 1.0f / (transform_valid ? det : 1.0f);
+    const v3 row_x = vscale(adj_x, inv_det), row_y = vscale(vcross(cz, cx), inv_det),
+             row_z = vscale(vcross(cx, cy), inv_det);
+    const float half_angle = 0.5f * spot_angle;
+    const float tangent = tanf(half_angle);
+    const float denominator = tangent * tangent * iv_min(scale.x * scale.x, scale.y * scale.y);
+    const int geometry_valid = transform_valid && denominator > 0.0f;
+    const float shift = radius * sqrtf(1.0f + scale.z * scale.z / (geometry_valid ? denominator : 1.0f));
+    const v3 shifted = vsub(vadd(origin, vscale(vneg(az), shift)), center);
+    const v3 lo3 = V3(vdot(row_x, shifted), vdot(row_y, shifted), vdot(row_z, shifted));
+    const v3 ld = V3(vdot(row_x, direction), vdot(row_y, direction), vdot(row_z, direction));
+    const float ch = cosf(half_angle);
+    light_interval r = iv_cone(V3(0.0f, 0.0f, -1.0f), lo3, ld, ch * ch, lo, hi);
+    r.valid = r.valid && geometry_valid;
+    return r;
+}
+
+/* VolumeLightInterval::area (:513-647) */
+static light_interval iv_area(v3 ray_origin, v3 direction, float lo, float hi, v3 center, v3 au, v3 av, v3 az,
+                              float length_u, float length_v, float spread, int ellipse) {
+    const v3 normal = vneg(az);
+    const v3 origin = vsub(ray_origin, center);
+    const v3 lorig = V3(vdot(origin, au), vdot(origin, av), vdot(origin, normal));
+    const v3 ldir = V3(vdot(direction, au), vdot(direction, av), vdot(direction, normal));
+    const float half_u = 0.5f * length_u, half_v = 0.5f * length_v;
+    const float tangent = spread == 3.14159265358979323846f ? FLT_MAX : tanf(0.5f * iv_max(spread, 0.0f));
+    light_interval region;
+    if (tangent < 1.0e-5f) {
+        region = ellipse ? iv_elliptic_cylinder(lorig, ldir, half_u, half_v, lo, hi)
+                         : iv_box(V3(-half_u, -half_v, 0.0f), V3(half_u, half_v, FLT_MAX), lorig, ldir, lo, hi);
+    } else {
+        const float extent = 0.5f * (ellipse ? iv_max(length_u, length_v) : sqrtf(length_u * length_u + length_v * length_v));
+        const v3 apex_to_origin = vadd(origin, vscale(normal, extent / tangent));
+        region = iv_cone(normal, apex_to_origin, direction, 1.0f / (1.0f + tangent * tangent), lo, hi);
+    }
+    light_interval side = iv_plane(normal, origin, direction, region.minimum, region.maximum);
+    side.valid = region.valid && side.valid;
+    return side;
+}
+
 static vol_coeffs vol_block(const float *in) {
     vol_coeffs c;
     const uint32_t flags = (uint32_t)in[9];
@@ -5055,6 +5196,18 @@ static void volume_probe(const float *in, float *out) {
     case 10u: vol_sample_equiangular(load3(in + 1), load3(in + 4), load3(in + 7), in[10], in[11], in[12], out); break;
     case 11u: out[0] = vol_equiangular_pdf(load3(in + 1), load3(in + 4), load3(in + 7), in[10], in[11], in[12]); break;
     case 12u: out[0] = vol_power_heuristic(in[1], in[2]); break;
+    case 14u: {
+        const light_interval r = iv_spot(load3(in + 1), load3(in + 4), in[7], in[8], load3(in + 9), load3(in + 12),
+                                         load3(in + 15), load3(in + 18), load3(in + 21), in[24], in[25]);
+        out[0] = r.minimum; out[1] = r.maximum; out[2] = r.valid ? 1.0f : 0.0f;
+        break;
+    }
+    case 15u: {
+        const light_interval r = iv_area(load3(in + 1), load3(in + 4), in[7], in[8], load3(in + 9), load3(in + 12),
+                                         load3(in + 15), load3(in + 18), in[21], in[22], in[23], in[24] != 0.0f);
+        out[0] = r.minimum; out[1] = r.maximum; out[2] = r.valid ? 1.0f : 0.0f;
+        break;
+    }
     case 13u: {
         const vol_coeffs c = vol_block(in + 1);
         vol_sample_direct(&c, in[11], load3(in + 12), in[15], in[16], (uint32_t)in[17], in[18], load3(in + 19), in[22],
@@ -5066,7 +5219,7 @@ static void volume_probe(const float *in, float *out) {
 }
 
 uint32_t oracle_probe_input_width(uint32_t function) {
-    static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u, 21u, 24u};
+    static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u, 21u, 32u};
     return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
 }
 

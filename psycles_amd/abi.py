@@ -293,7 +293,7 @@ PROBE_VOLUME = 9
 PROBE_WIDTHS = {PROBE_FILM_CLAMP: (6, 3), PROBE_T_OFFSET: (1, 1), PROBE_SELF_EXCLUSION: (21, 4),
                 PROBE_SHADOW_TERMINATOR: (47, 4), PROBE_TRIANGLE_SAMPLE: (14, 10), PROBE_TRIANGLE_PDF: (15, 3),
                 PROBE_DIFFUSE_EVAL: (13, 4), PROBE_SURFACE_SHADOW_ORIGIN: (59, 4), PROBE_AREA_SAMPLE: (21, 10),
-                PROBE_VOLUME: (24, 24)}
+                PROBE_VOLUME: (32, 24)}
 
 
 def make_queries(records) -> "C.Array":

@@ -3332,7 +3332,7 @@ int psx_scene_intersect_device_noalloc(psx_scene *handle, const psx_ray_query *q
 }
 
 uint32_t psx_probe_input_width(uint32_t function) {
-    static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u, 21u, 24u};
+    static const uint32_t w[] = {6u, 1u, 21u, 47u, 14u, 15u, 13u, 59u, 21u, 32u};
     return function < sizeof(w) / sizeof(w[0]) ? w[function] : 0u;
 }
 

@@ -210,6 +210,22 @@ __device__ void volume_probe(const float *in, float *out) {
         out[16] = d.scattered ? 1.0f : 0.0f;
         break;
     }
+    case 14u: {
+        const LightInterval r = volume_spot_interval(in3(in + 1), in3(in + 4), in[7], in[8], in3(in + 9), in3(in + 12),
+                                                     in3(in + 15), in3(in + 18), in3(in + 21), in[24], in[25]);
+        out[0] = r.minimum;
+        out[1] = r.maximum;
+        out[2] = r.valid ? 1.0f : 0.0f;
+        break;
+    }
+    case 15u: {
+        const LightInterval r = volume_area_interval(in3(in + 1), in3(in + 4), in[7], in[8], in3(in + 9), in3(in + 12),
+                                                     in3(in + 15), in3(in + 18), in[21], in[22], in[23], in[24] != 0.0f);
+        out[0] = r.minimum;
+        out[1] = r.maximum;
+        out[2] = r.valid ? 1.0f : 0.0f;
+        break;
+    }
     default:
         break;
     }

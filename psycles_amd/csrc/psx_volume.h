@@ -335,4 +335,169 @@ __device__ inline HomogeneousDirectSample volume_sample_direct(const VolumeCoeff
     return r;
 }
 
+// ---- light intervals ------------------------------------------------------
+// The part of a ray segment from which a spot or area light can be seen,
+// which bounds the equiangular proposal (VolumeLightInterval,
+// src/luisa/volume_light_interval.cpp:1-647): a quadratic solve, slab, box,
+// plane, elliptic-cylinder and cone clipping, then the spot cone (apex moved
+// back for a finite radius, in the lamp's possibly non-uniform frame) and the
+// area light's spread cone or zero-spread prism on its emitting side.
+
+struct LightInterval {
+    float minimum, maximum;
+    bool valid;
+};
+
+__device__ __forceinline__ float interval_min(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float interval_max(float a, float b) { return b > a ? b : a; }
+
+struct QuadraticRoots {
+    float minimum, maximum;
+    bool valid;
+};
+
+__device__ inline QuadraticRoots interval_quadratic(float a, float b, float c) {
+    const bool valid_linear = a == 0.0f && b != 0.0f;
+    const float linear_root = -c / (b != 0.0f ? b : 1.0f);
+    const float discriminant = b * b - 4.0f * a * c;
+    const bool valid_quadratic = a != 0.0f && discriminant > -1.0e-5f;
+    const float root = sqrtf(interval_max(discriminant, 0.0f));
+    const float temporary = -0.5f * (b + copysignf(root, b));
+    const float r1 = temporary / (a != 0.0f ? a : 1.0f);
+    const float r2 = temporary != 0.0f ? c / temporary : r1;
+    const float qmin = r2 < r1 ? r2 : r1, qmax = r2 > r1 ? r2 : r1;
+    return QuadraticRoots{valid_quadratic ? qmin : linear_root, valid_quadratic ? qmax : linear_root,
+                          valid_linear || valid_quadratic};
+}
+
+__device__ __forceinline__ LightInterval interval_and(float lo0, float hi0, float lo1, float hi1, bool valid) {
+    const float lo = interval_max(lo0, lo1), hi = interval_min(hi0, hi1);
+    return LightInterval{lo, hi, valid && lo < hi};
+}
+
+__device__ inline LightInterval interval_plane(f3 normal, f3 plane_to_origin, f3 direction, float lo, float hi) {
+    const float cosine = dot(direction, normal);
+    const float distance = -dot(plane_to_origin, normal) / cosine;
+    const float mn = cosine > 0.0f ? interval_max(lo, distance) : lo;
+    const float mx = cosine > 0.0f ? hi : interval_min(hi, distance);
+    return LightInterval{mn, mx, mn < mx};
+}
+
+__device__ inline LightInterval interval_slab(float bound_lo, float bound_hi, float origin, float direction, float lo,
+                                              float hi) {
+    const bool moving = direction != 0.0f;
+    const float safe = moving ? direction : 1.0f;
+    const float a = (bound_lo - origin) / safe, b = (bound_hi - origin) / safe;
+    const LightInterval bounded = interval_and(lo, hi, interval_min(a, b), interval_max(a, b), moving);
+    const bool inside = !moving && origin >= bound_lo && origin <= bound_hi && lo < hi;
+    return moving ? bounded : LightInterval{lo, hi, inside};
+}
+
+__device__ inline LightInterval interval_box(f3 bound_lo, f3 bound_hi, f3 origin, f3 direction, float lo, float hi) {
+    const LightInterval x = interval_slab(bound_lo.x, bound_hi.x, origin.x, direction.x, lo, hi);
+    const LightInterval y = interval_slab(bound_lo.y, bound_hi.y, origin.y, direction.y, x.minimum, x.maximum);
+    const LightInterval z = interval_slab(bound_lo.z, bound_hi.z, origin.z, direction.z, y.minimum, y.maximum);
+    return LightInterval{z.minimum, z.maximum, x.valid && y.valid && z.valid};
+}
+
+__device__ inline LightInterval interval_elliptic_cylinder(f3 origin, f3 direction, float radius_u, float radius_v,
+                                                           float lo, float hi) {
+    const bool radii = radius_u > 0.0f && radius_v > 0.0f;
+    const float su = radii ? radius_u : 1.0f, sv = radii ? radius_v : 1.0f;
+    float ox = origin.x / su, oy = origin.y / sv;
+    const float dx = direction.x / su, dy = direction.y / sv;
+    const float a = dx * dx + dy * dy;
+    float b = ox * dx + oy * dy;
+    const bool movable = a != 0.0f;
+    const float middle = -b / (movable ? a : 1.0f);
+    ox = ox + dx * middle;
+    oy = oy + dy * middle;
+    b = ox * dx + oy * dy;
+    const float c = (ox * ox + oy * oy) - 1.0f;
+    const QuadraticRoots r = interval_quadratic(a, 2.0f * b, c);
+    const LightInterval bounded = interval_and(lo, hi, r.minimum + middle, r.maximum + middle, r.valid);
+    return LightInterval{bounded.minimum, bounded.maximum, radii && movable && bounded.valid};
+}
+
+__device__ inline LightInterval interval_cone(f3 axis, f3 apex_to_origin, f3 direction, float cos2, float lo,
+                                              float hi) {
+    const bool plane_limit = cos2 < 1.0e-4f;
+    const LightInterval plane = interval_plane(axis, apex_to_origin, direction, lo, hi);
+    const float length2 = dot(direction, direction);
+    const bool valid_direction = length2 > 0.0f;
+    const float inverse_length = 1.0f / sqrtf(interval_max(length2, 1.0e-30f));
+    const f3 nd = direction * inverse_length;
+    const float ad = dot(axis, nd), ao = dot(axis, apex_to_origin);
+    const float a = ad * ad - cos2;
+    const float b = 2.0f * (ad * ao - cos2 * dot(nd, apex_to_origin));
+    const float c = ao * ao - cos2 * dot(apex_to_origin, apex_to_origin);
+    const QuadraticRoots r = interval_quadratic(a, b, c);
+    const bool min_valid = ao + r.minimum * ad > 0.0f;
+    const bool max_valid = ao + r.maximum * ad > 0.0f;
+    const bool neither = !max_valid, only_max = max_valid && !min_valid;
+    const float hemi_min = only_max ? r.maximum : (neither ? 0.0f : r.minimum);
+    const float hemi_max = only_max ? FLT_MAX : (neither ? r.minimum : r.maximum);
+    const LightInterval cone = interval_and(lo, hi, hemi_min * inverse_length, hemi_max * inverse_length,
+                                            valid_direction && r.valid && (min_valid || max_valid));
+    return plane_limit ? plane : cone;
+}
+
+__device__ __forceinline__ f3 interval_cross(f3 a, f3 b) {
+    return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// VolumeLightInterval::spot: the spot cone in the lamp's frame (columns
+// axis * scale, inverted through the adjugate), apex moved back by the
+// lamp radius so a finite sphere's cone is covered
+__device__ inline LightInterval volume_spot_interval(f3 origin, f3 direction, float lo, float hi, f3 center, f3 ax,
+                                                     f3 ay, f3 az, f3 scale, float radius, float spot_angle) {
+    const f3 cx = ax * scale.x, cy = ay * scale.y, cz = az * scale.z;
+    const f3 adj_x = interval_cross(cy, cz);
+    const float det = dot(cx, adj_x);
+    const bool transform_valid = fabsf(det) > 1.0e-20f;
+    const float inv_det = 1.0f / (transform_valid ? det : 1.0f);
+    const f3 row_x = adj_x * inv_det, row_y = interval_cross(cz, cx) * inv_det, row_z = interval_cross(cx, cy) * inv_det;
+    const float half_angle = 0.5f * spot_angle;
+    const float tangent = tanf(half_angle);
+    const float transverse = interval_min(scale.x * scale.x, scale.y * scale.y);
+    const float axial = scale.z * scale.z;
+    const float denominator = tangent * tangent * transverse;
+    const bool geometry_valid = transform_valid && denominator > 0.0f;
+    const float apex_shift = radius * sqrtf(1.0f + axial / (geometry_valid ? denominator : 1.0f));
+    const f3 shifted = origin + vol_neg(az) * apex_shift - center;
+    const f3 lo3 = V(dot(row_x, shifted), dot(row_y, shifted), dot(row_z, shifted));
+    const f3 ld = V(dot(row_x, direction), dot(row_y, direction), dot(row_z, direction));
+    const float ch = cosf(half_angle);
+    LightInterval r = interval_cone(V(0.0f, 0.0f, -1.0f), lo3, ld, ch * ch, lo, hi);
+    r.valid = r.valid && geometry_valid;
+    return r;
+}
+
+// VolumeLightInterval::area: the spread cone whose apex sits behind the
+// light by its maximal extent / tan(spread / 2), or, for a spread below
+// 1e-5, the prism (box or elliptic cylinder) over the light; then the
+// emitting side of its plane
+__device__ inline LightInterval volume_area_interval(f3 ray_origin, f3 direction, float lo, float hi, f3 center,
+                                                     f3 au, f3 av, f3 az, float length_u, float length_v,
+                                                     float spread, bool ellipse) {
+    const f3 normal = vol_neg(az);
+    const f3 origin = ray_origin - center;
+    const f3 lorig = V(dot(origin, au), dot(origin, av), dot(origin, normal));
+    const f3 ldir = V(dot(direction, au), dot(direction, av), dot(direction, normal));
+    const float half_u = 0.5f * length_u, half_v = 0.5f * length_v;
+    const float tangent = spread == 3.14159265358979323846f ? FLT_MAX : tanf(0.5f * interval_max(spread, 0.0f));
+    const bool nearly_parallel = tangent < 1.0e-5f;
+    const LightInterval cylinder = interval_elliptic_cylinder(lorig, ldir, half_u, half_v, lo, hi);
+    const LightInterval box = interval_box(V(-half_u, -half_v, 0.0f), V(half_u, half_v, FLT_MAX), lorig, ldir, lo, hi);
+    const LightInterval zero_spread = ellipse ? cylinder : box;
+    const float extent = 0.5f * (ellipse ? interval_max(length_u, length_v)
+                                         : sqrtf(length_u * length_u + length_v * length_v));
+    const float safe_tangent = tangent != 0.0f ? tangent : 1.0f;
+    const f3 apex_to_origin = origin + normal * (extent / safe_tangent);
+    const LightInterval cone = interval_cone(normal, apex_to_origin, direction, 1.0f / (1.0f + tangent * tangent), lo, hi);
+    const LightInterval region = nearly_parallel ? zero_spread : cone;
+    const LightInterval side = interval_plane(normal, origin, direction, region.minimum, region.maximum);
+    return LightInterval{side.minimum, side.maximum, region.valid && side.valid};
+}
+
 }  // namespace psx

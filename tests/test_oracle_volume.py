@@ -1,7 +1,7 @@
 """Homogeneous volume transport in the oracle (PSX_PROBE_VOLUME).
 
-Pinned by records 0-45 of the reference's fixture
-(tests/test_luisa_homogeneous_volume.cpp:1052-1100, transcribed with their
+Pinned by records 0-56 of the reference's fixture
+(tests/test_luisa_homogeneous_volume.cpp:1052-1130, transcribed with their
 inputs in tests/golden/homogeneous_volume.json) under the reference's own
 tolerance, and by what the mathematics fixes: the bounded free-flight
 sampler inverts its CDF and its pdf integrates to one on the interval, the
@@ -31,7 +31,7 @@ def test_reference_records():
 
 
 def _row(op, *args):
-    x = np.zeros(24, np.float32)
+    x = np.zeros(vc.WIDTH, np.float32)
     x[0] = op
     x[1:1 + len(args)] = args
     return x
@@ -82,7 +82,7 @@ def test_transmit_and_scatter_estimators_are_unbiased():
     d = 1.3
     rng = np.random.default_rng(3)
     n = 200000
-    rows = np.zeros((n, 24), np.float32)
+    rows = np.zeros((n, vc.WIDTH), np.float32)
     rows[:, 0] = 6
     rows[:, 1:11] = block
     rows[:, 11] = d
@@ -105,7 +105,7 @@ def test_direct_estimators_are_unbiased(method):
     lo, hi = 0.2, 2.5
     rng = np.random.default_rng(11 + method)
     n = 400000
-    rows = np.zeros((n, 24), np.float32)
+    rows = np.zeros((n, vc.WIDTH), np.float32)
     rows[:, 0] = 13
     rows[:, 1:11] = block
     rows[:, 11] = hi

@@ -11,8 +11,10 @@ GOLDEN = Path(__file__).resolve().parent / "golden" / "homogeneous_volume.json"
 # outputs each operation writes: discrete ones are compared exactly
 OUTPUTS = {0: [0, 1, 2], 1: [0, 1, 2], 2: [0, 1, 2], 3: [0, 1, 2, 3, 4, 5], 4: [0], 5: [0, 1, 2],
            6: list(range(22)), 7: list(range(22)), 8: [0, 1, 2], 9: [0, 1, 2, 3], 10: [0, 1, 2], 11: [0],
-           12: [0], 13: list(range(17))}
-DISCRETE = {3: [0, 2], 6: [19, 20, 21], 7: [19, 20, 21], 9: [0, 2, 3], 10: [2], 13: [13, 14, 15, 16]}
+           12: [0], 13: list(range(17)), 14: [0, 1, 2], 15: [0, 1, 2]}
+DISCRETE = {3: [0, 2], 6: [19, 20, 21], 7: [19, 20, 21], 9: [0, 2, 3], 10: [2], 13: [13, 14, 15, 16], 14: [2],
+            15: [2]}
+WIDTH = 32
 
 
 def golden():
@@ -21,7 +23,18 @@ def golden():
 
 
 def picked(out: np.ndarray, outputs) -> np.ndarray:
-    return np.array([out[0] + out[1] + out[2] if o == "sum012" else out[o] for o in outputs], np.float32)
+    """The record's values: output indices, the sum of the first three, or
+    an interval end zeroed when the interval is not valid (as the
+    reference's write_interval records it)."""
+    def one(o):
+        if o == "sum012":
+            return out[0] + out[1] + out[2]
+        if o == "interval_lo":
+            return out[0] if out[2] != 0.0 else 0.0
+        if o == "interval_hi":
+            return out[1] if out[2] != 0.0 else 0.0
+        return out[o]
+    return np.array([one(o) for o in outputs], np.float32)
 
 
 def approximately_equal(actual, expected, tolerance):
@@ -40,7 +53,7 @@ def _block(rng, n):
 
 def random_records(op: int, n: int, seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed * 131 + op)
-    x = np.zeros((n, 24), np.float64)
+    x = np.zeros((n, WIDTH), np.float64)
     x[:, 0] = op
     u = lambda *shape: rng.random(shape)
     if op == 0:
@@ -103,4 +116,23 @@ def random_records(op: int, n: int, seed: int) -> np.ndarray:
         x[:, 19:22] = rng.normal(0, 3, (n, 3))
         x[:, 22] = rng.uniform(-1, 2, n)
         x[:, 23] = x[:, 22] + rng.uniform(0, 4, n) * (rng.random(n) > 0.05)
+    elif op in (14, 15):
+        x[:, 1:4] = rng.normal(0, 4, (n, 3))
+        d = rng.normal(0, 1, (n, 3))
+        x[:, 4:7] = d * rng.uniform(0.5, 2.0, (n, 1)) / np.linalg.norm(d, axis=1, keepdims=True)
+        x[:, 7] = rng.uniform(0, 1, n)
+        x[:, 8] = x[:, 7] + rng.uniform(0, 12, n)
+        x[:, 9:12] = rng.normal(0, 1, (n, 3))
+        # an orthonormal lamp frame
+        q, _ = np.linalg.qr(rng.normal(0, 1, (n, 3, 3)))
+        x[:, 12:15], x[:, 15:18], x[:, 18:21] = q[:, :, 0], q[:, :, 1], q[:, :, 2]
+        if op == 14:
+            x[:, 21:24] = rng.uniform(0.5, 2.0, (n, 3))
+            x[:, 24] = rng.uniform(0, 1, n) * (rng.random(n) > 0.3)
+            x[:, 25] = rng.uniform(0.05, 3.1, n)
+        else:
+            x[:, 21] = rng.uniform(0.2, 3, n)
+            x[:, 22] = rng.uniform(0.2, 3, n)
+            x[:, 23] = np.where(rng.random(n) < 0.2, 0.0, rng.uniform(0.0, 3.14159265, n))
+            x[:, 24] = rng.random(n) < 0.5
     return x.astype(np.float32)
