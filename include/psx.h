@@ -746,10 +746,18 @@ int psx_scene_intersect_device_noalloc(psx_scene *scene, const psx_ray_query *de
  *                         min, max, centre[3], axis u[3], v[3], z[3],
  *                         length u, length v, spread, ellipse -> min,
  *                         max, valid
+ *                       16 spot lamp sample for a volume segment and at a
+ *                         collision: reference point[3], centre[3],
+ *                         radius, sphere, axes x[3], y[3], z[3], axis
+ *                         scale[3], random[2], normalized power, spot
+ *                         angle, spot smoothness -> segment valid,
+ *                         evaluation, pdf, collision valid, evaluation,
+ *                         pdf
  *                       (src/luisa/homogeneous_volume_transport.cpp,
  *                        src/luisa/volume_scatter_probability.cpp:20-98,
  *                        src/luisa/volume_direct_sampling.cpp,
- *                        src/luisa/volume_light_interval.cpp) */
+ *                        src/luisa/volume_light_interval.cpp,
+ *                        src/luisa/volume_analytic_light_sampling.cpp) */
 enum psx_probe_function {
     PSX_PROBE_FILM_CLAMP = 0,
     PSX_PROBE_T_OFFSET = 1,

@@ -3180,7 +3180,13 @@ typedef struct lamp_sample_t {
 } lamp_sample_t;
 
 /* sample_point_light (:898-979) and sample_spot_light (:981-1105) */
-static lamp_sample_t sample_point_or_spot(const olamp *l, v3 ref, v3 sn, int has_transmission, v2 rnd) {
+/* segment: the spot proposal of a volume segment
+ * (VolumeAnalyticLightSampling::spot_from_segment,
+ * src/luisa/volume_analytic_light_sampling.cpp): the visible sphere cap
+ * outside a finite sphere, never the spread cone, and no reject of a zero
+ * attenuation */
+static lamp_sample_t sample_point_or_spot_ex(const olamp *l, v3 ref, v3 sn, int has_transmission, v2 rnd,
+                                             int segment) {
     const int spot = l->type == PSX_LIGHT_SPOT;
     const int sphere = (l->flags & LAMP_SPHERE) != 0u;
     const int normalize = (l->flags & LAMP_NORMALIZE) != 0u;
@@ -3192,7 +3198,7 @@ static lamp_sample_t sample_point_or_spot(const olamp *l, v3 ref, v3 sn, int has
     v3 outside_axis = vneg(light_normal);
     float outside_cap = sphere_cap;
     int spread_cone = 0;
-    if (spot) {
+    if (spot && !segment) {
         const float spot_cap = spot_omc_larger_spread(l->spot_angle, l->axis_scale);
         spread_cone = (cd2 > r2) && !(sphere_cap < spot_cap);
         if (spread_cone) {
@@ -3221,9 +3227,13 @@ static lamp_sample_t sample_point_or_spot(const olamp *l, v3 ref, v3 sn, int has
         const float attenuation = spot_light_attenuation(local, l->spot_angle, l->spot_smooth);
         const int use_attenuation = !finite_sphere || (cd2 > r2);
         s.eval = s.eval * (use_attenuation ? attenuation : 1.0f);
-        s.valid = g.valid && (s.eval != 0.0f);
+        if (!segment) s.valid = g.valid && (s.eval != 0.0f);
     }
     return s;
+}
+
+static lamp_sample_t sample_point_or_spot(const olamp *l, v3 ref, v3 sn, int has_transmission, v2 rnd) {
+    return sample_point_or_spot_ex(l, ref, sn, has_transmission, rnd, 0);
 }
 
 /* sample_distant_light (:84-108) */
@@ -5200,6 +5210,26 @@ static void volume_probe(const float *in, float *out) {
         const light_interval r = iv_spot(load3(in + 1), load3(in + 4), in[7], in[8], load3(in + 9), load3(in + 12),
                                          load3(in + 15), load3(in + 18), load3(in + 21), in[24], in[25]);
         out[0] = r.minimum; out[1] = r.maximum; out[2] = r.valid ? 1.0f : 0.0f;
+        break;
+    }
+    case 16u: {
+        olamp l;
+        memset(&l, 0, sizeof(l));
+        l.type = PSX_LIGHT_SPOT;
+        l.position = load3(in + 4);
+        l.radius = in[7];
+        l.axis_x = load3(in + 9);
+        l.axis_y = load3(in + 12);
+        l.axis_z = load3(in + 15);
+        l.axis_scale = load3(in + 18);
+        l.spot_angle = in[24];
+        l.spot_smooth = in[25];
+        l.flags = (in[8] != 0.0f ? LAMP_SPHERE : 0u) | (in[23] != 0.0f ? LAMP_NORMALIZE : 0u);
+        const v2 rnd = {in[21], in[22]};
+        const lamp_sample_t seg = sample_point_or_spot_ex(&l, load3(in + 1), V3(0, 0, 0), 1, rnd, 1);
+        const lamp_sample_t pos = sample_point_or_spot_ex(&l, load3(in + 1), V3(0, 0, 0), 1, rnd, 0);
+        out[0] = seg.valid ? 1.0f : 0.0f; out[1] = seg.eval; out[2] = seg.pdf;
+        out[3] = pos.valid ? 1.0f : 0.0f; out[4] = pos.eval; out[5] = pos.pdf;
         break;
     }
     case 15u: {

@@ -1547,7 +1547,12 @@ struct LampSample {
 };
 
 // sample_point_light (:898-979), sample_spot_light (:981-1105) and their
-// sphere/disk geometry (:429-603)
+// sphere/disk geometry (:429-603).  SEGMENT: the spot proposal of a volume
+// segment (VolumeAnalyticLightSampling::spot_from_segment,
+// src/luisa/volume_analytic_light_sampling.cpp): outside a finite sphere it
+// always samples the visible cap, never the spread cone, and it keeps a
+// geometrically valid sample whose attenuation is zero.
+template <bool SEGMENT = false>
 __device__ LampSample sample_point_or_spot(const Lamp &l, f3 ref, f3 sn, bool has_transmission, float rx, float ry) {
     const bool spot = l.type == kLampSpot;
     const bool sphere = (l.flags & kLampSphere) != 0u;
@@ -1561,7 +1566,7 @@ __device__ LampSample sample_point_or_spot(const Lamp &l, f3 ref, f3 sn, bool ha
     f3 outside_axis = neg(light_normal);
     float outside_cap = sphere_cap;
     bool spread_cone = false;
-    if (spot) {
+    if (spot && !SEGMENT) {
         const float spot_cap = spot_omc_larger_spread(l);
         spread_cone = (cd2 > r2) && !(sphere_cap < spot_cap);
         if (spread_cone) {
@@ -1643,7 +1648,7 @@ __device__ LampSample sample_point_or_spot(const Lamp &l, f3 ref, f3 sn, bool ha
         const float attenuation = spot_light_attenuation(local, l.spot_angle, l.spot_smooth);
         const bool use_attenuation = !finite_sphere || (cd2 > r2);
         s.eval = s.eval * (use_attenuation ? attenuation : 1.0f);
-        s.valid = s.valid && (s.eval != 0.0f);
+        if (!SEGMENT) s.valid = s.valid && (s.eval != 0.0f);
     }
     return s;
 }

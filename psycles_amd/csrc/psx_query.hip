@@ -218,6 +218,34 @@ __device__ void volume_probe(const float *in, float *out) {
         out[2] = r.valid ? 1.0f : 0.0f;
         break;
     }
+    case 16u: {
+        Lamp l{};
+        l.type = kLampSpot;
+        const f3 c = in3(in + 4), ax = in3(in + 9), ay = in3(in + 12), az = in3(in + 15), sc = in3(in + 18);
+        const float cv[3] = {c.x, c.y, c.z}, xv[3] = {ax.x, ax.y, ax.z}, yv[3] = {ay.x, ay.y, ay.z},
+                    zv[3] = {az.x, az.y, az.z}, sv[3] = {sc.x, sc.y, sc.z};
+        for (int i = 0; i < 3; ++i) {
+            l.position[i] = cv[i];
+            l.axis_x[i] = xv[i];
+            l.axis_y[i] = yv[i];
+            l.axis_z[i] = zv[i];
+            l.axis_scale[i] = sv[i];
+        }
+        l.radius = in[7];
+        l.spot_angle = in[24];
+        l.spot_smooth = in[25];
+        l.flags = (in[8] != 0.0f ? kLampSphere : 0u) | (in[23] != 0.0f ? kLampNormalize : 0u);
+        const f3 ref = in3(in + 1), zero = V(0.0f, 0.0f, 0.0f);
+        const LampSample seg = sample_point_or_spot<true>(l, ref, zero, true, in[21], in[22]);
+        const LampSample pos = sample_point_or_spot<false>(l, ref, zero, true, in[21], in[22]);
+        out[0] = seg.valid ? 1.0f : 0.0f;
+        out[1] = seg.eval;
+        out[2] = seg.pdf;
+        out[3] = pos.valid ? 1.0f : 0.0f;
+        out[4] = pos.eval;
+        out[5] = pos.pdf;
+        break;
+    }
     case 15u: {
         const LightInterval r = volume_area_interval(in3(in + 1), in3(in + 4), in[7], in[8], in3(in + 9), in3(in + 12),
                                                      in3(in + 15), in3(in + 18), in[21], in[22], in[23], in[24] != 0.0f);

@@ -1,7 +1,7 @@
 """Homogeneous volume transport on the GPU (PSX_PROBE_VOLUME, psx_volume.h)
 against the reference's fixture records and the oracle.
 
-* Records 0-56 of tests/test_luisa_homogeneous_volume.cpp:1052-1130 under
+* Records 0-58 of tests/test_luisa_homogeneous_volume.cpp:1052-1140 under
   the reference's tolerance (3e-5 relative to max(1, |value|)).
 * Seeded random records per operation against the oracle: discrete outputs
   (channel, matched / scattered / active flags, technique) equal on
@@ -28,7 +28,7 @@ from psycles_amd.render import RendererBackend
 pytestmark = pytest.mark.gpu
 
 # operations whose floats amplify one ulp of a transcendental (see above)
-ILL_CONDITIONED = {1, 4, 5, 6, 7, 10, 11, 13, 14, 15}
+ILL_CONDITIONED = {1, 4, 5, 6, 7, 10, 11, 13, 14, 15, 16}
 
 
 def log(msg):

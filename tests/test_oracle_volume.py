@@ -1,7 +1,7 @@
 """Homogeneous volume transport in the oracle (PSX_PROBE_VOLUME).
 
-Pinned by records 0-56 of the reference's fixture
-(tests/test_luisa_homogeneous_volume.cpp:1052-1130, transcribed with their
+Pinned by all 59 records of the reference's fixture
+(tests/test_luisa_homogeneous_volume.cpp:1052-1140, transcribed with their
 inputs in tests/golden/homogeneous_volume.json) under the reference's own
 tolerance, and by what the mathematics fixes: the bounded free-flight
 sampler inverts its CDF and its pdf integrates to one on the interval, the

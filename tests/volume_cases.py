@@ -11,9 +11,9 @@ GOLDEN = Path(__file__).resolve().parent / "golden" / "homogeneous_volume.json"
 # outputs each operation writes: discrete ones are compared exactly
 OUTPUTS = {0: [0, 1, 2], 1: [0, 1, 2], 2: [0, 1, 2], 3: [0, 1, 2, 3, 4, 5], 4: [0], 5: [0, 1, 2],
            6: list(range(22)), 7: list(range(22)), 8: [0, 1, 2], 9: [0, 1, 2, 3], 10: [0, 1, 2], 11: [0],
-           12: [0], 13: list(range(17)), 14: [0, 1, 2], 15: [0, 1, 2]}
+           12: [0], 13: list(range(17)), 14: [0, 1, 2], 15: [0, 1, 2], 16: [0, 1, 2, 3, 4, 5]}
 DISCRETE = {3: [0, 2], 6: [19, 20, 21], 7: [19, 20, 21], 9: [0, 2, 3], 10: [2], 13: [13, 14, 15, 16], 14: [2],
-            15: [2]}
+            15: [2], 16: [0, 3]}
 WIDTH = 32
 
 
@@ -135,4 +135,17 @@ def random_records(op: int, n: int, seed: int) -> np.ndarray:
             x[:, 22] = rng.uniform(0.2, 3, n)
             x[:, 23] = np.where(rng.random(n) < 0.2, 0.0, rng.uniform(0.0, 3.14159265, n))
             x[:, 24] = rng.random(n) < 0.5
+    elif op == 16:
+        x[:, 1:4] = rng.normal(0, 3, (n, 3))
+        x[:, 4:7] = rng.normal(0, 1, (n, 3))
+        x[:, 7] = rng.uniform(0, 1.5, n) * (rng.random(n) > 0.3)
+        x[:, 8] = rng.random(n) < 0.7
+        q, _ = np.linalg.qr(rng.normal(0, 1, (n, 3, 3)))
+        x[:, 9:12], x[:, 12:15], x[:, 15:18] = q[:, :, 0], q[:, :, 1], q[:, :, 2]
+        x[:, 18:21] = rng.uniform(0.5, 2.0, (n, 3))
+        x[:, 21] = u(n)
+        x[:, 22] = u(n)
+        x[:, 23] = rng.random(n) < 0.5
+        x[:, 24] = rng.uniform(0.05, 3.1, n)
+        x[:, 25] = rng.uniform(0, 1, n)
     return x.astype(np.float32)
