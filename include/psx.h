@@ -753,11 +753,16 @@ int psx_scene_intersect_device_noalloc(psx_scene *scene, const psx_ray_query *de
  *                         angle, spot smoothness -> segment valid,
  *                         evaluation, pdf, collision valid, evaluation,
  *                         pdf
+ *                       17 Henyey-Greenstein phase: anisotropy, cosine,
+ *                         axis[3], random[2] -> pdf at the cosine, clamped
+ *                         anisotropy, sampled direction[3], its pdf,
+ *                         Rayleigh pdf at the cosine
  *                       (src/luisa/homogeneous_volume_transport.cpp,
  *                        src/luisa/volume_scatter_probability.cpp:20-98,
  *                        src/luisa/volume_direct_sampling.cpp,
  *                        src/luisa/volume_light_interval.cpp,
- *                        src/luisa/volume_analytic_light_sampling.cpp) */
+ *                        src/luisa/volume_analytic_light_sampling.cpp,
+ *                        include/psycles/luisa/cycles_volume_phase.h) */
 enum psx_probe_function {
     PSX_PROBE_FILM_CLAMP = 0,
     PSX_PROBE_T_OFFSET = 1,

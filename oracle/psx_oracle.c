@@ -5154,6 +5154,40 @@ static light_interval iv_area(v3 ray_origin, v3 direction, float lo, float hi, v
     return side;
 }
 
+/* ---- phase functions (include/psycles/luisa/cycles_volume_phase.h) */
+static float phase_clamp_anisotropy(float g) {
+    const float s = g > 0.0f ? 1.0f : (g < 0.0f ? -1.0f : 0.0f);
+    return s * fminf(fabsf(g), 1.0f - 1.0e-3f);
+}
+
+static float phase_hg_pdf(float cosine, float g) {
+    float result = 0.07957747154594766788f;
+    if (fabsf(g) >= 1.0e-3f) {
+        const float factor = 1.0f + g * (g - 2.0f * cosine);
+        result = (1.0f - g * g) / (4.0f * 3.14159265358979323846f * factor * sqrtf(fmaxf(factor, 0.0f)));
+    }
+    return result;
+}
+
+static float phase_rayleigh_pdf(float cosine) { return (0.1875f * 0.31830988618379067154f) * (1.0f + cosine * cosine); }
+
+static float phase_hg_sample_cosine(float g, float random) {
+    float cosine = 1.0f - 2.0f * random;
+    if (fabsf(g) >= 1.0e-3f) {
+        const float ratio = (1.0f - g * g) / (1.0f - g * cosine);
+        cosine = (1.0f + g * g - ratio * ratio) / (2.0f * g);
+    }
+    return cosine;
+}
+
+static v3 phase_direction(v3 axis, float cosine, float azimuth_sample) {
+    const float phi = 2.0f * 3.14159265358979323846f * azimuth_sample;
+    const float sine = sqrtf(fmaxf(1.0f - cosine * cosine, 0.0f));
+    v3 t, b;
+    make_orthonormals(axis, &t, &b);
+    return vadd(vadd(vscale(t, sine * cosf(phi)), vscale(b, sine * sinf(phi))), vscale(axis, cosine));
+}
+
 static vol_coeffs vol_block(const float *in) {
     vol_coeffs c;
     const uint32_t flags = (uint32_t)in[9];
@@ -5230,6 +5264,17 @@ static void volume_probe(const float *in, float *out) {
         const lamp_sample_t pos = sample_point_or_spot_ex(&l, load3(in + 1), V3(0, 0, 0), 1, rnd, 0);
         out[0] = seg.valid ? 1.0f : 0.0f; out[1] = seg.eval; out[2] = seg.pdf;
         out[3] = pos.valid ? 1.0f : 0.0f; out[4] = pos.eval; out[5] = pos.pdf;
+        break;
+    }
+    case 17u: {
+        const float g = phase_clamp_anisotropy(in[1]);
+        const float cosine = phase_hg_sample_cosine(g, in[6]);
+        const v3 d = phase_direction(load3(in + 3), cosine, in[7]);
+        out[0] = phase_hg_pdf(in[2], g);
+        out[1] = g;
+        store3(out + 2, d);
+        out[5] = phase_hg_pdf(cosine, g);
+        out[6] = phase_rayleigh_pdf(in[2]);
         break;
     }
     case 15u: {

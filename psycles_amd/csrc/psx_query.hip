@@ -246,6 +246,19 @@ __device__ void volume_probe(const float *in, float *out) {
         out[5] = pos.pdf;
         break;
     }
+    case 17u: {
+        // Henyey-Greenstein phase of anisotropy in[1]: pdf at cosine in[2],
+        // the clamped anisotropy, and a sample around axis in[3..5]
+        const float g = phase_clamp_anisotropy(in[1]);
+        const float cosine = phase_hg_sample_cosine(g, in[6]);
+        const f3 d = phase_direction(in3(in + 3), cosine, in[7]);
+        out[0] = phase_hg_pdf(in[2], g);
+        out[1] = g;
+        out3(out + 2, d);
+        out[5] = phase_hg_pdf(cosine, g);
+        out[6] = phase_rayleigh_pdf(in[2]);
+        break;
+    }
     case 15u: {
         const LightInterval r = volume_area_interval(in3(in + 1), in3(in + 4), in[7], in[8], in3(in + 9), in3(in + 12),
                                                      in3(in + 15), in3(in + 18), in[21], in[22], in[23], in[24] != 0.0f);

@@ -335,6 +335,49 @@ __device__ inline HomogeneousDirectSample volume_sample_direct(const VolumeCoeff
     return r;
 }
 
+// ---- phase functions ------------------------------------------------------
+// Henyey-Greenstein and Rayleigh scattering at a volume collision
+// (include/psycles/luisa/cycles_volume_phase.h: clamp_anisotropy :46-50,
+// henyey_greenstein_pdf :121-139, rayleigh_pdf :140-143, sample_direction
+// :280-292, sample_henyey_greenstein_cosine :294-312).
+
+__device__ __forceinline__ float phase_clamp_anisotropy(float g) {
+    const float s = g > 0.0f ? 1.0f : (g < 0.0f ? -1.0f : 0.0f);
+    return s * fminf(fabsf(g), 1.0f - 1.0e-3f);
+}
+
+__device__ inline float phase_hg_pdf(float cosine, float g) {
+    float result = 0.07957747154594766788f;  // 1 / (4 pi)
+    if (fabsf(g) >= 1.0e-3f) {
+        const float factor = 1.0f + g * (g - 2.0f * cosine);
+        result = (1.0f - g * g) / (4.0f * 3.14159265358979323846f * factor * sqrtf(fmaxf(factor, 0.0f)));
+    }
+    return result;
+}
+
+__device__ __forceinline__ float phase_rayleigh_pdf(float cosine) {
+    return (0.1875f * 0.31830988618379067154f) * (1.0f + cosine * cosine);
+}
+
+__device__ inline float phase_hg_sample_cosine(float g, float random) {
+    float cosine = 1.0f - 2.0f * random;
+    if (fabsf(g) >= 1.0e-3f) {
+        const float ratio = (1.0f - g * g) / (1.0f - g * cosine);
+        cosine = (1.0f + g * g - ratio * ratio) / (2.0f * g);
+    }
+    return cosine;
+}
+
+// the direction at `cosine` from `axis`, azimuth 2 pi * azimuth_sample
+__device__ inline f3 phase_direction(f3 axis, float cosine, float azimuth_sample) {
+    const float phi = 2.0f * 3.14159265358979323846f * azimuth_sample;
+    const float sine = sqrtf(fmaxf(1.0f - cosine * cosine, 0.0f));
+    const f3 local = V(sine * cosf(phi), sine * sinf(phi), cosine);
+    f3 t, b;
+    make_orthonormals(axis, t, b);
+    return t * local.x + b * local.y + axis * local.z;
+}
+
 // ---- light intervals ------------------------------------------------------
 // The part of a ray segment from which a spot or area light can be seen,
 // which bounds the equiangular proposal (VolumeLightInterval,

@@ -16,8 +16,9 @@ import volume_cases as vc
 from psycles_amd import abi
 
 
-def test_reference_records():
-    doc = vc.golden()
+@pytest.mark.parametrize("path", [vc.GOLDEN, vc.PHASE], ids=["homogeneous_volume", "volume_phase"])
+def test_reference_records(path):
+    doc = vc.golden(path)
     tol = doc["tolerance"]
     bad = []
     for r in doc["records"]:
@@ -126,3 +127,19 @@ def test_direct_estimators_are_unbiased(method):
     if method == 3:
         # the technique random picks distance below 0.5, equiangular above
         assert set(np.unique(out[:, 14])) == {1.0, 2.0}
+
+
+def test_henyey_greenstein_is_normalised_and_sampled_by_its_pdf():
+    """The pdf integrates to one over the sphere, and sampled cosines follow
+    it: their mean is the anisotropy g."""
+    cs = np.linspace(-1.0, 1.0, 20001)
+    for g in (-0.7, 0.0, 0.42, 0.9):
+        pdf = ob.probe(abi.PROBE_VOLUME, [_row(17, g, c, 0, 0, 1, 0.5, 0.5) for c in cs])[:, 0].astype(np.float64)
+        assert abs(2.0 * np.pi * np.trapezoid(pdf, cs) - 1.0) < 2e-3
+        rng = np.random.default_rng(5)
+        rows = np.zeros((20000, vc.WIDTH), np.float32)
+        rows[:, 0], rows[:, 1], rows[:, 5] = 17, g, 1.0
+        rows[:, 6], rows[:, 7] = rng.random(20000), rng.random(20000)
+        out = ob.probe(abi.PROBE_VOLUME, rows)
+        assert abs(out[:, 4].astype(np.float64).mean() - g) < 0.02
+        assert np.allclose(np.linalg.norm(out[:, 2:5], axis=1), 1.0, atol=1e-5)

@@ -7,18 +7,19 @@ from pathlib import Path
 import numpy as np
 
 GOLDEN = Path(__file__).resolve().parent / "golden" / "homogeneous_volume.json"
+PHASE = Path(__file__).resolve().parent / "golden" / "volume_phase.json"
 
 # outputs each operation writes: discrete ones are compared exactly
 OUTPUTS = {0: [0, 1, 2], 1: [0, 1, 2], 2: [0, 1, 2], 3: [0, 1, 2, 3, 4, 5], 4: [0], 5: [0, 1, 2],
            6: list(range(22)), 7: list(range(22)), 8: [0, 1, 2], 9: [0, 1, 2, 3], 10: [0, 1, 2], 11: [0],
-           12: [0], 13: list(range(17)), 14: [0, 1, 2], 15: [0, 1, 2], 16: [0, 1, 2, 3, 4, 5]}
+           12: [0], 13: list(range(17)), 14: [0, 1, 2], 15: [0, 1, 2], 16: [0, 1, 2, 3, 4, 5], 17: [0, 1, 2, 3, 4, 5, 6]}
 DISCRETE = {3: [0, 2], 6: [19, 20, 21], 7: [19, 20, 21], 9: [0, 2, 3], 10: [2], 13: [13, 14, 15, 16], 14: [2],
             15: [2], 16: [0, 3]}
 WIDTH = 32
 
 
-def golden():
-    with open(GOLDEN) as f:
+def golden(path=GOLDEN):
+    with open(path) as f:
         return json.load(f)
 
 
@@ -148,4 +149,11 @@ def random_records(op: int, n: int, seed: int) -> np.ndarray:
         x[:, 23] = rng.random(n) < 0.5
         x[:, 24] = rng.uniform(0.05, 3.1, n)
         x[:, 25] = rng.uniform(0, 1, n)
+    elif op == 17:
+        x[:, 1] = rng.uniform(-1.2, 1.2, n) * (rng.random(n) > 0.1)
+        x[:, 2] = rng.uniform(-1, 1, n)
+        d = rng.normal(0, 1, (n, 3))
+        x[:, 3:6] = d / np.linalg.norm(d, axis=1, keepdims=True)
+        x[:, 6] = u(n)
+        x[:, 7] = u(n)
     return x.astype(np.float32)
