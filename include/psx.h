@@ -756,7 +756,8 @@ int psx_scene_intersect_device_noalloc(psx_scene *scene, const psx_ray_query *de
  *                       17 Henyey-Greenstein phase: anisotropy, cosine,
  *                         axis[3], random[2] -> pdf at the cosine, clamped
  *                         anisotropy, sampled direction[3], its pdf,
- *                         Rayleigh pdf at the cosine
+ *                         Rayleigh pdf at the cosine, Rayleigh sampled
+ *                         direction[3] and its pdf
  *                       (src/luisa/homogeneous_volume_transport.cpp,
  *                        src/luisa/volume_scatter_probability.cpp:20-98,
  *                        src/luisa/volume_direct_sampling.cpp,

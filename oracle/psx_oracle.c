@@ -5180,6 +5180,24 @@ static float phase_hg_sample_cosine(float g, float random) {
     return cosine;
 }
 
+/* Cycles' fast_inv_cbrtf (include/psycles/luisa/cycles_fast_math.h:229-243) */
+static float phase_inverse_cube_root(float value) {
+    int32_t bits;
+    memcpy(&bits, &value, 4);
+    const int32_t guess = 0x54a24242 - bits / 3;
+    float r;
+    memcpy(&r, &guess, 4);
+    r = (2.0f / 3.0f) * r + 1.0f / (3.0f * r * r * value);
+    r = (2.0f / 3.0f) * r + 1.0f / (3.0f * r * r * value);
+    return r;
+}
+
+static float phase_rayleigh_sample_cosine(float random) {
+    const float a = 2.0f - 4.0f * random;
+    const float inverse_u = -phase_inverse_cube_root(sqrtf(1.0f + a * a) + a);
+    return 1.0f / inverse_u - inverse_u;
+}
+
 static v3 phase_direction(v3 axis, float cosine, float azimuth_sample) {
     const float phi = 2.0f * 3.14159265358979323846f * azimuth_sample;
     const float sine = sqrtf(fmaxf(1.0f - cosine * cosine, 0.0f));
@@ -5275,6 +5293,9 @@ static void volume_probe(const float *in, float *out) {
         store3(out + 2, d);
         out[5] = phase_hg_pdf(cosine, g);
         out[6] = phase_rayleigh_pdf(in[2]);
+        const float rc = phase_rayleigh_sample_cosine(in[6]);
+        store3(out + 7, phase_direction(load3(in + 3), rc, in[7]));
+        out[10] = phase_rayleigh_pdf(rc);
         break;
     }
     case 15u: {

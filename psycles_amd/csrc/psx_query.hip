@@ -257,6 +257,9 @@ __device__ void volume_probe(const float *in, float *out) {
         out3(out + 2, d);
         out[5] = phase_hg_pdf(cosine, g);
         out[6] = phase_rayleigh_pdf(in[2]);
+        const float rc = phase_rayleigh_sample_cosine(in[6]);
+        out3(out + 7, phase_direction(in3(in + 3), rc, in[7]));
+        out[10] = phase_rayleigh_pdf(rc);
         break;
     }
     case 15u: {

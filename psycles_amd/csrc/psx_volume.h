@@ -368,6 +368,22 @@ __device__ inline float phase_hg_sample_cosine(float g, float random) {
     return cosine;
 }
 
+// 1 / cbrt(value): a bit-pattern first guess and two Newton steps
+// (Cycles' fast_inv_cbrtf; include/psycles/luisa/cycles_fast_math.h:229-243)
+__device__ __forceinline__ float phase_inverse_cube_root(float value) {
+    float r = __int_as_float(0x54a24242 - __float_as_int(value) / 3);
+    r = (2.0f / 3.0f) * r + 1.0f / (3.0f * r * r * value);
+    r = (2.0f / 3.0f) * r + 1.0f / (3.0f * r * r * value);
+    return r;
+}
+
+// sample_rayleigh_cosine (cycles_volume_phase.h:313-320): the cubic's root
+__device__ __forceinline__ float phase_rayleigh_sample_cosine(float random) {
+    const float a = 2.0f - 4.0f * random;
+    const float inverse_u = -phase_inverse_cube_root(sqrtf(1.0f + a * a) + a);
+    return 1.0f / inverse_u - inverse_u;
+}
+
 // the direction at `cosine` from `axis`, azimuth 2 pi * azimuth_sample
 __device__ inline f3 phase_direction(f3 axis, float cosine, float azimuth_sample) {
     const float phi = 2.0f * 3.14159265358979323846f * azimuth_sample;
